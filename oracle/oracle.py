@@ -1,0 +1,452 @@
+"""CPU oracle for the bsr_mat radial core -- TEST INFRASTRUCTURE ONLY.
+
+Thin ctypes layer over ``liborc.so`` (the C restatement in ``bsr_oracle.c``)
+plus the partial-wave driver that strings the reference steps together in the
+order of ``src/bsr/BSR_MAT3/sub1.f90`` and the ``bsr_mat.nnn`` writer/reader
+of ``src/bsr/BSR_MAT3/record_matrix.f90``.  Only ``tests/``, the smoke check
+and the CPU-baseline leg of ``bench.py`` use this module, and only as the
+checker; the product package never imports it.
+
+Parity status of each function is recorded in the test files that pin it
+(``tests/test_oracle_*.py``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import struct
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+IBI = 2 ** 15
+IBO = 2 ** 15
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile")], check=True)
+    return os.path.join(HERE, "liborc.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liborc.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        sig = {
+            "orc_last_error": (C.c_char_p, []),
+            "orc_mkgrid1": (i, [i, d, d, d, d, i, _dp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(d)]),
+            "orc_mkgrid0": (i, [i, d, d, d, d, i, _dp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(d)]),
+            "orc_gauss": (None, [i, _dp, _dp]),
+            "orc_grid_new": (vp, [i, i, i, i, d, _dp]),
+            "orc_grid_free": (None, [vp]),
+            "orc_grid_nv": (i, [vp]),
+            "orc_grid_tables": (None, [vp, _dp, _dp, _dp, _dp]),
+            "orc_bspline_at": (None, [vp, d, _dp]),
+            "orc_mrm": (None, [vp, i, _dp]),
+            "orc_moments": (None, [vp, i, _dp]),
+            "orc_rk_moments": (None, [vp, i, _dp, _dp, _dp]),
+            "orc_mrk_cell": (None, [vp, i, _dp]),
+            "orc_density": (None, [i, i, _dp, _dp, _dp, C.c_char]),
+            "orc_convol": (i, [i, i, _dp, _dp, i, C.c_char, C.c_char, _dp]),
+            "orc_sum_amb": (d, [i, i, _dp, _dp, C.c_char]),
+            "orc_bav": (None, [i, i, _dp, _dp, _dp, C.c_char, C.c_char]),
+            "orc_bxv": (None, [i, i, _dp, _dp, _dp]),
+            "orc_bvmv": (d, [i, i, _dp, _dp, _dp]),
+            "orc_quadr": (d, [vp, _dp, _dp, i]),
+            "orc_wave_new": (vp, [vp, i, _dp, _ip, _ip, _ip, i, i, i, d, d]),
+            "orc_wave_free": (None, [vp]),
+            "orc_wave_get": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+            "orc_wave_qbs": (None, [vp, _dp, _dp]),
+            "orc_wave_update_hl": (i, [vp, i, i, d]),
+            "orc_wave_idata": (i, [vp, i, i, i, i, _ip, _ip, _ip, _ip, _ip, _dp]),
+            "orc_wave_odata": (i, [vp, i, _ip, _ip, _ip, _dp]),
+            "orc_wave_symmetrize_diag": (None, [vp]),
+            "orc_wave_scale": (None, [vp, d]),
+            "orc_wave_acf_finish": (None, [vp, i]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _f(a, dtype=np.float64):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise OracleError(lib().orc_last_error().decode())
+
+
+# ---------------------------------------------------------------- grid
+
+def mkgrid1(ks, z, h, hmax, rmax):
+    """Knots of mkgrid1 (define_grid.f90:213-276): (t, ns, ml, me, h)."""
+    cap = int((rmax * z / h + 1 + ks) * 2) + 2 * ks + 8
+    t = np.zeros(cap, order="F")
+    ns, ml, me, hh = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    _check(lib().orc_mkgrid1(ks, z, h, hmax, rmax, cap, t, ns, ml, me, hh))
+    return t[: ns.value + ks].copy(), ns.value, ml.value, me.value, hh.value
+
+
+def mkgrid0(ks, z, h, hmax, rmax):
+    """Knots of mkgrid (define_grid.f90:86-153): (t, ns, ml, me, h)."""
+    cap = 100000
+    t = np.zeros(cap, order="F")
+    ns, ml, me, hh = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    _check(lib().orc_mkgrid0(ks, z, h, hmax, rmax, cap, t, ns, ml, me, hh))
+    return t[: ns.value + ks].copy(), ns.value, ml.value, me.value, hh.value
+
+
+def gauss(k):
+    x = np.zeros(k, order="F")
+    w = np.zeros(k, order="F")
+    lib().orc_gauss(k, x, w)
+    return x, w
+
+
+class Grid:
+    """Gauss tables, B-spline values and the Gram band on one knot grid."""
+
+    def __init__(self, ks, ns, ml, me, h, t):
+        self.ks, self.ns, self.ml, self.me, self.h = ks, ns, ml, me, h
+        self.t = _f(t)
+        self.nv = ns - ks + 1
+        self.ptr = lib().orc_grid_new(ks, ns, ml, me, h, self.t)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().orc_grid_free(self.ptr)
+            self.ptr = None
+
+    @property
+    def ik(self):
+        return self.ks * (self.ks + 1) // 2
+
+    def tables(self):
+        ks, nv, ns = self.ks, self.nv, self.ns
+        gr = np.zeros(nv * ks, order="F")
+        grw = np.zeros(nv * ks, order="F")
+        bsp = np.zeros(nv * ks * ks, order="F")
+        sb = np.zeros((ns, ks), order="F")
+        lib().orc_grid_tables(self.ptr, gr, grw, bsp, sb)
+        return gr.reshape(nv, ks), grw.reshape(nv, ks), bsp.reshape(nv, ks, ks), sb
+
+    def bsplines_at(self, r):
+        out = np.zeros(self.ns, order="F")
+        lib().orc_bspline_at(self.ptr, float(r), out)
+        return out
+
+    def mrm(self, m):
+        rm = np.zeros((self.ns, self.ks), order="F")
+        lib().orc_mrm(self.ptr, m, rm)
+        return rm
+
+    def moments(self, k):
+        rkm = np.zeros(self.ik * self.nv, order="F")
+        lib().orc_moments(self.ptr, k, rkm)
+        return rkm.reshape(self.nv, self.ik)
+
+    def rk_moments(self, k):
+        ik, nv = self.ik, self.nv
+        r1 = np.zeros(ik * nv, order="F")
+        r2 = np.zeros(ik * nv, order="F")
+        rd = np.zeros(ik * ik * nv, order="F")
+        lib().orc_rk_moments(self.ptr, k, r1, r2, rd)
+        return r1.reshape(nv, ik), r2.reshape(nv, ik), rd.reshape(nv, ik, ik)
+
+    def mrk_cell(self, k):
+        """rkb(ns,ns,ks,ks), Fortran order."""
+        rkb = np.zeros((self.ns, self.ns, self.ks, self.ks), order="F")
+        lib().orc_mrk_cell(self.ptr, k, rkb)
+        return rkb
+
+    def quadr(self, p1, p2, m):
+        return lib().orc_quadr(self.ptr, _f(p1), _f(p2), m)
+
+
+# ---------------------------------------------------------------- leaves
+
+def density(ns, ks, p1, p2, typ):
+    cols = {"s": ks, "n": 2 * ks - 1, "x": ns}[typ]
+    d = np.zeros((ns, cols), order="F")
+    lib().orc_density(ns, ks, d, _f(p1), _f(p2), typ.encode())
+    return d
+
+
+def convol(ns, ks, d, icase, rkb):
+    a = np.zeros((ns, ns), order="F")
+    dd = np.zeros((ns, ns), order="F")
+    dd[:, : d.shape[1]] = d
+    _check(lib().orc_convol(ns, ks, a, dd, icase, b"s", b"s", _f(rkb)))
+    return a if icase > 2 else a[:, :ks].copy(order="F")
+
+
+def sum_amb(ns, ks, a, b, sym):
+    return lib().orc_sum_amb(ns, ks, _f(a), _f(b), sym.encode())
+
+
+def bav(ns, ks, b, v, typ="s", side="r"):
+    y = np.zeros(ns, order="F")
+    lib().orc_bav(ns, ks, _f(b), _f(v), y, typ.encode(), side.encode())
+    return y
+
+
+def bxv(ks, ns, b, v):
+    y = np.zeros(ns, order="F")
+    lib().orc_bxv(ks, ns, _f(b), _f(v), y)
+    return y
+
+
+def bvmv(ns, ks, a, x, y):
+    return lib().orc_bvmv(ns, ks, _f(a), _f(x), _f(y))
+
+
+# ---------------------------------------------------------------- a wave
+
+class WaveMatrices:
+    """hcc(ns,ns,nch(nch+1)/2), hcb(ns,nch,ncp), hbb, acf(nch,nch,0:mk),
+    htarg, otarg -- the contents of module bsr_matrix."""
+
+    def __init__(self, ns, nch, ncp, mk):
+        nij = nch * (nch + 1) // 2
+        self.hcc = np.zeros((ns, ns, nij), order="F")
+        self.hcb = np.zeros((ns, nch, max(ncp, 1)), order="F")
+        self.hbb = np.zeros(max(ncp * (ncp + 1) // 2, 1), order="F")
+        self.acf = np.zeros((nch, nch, mk + 1), order="F")
+        self.htarg = np.zeros(nij, order="F")
+        self.otarg = np.zeros(nij, order="F")
+        self.ns, self.nch, self.ncp, self.mk = ns, nch, ncp, mk
+
+
+class Wave:
+    """State of one partial wave in the oracle (modules bsr_matrix,
+    spline_orbitals and cmdata of the reference)."""
+
+    def __init__(self, grid: Grid, job):
+        self.grid = grid
+        self.job = job
+        self.ptr = lib().orc_wave_new(
+            grid.ptr, job.nbf, _f(job.pbs), _f(job.iech, np.int32), _f(job.lbs, np.int32),
+            _f(job.ibort, np.int32), job.nch, job.ncp, job.mk, job.eps_ovl, job.eps_c)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().orc_wave_free(self.ptr)
+            self.ptr = None
+
+    def get(self) -> WaveMatrices:
+        j = self.job
+        m = WaveMatrices(j.ns, j.nch, j.ncp, j.mk)
+        lib().orc_wave_get(self.ptr, m.hcc, m.hcb, m.hbb, m.acf, m.htarg, m.otarg)
+        return m
+
+    def update_hl(self, ich, jch, c):
+        _check(lib().orc_wave_update_hl(self.ptr, ich, jch, c))
+
+    def idata(self, jtype, k, lst, icase=5):
+        n = len(lst.c)
+        ipt = np.arange(1, n + 1, dtype=np.int32)
+        _check(lib().orc_wave_idata(self.ptr, icase, jtype, k, n, ipt, _f(lst.k1, np.int32),
+                                    _f(lst.k2, np.int32), _f(lst.k3, np.int32),
+                                    _f(lst.k4, np.int32), _f(lst.c)))
+
+    def odata(self, lst):
+        n = len(lst.c)
+        ipt = np.arange(1, n + 1, dtype=np.int32)
+        _check(lib().orc_wave_odata(self.ptr, n, ipt, _f(lst.k3, np.int32), _f(lst.k4, np.int32),
+                                    _f(lst.c)))
+
+    def symmetrize_diag(self):
+        lib().orc_wave_symmetrize_diag(self.ptr)
+
+    def scale(self, ec):
+        lib().orc_wave_scale(self.ptr, ec)
+
+    def acf_finish(self, kcore=0):
+        lib().orc_wave_acf_finish(self.ptr, kcore)
+
+
+def run_wave(job, grid: Grid | None = None):
+    """One partial wave in the order of SUB1 (src/bsr/BSR_MAT3/sub1.f90:70-219):
+    S = sb on the channel diagonals + O_data, symmetrised; H = EC*S + R^k
+    contributions (jpol outer, jtype inner, state_res.f90:155-159) +
+    (E_t - EC)*sb, symmetrised; ACF finished.  Returns (S, H) as
+    WaveMatrices (S.htarg/acf unused)."""
+    if grid is None:
+        grid = Grid(job.ks, job.ns, job.ml, job.me, job.h, job.t)
+    w = Wave(grid, job)
+    for ich in range(1, job.nch + 1):
+        w.update_hl(ich, ich, 1.0)
+    if job.olist is not None and len(job.olist.c):
+        w.odata(job.olist)
+    w.symmetrize_diag()
+    S = w.get()
+    w.scale(job.ec)
+    for k in range(0, job.mk + 1):
+        for jtype in range(1, 10):
+            lst = job.rk_lists.get((jtype, k))
+            if lst is not None and len(lst.c):
+                w.idata(jtype, k, lst)
+    for ich in range(1, job.nch + 1):
+        w.update_hl(ich, ich, float(job.etarg[job.iptar[ich - 1] - 1] - job.ec))
+    w.symmetrize_diag()
+    w.acf_finish(0)
+    H = w.get()
+    return S, H
+
+
+# ---------------------------------------------------------------- file format
+
+def _rec(f, payload: bytes):
+    n = struct.pack("<i", len(payload))
+    f.write(n + payload + n)
+
+
+def _record_matrix(f, m: WaveMatrices, hcc, hcb, hbb):
+    """Record_matrix, src/bsr/BSR_MAT3/record_matrix.f90:2-58."""
+    kch, kcp = m.nch, m.ncp
+    for ib in range(1, kch + 1):
+        _rec(f, np.asfortranarray(hcc[:, :, ib * (ib + 1) // 2 - 1]).tobytes(order="F"))
+    idiag = 1
+    for ich in range(1, kch + 1):
+        for jch in range(1, ich):
+            ib = ich * (ich - 1) // 2 + jch - 1
+            blk = hcc[:, :, ib]
+            if np.sum(np.abs(blk)) == 0.0:
+                continue
+            _rec(f, struct.pack("<2i", ich, jch))
+            _rec(f, np.asfortranarray(blk).tobytes(order="F"))
+            idiag = 0
+    if kcp > 0:
+        for ich in range(1, kch + 1):
+            for ip in range(1, kcp + 1):
+                v = hcb[:, ich - 1, ip - 1]
+                if np.sum(np.abs(v)) == 0.0:
+                    continue
+                _rec(f, struct.pack("<2i", ip + kch, ich))
+                _rec(f, v.tobytes())
+                idiag = 0
+        for ip in range(1, kcp + 1):
+            for jp in range(1, ip + 1):
+                ib = ip * (ip - 1) // 2 + jp - 1
+                if hbb[ib] == 0.0:
+                    continue
+                _rec(f, struct.pack("<2i", ip + kch, jp + kch))
+                _rec(f, struct.pack("<d", hbb[ib]))
+                if ip != jp:
+                    idiag = 0
+    _rec(f, struct.pack("<2i", 0, idiag))
+
+
+def write_bsr_mat(path, job, S: WaveMatrices, H: WaveMatrices):
+    """bsr_mat.nnn as SUB1 writes it (sub1.f90:107-110, 202, 258, 266-270)."""
+    with open(path, "wb") as f:
+        _rec(f, struct.pack("<3i", job.ns, job.nch, job.ncp))
+        _record_matrix(f, S, S.hcc, S.hcb, S.hbb)
+        _record_matrix(f, H, H.hcc, H.hcb, H.hbb)
+        _rec(f, struct.pack("<i", job.mk))
+        _rec(f, np.asfortranarray(H.acf).tobytes(order="F"))
+        _rec(f, struct.pack("<di", float(job.t[job.ns]), job.ns))
+        _rec(f, H.htarg.tobytes())
+        _rec(f, H.otarg.tobytes())
+        _rec(f, np.asarray(job.etarg, np.float64).tobytes())
+        _rec(f, struct.pack("<d", job.ec))
+
+
+def read_records(path):
+    """All records of a Fortran unformatted sequential file (4-byte markers)."""
+    out = []
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+    while pos < len(data):
+        (n,) = struct.unpack_from("<i", data, pos)
+        out.append(data[pos + 4: pos + 4 + n])
+        (m,) = struct.unpack_from("<i", data, pos + 4 + n)
+        if m != n:
+            raise ValueError("corrupt record markers")
+        pos += 8 + n
+    return out
+
+
+def read_bsr_mat(path):
+    """Parse bsr_mat.nnn the way BSR_HD reads it (src/bsr/BSR_HD3/diag_hd.f90:37-64):
+    returns dict with ns, nch, ncp, S and H (each: dict of diagonal blocks,
+    off-diagonal blocks keyed (i,j), idiag) and the trailing records."""
+    recs = read_records(path)
+    ns, nch, ncp = struct.unpack("<3i", recs[0])
+    pos = 1
+
+    def one():
+        nonlocal pos
+        m = {"diag": [], "blocks": {}, "vec": {}, "scal": {}}
+        for _ in range(nch):
+            m["diag"].append(np.frombuffer(recs[pos], np.float64).reshape(ns, ns, order="F"))
+            pos += 1
+        while True:
+            i, j = struct.unpack("<2i", recs[pos])
+            pos += 1
+            if i == 0:
+                m["idiag"] = j
+                return m
+            body = recs[pos]
+            pos += 1
+            if i <= nch:
+                m["blocks"][(i, j)] = np.frombuffer(body, np.float64).reshape(ns, ns, order="F")
+            elif j <= nch:
+                m["vec"][(i, j)] = np.frombuffer(body, np.float64).copy()
+            else:
+                m["scal"][(i, j)] = struct.unpack("<d", body)[0]
+
+    S = one()
+    H = one()
+    (mk,) = struct.unpack("<i", recs[pos]); pos += 1
+    acf = np.frombuffer(recs[pos], np.float64).reshape(nch, nch, mk + 1, order="F"); pos += 1
+    tmax, ns2 = struct.unpack("<di", recs[pos]); pos += 1
+    htarg = np.frombuffer(recs[pos], np.float64); pos += 1
+    otarg = np.frombuffer(recs[pos], np.float64); pos += 1
+    etarg = np.frombuffer(recs[pos], np.float64); pos += 1
+    (ec,) = struct.unpack("<d", recs[pos]); pos += 1
+    return dict(ns=ns, nch=nch, ncp=ncp, S=S, H=H, mk=mk, acf=acf, tmax=tmax,
+                ns2=ns2, htarg=htarg, otarg=otarg, etarg=etarg, ec=ec)
+
+
+def dense_from_wave(m: WaveMatrices):
+    """Full symmetric nhm x nhm matrix (nhm = nch*ns + ncp) from the block
+    arrays: channel blocks, channel-perturber vectors, perturber scalars."""
+    ns, nch, ncp = m.ns, m.nch, m.ncp
+    n = nch * ns + ncp
+    A = np.zeros((n, n))
+    for i in range(1, nch + 1):
+        for j in range(1, i + 1):
+            b = m.hcc[:, :, i * (i - 1) // 2 + j - 1]
+            A[(i - 1) * ns: i * ns, (j - 1) * ns: j * ns] = b
+            if i != j:
+                A[(j - 1) * ns: j * ns, (i - 1) * ns: i * ns] = b.T
+    for ip in range(ncp):
+        for ich in range(nch):
+            v = m.hcb[:, ich, ip]
+            A[nch * ns + ip, ich * ns:(ich + 1) * ns] = v
+            A[ich * ns:(ich + 1) * ns, nch * ns + ip] = v
+        for jp in range(ip + 1):
+            c = m.hbb[(ip + 1) * ip // 2 + jp]
+            A[nch * ns + ip, nch * ns + jp] = c
+            A[nch * ns + jp, nch * ns + ip] = c
+    return A

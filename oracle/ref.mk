@@ -1,0 +1,41 @@
+# Builds oracle/_ref/libbsr_ref.so from the reference's own Fortran sources
+# (read in place under REF, never copied) plus oracle/ref_driver.f90, with
+# amdflang.  Test infrastructure only.  LTO plus a version script that
+# exports only the ref_* entry points lets the linker drop routines this
+# driver never reaches (Target_new in mod_intmat.f90 calls LAPACK, which
+# the image does not have); nothing is stubbed.
+HERE := $(dir $(abspath $(lastword $(MAKEFILE_LIST))))
+REF ?= /root/reference/src
+FC := amdflang
+OUT := $(HERE)_ref
+FFLAGS := -O2 -fPIC -flto -module-dir $(OUT)/mod
+
+SRCS := \
+  $(REF)/libs/BS/spline_param.f90 $(REF)/libs/BS/spline_grid.f90 \
+  $(REF)/libs/BS/spline_atomic.f90 $(REF)/libs/BS/spline_moments.f90 \
+  $(REF)/libs/BS/spline_integrals.f90 $(REF)/libs/BS/spline_galerkin.f90 \
+  $(REF)/libs/BS/spline_orbitals.f90 $(REF)/libs/ZCOM/read_par.f90 \
+  $(REF)/libs/BS/define_grid.f90 $(REF)/libs/BS/gauss.f90 \
+  $(REF)/libs/BS/vbsplvd.f90 $(REF)/libs/BS/mrm.f90 \
+  $(REF)/libs/BS/moments.f90 $(REF)/libs/BS/rk_moments.f90 \
+  $(REF)/libs/BS/mrk_cell.f90 $(REF)/libs/BS/density.f90 \
+  $(REF)/libs/BS/convol.f90 $(REF)/libs/BS/sum_amb.f90 \
+  $(REF)/libs/BS/bav.f90 $(REF)/libs/BS/bxv.f90 $(REF)/libs/BS/bvmv.f90 \
+  $(REF)/libs/BS/quadr.f90 $(REF)/bsr/BSR_MAT3/mod_param.f90 \
+  $(REF)/libs/ZCONFLS/MOD_target.f90 $(REF)/libs/ZCONFLS/MOD_channel.f90 \
+  $(REF)/bsr/BSR_MAT3/mod_intmat.f90
+
+OBJS := $(addprefix $(OUT)/,$(notdir $(SRCS:.f90=.o)))
+
+all: $(OUT)/libbsr_ref.so
+
+$(OUT)/libbsr_ref.so: $(SRCS) $(HERE)ref_driver.f90 $(HERE)ref.mk
+	mkdir -p $(OUT)/mod
+	set -e; for f in $(SRCS); do \
+	  $(FC) $(FFLAGS) -c $$f -o $(OUT)/$$(basename $$f .f90).o; done
+	$(FC) $(FFLAGS) -c $(HERE)ref_driver.f90 -o $(OUT)/ref_driver.o
+	echo '{ global: ref_*; local: *; };' > $(OUT)/exports.map
+	$(FC) -flto -fuse-ld=lld -shared -o $@ $(OBJS) $(OUT)/ref_driver.o \
+	  -Wl,--version-script=$(OUT)/exports.map
+
+.PHONY: all

@@ -1,0 +1,206 @@
+"""ctypes access to oracle/_ref/libbsr_ref.so -- TEST INFRASTRUCTURE ONLY.
+
+The library is the reference's own Fortran (compiled by oracle/ref.mk from
+the reference checkout) driven through oracle/ref_driver.f90.  It exists
+only where it was built; ``available()`` says whether it can be loaded.
+Module state lives inside the library, so one grid is active at a time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PATH = os.path.join(HERE, "_ref", "libbsr_ref.so")
+_L = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+
+
+def build() -> bool:
+    """Build from the reference checkout when it is present."""
+    if not os.path.isdir("/root/reference/src"):
+        return os.path.exists(PATH)
+    subprocess.run(["make", "-s", "-f", os.path.join(HERE, "ref.mk")], check=True)
+    return True
+
+
+def available() -> bool:
+    return os.path.exists(PATH)
+
+
+def lib():
+    global _L
+    if _L is None:
+        L = C.CDLL(PATH)
+        i, d, ch = C.c_int, C.c_double, C.c_char
+        sig = {
+            "ref_mkgrid": (None, [i, i, d, d, d, d, i, _dp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(d)]),
+            "ref_gauss": (None, [i, _dp, _dp]),
+            "ref_init_grid": (None, [i, i, i, i, d, _dp]),
+            "ref_tables": (None, [_dp, _dp, _dp, _dp]),
+            "ref_moments": (None, [i, _dp]),
+            "ref_rk_moments": (None, [i, _dp, _dp, _dp]),
+            "ref_mrk_cell": (None, [i, _dp]),
+            "ref_density": (None, [i, i, _dp, _dp, _dp, ch]),
+            "ref_convol": (None, [i, i, i, _dp, _dp, i]),
+            "ref_sum_amb": (d, [i, i, _dp, _dp, ch]),
+            "ref_bav": (None, [i, i, _dp, _dp, _dp, ch, ch]),
+            "ref_bxv": (None, [i, i, _dp, _dp, _dp]),
+            "ref_bvmv": (d, [i, i, _dp, _dp, _dp]),
+            "ref_set_orbitals": (None, [i, _dp]),
+            "ref_quadr": (d, [i, i, i]),
+            "ref_alloc_matrix": (None, [i, i, i]),
+            "ref_update_hx": (None, [i, i, _dp, ch]),
+            "ref_update_hl": (None, [i, i, d]),
+            "ref_update_hw": (None, [i, i, _dp, _dp]),
+            "ref_update_hv": (None, [i, i, _dp, d]),
+            "ref_update_hb": (None, [i, i, d]),
+            "ref_update_cf": (None, [i, i, i, d]),
+            "ref_get_matrix": (None, [_dp, _dp, _dp, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _L = L
+    return _L
+
+
+def _f(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def mkgrid(kind, ks, z, h, hmax, rmax):
+    cap = 100000
+    t = np.zeros(cap, order="F")
+    ns, ml, me, hh = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    lib().ref_mkgrid(kind, ks, z, h, hmax, rmax, cap, t, ns, ml, me, hh)
+    return t[: ns.value + ks].copy(), ns.value, ml.value, me.value, hh.value
+
+
+def gauss(k):
+    x = np.zeros(k, order="F")
+    w = np.zeros(k, order="F")
+    lib().ref_gauss(k, x, w)
+    return x, w
+
+
+class RefGrid:
+    def __init__(self, ks, ns, ml, me, h, t):
+        self.ks, self.ns, self.nv = ks, ns, ns - ks + 1
+        lib().ref_init_grid(ks, ns, ml, me, h, _f(t))
+
+    @property
+    def ik(self):
+        return self.ks * (self.ks + 1) // 2
+
+    def tables(self):
+        ks, nv, ns = self.ks, self.nv, self.ns
+        gr = np.zeros((nv, ks), order="F")
+        grw = np.zeros((nv, ks), order="F")
+        bsp = np.zeros((nv, ks, ks), order="F")
+        sb = np.zeros((ns, ks), order="F")
+        lib().ref_tables(gr, grw, bsp, sb)
+        return gr, grw, bsp, sb
+
+    def moments(self, k):
+        r = np.zeros((self.ks * self.ks, self.nv), order="F")
+        lib().ref_moments(k, r)
+        return r[: self.ik, :].T.copy()
+
+    def rk_moments(self, k):
+        ik, nv = self.ik, self.nv
+        r1 = np.zeros((ik, nv), order="F")
+        r2 = np.zeros((ik, nv), order="F")
+        rd = np.zeros((ik, ik, nv), order="F")
+        lib().ref_rk_moments(k, r1, r2, rd)
+        # to the oracle's [iv][jj][ii] layout
+        return r1.T.copy(), r2.T.copy(), np.transpose(rd, (2, 1, 0)).copy()
+
+    def mrk_cell(self, k):
+        rkb = np.zeros((self.ns, self.ns, self.ks, self.ks), order="F")
+        lib().ref_mrk_cell(k, rkb)
+        return rkb
+
+    def convol(self, k, d, icase):
+        ns, ks = self.ns, self.ks
+        a = np.zeros((ns, ns), order="F")
+        dd = np.zeros((ns, ns), order="F")
+        dd[:, : d.shape[1]] = d
+        lib().ref_convol(k, ns, ks, a, dd, icase)
+        return a if icase > 2 else a[:, :ks].copy(order="F")
+
+    def set_orbitals(self, pbs):
+        lib().ref_set_orbitals(pbs.shape[1], _f(pbs))
+
+    def quadr(self, i, j, m):
+        return lib().ref_quadr(i, j, m)
+
+
+def density(ns, ks, p1, p2, typ):
+    cols = {"s": ks, "n": 2 * ks - 1, "x": ns}[typ]
+    d = np.zeros((ns, cols), order="F")
+    lib().ref_density(ns, ks, d, _f(p1), _f(p2), typ.encode())
+    return d
+
+
+def sum_amb(ns, ks, a, b, sym):
+    return lib().ref_sum_amb(ns, ks, _f(a), _f(b), sym.encode())
+
+
+def bav(ns, ks, b, v, typ="s", side="r"):
+    y = np.zeros(ns, order="F")
+    lib().ref_bav(ns, ks, _f(b), _f(v), y, typ.encode(), side.encode())
+    return y
+
+
+def bxv(ks, ns, b, v):
+    y = np.zeros(ns, order="F")
+    lib().ref_bxv(ks, ns, _f(b), _f(v), y)
+    return y
+
+
+def bvmv(ns, ks, a, x, y):
+    return lib().ref_bvmv(ns, ks, _f(a), _f(x), _f(y))
+
+
+class RefMatrix:
+    """module bsr_matrix with the UPDATE_* accumulators."""
+
+    def __init__(self, ns, nch, ncp, mk):
+        self.ns, self.nch, self.ncp, self.mk = ns, nch, ncp, mk
+        lib().ref_alloc_matrix(nch, ncp, mk)
+
+    def update_hx(self, ich, jch, d, sym):
+        dd = np.zeros((self.ns, self.ns), order="F")
+        dd[:, : d.shape[1]] = d
+        lib().ref_update_hx(ich, jch, dd, sym.encode())
+
+    def update_hl(self, ich, jch, c):
+        lib().ref_update_hl(ich, jch, c)
+
+    def update_hw(self, ich, jch, v, w):
+        lib().ref_update_hw(ich, jch, _f(v), _f(w))
+
+    def update_hv(self, ich, ic, v, c):
+        lib().ref_update_hv(ich, ic, _f(v), c)
+
+    def update_hb(self, ic, jc, c):
+        lib().ref_update_hb(ic, jc, c)
+
+    def update_cf(self, k, ich, jch, c):
+        lib().ref_update_cf(k, ich, jch, c)
+
+    def get(self):
+        ns, nch, ncp, mk = self.ns, self.nch, self.ncp, self.mk
+        nij = nch * (nch + 1) // 2
+        hcc = np.zeros((ns, ns, nij), order="F")
+        hcb = np.zeros((ns, nch, max(ncp, 1)), order="F")
+        hbb = np.zeros(max(ncp * (ncp + 1) // 2, 1), order="F")
+        acf = np.zeros((nch, nch, mk + 1), order="F")
+        lib().ref_get_matrix(hcc.reshape(-1, order="F"), hcb.reshape(-1, order="F"),
+                             hbb, acf.reshape(-1, order="F"))
+        return hcc, hcb, hbb, acf
