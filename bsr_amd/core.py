@@ -1,0 +1,149 @@
+"""Host mirror of the reference's bsr_mat interface over the C ABI.
+
+Method names follow the reference routines they replace (MRK_cell,
+rk_moments, convol, I_data, O_data, UPDATE_HL, ...); arguments keep the
+reference's meaning and 1-based indices, and errors raise BsrGpuError with
+the reference's Stop text where one exists.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib as L
+from .lib import f64, i32
+
+
+class Context:
+    """One device context: resident grid tables, R^k bands, orbital bases
+    and the matrices of the current partial wave."""
+
+    def __init__(self, device: int = 0):
+        self._p = C.c_void_p()
+        L.check(L.lib().bsrg_create(C.byref(self._p), device))
+        self.ks = self.ns = 0
+        self.nch = self.ncp = self.mk = 0
+        self._wave = None
+
+    def close(self):
+        if self._p:
+            L.lib().bsrg_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- grid and integrals ------------------------------------------------
+    def set_grid(self, ks, ns, ml, me, h, t):
+        L.check(L.lib().bsrg_set_grid(self._p, ks, ns, ml, me, h, f64(t)))
+        self.ks, self.ns = ks, ns
+
+    def mrk_cell(self, k, copy=True):
+        """R^k band rkb(ns,ns,ks,ks) (mrk_cell.f90:2-82)."""
+        out = np.zeros((self.ns, self.ns, self.ks, self.ks), order="F") if copy else None
+        L.check(L.lib().bsrg_mrk_cell(self._p, k, out))
+        return out
+
+    def rk_moments(self, k):
+        ks, ns = self.ks, self.ns
+        nv, ik = ns - ks + 1, ks * (ks + 1) // 2
+        r1 = np.zeros(nv * ik, order="F")
+        r2 = np.zeros(nv * ik, order="F")
+        rd = np.zeros(nv * ik * ik, order="F")
+        L.check(L.lib().bsrg_rk_moments(self._p, k, r1, r2, rd))
+        return r1.reshape(nv, ik), r2.reshape(nv, ik), rd.reshape(nv, ik, ik)
+
+    def convol(self, k, d, icase):
+        """convol(ns,ks,a,d,icase,'s','s') for the band of multipole k."""
+        ns, ks = self.ns, self.ks
+        shape = (ns, ks) if icase <= 2 else (ns, ns)
+        d = f64(d)
+        if d.shape != shape:
+            raise ValueError(f"density must have shape {shape}")
+        a = np.zeros(shape, order="F")
+        L.check(L.lib().bsrg_convol(self._p, k, icase, d, a))
+        return a
+
+    # ---- one partial wave, call by call --------------------------------------
+    def set_orbitals(self, pbs, iech, lbs, ibort, eps_ovl=1e-8):
+        pbs = f64(pbs)
+        L.check(L.lib().bsrg_set_orbitals(self._p, pbs.shape[1], pbs, i32(iech), i32(lbs), i32(ibort), eps_ovl))
+
+    def allocate_matrix(self, nch, ncp, mk):
+        L.check(L.lib().bsrg_alloc_matrix(self._p, nch, ncp, mk))
+        self.nch, self.ncp, self.mk = nch, ncp, mk
+
+    def update_hl(self, ich, jch, c):
+        L.check(L.lib().bsrg_update_hl(self._p, ich, jch, float(c)))
+
+    def i_data(self, jtype, jpol, lst, icase=5, ipt=None):
+        n = len(lst.c)
+        ipt = np.arange(1, n + 1) if ipt is None else ipt
+        L.check(L.lib().bsrg_idata(self._p, icase, jtype, jpol, n, i32(ipt), i32(lst.k1), i32(lst.k2),
+                                   i32(lst.k3), i32(lst.k4), f64(lst.c)))
+
+    def o_data(self, lst, eps_c=1e-10, ipt=None):
+        n = len(lst.c)
+        ipt = np.arange(1, n + 1) if ipt is None else ipt
+        L.check(L.lib().bsrg_odata(self._p, n, i32(ipt), i32(lst.k3), i32(lst.k4), f64(lst.c), eps_c))
+
+    def symmetrize_diag(self):
+        L.check(L.lib().bsrg_symmetrize_diag(self._p))
+
+    def scale(self, ec):
+        L.check(L.lib().bsrg_scale(self._p, float(ec)))
+
+    def acf_finish(self, kcore=0):
+        L.check(L.lib().bsrg_acf_finish(self._p, kcore))
+
+    def _alloc_out(self, nch, ncp, mk):
+        from .matrices import WaveMatrices
+        return WaveMatrices(self.ns, nch, ncp, mk)
+
+    def get_matrix(self):
+        m = self._alloc_out(self.nch, self.ncp, self.mk)
+        L.check(L.lib().bsrg_get_matrix(self._p, m.hcc, m.hcb if self.ncp else None, m.hbb if self.ncp else None,
+                                        m.acf, m.htarg, m.otarg))
+        return m
+
+    # ---- one partial wave, fused ---------------------------------------------
+    def wave_assemble(self, w, set_inputs=True):
+        """S and H of a PartialWave in one pass (SUB1, sub1.f90:70-219)."""
+        if set_inputs:
+            self.set_grid(w.ks, w.ns, w.ml, w.me, w.h, w.t)
+            self.set_orbitals(w.pbs, w.iech, w.lbs, w.ibort, w.eps_ovl)
+        jt, kp, k1, k2, k3, k4, c = w.concat()
+        o = w.olist
+        L.check(L.lib().bsrg_wave_assemble(
+            self._p, w.nch, w.ncp, w.mk, len(c), i32(jt), i32(kp), i32(k1), i32(k2), i32(k3), i32(k4), f64(c),
+            len(o.c), i32(o.k3), i32(o.k4), f64(o.c), w.eps_c, w.ec, w.ntarg, f64(w.etarg), i32(w.iptar)))
+        self._wave = (w.nch, w.ncp, w.mk)
+
+    def wave_get(self, which):
+        nch, ncp, mk = self._wave
+        m = self._alloc_out(nch, ncp, mk)
+        L.check(L.lib().bsrg_wave_get(self._p, which, m.hcc, m.hcb if ncp else None, m.hbb if ncp else None,
+                                      m.acf, m.htarg, m.otarg))
+        return m
+
+    def wave_write(self, path, w):
+        L.check(L.lib().bsrg_wave_write(self._p, str(path).encode(), float(w.t[w.ns]), w.ntarg, f64(w.etarg),
+                                        float(w.ec)))
+
+    def wave_timings(self):
+        t = np.zeros(5, order="F")
+        L.check(L.lib().bsrg_wave_timings(self._p, t))
+        return dict(bands=t[0], bases=t[1], gemm=t[2], terms=t[3], total=t[4])
+
+    def set_precision(self, mode, nslices=0):
+        L.check(L.lib().bsrg_set_precision(self._p, mode, nslices))

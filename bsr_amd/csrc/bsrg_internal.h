@@ -1,0 +1,87 @@
+// Internal declarations shared by the host set-up, the kernels and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bsrg {
+
+constexpr int kMaxKs = 16;    // largest spline order supported
+constexpr int kMaxNb = 8;     // largest number of bound orbitals per wave
+constexpr int kIbi = 32768;   // packing base of orbital pairs (cmdata ibi)
+constexpr int kIbo = 32768;   // packing base of overlaps (bsr_mat ibo)
+
+int fail(int code, const std::string& msg);
+double powi(double a, int n);
+int mkgrid(int kind, int ks, double z, double h, double hmax, double rmax, std::vector<double>& t,
+           int& ns, int& ml, int& me, double& hout);
+void gauss_legendre01(int k, double* x, double* w);
+void bspline_values(const double* t, int ks, int left, double x, double* b);
+double bvmv_sym(int n, int k, const double* a, const double* x, const double* y);
+void bxv_sym(int k, int n, const double* b, const double* v, double* y);
+
+// Host copy of the grid tables.  gr/grm/grw: [iv*ks+m];
+// bsp: [(iv*ks+m)*ks+ith] = B_{iv+ith}(gr(iv,m)) (0-based spline index);
+// sb: (ns,ks) lower band.
+struct GridTables {
+  int ks = 0, ns = 0, nv = 0, ml = 0, me = 0;
+  double h = 0;
+  std::vector<double> t, gx, gw, gr, grm, grw, bsp, sb;
+  void build(int ks, int ns, int ml, int me, double h, const double* t);
+  std::vector<double> mrm(int mm) const;
+};
+
+// ---- kernel launchers (kernels.hip) -------------------------------------
+struct DevGrid {
+  int ks, ns, nv, ml, me;
+  double h;
+  const double *t, *gx, *gw, *gr, *grm, *grw, *bsp;
+};
+
+// moments for multipoles k0..k0+nk-1: mom[((kk*2+s)*nv+iv)*ik+ii],
+// s=0: r^k, s=1: r^-(k+1); diagonal cells rkd[((kk*nv+iv)*ik+jj)*ik+ii].
+void launch_moments(const DevGrid& g, int k0, int nk, double* mom, double* rkd, hipStream_t s);
+// R^k bands for the same multipoles: band kk at rkb + kk*ns*ns*ks*ks.
+void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd, double* rkb,
+                 hipStream_t s);
+// upper-band densities dens[p](j,jp) of all unordered bound pairs p.
+void launch_pair_dens(int ns, int ks, int nb, const double* P, double* dens, hipStream_t s);
+// direct basis D[kk][p](i,ip) = sum d_p(j,jp) rkb_k(i,j,ip,jp)
+void launch_direct_basis(int ns, int ks, int nk, int npair, const double* rkb, const double* dens,
+                         double* D, hipStream_t s);
+// exchange basis E[kk][a][b](x,y) = sum K(x,z,y,w) p_a(z) p_b(w)
+void launch_exchange_basis(int ns, int ks, int nk, int nb, const double* rkb, const double* P,
+                           double* E, hipStream_t s);
+// scalar table St[kk][p1][p2] = sum_band D[kk][p1]*dens[p2]
+void launch_stab(int ns, int ks, int nk, int npair, const double* D, const double* dens, double* St,
+                 hipStream_t s);
+// vector table V[(kk*npair+p)*nb+b](x) = (D[kk][p] as symmetric band) * p_b
+void launch_vtab(int ns, int ks, int nk, int npair, int nb, const double* D, const double* P,
+                 double* V, hipStream_t s);
+// sb (lower band) -> upper-band form sbu(i,ith) = <B_i|B_{i+ith}>
+void launch_sb_upper(int ns, int ks, const double* sb, double* sbu, hipStream_t s);
+// hcc[blk] += sum_t c_t * band_t (symmetric), blocks listed in blks
+void launch_band_terms(int ns, int ks, int nblk, const int* blks, const int* ptr, const int* bid,
+                       const double* bc, const double* bands, double* hcc, hipStream_t s);
+// hcc[blk](x,y) += sum_r c_r * sfac_r * V[u_r](x) * V[w_r](y)
+void launch_rank1_terms(int ns, int nblk, const int* blks, const int* ptr, const int* uid,
+                        const int* wid, const int* sidx, const double* c, const double* stab,
+                        const double* V, double* hcc, hipStream_t s);
+// hcb[q] += sum_r c_r * sfac_r * V[v_r]
+void launch_vec_terms(int ns, int nq, const int* qs, const int* ptr, const int* vid, const int* sidx,
+                      const double* c, const double* stab, const double* V, double* hcb,
+                      hipStream_t s);
+void launch_symmetrize_diag(int ns, int nch, double* hcc, hipStream_t s);
+void launch_scale(size_t n, double a, double* x, hipStream_t s);
+void launch_fill(size_t n, double v, double* x, hipStream_t s);
+// leaf convolutions of one band with a general density
+void launch_convol_direct(int ns, int ks, int jcase, const double* rkb, const double* d, double* a,
+                          hipStream_t s);
+void launch_convol_exchange(int ns, int ks, const double* rkb, const double* d, double* a,
+                            hipStream_t s);
+
+}  // namespace bsrg

@@ -1,0 +1,1025 @@
+// C ABI of the MI355X radial core (declarations in include/bsr_gpu.h).
+//
+// Design: the R^k bands of all multipoles stay resident.  The orbital
+// contractions of I_data are linear in the coefficient lists, so the
+// contraction of every band with every bound-orbital pair is done once per
+// wave ("bases": direct bands D, exchange matrices E, scalars St and
+// vectors V).  A list then becomes terms on the channel blocks:
+//   exchange (jtype 8,9)  -> coefficient matrix of one FP64 GEMM  E^T * C
+//   direct   (jtype 6,7)  -> band terms  c * D   (symmetric band)
+//   rank-1   (UPDATE_HW)  -> c * u (x) w
+//   vectors  (UPDATE_HV)  -> c * v into hcb
+//   scalars  (HB, Target_h, ACF) on the host.
+// Each output element is summed by one thread in list order: no atomics,
+// results do not depend on what else runs on the device.
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/bsr_gpu.h"
+#include "bsrg_internal.h"
+
+namespace bsrg {
+
+static thread_local std::string g_err;
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail(-100, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+  } while (0)
+
+template <typename T>
+struct DevVec {
+  T* p = nullptr;
+  size_t n = 0;
+  DevVec() = default;
+  DevVec(const DevVec&) = delete;
+  DevVec& operator=(const DevVec&) = delete;
+  ~DevVec() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t resize(size_t m) {
+    if (m <= n && p) return hipSuccess;
+    release();
+    if (m == 0) return hipSuccess;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), m * sizeof(T));
+    if (e == hipSuccess) n = m;
+    return e;
+  }
+  hipError_t upload(const T* h, size_t m, hipStream_t s) {
+    hipError_t e = resize(m);
+    if (e != hipSuccess || m == 0) return e;
+    return hipMemcpyAsync(p, h, m * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+};
+
+struct Matrices {
+  int nch = 0, ncp = 0, mk = 0, ns = 0;
+  DevVec<double> hcc, hcb;
+  std::vector<double> hbb, acf, htarg, otarg;
+  size_t nblk() const { return static_cast<size_t>(nch) * (nch + 1) / 2; }
+  size_t hcc_n() const { return nblk() * ns * ns; }
+  size_t hcb_n() const { return static_cast<size_t>(ns) * nch * ncp; }
+  hipError_t alloc(int ns_, int nch_, int ncp_, int mk_) {
+    ns = ns_; nch = nch_; ncp = ncp_; mk = mk_;
+    hipError_t e = hcc.resize(std::max<size_t>(hcc_n(), 1));
+    if (e != hipSuccess) return e;
+    e = hcb.resize(std::max<size_t>(hcb_n(), 1));
+    hbb.assign(ncp > 0 ? static_cast<size_t>(ncp) * (ncp + 1) / 2 : 0, 0.0);
+    acf.assign(static_cast<size_t>(nch) * nch * (mk + 1), 0.0);
+    htarg.assign(nblk(), 0.0);
+    otarg.assign(nblk(), 0.0);
+    return e;
+  }
+};
+
+// Term lists for one matrix set.
+struct Terms {
+  struct Band { int blk, bid; double c; };
+  struct R1 { int blk, u, w, sidx; double c; };
+  struct Vec { int q, v, sidx; double c; };
+  struct Scal { int kind, idx, sidx; double c; };  // kind 0 hbb, 1 htarg, 2 otarg
+  std::vector<double> gemm;  // nt x nblk, t = kk*nb*nb + a*nb + b
+  int gemm_klo = 1 << 30, gemm_khi = -1;
+  std::vector<Band> band;
+  std::vector<R1> r1;
+  std::vector<Vec> vec;
+  std::vector<Scal> scal;
+  std::vector<double> acf_add;  // same shape as acf
+};
+
+}  // namespace bsrg
+
+using namespace bsrg;
+
+struct bsrg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  hipEvent_t ev[6] = {};
+  double timings[5] = {0, 0, 0, 0, 0};
+  int precision_mode = 0, nslices = 0;
+
+  GridTables G;
+  bool grid_ok = false;
+  DevVec<double> t, gx, gw, gr, grm, grw, bsp, sb;
+  int band_nk = 0;  // bands built for k = 0..band_nk-1
+  DevVec<double> rkb, mom, rkd;
+
+  // orbitals
+  bool orb_ok = false;
+  int nbf = 0, nb = 0, npair = 0;
+  std::vector<double> pbs, qbs, obs;
+  std::vector<int> iech, lbs, ibort, bound, bidx;
+  double eps_ovl = 1e-8;
+  DevVec<double> P, dens;
+  // bases for k = 0..basis_nk-1
+  int basis_nk = 0, exch_nk = 0;
+  DevVec<double> bands;  // [sbu | D(kk,p)] each ns*ks
+  DevVec<double> E, St, VT;
+  std::vector<double> St_host;
+  std::vector<std::vector<double>> Q;  // Q[k][(i-1)+nbf*(j-1)] = <p_i|r^k|p_j>, computed lazily
+  std::vector<char> Qok;
+  std::map<long long, int> wmap;        // GET_V registry (i1,i2) -> id
+  std::vector<double> wvec;             // nW x ns
+  int wvec_uploaded = -1;
+
+  Matrices cur, S, H;
+  bool wave_ok = false;
+  int wave_ntarg = 0;
+
+  // scratch for term uploads
+  DevVec<int> ti[6];
+  DevVec<double> td[2];
+  DevVec<double> gemm_coef;
+  DevVec<double> dtmp, dtmp2;
+};
+
+namespace {
+
+DevGrid devgrid(bsrg_ctx* c) {
+  DevGrid g;
+  g.ks = c->G.ks; g.ns = c->G.ns; g.nv = c->G.nv; g.ml = c->G.ml; g.me = c->G.me; g.h = c->G.h;
+  g.t = c->t.p; g.gx = c->gx.p; g.gw = c->gw.p; g.gr = c->gr.p; g.grm = c->grm.p; g.grw = c->grw.p;
+  g.bsp = c->bsp.p;
+  return g;
+}
+
+int check_ctx(bsrg_ctx* c) {
+  if (!c) return fail(-2, "null context");
+  return 0;
+}
+
+// Bands for k = 0..nk-1 (rebuilt as a whole when more multipoles are needed).
+int ensure_bands(bsrg_ctx* c, int nk) {
+  if (!c->grid_ok) return fail(-3, "grid not set (bsrg_set_grid)");
+  if (nk <= c->band_nk) return 0;
+  const int ks = c->G.ks, ns = c->G.ns, nv = c->G.nv, ik = ks * (ks + 1) / 2;
+  HIPCHK(c->mom.resize(static_cast<size_t>(nk) * 2 * nv * ik));
+  HIPCHK(c->rkd.resize(static_cast<size_t>(nk) * nv * ik * ik));
+  HIPCHK(c->rkb.resize(static_cast<size_t>(nk) * ns * ns * ks * ks));
+  DevGrid g = devgrid(c);
+  launch_moments(g, 0, nk, c->mom.p, c->rkd.p, c->stream);
+  launch_band(g, nk, c->mom.p, c->rkd.p, c->rkb.p, c->stream);
+  HIPCHK(hipGetLastError());
+  c->band_nk = nk;
+  c->basis_nk = 0;  // bases depend on the bands
+  c->exch_nk = 0;
+  return 0;
+}
+
+int pair_index(bsrg_ctx* c, int o1, int o2, int& p) {
+  if (o1 < 1 || o1 > c->nbf || o2 < 1 || o2 > c->nbf) return fail(-30, "orbital index out of range");
+  int a = c->bidx[o1 - 1], b = c->bidx[o2 - 1];
+  if (a < 0 || b < 0) return fail(-31, "R^k density of a channel orbital (no B-spline expansion)");
+  if (a > b) std::swap(a, b);
+  p = a * c->nb - a * (a - 1) / 2 + (b - a);
+  return 0;
+}
+
+int ensure_bases(bsrg_ctx* c, int nk, bool exchange) {
+  if (!c->orb_ok) return fail(-4, "orbitals not set (bsrg_set_orbitals)");
+  int rc = ensure_bands(c, nk);
+  if (rc) return rc;
+  const int ks = c->G.ks, ns = c->G.ns, nb = c->nb, np = c->npair;
+  if (c->basis_nk < nk) {
+    HIPCHK(c->bands.resize((1 + static_cast<size_t>(nk) * np) * ns * ks));
+    launch_sb_upper(ns, ks, c->sb.p, c->bands.p, c->stream);
+    double* D = c->bands.p + static_cast<size_t>(ns) * ks;
+    if (np > 0) {
+      launch_direct_basis(ns, ks, nk, np, c->rkb.p, c->dens.p, D, c->stream);
+      HIPCHK(c->St.resize(static_cast<size_t>(nk) * np * np));
+      launch_stab(ns, ks, nk, np, D, c->dens.p, c->St.p, c->stream);
+    }
+    const size_t nV = static_cast<size_t>(nk) * np * nb;
+    HIPCHK(c->VT.resize((nV + c->wvec.size() / std::max(ns, 1) + 1) * ns));
+    if (np > 0) launch_vtab(ns, ks, nk, np, nb, D, c->P.p, c->VT.p, c->stream);
+    HIPCHK(hipGetLastError());
+    c->St_host.assign(static_cast<size_t>(nk) * np * np, 0.0);
+    if (np > 0)
+      HIPCHK(hipMemcpyAsync(c->St_host.data(), c->St.p, c->St_host.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->basis_nk = nk;
+    c->wvec_uploaded = -1;
+  }
+  if (exchange && c->exch_nk < nk && nb > 0) {
+    if (nb > kMaxNb) return fail(-32, "too many bound orbitals for the exchange basis");
+    HIPCHK(c->E.resize(static_cast<size_t>(nk) * nb * nb * ns * ns));
+    launch_exchange_basis(ns, ks, nk, nb, c->rkb.p, c->P.p, c->E.p, c->stream);
+    HIPCHK(hipGetLastError());
+    c->exch_nk = nk;
+  }
+  return 0;
+}
+
+// GET_V (src/bsr/BSR_MAT3/get_V.f90:2-27) on the host: qbs(:,jj) minus the
+// projections on same-l bound orbitals that the channel orbital ii must be
+// orthogonal to.  Returns the vector id in the W part of the vector table.
+int get_v(bsrg_ctx* c, int ii, int jj, int& id) {
+  const int nbf = c->nbf, ns = c->G.ns;
+  if (ii < 1 || ii > nbf || jj < 1 || jj > nbf) return fail(-18, "get_V: orbital index out of range");
+  const long long key = static_cast<long long>(ii) * 1000003LL + jj;
+  auto it = c->wmap.find(key);
+  if (it != c->wmap.end()) { id = it->second; return 0; }
+  if (c->ibort[(ii - 1) + static_cast<size_t>(nbf) * (jj - 1)] == 0) return fail(-19, "get_V: <kl|nl> = 0 ?");
+  std::vector<double> v(c->qbs.begin() + static_cast<size_t>(ns) * (jj - 1),
+                        c->qbs.begin() + static_cast<size_t>(ns) * jj);
+  for (int i = 1; i <= nbf; ++i) {
+    if (i == jj || c->iech[i - 1] != 0 || c->lbs[i - 1] != c->lbs[ii - 1]) continue;
+    if (c->ibort[(i - 1) + static_cast<size_t>(nbf) * (ii - 1)] != 0) continue;
+    const double o = c->obs[(i - 1) + static_cast<size_t>(nbf) * (jj - 1)];
+    if (std::fabs(o) < c->eps_ovl) continue;
+    for (int r = 0; r < ns; ++r) v[r] -= o * c->qbs[static_cast<size_t>(ns) * (i - 1) + r];
+  }
+  id = static_cast<int>(c->wvec.size() / ns);
+  c->wvec.insert(c->wvec.end(), v.begin(), v.end());
+  c->wmap[key] = id;
+  return 0;
+}
+
+double quadr(bsrg_ctx* c, int i, int j, int m) {
+  if (static_cast<int>(c->Q.size()) <= m) { c->Q.resize(m + 1); c->Qok.resize(m + 1, 0); }
+  if (!c->Qok[m]) {
+    const int ns = c->G.ns, ks = c->G.ks, nbf = c->nbf;
+    std::vector<double> rm = c->G.mrm(m);
+    c->Q[m].assign(static_cast<size_t>(nbf) * nbf, 0.0);
+    for (int a = 0; a < nbf; ++a) {
+      if (c->iech[a] != 0) continue;
+      for (int b = 0; b < nbf; ++b) {
+        if (c->iech[b] != 0) continue;
+        c->Q[m][a + static_cast<size_t>(nbf) * b] =
+            bvmv_sym(ns, ks, rm.data(), &c->pbs[static_cast<size_t>(ns) * a], &c->pbs[static_cast<size_t>(ns) * b]);
+      }
+    }
+    c->Qok[m] = 1;
+  }
+  return c->Q[m][(i - 1) + static_cast<size_t>(c->nbf) * (j - 1)];
+}
+
+inline int blk_index(int ich, int jch) {
+  const int i = std::max(ich, jch), j = std::min(ich, jch);
+  return i * (i - 1) / 2 + j - 1;
+}
+
+int chk_ch(const Matrices& M, int ich, int jch, const char* who) {
+  if (ich < 1 || ich > M.nch || jch < 1 || jch > M.nch)
+    return fail(-10, std::string(who) + ": channel index out of range");
+  return 0;
+}
+
+// Add one structure-1 record (I_data jtype 1 or O_data) to the terms.
+// sidx >= 0 multiplies by the scalar R^k table entry (I_data), -1 for O_data.
+int add_struct1(bsrg_ctx* c, const Matrices& M, Terms& T, int K3, int K4, int sidx, double cc, bool odata,
+                bool scal_targets) {
+  const int ic = -K3, jc = -K4;
+  const int nV = c->basis_nk * c->npair * c->nb;
+  if (ic > 0 && jc > 0) {
+    if (ic > M.ncp || jc > M.ncp) return fail(-12, "UPDATE_HB: indeces out of range");
+    const int i = std::max(ic, jc), j = std::min(ic, jc);
+    T.scal.push_back({0, i * (i - 1) / 2 + j - 1, sidx, cc});
+  } else if (ic < 0 && jc > 0) {
+    const int io = -ic, i1 = io / kIbo, i2 = io % kIbo;
+    int w, rc = get_v(c, i1, i2, w);
+    if (rc) return rc;
+    const int ich = c->iech[i1 - 1];
+    if (ich < 1 || ich > M.nch) return fail(-13, "UPDATE_HV: channel index out of range");
+    if (jc > M.ncp) return fail(-14, "UPDATE_HV: bound index out of range");
+    T.vec.push_back({(jc - 1) * M.nch + (ich - 1), nV + w, sidx, cc});
+  } else if (ic < 0 && jc < 0) {
+    const int io = -ic, i1 = io / kIbo, i2 = io % kIbo, jo = -jc, j1 = jo / kIbo, j2 = jo % kIbo;
+    int v, w, rc = get_v(c, i1, i2, v);
+    if (!rc) rc = get_v(c, j1, j2, w);
+    if (rc) return rc;
+    const int ich = c->iech[i1 - 1], jch = c->iech[j1 - 1];
+    if ((rc = chk_ch(M, ich, jch, "UPDATE_HW"))) return rc;
+    if (ich >= jch) T.r1.push_back({blk_index(ich, jch), nV + v, nV + w, sidx, cc});
+    else T.r1.push_back({blk_index(ich, jch), nV + w, nV + v, sidx, cc});
+  } else if (ic < 0 && jc == 0) {
+    const int io = -ic, i1 = io / kIbo, i2 = io % kIbo;
+    if (i1 < 1 || i1 > c->nbf || i2 < 1 || i2 > c->nbf) return fail(-30, "orbital index out of range");
+    const int ich = c->iech[i1 - 1], jch = c->iech[i2 - 1];
+    int rc = chk_ch(M, ich, jch, "UPDATE_CF");
+    if (rc) return rc;
+    if (scal_targets) T.scal.push_back({odata ? 2 : 1, blk_index(ich, jch), sidx, cc});
+  } else {
+    return fail(odata ? -24 : -22, odata ? " O_data: unknown structure " : " I_data: unknown structure for itype=1");
+  }
+  return 0;
+}
+
+const int kIjcase[9] = {2, 1, 2, 1, 2, 1, 2, 3, 4};  // IJCASE, mod_cmdata.f90:54
+
+// I_data for one (jtype, k) list -> terms (I_data.f90:2-224).
+int add_idata(bsrg_ctx* c, const Matrices& M, Terms& T, int jtype, int k, int n, const int* ipt, const int* K1,
+              const int* K2, const int* K3, const int* K4, const double* cd, double sc) {
+  if (k < 0 || k >= c->basis_nk) return fail(-33, "multipole outside the prepared range");
+  const int nb = c->nb, np = c->npair, nV = c->basis_nk * np * nb;
+  const int jcase = kIjcase[jtype - 1];
+  (void)nV;
+  if (jtype == 1) {
+    for (int q = 0; q < n; ++q) {
+      const int i = ipt ? ipt[q] - 1 : q;
+      int p1, p2, rc = pair_index(c, K1[i] / kIbi, K1[i] % kIbi, p1);
+      if (!rc) rc = pair_index(c, K2[i] / kIbi, K2[i] % kIbi, p2);
+      if (!rc) rc = add_struct1(c, M, T, K3[i], K4[i], (k * np + p1) * np + p2, cd[i] * sc, false, true);
+      if (rc) return rc;
+    }
+  } else if (jtype <= 5) {
+    for (int q = 0; q < n; ++q) {
+      const int i = ipt ? ipt[q] - 1 : q;
+      int p, rc = pair_index(c, K1[i] / kIbi, K1[i] % kIbi, p);
+      if (rc) return rc;
+      const int ii = K2[i];
+      if (ii < 1 || ii > c->nbf || c->bidx[ii - 1] < 0) return fail(-31, "BAV with a channel orbital");
+      const int vid = (k * np + p) * nb + c->bidx[ii - 1];
+      const int ich = K3[i], io = K4[i];
+      if (io > 0) {
+        const int j1 = io / kIbo, j2 = io % kIbo;
+        int w;
+        if ((rc = get_v(c, j1, j2, w))) return rc;
+        const int jch = c->iech[j1 - 1];
+        if ((rc = chk_ch(M, ich, jch, "UPDATE_HW"))) return rc;
+        const int nVv = c->basis_nk * np * nb;
+        if (ich >= jch) T.r1.push_back({blk_index(ich, jch), vid, nVv + w, -1, cd[i] * sc});
+        else T.r1.push_back({blk_index(ich, jch), nVv + w, vid, -1, cd[i] * sc});
+      } else if (io < 0) {
+        if (ich < 1 || ich > M.nch) return fail(-13, "UPDATE_HV: channel index out of range");
+        if (-io > M.ncp) return fail(-14, "UPDATE_HV: bound index out of range");
+        T.vec.push_back({(-io - 1) * M.nch + (ich - 1), vid, -1, cd[i] * sc});
+      } else {
+        return fail(-23, " I_data: uknown structure for itype 2-5 ");
+      }
+    }
+  } else {
+    double CA = 0.0;
+    for (int q = 0; q < n; ++q) {
+      const int i = ipt ? ipt[q] - 1 : q;
+      const int o1 = K2[i], o2 = K3[i];
+      if (o1 < 1 || o1 > c->nbf || o2 < 1 || o2 > c->nbf || c->bidx[o1 - 1] < 0 || c->bidx[o2 - 1] < 0)
+        return fail(-31, "R^k density of a channel orbital (no B-spline expansion)");
+      const int ich = K1[i] / kIbi, jch = K1[i] % kIbi;
+      int rc = chk_ch(M, ich, jch, "UPDATE_HX");
+      if (rc) return rc;
+      const int a = c->bidx[o1 - 1], b = c->bidx[o2 - 1];
+      const double cc = cd[i] * sc;
+      if (jcase <= 2) {
+        int p;
+        pair_index(c, o1, o2, p);
+        T.band.push_back({blk_index(ich, jch), 1 + k * np + p, cc});
+        CA += cd[i] * quadr(c, o1, o2, k);
+      } else {
+        const size_t nt = static_cast<size_t>(c->basis_nk) * nb * nb;
+        if (T.gemm.empty()) T.gemm.assign(nt * M.nblk(), 0.0);
+        const int t = k * nb * nb + (ich >= jch ? a * nb + b : b * nb + a);
+        T.gemm[t + nt * blk_index(ich, jch)] += cc;
+        T.gemm_klo = std::min(T.gemm_klo, k);
+        T.gemm_khi = std::max(T.gemm_khi, k);
+      }
+      const bool last = (q == n - 1) || (K1[ipt ? ipt[q + 1] - 1 : q + 1] != K1[i]);
+      if (last) {
+        if (jcase <= 2) {
+          if (k > M.mk) return fail(-17, "UPDATE_CF: multipole index out of range");
+          if (T.acf_add.empty()) T.acf_add.assign(M.acf.size(), 0.0);
+          T.acf_add[static_cast<size_t>(k) * M.nch * M.nch + (ich - 1) + static_cast<size_t>(M.nch) * (jch - 1)] += CA;
+        }
+        CA = 0.0;
+      }
+    }
+  }
+  return 0;
+}
+
+template <typename T, typename Key>
+void stable_sort_by(std::vector<T>& v, Key key) {
+  std::stable_sort(v.begin(), v.end(), [&](const T& a, const T& b) { return key(a) < key(b); });
+}
+
+// Apply terms to a matrix set.  overwrite: hcc/hcb start from zero.
+int apply_terms(bsrg_ctx* c, Matrices& M, Terms& T, bool overwrite) {
+  const int ns = c->G.ns, ks = c->G.ks, nb = c->nb;
+  hipStream_t s = c->stream;
+  // W vectors (GET_V) into the vector table
+  const size_t nV = static_cast<size_t>(c->basis_nk) * c->npair * nb;
+  const size_t nW = c->wvec.size() / std::max(ns, 1);
+  if (static_cast<int>(nW) != c->wvec_uploaded) {
+    if (c->VT.n < (nV + nW) * ns) {
+      // grow, keeping the V part
+      DevVec<double> nvt;
+      HIPCHK(nvt.resize((nV + nW + 16) * ns));
+      if (nV) HIPCHK(hipMemcpyAsync(nvt.p, c->VT.p, nV * ns * sizeof(double), hipMemcpyDeviceToDevice, s));
+      std::swap(nvt.p, c->VT.p);
+      std::swap(nvt.n, c->VT.n);
+      HIPCHK(hipStreamSynchronize(s));
+    }
+    if (nW) HIPCHK(hipMemcpyAsync(c->VT.p + nV * ns, c->wvec.data(), nW * ns * sizeof(double), hipMemcpyHostToDevice, s));
+    c->wvec_uploaded = static_cast<int>(nW);
+  }
+  (void)hipEventRecord(c->ev[2], s);
+  // exchange GEMM (or clear)
+  const size_t nblk = M.nblk();
+  const size_t nss = static_cast<size_t>(ns) * ns;
+  if (!T.gemm.empty() && T.gemm_khi >= 0) {
+    const size_t nt = static_cast<size_t>(c->basis_nk) * nb * nb;
+    const int klo = T.gemm_klo, khi = T.gemm_khi;
+    const size_t r0 = static_cast<size_t>(klo) * nb * nb, nr = static_cast<size_t>(khi - klo + 1) * nb * nb;
+    HIPCHK(c->gemm_coef.upload(T.gemm.data(), T.gemm.size(), s));
+    const double alpha = 1.0, beta = overwrite ? 0.0 : 1.0;
+    rocblas_status st = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, static_cast<int>(nss),
+                                      static_cast<int>(nblk), static_cast<int>(nr), &alpha, c->E.p + r0 * nss,
+                                      static_cast<int>(nss), c->gemm_coef.p + r0, static_cast<int>(nt), &beta,
+                                      M.hcc.p, static_cast<int>(nss));
+    if (st != rocblas_status_success) return fail(-101, "rocblas_dgemm failed");
+  } else if (overwrite) {
+    launch_fill(M.hcc_n(), 0.0, M.hcc.p, s);
+  }
+  (void)hipEventRecord(c->ev[3], s);
+  if (overwrite) launch_fill(M.hcb_n(), 0.0, M.hcb.p, s);
+  // band terms
+  if (!T.band.empty()) {
+    stable_sort_by(T.band, [](const Terms::Band& b) { return b.blk; });
+    std::vector<int> blks, ptr, bid;
+    std::vector<double> bc;
+    for (size_t q = 0; q < T.band.size(); ++q) {
+      if (q == 0 || T.band[q].blk != T.band[q - 1].blk) { blks.push_back(T.band[q].blk); ptr.push_back(static_cast<int>(q)); }
+      bid.push_back(T.band[q].bid);
+      bc.push_back(T.band[q].c);
+    }
+    ptr.push_back(static_cast<int>(T.band.size()));
+    HIPCHK(c->ti[0].upload(blks.data(), blks.size(), s));
+    HIPCHK(c->ti[1].upload(ptr.data(), ptr.size(), s));
+    HIPCHK(c->ti[2].upload(bid.data(), bid.size(), s));
+    HIPCHK(c->td[0].upload(bc.data(), bc.size(), s));
+    launch_band_terms(ns, ks, static_cast<int>(blks.size()), c->ti[0].p, c->ti[1].p, c->ti[2].p, c->td[0].p,
+                      c->bands.p, M.hcc.p, s);
+    HIPCHK(hipStreamSynchronize(s));  // scratch buffers are reused below
+  }
+  // rank-1 terms
+  if (!T.r1.empty()) {
+    stable_sort_by(T.r1, [](const Terms::R1& r) { return r.blk; });
+    std::vector<int> blks, ptr, u, w, si;
+    std::vector<double> cc;
+    for (size_t q = 0; q < T.r1.size(); ++q) {
+      if (q == 0 || T.r1[q].blk != T.r1[q - 1].blk) { blks.push_back(T.r1[q].blk); ptr.push_back(static_cast<int>(q)); }
+      u.push_back(T.r1[q].u); w.push_back(T.r1[q].w); si.push_back(T.r1[q].sidx); cc.push_back(T.r1[q].c);
+    }
+    ptr.push_back(static_cast<int>(T.r1.size()));
+    HIPCHK(c->ti[0].upload(blks.data(), blks.size(), s));
+    HIPCHK(c->ti[1].upload(ptr.data(), ptr.size(), s));
+    HIPCHK(c->ti[2].upload(u.data(), u.size(), s));
+    HIPCHK(c->ti[3].upload(w.data(), w.size(), s));
+    HIPCHK(c->ti[4].upload(si.data(), si.size(), s));
+    HIPCHK(c->td[0].upload(cc.data(), cc.size(), s));
+    launch_rank1_terms(ns, static_cast<int>(blks.size()), c->ti[0].p, c->ti[1].p, c->ti[2].p, c->ti[3].p, c->ti[4].p,
+                       c->td[0].p, c->St.p, c->VT.p, M.hcc.p, s);
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  // hcb vector terms
+  if (!T.vec.empty()) {
+    stable_sort_by(T.vec, [](const Terms::Vec& v) { return v.q; });
+    std::vector<int> qs, ptr, vid, si;
+    std::vector<double> cc;
+    for (size_t q = 0; q < T.vec.size(); ++q) {
+      if (q == 0 || T.vec[q].q != T.vec[q - 1].q) { qs.push_back(T.vec[q].q); ptr.push_back(static_cast<int>(q)); }
+      vid.push_back(T.vec[q].v); si.push_back(T.vec[q].sidx); cc.push_back(T.vec[q].c);
+    }
+    ptr.push_back(static_cast<int>(T.vec.size()));
+    HIPCHK(c->ti[0].upload(qs.data(), qs.size(), s));
+    HIPCHK(c->ti[1].upload(ptr.data(), ptr.size(), s));
+    HIPCHK(c->ti[2].upload(vid.data(), vid.size(), s));
+    HIPCHK(c->ti[3].upload(si.data(), si.size(), s));
+    HIPCHK(c->td[0].upload(cc.data(), cc.size(), s));
+    launch_vec_terms(ns, static_cast<int>(qs.size()), c->ti[0].p, c->ti[1].p, c->ti[2].p, c->ti[3].p, c->td[0].p,
+                     c->St.p, c->VT.p, M.hcb.p, s);
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  HIPCHK(hipGetLastError());
+  (void)hipEventRecord(c->ev[4], s);
+  // host scalars
+  if (overwrite) {
+    std::fill(M.hbb.begin(), M.hbb.end(), 0.0);
+  }
+  for (const auto& sc : T.scal) {
+    const double f = sc.c * (sc.sidx >= 0 ? c->St_host[sc.sidx] : 1.0);
+    if (sc.kind == 0) M.hbb[sc.idx] += f;
+    else if (sc.kind == 1) M.htarg[sc.idx] += f;
+    else M.otarg[sc.idx] += f;
+  }
+  if (!T.acf_add.empty())
+    for (size_t q = 0; q < M.acf.size(); ++q) M.acf[q] += T.acf_add[q];
+  return 0;
+}
+
+void band_hl(Terms& T, int ich, int jch, double c) { T.band.push_back({blk_index(ich, jch), 0, c}); }
+
+int add_odata(bsrg_ctx* c, const Matrices& M, Terms& T, int n, const int* ipt, const int* K3, const int* K4,
+              const double* cd, double eps_c, double sc, bool targets) {
+  for (int q = 0; q < n; ++q) {
+    const int i = ipt ? ipt[q] - 1 : q;
+    if (std::fabs(cd[i]) < eps_c) continue;
+    int rc = add_struct1(c, M, T, K3[i], K4[i], -1, cd[i] * sc, true, targets);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+void acf_finish(Matrices& M, int kcore) {
+  const int nch = M.nch;
+  const double eps_acf = 1e-5;
+  for (int k = 0; k <= M.mk; ++k)
+    for (int i = 0; i < nch; ++i)
+      for (int j = i; j < nch; ++j) {
+        double* a = &M.acf[static_cast<size_t>(k) * nch * nch];
+        double v = a[i + static_cast<size_t>(nch) * j] + a[j + static_cast<size_t>(nch) * i];
+        if (i != j) v *= 2.0;
+        if (std::fabs(v) < eps_acf) v = 0.0;
+        a[i + static_cast<size_t>(nch) * j] = v;
+        a[j + static_cast<size_t>(nch) * i] = v;
+      }
+  for (int i = 0; i < nch; ++i) M.acf[i + static_cast<size_t>(nch) * i] += kcore;
+}
+
+int get_matrices(bsrg_ctx* c, Matrices& M, double* hcc, double* hcb, double* hbb, double* acf, double* htarg,
+                 double* otarg) {
+  if (hcc && M.hcc_n()) HIPCHK(hipMemcpyAsync(hcc, M.hcc.p, M.hcc_n() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (hcb && M.hcb_n()) HIPCHK(hipMemcpyAsync(hcb, M.hcb.p, M.hcb_n() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (hbb && !M.hbb.empty()) std::memcpy(hbb, M.hbb.data(), M.hbb.size() * sizeof(double));
+  if (acf) std::memcpy(acf, M.acf.data(), M.acf.size() * sizeof(double));
+  if (htarg) std::memcpy(htarg, M.htarg.data(), M.htarg.size() * sizeof(double));
+  if (otarg) std::memcpy(otarg, M.otarg.data(), M.otarg.size() * sizeof(double));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bsrg_last_error(void) { return g_err.c_str(); }
+int bsrg_abi_version(void) { return 1; }
+
+int bsrg_mkgrid(int kind, int ks, double z, double h, double hmax, double rmax, int cap, double* t, int* ns, int* ml,
+                int* me, double* h_out) {
+  std::vector<double> tv;
+  int n, l, e;
+  double hh;
+  int rc = mkgrid(kind, ks, z, h, hmax, rmax, tv, n, l, e, hh);
+  if (rc) return rc;
+  if (static_cast<int>(tv.size()) > cap) return fail(-1, "bsrg_mkgrid: knot buffer too small");
+  std::memcpy(t, tv.data(), tv.size() * sizeof(double));
+  *ns = n; *ml = l; *me = e; *h_out = hh;
+  return 0;
+}
+
+int bsrg_grid_tables(int ks, int ns, const double* t, double* gr, double* grw, double* bsp, double* sb) {
+  if (ks < 2 || ks > kMaxKs || ns < ks) return fail(-1, "bsrg_grid_tables: bad ks/ns");
+  GridTables G;
+  G.build(ks, ns, 0, 0, 0.0, t);
+  if (gr) std::memcpy(gr, G.gr.data(), G.gr.size() * sizeof(double));
+  if (grw) std::memcpy(grw, G.grw.data(), G.grw.size() * sizeof(double));
+  if (bsp) std::memcpy(bsp, G.bsp.data(), G.bsp.size() * sizeof(double));
+  if (sb) std::memcpy(sb, G.sb.data(), G.sb.size() * sizeof(double));
+  return 0;
+}
+
+int bsrg_create(bsrg_ctx** out, int device) {
+  if (!out) return fail(-2, "null output pointer");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(-5, "no such HIP device");
+  HIPCHK(hipSetDevice(device));
+  std::unique_ptr<bsrg_ctx> c(new bsrg_ctx());
+  c->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if (rocblas_create_handle(&c->blas) != rocblas_status_success) return fail(-101, "rocblas_create_handle failed");
+  rocblas_set_stream(c->blas, c->stream);
+  for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+  *out = c.release();
+  return 0;
+}
+
+int bsrg_destroy(bsrg_ctx* c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->blas) rocblas_destroy_handle(c->blas);
+  for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  hipStream_t s = c->stream;
+  delete c;
+  if (s) (void)hipStreamDestroy(s);
+  return 0;
+}
+
+int bsrg_set_grid(bsrg_ctx* c, int ks, int ns, int ml, int me, double h, const double* t) {
+  if (int rc = check_ctx(c)) return rc;
+  if (ks < 2 || ks > kMaxKs || ns < ks || !t) return fail(-1, "bsrg_set_grid: bad ks/ns");
+  for (int i = 1; i < ns + ks; ++i)
+    if (t[i] < t[i - 1]) return fail(-1, "bsrg_set_grid: knots must be non-decreasing");
+  HIPCHK(hipSetDevice(c->device));
+  c->G.build(ks, ns, ml, me, h, t);
+  hipStream_t s = c->stream;
+  HIPCHK(c->t.upload(c->G.t.data(), c->G.t.size(), s));
+  HIPCHK(c->gx.upload(c->G.gx.data(), c->G.gx.size(), s));
+  HIPCHK(c->gw.upload(c->G.gw.data(), c->G.gw.size(), s));
+  HIPCHK(c->gr.upload(c->G.gr.data(), c->G.gr.size(), s));
+  HIPCHK(c->grm.upload(c->G.grm.data(), c->G.grm.size(), s));
+  HIPCHK(c->grw.upload(c->G.grw.data(), c->G.grw.size(), s));
+  HIPCHK(c->bsp.upload(c->G.bsp.data(), c->G.bsp.size(), s));
+  HIPCHK(c->sb.upload(c->G.sb.data(), c->G.sb.size(), s));
+  HIPCHK(hipStreamSynchronize(s));
+  c->grid_ok = true;
+  c->band_nk = 0;
+  c->basis_nk = 0;
+  c->exch_nk = 0;
+  c->orb_ok = false;
+  c->wave_ok = false;
+  return 0;
+}
+
+int bsrg_mrk_cell(bsrg_ctx* c, int k, double* rkb) {
+  if (int rc = check_ctx(c)) return rc;
+  if (k < 0) return fail(-1, "bsrg_mrk_cell: k < 0");
+  HIPCHK(hipSetDevice(c->device));
+  if (int rc = ensure_bands(c, k + 1)) return rc;
+  const size_t n = static_cast<size_t>(c->G.ns) * c->G.ns * c->G.ks * c->G.ks;
+  if (rkb) HIPCHK(hipMemcpyAsync(rkb, c->rkb.p + k * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int bsrg_rk_moments(bsrg_ctx* c, int k, double* rkd1, double* rkd2, double* rkd) {
+  if (int rc = check_ctx(c)) return rc;
+  if (k < 0) return fail(-1, "bsrg_rk_moments: k < 0");
+  HIPCHK(hipSetDevice(c->device));
+  if (int rc = ensure_bands(c, k + 1)) return rc;
+  const int ks = c->G.ks, nv = c->G.nv, ik = ks * (ks + 1) / 2;
+  const double* m1 = c->mom.p + static_cast<size_t>(k * 2) * nv * ik;
+  if (rkd1) HIPCHK(hipMemcpyAsync(rkd1, m1, sizeof(double) * nv * ik, hipMemcpyDeviceToHost, c->stream));
+  if (rkd2) HIPCHK(hipMemcpyAsync(rkd2, m1 + static_cast<size_t>(nv) * ik, sizeof(double) * nv * ik, hipMemcpyDeviceToHost, c->stream));
+  if (rkd) HIPCHK(hipMemcpyAsync(rkd, c->rkd.p + static_cast<size_t>(k) * nv * ik * ik, sizeof(double) * nv * ik * ik,
+                                 hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int bsrg_convol(bsrg_ctx* c, int k, int icase, const double* d, double* a) {
+  if (int rc = check_ctx(c)) return rc;
+  if (icase < 1 || icase > 4) return fail(-2, " CONVOL:  unknown case ");
+  HIPCHK(hipSetDevice(c->device));
+  if (int rc = ensure_bands(c, k + 1)) return rc;
+  const int ns = c->G.ns, ks = c->G.ks;
+  const size_t nin = icase <= 2 ? static_cast<size_t>(ns) * ks : static_cast<size_t>(ns) * ns;
+  HIPCHK(c->dtmp.upload(d, nin, c->stream));
+  HIPCHK(c->dtmp2.resize(nin));
+  const double* R = c->rkb.p + static_cast<size_t>(k) * ns * ns * ks * ks;
+  if (icase <= 2) launch_convol_direct(ns, ks, icase, R, c->dtmp.p, c->dtmp2.p, c->stream);
+  else launch_convol_exchange(ns, ks, R, c->dtmp.p, c->dtmp2.p, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(a, c->dtmp2.p, nin * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int bsrg_set_orbitals(bsrg_ctx* c, int nbf, const double* pbs, const int* iech, const int* lbs, const int* ibort,
+                      double eps_ovl) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->grid_ok) return fail(-3, "grid not set (bsrg_set_grid)");
+  if (nbf < 1 || !pbs || !iech || !lbs || !ibort) return fail(-1, "bsrg_set_orbitals: bad arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const int ns = c->G.ns, ks = c->G.ks;
+  c->nbf = nbf;
+  c->pbs.assign(pbs, pbs + static_cast<size_t>(ns) * nbf);
+  c->iech.assign(iech, iech + nbf);
+  c->lbs.assign(lbs, lbs + nbf);
+  c->ibort.assign(ibort, ibort + static_cast<size_t>(nbf) * nbf);
+  c->eps_ovl = eps_ovl;
+  c->bound.clear();
+  c->bidx.assign(nbf, -1);
+  for (int i = 0; i < nbf; ++i)
+    if (iech[i] == 0) { c->bidx[i] = static_cast<int>(c->bound.size()); c->bound.push_back(i + 1); }
+  c->nb = static_cast<int>(c->bound.size());
+  c->npair = c->nb * (c->nb + 1) / 2;
+  // qbs = sb * p, OBS for bound orbitals of equal l (read_data.f90:142-208)
+  c->qbs.assign(static_cast<size_t>(ns) * nbf, 0.0);
+  c->obs.assign(static_cast<size_t>(nbf) * nbf, 0.0);
+  for (int i = 0; i < nbf; ++i)
+    if (iech[i] == 0) bxv_sym(ks, ns, c->G.sb.data(), &c->pbs[static_cast<size_t>(ns) * i], &c->qbs[static_cast<size_t>(ns) * i]);
+  for (int i = 0; i < nbf; ++i) {
+    if (iech[i] != 0) continue;
+    for (int j = 0; j <= i; ++j) {
+      if (iech[j] != 0 || lbs[i] != lbs[j]) continue;
+      double s = bvmv_sym(ns, ks, c->G.sb.data(), &c->pbs[static_cast<size_t>(ns) * i], &c->pbs[static_cast<size_t>(ns) * j]);
+      if (std::fabs(s) < eps_ovl) s = 0.0;
+      if (i == j) s = 1.0;
+      c->obs[i + static_cast<size_t>(nbf) * j] = s;
+      c->obs[j + static_cast<size_t>(nbf) * i] = s;
+    }
+  }
+  std::vector<double> P(static_cast<size_t>(ns) * std::max(c->nb, 1), 0.0);
+  for (int b = 0; b < c->nb; ++b)
+    std::memcpy(&P[static_cast<size_t>(ns) * b], &c->pbs[static_cast<size_t>(ns) * (c->bound[b] - 1)], sizeof(double) * ns);
+  HIPCHK(c->P.upload(P.data(), P.size(), c->stream));
+  HIPCHK(c->dens.resize(std::max<size_t>(static_cast<size_t>(c->npair) * ns * ks, 1)));
+  if (c->npair) launch_pair_dens(ns, ks, c->nb, c->P.p, c->dens.p, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->Q.clear();
+  c->Qok.clear();
+  c->wmap.clear();
+  c->wvec.clear();
+  c->wvec_uploaded = -1;
+  c->basis_nk = 0;
+  c->exch_nk = 0;
+  c->orb_ok = true;
+  return 0;
+}
+
+int bsrg_alloc_matrix(bsrg_ctx* c, int nch, int ncp, int mk) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->grid_ok) return fail(-3, "grid not set (bsrg_set_grid)");
+  if (nch < 1 || ncp < 0 || mk < 0) return fail(-1, "bsrg_alloc_matrix: bad sizes");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(c->cur.alloc(c->G.ns, nch, ncp, mk));
+  launch_fill(c->cur.hcc_n(), 0.0, c->cur.hcc.p, c->stream);
+  launch_fill(c->cur.hcb_n(), 0.0, c->cur.hcb.p, c->stream);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int cur_ready(bsrg_ctx* c) {
+  if (c->cur.nch == 0) return fail(-6, "matrix not allocated (bsrg_alloc_matrix)");
+  return 0;
+}
+
+int bsrg_update_hl(bsrg_ctx* c, int ich, int jch, double v) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = cur_ready(c)) return rc;
+  if (ich < 1 || ich > c->cur.nch || jch < 1 || jch > c->cur.nch) return fail(-11, "UPDATE_HL: channel index out of range");
+  HIPCHK(hipSetDevice(c->device));
+  if (c->basis_nk == 0) {
+    // the band table must at least hold sb
+    HIPCHK(c->bands.resize(static_cast<size_t>(c->G.ns) * c->G.ks));
+    launch_sb_upper(c->G.ns, c->G.ks, c->sb.p, c->bands.p, c->stream);
+  }
+  Terms T;
+  band_hl(T, ich, jch, v);
+  return apply_terms(c, c->cur, T, false);
+}
+
+int bsrg_idata(bsrg_ctx* c, int icase, int jtype, int jpol, int n, const int* ipt, const int* k1, const int* k2,
+               const int* k3, const int* k4, const double* cd) {
+  if (int rc = check_ctx(c)) return rc;
+  if (n == 0) return 0;
+  if (icase != 5) return fail(-20, "unknown case in I_data");
+  if (jtype < 1 || jtype > 9) return fail(-21, " I_data: unknown itype ");
+  if (int rc = cur_ready(c)) return rc;
+  if (jpol < 0) return fail(-17, "UPDATE_CF: multipole index out of range");
+  HIPCHK(hipSetDevice(c->device));
+  const int need = std::max(jpol + 1, c->basis_nk);
+  if (int rc = ensure_bases(c, need, jtype >= 8)) return rc;
+  Terms T;
+  if (int rc = add_idata(c, c->cur, T, jtype, jpol, n, ipt, k1, k2, k3, k4, cd, 1.0)) return rc;
+  return apply_terms(c, c->cur, T, false);
+}
+
+int bsrg_odata(bsrg_ctx* c, int n, const int* ipt, const int* k3, const int* k4, const double* cd, double eps_c) {
+  if (int rc = check_ctx(c)) return rc;
+  if (n == 0) return 0;
+  if (int rc = cur_ready(c)) return rc;
+  if (!c->orb_ok) return fail(-4, "orbitals not set (bsrg_set_orbitals)");
+  HIPCHK(hipSetDevice(c->device));
+  if (c->basis_nk == 0)
+    if (int rc = ensure_bases(c, 1, false)) return rc;
+  Terms T;
+  if (int rc = add_odata(c, c->cur, T, n, ipt, k3, k4, cd, eps_c, 1.0, true)) return rc;
+  return apply_terms(c, c->cur, T, false);
+}
+
+int bsrg_symmetrize_diag(bsrg_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = cur_ready(c)) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  launch_symmetrize_diag(c->G.ns, c->cur.nch, c->cur.hcc.p, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int bsrg_scale(bsrg_ctx* c, double ec) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = cur_ready(c)) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  launch_scale(c->cur.hcc_n(), ec, c->cur.hcc.p, c->stream);
+  launch_scale(c->cur.hcb_n(), ec, c->cur.hcb.p, c->stream);
+  for (auto& v : c->cur.hbb) v *= ec;
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int bsrg_acf_finish(bsrg_ctx* c, int kcore) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = cur_ready(c)) return rc;
+  acf_finish(c->cur, kcore);
+  return 0;
+}
+
+int bsrg_get_matrix(bsrg_ctx* c, double* hcc, double* hcb, double* hbb, double* acf, double* htarg, double* otarg) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = cur_ready(c)) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  return get_matrices(c, c->cur, hcc, hcb, hbb, acf, htarg, otarg);
+}
+
+int bsrg_wave_assemble(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int* jtype, const int* kpol,
+                       const int* k1, const int* k2, const int* k3, const int* k4, const double* cd, int novl,
+                       const int* ok3, const int* ok4, const double* ocd, double eps_c, double ec, int ntarg,
+                       const double* etarg, const int* iptar) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->orb_ok) return fail(-4, "orbitals not set (bsrg_set_orbitals)");
+  if (nch < 1 || ncp < 0 || mk < 0 || nrec < 0 || novl < 0) return fail(-1, "bsrg_wave_assemble: bad sizes");
+  HIPCHK(hipSetDevice(c->device));
+  c->wave_ok = false;
+  bool any_exch = false;
+  for (int r = 0; r < nrec; ++r) {
+    if (jtype[r] < 1 || jtype[r] > 9) return fail(-21, " I_data: unknown itype ");
+    if (kpol[r] < 0 || kpol[r] > mk) return fail(-17, "UPDATE_CF: multipole index out of range");
+    if (jtype[r] >= 8) any_exch = true;
+  }
+  for (int i = 0; i < nch; ++i)
+    if (iptar[i] < 1 || iptar[i] > ntarg) return fail(-7, "iptar out of range");
+  (void)hipEventRecord(c->ev[0], c->stream);
+  const int nk = mk + 1;
+  if (int rc = ensure_bands(c, nk)) return rc;
+  (void)hipEventRecord(c->ev[1], c->stream);
+  if (int rc = ensure_bases(c, nk, any_exch)) return rc;
+  HIPCHK(c->S.alloc(c->G.ns, nch, ncp, mk));
+  HIPCHK(c->H.alloc(c->G.ns, nch, ncp, mk));
+  // S: sb on the channel diagonals + overlaps
+  Terms TS, TH;
+  for (int ich = 1; ich <= nch; ++ich) band_hl(TS, ich, ich, 1.0);
+  if (int rc = add_odata(c, c->S, TS, novl, nullptr, ok3, ok4, ocd, eps_c, 1.0, true)) return rc;
+  // H: ec * S + R^k lists + (E_t - ec) * sb
+  for (int ich = 1; ich <= nch; ++ich) band_hl(TH, ich, ich, ec);
+  if (int rc = add_odata(c, c->H, TH, novl, nullptr, ok3, ok4, ocd, eps_c, ec, false)) return rc;
+  int r = 0;
+  while (r < nrec) {
+    int e = r + 1;
+    while (e < nrec && jtype[e] == jtype[r] && kpol[e] == kpol[r]) ++e;
+    if (int rc = add_idata(c, c->H, TH, jtype[r], kpol[r], e - r, nullptr, k1 + r, k2 + r, k3 + r, k4 + r, cd + r, 1.0))
+      return rc;
+    r = e;
+  }
+  for (int ich = 1; ich <= nch; ++ich) band_hl(TH, ich, ich, etarg[iptar[ich - 1] - 1] - ec);
+  if (int rc = apply_terms(c, c->S, TS, true)) return rc;
+  launch_symmetrize_diag(c->G.ns, nch, c->S.hcc.p, c->stream);
+  hipEvent_t e_h0 = c->ev[2];
+  (void)e_h0;
+  double t_gemm = 0, t_terms = 0;
+  if (int rc = apply_terms(c, c->H, TH, true)) return rc;
+  launch_symmetrize_diag(c->G.ns, nch, c->H.hcc.p, c->stream);
+  HIPCHK(hipGetLastError());
+  (void)hipEventRecord(c->ev[5], c->stream);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]); c->timings[0] = ms;
+  (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]); c->timings[1] = ms;
+  (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]); t_gemm = ms; c->timings[2] = t_gemm;
+  (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); t_terms = ms; c->timings[3] = t_terms;
+  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[5]); c->timings[4] = ms;
+  acf_finish(c->H, 0);
+  c->S.htarg = c->H.htarg;  // htarg/otarg/acf belong to the one bsr_matrix module
+  c->H.otarg = c->S.otarg;
+  c->wave_ok = true;
+  c->wave_ntarg = ntarg;
+  return 0;
+}
+
+int bsrg_wave_get(bsrg_ctx* c, int which, double* hcc, double* hcb, double* hbb, double* acf, double* htarg,
+                  double* otarg) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->wave_ok) return fail(-8, "no assembled wave (bsrg_wave_assemble)");
+  HIPCHK(hipSetDevice(c->device));
+  return get_matrices(c, which == 0 ? c->S : c->H, hcc, hcb, hbb, acf, htarg, otarg);
+}
+
+}  // extern "C"
+
+namespace {
+
+// One Fortran unformatted sequential record: 4-byte length, data, length.
+bool put_record(std::FILE* f, const void* data, size_t bytes) {
+  if (bytes > 0x7fffffffu) return false;
+  const int32_t n = static_cast<int32_t>(bytes);
+  return std::fwrite(&n, 4, 1, f) == 1 && (bytes == 0 || std::fwrite(data, 1, bytes, f) == bytes) &&
+         std::fwrite(&n, 4, 1, f) == 1;
+}
+
+// Record_matrix, src/bsr/BSR_MAT3/record_matrix.f90:2-58: diagonal blocks,
+// then non-zero lower blocks with (i,j) headers, channel-perturber vectors,
+// perturber scalars, and the (0,idiag) terminator.
+bool record_matrix(std::FILE* f, const Matrices& M, const std::vector<double>& hcc, const std::vector<double>& hcb) {
+  const int ns = M.ns, kch = M.nch, kcp = M.ncp;
+  const size_t nss = static_cast<size_t>(ns) * ns;
+  bool ok = true;
+  for (int ib = 1; ib <= kch && ok; ++ib) ok = put_record(f, &hcc[(static_cast<size_t>(ib) * (ib + 1) / 2 - 1) * nss], nss * 8);
+  int idiag = 1;
+  for (int ich = 1; ich <= kch && ok; ++ich)
+    for (int jch = 1; jch < ich && ok; ++jch) {
+      const double* b = &hcc[(static_cast<size_t>(ich) * (ich - 1) / 2 + jch - 1) * nss];
+      double s = 0.0;
+      for (size_t q = 0; q < nss; ++q) s += std::fabs(b[q]);
+      if (s == 0.0) continue;
+      const int32_t hdr[2] = {ich, jch};
+      ok = put_record(f, hdr, 8) && put_record(f, b, nss * 8);
+      idiag = 0;
+    }
+  if (kcp > 0) {
+    for (int ich = 1; ich <= kch && ok; ++ich)
+      for (int ip = 1; ip <= kcp && ok; ++ip) {
+        const double* v = &hcb[(static_cast<size_t>(ip - 1) * kch + (ich - 1)) * ns];
+        double s = 0.0;
+        for (int q = 0; q < ns; ++q) s += std::fabs(v[q]);
+        if (s == 0.0) continue;
+        const int32_t hdr[2] = {ip + kch, ich};
+        ok = put_record(f, hdr, 8) && put_record(f, v, static_cast<size_t>(ns) * 8);
+        idiag = 0;
+      }
+    for (int ip = 1; ip <= kcp && ok; ++ip)
+      for (int jp = 1; jp <= ip && ok; ++jp) {
+        const double v = M.hbb[static_cast<size_t>(ip) * (ip - 1) / 2 + jp - 1];
+        if (v == 0.0) continue;
+        const int32_t hdr[2] = {ip + kch, jp + kch};
+        ok = put_record(f, hdr, 8) && put_record(f, &v, 8);
+        if (ip != jp) idiag = 0;
+      }
+  }
+  const int32_t tail[2] = {0, idiag};
+  return ok && put_record(f, tail, 8);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsrg_wave_write(bsrg_ctx* c, const char* path, double tmax, int ntarg, const double* etarg, double ec) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->wave_ok) return fail(-8, "no assembled wave (bsrg_wave_assemble)");
+  HIPCHK(hipSetDevice(c->device));
+  std::vector<double> shcc(c->S.hcc_n()), shcb(c->S.hcb_n()), hhcc(c->H.hcc_n()), hhcb(c->H.hcb_n());
+  if (int rc = get_matrices(c, c->S, shcc.data(), shcb.data(), nullptr, nullptr, nullptr, nullptr)) return rc;
+  if (int rc = get_matrices(c, c->H, hhcc.data(), hhcb.data(), nullptr, nullptr, nullptr, nullptr)) return rc;
+  std::FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(-40, std::string("cannot open ") + path);
+  const int ns = c->G.ns;
+  const int32_t head[3] = {ns, c->H.nch, c->H.ncp};
+  bool ok = put_record(f, head, 12);
+  ok = ok && record_matrix(f, c->S, shcc, shcb);
+  ok = ok && record_matrix(f, c->H, hhcc, hhcb);
+  const int32_t mk = c->H.mk;
+  ok = ok && put_record(f, &mk, 4);
+  ok = ok && put_record(f, c->H.acf.data(), c->H.acf.size() * 8);
+  unsigned char tn[12];
+  const int32_t ns32 = ns;
+  std::memcpy(tn, &tmax, 8);
+  std::memcpy(tn + 8, &ns32, 4);
+  ok = ok && put_record(f, tn, 12);
+  ok = ok && put_record(f, c->H.htarg.data(), c->H.htarg.size() * 8);
+  ok = ok && put_record(f, c->H.otarg.data(), c->H.otarg.size() * 8);
+  ok = ok && put_record(f, etarg, static_cast<size_t>(ntarg) * 8);
+  ok = ok && put_record(f, &ec, 8);
+  if (std::fclose(f) != 0) ok = false;
+  if (!ok) return fail(-41, std::string("write failed: ") + path);
+  return 0;
+}
+
+int bsrg_wave_timings(bsrg_ctx* c, double* ms5) {
+  if (int rc = check_ctx(c)) return rc;
+  for (int i = 0; i < 5; ++i) ms5[i] = c->timings[i];
+  return 0;
+}
+
+int bsrg_set_precision(bsrg_ctx* c, int mode, int nslices) {
+  if (int rc = check_ctx(c)) return rc;
+  if (mode != 0) return fail(-9, "split precision is not available in this build");
+  c->precision_mode = mode;
+  c->nslices = nslices;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+"""ctypes binding of libbsr_gpu.so (the C ABI in include/bsr_gpu.h).
+
+Loading fails loudly when the library has not been built; there is no CPU
+fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbsr_gpu.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
+
+
+class BsrGpuError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _opt_dp():
+    """ndpointer that also accepts None (NULL)."""
+    base = _dp
+
+    class _Opt(base):
+        @classmethod
+        def from_param(cls, obj):
+            if obj is None:
+                return None
+            return base.from_param(obj)
+
+    return _Opt
+
+
+_odp = _opt_dp()
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+SIGNATURES = {
+    "bsrg_last_error": [],
+    "bsrg_abi_version": [],
+    "bsrg_mkgrid": [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
+                    C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)],
+    "bsrg_grid_tables": [C.c_int, C.c_int, _dp, _odp, _odp, _odp, _odp],
+    "bsrg_create": [C.POINTER(C.c_void_p), C.c_int],
+    "bsrg_destroy": [C.c_void_p],
+    "bsrg_set_grid": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp],
+    "bsrg_mrk_cell": [C.c_void_p, C.c_int, _odp],
+    "bsrg_rk_moments": [C.c_void_p, C.c_int, _odp, _odp, _odp],
+    "bsrg_convol": [C.c_void_p, C.c_int, C.c_int, _dp, _dp],
+    "bsrg_set_orbitals": [C.c_void_p, C.c_int, _dp, _ip, _ip, _ip, C.c_double],
+    "bsrg_alloc_matrix": [C.c_void_p, C.c_int, C.c_int, C.c_int],
+    "bsrg_update_hl": [C.c_void_p, C.c_int, C.c_int, C.c_double],
+    "bsrg_idata": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp],
+    "bsrg_odata": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, C.c_double],
+    "bsrg_symmetrize_diag": [C.c_void_p],
+    "bsrg_scale": [C.c_void_p, C.c_double],
+    "bsrg_acf_finish": [C.c_void_p, C.c_int],
+    "bsrg_get_matrix": [C.c_void_p, _odp, _odp, _odp, _odp, _odp, _odp],
+    "bsrg_wave_assemble": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
+                           C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, _dp, _ip],
+    "bsrg_wave_get": [C.c_void_p, C.c_int, _odp, _odp, _odp, _odp, _odp, _odp],
+    "bsrg_wave_write": [C.c_void_p, C.c_char_p, C.c_double, C.c_int, _dp, C.c_double],
+    "bsrg_wave_timings": [C.c_void_p, _dp],
+    "bsrg_set_precision": [C.c_void_p, C.c_int, C.c_int],
+}
+_RESTYPE = {"bsrg_last_error": C.c_char_p}
+
+_LIB = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m bsr_amd.build` "
+                "(there is no fallback path)")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, C.c_int)
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != 0:
+        raise BsrGpuError(rc, lib().bsrg_last_error().decode())
+    return rc
+
+
+def f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def i32(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.int32))

@@ -1,0 +1,159 @@
+/*
+ * bsr_gpu.h -- C ABI of the MI355X radial core of bsr_mat.
+ *
+ * Plain pointers and sizes only.  Arrays use the reference's Fortran
+ * column-major layouts so a Fortran host can pass its module arrays
+ * directly (ISO_C_BINDING, see bsr_amd/fortran/bsr_gpu.f90):
+ *   t(ns+ks), pbs(ns,nbf), rkb(ns,ns,ks,ks), d(ns,ks)|d(ns,ns),
+ *   hcc(ns,ns,nch*(nch+1)/2), hcb(ns,nch,ncp), hbb(ncp*(ncp+1)/2),
+ *   acf(nch,nch,0:mk), htarg/otarg(nch*(nch+1)/2).
+ * Orbital, channel and perturber indices are 1-based as in the reference;
+ * K1..K4 use the reference packing (ibi = ibo = 2**15).
+ *
+ * Every call returns 0 on success or a negative status; bsrg_last_error()
+ * gives the message (the reference's Stop text where one exists).
+ * A context is used from one host thread at a time.  All arrays passed in
+ * are host memory unless the name says _dev.
+ *
+ * Reference paths below are relative to the jagot/bsr checkout.
+ */
+#ifndef BSR_GPU_H
+#define BSR_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bsrg_ctx bsrg_ctx;
+
+/* Message for the last failing call on this thread. */
+const char* bsrg_last_error(void);
+int bsrg_abi_version(void);
+
+/* ---- host-only helpers (no device needed) ---------------------------- */
+
+/* Knots of the semi-exponential grid.
+ * Replaces mkgrid1, src/libs/BS/define_grid.f90:213-276 (kind=1) and
+ * mkgrid, :86-153 (kind=0).  t must hold cap doubles. */
+int bsrg_mkgrid(int kind, int ks, double z, double h, double hmax, double rmax, int cap,
+                double* t, int* ns, int* ml, int* me, double* h_out);
+
+/* Gauss points/weights per interval, B-spline values there and the Gram
+ * band.  Replaces initvb + mrm(0,sb), src/libs/BS/define_spline.f90:67-129,
+ * :162.  gr,grw: (ks,nv) [m fastest]; bsp: (ks,ks,nv) [ith fastest, then m];
+ * sb: (ns,ks) lower band. */
+int bsrg_grid_tables(int ks, int ns, const double* t, double* gr, double* grw, double* bsp,
+                     double* sb);
+
+/* ---- context ---------------------------------------------------------- */
+
+int bsrg_create(bsrg_ctx** ctx, int device);
+int bsrg_destroy(bsrg_ctx* ctx);
+
+/* Knot grid and spline tables for the following calls; keeps them resident.
+ * Replaces define_grid + define_spline module state (spline_param,
+ * spline_grid, spline_galerkin), src/libs/BS/define_spline.f90:15-63.
+ * ml, me, h describe the geometric region used by the scaling law. */
+int bsrg_set_grid(bsrg_ctx* ctx, int ks, int ns, int ml, int me, double h, const double* t);
+
+/* ---- R^k band (a5-a7) --------------------------------------------------- */
+
+/* Build (or reuse) the resident R^k band for multipole k and optionally
+ * copy it to rkb(ns,ns,ks,ks).  Replaces MRK_cell(k),
+ * src/libs/BS/mrk_cell.f90:2-82 (with rk_moments, rk_moments.f90:2-101). */
+int bsrg_mrk_cell(bsrg_ctx* ctx, int k, double* rkb);
+
+/* Moments and diagonal-cell integrals of multipole k, packed as the
+ * reference's first ik=ks(ks+1)/2 rows: rkd1,rkd2 (ik,nv), rkd (ik,ik,nv).
+ * Replaces rk_moments(k), src/libs/BS/rk_moments.f90:2-50. */
+int bsrg_rk_moments(bsrg_ctx* ctx, int k, double* rkd1, double* rkd2, double* rkd);
+
+/* Convolution of the R^k band with a density, symmetric-symmetric cases.
+ * icase 1,2: d(ns,ks) -> a(ns,ks); icase 3,4: d(ns,ns) -> a(ns,ns).
+ * Replaces convol(ns,ks,a,d,icase,'s','s'), src/libs/BS/convol.f90:2-330. */
+int bsrg_convol(bsrg_ctx* ctx, int k, int icase, const double* d, double* a);
+
+/* ---- one partial wave (a11-a15) ---------------------------------------- */
+
+/* Orbitals of the wave: pbs(ns,nbf); iech(i)=channel of orbital i (0 for
+ * bound orbitals); lbs(i)=l; ibort(nbf,nbf) nonzero where the overlap
+ * <kl|nl> of channel orbital kl and bound orbital nl is kept.
+ * Sets up qbs=sb*pbs and the bound overlaps OBS as Read_data does,
+ * src/bsr/BSR_MAT3/read_data.f90:142-208, and keeps pbs resident. */
+int bsrg_set_orbitals(bsrg_ctx* ctx, int nbf, const double* pbs, const int* iech,
+                      const int* lbs, const int* ibort, double eps_ovl);
+
+/* Allocate and clear hcc,hcb,hbb,acf,htarg,otarg on the device.
+ * Replaces allocate_matrix, src/bsr/BSR_MAT3/mod_intmat.f90:33-59. */
+int bsrg_alloc_matrix(bsrg_ctx* ctx, int nch, int ncp, int mk);
+
+/* hcc(:,:,ij(ich,jch)) += c * sb (symmetric band).
+ * Replaces UPDATE_HL(ich,jch,ns,ks,sb,c), mod_intmat.f90:132-160. */
+int bsrg_update_hl(bsrg_ctx* ctx, int ich, int jch, double c);
+
+/* Process one sorted coefficient list of structure jtype (1..9) and
+ * multipole jpol for R^k (icase 5), accumulating into the device matrices.
+ * Records are visited in the order ipt (1-based); K1..K4 and cdata as in
+ * module cmdata.  Replaces I_data(jtype,jpol),
+ * src/bsr/BSR_MAT3/I_data.f90:2-224 (coupling LS, iitar=0). */
+int bsrg_idata(bsrg_ctx* ctx, int icase, int jtype, int jpol, int ncdata, const int* ipt,
+               const int* k1, const int* k2, const int* k3, const int* k4, const double* cdata);
+
+/* Overlap records (structure 1 only).  Replaces O_data,
+ * src/bsr/BSR_MAT3/O_data.f90:2-76; records with |c| < eps_c skipped. */
+int bsrg_odata(bsrg_ctx* ctx, int ncdata, const int* ipt, const int* k3, const int* k4,
+               const double* cdata, double eps_c);
+
+/* Diagonal channel blocks -> (C+C^T)/2.  Replaces sub1.f90:80-84, :187-191. */
+int bsrg_symmetrize_diag(bsrg_ctx* ctx);
+
+/* hcc,hcb,hbb *= ec.  Replaces the core-energy shift, sub1.f90:121-122. */
+int bsrg_scale(bsrg_ctx* ctx, double ec);
+
+/* ACF symmetrisation, eps_acf=1e-5 cut and k=0 core term.
+ * Replaces sub1.f90:211-219. */
+int bsrg_acf_finish(bsrg_ctx* ctx, int kcore);
+
+/* Copy the matrices to host arrays (any pointer may be NULL). */
+int bsrg_get_matrix(bsrg_ctx* ctx, double* hcc, double* hcb, double* hbb, double* acf,
+                    double* htarg, double* otarg);
+
+/* ---- fused partial wave --------------------------------------------------
+ * The whole of SUB1's S and H assembly for R^k (sub1.f90:70-219) in one
+ * pass: S = sb on channel diagonals + O_data, symmetrised; H = ec*S + all
+ * R^k lists (jpol outer, jtype inner, state_res.f90:155-159) +
+ * (E_t - ec)*sb on diagonals, symmetrised; ACF finished (kcore=0).
+ * Lists are concatenated: record r has structure jtype[r] and multipole
+ * kpol[r]; records of one (jtype,kpol) are contiguous and in cmdata order.
+ * Orbitals and grid must be set; matrices are (re)allocated here. */
+int bsrg_wave_assemble(bsrg_ctx* ctx, int nch, int ncp, int mk, int nrec, const int* jtype,
+                       const int* kpol, const int* k1, const int* k2, const int* k3,
+                       const int* k4, const double* cdata, int novl, const int* ok3,
+                       const int* ok4, const double* ocdata, double eps_c, double ec,
+                       int ntarg, const double* etarg, const int* iptar);
+
+/* After bsrg_wave_assemble: copy S (which=0) or H (which=1) to host. */
+int bsrg_wave_get(bsrg_ctx* ctx, int which, double* hcc, double* hcb, double* hbb,
+                  double* acf, double* htarg, double* otarg);
+
+/* After bsrg_wave_assemble: write bsr_mat.nnn in the reference format
+ * (Fortran unformatted sequential, 4-byte markers, little endian).
+ * Replaces Record_matrix + the trailing writes of SUB1,
+ * src/bsr/BSR_MAT3/record_matrix.f90:2-58, sub1.f90:107-110,202,258,266-270. */
+int bsrg_wave_write(bsrg_ctx* ctx, const char* path, double tmax, int ntarg,
+                    const double* etarg, double ec);
+
+/* Device time (ms) of the phases of the last bsrg_wave_assemble, measured
+ * with HIP events on the context's stream: [0] bands, [1] orbital bases,
+ * [2] exchange GEMM, [3] band/rank-1/vector terms, [4] total. */
+int bsrg_wave_timings(bsrg_ctx* ctx, double* ms5);
+
+/* Precision of the exchange contraction for later bsrg_wave_assemble
+ * calls: 0 = plain FP64 (default), 1 = split precision (Ozaki slices on
+ * bf16 MFMA, FP32 accumulation, FP64 recombination) with nslices slices. */
+int bsrg_set_precision(bsrg_ctx* ctx, int mode, int nslices);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,210 @@
+"""GPU parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Tolerance: 1e-12 per element relative to the oracle, with a floor of 1e-3
+of the block maximum for elements that cancel (coefficients are signed).
+The R^k band itself is non-negative and is compared element by element.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _grids():
+    from bsr_amd import grid as G
+    out = []
+    t, ns, ml, me, h = G.mkgrid(0, 8, 2.0, 0.125, 160.0, 160.0)      # the reference's TEST_BS knot.dat
+    out.append(("test_bs", 8, ns, ml, me, h, t))
+    g = G.fixed_ns_grid(8, 60, 6.0, 20.0, 0.05)                        # 60-spline grid
+    out.append(("ns60", 8, g.ns, g.ml, g.me, g.h, g.t))
+    t, ns, ml, me, h = G.mkgrid(1, 9, 3.0, 0.25, 1.5, 15.0)            # ks=9, geometric region
+    out.append(("ks9", 9, ns, ml, me, h, t))
+    return out
+
+
+@pytest.mark.parametrize("case", _grids(), ids=lambda c: c[0])
+def test_band_and_moments(ctx, oracle_lib, case):
+    O = oracle_lib
+    name, ks, ns, ml, me, h, t = case
+    print(f"[band] grid={name} ks={ks} ns={ns} ml={ml} me={me}", flush=True)
+    ctx.set_grid(ks, ns, ml, me, h, t)
+    og = O.Grid(ks, ns, ml, me, h, t)
+    for k in range(0, 5):
+        r1, r2, rd = ctx.rk_moments(k)
+        o1, o2, od = og.rk_moments(k)
+        assert rel_err(r1, o1, 1e-14) < TOL
+        assert rel_err(r2, o2, 1e-14) < TOL
+        assert rel_err(rd, od, 1e-14) < TOL
+        rkb = ctx.mrk_cell(k)
+        orb = og.mrk_cell(k)
+        assert np.all(rkb >= 0.0)
+        assert rel_err(rkb, orb, 1e-14) < TOL, f"k={k}"
+
+
+def test_convol_all_cases(ctx, oracle_lib):
+    O = oracle_lib
+    from bsr_amd import grid as G
+    g = G.fixed_ns_grid(8, 60, 6.0, 20.0, 0.05)
+    ctx.set_grid(g.ks, g.ns, g.ml, g.me, g.h, g.t)
+    og = O.Grid(g.ks, g.ns, g.ml, g.me, g.h, g.t)
+    rng = np.random.default_rng(7)
+    for k in (0, 3):
+        rkb = og.mrk_cell(k)
+        p1, p2 = rng.standard_normal(g.ns), rng.standard_normal(g.ns)
+        for icase in (1, 2, 3, 4):
+            typ = "s" if icase <= 2 else "x"
+            d = O.density(g.ns, g.ks, p1, p2, typ)
+            ref = O.convol(g.ns, g.ks, d, icase, rkb)
+            got = ctx.convol(k, d, icase)
+            assert rel_err(got, ref) < TOL, (k, icase)
+
+
+def _wave(ctx, w):
+    ctx.set_grid(w.ks, w.ns, w.ml, w.me, w.h, w.t)
+    ctx.set_orbitals(w.pbs, w.iech, w.lbs, w.ibort, w.eps_ovl)
+
+
+def _cmp_mats(got, ref, what, tol=TOL):
+    for name in ("hcc", "hcb", "hbb", "acf", "htarg", "otarg"):
+        a, b = getattr(got, name), getattr(ref, name)
+        if b.size and np.abs(b).max() > 0:
+            e = rel_err(a, b)
+            assert e < tol, f"{what}.{name}: {e:.3e}"
+        else:
+            assert np.abs(a).max() == 0.0, f"{what}.{name} should be zero"
+
+
+@pytest.mark.parametrize("jtype", range(1, 10))
+def test_idata_each_structure(ctx, oracle_lib, jtype):
+    """Per-call I_data for one list of each structure vs the oracle."""
+    O = oracle_lib
+    from bsr_amd import synth
+    w = synth.make_wave(11, 8, 40, 4.0, 12.0, nch=6, ncp=7, mk=3, nb=4, ntarg=3, fill=0.3, name="small")
+    _wave(ctx, w)
+    ctx.allocate_matrix(w.nch, w.ncp, w.mk)
+    ow = O.Wave(O.Grid(w.ks, w.ns, w.ml, w.me, w.h, w.t), w)
+    for k in range(w.mk + 1):
+        lst = w.rk_lists.get((jtype, k))
+        if lst is None or len(lst) == 0:
+            continue
+        print(f"[idata] jtype={jtype} k={k} n={len(lst)}", flush=True)
+        ctx.i_data(jtype, k, lst)
+        ow.idata(jtype, k, lst)
+    _cmp_mats(ctx.get_matrix(), ow.get(), f"jtype{jtype}")
+
+
+def test_call_by_call_wave(ctx, oracle_lib):
+    """The SUB1 sequence through the per-call entry points vs the oracle."""
+    O = oracle_lib
+    from bsr_amd import synth
+    w = synth.make_wave(12, 8, 40, 4.0, 12.0, nch=5, ncp=6, mk=2, nb=3, ntarg=2, fill=0.3, name="small")
+    _wave(ctx, w)
+    ctx.allocate_matrix(w.nch, w.ncp, w.mk)
+    for ich in range(1, w.nch + 1):
+        ctx.update_hl(ich, ich, 1.0)
+    ctx.o_data(w.olist, w.eps_c)
+    ctx.symmetrize_diag()
+    S = ctx.get_matrix()
+    ctx.scale(w.ec)
+    for k in range(w.mk + 1):
+        for jt in range(1, 10):
+            lst = w.rk_lists.get((jt, k))
+            if lst is not None and len(lst):
+                ctx.i_data(jt, k, lst)
+    for ich in range(1, w.nch + 1):
+        ctx.update_hl(ich, ich, w.etarg[w.iptar[ich - 1] - 1] - w.ec)
+    ctx.symmetrize_diag()
+    ctx.acf_finish(0)
+    H = ctx.get_matrix()
+    oS, oH = O.run_wave(w)
+    _cmp_mats(S, oS, "S")
+    _cmp_mats(H, oH, "H")
+
+
+def test_fused_wave_c1(ctx, oracle_lib, tmp_path):
+    """Config 1 (ks=8, ns=60, 20 channels, k<=4) fused on the GPU vs the
+    oracle: every element of S and H, ACF, target matrices, the bsr_mat
+    file, and the lowest 10 generalized eigenvalues (<= 1e-9)."""
+    O = oracle_lib
+    from bsr_amd import synth
+    import scipy.linalg as sl
+    w = synth.c1()
+    print(f"[c1] nhm={w.nhm} records={w.n_records()}", flush=True)
+    ctx.wave_assemble(w)
+    S, H = ctx.wave_get(0), ctx.wave_get(1)
+    oS, oH = O.run_wave(w)
+    for name in ("hcc",):
+        assert rel_err(getattr(S, name), getattr(oS, name)) < TOL
+        assert rel_err(getattr(H, name), getattr(oH, name)) < TOL
+    assert rel_err(H.acf, oH.acf) < TOL
+    assert rel_err(H.htarg, oH.htarg) < TOL
+    assert rel_err(S.otarg, oS.otarg) < TOL
+    # the file
+    p1, p2 = tmp_path / "gpu.mat", tmp_path / "oracle.mat"
+    ctx.wave_write(p1, w)
+    O.write_bsr_mat(p2, w, oS, oH)
+    g, r = O.read_bsr_mat(p1), O.read_bsr_mat(p2)
+    assert (g["ns"], g["nch"], g["ncp"], g["mk"]) == (r["ns"], r["nch"], r["ncp"], r["mk"])
+    for part in ("S", "H"):
+        assert g[part]["idiag"] == r[part]["idiag"]
+        assert sorted(g[part]["blocks"]) == sorted(r[part]["blocks"])
+        for key in r[part]["blocks"]:
+            assert rel_err(g[part]["blocks"][key], r[part]["blocks"][key]) < TOL
+        for i in range(w.nch):
+            assert rel_err(g[part]["diag"][i], r[part]["diag"][i]) < TOL
+    assert rel_err(g["acf"], r["acf"]) < TOL
+    # eigenvalues of H c = E S c, lowest 10
+    e_g = sl.eigh(H.dense(), S.dense(), eigvals_only=True, subset_by_index=[0, 9])
+    e_o = sl.eigh(oH.dense(), oS.dense(), eigvals_only=True, subset_by_index=[0, 9])
+    print("[c1] eig", e_o[:3], np.abs(e_g - e_o).max(), flush=True)
+    assert np.abs(e_g - e_o).max() <= 1e-9
+
+
+def test_fused_wave_with_perturbers(ctx, oracle_lib):
+    """Production-shaped wave (ks=8, ns=200, k<=10, perturbers) with few
+    channels so the oracle finishes in seconds."""
+    O = oracle_lib
+    from bsr_amd import synth
+    w = synth.make_wave(21, 8, 200, 18.0, 40.0, nch=4, ncp=12, mk=10, nb=5, ntarg=5, fill=0.05, name="c2s")
+    print(f"[c2-small] nhm={w.nhm} records={w.n_records()}", flush=True)
+    ctx.wave_assemble(w)
+    S, H = ctx.wave_get(0), ctx.wave_get(1)
+    oS, oH = O.run_wave(w)
+    _cmp_mats(S, oS, "S")
+    _cmp_mats(H, oH, "H")
+
+
+def test_fused_equals_call_by_call_and_is_deterministic(ctx):
+    from bsr_amd import synth
+    w = synth.make_wave(31, 8, 80, 10.0, 25.0, nch=12, ncp=30, mk=6, nb=5, ntarg=4, fill=0.1, name="mid")
+    ctx.wave_assemble(w)
+    H1 = ctx.wave_get(1)
+    ctx.wave_assemble(w, set_inputs=False)
+    H2 = ctx.wave_get(1)
+    assert np.array_equal(H1.hcc, H2.hcc) and np.array_equal(H1.hcb, H2.hcb)
+    # diagonal blocks are symmetric
+    for i in range(1, w.nch + 1):
+        b = H1.hcc[:, :, i * (i + 1) // 2 - 1]
+        assert np.array_equal(b, b.T)
+
+
+def test_errors_are_reported(ctx):
+    from bsr_amd import synth, BsrGpuError
+    w = synth.make_wave(41, 8, 30, 4.0, 10.0, nch=3, ncp=2, mk=1, nb=2, ntarg=2, fill=0.5, name="err")
+    _wave(ctx, w)
+    ctx.allocate_matrix(w.nch, w.ncp, w.mk)
+    lst = w.rk_lists[(8, 0)]
+    bad = synth.RecordList(lst.k1.copy(), lst.k2, lst.k3, lst.k4, lst.c)
+    bad.k1[0] = 9 * synth.IBI + 1
+    with pytest.raises(BsrGpuError, match="UPDATE_HX: channel index out of range"):
+        ctx.i_data(8, 0, bad)
+    with pytest.raises(BsrGpuError, match="unknown case in I_data"):
+        ctx.i_data(8, 0, lst, icase=3)
+    with pytest.raises(BsrGpuError, match="unknown itype"):
+        ctx.i_data(10, 0, lst)
+    with pytest.raises(BsrGpuError, match="unknown case"):
+        ctx.convol(0, np.zeros((w.ns, w.ns)), 5)
