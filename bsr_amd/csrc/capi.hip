@@ -903,7 +903,8 @@ int bsrg_wave_assemble(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const in
   (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); t_terms = ms; c->timings[3] = t_terms;
   (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[5]); c->timings[4] = ms;
   acf_finish(c->H, 0);
-  c->S.htarg = c->H.htarg;  // htarg/otarg/acf belong to the one bsr_matrix module
+  // the reference keeps one bsr_matrix module: H is recorded after the
+  // overlap pass, so it carries the otarg accumulated there
   c->H.otarg = c->S.otarg;
   c->wave_ok = true;
   c->wave_ntarg = ntarg;

@@ -1020,3 +1020,16 @@ void orc_wave_acf_finish(orc_wave* w, int kcore) {
       }
   for (int i = 0; i < nch; i++) w->acf[i + (size_t)nch * i] += kcore;
 }
+
+/* Exported accumulators, for tests against the reference's mod_intmat.f90 */
+int orc_wave_update_hx(orc_wave* w, int ich, int jch, const double* d, char sym) {
+  return update_hx(w, ich, jch, d, sym);
+}
+int orc_wave_update_hw(orc_wave* w, int ich, int jch, const double* v, const double* u) {
+  return update_hw(w, ich, jch, v, u);
+}
+int orc_wave_update_hv(orc_wave* w, int ich, int ic, const double* v, double c) {
+  return update_hv(w, ich, ic, v, c);
+}
+int orc_wave_update_hb(orc_wave* w, int ic, int jc, double c) { return update_hb(w, ic, jc, c); }
+int orc_wave_update_cf(orc_wave* w, int k, int ich, int jch, double c) { return update_cf(w, k, ich, jch, c); }

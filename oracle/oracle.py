@@ -72,6 +72,11 @@ def lib():
             "orc_wave_symmetrize_diag": (None, [vp]),
             "orc_wave_scale": (None, [vp, d]),
             "orc_wave_acf_finish": (None, [vp, i]),
+            "orc_wave_update_hx": (i, [vp, i, i, _dp, C.c_char]),
+            "orc_wave_update_hw": (i, [vp, i, i, _dp, _dp]),
+            "orc_wave_update_hv": (i, [vp, i, i, _dp, d]),
+            "orc_wave_update_hb": (i, [vp, i, i, d]),
+            "orc_wave_update_cf": (i, [vp, i, i, i, d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -259,6 +264,23 @@ class Wave:
 
     def update_hl(self, ich, jch, c):
         _check(lib().orc_wave_update_hl(self.ptr, ich, jch, c))
+
+    def update_hx(self, ich, jch, d, sym):
+        dd = np.zeros((self.job.ns, self.job.ns), order="F")
+        dd[:, : d.shape[1]] = d
+        _check(lib().orc_wave_update_hx(self.ptr, ich, jch, dd, sym.encode()))
+
+    def update_hw(self, ich, jch, v, u):
+        _check(lib().orc_wave_update_hw(self.ptr, ich, jch, _f(v), _f(u)))
+
+    def update_hv(self, ich, ic, v, c):
+        _check(lib().orc_wave_update_hv(self.ptr, ich, ic, _f(v), c))
+
+    def update_hb(self, ic, jc, c):
+        _check(lib().orc_wave_update_hb(self.ptr, ic, jc, c))
+
+    def update_cf(self, k, ich, jch, c):
+        _check(lib().orc_wave_update_cf(self.ptr, k, ich, jch, c))
 
     def idata(self, jtype, k, lst, icase=5):
         n = len(lst.c)
