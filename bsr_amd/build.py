@@ -38,7 +38,28 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "bsr_mat_gpu")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The Fortran host: module bsr_gpu (ISO_C_BINDING) and the driver
+    program bsr_mat_gpu, linked against libbsr_gpu.so."""
+    fc = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+    mod = os.path.join(FORTRAN, "_mod")
+    os.makedirs(mod, exist_ok=True)
+    cmd = [fc, "-O2", "-module-dir", mod, os.path.join(FORTRAN, "bsr_gpu.f90"),
+           os.path.join(FORTRAN, "bsr_mat_gpu.f90"), "-o", DRIVER + ".tmp", "-L" + HERE, "-lbsr_gpu",
+           "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -46,7 +46,8 @@ struct DevGrid {
 // s=0: r^k, s=1: r^-(k+1); diagonal cells rkd[((kk*nv+iv)*ik+jj)*ik+ii].
 void launch_moments(const DevGrid& g, int k0, int nk, double* mom, double* rkd, hipStream_t s);
 // R^k bands for the same multipoles: band kk at rkb + kk*ns*ns*ks*ks.
-void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd, double* rkb,
+// msum: scratch of nk*2*ks*ns doubles (per-pair sums of the moments)
+void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd, double* msum, double* rkb,
                  hipStream_t s);
 // upper-band densities dens[p](j,jp) of all unordered bound pairs p.
 void launch_pair_dens(int ns, int ks, int nb, const double* P, double* dens, hipStream_t s);
@@ -83,5 +84,10 @@ void launch_convol_direct(int ns, int ks, int jcase, const double* rkb, const do
                           hipStream_t s);
 void launch_convol_exchange(int ns, int ks, const double* rkb, const double* d, double* a,
                             hipStream_t s);
+
+// split-precision FP64 GEMM (ozaki.hip): C = A*B (+C), column-major
+size_t ozaki_workspace_bytes(int m, int n, int K, int S);
+int ozaki_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B, size_t ldb, double* C,
+                size_t ldc, bool accumulate, int S, void* work, hipStream_t s);
 
 }  // namespace bsrg

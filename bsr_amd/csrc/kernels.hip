@@ -152,10 +152,30 @@ __global__ void k_diag_scale(DevGrid g, double* rkd) {
 }
 
 // ---- a7: band assembly (mrk_cell) ----------------------------------------
-// one thread per element, i fastest (coalesced writes); contributions are
-// summed in the reference's order: cell jv ascending, then cell iv.
+// Sum over the cells of one spline pair of each one-cell moment:
+// msum[((kk*2+s)*ks+ip)*ns+i] = sum_{iv in cells(i,i+ip)} mom_s(iv; i,i+ip).
+__global__ void k_pair_sums(int ns, int ks, int nv, const double* __restrict__ mom, double* __restrict__ msum) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ip = blockIdx.y, kk = blockIdx.z >> 1, s = blockIdx.z & 1;
+  if (i >= ns) return;
+  const int ik = ks * (ks + 1) / 2;
+  const double* m = mom + static_cast<size_t>(kk * 2 + s) * nv * ik;
+  double acc = 0.0;
+  if (i + ip < ns) {
+    const int ivlo = max(0, i + ip - (ks - 1)), ivhi = min(i, nv - 1);
+    for (int iv = ivlo; iv <= ivhi; ++iv) acc += m[iv * ik + pack2(i - iv, i + ip - iv, ks)];
+  }
+  msum[(static_cast<size_t>(kk * 2 + s) * ks + ip) * ns + i] = acc;
+}
+
+// One thread per element, i fastest (coalesced writes).  When every cell of
+// the r1 pair lies below every cell of the r2 pair (or above), the cell sum
+// factorises: rkb = (sum of r^k moments) * (sum of r^-(k+1) moments).
+// Otherwise the cells are summed in the reference's order: cell jv
+// ascending, then cell iv.
 __global__ void k_band(int ns, int ks, int nv, int nk, const double* __restrict__ mom,
-                       const double* __restrict__ rkd, double* __restrict__ rkb) {
+                       const double* __restrict__ rkd, const double* __restrict__ msum,
+                       double* __restrict__ rkb) {
   const size_t per = static_cast<size_t>(ns) * ns * ks * ks;
   const size_t tid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (tid >= per * nk) return;
@@ -173,6 +193,16 @@ __global__ void k_band(int ns, int ks, int nv, int nk, const double* __restrict_
     const double* rd = rkd + static_cast<size_t>(kk) * nv * ik * ik;
     const int ivlo = max(0, i + ip - (ks - 1)), ivhi = min(i, nv - 1);
     const int jvlo = max(0, j + jp - (ks - 1)), jvhi = min(j, nv - 1);
+    const double* s1 = msum + static_cast<size_t>(kk * 2) * ks * ns;
+    const double* s2 = s1 + static_cast<size_t>(ks) * ns;
+    if (ivhi < jvlo) {
+      rkb[tid] = s1[ip * ns + i] * s2[jp * ns + j];
+      return;
+    }
+    if (ivlo > jvhi) {
+      rkb[tid] = s1[jp * ns + j] * s2[ip * ns + i];
+      return;
+    }
     for (int jv = jvlo; jv <= jvhi; ++jv) {
       const int jj = pack2(j - jv, j + jp - jv, ks);
       for (int iv = ivlo; iv <= ivhi; ++iv) {
@@ -484,9 +514,12 @@ void launch_moments(const DevGrid& g, int k0, int nk, double* mom, double* rkd, 
   }
 }
 
-void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd, double* rkb, hipStream_t s) {
+void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd, double* msum, double* rkb,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_pair_sums, dim3(nblocks(g.ns, 256), g.ks, nk * 2), dim3(256), 0, s, g.ns, g.ks, g.nv, mom,
+                     msum);
   const size_t n = static_cast<size_t>(g.ns) * g.ns * g.ks * g.ks * nk;
-  hipLaunchKernelGGL(k_band, dim3(nblocks(n, 256)), dim3(256), 0, s, g.ns, g.ks, g.nv, nk, mom, rkd, rkb);
+  hipLaunchKernelGGL(k_band, dim3(nblocks(n, 256)), dim3(256), 0, s, g.ns, g.ks, g.nv, nk, mom, rkd, msum, rkb);
 }
 
 void launch_pair_dens(int ns, int ks, int nb, const double* P, double* dens, hipStream_t s) {

@@ -1,0 +1,214 @@
+!======================================================================
+!  Module bsr_gpu -- ISO_C_BINDING interface to libbsr_gpu.so.
+!
+!  The entry points replace the reference's radial-core routines with the
+!  same argument meaning (1-based indices, Fortran column-major arrays):
+!    gpu_mrk_cell(k, rkb)          MRK_cell(k)        src/libs/BS/mrk_cell.f90:2
+!    gpu_convol(k, a, d, icase)    convol(ns,ks,a,d,icase,'s','s')
+!                                                     src/libs/BS/convol.f90:2
+!    gpu_i_data(jtype, jpol, ...)  I_data(jtype,jpol) src/bsr/BSR_MAT3/I_data.f90:2
+!    gpu_o_data(...)               O_data             src/bsr/BSR_MAT3/O_data.f90:2
+!    gpu_update_hl(ich, jch, c)    UPDATE_HL(...,sb,c) src/bsr/BSR_MAT3/mod_intmat.f90:132
+!  A non-zero status stops the program with the library's message, as the
+!  reference's Stop does.
+!======================================================================
+module bsr_gpu
+  use iso_c_binding
+  implicit none
+  type(c_ptr), save :: gpu_ctx = c_null_ptr
+
+  interface
+    function bsrg_last_error() bind(C, name="bsrg_last_error") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+    integer(c_int) function bsrg_create(ctx, device) bind(C, name="bsrg_create")
+      import :: c_ptr, c_int
+      type(c_ptr) :: ctx
+      integer(c_int), value :: device
+    end function
+    integer(c_int) function bsrg_destroy(ctx) bind(C, name="bsrg_destroy")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function bsrg_set_grid(ctx, ks, ns, ml, me, h, t) bind(C, name="bsrg_set_grid")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ks, ns, ml, me
+      real(c_double), value :: h
+      real(c_double) :: t(*)
+    end function
+    integer(c_int) function bsrg_mrk_cell(ctx, k, rkb) bind(C, name="bsrg_mrk_cell")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k
+      real(c_double) :: rkb(*)
+    end function
+    integer(c_int) function bsrg_convol(ctx, k, icase, d, a) bind(C, name="bsrg_convol")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: k, icase
+      real(c_double) :: d(*), a(*)
+    end function
+    integer(c_int) function bsrg_set_orbitals(ctx, nbf, pbs, iech, lbs, ibort, eps_ovl) &
+        bind(C, name="bsrg_set_orbitals")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbf
+      real(c_double) :: pbs(*)
+      integer(c_int) :: iech(*), lbs(*), ibort(*)
+      real(c_double), value :: eps_ovl
+    end function
+    integer(c_int) function bsrg_alloc_matrix(ctx, nch, ncp, mk) bind(C, name="bsrg_alloc_matrix")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nch, ncp, mk
+    end function
+    integer(c_int) function bsrg_update_hl(ctx, ich, jch, c) bind(C, name="bsrg_update_hl")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ich, jch
+      real(c_double), value :: c
+    end function
+    integer(c_int) function bsrg_idata(ctx, icase, jtype, jpol, n, ipt, k1, k2, k3, k4, c) &
+        bind(C, name="bsrg_idata")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: icase, jtype, jpol, n
+      integer(c_int) :: ipt(*), k1(*), k2(*), k3(*), k4(*)
+      real(c_double) :: c(*)
+    end function
+    integer(c_int) function bsrg_odata(ctx, n, ipt, k3, k4, c, eps_c) bind(C, name="bsrg_odata")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+      integer(c_int) :: ipt(*), k3(*), k4(*)
+      real(c_double) :: c(*)
+      real(c_double), value :: eps_c
+    end function
+    integer(c_int) function bsrg_symmetrize_diag(ctx) bind(C, name="bsrg_symmetrize_diag")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function bsrg_scale(ctx, ec) bind(C, name="bsrg_scale")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: ec
+    end function
+    integer(c_int) function bsrg_acf_finish(ctx, kcore) bind(C, name="bsrg_acf_finish")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kcore
+    end function
+    integer(c_int) function bsrg_get_matrix(ctx, hcc, hcb, hbb, acf, htarg, otarg) &
+        bind(C, name="bsrg_get_matrix")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: hcc(*), hcb(*), hbb(*), acf(*), htarg(*), otarg(*)
+    end function
+    integer(c_int) function bsrg_wave_assemble(ctx, nch, ncp, mk, nrec, jtype, kpol, k1, k2, k3, k4, c, &
+        novl, ok3, ok4, oc, eps_c, ec, ntarg, etarg, iptar) bind(C, name="bsrg_wave_assemble")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nch, ncp, mk, nrec, novl, ntarg
+      integer(c_int) :: jtype(*), kpol(*), k1(*), k2(*), k3(*), k4(*), ok3(*), ok4(*), iptar(*)
+      real(c_double) :: c(*), oc(*), etarg(*)
+      real(c_double), value :: eps_c, ec
+    end function
+    integer(c_int) function bsrg_wave_write(ctx, path, tmax, ntarg, etarg, ec) bind(C, name="bsrg_wave_write")
+      import :: c_ptr, c_int, c_double, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: path(*)
+      real(c_double), value :: tmax, ec
+      integer(c_int), value :: ntarg
+      real(c_double) :: etarg(*)
+    end function
+  end interface
+
+contains
+
+  subroutine gpu_check(rc)
+    integer(c_int), intent(in) :: rc
+    character(kind=c_char), pointer :: msg(:)
+    character(256) :: text
+    integer :: i
+    if (rc == 0) return
+    call c_f_pointer(bsrg_last_error(), msg, [256])
+    text = ' '
+    do i = 1, 256
+      if (msg(i) == c_null_char) exit
+      text(i:i) = msg(i)
+    end do
+    write(*, '(a,i6,2a)') 'bsr_gpu: status ', rc, ': ', trim(text)
+    stop 'bsr_gpu failure'
+  end subroutine
+
+  subroutine gpu_init(device)
+    integer, intent(in) :: device
+    if (c_associated(gpu_ctx)) return
+    call gpu_check(bsrg_create(gpu_ctx, int(device, c_int)))
+  end subroutine
+
+  subroutine gpu_finish()
+    integer(c_int) :: rc
+    if (.not. c_associated(gpu_ctx)) return
+    rc = bsrg_destroy(gpu_ctx)
+    gpu_ctx = c_null_ptr
+  end subroutine
+
+  subroutine gpu_grid(ks, ns, ml, me, h, t)
+    integer, intent(in) :: ks, ns, ml, me
+    real(8), intent(in) :: h, t(ns+ks)
+    real(c_double) :: tt(ns+ks)
+    tt = t
+    call gpu_check(bsrg_set_grid(gpu_ctx, ks, ns, ml, me, h, tt))
+  end subroutine
+
+  subroutine gpu_mrk_cell(k, rkb)
+    integer, intent(in) :: k
+    real(8), intent(out) :: rkb(*)
+    call gpu_check(bsrg_mrk_cell(gpu_ctx, k, rkb))
+  end subroutine
+
+  subroutine gpu_convol(k, a, d, icase)
+    integer, intent(in) :: k, icase
+    real(8) :: a(*), d(*)
+    call gpu_check(bsrg_convol(gpu_ctx, k, icase, d, a))
+  end subroutine
+
+  subroutine gpu_orbitals(ns, nbf, pbs, iech, lbs, ibort, eps_ovl)
+    integer, intent(in) :: ns, nbf, iech(nbf), lbs(nbf), ibort(nbf, nbf)
+    real(8), intent(in) :: pbs(ns, nbf), eps_ovl
+    real(c_double) :: p(ns, nbf)
+    integer(c_int) :: ie(nbf), lb(nbf), ib(nbf, nbf)
+    p = pbs; ie = iech; lb = lbs; ib = ibort
+    call gpu_check(bsrg_set_orbitals(gpu_ctx, nbf, p, ie, lb, ib, eps_ovl))
+  end subroutine
+
+  subroutine gpu_allocate_matrix(nch, ncp, mk)
+    integer, intent(in) :: nch, ncp, mk
+    call gpu_check(bsrg_alloc_matrix(gpu_ctx, nch, ncp, mk))
+  end subroutine
+
+  subroutine gpu_update_hl(ich, jch, c)
+    integer, intent(in) :: ich, jch
+    real(8), intent(in) :: c
+    call gpu_check(bsrg_update_hl(gpu_ctx, ich, jch, c))
+  end subroutine
+
+  subroutine gpu_i_data(jtype, jpol, ncdata, ipt, k1, k2, k3, k4, cdata)
+    integer, intent(in) :: jtype, jpol, ncdata
+    integer(c_int), intent(in) :: ipt(*), k1(*), k2(*), k3(*), k4(*)
+    real(c_double), intent(in) :: cdata(*)
+    call gpu_check(bsrg_idata(gpu_ctx, 5, jtype, jpol, ncdata, ipt, k1, k2, k3, k4, cdata))
+  end subroutine
+
+  subroutine gpu_o_data(ncdata, ipt, k3, k4, cdata, eps_c)
+    integer, intent(in) :: ncdata
+    integer(c_int), intent(in) :: ipt(*), k3(*), k4(*)
+    real(c_double), intent(in) :: cdata(*)
+    real(8), intent(in) :: eps_c
+    call gpu_check(bsrg_odata(gpu_ctx, ncdata, ipt, k3, k4, cdata, eps_c))
+  end subroutine
+
+end module bsr_gpu

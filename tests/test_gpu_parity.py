@@ -213,3 +213,60 @@ def test_errors_are_reported(ctx):
         ctx.i_data(10, 0, lst)
     with pytest.raises(BsrGpuError, match="unknown case"):
         ctx.convol(0, np.zeros((w.ns, w.ns)), 5)
+
+
+def test_fortran_host_driver(oracle_lib, tmp_path):
+    """The Fortran host (module bsr_gpu over the C ABI, program bsr_mat_gpu)
+    runs a whole wave call by call and writes bsr_mat.nnn with the Fortran
+    runtime's own unformatted records; compared with the oracle's file."""
+    import subprocess
+    from tests.conftest import HAS_GPU
+    if not HAS_GPU:
+        pytest.skip("no HIP device")
+    O = oracle_lib
+    from bsr_amd import synth, jobfile
+    from bsr_amd.build import DRIVER
+    w = synth.make_wave(61, 8, 40, 4.0, 12.0, nch=5, ncp=6, mk=3, nb=3, ntarg=2, fill=0.3, name="fortran")
+    job, out, ref = tmp_path / "wave.bin", tmp_path / "bsr_mat.001", tmp_path / "oracle.001"
+    jobfile.write(job, w)
+    r = subprocess.run([DRIVER, str(job), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    oS, oH = O.run_wave(w)
+    O.write_bsr_mat(ref, w, oS, oH)
+    g, o = O.read_bsr_mat(out), O.read_bsr_mat(ref)
+    assert (g["ns"], g["nch"], g["ncp"], g["mk"], g["ns2"]) == (o["ns"], o["nch"], o["ncp"], o["mk"], o["ns2"])
+    for part, M in (("S", oS), ("H", oH)):
+        scale = np.abs(M.hcc).max()
+        assert sorted(g[part]["blocks"]) == sorted(o[part]["blocks"])
+        assert sorted(g[part]["vec"]) == sorted(o[part]["vec"])
+        assert sorted(g[part]["scal"]) == sorted(o[part]["scal"])
+        assert g[part]["idiag"] == o[part]["idiag"]
+        for key, b in list(o[part]["blocks"].items()) + list(enumerate(o[part]["diag"])):
+            gb = g[part]["blocks"][key] if isinstance(key, tuple) else g[part]["diag"][key]
+            assert np.max(np.abs(gb - b) / np.maximum(np.abs(b), 1e-3 * scale)) < TOL
+        for key, v in o[part]["vec"].items():
+            assert rel_err(g[part]["vec"][key], v) < TOL
+    for name in ("acf", "htarg", "otarg", "etarg"):
+        assert rel_err(g[name], o[name]) < TOL
+    assert g["ec"] == o["ec"]
+
+
+@pytest.mark.parametrize("nslices", [5, 6])
+def test_split_precision_wave(ctx, oracle_lib, nslices):
+    """Exchange contraction as Ozaki slices on bf16 MFMA: every element of H
+    within 1e-10 (floor 1e-3 of max|H|) of the FP64 oracle; S unaffected."""
+    O = oracle_lib
+    from bsr_amd import synth
+    w = synth.make_wave(71, 8, 120, 18.0, 40.0, nch=6, ncp=10, mk=6, nb=5, ntarg=5, fill=0.1, name="split")
+    try:
+        ctx.set_precision(1, nslices)
+        ctx.wave_assemble(w)
+        S, H = ctx.wave_get(0), ctx.wave_get(1)
+    finally:
+        ctx.set_precision(0, 0)
+    oS, oH = O.run_wave(w)
+    e = rel_err(H.hcc, oH.hcc)
+    print(f"[split] S={nslices} max rel err H.hcc {e:.2e}", flush=True)
+    assert e <= 1e-10
+    assert rel_err(S.hcc, oS.hcc) < TOL
+    assert rel_err(H.hcb, oH.hcb) < TOL
