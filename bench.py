@@ -66,6 +66,7 @@ def main():
     def step():
         # a new partial wave: grid and orbitals arrive, bands and bases are
         # rebuilt, lists processed, S and H assembled on the device
+        ctx.drop_caches()
         ctx.set_grid(w.ks, w.ns, w.ml, w.me, w.h, w.t)
         ctx.set_orbitals(w.pbs, w.iech, w.lbs, w.ibort, w.eps_ovl)
         ctx.wave_assemble(w, set_inputs=False)
@@ -73,14 +74,17 @@ def main():
     for i in range(a.warmup):
         step()
         log(f"warmup {i + 1}/{a.warmup}")
-    dev = []
+    dev, host = [], []
     t0 = time.perf_counter()
     for i in range(a.steps):
         step()
         dev.append(ctx.wave_timings())
+        host.append(ctx.wave_host_timings())
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / a.steps
     log(f"timed {a.steps} steps: {ms:.3f} ms/step")
+    log("device ms: " + json.dumps({k: round(sum(d[k] for d in dev) / len(dev), 3) for k in dev[0]}))
+    log("host ms:   " + json.dumps({k: round(sum(d[k] for d in host) / len(host), 3) for k in host[0]}))
     gemm_ms = sum(d["gemm"] for d in dev) / len(dev)
     total_dev = sum(d["total"] for d in dev) / len(dev)
     nss = w.ns * w.ns

@@ -129,6 +129,34 @@ class Context:
             len(o.c), i32(o.k3), i32(o.k4), f64(o.c), w.eps_c, w.ec, w.ntarg, f64(w.etarg), i32(w.iptar)))
         self._wave = (w.nch, w.ncp, w.mk)
 
+    def wave_prepare(self, w, set_inputs=True):
+        """Host lists, uploads, bands and bases; inputs stay resident."""
+        if set_inputs:
+            self.set_grid(w.ks, w.ns, w.ml, w.me, w.h, w.t)
+            self.set_orbitals(w.pbs, w.iech, w.lbs, w.ibort, w.eps_ovl)
+        jt, kp, k1, k2, k3, k4, c = w.concat()
+        o = w.olist
+        L.check(L.lib().bsrg_wave_prepare(
+            self._p, w.nch, w.ncp, w.mk, len(c), i32(jt), i32(kp), i32(k1), i32(k2), i32(k3), i32(k4), f64(c),
+            len(o.c), i32(o.k3), i32(o.k4), f64(o.c), w.eps_c, w.ec, w.ntarg, f64(w.etarg), i32(w.iptar)))
+        self._wave = (w.nch, w.ncp, w.mk)
+
+    def wave_launch(self):
+        L.check(L.lib().bsrg_wave_launch(self._p))
+
+    def wave_finish(self):
+        L.check(L.lib().bsrg_wave_finish(self._p))
+
+    def wave_engine(self):
+        return L.lib().bsrg_wave_engine(self._p)
+
+    def set_gemm_engine(self, engine=-1):
+        """0 library FP64 GEMM, 1-3 FP64 MFMA kernel tiles, -1 autotune."""
+        L.check(L.lib().bsrg_set_gemm_engine(self._p, engine))
+
+    def set_capture_safe(self, on=True):
+        L.check(L.lib().bsrg_set_capture_safe(self._p, 1 if on else 0))
+
     def wave_get(self, which):
         nch, ncp, mk = self._wave
         m = self._alloc_out(nch, ncp, mk)
@@ -144,6 +172,15 @@ class Context:
         t = np.zeros(5, order="F")
         L.check(L.lib().bsrg_wave_timings(self._p, t))
         return dict(bands=t[0], bases=t[1], gemm=t[2], terms=t[3], total=t[4])
+
+    def wave_host_timings(self):
+        t = np.zeros(4, order="F")
+        L.check(L.lib().bsrg_wave_host_timings(self._p, t))
+        return dict(terms=t[0], sort=t[1], upload=t[2], total=t[3])
+
+    def drop_caches(self):
+        """Forget resident bands and bases (the next wave rebuilds them)."""
+        L.check(L.lib().bsrg_drop_caches(self._p))
 
     def set_precision(self, mode, nslices=0):
         L.check(L.lib().bsrg_set_precision(self._p, mode, nslices))

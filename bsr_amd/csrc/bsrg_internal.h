@@ -51,9 +51,10 @@ void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd,
                  hipStream_t s);
 // upper-band densities dens[p](j,jp) of all unordered bound pairs p.
 void launch_pair_dens(int ns, int ks, int nb, const double* P, double* dens, hipStream_t s);
-// direct basis D[kk][p](i,ip) = sum d_p(j,jp) rkb_k(i,j,ip,jp)
+// direct basis D[kk][p](i,ip) = sum d_p(j,jp) rkb_k(i,j,ip,jp);
+// part: scratch of nk*npair*ks*ns*ks doubles
 void launch_direct_basis(int ns, int ks, int nk, int npair, const double* rkb, const double* dens,
-                         double* D, hipStream_t s);
+                         double* part, double* D, hipStream_t s);
 // exchange basis E[kk][a][b](x,y) = sum K(x,z,y,w) p_a(z) p_b(w)
 void launch_exchange_basis(int ns, int ks, int nk, int nb, const double* rkb, const double* P,
                            double* E, hipStream_t s);
@@ -85,6 +86,9 @@ void launch_convol_direct(int ns, int ks, int jcase, const double* rkb, const do
 void launch_convol_exchange(int ns, int ks, const double* rkb, const double* d, double* a,
                             hipStream_t s);
 
+// FP64 MFMA GEMM (ozaki.hip): C = A*B (+C), column-major; tm = 1, 2, 4
+int mfma_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B, size_t ldb, double* C,
+               size_t ldc, bool accumulate, int tm, hipStream_t s);
 // split-precision FP64 GEMM (ozaki.hip): C = A*B (+C), column-major
 size_t ozaki_workspace_bytes(int m, int n, int K, int S);
 int ozaki_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B, size_t ldb, double* C,

@@ -20,6 +20,8 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <tuple>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -110,6 +112,8 @@ struct Staged {
   std::vector<double> bc, rc, vc;
   DevVec<int> d[12];
   DevVec<double> dd[3], gemm;
+  bool has_gemm = false;
+  int gemm_klo = 0, gemm_khi = -1, engine = 2;
 };
 
 }  // namespace bsrg
@@ -120,8 +124,9 @@ struct bsrg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;
-  hipEvent_t ev[6] = {};
+  hipEvent_t ev[8] = {};
   double timings[5] = {0, 0, 0, 0, 0};
+  double host_ms[4] = {0, 0, 0, 0};  // prepare: lists->terms, sort, uploads, total
   int precision_mode = 0, nslices = 0;
 
   GridTables G;
@@ -140,21 +145,24 @@ struct bsrg_ctx {
   // bases for k = 0..basis_nk-1
   int basis_nk = 0, exch_nk = 0;
   DevVec<double> bands;  // [sbu | D(kk,p)] each ns*ks
-  DevVec<double> E, St, VT;
+  DevVec<double> E, St, VT, dpart;
   std::vector<double> St_host;
   std::vector<std::vector<double>> Q;  // Q[k][(i-1)+nbf*(j-1)] = <p_i|r^k|p_j>, computed lazily
   std::vector<char> Qok;
-  std::map<long long, int> wmap;        // GET_V registry (i1,i2) -> id
+  std::vector<int> wtab;                // GET_V registry (i1,i2) -> id, -1 if not yet made
   std::vector<double> wvec;             // nW x ns
   int wvec_uploaded = -1;
 
   Matrices cur, S, H;
-  bool wave_ok = false;
+  bool wave_ok = false, wave_prepared = false, capture_safe = false, capturing = false;
   int wave_ntarg = 0;
 
   // staged term uploads: S and H of the fused wave, and the per-call path
   Staged stS, stH, stC;
   DevVec<char> ozk;  // split-precision workspace
+  Terms TS, TH;      // term sets of the prepared wave (host scalars at finish)
+  std::map<std::tuple<int, int, int>, int> tuned;  // GEMM shape -> engine
+  int last_engine = -1, forced_engine = -1;
   DevVec<double> dtmp, dtmp2;
 };
 
@@ -211,7 +219,8 @@ int ensure_bases(bsrg_ctx* c, int nk, bool exchange) {
     launch_sb_upper(ns, ks, c->sb.p, c->bands.p, c->stream);
     double* D = c->bands.p + static_cast<size_t>(ns) * ks;
     if (np > 0) {
-      launch_direct_basis(ns, ks, nk, np, c->rkb.p, c->dens.p, D, c->stream);
+      HIPCHK(c->dpart.resize(static_cast<size_t>(nk) * np * ks * ns * ks));
+      launch_direct_basis(ns, ks, nk, np, c->rkb.p, c->dens.p, c->dpart.p, D, c->stream);
       HIPCHK(c->St.resize(static_cast<size_t>(nk) * np * np));
       launch_stab(ns, ks, nk, np, D, c->dens.p, c->St.p, c->stream);
     }
@@ -242,9 +251,8 @@ int ensure_bases(bsrg_ctx* c, int nk, bool exchange) {
 int get_v(bsrg_ctx* c, int ii, int jj, int& id) {
   const int nbf = c->nbf, ns = c->G.ns;
   if (ii < 1 || ii > nbf || jj < 1 || jj > nbf) return fail(-18, "get_V: orbital index out of range");
-  const long long key = static_cast<long long>(ii) * 1000003LL + jj;
-  auto it = c->wmap.find(key);
-  if (it != c->wmap.end()) { id = it->second; return 0; }
+  const size_t key = static_cast<size_t>(ii - 1) + static_cast<size_t>(nbf) * (jj - 1);
+  if (c->wtab[key] >= 0) { id = c->wtab[key]; return 0; }
   if (c->ibort[(ii - 1) + static_cast<size_t>(nbf) * (jj - 1)] == 0) return fail(-19, "get_V: <kl|nl> = 0 ?");
   std::vector<double> v(c->qbs.begin() + static_cast<size_t>(ns) * (jj - 1),
                         c->qbs.begin() + static_cast<size_t>(ns) * jj);
@@ -257,7 +265,7 @@ int get_v(bsrg_ctx* c, int ii, int jj, int& id) {
   }
   id = static_cast<int>(c->wvec.size() / ns);
   c->wvec.insert(c->wvec.end(), v.begin(), v.end());
-  c->wmap[key] = id;
+  c->wtab[key] = id;
   return 0;
 }
 
@@ -466,48 +474,28 @@ int upload_wvec(bsrg_ctx* c) {
   return 0;
 }
 
-// Queue the device work of a staged term set on the context's stream
-// (no host synchronisation).  overwrite: hcc/hcb start from zero.
-// e0/e1 (optional) bracket the exchange GEMM.
-int launch_terms(bsrg_ctx* c, Matrices& M, Terms& T, Staged& st, bool overwrite, hipEvent_t e0, hipEvent_t e1) {
-  const int ns = c->G.ns, ks = c->G.ks, nb = c->nb;
+// Upload a staged term set (and the GET_V vectors) for a later enqueue.
+int upload_terms(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
   hipStream_t s = c->stream;
   if (int rc = upload_wvec(c)) return rc;
-  if (e0) (void)hipEventRecord(e0, s);
-  const size_t nblk = M.nblk();
-  const size_t nss = static_cast<size_t>(ns) * ns;
-  if (!T.gemm.empty() && T.gemm_khi >= 0) {
-    const size_t nt = static_cast<size_t>(c->basis_nk) * nb * nb;
-    const size_t r0 = static_cast<size_t>(T.gemm_klo) * nb * nb;
-    const size_t nr = static_cast<size_t>(T.gemm_khi - T.gemm_klo + 1) * nb * nb;
+  st.has_gemm = !T.gemm.empty() && T.gemm_khi >= 0;
+  if (st.has_gemm) {
     HIPCHK(st.gemm.upload(T.gemm.data(), T.gemm.size(), s));
+    st.gemm_klo = T.gemm_klo;
+    st.gemm_khi = T.gemm_khi;
     if (c->precision_mode == 1) {
-      const size_t wb = ozaki_workspace_bytes(static_cast<int>(nss), static_cast<int>(nblk), static_cast<int>(nr),
-                                              c->nslices);
-      HIPCHK(c->ozk.resize(wb));
-      if (int rc = ozaki_dgemm(static_cast<int>(nss), static_cast<int>(nblk), static_cast<int>(nr), c->E.p + r0 * nss,
-                               nss, st.gemm.p + r0, nt, M.hcc.p, nss, !overwrite, c->nslices, c->ozk.p, s))
-        return rc;
-    } else {
-    const double alpha = 1.0, beta = overwrite ? 0.0 : 1.0;
-    rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, static_cast<int>(nss),
-                                      static_cast<int>(nblk), static_cast<int>(nr), &alpha, c->E.p + r0 * nss,
-                                      static_cast<int>(nss), st.gemm.p + r0, static_cast<int>(nt), &beta, M.hcc.p,
-                                      static_cast<int>(nss));
-    if (rs != rocblas_status_success) return fail(-101, "rocblas_dgemm failed");
+      const int nb = c->nb;
+      const size_t nss = static_cast<size_t>(c->G.ns) * c->G.ns;
+      const int nr = (T.gemm_khi - T.gemm_klo + 1) * nb * nb;
+      HIPCHK(c->ozk.resize(ozaki_workspace_bytes(static_cast<int>(nss), static_cast<int>(M.nblk()), nr,
+                                                 c->nslices)));
     }
-  } else if (overwrite) {
-    launch_fill(M.hcc_n(), 0.0, M.hcc.p, s);
   }
-  if (e1) (void)hipEventRecord(e1, s);
-  if (overwrite) launch_fill(M.hcb_n(), 0.0, M.hcb.p, s);
   if (!st.bblk.empty()) {
     HIPCHK(st.d[0].upload(st.bblk.data(), st.bblk.size(), s));
     HIPCHK(st.d[1].upload(st.bptr.data(), st.bptr.size(), s));
     HIPCHK(st.d[2].upload(st.bid.data(), st.bid.size(), s));
     HIPCHK(st.dd[0].upload(st.bc.data(), st.bc.size(), s));
-    launch_band_terms(ns, ks, static_cast<int>(st.bblk.size()), st.d[0].p, st.d[1].p, st.d[2].p, st.dd[0].p,
-                      c->bands.p, M.hcc.p, s);
   }
   if (!st.rblk.empty()) {
     HIPCHK(st.d[3].upload(st.rblk.data(), st.rblk.size(), s));
@@ -516,8 +504,6 @@ int launch_terms(bsrg_ctx* c, Matrices& M, Terms& T, Staged& st, bool overwrite,
     HIPCHK(st.d[6].upload(st.rw.data(), st.rw.size(), s));
     HIPCHK(st.d[7].upload(st.rs.data(), st.rs.size(), s));
     HIPCHK(st.dd[1].upload(st.rc.data(), st.rc.size(), s));
-    launch_rank1_terms(ns, static_cast<int>(st.rblk.size()), st.d[3].p, st.d[4].p, st.d[5].p, st.d[6].p, st.d[7].p,
-                       st.dd[1].p, c->St.p, c->VT.p, M.hcc.p, s);
   }
   if (!st.vq.empty()) {
     HIPCHK(st.d[8].upload(st.vq.data(), st.vq.size(), s));
@@ -525,9 +511,85 @@ int launch_terms(bsrg_ctx* c, Matrices& M, Terms& T, Staged& st, bool overwrite,
     HIPCHK(st.d[10].upload(st.vid.data(), st.vid.size(), s));
     HIPCHK(st.d[11].upload(st.vs.data(), st.vs.size(), s));
     HIPCHK(st.dd[2].upload(st.vc.data(), st.vc.size(), s));
+  }
+  return 0;
+}
+
+// The exchange-contraction GEMM hcc = E^T C (+hcc): split precision in
+// mode 1, else engine 0 = rocBLAS dgemm, 1/2/3 = FP64 MFMA kernel with
+// 32/64/128-row workgroup tiles.
+int run_gemm(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, int engine) {
+  const int nb = c->nb;
+  const size_t nss = static_cast<size_t>(c->G.ns) * c->G.ns, nblk = M.nblk();
+  const size_t nt = static_cast<size_t>(c->basis_nk) * nb * nb;
+  const size_t r0 = static_cast<size_t>(st.gemm_klo) * nb * nb;
+  const int nr = (st.gemm_khi - st.gemm_klo + 1) * nb * nb;
+  const double* A = c->E.p + r0 * nss;
+  const double* B = st.gemm.p + r0;
+  if (c->precision_mode == 1)
+    return ozaki_dgemm(static_cast<int>(nss), static_cast<int>(nblk), nr, A, nss, B, nt, M.hcc.p, nss, !overwrite,
+                       c->nslices, c->ozk.p, c->stream);
+  if (engine == 0) {
+    const double alpha = 1.0, beta = overwrite ? 0.0 : 1.0;
+    rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, static_cast<int>(nss),
+                                      static_cast<int>(nblk), nr, &alpha, A, static_cast<int>(nss), B,
+                                      static_cast<int>(nt), &beta, M.hcc.p, static_cast<int>(nss));
+    return rs == rocblas_status_success ? 0 : fail(-101, "rocblas_dgemm failed");
+  }
+  return mfma_dgemm(static_cast<int>(nss), static_cast<int>(nblk), nr, A, nss, B, nt, M.hcc.p, nss, !overwrite,
+                    engine == 1 ? 1 : (engine == 2 ? 2 : 4), c->stream);
+}
+
+// Autotuner: time every FP64 engine once on this GEMM shape (overwriting
+// the output, which the real launch rewrites) and keep the fastest; the
+// choice is cached per shape.  capture_safe excludes the library GEMM.
+int autotune_gemm(bsrg_ctx* c, Matrices& M, Staged& st, bool capture_safe) {
+  if (!st.has_gemm || c->precision_mode == 1) return 0;
+  if (c->forced_engine >= 0) { st.engine = c->forced_engine; return 0; }
+  const int nr = (st.gemm_khi - st.gemm_klo + 1) * c->nb * c->nb;
+  const auto key = std::make_tuple(c->G.ns, static_cast<int>(M.nblk()), nr * (capture_safe ? -1 : 1));
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) { st.engine = it->second; return 0; }
+  float best = 1e30f;
+  int best_e = 2;
+  for (int e = capture_safe ? 1 : 0; e <= 3; ++e) {
+    if (int rc = run_gemm(c, M, st, true, e)) return rc;  // warm-up
+    (void)hipEventRecord(c->ev[6], c->stream);
+    if (int rc = run_gemm(c, M, st, true, e)) return rc;
+    (void)hipEventRecord(c->ev[7], c->stream);
+    HIPCHK(hipEventSynchronize(c->ev[7]));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
+    if (ms < best) { best = ms; best_e = e; }
+  }
+  c->tuned[key] = best_e;
+  st.engine = best_e;
+  return 0;
+}
+
+// Device-only work of an uploaded term set (no host synchronisation, no
+// allocation): the part of a wave that a hipGraph captures.
+int enqueue_terms(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, hipEvent_t e0, hipEvent_t e1) {
+  const int ns = c->G.ns, ks = c->G.ks;
+  hipStream_t s = c->stream;
+  if (e0) (void)hipEventRecord(e0, s);
+  if (st.has_gemm) {
+    if (int rc = run_gemm(c, M, st, overwrite, st.engine)) return rc;
+    c->last_engine = c->precision_mode == 1 ? 10 + c->nslices : st.engine;
+  } else if (overwrite) {
+    launch_fill(M.hcc_n(), 0.0, M.hcc.p, s);
+  }
+  if (e1) (void)hipEventRecord(e1, s);
+  if (overwrite) launch_fill(M.hcb_n(), 0.0, M.hcb.p, s);
+  if (!st.bblk.empty())
+    launch_band_terms(ns, ks, static_cast<int>(st.bblk.size()), st.d[0].p, st.d[1].p, st.d[2].p, st.dd[0].p,
+                      c->bands.p, M.hcc.p, s);
+  if (!st.rblk.empty())
+    launch_rank1_terms(ns, static_cast<int>(st.rblk.size()), st.d[3].p, st.d[4].p, st.d[5].p, st.d[6].p, st.d[7].p,
+                       st.dd[1].p, c->St.p, c->VT.p, M.hcc.p, s);
+  if (!st.vq.empty())
     launch_vec_terms(ns, static_cast<int>(st.vq.size()), st.d[8].p, st.d[9].p, st.d[10].p, st.d[11].p, st.dd[2].p,
                      c->St.p, c->VT.p, M.hcb.p, s);
-  }
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -548,7 +610,11 @@ void finish_scalars(bsrg_ctx* c, Matrices& M, const Terms& T, bool overwrite) {
 // Per-call path: stage, launch, wait, scalars.
 int apply_terms(bsrg_ctx* c, Matrices& M, Terms& T, bool overwrite) {
   stage_terms(T, c->stC);
-  if (int rc = launch_terms(c, M, T, c->stC, overwrite, nullptr, nullptr)) return rc;
+  if (int rc = upload_terms(c, M, T, c->stC)) return rc;
+  c->stC.engine = c->forced_engine >= 0 ? c->forced_engine : 2;
+  if (overwrite)
+    if (int rc = autotune_gemm(c, M, c->stC, false)) return rc;
+  if (int rc = enqueue_terms(c, M, c->stC, overwrite, nullptr, nullptr)) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   finish_scalars(c, M, T, overwrite);
   return 0;
@@ -659,6 +725,9 @@ int bsrg_set_grid(bsrg_ctx* c, int ks, int ns, int ml, int me, double h, const d
   if (ks < 2 || ks > kMaxKs || ns < ks || !t) return fail(-1, "bsrg_set_grid: bad ks/ns");
   for (int i = 1; i < ns + ks; ++i)
     if (t[i] < t[i - 1]) return fail(-1, "bsrg_set_grid: knots must be non-decreasing");
+  if (c->grid_ok && c->G.ks == ks && c->G.ns == ns && c->G.ml == ml && c->G.me == me && c->G.h == h &&
+      std::equal(c->G.t.begin(), c->G.t.end(), t))
+    return 0;  // same grid: tables and R^k bands stay resident
   HIPCHK(hipSetDevice(c->device));
   c->G.build(ks, ns, ml, me, h, t);
   hipStream_t s = c->stream;
@@ -729,6 +798,10 @@ int bsrg_set_orbitals(bsrg_ctx* c, int nbf, const double* pbs, const int* iech, 
   if (int rc = check_ctx(c)) return rc;
   if (!c->grid_ok) return fail(-3, "grid not set (bsrg_set_grid)");
   if (nbf < 1 || !pbs || !iech || !lbs || !ibort) return fail(-1, "bsrg_set_orbitals: bad arguments");
+  if (c->orb_ok && c->nbf == nbf && c->eps_ovl == eps_ovl &&
+      std::equal(c->pbs.begin(), c->pbs.end(), pbs) && std::equal(c->iech.begin(), c->iech.end(), iech) &&
+      std::equal(c->lbs.begin(), c->lbs.end(), lbs) && std::equal(c->ibort.begin(), c->ibort.end(), ibort))
+    return 0;  // same orbitals: the bases stay resident
   HIPCHK(hipSetDevice(c->device));
   const int ns = c->G.ns, ks = c->G.ks;
   c->nbf = nbf;
@@ -769,7 +842,7 @@ int bsrg_set_orbitals(bsrg_ctx* c, int nbf, const double* pbs, const int* iech, 
   HIPCHK(hipStreamSynchronize(c->stream));
   c->Q.clear();
   c->Qok.clear();
-  c->wmap.clear();
+  c->wtab.assign(static_cast<size_t>(nbf) * nbf, -1);
   c->wvec.clear();
   c->wvec_uploaded = -1;
   c->basis_nk = 0;
@@ -875,15 +948,18 @@ int bsrg_get_matrix(bsrg_ctx* c, double* hcc, double* hcb, double* hbb, double* 
   return get_matrices(c, c->cur, hcc, hcb, hbb, acf, htarg, otarg);
 }
 
-int bsrg_wave_assemble(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int* jtype, const int* kpol,
-                       const int* k1, const int* k2, const int* k3, const int* k4, const double* cd, int novl,
-                       const int* ok3, const int* ok4, const double* ocd, double eps_c, double ec, int ntarg,
-                       const double* etarg, const int* iptar) {
+int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int* jtype, const int* kpol,
+                      const int* k1, const int* k2, const int* k3, const int* k4, const double* cd, int novl,
+                      const int* ok3, const int* ok4, const double* ocd, double eps_c, double ec, int ntarg,
+                      const double* etarg, const int* iptar) {
   if (int rc = check_ctx(c)) return rc;
   if (!c->orb_ok) return fail(-4, "orbitals not set (bsrg_set_orbitals)");
   if (nch < 1 || ncp < 0 || mk < 0 || nrec < 0 || novl < 0) return fail(-1, "bsrg_wave_assemble: bad sizes");
   HIPCHK(hipSetDevice(c->device));
   c->wave_ok = false;
+  c->wave_prepared = false;
+  using clk = std::chrono::steady_clock;
+  const auto h0 = clk::now();
   bool any_exch = false;
   for (int r = 0; r < nrec; ++r) {
     if (jtype[r] < 1 || jtype[r] > 9) return fail(-21, " I_data: unknown itype ");
@@ -901,7 +977,10 @@ int bsrg_wave_assemble(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const in
   HIPCHK(c->S.alloc(c->G.ns, nch, ncp, mk));
   HIPCHK(c->H.alloc(c->G.ns, nch, ncp, mk));
   // list processing on the host overlaps the band and basis kernels
-  Terms TS, TH;
+  Terms& TS = c->TS;
+  Terms& TH = c->TH;
+  TS = Terms();
+  TH = Terms();
   // S: sb on the channel diagonals + overlaps
   for (int ich = 1; ich <= nch; ++ich) band_hl(TS, ich, ich, 1.0);
   if (int rc = add_odata(c, c->S, TS, novl, nullptr, ok3, ok4, ocd, eps_c, 1.0, true)) return rc;
@@ -917,30 +996,79 @@ int bsrg_wave_assemble(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const in
     r = e;
   }
   for (int ich = 1; ich <= nch; ++ich) band_hl(TH, ich, ich, etarg[iptar[ich - 1] - 1] - ec);
+  const auto h1 = clk::now();
   stage_terms(TS, c->stS);
   stage_terms(TH, c->stH);
-  if (int rc = launch_terms(c, c->S, TS, c->stS, true, nullptr, nullptr)) return rc;
-  launch_symmetrize_diag(c->G.ns, nch, c->S.hcc.p, c->stream);
-  if (int rc = launch_terms(c, c->H, TH, c->stH, true, c->ev[3], c->ev[4])) return rc;
-  launch_symmetrize_diag(c->G.ns, nch, c->H.hcc.p, c->stream);
+  const auto h2 = clk::now();
+  if (int rc = upload_terms(c, c->S, TS, c->stS)) return rc;
+  if (int rc = upload_terms(c, c->H, TH, c->stH)) return rc;
+  if (int rc = autotune_gemm(c, c->H, c->stH, c->capture_safe)) return rc;
+  const auto h3 = clk::now();
+  auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  c->host_ms[0] = ms(h0, h1);
+  c->host_ms[1] = ms(h1, h2);
+  c->host_ms[2] = ms(h2, h3);
+  c->host_ms[3] = ms(h0, h3);
+  c->wave_prepared = true;
+  return 0;
+}
+
+int bsrg_wave_launch(bsrg_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->wave_prepared) return fail(-8, "no prepared wave (bsrg_wave_prepare)");
+  const bool timed = !c->capturing;  // no event records inside a graph capture
+  if (timed) (void)hipEventRecord(c->ev[2], c->stream);
+  if (int rc = enqueue_terms(c, c->S, c->stS, true, nullptr, nullptr)) return rc;
+  launch_symmetrize_diag(c->G.ns, c->S.nch, c->S.hcc.p, c->stream);
+  if (int rc = enqueue_terms(c, c->H, c->stH, true, timed ? c->ev[3] : nullptr, timed ? c->ev[4] : nullptr))
+    return rc;
+  launch_symmetrize_diag(c->G.ns, c->H.nch, c->H.hcc.p, c->stream);
   HIPCHK(hipGetLastError());
-  (void)hipEventRecord(c->ev[5], c->stream);
+  if (timed) (void)hipEventRecord(c->ev[5], c->stream);
+  return 0;
+}
+
+int bsrg_wave_finish(bsrg_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->wave_prepared) return fail(-8, "no prepared wave (bsrg_wave_prepare)");
+  HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
-  finish_scalars(c, c->S, TS, true);
-  finish_scalars(c, c->H, TH, true);
   float ms = 0, ms2 = 0;
-  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]); c->timings[0] = ms;
-  (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]); c->timings[1] = ms;
+  if (!c->wave_ok) {
+    finish_scalars(c, c->S, c->TS, true);
+    finish_scalars(c, c->H, c->TH, true);
+    acf_finish(c->H, 0);
+    // the reference keeps one bsr_matrix module: H is recorded after the
+    // overlap pass, so it carries the otarg accumulated there
+    c->H.otarg = c->S.otarg;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]); c->timings[0] = ms;
+    (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]); c->timings[1] = ms;
+  }
   (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); c->timings[2] = ms;
   (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
   (void)hipEventElapsedTime(&ms2, c->ev[4], c->ev[5]); c->timings[3] = ms + ms2;
-  (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[5]); c->timings[4] = ms;
-  acf_finish(c->H, 0);
-  // the reference keeps one bsr_matrix module: H is recorded after the
-  // overlap pass, so it carries the otarg accumulated there
-  c->H.otarg = c->S.otarg;
+  (void)hipEventElapsedTime(&ms, c->ev[c->wave_ok ? 2 : 0], c->ev[5]); c->timings[4] = ms;
   c->wave_ok = true;
-  c->wave_ntarg = ntarg;
+  c->wave_ntarg = c->H.nch > 0 ? c->wave_ntarg : 0;
+  return 0;
+}
+
+int bsrg_wave_assemble(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int* jtype, const int* kpol,
+                       const int* k1, const int* k2, const int* k3, const int* k4, const double* cd, int novl,
+                       const int* ok3, const int* ok4, const double* ocd, double eps_c, double ec, int ntarg,
+                       const double* etarg, const int* iptar) {
+  if (int rc = bsrg_wave_prepare(c, nch, ncp, mk, nrec, jtype, kpol, k1, k2, k3, k4, cd, novl, ok3, ok4, ocd, eps_c,
+                                 ec, ntarg, etarg, iptar))
+    return rc;
+  if (int rc = bsrg_wave_launch(c)) return rc;
+  return bsrg_wave_finish(c);
+}
+
+int bsrg_wave_engine(bsrg_ctx* c) { return c ? c->last_engine : -1; }
+
+int bsrg_wave_host_timings(bsrg_ctx* c, double* ms4) {
+  if (int rc = check_ctx(c)) return rc;
+  for (int i = 0; i < 4; ++i) ms4[i] = c->host_ms[i];
   return 0;
 }
 
@@ -1048,10 +1176,117 @@ int bsrg_wave_timings(bsrg_ctx* c, double* ms5) {
   return 0;
 }
 
+int bsrg_set_capture_safe(bsrg_ctx* c, int on) {
+  if (int rc = check_ctx(c)) return rc;
+  c->capture_safe = on != 0;
+  return 0;
+}
+
+int bsrg_set_gemm_engine(bsrg_ctx* c, int engine) {
+  if (int rc = check_ctx(c)) return rc;
+  if (engine < -1 || engine > 3) return fail(-1, "bsrg_set_gemm_engine: engine must be -1..3");
+  c->forced_engine = engine;
+  return 0;
+}
+
+struct bsrg_graph {
+  std::vector<bsrg_ctx*> ctx;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t t0 = nullptr, t1 = nullptr, fork = nullptr;
+  std::vector<hipEvent_t> join;
+};
+
+int bsrg_graph_destroy(bsrg_graph* g) {
+  if (!g) return 0;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  for (hipEvent_t e : g->join) if (e) (void)hipEventDestroy(e);
+  if (g->t0) (void)hipEventDestroy(g->t0);
+  if (g->t1) (void)hipEventDestroy(g->t1);
+  if (g->fork) (void)hipEventDestroy(g->fork);
+  delete g;
+  return 0;
+}
+
+int bsrg_graph_capture(int n, bsrg_ctx** ctxs, bsrg_graph** out) {
+  if (n < 1 || !ctxs || !out) return fail(-1, "bsrg_graph_capture: bad arguments");
+  for (int i = 0; i < n; ++i) {
+    if (int rc = check_ctx(ctxs[i])) return rc;
+    if (!ctxs[i]->wave_prepared) return fail(-8, "bsrg_graph_capture: wave not prepared");
+    if (ctxs[i]->device != ctxs[0]->device) return fail(-5, "bsrg_graph_capture: contexts on different devices");
+    if (ctxs[i]->stH.has_gemm && ctxs[i]->precision_mode == 0 && ctxs[i]->stH.engine == 0)
+      return fail(-9, "bsrg_graph_capture: prepare with bsrg_set_capture_safe(ctx,1)");
+  }
+  HIPCHK(hipSetDevice(ctxs[0]->device));
+  std::unique_ptr<bsrg_graph, int (*)(bsrg_graph*)> g(new bsrg_graph(), bsrg_graph_destroy);
+  g->ctx.assign(ctxs, ctxs + n);
+  HIPCHK(hipEventCreate(&g->t0));
+  HIPCHK(hipEventCreate(&g->t1));
+  HIPCHK(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+  g->join.assign(n, nullptr);
+  for (int i = 0; i < n; ++i) HIPCHK(hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming));
+  hipStream_t s0 = ctxs[0]->stream;
+  for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(ctxs[i]->stream));
+  HIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeGlobal));
+  int rc = 0;
+  for (int i = 0; i < n; ++i) ctxs[i]->capturing = true;
+  (void)hipEventRecord(g->fork, s0);
+  for (int i = 1; i < n && !rc; ++i)
+    if (hipStreamWaitEvent(ctxs[i]->stream, g->fork, 0) != hipSuccess) rc = fail(-100, "graph fork failed");
+  for (int i = 0; i < n && !rc; ++i) rc = bsrg_wave_launch(ctxs[i]);
+  for (int i = 1; i < n && !rc; ++i) {
+    (void)hipEventRecord(g->join[i], ctxs[i]->stream);
+    if (hipStreamWaitEvent(s0, g->join[i], 0) != hipSuccess) rc = fail(-100, "graph join failed");
+  }
+  hipError_t e = hipStreamEndCapture(s0, &g->graph);
+  for (int i = 0; i < n; ++i) ctxs[i]->capturing = false;
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(-100, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  *out = g.release();
+  return 0;
+}
+
+int bsrg_graph_launch(bsrg_graph* g) {
+  if (!g || !g->exec) return fail(-1, "bsrg_graph_launch: no graph");
+  HIPCHK(hipSetDevice(g->ctx[0]->device));
+  HIPCHK(hipEventRecord(g->t0, g->ctx[0]->stream));
+  HIPCHK(hipGraphLaunch(g->exec, g->ctx[0]->stream));
+  HIPCHK(hipEventRecord(g->t1, g->ctx[0]->stream));
+  return 0;
+}
+
+int bsrg_graph_sync(bsrg_graph* g, double* wave_ms, double* total_ms) {
+  if (!g || !g->exec) return fail(-1, "bsrg_graph_sync: no graph");
+  HIPCHK(hipStreamSynchronize(g->ctx[0]->stream));
+  for (size_t i = 0; i < g->ctx.size(); ++i) {
+    bsrg_ctx* c = g->ctx[i];
+    if (!c->wave_ok)
+      if (int rc = bsrg_wave_finish(c)) return rc;
+    // device time of the wave's launch step when it ran alone (eager)
+    if (wave_ms) wave_ms[i] = c->timings[2] + c->timings[3];
+  }
+  if (total_ms) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, g->t0, g->t1));
+    *total_ms = ms;
+  }
+  return 0;
+}
+
+int bsrg_drop_caches(bsrg_ctx* c) {
+  if (int rc = check_ctx(c)) return rc;
+  c->band_nk = 0;
+  c->basis_nk = 0;
+  c->exch_nk = 0;
+  return 0;
+}
+
 int bsrg_set_precision(bsrg_ctx* c, int mode, int nslices) {
   if (int rc = check_ctx(c)) return rc;
   if (mode != 0 && mode != 1) return fail(-9, "precision mode must be 0 (FP64) or 1 (split)");
-  if (mode == 1 && nslices == 0) nslices = 5;
+  if (mode == 1 && nslices == 0) nslices = 6;
   if (mode == 1 && (nslices < 2 || nslices > 8)) return fail(-9, "split precision: slices must be 2..8");
   c->precision_mode = mode;
   c->nslices = nslices;

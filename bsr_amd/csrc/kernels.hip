@@ -81,7 +81,12 @@ __global__ void k_moments_scale(DevGrid g, int k0, double* mom) {
   const double hp = dpowi(1.0 + g.h, p) * (1.0 + g.h);
   double* m = mom + static_cast<size_t>(kk * 2 + s) * nv * ik;
   const int lo = g.ml + ks - 1, hi = g.ml + g.me - ks + 1;  // 0-based, inclusive
-  for (int iv = lo; iv <= hi && iv < nv; ++iv) m[iv * ik + ii] = m[(iv - 1) * ik + ii] * hp;
+  if (lo > hi || lo >= nv) return;
+  double v = m[(lo - 1) * ik + ii];  // carried in a register: same products, no reload
+  for (int iv = lo; iv <= hi && iv < nv; ++iv) {
+    v *= hp;
+    m[iv * ik + ii] = v;
+  }
 }
 
 // ---- a6: diagonal-cell quadrature (rk_triang) ---------------------------
@@ -146,9 +151,14 @@ __global__ void k_diag_scale(DevGrid g, double* rkd) {
   const double hp = g.h + 1.0;
   const int lo = g.ml + ks - 1, hi = g.ml + g.me - ks + 1;
   double* base = rkd + static_cast<size_t>(kk) * nv * ik * ik;
-  for (int q = threadIdx.x; q < ik * ik; q += blockDim.x)
-    for (int iv = lo; iv <= hi && iv < nv; ++iv)
-      base[static_cast<size_t>(iv) * ik * ik + q] = base[static_cast<size_t>(iv - 1) * ik * ik + q] * hp;
+  if (lo > hi || lo >= nv) return;
+  for (int q = threadIdx.x; q < ik * ik; q += blockDim.x) {
+    double v = base[static_cast<size_t>(lo - 1) * ik * ik + q];
+    for (int iv = lo; iv <= hi && iv < nv; ++iv) {
+      v *= hp;
+      base[static_cast<size_t>(iv) * ik * ik + q] = v;
+    }
+  }
 }
 
 // ---- a7: band assembly (mrk_cell) ----------------------------------------
@@ -176,15 +186,15 @@ __global__ void k_pair_sums(int ns, int ks, int nv, const double* __restrict__ m
 __global__ void k_band(int ns, int ks, int nv, int nk, const double* __restrict__ mom,
                        const double* __restrict__ rkd, const double* __restrict__ msum,
                        double* __restrict__ rkb) {
+  // grid (ceil(ns/64), ceil(ns/4), ks*ks*nk), block 256 = 64 i x 4 j
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (i >= ns || j >= ns) return;
+  const int kk = blockIdx.z / (ks * ks), r = blockIdx.z % (ks * ks);
+  const int ip = r % ks, jp = r / ks;
   const size_t per = static_cast<size_t>(ns) * ns * ks * ks;
-  const size_t tid = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid >= per * nk) return;
-  const int kk = static_cast<int>(tid / per);
-  size_t r = tid % per;
-  const int i = static_cast<int>(r % ns); r /= ns;
-  const int j = static_cast<int>(r % ns); r /= ns;
-  const int ip = static_cast<int>(r % ks);
-  const int jp = static_cast<int>(r / ks);
+  const size_t tid = kk * per + i + static_cast<size_t>(ns) * (j + static_cast<size_t>(ns) * r);
+  (void)nk;
   double acc = 0.0;
   if (i + ip < ns && j + jp < ns) {
     const int ik = ks * (ks + 1) / 2;
@@ -236,42 +246,67 @@ __global__ void k_pair_dens(int ns, int ks, int nb, const double* P, double* den
 }
 
 // ---- a9 direct: D[kk][p](i,ip) = sum_{j,jp} d_p(j,jp) rkb_k(i,j,ip,jp) -----
-// grid (ceil(ns/64), ks, nk*nchunk), block 256 = 64 rows x 4 jp-groups,
-// up to 16 pairs per chunk; partial sums reduced through LDS.
-__global__ void k_direct_basis(int ns, int ks, int npair, const double* __restrict__ rkb,
-                               const double* __restrict__ dens, double* __restrict__ D) {
-  __shared__ double red[4][64][16];
-  const int il = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  const int i = blockIdx.x * 64 + il, ip = blockIdx.y;
+// Split over the r2 offset jp: one wavefront per (64 rows i, ip, jp, kk,
+// chunk of 16 pairs) sums over j; the partials part[kk][p][jp](i,ip) are
+// then added in jp order by k_direct_reduce (deterministic).  The pair
+// densities of the current jp are staged in LDS.
+__global__ void k_direct_partial(int ns, int ks, int npair, const double* __restrict__ rkb,
+                                 const double* __restrict__ dens, double* __restrict__ part) {
+  extern __shared__ double dl[];  // [16][ns]
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int ip = blockIdx.y % ks, jp = blockIdx.y / ks;
   const int nchunk = (npair + 15) / 16;
   const int kk = blockIdx.z / nchunk, p0 = (blockIdx.z % nchunk) * 16;
   const int np = min(16, npair - p0);
-  const size_t band = static_cast<size_t>(ns) * ns * ks * ks;
-  const double* R = rkb + kk * band;
+  for (int q = threadIdx.x; q < 16 * ns; q += 64) {
+    const int p = q / ns, j = q % ns;
+    dl[q] = p < np ? dens[static_cast<size_t>(p0 + p) * ns * ks + j + static_cast<size_t>(ns) * jp] : 0.0;
+  }
+  __syncthreads();
   double acc[16];
 #pragma unroll
   for (int p = 0; p < 16; ++p) acc[p] = 0.0;
   if (i < ns - ip) {
-    for (int jp = grp; jp < ks; jp += 4) {
-      const double* Rc = R + i + static_cast<size_t>(ns) * ns * (ip + static_cast<size_t>(ks) * jp);
-      for (int j = 0; j < ns - jp; ++j) {
-        const double r = Rc[static_cast<size_t>(ns) * j];
-        const double* dj = dens + j + static_cast<size_t>(ns) * jp;
+    const double* Rc = rkb + static_cast<size_t>(kk) * ns * ns * ks * ks + i +
+                       static_cast<size_t>(ns) * ns * (ip + static_cast<size_t>(ks) * jp);
+    const int nj = ns - jp;
+    int j = 0;
+    for (; j + 4 <= nj; j += 4) {
+      const double r0 = Rc[static_cast<size_t>(ns) * j], r1 = Rc[static_cast<size_t>(ns) * (j + 1)];
+      const double r2 = Rc[static_cast<size_t>(ns) * (j + 2)], r3 = Rc[static_cast<size_t>(ns) * (j + 3)];
 #pragma unroll
-        for (int p = 0; p < 16; ++p)
-          if (p < np) acc[p] += dj[static_cast<size_t>(p0 + p) * ns * ks] * r;
+      for (int p = 0; p < 16; ++p) {
+        const double* d = dl + p * ns + j;
+        acc[p] += d[0] * r0;
+        acc[p] += d[1] * r1;
+        acc[p] += d[2] * r2;
+        acc[p] += d[3] * r3;
       }
     }
-  }
+    for (; j < nj; ++j) {
+      const double r = Rc[static_cast<size_t>(ns) * j];
 #pragma unroll
-  for (int p = 0; p < 16; ++p) red[grp][il][p] = acc[p];
-  __syncthreads();
-  if (grp == 0 && i < ns) {
-    for (int p = 0; p < np; ++p) {
-      const double v = (i < ns - ip) ? ((red[0][il][p] + red[1][il][p]) + (red[2][il][p] + red[3][il][p])) : 0.0;
-      D[(static_cast<size_t>(kk) * npair + p0 + p) * ns * ks + i + static_cast<size_t>(ns) * ip] = v;
+      for (int p = 0; p < 16; ++p) acc[p] += dl[p * ns + j] * r;
     }
   }
+  if (i < ns) {
+    for (int p = 0; p < np; ++p)
+      part[((static_cast<size_t>(kk) * npair + p0 + p) * ks + jp) * ns * ks + i + static_cast<size_t>(ns) * ip] =
+          acc[p];
+  }
+}
+
+__global__ void k_direct_reduce(int ns, int ks, int npair, int nk, const double* __restrict__ part,
+                                double* __restrict__ D) {
+  const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t nsk = static_cast<size_t>(ns) * ks;
+  if (q >= nsk * npair * nk) return;
+  const size_t kp = q / nsk, e = q % nsk;  // kp = kk*npair + p
+  const int i = static_cast<int>(e % ns), ip = static_cast<int>(e / ns);
+  double v = 0.0;
+  if (i < ns - ip)
+    for (int jp = 0; jp < ks; ++jp) v += part[(kp * ks + jp) * nsk + e];
+  D[kp * nsk + e] = v;
 }
 
 // ---- a9 exchange basis ---------------------------------------------------
@@ -325,18 +360,19 @@ __global__ void k_exchange_basis(int ns, int ks, const double* __restrict__ rkb,
 }
 
 // ---- scalar / vector tables ------------------------------------------------
+// one wavefront per (p1, p2, kk): strided partial sums, then a fixed-order
+// shuffle-tree reduction (deterministic)
 __global__ void k_stab(int ns, int ks, int npair, const double* D, const double* dens, double* St) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nk = gridDim.y, kk = blockIdx.y;
-  (void)nk;
-  if (gid >= npair * npair) return;
-  const int p1 = gid / npair, p2 = gid % npair;
+  const int p1 = blockIdx.x / npair, p2 = blockIdx.x % npair, kk = blockIdx.y;
   const double* a = D + (static_cast<size_t>(kk) * npair + p1) * ns * ks;
   const double* b = dens + static_cast<size_t>(p2) * ns * ks;
   double x = 0.0;
-  for (int ith = 0; ith < ks; ++ith)
-    for (int i = 0; i < ns - ith; ++i) x += a[i + ns * ith] * b[i + ns * ith];
-  St[(static_cast<size_t>(kk) * npair + p1) * npair + p2] = x;
+  for (int q = threadIdx.x; q < ns * ks; q += 64) {
+    const int i = q % ns, ith = q / ns;
+    if (i < ns - ith) x += a[q] * b[q];
+  }
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  if (threadIdx.x == 0) St[(static_cast<size_t>(kk) * npair + p1) * npair + p2] = x;
 }
 
 // y = B v, B symmetric upper band; contributions in the reference's order
@@ -518,8 +554,8 @@ void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd,
                  hipStream_t s) {
   hipLaunchKernelGGL(k_pair_sums, dim3(nblocks(g.ns, 256), g.ks, nk * 2), dim3(256), 0, s, g.ns, g.ks, g.nv, mom,
                      msum);
-  const size_t n = static_cast<size_t>(g.ns) * g.ns * g.ks * g.ks * nk;
-  hipLaunchKernelGGL(k_band, dim3(nblocks(n, 256)), dim3(256), 0, s, g.ns, g.ks, g.nv, nk, mom, rkd, msum, rkb);
+  hipLaunchKernelGGL(k_band, dim3((g.ns + 63) / 64, (g.ns + 3) / 4, g.ks * g.ks * nk), dim3(256), 0, s, g.ns, g.ks,
+                     g.nv, nk, mom, rkd, msum, rkb);
 }
 
 void launch_pair_dens(int ns, int ks, int nb, const double* P, double* dens, hipStream_t s) {
@@ -527,11 +563,13 @@ void launch_pair_dens(int ns, int ks, int nb, const double* P, double* dens, hip
   hipLaunchKernelGGL(k_pair_dens, dim3(nblocks(n, 256)), dim3(256), 0, s, ns, ks, nb, P, dens);
 }
 
-void launch_direct_basis(int ns, int ks, int nk, int npair, const double* rkb, const double* dens, double* D,
-                         hipStream_t s) {
+void launch_direct_basis(int ns, int ks, int nk, int npair, const double* rkb, const double* dens, double* part,
+                         double* D, hipStream_t s) {
   const int nchunk = (npair + 15) / 16;
-  hipLaunchKernelGGL(k_direct_basis, dim3((ns + 63) / 64, ks, nk * nchunk), dim3(256), 0, s, ns, ks, npair, rkb,
-                     dens, D);
+  hipLaunchKernelGGL(k_direct_partial, dim3((ns + 63) / 64, ks * ks, nk * nchunk), dim3(64),
+                     sizeof(double) * 16 * ns, s, ns, ks, npair, rkb, dens, part);
+  const size_t n = static_cast<size_t>(ns) * ks * npair * nk;
+  hipLaunchKernelGGL(k_direct_reduce, dim3(nblocks(n, 256)), dim3(256), 0, s, ns, ks, npair, nk, part, D);
 }
 
 void launch_exchange_basis(int ns, int ks, int nk, int nb, const double* rkb, const double* P, double* E,
@@ -547,8 +585,7 @@ void launch_exchange_basis(int ns, int ks, int nk, int nb, const double* rkb, co
 }
 
 void launch_stab(int ns, int ks, int nk, int npair, const double* D, const double* dens, double* St, hipStream_t s) {
-  hipLaunchKernelGGL(k_stab, dim3(nblocks(static_cast<size_t>(npair) * npair, 64), nk), dim3(64), 0, s, ns, ks,
-                     npair, D, dens, St);
+  hipLaunchKernelGGL(k_stab, dim3(npair * npair, nk), dim3(64), 0, s, ns, ks, npair, D, dens, St);
 }
 
 void launch_vtab(int ns, int ks, int nk, int npair, int nb, const double* D, const double* P, double* V,

@@ -165,6 +165,57 @@ __global__ __launch_bounds__(256) void k_ozaki(int m, int n, int mp, int np, int
     }
 }
 
+// Plain FP64 GEMM on FP64 MFMA (16x16x4): workgroup tile (TM*32) x 64,
+// 4 waves of (TM*16) x 32; A column-major (rows fastest), B column-major
+// (K fastest).  K is zero-padded to a multiple of 4 by predicated loads.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <int TM>
+__global__ __launch_bounds__(256) void k_dgemm(int m, int n, int K, const double* __restrict__ A, size_t lda,
+                                                const double* __restrict__ B, size_t ldb, double* __restrict__ C,
+                                                size_t ldc, int accumulate) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row0 = blockIdx.x * (TM * 32) + (wave & 1) * (TM * 16), col0 = blockIdx.y * 64 + (wave >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  f64x4 acc[TM][2];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f64x4{0.0, 0.0, 0.0, 0.0};
+  int rows[TM], cols[2];
+#pragma unroll
+  for (int a = 0; a < TM; ++a) rows[a] = min(row0 + a * 16 + lr, m - 1);
+#pragma unroll
+  for (int b = 0; b < 2; ++b) cols[b] = min(col0 + b * 16 + lr, n - 1);
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const int k = k0 + lk;
+    const bool kin = k < K;
+    double av[TM], bv[2];
+#pragma unroll
+    for (int a = 0; a < TM; ++a) av[a] = kin ? A[rows[a] + lda * k] : 0.0;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) bv[b] = kin ? B[k + ldb * cols[b]] : 0.0;
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+  }
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int col = col0 + b * 16 + lr;
+      if (col >= n) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        // FP64 16x16x4 output: D[i][j] in lane j + 16*(i%4), register i/4
+        const int row = row0 + a * 16 + lk + 4 * e;
+        if (row >= m) continue;
+        double* c = C + row + ldc * col;
+        *c = accumulate ? *c + acc[a][b][e] : acc[a][b][e];
+      }
+    }
+}
+
 template <int S>
 void launch_k(dim3 g, hipStream_t s, int m, int n, int mp, int np, int Kp, const __bf16* As, const __bf16* Bs,
               const double* sig, const double* tau, double* C, size_t ldc, int acc) {
@@ -172,6 +223,23 @@ void launch_k(dim3 g, hipStream_t s, int m, int n, int mp, int np, int Kp, const
 }
 
 }  // namespace
+
+int mfma_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B, size_t ldb, double* C, size_t ldc,
+               bool accumulate, int tm, hipStream_t s) {
+  if (m <= 0 || n <= 0 || K <= 0) return 0;
+  const int acc = accumulate ? 1 : 0;
+  if (tm == 4) {
+    hipLaunchKernelGGL(k_dgemm<4>, dim3((m + 127) / 128, (n + 63) / 64), dim3(256), 0, s, m, n, K, A, lda, B, ldb, C,
+                       ldc, acc);
+  } else if (tm == 2) {
+    hipLaunchKernelGGL(k_dgemm<2>, dim3((m + 63) / 64, (n + 63) / 64), dim3(256), 0, s, m, n, K, A, lda, B, ldb, C,
+                       ldc, acc);
+  } else {
+    hipLaunchKernelGGL(k_dgemm<1>, dim3((m + 31) / 32, (n + 63) / 64), dim3(256), 0, s, m, n, K, A, lda, B, ldb, C,
+                       ldc, acc);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : fail(-100, "FP64 MFMA GEMM launch failed");
+}
 
 size_t ozaki_workspace_bytes(int m, int n, int K, int S) {
   const size_t mp = (static_cast<size_t>(m) + 63) / 64 * 64, np = (static_cast<size_t>(n) + 63) / 64 * 64;

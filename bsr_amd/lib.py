@@ -69,6 +69,19 @@ SIGNATURES = {
     "bsrg_wave_write": [C.c_void_p, C.c_char_p, C.c_double, C.c_int, _dp, C.c_double],
     "bsrg_wave_timings": [C.c_void_p, _dp],
     "bsrg_set_precision": [C.c_void_p, C.c_int, C.c_int],
+    "bsrg_drop_caches": [C.c_void_p],
+    "bsrg_wave_prepare": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
+                          C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, _dp, _ip],
+    "bsrg_wave_launch": [C.c_void_p],
+    "bsrg_wave_finish": [C.c_void_p],
+    "bsrg_wave_engine": [C.c_void_p],
+    "bsrg_set_capture_safe": [C.c_void_p, C.c_int],
+    "bsrg_set_gemm_engine": [C.c_void_p, C.c_int],
+    "bsrg_wave_host_timings": [C.c_void_p, _dp],
+    "bsrg_graph_capture": [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
+    "bsrg_graph_launch": [C.c_void_p],
+    "bsrg_graph_sync": [C.c_void_p, _odp, C.POINTER(C.c_double)],
+    "bsrg_graph_destroy": [C.c_void_p],
 }
 _RESTYPE = {"bsrg_last_error": C.c_char_p}
 

@@ -108,7 +108,16 @@ class PartialWave:
 
     def concat(self):
         """All R^k records in call order (k outer, jtype inner,
-        state_res.f90:155-159): (jtype, kpol, k1, k2, k3, k4, c)."""
+        state_res.f90:155-159): (jtype, kpol, k1, k2, k3, k4, c), packed
+        once per wave as contiguous int32/float64 arrays."""
+        packed = self.__dict__.get("_packed")
+        if packed is None:
+            packed = tuple(np.ascontiguousarray(a, dtype=np.float64 if q == 6 else np.int32)
+                           for q, a in enumerate(self._concat()))
+            self.__dict__["_packed"] = packed
+        return packed
+
+    def _concat(self):
         parts = [[], [], [], [], [], [], []]
         for k in range(self.mk + 1):
             for jt in range(1, 10):
@@ -150,10 +159,12 @@ def _bound_orbitals(rng, g: G.Grid, nb: int):
 
 def make_wave(seed: int, ks: int, ns: int, z: float, rmax: float, nch: int, ncp: int, mk: int,
               nb: int = 5, ntarg: int = 5, fill: float = 0.05, h: float = 0.05,
-              name: str = "wave") -> PartialWave:
+              name: str = "wave", target_seed: int | None = None) -> PartialWave:
+    """target_seed, when given, draws the bound (target) orbitals from their
+    own stream, so that waves of one target share them."""
     rng = np.random.default_rng(seed)
     g = G.fixed_ns_grid(ks, ns, z, rmax, h)
-    P, lb = _bound_orbitals(rng, g, nb)
+    P, lb = _bound_orbitals(rng if target_seed is None else np.random.default_rng(target_seed), g, nb)
     lch = rng.integers(0, 4, nch)
     nbf = nb + nch
     pbs = np.zeros((g.ns, nbf), order="F")
@@ -276,3 +287,19 @@ def c3_specs(seed: int = 3, nwaves: int = 48):
 def c3_wave(spec, ncp: int = 0) -> PartialWave:
     return make_wave(spec["seed"], 8, spec["ns"], 18.0, 40.0, nch=spec["nch"], ncp=ncp, mk=8, nb=5, ntarg=5,
                      fill=0.05, name=f"c3-{spec['seed']}")
+
+
+def c4_stream(seed: int = 4, njobs: int = 60, rate: float = 20.0):
+    """Served stream: Poisson arrivals (mean `rate` jobs/s) of partial waves
+    of one target drawn around the production wave -- same grid (ks=8,
+    ns=200, Z=18, Rmax=40) and bound orbitals, k=0..10, 5% fill, nch ~
+    U[40,80], ncp ~ U[200,400].  Returns [(arrival_s, PartialWave)]."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, njobs))
+    out = []
+    for j in range(njobs):
+        w = make_wave(seed * 100000 + j, 8, 200, 18.0, 40.0, nch=int(rng.integers(40, 81)),
+                      ncp=int(rng.integers(200, 401)), mk=10, nb=5, ntarg=5, fill=0.05,
+                      name=f"c4-{j}", target_seed=seed)
+        out.append((float(t[j]), w))
+    return out

@@ -148,6 +148,54 @@ int bsrg_wave_write(bsrg_ctx* ctx, const char* path, double tmax, int ntarg,
  * [2] exchange GEMM, [3] band/rank-1/vector terms, [4] total. */
 int bsrg_wave_timings(bsrg_ctx* ctx, double* ms5);
 
+/* bsrg_wave_assemble in three steps, for batching:
+ *   prepare  host list processing, uploads, bands and bases, GEMM engine
+ *            choice (autotuned per shape); leaves the inputs resident;
+ *   launch   device-only work of S and H on the context's stream (no
+ *            host synchronisation or allocation: capturable in a hipGraph);
+ *   finish   wait, host scalars (hbb, htarg, otarg, ACF), timings.
+ * Arguments of prepare as bsrg_wave_assemble.  No counterpart in the
+ * reference (its SUB1 does the whole wave in one pass). */
+int bsrg_wave_prepare(bsrg_ctx* ctx, int nch, int ncp, int mk, int nrec, const int* jtype,
+                      const int* kpol, const int* k1, const int* k2, const int* k3,
+                      const int* k4, const double* cdata, int novl, const int* ok3,
+                      const int* ok4, const double* ocdata, double eps_c, double ec,
+                      int ntarg, const double* etarg, const int* iptar);
+int bsrg_wave_launch(bsrg_ctx* ctx);
+int bsrg_wave_finish(bsrg_ctx* ctx);
+
+/* Engine of the last exchange contraction: 0 library FP64 GEMM, 1-3 FP64
+ * MFMA kernel (32/64/128-row tiles), 10+S split precision with S slices. */
+int bsrg_wave_engine(bsrg_ctx* ctx);
+
+/* Force the FP64 exchange-contraction engine (0..3 as above) or -1 to let
+ * the autotuner choose (default). */
+int bsrg_set_gemm_engine(bsrg_ctx* ctx, int engine);
+
+/* Host time (ms) of the last prepare: [0] lists to terms (overlapping the
+ * band and basis kernels), [1] sorting, [2] uploads and GEMM autotune,
+ * [3] total. */
+int bsrg_wave_host_timings(bsrg_ctx* ctx, double* ms4);
+
+/* 1: the autotuner of later prepares only considers engines that can be
+ * captured in a hipGraph (the hand-written kernels). */
+int bsrg_set_capture_safe(bsrg_ctx* ctx, int on);
+
+/* One hipGraph holding the launch step of n prepared waves (one branch per
+ * context/stream, joined on the first context's stream).  launch enqueues
+ * it; sync waits, finishes every wave and returns per-wave device time and
+ * the whole graph's time (ms).  No counterpart in the reference. */
+typedef struct bsrg_graph bsrg_graph;
+int bsrg_graph_capture(int n, bsrg_ctx** ctxs, bsrg_graph** graph);
+int bsrg_graph_launch(bsrg_graph* graph);
+int bsrg_graph_sync(bsrg_graph* graph, double* wave_ms, double* total_ms);
+int bsrg_graph_destroy(bsrg_graph* graph);
+
+/* Forget the resident R^k bands and orbital bases so the next call
+ * rebuilds them (bsrg_set_grid / bsrg_set_orbitals keep them when called
+ * again with identical inputs).  No counterpart in the reference. */
+int bsrg_drop_caches(bsrg_ctx* ctx);
+
 /* Precision of the exchange contraction for later bsrg_wave_assemble
  * calls: 0 = plain FP64 (default), 1 = split precision (Ozaki slices on
  * bf16 MFMA, FP32 accumulation, FP64 recombination) with nslices slices. */

@@ -154,7 +154,8 @@ def test_fused_wave_c1(ctx, oracle_lib, tmp_path):
         assert g[part]["idiag"] == r[part]["idiag"]
         assert sorted(g[part]["blocks"]) == sorted(r[part]["blocks"])
         blocks = list(r[part]["blocks"].items()) + [((i, i), b) for i, b in enumerate(r[part]["diag"])]
-        gb = dict(g[part]["blocks"], **{(i, i): b for i, b in enumerate(g[part]["diag"])})
+        gb = dict(g[part]["blocks"])
+        gb.update({(i, i): b for i, b in enumerate(g[part]["diag"])})
         worst = 0.0
         for key, b in blocks:
             den = np.maximum(np.abs(b), 1e-3 * scale)
@@ -251,7 +252,7 @@ def test_fortran_host_driver(oracle_lib, tmp_path):
     assert g["ec"] == o["ec"]
 
 
-@pytest.mark.parametrize("nslices", [5, 6])
+@pytest.mark.parametrize("nslices", [6, 7])
 def test_split_precision_wave(ctx, oracle_lib, nslices):
     """Exchange contraction as Ozaki slices on bf16 MFMA: every element of H
     within 1e-10 (floor 1e-3 of max|H|) of the FP64 oracle; S unaffected."""
@@ -270,3 +271,50 @@ def test_split_precision_wave(ctx, oracle_lib, nslices):
     assert e <= 1e-10
     assert rel_err(S.hcc, oS.hcc) < TOL
     assert rel_err(H.hcb, oH.hcb) < TOL
+
+
+@pytest.mark.parametrize("engine", [0, 1, 2, 3])
+def test_fp64_gemm_engines(ctx, oracle_lib, engine):
+    """Each FP64 engine of the exchange contraction (library GEMM and the
+    three MFMA kernel tilings) against the oracle."""
+    O = oracle_lib
+    from bsr_amd import synth
+    w = synth.make_wave(81, 8, 90, 18.0, 40.0, nch=7, ncp=5, mk=4, nb=5, ntarg=4, fill=0.15, name="eng")
+    try:
+        ctx.set_gemm_engine(engine)
+        ctx.wave_assemble(w)
+        assert ctx.wave_engine() == engine
+        H = ctx.wave_get(1)
+    finally:
+        ctx.set_gemm_engine(-1)
+    oS, oH = O.run_wave(w)
+    assert rel_err(H.hcc, oH.hcc) < TOL
+
+
+def test_batch_graph(oracle_lib):
+    """Uneven waves captured into one hipGraph: results match the oracle and
+    two replays give identical bits."""
+    from tests.conftest import HAS_GPU
+    if not HAS_GPU:
+        pytest.skip("no HIP device")
+    O = oracle_lib
+    from bsr_amd import synth
+    from bsr_amd.batch import Batch
+    waves = [synth.make_wave(90 + j, 8, ns, 18.0, 40.0, nch=nch, ncp=4, mk=3, nb=5, ntarg=3, fill=0.1,
+                             name=f"b{j}") for j, (ns, nch) in enumerate([(60, 5), (90, 3), (75, 6)])]
+    b = Batch(waves)
+    try:
+        b.prepare()
+        b.capture()
+        per, tot = b.run()
+        assert len(per) == 3 and tot > 0
+        first = [c.wave_get(1).hcc.copy() for c in b.ctx]
+        b.run()
+        for c, h in zip(b.ctx, first):
+            assert np.array_equal(c.wave_get(1).hcc, h)
+        for c, w in zip(b.ctx, waves):
+            oS, oH = O.run_wave(w)
+            assert rel_err(c.wave_get(1).hcc, oH.hcc) < TOL
+            assert rel_err(c.wave_get(0).hcc, oS.hcc) < TOL
+    finally:
+        b.close()
