@@ -90,30 +90,70 @@ struct Matrices {
   }
 };
 
+// Page-locked host buffer: uploads from it are asynchronous.  The owner
+// makes sure no copy from the old buffer is pending before it grows.
+template <typename T>
+struct PinnedVec {
+  T* p = nullptr;
+  size_t cap = 0;
+  PinnedVec() = default;
+  PinnedVec(const PinnedVec&) = delete;
+  PinnedVec& operator=(const PinnedVec&) = delete;
+  ~PinnedVec() { if (p) (void)hipHostFree(p); }
+  hipError_t reserve(size_t m) {
+    if (m <= cap) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max(m, cap + cap / 2);
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), want * sizeof(T), hipHostMallocDefault);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+};
+
 // Term lists for one matrix set.
 struct Terms {
   struct Band { int blk, bid; double c; };
   struct R1 { int blk, u, w, sidx; double c; };
   struct Vec { int q, v, sidx; double c; };
   struct Scal { int kind, idx, sidx; double c; };  // kind 0 hbb, 1 htarg, 2 otarg
-  std::vector<double> gemm;  // nt x nblk, t = kk*nb*nb + a*nb + b
+  PinnedVec<double> gemm;    // nt x nblk, t = kk*nb*nb + a*nb + b (pinned: uploaded as is)
+  size_t gemm_n = 0;         // 0 while no exchange term was added
   int gemm_klo = 1 << 30, gemm_khi = -1;
   std::vector<Band> band;
   std::vector<R1> r1;
   std::vector<Vec> vec;
   std::vector<Scal> scal;
   std::vector<double> acf_add;  // same shape as acf
+  void clear() {
+    gemm_n = 0;
+    gemm_klo = 1 << 30;
+    gemm_khi = -1;
+    band.clear(); r1.clear(); vec.clear(); scal.clear(); acf_add.clear();
+  }
 };
 
-// CSR arrays of one term set on the host and the device buffers they go to.
-// The host arrays stay alive until the asynchronous uploads have completed.
+// CSR arrays of one term set, sorted by output (list order kept within an
+// output), in one pinned host arena that goes to the device in one copy.
 struct Staged {
-  std::vector<int> bblk, bptr, bid, rblk, rptr, ru, rw, rs, vq, vptr, vid, vs;
-  std::vector<double> bc, rc, vc;
-  DevVec<int> d[12];
-  DevVec<double> dd[3], gemm;
+  PinnedVec<char> h;
+  DevVec<char> d;
+  DevVec<double> gemm;
+  hipEvent_t done = nullptr;  // the last upload from h / the term set's gemm has completed
+  bool pending = false;
+  std::vector<int> cnt;       // counting-sort scratch
+  int nbblk = 0, nrblk = 0, nvq = 0;
+  size_t nband = 0, nr1 = 0, nvec = 0, bytes = 0;
+  // byte offsets of the arrays in the arena
+  size_t o_bc = 0, o_rc = 0, o_vc = 0, o_bblk = 0, o_bptr = 0, o_bid = 0, o_rblk = 0, o_rptr = 0, o_ru = 0,
+         o_rw = 0, o_rs = 0, o_vq = 0, o_vptr = 0, o_vid = 0, o_vs = 0;
   bool has_gemm = false;
   int gemm_klo = 0, gemm_khi = -1, engine = 2;
+  template <typename T>
+  T* hp(size_t o) { return reinterpret_cast<T*>(h.p + o); }
+  template <typename T>
+  const T* dp(size_t o) const { return reinterpret_cast<const T*>(d.p + o); }
 };
 
 }  // namespace bsrg
@@ -146,7 +186,7 @@ struct bsrg_ctx {
   int basis_nk = 0, exch_nk = 0;
   DevVec<double> bands;  // [sbu | D(kk,p)] each ns*ks
   DevVec<double> E, St, VT, dpart;
-  std::vector<double> St_host;
+  PinnedVec<double> St_host;           // scalar table, read back for the host scalars
   std::vector<std::vector<double>> Q;  // Q[k][(i-1)+nbf*(j-1)] = <p_i|r^k|p_j>, computed lazily
   std::vector<char> Qok;
   std::vector<int> wtab;                // GET_V registry (i1,i2) -> id, -1 if not yet made
@@ -161,6 +201,7 @@ struct bsrg_ctx {
   Staged stS, stH, stC;
   DevVec<char> ozk;  // split-precision workspace
   Terms TS, TH;      // term sets of the prepared wave (host scalars at finish)
+  Terms TC;          // term set of the per-call path
   std::map<std::tuple<int, int, int>, int> tuned;  // GEMM shape -> engine
   int last_engine = -1, forced_engine = -1;
   DevVec<double> dtmp, dtmp2;
@@ -228,10 +269,13 @@ int ensure_bases(bsrg_ctx* c, int nk, bool exchange) {
     HIPCHK(c->VT.resize((nV + c->wvec.size() / std::max(ns, 1) + 1) * ns));
     if (np > 0) launch_vtab(ns, ks, nk, np, nb, D, c->P.p, c->VT.p, c->stream);
     HIPCHK(hipGetLastError());
-    c->St_host.assign(static_cast<size_t>(nk) * np * np, 0.0);
+    const size_t nst = static_cast<size_t>(nk) * np * np;
+    if (nst > c->St_host.cap) {
+      HIPCHK(hipStreamSynchronize(c->stream));  // no copy into the old buffer is pending
+      HIPCHK(c->St_host.reserve(nst));
+    }
     if (np > 0)
-      HIPCHK(hipMemcpyAsync(c->St_host.data(), c->St.p, c->St_host.size() * sizeof(double), hipMemcpyDeviceToHost,
-                            c->stream));
+      HIPCHK(hipMemcpyAsync(c->St_host.p, c->St.p, nst * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     c->basis_nk = nk;
     c->wvec_uploaded = -1;
   }
@@ -401,9 +445,13 @@ int add_idata(bsrg_ctx* c, const Matrices& M, Terms& T, int jtype, int k, int n,
         CA += cd[i] * quadr(c, o1, o2, k);
       } else {
         const size_t nt = static_cast<size_t>(c->basis_nk) * nb * nb;
-        if (T.gemm.empty()) T.gemm.assign(nt * M.nblk(), 0.0);
+        if (T.gemm_n == 0) {
+          T.gemm_n = nt * M.nblk();
+          if (T.gemm.reserve(T.gemm_n) != hipSuccess) return fail(-100, "HIP error: hipHostMalloc (exchange coefficients)");
+          std::memset(T.gemm.p, 0, T.gemm_n * sizeof(double));
+        }
         const int t = k * nb * nb + (ich >= jch ? a * nb + b : b * nb + a);
-        T.gemm[t + nt * blk_index(ich, jch)] += cc;
+        T.gemm.p[t + nt * blk_index(ich, jch)] += cc;
         T.gemm_klo = std::min(T.gemm_klo, k);
         T.gemm_khi = std::max(T.gemm_khi, k);
       }
@@ -421,37 +469,89 @@ int add_idata(bsrg_ctx* c, const Matrices& M, Terms& T, int jtype, int k, int n,
   return 0;
 }
 
-template <typename T, typename Key>
-void stable_sort_by(std::vector<T>& v, Key key) {
-  std::stable_sort(v.begin(), v.end(), [&](const T& a, const T& b) { return key(a) < key(b); });
+// Before a term set or its arena is rewritten: the last upload from them
+// has completed.
+int wait_staged(Staged& st) {
+  if (st.pending) {
+    HIPCHK(hipEventSynchronize(st.done));
+    st.pending = false;
+  }
+  return 0;
 }
 
-// Sort the terms by output (stable: list order within an output) and build
-// the CSR arrays.
-void stage_terms(Terms& T, Staged& st) {
-  stable_sort_by(T.band, [](const Terms::Band& b) { return b.blk; });
-  stable_sort_by(T.r1, [](const Terms::R1& r) { return r.blk; });
-  stable_sort_by(T.vec, [](const Terms::Vec& v) { return v.q; });
-  for (auto* v : {&st.bblk, &st.bptr, &st.bid, &st.rblk, &st.rptr, &st.ru, &st.rw, &st.rs, &st.vq, &st.vptr,
-                  &st.vid, &st.vs})
-    v->clear();
-  st.bc.clear(); st.rc.clear(); st.vc.clear();
-  for (size_t q = 0; q < T.band.size(); ++q) {
-    if (q == 0 || T.band[q].blk != T.band[q - 1].blk) { st.bblk.push_back(T.band[q].blk); st.bptr.push_back(static_cast<int>(q)); }
-    st.bid.push_back(T.band[q].bid);
-    st.bc.push_back(T.band[q].c);
+// Stable counting sort of n terms by key in [0, nkeys): position of each
+// term in the sorted order (list order kept within a key), and the distinct
+// keys with their CSR row pointers.
+template <typename KeyAt>
+void counting_sort(size_t n, int nkeys, KeyAt key, std::vector<int>& cnt, int* pos, int* keys, int* ptr,
+                   int& nrow) {
+  cnt.assign(static_cast<size_t>(nkeys) + 1, 0);
+  for (size_t q = 0; q < n; ++q) ++cnt[key(q) + 1];
+  nrow = 0;
+  for (int k = 0; k < nkeys; ++k) {
+    if (cnt[k + 1]) {
+      keys[nrow] = k;
+      ptr[nrow] = cnt[k];
+      ++nrow;
+    }
+    cnt[k + 1] += cnt[k];
   }
-  st.bptr.push_back(static_cast<int>(T.band.size()));
-  for (size_t q = 0; q < T.r1.size(); ++q) {
-    if (q == 0 || T.r1[q].blk != T.r1[q - 1].blk) { st.rblk.push_back(T.r1[q].blk); st.rptr.push_back(static_cast<int>(q)); }
-    st.ru.push_back(T.r1[q].u); st.rw.push_back(T.r1[q].w); st.rs.push_back(T.r1[q].sidx); st.rc.push_back(T.r1[q].c);
+  ptr[nrow] = static_cast<int>(n);
+  for (size_t q = 0; q < n; ++q) pos[q] = cnt[key(q)]++;
+}
+
+// Sort the terms by output and lay the CSR arrays out in the pinned arena:
+// doubles (coefficients) first, then ints.
+int stage_terms(Terms& T, Staged& st, const Matrices& M) {
+  if (int rc = wait_staged(st)) return rc;
+  st.nband = T.band.size();
+  st.nr1 = T.r1.size();
+  st.nvec = T.vec.size();
+  const int nblk = static_cast<int>(M.nblk()), nq = M.nch * M.ncp;
+  const size_t nbr = std::min<size_t>(st.nband, nblk), nrr = std::min<size_t>(st.nr1, nblk),
+               nvr = std::min<size_t>(st.nvec, std::max(nq, 0));
+  size_t o = 0;
+  auto put = [&o](size_t count, size_t elem) { const size_t at = o; o += ((count * elem + 255) / 256) * 256; return at; };
+  st.o_bc = put(st.nband, 8); st.o_rc = put(st.nr1, 8); st.o_vc = put(st.nvec, 8);
+  st.o_bblk = put(nbr, 4); st.o_bptr = put(nbr + 1, 4); st.o_bid = put(st.nband, 4);
+  st.o_rblk = put(nrr, 4); st.o_rptr = put(nrr + 1, 4);
+  st.o_ru = put(st.nr1, 4); st.o_rw = put(st.nr1, 4); st.o_rs = put(st.nr1, 4);
+  st.o_vq = put(nvr, 4); st.o_vptr = put(nvr + 1, 4); st.o_vid = put(st.nvec, 4); st.o_vs = put(st.nvec, 4);
+  const size_t npos = std::max({st.nband, st.nr1, st.nvec, size_t(1)});
+  const size_t o_pos = put(npos, 4);
+  st.bytes = o_pos;  // the position scratch is not uploaded
+  if (o > st.h.cap) HIPCHK(st.h.reserve(o));
+  int* pos = st.hp<int>(o_pos);
+
+  counting_sort(st.nband, nblk, [&](size_t q) { return T.band[q].blk; }, st.cnt, pos, st.hp<int>(st.o_bblk),
+                st.hp<int>(st.o_bptr), st.nbblk);
+  {
+    double* bc = st.hp<double>(st.o_bc);
+    int* bid = st.hp<int>(st.o_bid);
+    for (size_t q = 0; q < st.nband; ++q) { bc[pos[q]] = T.band[q].c; bid[pos[q]] = T.band[q].bid; }
   }
-  st.rptr.push_back(static_cast<int>(T.r1.size()));
-  for (size_t q = 0; q < T.vec.size(); ++q) {
-    if (q == 0 || T.vec[q].q != T.vec[q - 1].q) { st.vq.push_back(T.vec[q].q); st.vptr.push_back(static_cast<int>(q)); }
-    st.vid.push_back(T.vec[q].v); st.vs.push_back(T.vec[q].sidx); st.vc.push_back(T.vec[q].c);
+  counting_sort(st.nr1, nblk, [&](size_t q) { return T.r1[q].blk; }, st.cnt, pos, st.hp<int>(st.o_rblk),
+                st.hp<int>(st.o_rptr), st.nrblk);
+  {
+    double* rc = st.hp<double>(st.o_rc);
+    int *ru = st.hp<int>(st.o_ru), *rw = st.hp<int>(st.o_rw), *rs = st.hp<int>(st.o_rs);
+    for (size_t q = 0; q < st.nr1; ++q) {
+      const Terms::R1& r = T.r1[q];
+      const int p = pos[q];
+      rc[p] = r.c; ru[p] = r.u; rw[p] = r.w; rs[p] = r.sidx;
+    }
   }
-  st.vptr.push_back(static_cast<int>(T.vec.size()));
+  counting_sort(st.nvec, std::max(nq, 1), [&](size_t q) { return T.vec[q].q; }, st.cnt, pos, st.hp<int>(st.o_vq),
+                st.hp<int>(st.o_vptr), st.nvq);
+  {
+    double* vc = st.hp<double>(st.o_vc);
+    int *vid = st.hp<int>(st.o_vid), *vs = st.hp<int>(st.o_vs);
+    for (size_t q = 0; q < st.nvec; ++q) {
+      const int p = pos[q];
+      vc[p] = T.vec[q].c; vid[p] = T.vec[q].v; vs[p] = T.vec[q].sidx;
+    }
+  }
+  return 0;
 }
 
 // GET_V vectors registered so far -> the W part of the vector table.
@@ -474,13 +574,15 @@ int upload_wvec(bsrg_ctx* c) {
   return 0;
 }
 
-// Upload a staged term set (and the GET_V vectors) for a later enqueue.
+// Upload a staged term set (and the GET_V vectors) for a later enqueue:
+// one copy of the arena, one of the exchange coefficients.
 int upload_terms(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
   hipStream_t s = c->stream;
   if (int rc = upload_wvec(c)) return rc;
-  st.has_gemm = !T.gemm.empty() && T.gemm_khi >= 0;
+  st.has_gemm = T.gemm_n > 0 && T.gemm_khi >= 0;
   if (st.has_gemm) {
-    HIPCHK(st.gemm.upload(T.gemm.data(), T.gemm.size(), s));
+    HIPCHK(st.gemm.resize(T.gemm_n));
+    HIPCHK(hipMemcpyAsync(st.gemm.p, T.gemm.p, T.gemm_n * sizeof(double), hipMemcpyHostToDevice, s));
     st.gemm_klo = T.gemm_klo;
     st.gemm_khi = T.gemm_khi;
     if (c->precision_mode == 1) {
@@ -491,27 +593,12 @@ int upload_terms(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
                                                  c->nslices)));
     }
   }
-  if (!st.bblk.empty()) {
-    HIPCHK(st.d[0].upload(st.bblk.data(), st.bblk.size(), s));
-    HIPCHK(st.d[1].upload(st.bptr.data(), st.bptr.size(), s));
-    HIPCHK(st.d[2].upload(st.bid.data(), st.bid.size(), s));
-    HIPCHK(st.dd[0].upload(st.bc.data(), st.bc.size(), s));
+  if (st.bytes) {
+    HIPCHK(st.d.resize(st.bytes));
+    HIPCHK(hipMemcpyAsync(st.d.p, st.h.p, st.bytes, hipMemcpyHostToDevice, s));
   }
-  if (!st.rblk.empty()) {
-    HIPCHK(st.d[3].upload(st.rblk.data(), st.rblk.size(), s));
-    HIPCHK(st.d[4].upload(st.rptr.data(), st.rptr.size(), s));
-    HIPCHK(st.d[5].upload(st.ru.data(), st.ru.size(), s));
-    HIPCHK(st.d[6].upload(st.rw.data(), st.rw.size(), s));
-    HIPCHK(st.d[7].upload(st.rs.data(), st.rs.size(), s));
-    HIPCHK(st.dd[1].upload(st.rc.data(), st.rc.size(), s));
-  }
-  if (!st.vq.empty()) {
-    HIPCHK(st.d[8].upload(st.vq.data(), st.vq.size(), s));
-    HIPCHK(st.d[9].upload(st.vptr.data(), st.vptr.size(), s));
-    HIPCHK(st.d[10].upload(st.vid.data(), st.vid.size(), s));
-    HIPCHK(st.d[11].upload(st.vs.data(), st.vs.size(), s));
-    HIPCHK(st.dd[2].upload(st.vc.data(), st.vc.size(), s));
-  }
+  HIPCHK(hipEventRecord(st.done, s));
+  st.pending = true;
   return 0;
 }
 
@@ -581,15 +668,15 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, hi
   }
   if (e1) (void)hipEventRecord(e1, s);
   if (overwrite) launch_fill(M.hcb_n(), 0.0, M.hcb.p, s);
-  if (!st.bblk.empty())
-    launch_band_terms(ns, ks, static_cast<int>(st.bblk.size()), st.d[0].p, st.d[1].p, st.d[2].p, st.dd[0].p,
-                      c->bands.p, M.hcc.p, s);
-  if (!st.rblk.empty())
-    launch_rank1_terms(ns, static_cast<int>(st.rblk.size()), st.d[3].p, st.d[4].p, st.d[5].p, st.d[6].p, st.d[7].p,
-                       st.dd[1].p, c->St.p, c->VT.p, M.hcc.p, s);
-  if (!st.vq.empty())
-    launch_vec_terms(ns, static_cast<int>(st.vq.size()), st.d[8].p, st.d[9].p, st.d[10].p, st.d[11].p, st.dd[2].p,
-                     c->St.p, c->VT.p, M.hcb.p, s);
+  if (st.nbblk)
+    launch_band_terms(ns, ks, st.nbblk, st.dp<int>(st.o_bblk), st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
+                      st.dp<double>(st.o_bc), c->bands.p, M.hcc.p, s);
+  if (st.nrblk)
+    launch_rank1_terms(ns, st.nrblk, st.dp<int>(st.o_rblk), st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru),
+                       st.dp<int>(st.o_rw), st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p, s);
+  if (st.nvq)
+    launch_vec_terms(ns, st.nvq, st.dp<int>(st.o_vq), st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid),
+                     st.dp<int>(st.o_vs), st.dp<double>(st.o_vc), c->St.p, c->VT.p, M.hcb.p, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -598,7 +685,7 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, hi
 void finish_scalars(bsrg_ctx* c, Matrices& M, const Terms& T, bool overwrite) {
   if (overwrite) std::fill(M.hbb.begin(), M.hbb.end(), 0.0);
   for (const auto& sc : T.scal) {
-    const double f = sc.c * (sc.sidx >= 0 ? c->St_host[sc.sidx] : 1.0);
+    const double f = sc.c * (sc.sidx >= 0 ? c->St_host.p[sc.sidx] : 1.0);
     if (sc.kind == 0) M.hbb[sc.idx] += f;
     else if (sc.kind == 1) M.htarg[sc.idx] += f;
     else M.otarg[sc.idx] += f;
@@ -609,7 +696,7 @@ void finish_scalars(bsrg_ctx* c, Matrices& M, const Terms& T, bool overwrite) {
 
 // Per-call path: stage, launch, wait, scalars.
 int apply_terms(bsrg_ctx* c, Matrices& M, Terms& T, bool overwrite) {
-  stage_terms(T, c->stC);
+  if (int rc = stage_terms(T, c->stC, M)) return rc;
   if (int rc = upload_terms(c, M, T, c->stC)) return rc;
   c->stC.engine = c->forced_engine >= 0 ? c->forced_engine : 2;
   if (overwrite)
@@ -704,6 +791,7 @@ int bsrg_create(bsrg_ctx** out, int device) {
   if (rocblas_create_handle(&c->blas) != rocblas_status_success) return fail(-101, "rocblas_create_handle failed");
   rocblas_set_stream(c->blas, c->stream);
   for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+  for (Staged* st : {&c->stS, &c->stH, &c->stC}) HIPCHK(hipEventCreateWithFlags(&st->done, hipEventDisableTiming));
   *out = c.release();
   return 0;
 }
@@ -714,6 +802,7 @@ int bsrg_destroy(bsrg_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   if (c->blas) rocblas_destroy_handle(c->blas);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  for (Staged* st : {&c->stS, &c->stH, &c->stC}) if (st->done) (void)hipEventDestroy(st->done);
   hipStream_t s = c->stream;
   delete c;
   if (s) (void)hipStreamDestroy(s);
@@ -878,7 +967,9 @@ int bsrg_update_hl(bsrg_ctx* c, int ich, int jch, double v) {
     HIPCHK(c->bands.resize(static_cast<size_t>(c->G.ns) * c->G.ks));
     launch_sb_upper(c->G.ns, c->G.ks, c->sb.p, c->bands.p, c->stream);
   }
-  Terms T;
+  Terms& T = c->TC;
+  if (int rc = wait_staged(c->stC)) return rc;
+  T.clear();
   band_hl(T, ich, jch, v);
   return apply_terms(c, c->cur, T, false);
 }
@@ -894,7 +985,9 @@ int bsrg_idata(bsrg_ctx* c, int icase, int jtype, int jpol, int n, const int* ip
   HIPCHK(hipSetDevice(c->device));
   const int need = std::max(jpol + 1, c->basis_nk);
   if (int rc = ensure_bases(c, need, jtype >= 8)) return rc;
-  Terms T;
+  Terms& T = c->TC;
+  if (int rc = wait_staged(c->stC)) return rc;
+  T.clear();
   if (int rc = add_idata(c, c->cur, T, jtype, jpol, n, ipt, k1, k2, k3, k4, cd, 1.0)) return rc;
   return apply_terms(c, c->cur, T, false);
 }
@@ -907,7 +1000,9 @@ int bsrg_odata(bsrg_ctx* c, int n, const int* ipt, const int* k3, const int* k4,
   HIPCHK(hipSetDevice(c->device));
   if (c->basis_nk == 0)
     if (int rc = ensure_bases(c, 1, false)) return rc;
-  Terms T;
+  Terms& T = c->TC;
+  if (int rc = wait_staged(c->stC)) return rc;
+  T.clear();
   if (int rc = add_odata(c, c->cur, T, n, ipt, k3, k4, cd, eps_c, 1.0, true)) return rc;
   return apply_terms(c, c->cur, T, false);
 }
@@ -979,8 +1074,10 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   // list processing on the host overlaps the band and basis kernels
   Terms& TS = c->TS;
   Terms& TH = c->TH;
-  TS = Terms();
-  TH = Terms();
+  if (int rc = wait_staged(c->stS)) return rc;
+  if (int rc = wait_staged(c->stH)) return rc;
+  TS.clear();
+  TH.clear();
   // S: sb on the channel diagonals + overlaps
   for (int ich = 1; ich <= nch; ++ich) band_hl(TS, ich, ich, 1.0);
   if (int rc = add_odata(c, c->S, TS, novl, nullptr, ok3, ok4, ocd, eps_c, 1.0, true)) return rc;
@@ -997,8 +1094,8 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   }
   for (int ich = 1; ich <= nch; ++ich) band_hl(TH, ich, ich, etarg[iptar[ich - 1] - 1] - ec);
   const auto h1 = clk::now();
-  stage_terms(TS, c->stS);
-  stage_terms(TH, c->stH);
+  if (int rc = stage_terms(TS, c->stS, c->S)) return rc;
+  if (int rc = stage_terms(TH, c->stH, c->H)) return rc;
   const auto h2 = clk::now();
   if (int rc = upload_terms(c, c->S, TS, c->stS)) return rc;
   if (int rc = upload_terms(c, c->H, TH, c->stH)) return rc;

@@ -165,7 +165,7 @@ def test_fused_wave_c1(ctx, oracle_lib, tmp_path):
     assert rel_err(g["acf"], r["acf"]) < TOL
     # eigenvalues of H c = E S c, lowest 10
     e_g = sl.eigh(H.dense(), S.dense(), eigvals_only=True, subset_by_index=[0, 9])
-    e_o = sl.eigh(oH.dense(), oS.dense(), eigvals_only=True, subset_by_index=[0, 9])
+    e_o = sl.eigh(O.dense_from_wave(oH), O.dense_from_wave(oS), eigvals_only=True, subset_by_index=[0, 9])
     print("[c1] eig", e_o[:3], np.abs(e_g - e_o).max(), flush=True)
     assert np.abs(e_g - e_o).max() <= 1e-9
 
