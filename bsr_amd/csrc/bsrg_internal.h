@@ -46,9 +46,10 @@ struct DevGrid {
 // s=0: r^k, s=1: r^-(k+1); diagonal cells rkd[((kk*nv+iv)*ik+jj)*ik+ii].
 void launch_moments(const DevGrid& g, int k0, int nk, double* mom, double* rkd, hipStream_t s);
 // R^k bands for the same multipoles: band kk at rkb + kk*ns*ns*ks*ks.
-// msum: scratch of nk*2*ks*ns doubles (per-pair sums of the moments)
-void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd, double* msum, double* rkb,
-                 hipStream_t s);
+// Scratch: msum nk*2*ks*ns doubles (per-pair sums of the moments), momT and
+// rdT cell-major copies of mom and rkd (same sizes).
+void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd, double* msum, double* momT,
+                 double* rdT, double* rkb, hipStream_t s);
 // upper-band densities dens[p](j,jp) of all unordered bound pairs p.
 void launch_pair_dens(int ns, int ks, int nb, const double* P, double* dens, hipStream_t s);
 // direct basis D[kk][p](i,ip) = sum d_p(j,jp) rkb_k(i,j,ip,jp);
@@ -66,17 +67,15 @@ void launch_vtab(int ns, int ks, int nk, int npair, int nb, const double* D, con
                  double* V, hipStream_t s);
 // sb (lower band) -> upper-band form sbu(i,ith) = <B_i|B_{i+ith}>
 void launch_sb_upper(int ns, int ks, const double* sb, double* sbu, hipStream_t s);
-// hcc[blk] += sum_t c_t * band_t (symmetric), blocks listed in blks
-void launch_band_terms(int ns, int ks, int nblk, const int* blks, const int* ptr, const int* bid,
-                       const double* bc, const double* bands, double* hcc, hipStream_t s);
-// hcc[blk](x,y) += sum_r c_r * sfac_r * V[u_r](x) * V[w_r](y)
-void launch_rank1_terms(int ns, int nblk, const int* blks, const int* ptr, const int* uid,
-                        const int* wid, const int* sidx, const double* c, const double* stab,
-                        const double* V, double* hcc, hipStream_t s);
-// hcb[q] += sum_r c_r * sfac_r * V[v_r]
-void launch_vec_terms(int ns, int nq, const int* qs, const int* ptr, const int* vid, const int* sidx,
-                      const double* c, const double* stab, const double* V, double* hcb,
-                      hipStream_t s);
+// Channel blocks in one pass: hcc[blk] = base + sum_t c_t * band_t (symmetric band)
+// + sum_r c_r * sfac_r * V[u_r] (x) V[w_r]; CSR term lists over all nblk blocks;
+// base = current hcc if has_base, else 0 (then every block is written)
+void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, const int* bid, const double* bc,
+                     const double* bands, const int* rptr, const int* ru, const int* rw, const int* rs,
+                     const double* rc, const double* stab, const double* V, double* hcc, hipStream_t s);
+// hcb[q] = base + sum_r c_r * sfac_r * V[v_r]; CSR over all nq positions
+void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,
+                         const double* c, const double* stab, const double* V, double* hcb, hipStream_t s);
 void launch_symmetrize_diag(int ns, int nch, double* hcc, hipStream_t s);
 void launch_scale(size_t n, double a, double* x, hipStream_t s);
 void launch_fill(size_t n, double v, double* x, hipStream_t s);

@@ -23,6 +23,7 @@
 #include <tuple>
 #include <chrono>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/bsr_gpu.h"
@@ -126,16 +127,27 @@ struct Terms {
   std::vector<Vec> vec;
   std::vector<Scal> scal;
   std::vector<double> acf_add;  // same shape as acf
+  // A worker part of a parallel pass lists its exchange coefficients and
+  // ACF additions (index, value) instead of applying them; they are applied
+  // afterwards in part order, i.e. in list order.
+  bool deferred = false;
+  std::vector<std::pair<size_t, double>> gx, ax;
+  int err = 0;
+  std::string msg;
   void clear() {
     gemm_n = 0;
     gemm_klo = 1 << 30;
     gemm_khi = -1;
     band.clear(); r1.clear(); vec.clear(); scal.clear(); acf_add.clear();
+    gx.clear(); ax.clear();
+    err = 0;
+    msg.clear();
   }
 };
 
 // CSR arrays of one term set, sorted by output (list order kept within an
-// output), in one pinned host arena that goes to the device in one copy.
+// output), row pointers over every output, in one pinned host arena that
+// goes to the device in one copy.
 struct Staged {
   PinnedVec<char> h;
   DevVec<char> d;
@@ -143,11 +155,11 @@ struct Staged {
   hipEvent_t done = nullptr;  // the last upload from h / the term set's gemm has completed
   bool pending = false;
   std::vector<int> cnt;       // counting-sort scratch
-  int nbblk = 0, nrblk = 0, nvq = 0;
+  int nblk = 0, nq = 0;
   size_t nband = 0, nr1 = 0, nvec = 0, bytes = 0;
   // byte offsets of the arrays in the arena
-  size_t o_bc = 0, o_rc = 0, o_vc = 0, o_bblk = 0, o_bptr = 0, o_bid = 0, o_rblk = 0, o_rptr = 0, o_ru = 0,
-         o_rw = 0, o_rs = 0, o_vq = 0, o_vptr = 0, o_vid = 0, o_vs = 0;
+  size_t o_bc = 0, o_rc = 0, o_vc = 0, o_bptr = 0, o_bid = 0, o_rptr = 0, o_ru = 0, o_rw = 0, o_rs = 0, o_vptr = 0,
+         o_vid = 0, o_vs = 0;
   bool has_gemm = false;
   int gemm_klo = 0, gemm_khi = -1, engine = 2;
   template <typename T>
@@ -173,7 +185,7 @@ struct bsrg_ctx {
   bool grid_ok = false;
   DevVec<double> t, gx, gw, gr, grm, grw, bsp, sb;
   int band_nk = 0;  // bands built for k = 0..band_nk-1
-  DevVec<double> rkb, mom, rkd, msum;
+  DevVec<double> rkb, mom, rkd, msum, momT, rdT;
 
   // orbitals
   bool orb_ok = false;
@@ -202,6 +214,10 @@ struct bsrg_ctx {
   DevVec<char> ozk;  // split-precision workspace
   Terms TS, TH;      // term sets of the prepared wave (host scalars at finish)
   Terms TC;          // term set of the per-call path
+  Terms THt;         // H terms after the lists: (E_t - ec) * sb on the diagonal
+  std::vector<std::unique_ptr<Terms>> work;  // worker parts of the parallel list pass
+  int nwork = 0;                             // worker parts used by the last prepare
+  std::vector<Terms*> partsS, partsH;        // term parts of S and H in list order
   std::map<std::tuple<int, int, int>, int> tuned;  // GEMM shape -> engine
   int last_engine = -1, forced_engine = -1;
   DevVec<double> dtmp, dtmp2;
@@ -233,7 +249,9 @@ int ensure_bands(bsrg_ctx* c, int nk) {
   DevGrid g = devgrid(c);
   launch_moments(g, 0, nk, c->mom.p, c->rkd.p, c->stream);
   HIPCHK(c->msum.resize(static_cast<size_t>(nk) * 2 * ks * ns));
-  launch_band(g, nk, c->mom.p, c->rkd.p, c->msum.p, c->rkb.p, c->stream);
+  HIPCHK(c->momT.resize(c->mom.n));
+  HIPCHK(c->rdT.resize(c->rkd.n));
+  launch_band(g, nk, c->mom.p, c->rkd.p, c->msum.p, c->momT.p, c->rdT.p, c->rkb.p, c->stream);
   HIPCHK(hipGetLastError());
   c->band_nk = nk;
   c->basis_nk = 0;  // bases depend on the bands
@@ -292,12 +310,13 @@ int ensure_bases(bsrg_ctx* c, int nk, bool exchange) {
 // GET_V (src/bsr/BSR_MAT3/get_V.f90:2-27) on the host: qbs(:,jj) minus the
 // projections on same-l bound orbitals that the channel orbital ii must be
 // orthogonal to.  Returns the vector id in the W part of the vector table.
-int get_v(bsrg_ctx* c, int ii, int jj, int& id) {
+int get_v(bsrg_ctx* c, int ii, int jj, int& id, bool create = true) {
   const int nbf = c->nbf, ns = c->G.ns;
   if (ii < 1 || ii > nbf || jj < 1 || jj > nbf) return fail(-18, "get_V: orbital index out of range");
   const size_t key = static_cast<size_t>(ii - 1) + static_cast<size_t>(nbf) * (jj - 1);
   if (c->wtab[key] >= 0) { id = c->wtab[key]; return 0; }
   if (c->ibort[(ii - 1) + static_cast<size_t>(nbf) * (jj - 1)] == 0) return fail(-19, "get_V: <kl|nl> = 0 ?");
+  if (!create) return fail(-19, "get_V: vector not registered before the parallel pass");
   std::vector<double> v(c->qbs.begin() + static_cast<size_t>(ns) * (jj - 1),
                         c->qbs.begin() + static_cast<size_t>(ns) * jj);
   for (int i = 1; i <= nbf; ++i) {
@@ -355,7 +374,7 @@ int add_struct1(bsrg_ctx* c, const Matrices& M, Terms& T, int K3, int K4, int si
     T.scal.push_back({0, i * (i - 1) / 2 + j - 1, sidx, cc});
   } else if (ic < 0 && jc > 0) {
     const int io = -ic, i1 = io / kIbo, i2 = io % kIbo;
-    int w, rc = get_v(c, i1, i2, w);
+    int w, rc = get_v(c, i1, i2, w, !T.deferred);
     if (rc) return rc;
     const int ich = c->iech[i1 - 1];
     if (ich < 1 || ich > M.nch) return fail(-13, "UPDATE_HV: channel index out of range");
@@ -363,8 +382,8 @@ int add_struct1(bsrg_ctx* c, const Matrices& M, Terms& T, int K3, int K4, int si
     T.vec.push_back({(jc - 1) * M.nch + (ich - 1), nV + w, sidx, cc});
   } else if (ic < 0 && jc < 0) {
     const int io = -ic, i1 = io / kIbo, i2 = io % kIbo, jo = -jc, j1 = jo / kIbo, j2 = jo % kIbo;
-    int v, w, rc = get_v(c, i1, i2, v);
-    if (!rc) rc = get_v(c, j1, j2, w);
+    int v, w, rc = get_v(c, i1, i2, v, !T.deferred);
+    if (!rc) rc = get_v(c, j1, j2, w, !T.deferred);
     if (rc) return rc;
     const int ich = c->iech[i1 - 1], jch = c->iech[j1 - 1];
     if ((rc = chk_ch(M, ich, jch, "UPDATE_HW"))) return rc;
@@ -412,7 +431,7 @@ int add_idata(bsrg_ctx* c, const Matrices& M, Terms& T, int jtype, int k, int n,
       if (io > 0) {
         const int j1 = io / kIbo, j2 = io % kIbo;
         int w;
-        if ((rc = get_v(c, j1, j2, w))) return rc;
+        if ((rc = get_v(c, j1, j2, w, !T.deferred))) return rc;
         const int jch = c->iech[j1 - 1];
         if ((rc = chk_ch(M, ich, jch, "UPDATE_HW"))) return rc;
         const int nVv = c->basis_nk * np * nb;
@@ -445,22 +464,31 @@ int add_idata(bsrg_ctx* c, const Matrices& M, Terms& T, int jtype, int k, int n,
         CA += cd[i] * quadr(c, o1, o2, k);
       } else {
         const size_t nt = static_cast<size_t>(c->basis_nk) * nb * nb;
+        const int t = k * nb * nb + (ich >= jch ? a * nb + b : b * nb + a);
+        T.gemm_klo = std::min(T.gemm_klo, k);
+        T.gemm_khi = std::max(T.gemm_khi, k);
+        if (T.deferred) {
+          T.gx.push_back({t + nt * blk_index(ich, jch), cc});
+        } else {
         if (T.gemm_n == 0) {
           T.gemm_n = nt * M.nblk();
           if (T.gemm.reserve(T.gemm_n) != hipSuccess) return fail(-100, "HIP error: hipHostMalloc (exchange coefficients)");
           std::memset(T.gemm.p, 0, T.gemm_n * sizeof(double));
         }
-        const int t = k * nb * nb + (ich >= jch ? a * nb + b : b * nb + a);
         T.gemm.p[t + nt * blk_index(ich, jch)] += cc;
-        T.gemm_klo = std::min(T.gemm_klo, k);
-        T.gemm_khi = std::max(T.gemm_khi, k);
+        }
       }
       const bool last = (q == n - 1) || (K1[ipt ? ipt[q + 1] - 1 : q + 1] != K1[i]);
       if (last) {
         if (jcase <= 2) {
           if (k > M.mk) return fail(-17, "UPDATE_CF: multipole index out of range");
-          if (T.acf_add.empty()) T.acf_add.assign(M.acf.size(), 0.0);
-          T.acf_add[static_cast<size_t>(k) * M.nch * M.nch + (ich - 1) + static_cast<size_t>(M.nch) * (jch - 1)] += CA;
+          const size_t at = static_cast<size_t>(k) * M.nch * M.nch + (ich - 1) + static_cast<size_t>(M.nch) * (jch - 1);
+          if (T.deferred) {
+            T.ax.push_back({at, CA});
+          } else {
+            if (T.acf_add.empty()) T.acf_add.assign(M.acf.size(), 0.0);
+            T.acf_add[at] += CA;
+          }
         }
         CA = 0.0;
       }
@@ -479,77 +507,72 @@ int wait_staged(Staged& st) {
   return 0;
 }
 
-// Stable counting sort of n terms by key in [0, nkeys): position of each
-// term in the sorted order (list order kept within a key), and the distinct
-// keys with their CSR row pointers.
-template <typename KeyAt>
-void counting_sort(size_t n, int nkeys, KeyAt key, std::vector<int>& cnt, int* pos, int* keys, int* ptr,
-                   int& nrow) {
+// Stable counting sort of the terms of all parts (in part order) by key in
+// [0, nkeys): position of each term in the sorted order (list order kept
+// within a key) and the row pointers ptr[0..nkeys].
+template <typename Vec, typename Key>
+void counting_sort(const std::vector<Terms*>& parts, Vec vec, int nkeys, Key key, std::vector<int>& cnt, int* pos,
+                   int* ptr) {
   cnt.assign(static_cast<size_t>(nkeys) + 1, 0);
-  for (size_t q = 0; q < n; ++q) ++cnt[key(q) + 1];
-  nrow = 0;
-  for (int k = 0; k < nkeys; ++k) {
-    if (cnt[k + 1]) {
-      keys[nrow] = k;
-      ptr[nrow] = cnt[k];
-      ++nrow;
-    }
-    cnt[k + 1] += cnt[k];
-  }
-  ptr[nrow] = static_cast<int>(n);
-  for (size_t q = 0; q < n; ++q) pos[q] = cnt[key(q)]++;
+  for (const Terms* P : parts)
+    for (const auto& x : (P->*vec)) ++cnt[key(x) + 1];
+  for (int k = 0; k < nkeys; ++k) cnt[k + 1] += cnt[k];
+  std::memcpy(ptr, cnt.data(), (static_cast<size_t>(nkeys) + 1) * sizeof(int));
+  size_t q = 0;
+  for (const Terms* P : parts)
+    for (const auto& x : (P->*vec)) pos[q++] = cnt[key(x)]++;
 }
 
-// Sort the terms by output and lay the CSR arrays out in the pinned arena:
-// doubles (coefficients) first, then ints.
-int stage_terms(Terms& T, Staged& st, const Matrices& M) {
+// Sort the terms of the parts by output and lay the CSR arrays out in the
+// pinned arena: doubles (coefficients) first, then ints.
+int stage_terms(const std::vector<Terms*>& parts, Staged& st, const Matrices& M) {
   if (int rc = wait_staged(st)) return rc;
-  st.nband = T.band.size();
-  st.nr1 = T.r1.size();
-  st.nvec = T.vec.size();
-  const int nblk = static_cast<int>(M.nblk()), nq = M.nch * M.ncp;
-  const size_t nbr = std::min<size_t>(st.nband, nblk), nrr = std::min<size_t>(st.nr1, nblk),
-               nvr = std::min<size_t>(st.nvec, std::max(nq, 0));
+  st.nband = st.nr1 = st.nvec = 0;
+  for (const Terms* P : parts) {
+    st.nband += P->band.size();
+    st.nr1 += P->r1.size();
+    st.nvec += P->vec.size();
+  }
+  st.nblk = static_cast<int>(M.nblk());
+  st.nq = M.nch * M.ncp;
   size_t o = 0;
   auto put = [&o](size_t count, size_t elem) { const size_t at = o; o += ((count * elem + 255) / 256) * 256; return at; };
   st.o_bc = put(st.nband, 8); st.o_rc = put(st.nr1, 8); st.o_vc = put(st.nvec, 8);
-  st.o_bblk = put(nbr, 4); st.o_bptr = put(nbr + 1, 4); st.o_bid = put(st.nband, 4);
-  st.o_rblk = put(nrr, 4); st.o_rptr = put(nrr + 1, 4);
-  st.o_ru = put(st.nr1, 4); st.o_rw = put(st.nr1, 4); st.o_rs = put(st.nr1, 4);
-  st.o_vq = put(nvr, 4); st.o_vptr = put(nvr + 1, 4); st.o_vid = put(st.nvec, 4); st.o_vs = put(st.nvec, 4);
+  st.o_bptr = put(st.nblk + 1, 4); st.o_bid = put(st.nband, 4);
+  st.o_rptr = put(st.nblk + 1, 4); st.o_ru = put(st.nr1, 4); st.o_rw = put(st.nr1, 4); st.o_rs = put(st.nr1, 4);
+  st.o_vptr = put(st.nq + 1, 4); st.o_vid = put(st.nvec, 4); st.o_vs = put(st.nvec, 4);
   const size_t npos = std::max({st.nband, st.nr1, st.nvec, size_t(1)});
   const size_t o_pos = put(npos, 4);
   st.bytes = o_pos;  // the position scratch is not uploaded
   if (o > st.h.cap) HIPCHK(st.h.reserve(o));
   int* pos = st.hp<int>(o_pos);
 
-  counting_sort(st.nband, nblk, [&](size_t q) { return T.band[q].blk; }, st.cnt, pos, st.hp<int>(st.o_bblk),
-                st.hp<int>(st.o_bptr), st.nbblk);
+  counting_sort(parts, &Terms::band, st.nblk, [](const Terms::Band& x) { return x.blk; }, st.cnt, pos,
+                st.hp<int>(st.o_bptr));
   {
     double* bc = st.hp<double>(st.o_bc);
     int* bid = st.hp<int>(st.o_bid);
-    for (size_t q = 0; q < st.nband; ++q) { bc[pos[q]] = T.band[q].c; bid[pos[q]] = T.band[q].bid; }
+    size_t q = 0;
+    for (const Terms* P : parts)
+      for (const auto& x : P->band) { const int p = pos[q++]; bc[p] = x.c; bid[p] = x.bid; }
   }
-  counting_sort(st.nr1, nblk, [&](size_t q) { return T.r1[q].blk; }, st.cnt, pos, st.hp<int>(st.o_rblk),
-                st.hp<int>(st.o_rptr), st.nrblk);
+  counting_sort(parts, &Terms::r1, st.nblk, [](const Terms::R1& x) { return x.blk; }, st.cnt, pos,
+                st.hp<int>(st.o_rptr));
   {
     double* rc = st.hp<double>(st.o_rc);
     int *ru = st.hp<int>(st.o_ru), *rw = st.hp<int>(st.o_rw), *rs = st.hp<int>(st.o_rs);
-    for (size_t q = 0; q < st.nr1; ++q) {
-      const Terms::R1& r = T.r1[q];
-      const int p = pos[q];
-      rc[p] = r.c; ru[p] = r.u; rw[p] = r.w; rs[p] = r.sidx;
-    }
+    size_t q = 0;
+    for (const Terms* P : parts)
+      for (const auto& x : P->r1) { const int p = pos[q++]; rc[p] = x.c; ru[p] = x.u; rw[p] = x.w; rs[p] = x.sidx; }
   }
-  counting_sort(st.nvec, std::max(nq, 1), [&](size_t q) { return T.vec[q].q; }, st.cnt, pos, st.hp<int>(st.o_vq),
-                st.hp<int>(st.o_vptr), st.nvq);
+  counting_sort(parts, &Terms::vec, st.nq, [](const Terms::Vec& x) { return x.q; }, st.cnt, pos,
+                st.hp<int>(st.o_vptr));
   {
     double* vc = st.hp<double>(st.o_vc);
     int *vid = st.hp<int>(st.o_vid), *vs = st.hp<int>(st.o_vs);
-    for (size_t q = 0; q < st.nvec; ++q) {
-      const int p = pos[q];
-      vc[p] = T.vec[q].c; vid[p] = T.vec[q].v; vs[p] = T.vec[q].sidx;
-    }
+    size_t q = 0;
+    for (const Terms* P : parts)
+      for (const auto& x : P->vec) { const int p = pos[q++]; vc[p] = x.c; vid[p] = x.v; vs[p] = x.sidx; }
   }
   return 0;
 }
@@ -663,47 +686,143 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, hi
   if (st.has_gemm) {
     if (int rc = run_gemm(c, M, st, overwrite, st.engine)) return rc;
     c->last_engine = c->precision_mode == 1 ? 10 + c->nslices : st.engine;
-  } else if (overwrite) {
-    launch_fill(M.hcc_n(), 0.0, M.hcc.p, s);
   }
   if (e1) (void)hipEventRecord(e1, s);
-  if (overwrite) launch_fill(M.hcb_n(), 0.0, M.hcb.p, s);
-  if (st.nbblk)
-    launch_band_terms(ns, ks, st.nbblk, st.dp<int>(st.o_bblk), st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
-                      st.dp<double>(st.o_bc), c->bands.p, M.hcc.p, s);
-  if (st.nrblk)
-    launch_rank1_terms(ns, st.nrblk, st.dp<int>(st.o_rblk), st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru),
-                       st.dp<int>(st.o_rw), st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p, s);
-  if (st.nvq)
-    launch_vec_terms(ns, st.nvq, st.dp<int>(st.o_vq), st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid),
-                     st.dp<int>(st.o_vs), st.dp<double>(st.o_vc), c->St.p, c->VT.p, M.hcb.p, s);
+  // the GEMM wrote (or added to) hcc: the term pass adds to it; without a
+  // GEMM an overwrite pass writes every block
+  const int base = (st.has_gemm || !overwrite) ? 1 : 0;
+  launch_assemble(ns, ks, st.nblk, base, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid), st.dp<double>(st.o_bc),
+                  c->bands.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw), st.dp<int>(st.o_rs),
+                  st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p, s);
+  launch_vec_assemble(ns, st.nq, overwrite ? 0 : 1, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid),
+                      st.dp<int>(st.o_vs), st.dp<double>(st.o_vc), c->St.p, c->VT.p, M.hcb.p, s);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-// Host scalars (HB, Target_h, ACF); needs St_host, i.e. a completed stream.
-void finish_scalars(bsrg_ctx* c, Matrices& M, const Terms& T, bool overwrite) {
+// Host scalars (HB, Target_h, ACF) of the parts in list order; needs
+// St_host, i.e. a completed stream.  The ACF additions are in parts[0].
+void finish_scalars(bsrg_ctx* c, Matrices& M, const std::vector<Terms*>& parts, bool overwrite) {
   if (overwrite) std::fill(M.hbb.begin(), M.hbb.end(), 0.0);
-  for (const auto& sc : T.scal) {
-    const double f = sc.c * (sc.sidx >= 0 ? c->St_host.p[sc.sidx] : 1.0);
-    if (sc.kind == 0) M.hbb[sc.idx] += f;
-    else if (sc.kind == 1) M.htarg[sc.idx] += f;
-    else M.otarg[sc.idx] += f;
-  }
+  for (const Terms* P : parts)
+    for (const auto& sc : P->scal) {
+      const double f = sc.c * (sc.sidx >= 0 ? c->St_host.p[sc.sidx] : 1.0);
+      if (sc.kind == 0) M.hbb[sc.idx] += f;
+      else if (sc.kind == 1) M.htarg[sc.idx] += f;
+      else M.otarg[sc.idx] += f;
+    }
+  const Terms& T = *parts[0];
   if (!T.acf_add.empty())
     for (size_t q = 0; q < M.acf.size(); ++q) M.acf[q] += T.acf_add[q];
 }
 
 // Per-call path: stage, launch, wait, scalars.
 int apply_terms(bsrg_ctx* c, Matrices& M, Terms& T, bool overwrite) {
-  if (int rc = stage_terms(T, c->stC, M)) return rc;
+  const std::vector<Terms*> one{&T};
+  if (int rc = stage_terms(one, c->stC, M)) return rc;
   if (int rc = upload_terms(c, M, T, c->stC)) return rc;
   c->stC.engine = c->forced_engine >= 0 ? c->forced_engine : 2;
   if (overwrite)
     if (int rc = autotune_gemm(c, M, c->stC, false)) return rc;
   if (int rc = enqueue_terms(c, M, c->stC, overwrite, nullptr, nullptr)) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
-  finish_scalars(c, M, T, overwrite);
+  finish_scalars(c, M, one, overwrite);
+  return 0;
+}
+
+// GET_V vectors of every record that needs one, registered in list order
+// before the parallel pass (which then only reads the registry).  Failures
+// are left for the pass itself to report at the right record.
+void register_vectors(bsrg_ctx* c, int nrec, const int* jtype, const int* K3, const int* K4) {
+  int id;
+  for (int r = 0; r < nrec; ++r) {
+    if (jtype[r] == 1) {
+      const int ic = -K3[r], jc = -K4[r];
+      if (ic < 0) (void)get_v(c, (-ic) / kIbo, (-ic) % kIbo, id);
+      if (ic < 0 && jc < 0) (void)get_v(c, (-jc) / kIbo, (-jc) % kIbo, id);
+    } else if (jtype[r] <= 5 && K4[r] > 0) {
+      (void)get_v(c, K4[r] / kIbo, K4[r] % kIbo, id);
+    }
+  }
+}
+
+// I_data over all lists of a wave on several host threads.  Records are cut
+// into contiguous ranges at list boundaries or where K1 changes (so the ACF
+// group sums of I_data stay whole); each thread decodes its range into its
+// own part, and the parts are kept in range order, which is list order.
+// Exchange coefficients and ACF additions are then applied in that order.
+int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, const int* k1, const int* k2,
+                   const int* k3, const int* k4, const double* cd, bool any_exch) {
+  Terms& TH = c->TH;
+  const Matrices& M = c->H;
+  // lazily built tables the workers read
+  for (int r = 0; r < nrec; ++r)
+    if (jtype[r] == 6 || jtype[r] == 7) {
+      for (int m = 0; m <= M.mk; ++m) (void)quadr(c, 1, 1, m);
+      break;
+    }
+  register_vectors(c, nrec, jtype, k3, k4);
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nth = static_cast<int>(std::min<long>({8L, static_cast<long>(hw), nrec / 16384L + 1}));
+  std::vector<int> cuts{0};
+  for (int t = 1; t < nth; ++t) {
+    int b = static_cast<int>(static_cast<long>(nrec) * t / nth);
+    if (b <= cuts.back()) continue;
+    while (b < nrec && jtype[b] == jtype[b - 1] && kpol[b] == kpol[b - 1] && k1[b] == k1[b - 1]) ++b;
+    if (b < nrec) cuts.push_back(b);
+  }
+  cuts.push_back(nrec);
+  const int np = static_cast<int>(cuts.size()) - 1;
+  while (static_cast<int>(c->work.size()) < np) c->work.emplace_back(new Terms());
+  c->nwork = np;
+  // the exchange coefficient matrix, zeroed by the workers in slices
+  const size_t nt = static_cast<size_t>(c->basis_nk) * c->nb * c->nb;
+  if (any_exch && c->nb > 0) {
+    TH.gemm_n = nt * M.nblk();
+    HIPCHK(TH.gemm.reserve(TH.gemm_n));
+  }
+  auto work = [&](int t) {
+    Terms& P = *c->work[t];
+    P.clear();
+    P.deferred = true;
+    if (TH.gemm_n) {
+      const size_t lo = TH.gemm_n * t / np, hi = TH.gemm_n * (t + 1) / np;
+      std::memset(TH.gemm.p + lo, 0, (hi - lo) * sizeof(double));
+    }
+    int r = cuts[t];
+    while (r < cuts[t + 1]) {
+      int e = r + 1;
+      while (e < cuts[t + 1] && jtype[e] == jtype[r] && kpol[e] == kpol[r]) ++e;
+      if (int rc = add_idata(c, M, P, jtype[r], kpol[r], e - r, nullptr, k1 + r, k2 + r, k3 + r, k4 + r, cd + r, 1.0)) {
+        P.err = rc;
+        P.msg = g_err;
+        return;
+      }
+      r = e;
+    }
+  };
+  if (np == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> th;
+    th.reserve(np - 1);
+    for (int t = 1; t < np; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+  }
+  for (int t = 0; t < np; ++t) {
+    const Terms& P = *c->work[t];
+    if (P.err) return fail(P.err, P.msg);
+  }
+  for (int t = 0; t < np; ++t) {
+    const Terms& P = *c->work[t];
+    for (const auto& g : P.gx) TH.gemm.p[g.first] += g.second;
+    if (!P.ax.empty() && TH.acf_add.empty()) TH.acf_add.assign(M.acf.size(), 0.0);
+    for (const auto& a : P.ax) TH.acf_add[a.first] += a.second;
+    TH.gemm_klo = std::min(TH.gemm_klo, P.gemm_klo);
+    TH.gemm_khi = std::max(TH.gemm_khi, P.gemm_khi);
+  }
+  if (TH.gemm_khi < 0) TH.gemm_n = 0;
   return 0;
 }
 
@@ -1074,28 +1193,27 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   // list processing on the host overlaps the band and basis kernels
   Terms& TS = c->TS;
   Terms& TH = c->TH;
+  Terms& THt = c->THt;
   if (int rc = wait_staged(c->stS)) return rc;
   if (int rc = wait_staged(c->stH)) return rc;
   TS.clear();
   TH.clear();
+  THt.clear();
   // S: sb on the channel diagonals + overlaps
   for (int ich = 1; ich <= nch; ++ich) band_hl(TS, ich, ich, 1.0);
   if (int rc = add_odata(c, c->S, TS, novl, nullptr, ok3, ok4, ocd, eps_c, 1.0, true)) return rc;
   // H: ec * S + R^k lists + (E_t - ec) * sb
   for (int ich = 1; ich <= nch; ++ich) band_hl(TH, ich, ich, ec);
   if (int rc = add_odata(c, c->H, TH, novl, nullptr, ok3, ok4, ocd, eps_c, ec, false)) return rc;
-  int r = 0;
-  while (r < nrec) {
-    int e = r + 1;
-    while (e < nrec && jtype[e] == jtype[r] && kpol[e] == kpol[r]) ++e;
-    if (int rc = add_idata(c, c->H, TH, jtype[r], kpol[r], e - r, nullptr, k1 + r, k2 + r, k3 + r, k4 + r, cd + r, 1.0))
-      return rc;
-    r = e;
-  }
-  for (int ich = 1; ich <= nch; ++ich) band_hl(TH, ich, ich, etarg[iptar[ich - 1] - 1] - ec);
+  if (int rc = idata_parallel(c, nrec, jtype, kpol, k1, k2, k3, k4, cd, any_exch)) return rc;
+  for (int ich = 1; ich <= nch; ++ich) band_hl(THt, ich, ich, etarg[iptar[ich - 1] - 1] - ec);
+  c->partsS.assign(1, &TS);
+  c->partsH.assign(1, &TH);
+  for (int t = 0; t < c->nwork; ++t) c->partsH.push_back(c->work[t].get());
+  c->partsH.push_back(&THt);
   const auto h1 = clk::now();
-  if (int rc = stage_terms(TS, c->stS, c->S)) return rc;
-  if (int rc = stage_terms(TH, c->stH, c->H)) return rc;
+  if (int rc = stage_terms(c->partsS, c->stS, c->S)) return rc;
+  if (int rc = stage_terms(c->partsH, c->stH, c->H)) return rc;
   const auto h2 = clk::now();
   if (int rc = upload_terms(c, c->S, TS, c->stS)) return rc;
   if (int rc = upload_terms(c, c->H, TH, c->stH)) return rc;
@@ -1132,8 +1250,8 @@ int bsrg_wave_finish(bsrg_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   float ms = 0, ms2 = 0;
   if (!c->wave_ok) {
-    finish_scalars(c, c->S, c->TS, true);
-    finish_scalars(c, c->H, c->TH, true);
+    finish_scalars(c, c->S, c->partsS, true);
+    finish_scalars(c, c->H, c->partsH, true);
     acf_finish(c->H, 0);
     // the reference keeps one bsr_matrix module: H is recorded after the
     // overlap pass, so it carries the otarg accumulated there

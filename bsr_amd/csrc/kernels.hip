@@ -5,6 +5,8 @@
 //                                                       pair (j,j+jp) on r2
 //   upper band d(i,ith) at i + ns*ith                   pair (i,i+ith)
 //   hcc block b (x,y)  at b*ns*ns + x + ns*y
+#include <algorithm>
+
 #include "bsrg_internal.h"
 
 namespace bsrg {
@@ -178,54 +180,88 @@ __global__ void k_pair_sums(int ns, int ks, int nv, const double* __restrict__ m
   msum[(static_cast<size_t>(kk * 2 + s) * ks + ip) * ns + i] = acc;
 }
 
-// One thread per element, i fastest (coalesced writes).  When every cell of
-// the r1 pair lies below every cell of the r2 pair (or above), the cell sum
-// factorises: rkb = (sum of r^k moments) * (sum of r^-(k+1) moments).
-// Otherwise the cells are summed in the reference's order: cell jv
-// ascending, then cell iv.
-__global__ void k_band(int ns, int ks, int nv, int nk, const double* __restrict__ mom,
-                       const double* __restrict__ rkd, const double* __restrict__ msum,
-                       double* __restrict__ rkb) {
-  // grid (ceil(ns/64), ceil(ns/4), ks*ks*nk), block 256 = 64 i x 4 j
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (i >= ns || j >= ns) return;
-  const int kk = blockIdx.z / (ks * ks), r = blockIdx.z % (ks * ks);
-  const int ip = r % ks, jp = r / ks;
-  const size_t per = static_cast<size_t>(ns) * ns * ks * ks;
-  const size_t tid = kk * per + i + static_cast<size_t>(ns) * (j + static_cast<size_t>(ns) * r);
-  (void)nk;
-  double acc = 0.0;
+// The band splits into two element sets.  For a pair (i,i+ip) on r1 and
+// (j,j+jp) on r2, the cells of the two pairs are disjoint and ordered unless
+// d = j - i lies in [ip-ks+1, ks-1-jp] ("near"); the clamping of the cell
+// ranges at 0 and nv-1 does not change this for elements inside the band.
+// Far elements factorise: rkb = (sum of r^k moments) * (sum of r^-(k+1)
+// moments); near elements sum the cell pairs in the reference's order, cell
+// jv ascending, then cell iv.
+
+// Far elements and the zero padding outside the band: one thread per element
+// over the flattened (i,j) plane, fully coalesced writes.  Near elements are
+// written too (with the product value) and overwritten by k_band_near.
+// grid (ceil(ns*ns/256), ks*ks, nk), block 256
+__global__ void k_band_far(int ns, int ks, const double* __restrict__ msum, double* __restrict__ rkb) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  if (gid >= ns * ns) return;
+  const int i = gid % ns, j = gid / ns;
+  const int r = blockIdx.y, ip = r % ks, jp = r / ks, kk = blockIdx.z;
+  double v = 0.0;
   if (i + ip < ns && j + jp < ns) {
-    const int ik = ks * (ks + 1) / 2;
-    const double* m1 = mom + static_cast<size_t>(kk * 2) * nv * ik;
-    const double* m2 = m1 + static_cast<size_t>(nv) * ik;
-    const double* rd = rkd + static_cast<size_t>(kk) * nv * ik * ik;
-    const int ivlo = max(0, i + ip - (ks - 1)), ivhi = min(i, nv - 1);
-    const int jvlo = max(0, j + jp - (ks - 1)), jvhi = min(j, nv - 1);
     const double* s1 = msum + static_cast<size_t>(kk * 2) * ks * ns;
     const double* s2 = s1 + static_cast<size_t>(ks) * ns;
-    if (ivhi < jvlo) {
-      rkb[tid] = s1[ip * ns + i] * s2[jp * ns + j];
-      return;
-    }
-    if (ivlo > jvhi) {
-      rkb[tid] = s1[jp * ns + j] * s2[ip * ns + i];
-      return;
-    }
-    for (int jv = jvlo; jv <= jvhi; ++jv) {
-      const int jj = pack2(j - jv, j + jp - jv, ks);
-      for (int iv = ivlo; iv <= ivhi; ++iv) {
-        const int ii = pack2(i - iv, i + ip - iv, ks);
-        double c;
-        if (iv < jv) c = m1[iv * ik + ii] * m2[jv * ik + jj];
-        else if (iv > jv) c = m1[jv * ik + jj] * m2[iv * ik + ii];
-        else c = rd[(static_cast<size_t>(iv) * ik + jj) * ik + ii];
-        acc += c;
-      }
+    v = j > i ? s1[ip * ns + i] * s2[jp * ns + j] : s1[jp * ns + j] * s2[ip * ns + i];
+  }
+  rkb[(static_cast<size_t>(kk) * ks * ks + r) * ns * ns + gid] = v;
+}
+
+// Cell-major copies of the moments and diagonal-cell integrals for the near
+// elements: in[(b*nv + iv)*w + e] -> out[(b*w + e)*nv + iv].
+// grid (ceil(w/32), ceil(nv/32), nb), block 256 (32 x 8)
+__global__ void k_cells_transpose(int nv, int w, const double* __restrict__ in, double* __restrict__ out) {
+  __shared__ double tile[32][33];
+  const int e0 = blockIdx.x * 32, v0 = blockIdx.y * 32, b = blockIdx.z;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const double* src = in + static_cast<size_t>(b) * nv * w;
+  double* dst = out + static_cast<size_t>(b) * nv * w;
+  for (int r = ty; r < 32; r += 8) {
+    const int iv = v0 + r, e = e0 + tx;
+    if (iv < nv && e < w) tile[r][tx] = src[static_cast<size_t>(iv) * w + e];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int e = e0 + r, iv = v0 + tx;
+    if (iv < nv && e < w) dst[static_cast<size_t>(e) * nv + iv] = tile[tx][r];
+  }
+}
+
+// Near elements: one thread per i for a fixed (ip, jp, d = j - i).  The cells
+// are visited as iv = i - o, jv = j - q; for fixed (o, q) the lanes read
+// consecutive cells of the cell-major copies, and whether iv < jv depends on
+// (o, q, d) only, so the wavefront does not diverge.  Order as the reference:
+// cell jv ascending, then cell iv ascending.
+// grid (ceil(ns/64), 2*ks-1, ks*ks*nk), block 64; blockIdx.y = d + ks - 1
+__global__ void k_band_near(int ns, int ks, int nv, const double* __restrict__ momT, const double* __restrict__ rdT,
+                            double* __restrict__ rkb) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int d = static_cast<int>(blockIdx.y) - (ks - 1);
+  const int kk = blockIdx.z / (ks * ks), r = blockIdx.z % (ks * ks);
+  const int ip = r % ks, jp = r / ks;
+  const int j = i + d;
+  if (d < ip - ks + 1 || d > ks - 1 - jp) return;
+  if (i >= ns || j < 0 || i + ip >= ns || j + jp >= ns) return;
+  const int ik = ks * (ks + 1) / 2;
+  const double* m1 = momT + static_cast<size_t>(kk * 2) * ik * nv;  // [ii][iv]
+  const double* m2 = m1 + static_cast<size_t>(ik) * nv;
+  const double* rd = rdT + static_cast<size_t>(kk) * ik * ik * nv;   // [jj][ii][iv]
+  const int olo = max(0, i - (nv - 1)), ohi = min(ks - 1 - ip, i);
+  const int qlo = max(0, j - (nv - 1)), qhi = min(ks - 1 - jp, j);
+  double acc = 0.0;
+  for (int q = ks - 1 - jp; q >= 0; --q) {
+    if (q < qlo || q > qhi) continue;
+    const int jv = j - q, jj = pack2(q, q + jp, ks);
+    for (int o = ks - 1 - ip; o >= 0; --o) {
+      if (o < olo || o > ohi) continue;
+      const int iv = i - o, ii = pack2(o, o + ip, ks);
+      double c;
+      if (q - o < d) c = m1[static_cast<size_t>(ii) * nv + iv] * m2[static_cast<size_t>(jj) * nv + jv];
+      else if (q - o > d) c = m1[static_cast<size_t>(jj) * nv + jv] * m2[static_cast<size_t>(ii) * nv + iv];
+      else c = rd[(static_cast<size_t>(jj) * ik + ii) * nv + iv];
+      acc += c;
     }
   }
-  rkb[tid] = acc;
+  rkb[(static_cast<size_t>(kk) * ks * ks + r) * ns * ns + i + static_cast<size_t>(ns) * j] = acc;
 }
 
 // ---- a8: pair densities ('s' upper band) ----------------------------------
@@ -403,87 +439,164 @@ __global__ void k_sb_upper(int ns, int ks, const double* sb, double* sbu) {
 }
 
 // ---- accumulation of term lists into the channel blocks ------------------
-// band terms: positions |x-y|<ks; threads walk columns, 2ks-1 rows each.
-__global__ void k_band_terms(int ns, int ks, const int* blks, const int* ptr, const int* bid,
-                             const double* bc, const double* bands, double* hcc) {
-  const int e = blockIdx.x, blk = blks[e];
-  const int t0 = ptr[e], t1 = ptr[e + 1];
-  const int wid = 2 * ks - 1;
+// One pass over a channel block: out = base + band terms + rank-1 terms,
+// each sum in list order.  base is what hcc holds (the exchange GEMM result,
+// or earlier contributions) when has_base, else 0 and every block is
+// written.  Term lists are CSR over all blocks (ptr has nblk+1 entries).
+// grid (nblk, ceil(ns/32), ceil(ns/256)), block 256: lane -> rows
+// x0+lane+64a (a<4), wave -> 8 columns yt+8*wave+b.
+// The band terms of the block are summed once per workgroup into the band
+// entries (i, ith) its columns touch, staged in LDS; rank-1 terms go 16 at a
+// time through LDS (u over the rows, f*w over the columns).
+constexpr int kAsmCols = 32;
+__global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, const int* __restrict__ bptr,
+                                                  const int* __restrict__ bid, const double* __restrict__ bc,
+                                                  const double* __restrict__ bands, const int* __restrict__ rptr,
+                                                  const int* __restrict__ ru, const int* __restrict__ rw,
+                                                  const int* __restrict__ rs, const double* __restrict__ rc,
+                                                  const double* __restrict__ stab, const double* __restrict__ V,
+                                                  double* __restrict__ hcc) {
+  const int blk = blockIdx.x;
+  const int b0 = bptr[blk], b1 = bptr[blk + 1], r0 = rptr[blk], r1 = rptr[blk + 1];
+  if (has_base && b0 == b1 && r0 == r1) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int x0 = blockIdx.z * 256, yt = blockIdx.y * kAsmCols, y0 = yt + wv * 8;
+  const int nrow = min(256, ns - x0);
   double* h = hcc + static_cast<size_t>(blk) * ns * ns;
-  for (int q = threadIdx.x; q < ns * wid; q += blockDim.x) {
-    const int y = q / wid, x = y - (ks - 1) + q % wid;
-    if (x < 0 || x >= ns) continue;
-    const int i = min(x, y), ith = x > y ? x - y : y - x;
-    if (i + ith >= ns) continue;
-    double acc = 0.0;
-    for (int t = t0; t < t1; ++t) acc += bc[t] * bands[static_cast<size_t>(bid[t]) * ns * ks + i + ns * ith];
-    h[x + static_cast<size_t>(ns) * y] += acc;
-  }
-}
-
-// rank-1 terms: workgroup per (block, 32-column tile); 16 terms per LDS pass
-__global__ void k_rank1_terms(int ns, const int* blks, const int* ptr, const int* uid,
-                              const int* wid, const int* sidx, const double* c, const double* stab,
-                              const double* V, double* hcc) {
-  const int e = blockIdx.x, blk = blks[e], y0 = blockIdx.y * 32;
-  const int t0 = ptr[e], t1 = ptr[e + 1];
-  __shared__ double cw[16][32];
-  __shared__ int uu[16];
-  double* h = hcc + static_cast<size_t>(blk) * ns * ns;
-  const int ncol = min(32, ns - y0);
-  for (int xb = 0; xb < ns; xb += blockDim.x) {
-    const int x = xb + threadIdx.x;
-    double acc[32];
-#pragma unroll
-    for (int q = 0; q < 32; ++q) acc[q] = 0.0;
-    for (int tb = t0; tb < t1; tb += 16) {
-      const int nt = min(16, t1 - tb);
+  __shared__ double su[16][256];
+  __shared__ double sw[16][kAsmCols];
+  __shared__ double sband[(kAsmCols + kMaxKs - 1) * kMaxKs];
+  __shared__ int tbid[64];
+  __shared__ double tbc[64];
+  // band entries (i, ith) for i in [yt-ks+1, yt+kAsmCols): sband[(i-ilo)*ks + ith]
+  const int ilo = yt - (ks - 1), nent = (kAsmCols + ks - 1) * ks;
+  const bool has_band = b1 > b0;
+  if (has_band) {
+    double e[3] = {0.0, 0.0, 0.0};  // nent <= 47*16 = 752 <= 3*256
+    for (int tb = b0; tb < b1; tb += 64) {
+      const int nt = min(64, b1 - tb);
       __syncthreads();
-      for (int q = threadIdx.x; q < nt * 32; q += blockDim.x) {
-        const int r = q / 32, col = q % 32, t = tb + r;
-        const double f = c[t] * (sidx[t] >= 0 ? stab[sidx[t]] : 1.0);
-        cw[r][col] = col < ncol ? f * V[static_cast<size_t>(wid[t]) * ns + y0 + col] : 0.0;
-        if (col == 0) uu[r] = uid[t];
+      if (threadIdx.x < nt) {
+        tbid[threadIdx.x] = bid[tb + threadIdx.x];
+        tbc[threadIdx.x] = bc[tb + threadIdx.x];
       }
       __syncthreads();
-      if (x < ns) {
-        for (int r = 0; r < nt; ++r) {
-          const double u = V[static_cast<size_t>(uu[r]) * ns + x];
 #pragma unroll
-          for (int q = 0; q < 32; ++q) acc[q] += u * cw[r][q];
-        }
+      for (int m = 0; m < 3; ++m) {
+        const int q = threadIdx.x + 256 * m;
+        if (q >= nent) break;
+        const int i = ilo + q / ks, ith = q % ks;
+        if (i < 0 || i + ith >= ns) continue;
+        double s = e[m];
+        for (int t = 0; t < nt; ++t) s += tbc[t] * bands[static_cast<size_t>(tbid[t]) * ns * ks + i + ns * ith];
+        e[m] = s;
       }
     }
-    if (x < ns)
-      for (int q = 0; q < ncol; ++q) h[x + static_cast<size_t>(ns) * (y0 + q)] += acc[q];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int q = threadIdx.x + 256 * m;
+      if (q < nent) sband[q] = e[m];
+    }
+  }
+  double acc[4][8];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
+  for (int tb = r0; tb < r1; tb += 16) {
+    const int nt = min(16, r1 - tb);
+    __syncthreads();
+    for (int q = threadIdx.x; q < nt * nrow; q += 256) {
+      const int r = q / nrow, x = q % nrow;
+      su[r][x] = V[static_cast<size_t>(ru[tb + r]) * ns + x0 + x];
+    }
+    for (int q = threadIdx.x; q < nt * kAsmCols; q += 256) {
+      const int r = q / kAsmCols, col = q % kAsmCols, t = tb + r, y = yt + col;
+      const double f = rc[t] * (rs[t] >= 0 ? stab[rs[t]] : 1.0);
+      sw[r][col] = y < ns ? f * V[static_cast<size_t>(rw[t]) * ns + y] : 0.0;
+    }
+    __syncthreads();
+    for (int r = 0; r < nt; ++r) {
+      double u[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) u[a] = su[r][(lane + 64 * a) & 255];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const double w = sw[r][wv * 8 + b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[a][b] += u[a] * w;
+      }
+    }
+  }
+  __syncthreads();  // sband complete
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const int y = y0 + b;
+    if (y >= ns) continue;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int x = x0 + lane + 64 * a;
+      if (x >= ns) continue;
+      double* p = h + x + static_cast<size_t>(ns) * y;
+      double v = has_base ? *p : 0.0;
+      if (has_band) {
+        const int ith = x > y ? x - y : y - x, i = min(x, y);
+        if (ith < ks && i + ith < ns) v += sband[(i - ilo) * ks + ith];
+      }
+      v += acc[a][b];
+      *p = v;
+    }
   }
 }
 
-__global__ void k_vec_terms(int ns, const int* qs, const int* ptr, const int* vid, const int* sidx,
-                            const double* c, const double* stab, const double* V, double* hcb) {
-  const int e = blockIdx.x, q = qs[e];
-  const int t0 = ptr[e], t1 = ptr[e + 1];
+// Channel-perturber vectors: hcb(:, q) = base + sum of c * f * V(:, id), list
+// order; CSR over all nq = nch*ncp positions.  grid (nq), block 64*ceil(ns/64)<=256
+__global__ void k_vec_assemble(int ns, int has_base, const int* __restrict__ ptr, const int* __restrict__ vid,
+                               const int* __restrict__ sidx, const double* __restrict__ c,
+                               const double* __restrict__ stab, const double* __restrict__ V,
+                               double* __restrict__ hcb) {
+  const int q = blockIdx.x;
+  const int t0 = ptr[q], t1 = ptr[q + 1];
+  if (has_base && t0 == t1) return;
   for (int x = threadIdx.x; x < ns; x += blockDim.x) {
     double acc = 0.0;
     for (int t = t0; t < t1; ++t) {
       const double f = c[t] * (sidx[t] >= 0 ? stab[sidx[t]] : 1.0);
       acc += f * V[static_cast<size_t>(vid[t]) * ns + x];
     }
-    hcb[static_cast<size_t>(q) * ns + x] += acc;
+    double* p = hcb + static_cast<size_t>(q) * ns + x;
+    *p = has_base ? *p + acc : acc;
   }
 }
 
+// Diagonal blocks -> (C + C^T)/2 through 32x32 LDS tiles: the workgroup of
+// tile pair (ti >= tj) reads both tiles, writes both.
+// grid (nt*(nt+1)/2, nch), block 256 (32 x 8)
 __global__ void k_symmetrize_diag(int ns, int nch, double* hcc) {
   const int ich = blockIdx.y;  // 0-based
   const size_t blk = static_cast<size_t>(ich + 1) * (ich + 2) / 2 - 1;
   double* h = hcc + blk * ns * ns;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ns * ns) return;
-  const int i = q % ns, j = q / ns;
-  if (j > i) return;
-  const double s = h[i + static_cast<size_t>(ns) * j] + h[j + static_cast<size_t>(ns) * i];
-  h[i + static_cast<size_t>(ns) * j] = s / 2.0;
-  h[j + static_cast<size_t>(ns) * i] = s / 2.0;
+  int p = blockIdx.x, ti = 0;
+  while (p > ti) { p -= ti + 1; ++ti; }
+  const int tj = p;  // tj <= ti
+  __shared__ double A[32][33], B[32][33];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int xa = ti * 32 + tx, ya = tj * 32 + r;  // tile (ti,tj): rows ti, cols tj
+    const int xb = tj * 32 + tx, yb = ti * 32 + r;  // tile (tj,ti)
+    A[r][tx] = (xa < ns && ya < ns) ? h[xa + static_cast<size_t>(ns) * ya] : 0.0;
+    B[r][tx] = (xb < ns && yb < ns) ? h[xb + static_cast<size_t>(ns) * yb] : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    // element (x = ti*32+tx, y = tj*32+r) pairs with (y, x) = B[tx][r]
+    const double s = (A[r][tx] + B[tx][r]) / 2.0;
+    const double sb = (B[r][tx] + A[tx][r]) / 2.0;
+    const int xa = ti * 32 + tx, ya = tj * 32 + r;
+    const int xb = tj * 32 + tx, yb = ti * 32 + r;
+    if (xa < ns && ya < ns) h[xa + static_cast<size_t>(ns) * ya] = s;
+    if (ti != tj && xb < ns && yb < ns) h[xb + static_cast<size_t>(ns) * yb] = sb;
+  }
 }
 
 __global__ void k_scale(size_t n, double a, double* x) {
@@ -550,12 +663,19 @@ void launch_moments(const DevGrid& g, int k0, int nk, double* mom, double* rkd, 
   }
 }
 
-void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd, double* msum, double* rkb,
-                 hipStream_t s) {
+void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd, double* msum, double* momT,
+                 double* rdT, double* rkb, hipStream_t s) {
+  const int ik = g.ks * (g.ks + 1) / 2;
   hipLaunchKernelGGL(k_pair_sums, dim3(nblocks(g.ns, 256), g.ks, nk * 2), dim3(256), 0, s, g.ns, g.ks, g.nv, mom,
                      msum);
-  hipLaunchKernelGGL(k_band, dim3((g.ns + 63) / 64, (g.ns + 3) / 4, g.ks * g.ks * nk), dim3(256), 0, s, g.ns, g.ks,
-                     g.nv, nk, mom, rkd, msum, rkb);
+  hipLaunchKernelGGL(k_cells_transpose, dim3((ik + 31) / 32, (g.nv + 31) / 32, nk * 2), dim3(256), 0, s, g.nv, ik,
+                     mom, momT);
+  hipLaunchKernelGGL(k_cells_transpose, dim3((ik * ik + 31) / 32, (g.nv + 31) / 32, nk), dim3(256), 0, s, g.nv,
+                     ik * ik, rkd, rdT);
+  hipLaunchKernelGGL(k_band_far, dim3(nblocks(static_cast<size_t>(g.ns) * g.ns, 256), g.ks * g.ks, nk), dim3(256), 0, s,
+                     g.ns, g.ks, msum, rkb);
+  hipLaunchKernelGGL(k_band_near, dim3((g.ns + 63) / 64, 2 * g.ks - 1, g.ks * g.ks * nk), dim3(64), 0, s, g.ns, g.ks,
+                     g.nv, momT, rdT, rkb);
 }
 
 void launch_pair_dens(int ns, int ks, int nb, const double* P, double* dens, hipStream_t s) {
@@ -597,30 +717,26 @@ void launch_sb_upper(int ns, int ks, const double* sb, double* sbu, hipStream_t 
   hipLaunchKernelGGL(k_sb_upper, dim3(nblocks(static_cast<size_t>(ns) * ks, 256)), dim3(256), 0, s, ns, ks, sb, sbu);
 }
 
-void launch_band_terms(int ns, int ks, int nblk, const int* blks, const int* ptr, const int* bid, const double* bc,
-                       const double* bands, double* hcc, hipStream_t s) {
+void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, const int* bid, const double* bc,
+                     const double* bands, const int* rptr, const int* ru, const int* rw, const int* rs,
+                     const double* rc, const double* stab, const double* V, double* hcc, hipStream_t s) {
   if (nblk <= 0) return;
-  hipLaunchKernelGGL(k_band_terms, dim3(nblk), dim3(256), 0, s, ns, ks, blks, ptr, bid, bc, bands, hcc);
+  hipLaunchKernelGGL(k_assemble, dim3(nblk, (ns + kAsmCols - 1) / kAsmCols, (ns + 255) / 256), dim3(256), 0, s, ns, ks,
+                     has_base,
+                     bptr, bid, bc, bands, rptr, ru, rw, rs, rc, stab, V, hcc);
 }
 
-void launch_rank1_terms(int ns, int nblk, const int* blks, const int* ptr, const int* uid, const int* wid,
-                        const int* sidx, const double* c, const double* stab, const double* V, double* hcc,
-                        hipStream_t s) {
-  if (nblk <= 0) return;
-  hipLaunchKernelGGL(k_rank1_terms, dim3(nblk, (ns + 31) / 32), dim3(256), 0, s, ns, blks, ptr, uid, wid, sidx, c,
-                     stab, V, hcc);
-}
-
-void launch_vec_terms(int ns, int nq, const int* qs, const int* ptr, const int* vid, const int* sidx, const double* c,
-                      const double* stab, const double* V, double* hcb, hipStream_t s) {
+void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,
+                         const double* c, const double* stab, const double* V, double* hcb, hipStream_t s) {
   if (nq <= 0) return;
-  hipLaunchKernelGGL(k_vec_terms, dim3(nq), dim3(256), 0, s, ns, qs, ptr, vid, sidx, c, stab, V, hcb);
+  const int bs = std::min(256, 64 * ((ns + 63) / 64));
+  hipLaunchKernelGGL(k_vec_assemble, dim3(nq), dim3(bs), 0, s, ns, has_base, ptr, vid, sidx, c, stab, V, hcb);
 }
 
 void launch_symmetrize_diag(int ns, int nch, double* hcc, hipStream_t s) {
   if (nch <= 0) return;
-  hipLaunchKernelGGL(k_symmetrize_diag, dim3(nblocks(static_cast<size_t>(ns) * ns, 256), nch), dim3(256), 0, s, ns,
-                     nch, hcc);
+  const int nt = (ns + 31) / 32;
+  hipLaunchKernelGGL(k_symmetrize_diag, dim3(nt * (nt + 1) / 2, nch), dim3(256), 0, s, ns, nch, hcc);
 }
 
 void launch_scale(size_t n, double a, double* x, hipStream_t s) {
