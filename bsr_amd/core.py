@@ -174,9 +174,10 @@ class Context:
         return dict(bands=t[0], bases=t[1], gemm=t[2], terms=t[3], total=t[4])
 
     def wave_host_timings(self):
-        t = np.zeros(4, order="F")
+        t = np.zeros(10, order="F")
         L.check(L.lib().bsrg_wave_host_timings(self._p, t))
-        return dict(terms=t[0], sort=t[1], upload=t[2], total=t[3])
+        return dict(launch=t[0], overlaps=t[1], registry=t[2], decode=t[3], merge=t[4], sort=t[5],
+                    upload=t[6], total=t[7], decode_cpu=t[8], decode_longest=t[9])
 
     def drop_caches(self):
         """Forget resident bands and bases (the next wave rebuilds them)."""

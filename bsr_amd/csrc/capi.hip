@@ -23,6 +23,10 @@
 #include <tuple>
 #include <chrono>
 #include <string>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -154,7 +158,7 @@ struct Staged {
   DevVec<double> gemm;
   hipEvent_t done = nullptr;  // the last upload from h / the term set's gemm has completed
   bool pending = false;
-  std::vector<int> cnt;       // counting-sort scratch
+  std::vector<int> cnt[3];    // counting-sort scratch (band, rank-1, vector sets)
   int nblk = 0, nq = 0;
   size_t nband = 0, nr1 = 0, nvec = 0, bytes = 0;
   // byte offsets of the arrays in the arena
@@ -168,6 +172,68 @@ struct Staged {
   const T* dp(size_t o) const { return reinterpret_cast<const T*>(d.p + o); }
 };
 
+// Fixed set of host worker threads owned by a context: run(n, f) calls
+// f(0..n-1), each index once, on the workers and the calling thread (indices
+// are taken from a shared counter), and returns when all are done.
+class Pool {
+ public:
+  explicit Pool(int nthreads) {
+    for (int t = 1; t < nthreads; ++t) th_.emplace_back([this] { loop(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& x : th_) x.join();
+  }
+  int size() const { return static_cast<int>(th_.size()) + 1; }
+  void run(int n, const std::function<void(int)>& f) {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      f_ = &f;
+      n_ = n;
+      next_.store(0);
+      busy_ = static_cast<int>(th_.size());
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (int i; (i = next_.fetch_add(1)) < n;) f(i);
+    std::unique_lock<std::mutex> lk(m_);
+    done_.wait(lk, [this] { return busy_ == 0; });
+    f_ = nullptr;
+  }
+
+ private:
+  void loop() {
+    unsigned seen = 0;
+    for (;;) {
+      const std::function<void(int)>* f;
+      int n;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        f = f_;
+        n = n_;
+      }
+      for (int i; (i = next_.fetch_add(1)) < n;) (*f)(i);
+      std::lock_guard<std::mutex> g(m_);
+      if (--busy_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* f_ = nullptr;
+  std::atomic<int> next_{0};
+  int n_ = 0, busy_ = 0;
+  unsigned gen_ = 0;
+  bool stop_ = false;
+};
+
 }  // namespace bsrg
 
 using namespace bsrg;
@@ -178,7 +244,12 @@ struct bsrg_ctx {
   rocblas_handle blas = nullptr;
   hipEvent_t ev[8] = {};
   double timings[5] = {0, 0, 0, 0, 0};
-  double host_ms[4] = {0, 0, 0, 0};  // prepare: lists->terms, sort, uploads, total
+  // host ms of the last prepare: [0] checks and band/basis launches, [1] S and
+  // the overlap terms of H, [2] registry of GET_V vectors, [3] parallel list
+  // decoding, [4] merge of the parts, [5] sort and staging, [6] uploads and
+  // GEMM autotune, [7] total, [8] decoding time summed over the parts (cpu
+  // ms), [9] the longest part
+  double host_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   int precision_mode = 0, nslices = 0;
 
   GridTables G;
@@ -217,6 +288,7 @@ struct bsrg_ctx {
   Terms THt;         // H terms after the lists: (E_t - ec) * sb on the diagonal
   std::vector<std::unique_ptr<Terms>> work;  // worker parts of the parallel list pass
   int nwork = 0;                             // worker parts used by the last prepare
+  std::unique_ptr<Pool> pool;                // host workers of the list pass
   std::vector<Terms*> partsS, partsH;        // term parts of S and H in list order
   std::map<std::tuple<int, int, int>, int> tuned;  // GEMM shape -> engine
   int last_engine = -1, forced_engine = -1;
@@ -307,6 +379,8 @@ int ensure_bases(bsrg_ctx* c, int nk, bool exchange) {
   return 0;
 }
 
+constexpr int kNotRegistered = -1019;  // internal: a GET_V vector is not in the registry yet
+
 // GET_V (src/bsr/BSR_MAT3/get_V.f90:2-27) on the host: qbs(:,jj) minus the
 // projections on same-l bound orbitals that the channel orbital ii must be
 // orthogonal to.  Returns the vector id in the W part of the vector table.
@@ -316,7 +390,7 @@ int get_v(bsrg_ctx* c, int ii, int jj, int& id, bool create = true) {
   const size_t key = static_cast<size_t>(ii - 1) + static_cast<size_t>(nbf) * (jj - 1);
   if (c->wtab[key] >= 0) { id = c->wtab[key]; return 0; }
   if (c->ibort[(ii - 1) + static_cast<size_t>(nbf) * (jj - 1)] == 0) return fail(-19, "get_V: <kl|nl> = 0 ?");
-  if (!create) return fail(-19, "get_V: vector not registered before the parallel pass");
+  if (!create) return fail(kNotRegistered, "get_V: vector not registered");
   std::vector<double> v(c->qbs.begin() + static_cast<size_t>(ns) * (jj - 1),
                         c->qbs.begin() + static_cast<size_t>(ns) * jj);
   for (int i = 1; i <= nbf; ++i) {
@@ -524,8 +598,10 @@ void counting_sort(const std::vector<Terms*>& parts, Vec vec, int nkeys, Key key
 }
 
 // Sort the terms of the parts by output and lay the CSR arrays out in the
-// pinned arena: doubles (coefficients) first, then ints.
-int stage_terms(const std::vector<Terms*>& parts, Staged& st, const Matrices& M) {
+// pinned arena: doubles (coefficients) first, then ints.  The band, rank-1
+// and vector sets are independent and are staged concurrently when a pool
+// is given.
+int stage_terms(const std::vector<Terms*>& parts, Staged& st, const Matrices& M, Pool* pool = nullptr) {
   if (int rc = wait_staged(st)) return rc;
   st.nband = st.nr1 = st.nvec = 0;
   for (const Terms* P : parts) {
@@ -541,38 +617,43 @@ int stage_terms(const std::vector<Terms*>& parts, Staged& st, const Matrices& M)
   st.o_bptr = put(st.nblk + 1, 4); st.o_bid = put(st.nband, 4);
   st.o_rptr = put(st.nblk + 1, 4); st.o_ru = put(st.nr1, 4); st.o_rw = put(st.nr1, 4); st.o_rs = put(st.nr1, 4);
   st.o_vptr = put(st.nq + 1, 4); st.o_vid = put(st.nvec, 4); st.o_vs = put(st.nvec, 4);
-  const size_t npos = std::max({st.nband, st.nr1, st.nvec, size_t(1)});
-  const size_t o_pos = put(npos, 4);
-  st.bytes = o_pos;  // the position scratch is not uploaded
+  st.bytes = o;  // the position scratch below is not uploaded
+  const size_t o_pb = put(st.nband, 4), o_pr = put(st.nr1, 4), o_pv = put(st.nvec, 4);
   if (o > st.h.cap) HIPCHK(st.h.reserve(o));
-  int* pos = st.hp<int>(o_pos);
-
-  counting_sort(parts, &Terms::band, st.nblk, [](const Terms::Band& x) { return x.blk; }, st.cnt, pos,
-                st.hp<int>(st.o_bptr));
-  {
-    double* bc = st.hp<double>(st.o_bc);
-    int* bid = st.hp<int>(st.o_bid);
-    size_t q = 0;
-    for (const Terms* P : parts)
-      for (const auto& x : P->band) { const int p = pos[q++]; bc[p] = x.c; bid[p] = x.bid; }
-  }
-  counting_sort(parts, &Terms::r1, st.nblk, [](const Terms::R1& x) { return x.blk; }, st.cnt, pos,
-                st.hp<int>(st.o_rptr));
-  {
-    double* rc = st.hp<double>(st.o_rc);
-    int *ru = st.hp<int>(st.o_ru), *rw = st.hp<int>(st.o_rw), *rs = st.hp<int>(st.o_rs);
-    size_t q = 0;
-    for (const Terms* P : parts)
-      for (const auto& x : P->r1) { const int p = pos[q++]; rc[p] = x.c; ru[p] = x.u; rw[p] = x.w; rs[p] = x.sidx; }
-  }
-  counting_sort(parts, &Terms::vec, st.nq, [](const Terms::Vec& x) { return x.q; }, st.cnt, pos,
-                st.hp<int>(st.o_vptr));
-  {
-    double* vc = st.hp<double>(st.o_vc);
-    int *vid = st.hp<int>(st.o_vid), *vs = st.hp<int>(st.o_vs);
-    size_t q = 0;
-    for (const Terms* P : parts)
-      for (const auto& x : P->vec) { const int p = pos[q++]; vc[p] = x.c; vid[p] = x.v; vs[p] = x.sidx; }
+  const std::function<void(int)> task = [&](int which) {
+    if (which == 0) {
+      int* pos = st.hp<int>(o_pb);
+      counting_sort(parts, &Terms::band, st.nblk, [](const Terms::Band& x) { return x.blk; }, st.cnt[0], pos,
+                    st.hp<int>(st.o_bptr));
+      double* bc = st.hp<double>(st.o_bc);
+      int* bid = st.hp<int>(st.o_bid);
+      size_t q = 0;
+      for (const Terms* P : parts)
+        for (const auto& x : P->band) { const int p = pos[q++]; bc[p] = x.c; bid[p] = x.bid; }
+    } else if (which == 1) {
+      int* pos = st.hp<int>(o_pr);
+      counting_sort(parts, &Terms::r1, st.nblk, [](const Terms::R1& x) { return x.blk; }, st.cnt[1], pos,
+                    st.hp<int>(st.o_rptr));
+      double* rc = st.hp<double>(st.o_rc);
+      int *ru = st.hp<int>(st.o_ru), *rw = st.hp<int>(st.o_rw), *rs = st.hp<int>(st.o_rs);
+      size_t q = 0;
+      for (const Terms* P : parts)
+        for (const auto& x : P->r1) { const int p = pos[q++]; rc[p] = x.c; ru[p] = x.u; rw[p] = x.w; rs[p] = x.sidx; }
+    } else {
+      int* pos = st.hp<int>(o_pv);
+      counting_sort(parts, &Terms::vec, st.nq, [](const Terms::Vec& x) { return x.q; }, st.cnt[2], pos,
+                    st.hp<int>(st.o_vptr));
+      double* vc = st.hp<double>(st.o_vc);
+      int *vid = st.hp<int>(st.o_vid), *vs = st.hp<int>(st.o_vs);
+      size_t q = 0;
+      for (const Terms* P : parts)
+        for (const auto& x : P->vec) { const int p = pos[q++]; vc[p] = x.c; vid[p] = x.v; vs[p] = x.sidx; }
+    }
+  };
+  if (pool && pool->size() >= 3) {
+    pool->run(3, task);
+  } else {
+    for (int w = 0; w < 3; ++w) task(w);
   }
   return 0;
 }
@@ -746,13 +827,17 @@ void register_vectors(bsrg_ctx* c, int nrec, const int* jtype, const int* K3, co
   }
 }
 
-// I_data over all lists of a wave on several host threads.  Records are cut
-// into contiguous ranges at list boundaries or where K1 changes (so the ACF
-// group sums of I_data stay whole); each thread decodes its range into its
-// own part, and the parts are kept in range order, which is list order.
-// Exchange coefficients and ACF additions are then applied in that order.
+// I_data over all lists of a wave on the context's host workers.  Records
+// are cut into contiguous ranges at list boundaries or where K1 changes (so
+// the ACF group sums of I_data stay whole); each worker decodes its range
+// into its own part, and the parts are kept in range order, which is list
+// order.  Exchange coefficients and ACF additions are then applied in that
+// order.  The workers only read the GET_V registry; vectors missing from it
+// (the first wave of new orbitals) are registered in list order and the
+// pass is repeated.
 int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, const int* k1, const int* k2,
                    const int* k3, const int* k4, const double* cd, bool any_exch) {
+  const auto t1 = std::chrono::steady_clock::now();
   Terms& TH = c->TH;
   const Matrices& M = c->H;
   // lazily built tables the workers read
@@ -761,9 +846,13 @@ int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, con
       for (int m = 0; m <= M.mk; ++m) (void)quadr(c, 1, 1, m);
       break;
     }
-  register_vectors(c, nrec, jtype, k3, k4);
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const int nth = static_cast<int>(std::min<long>({8L, static_cast<long>(hw), nrec / 16384L + 1}));
+  const auto t2 = std::chrono::steady_clock::now();
+  if (!c->pool) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    c->pool.reset(new Pool(static_cast<int>(std::min(8u, hw))));
+  }
+  // more ranges than workers, taken dynamically: records differ in cost
+  const int nth = static_cast<int>(std::min<long>(4L * c->pool->size(), nrec / 4096L + 1));
   std::vector<int> cuts{0};
   for (int t = 1; t < nth; ++t) {
     int b = static_cast<int>(static_cast<long>(nrec) * t / nth);
@@ -781,11 +870,19 @@ int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, con
     TH.gemm_n = nt * M.nblk();
     HIPCHK(TH.gemm.reserve(TH.gemm_n));
   }
-  auto work = [&](int t) {
+  bool zeroed = false;
+  std::vector<double> part_ms(np, 0.0);
+  const std::function<void(int)> work = [&](int t) {
+    const auto p0 = std::chrono::steady_clock::now();
+    struct Stamp {
+      std::chrono::steady_clock::time_point t0;
+      double* out;
+      ~Stamp() { *out = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+    } stamp{p0, &part_ms[t]};
     Terms& P = *c->work[t];
     P.clear();
     P.deferred = true;
-    if (TH.gemm_n) {
+    if (TH.gemm_n && !zeroed) {
       const size_t lo = TH.gemm_n * t / np, hi = TH.gemm_n * (t + 1) / np;
       std::memset(TH.gemm.p + lo, 0, (hi - lo) * sizeof(double));
     }
@@ -801,15 +898,17 @@ int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, con
       r = e;
     }
   };
-  if (np == 1) {
-    work(0);
-  } else {
-    std::vector<std::thread> th;
-    th.reserve(np - 1);
-    for (int t = 1; t < np; ++t) th.emplace_back(work, t);
-    work(0);
-    for (auto& x : th) x.join();
+  c->pool->run(np, work);
+  // GET_V vectors not yet in the registry (new orbitals): register them in
+  // list order and decode again
+  bool missing = false;
+  for (int t = 0; t < np; ++t) missing = missing || c->work[t]->err == kNotRegistered;
+  if (missing) {
+    register_vectors(c, nrec, jtype, k3, k4);
+    zeroed = true;
+    c->pool->run(np, work);
   }
+  const auto t3 = std::chrono::steady_clock::now();
   for (int t = 0; t < np; ++t) {
     const Terms& P = *c->work[t];
     if (P.err) return fail(P.err, P.msg);
@@ -823,6 +922,16 @@ int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, con
     TH.gemm_khi = std::max(TH.gemm_khi, P.gemm_khi);
   }
   if (TH.gemm_khi < 0) TH.gemm_n = 0;
+  const auto t4 = std::chrono::steady_clock::now();
+  auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
+  c->host_ms[2] = ms(t1, t2);
+  c->host_ms[3] = ms(t2, t3);
+  c->host_ms[4] = ms(t3, t4);
+  c->host_ms[8] = 0.0;
+  c->host_ms[9] = 0.0;
+  for (double x : part_ms) { c->host_ms[8] += x; c->host_ms[9] = std::max(c->host_ms[9], x); }
   return 0;
 }
 
@@ -1190,6 +1299,7 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   (void)hipEventRecord(c->ev[2], c->stream);
   HIPCHK(c->S.alloc(c->G.ns, nch, ncp, mk));
   HIPCHK(c->H.alloc(c->G.ns, nch, ncp, mk));
+  const auto ha = clk::now();
   // list processing on the host overlaps the band and basis kernels
   Terms& TS = c->TS;
   Terms& TH = c->TH;
@@ -1205,6 +1315,7 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   // H: ec * S + R^k lists + (E_t - ec) * sb
   for (int ich = 1; ich <= nch; ++ich) band_hl(TH, ich, ich, ec);
   if (int rc = add_odata(c, c->H, TH, novl, nullptr, ok3, ok4, ocd, eps_c, ec, false)) return rc;
+  const auto hb = clk::now();
   if (int rc = idata_parallel(c, nrec, jtype, kpol, k1, k2, k3, k4, cd, any_exch)) return rc;
   for (int ich = 1; ich <= nch; ++ich) band_hl(THt, ich, ich, etarg[iptar[ich - 1] - 1] - ec);
   c->partsS.assign(1, &TS);
@@ -1213,17 +1324,18 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   c->partsH.push_back(&THt);
   const auto h1 = clk::now();
   if (int rc = stage_terms(c->partsS, c->stS, c->S)) return rc;
-  if (int rc = stage_terms(c->partsH, c->stH, c->H)) return rc;
+  if (int rc = stage_terms(c->partsH, c->stH, c->H, c->pool.get())) return rc;
   const auto h2 = clk::now();
   if (int rc = upload_terms(c, c->S, TS, c->stS)) return rc;
   if (int rc = upload_terms(c, c->H, TH, c->stH)) return rc;
   if (int rc = autotune_gemm(c, c->H, c->stH, c->capture_safe)) return rc;
   const auto h3 = clk::now();
   auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
-  c->host_ms[0] = ms(h0, h1);
-  c->host_ms[1] = ms(h1, h2);
-  c->host_ms[2] = ms(h2, h3);
-  c->host_ms[3] = ms(h0, h3);
+  c->host_ms[0] = ms(h0, ha);
+  c->host_ms[1] = ms(ha, hb);
+  c->host_ms[5] = ms(h1, h2);
+  c->host_ms[6] = ms(h2, h3);
+  c->host_ms[7] = ms(h0, h3);
   c->wave_prepared = true;
   return 0;
 }
@@ -1281,9 +1393,9 @@ int bsrg_wave_assemble(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const in
 
 int bsrg_wave_engine(bsrg_ctx* c) { return c ? c->last_engine : -1; }
 
-int bsrg_wave_host_timings(bsrg_ctx* c, double* ms4) {
+int bsrg_wave_host_timings(bsrg_ctx* c, double* ms10) {
   if (int rc = check_ctx(c)) return rc;
-  for (int i = 0; i < 4; ++i) ms4[i] = c->host_ms[i];
+  for (int i = 0; i < 10; ++i) ms10[i] = c->host_ms[i];
   return 0;
 }
 

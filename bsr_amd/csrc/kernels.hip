@@ -439,16 +439,17 @@ __global__ void k_sb_upper(int ns, int ks, const double* sb, double* sbu) {
 }
 
 // ---- accumulation of term lists into the channel blocks ------------------
-// One pass over a channel block: out = base + band terms + rank-1 terms,
+// One pass over a channel block: out = (base + rank-1 terms) + band terms,
 // each sum in list order.  base is what hcc holds (the exchange GEMM result,
 // or earlier contributions) when has_base, else 0 and every block is
 // written.  Term lists are CSR over all blocks (ptr has nblk+1 entries).
-// grid (nblk, ceil(ns/32), ceil(ns/256)), block 256: lane -> rows
-// x0+lane+64a (a<4), wave -> 8 columns yt+8*wave+b.
+// grid (nblk, ceil(ns/kAsmCols), ceil(ns/256)), block 256: lane -> rows
+// x0+lane+64a (a<4), wave -> kAsmCpw columns yt+kAsmCpw*wave+b.
 // The band terms of the block are summed once per workgroup into the band
 // entries (i, ith) its columns touch, staged in LDS; rank-1 terms go 16 at a
 // time through LDS (u over the rows, f*w over the columns).
-constexpr int kAsmCols = 32;
+constexpr int kAsmCpw = 8;                // columns per wavefront
+constexpr int kAsmCols = 4 * kAsmCpw;     // columns per workgroup
 __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, const int* __restrict__ bptr,
                                                   const int* __restrict__ bid, const double* __restrict__ bc,
                                                   const double* __restrict__ bands, const int* __restrict__ rptr,
@@ -460,7 +461,7 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
   const int b0 = bptr[blk], b1 = bptr[blk + 1], r0 = rptr[blk], r1 = rptr[blk + 1];
   if (has_base && b0 == b1 && r0 == r1) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int x0 = blockIdx.z * 256, yt = blockIdx.y * kAsmCols, y0 = yt + wv * 8;
+  const int x0 = blockIdx.z * 256, yt = blockIdx.y * kAsmCols, y0 = yt + wv * kAsmCpw;
   const int nrow = min(256, ns - x0);
   double* h = hcc + static_cast<size_t>(blk) * ns * ns;
   __shared__ double su[16][256];
@@ -498,17 +499,24 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
       if (q < nent) sband[q] = e[m];
     }
   }
-  double acc[4][8];
+  // the sum starts from the base values, loaded before the rank-1 work so
+  // their latency overlaps it: out = (base + rank-1 terms) + band terms
+  double acc[4][kAsmCpw];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int b = 0; b < kAsmCpw; ++b) {
+    const int y = min(y0 + b, ns - 1);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) acc[a][b] = 0.0;
+    for (int a = 0; a < 4; ++a) {
+      const int x = min(x0 + lane + 64 * a, ns - 1);
+      acc[a][b] = has_base ? h[x + static_cast<size_t>(ns) * y] : 0.0;
+    }
+  }
   for (int tb = r0; tb < r1; tb += 16) {
     const int nt = min(16, r1 - tb);
     __syncthreads();
-    for (int q = threadIdx.x; q < nt * nrow; q += 256) {
-      const int r = q / nrow, x = q % nrow;
-      su[r][x] = V[static_cast<size_t>(ru[tb + r]) * ns + x0 + x];
+    for (int r = 0; r < nt; ++r) {
+      const double* src = V + static_cast<size_t>(ru[tb + r]) * ns + x0;
+      for (int x = threadIdx.x; x < nrow; x += 256) su[r][x] = src[x];
     }
     for (int q = threadIdx.x; q < nt * kAsmCols; q += 256) {
       const int r = q / kAsmCols, col = q % kAsmCols, t = tb + r, y = yt + col;
@@ -521,8 +529,8 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
 #pragma unroll
       for (int a = 0; a < 4; ++a) u[a] = su[r][(lane + 64 * a) & 255];
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const double w = sw[r][wv * 8 + b];
+      for (int b = 0; b < kAsmCpw; ++b) {
+        const double w = sw[r][wv * kAsmCpw + b];
 #pragma unroll
         for (int a = 0; a < 4; ++a) acc[a][b] += u[a] * w;
       }
@@ -530,21 +538,19 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
   }
   __syncthreads();  // sband complete
 #pragma unroll
-  for (int b = 0; b < 8; ++b) {
+  for (int b = 0; b < kAsmCpw; ++b) {
     const int y = y0 + b;
     if (y >= ns) continue;
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       const int x = x0 + lane + 64 * a;
       if (x >= ns) continue;
-      double* p = h + x + static_cast<size_t>(ns) * y;
-      double v = has_base ? *p : 0.0;
+      double v = acc[a][b];
       if (has_band) {
         const int ith = x > y ? x - y : y - x, i = min(x, y);
         if (ith < ks && i + ith < ns) v += sband[(i - ilo) * ks + ith];
       }
-      v += acc[a][b];
-      *p = v;
+      h[x + static_cast<size_t>(ns) * y] = v;
     }
   }
 }

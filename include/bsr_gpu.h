@@ -172,10 +172,13 @@ int bsrg_wave_engine(bsrg_ctx* ctx);
  * the autotuner choose (default). */
 int bsrg_set_gemm_engine(bsrg_ctx* ctx, int engine);
 
-/* Host time (ms) of the last prepare: [0] lists to terms (overlapping the
- * band and basis kernels), [1] sorting, [2] uploads and GEMM autotune,
- * [3] total. */
-int bsrg_wave_host_timings(bsrg_ctx* ctx, double* ms4);
+/* Host time (ms) of the last prepare: [0] checks and band/basis launches,
+ * [1] S and the overlap terms of H, [2] registry of GET_V vectors,
+ * [3] parallel list decoding (overlapping the band and basis kernels),
+ * [4] merge of the decoded parts, [5] sorting and staging, [6] uploads and
+ * GEMM autotune, [7] total, [8] decoding time summed over the parts,
+ * [9] the longest part. */
+int bsrg_wave_host_timings(bsrg_ctx* ctx, double* ms10);
 
 /* 1: the autotuner of later prepares only considers engines that can be
  * captured in a hipGraph (the hand-written kernels). */
