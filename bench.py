@@ -1,13 +1,23 @@
 """Benchmark: partial-wave assembly of H and S on one MI355X.
 
-Workload (default): one production-size partial wave -- ks=8, ns=200
+Headline workload: one production-size partial wave -- ks=8, ns=200
 B-splines on a semi-exponential grid to Rmax=40 a0 (Z=18), multipoles
 k=0..10, 60 channels + 400 perturbers, 5 bound orbitals, angular lists at 5%
-fill with coefficients U[-1,1]: H and S of dimension 12400 (synthetic, seeded).
+fill with coefficients U[-1,1]: H and S of dimension 12400 (synthetic,
+seeded).  A step is the whole job from resident inputs: R^k bands for every
+k, orbital contractions, list processing, and the S and H blocks complete on
+the device, in plain FP64.  value = unique H elements nhm(nhm+1)/2 per second.
 
-A step is the whole job from resident inputs: R^k bands for every k,
-orbital contractions, list processing, and the S and H blocks complete on
-the device.  value = unique H elements nhm(nhm+1)/2 per second.
+Beside the headline line's required fields, the JSON line carries:
+  split    the same wave with the exchange contraction in split precision,
+           and its largest relative element error against the FP64 result;
+  serving  a Poisson stream (20 jobs/s) of waves of the same target
+           (nch ~ U[40,80], ncp ~ U[200,400]) through the serving loop, in
+           FP64 and in split precision: p50/p99 latency at that rate, the
+           saturated capacity (all jobs queued at once), and the error of
+           every split-precision job against its FP64 result;
+  batch    48 uneven waves (nch ~ U[10,120], ns ~ U[100,250], k=0..8) as one
+           hipGraph: total elements/s of a replay, per-wave kernel time.
 """
 from __future__ import annotations
 
@@ -28,7 +38,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(seconds_hint=20.0):
+def cpu_baseline():
     """The oracle (1 thread, the reference's loop structure) on a bounded
     sample of the same workload: same grid, k range, fill and orbital set,
     10 channels and 50 perturbers instead of 60 and 400."""
@@ -42,15 +52,102 @@ def cpu_baseline(seconds_hint=20.0):
                 sample=f"nch=10 ncp=50 ns=200 k<=10 (nhm={w.nhm}); {dt:.1f} s")
 
 
+def mean_dict(ds):
+    return {k: round(sum(d[k] for d in ds) / len(ds), 4) for k in ds[0]}
+
+
+def run_steps(ctx, w, warmup, steps):
+    def step():
+        # a new partial wave: grid and orbitals arrive, bands and bases are
+        # rebuilt, lists processed, S and H assembled on the device
+        ctx.drop_caches()
+        ctx.set_grid(w.ks, w.ns, w.ml, w.me, w.h, w.t)
+        ctx.set_orbitals(w.pbs, w.iech, w.lbs, w.ibort, w.eps_ovl)
+        ctx.wave_assemble(w, set_inputs=False)
+
+    for _ in range(warmup):
+        step()
+    dev, host = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+        dev.append(ctx.wave_timings())
+        host.append(ctx.wave_host_timings())
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return ms, mean_dict(dev), mean_dict(host)
+
+
+def serving(stream, precision, nslots=4):
+    """One pass of the stream at its arrival times, then the same jobs all
+    queued at once (capacity)."""
+    from bsr_amd.serve import Server, summarize
+    srv = Server(nslots=nslots, precision=precision)
+    try:
+        # warm every slot: grid tables, R^k bands and bases become resident
+        warm = [srv.submit(w) for _, w in stream[:nslots]]
+        for j in warm:
+            j.event.wait()
+        paced = summarize(srv.run_stream(stream))
+        burst = summarize(srv.run_stream([(0.0, w) for _, w in stream]))
+    finally:
+        srv.close()
+    return dict(rate_jobs_per_s=round(len(stream) / stream[-1][0], 2), p50_ms=paced["p50_ms"],
+                p99_ms=paced["p99_ms"], elements_per_s_at_rate=paced["elements_per_s"],
+                capacity_elements_per_s=burst["elements_per_s"], capacity_p50_ms=burst["p50_ms"],
+                capacity_p99_ms=burst["p99_ms"], service_ms_mean=paced["service_ms_mean"],
+                device_ms_mean=paced["device_ms_mean"], failed=paced["failed"] + burst["failed"])
+
+
+def split_errors(stream):
+    """Each job of the stream in FP64 and in split precision: the largest
+    relative element error of S and H (floor 1e-3 of the largest element)."""
+    from bsr_amd import Context
+    ref, spl = Context(0), Context(0)
+    spl.set_precision(1, 0)
+    errs = []
+    try:
+        for _, w in stream:
+            ref.wave_assemble(w)
+            spl.wave_assemble(w)
+            errs.append(max(spl.wave_diff(ref, 0), spl.wave_diff(ref, 1)))
+    finally:
+        ref.close()
+        spl.close()
+    return errs
+
+
+def batch_c3(nwaves):
+    from bsr_amd import synth
+    from bsr_amd.batch import Batch
+    waves = [synth.c3_wave(s) for s in synth.c3_specs(nwaves=nwaves)]
+    b = Batch(waves)
+    try:
+        b.prepare()
+        b.capture()
+        b.run()
+        reps = [b.run() for _ in range(3)]
+        tot = min(r[1] for r in reps)
+        per = sorted(b.eager_ms)
+        return dict(waves=len(waves), graph_ms=tot, elements_per_s=b.n_elements() / (tot * 1e-3),
+                    wave_kernel_ms_p50=per[len(per) // 2], wave_kernel_ms_max=per[-1],
+                    engines=sorted(set(b.engines())))
+    finally:
+        b.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", choices=["fp64", "split"], default="fp64")
     ap.add_argument("--nch", type=int, default=60)
     ap.add_argument("--ncp", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-split", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=60)
+    ap.add_argument("--batch-waves", type=int, default=48)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); partial waves shard as whole jobs "
@@ -60,39 +157,16 @@ def main():
     log("generating workload")
     w = synth.c2(nch=a.nch, ncp=a.ncp)
     ctx = Context(0)
-    if a.mode == "split":
-        ctx.set_precision(1, 0)
-
-    def step():
-        # a new partial wave: grid and orbitals arrive, bands and bases are
-        # rebuilt, lists processed, S and H assembled on the device
-        ctx.drop_caches()
-        ctx.set_grid(w.ks, w.ns, w.ml, w.me, w.h, w.t)
-        ctx.set_orbitals(w.pbs, w.iech, w.lbs, w.ibort, w.eps_ovl)
-        ctx.wave_assemble(w, set_inputs=False)
-
-    for i in range(a.warmup):
-        step()
-        log(f"warmup {i + 1}/{a.warmup}")
-    dev, host = [], []
-    t0 = time.perf_counter()
-    for i in range(a.steps):
-        step()
-        dev.append(ctx.wave_timings())
-        host.append(ctx.wave_host_timings())
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / a.steps
-    log(f"timed {a.steps} steps: {ms:.3f} ms/step")
-    log("device ms: " + json.dumps({k: round(sum(d[k] for d in dev) / len(dev), 3) for k in dev[0]}))
-    log("host ms:   " + json.dumps({k: round(sum(d[k] for d in host) / len(host), 3) for k in host[0]}))
-    gemm_ms = sum(d["gemm"] for d in dev) / len(dev)
-    total_dev = sum(d["total"] for d in dev) / len(dev)
+    ms, dev, host = run_steps(ctx, w, a.warmup, a.steps)
+    log(f"fp64: {a.steps} steps, {ms:.3f} ms/step")
+    log("  device ms: " + json.dumps(dev))
+    log("  host ms:   " + json.dumps(host))
+    gemm_ms = dev["gemm"]
     nss = w.ns * w.ns
     nblk = w.nch * (w.nch + 1) // 2
     nt = (w.mk + 1) * w.nb * w.nb
     flops = 2.0 * nss * nblk * nt
     ach = flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
-    hbm_bytes = 8.0 * (nss * nt + nt * nblk + nss * nblk)
     out = {
         "metric": "matrix elems/s FP64 vs split-prec @<=1e-10 rel err; p50/p99 partial-wave latency",
         "value": w.n_elements / (ms * 1e-3),
@@ -104,15 +178,42 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.mode == "fp64" else "f64(split bf16)",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"partial wave ks=8 ns=200 k<=10 nch={w.nch} ncp={w.ncp} nhm={w.nhm}",
-                   "mode": a.mode, "device_ms_per_step": total_dev},
+                   "device_ms_per_step": dev["total"], "device_ms": dev, "host_ms": host},
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
                      "frac": ach / PEAK_FP64_MFMA, "traffic": None,
-                     "kernel": "exchange contraction GEMM", "kernel_ms": gemm_ms,
-                     "algorithmic_hbm_GBps": hbm_bytes / (gemm_ms * 1e-3) / 1e9 if gemm_ms > 0 else 0.0},
+                     "kernel": "exchange contraction GEMM (FP64)", "kernel_ms": gemm_ms,
+                     "flops_per_launch": flops},
     }
+    if not a.no_split:
+        sctx = Context(0)
+        sctx.set_precision(1, 0)
+        sms, sdev, _ = run_steps(sctx, w, a.warmup, a.steps)
+        err = max(sctx.wave_diff(ctx, 0), sctx.wave_diff(ctx, 1))
+        log(f"split: {sms:.3f} ms/step, max rel element error vs FP64 {err:.2e}")
+        out["split"] = {"value": w.n_elements / (sms * 1e-3), "ms_per_step": sms, "gemm_ms": sdev["gemm"],
+                        "engine": sctx.wave_engine(), "max_rel_err_vs_fp64": err,
+                        "vs_fp64_plain": ms / sms}
+        sctx.close()
+    ctx.close()
+    if not a.no_serve:
+        log("generating served stream")
+        stream = synth.c4_stream(njobs=a.serve_jobs)
+        sv = {"jobs": len(stream)}
+        for name, prec in (("fp64", 0), ("split", 1)):
+            sv[name] = serving(stream, prec)
+            log(f"serving {name}: " + json.dumps(sv[name]))
+        errs = split_errors(stream)
+        sv["split"]["max_rel_err_vs_fp64"] = max(errs)
+        sv["split"]["rel_err_per_job"] = [float(f"{e:.2e}") for e in errs]
+        sv["split_capacity_vs_fp64_plain"] = sv["split"]["capacity_elements_per_s"] / out["value"]
+        out["serving"] = sv
+    if not a.no_batch:
+        log("batch of uneven waves")
+        out["batch"] = batch_c3(a.batch_waves)
+        log("batch: " + json.dumps(out["batch"]))
     if not a.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

@@ -19,7 +19,7 @@ from .core import Context
 
 
 class Batch:
-    def __init__(self, waves, precision: int = 0, nslices: int = 5, device: int = 0):
+    def __init__(self, waves, precision: int = 0, nslices: int = 0, device: int = 0):
         self.waves = list(waves)
         self.ctx = [Context(device) for _ in self.waves]
         for c in self.ctx:

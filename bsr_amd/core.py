@@ -179,6 +179,14 @@ class Context:
         return dict(launch=t[0], overlaps=t[1], registry=t[2], decode=t[3], merge=t[4], sort=t[5],
                     upload=t[6], total=t[7], decode_cpu=t[8], decode_longest=t[9])
 
+    def wave_diff(self, ref: "Context", which: int = 1, floor_frac: float = 1e-3) -> float:
+        """Largest relative element error of this context's last wave against
+        the same wave assembled by `ref` (S for which=0, H for which=1), with
+        the floor floor_frac * max|ref|."""
+        out = C.c_double()
+        L.check(L.lib().bsrg_wave_diff(self._p, ref._p, which, floor_frac, C.byref(out)))
+        return out.value
+
     def drop_caches(self):
         """Forget resident bands and bases (the next wave rebuilds them)."""
         L.check(L.lib().bsrg_drop_caches(self._p))

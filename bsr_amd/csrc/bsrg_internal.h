@@ -77,6 +77,10 @@ void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, co
 void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,
                          const double* c, const double* stab, const double* V, double* hcb, hipStream_t s);
 void launch_symmetrize_diag(int ns, int nch, double* hcc, hipStream_t s);
+// max-reductions over n doubles into kReduceBlocks partial results
+constexpr int kReduceBlocks = 1024;
+void launch_absmax(size_t n, const double* x, double* partial, hipStream_t s);
+void launch_reldiff(size_t n, const double* a, const double* b, double floor, double* partial, hipStream_t s);
 void launch_scale(size_t n, double a, double* x, hipStream_t s);
 void launch_fill(size_t n, double v, double* x, hipStream_t s);
 // leaf convolutions of one band with a general density

@@ -60,12 +60,23 @@ struct DevVec {
     p = nullptr;
     n = 0;
   }
+  // Grows only; a regrowth allocates 25% headroom so that a run of jobs of
+  // slowly increasing size does not free and allocate (both synchronise)
+  // every time.
   hipError_t resize(size_t m) {
     if (m <= n && p) return hipSuccess;
+    const size_t want = n ? std::max(m, n + n / 4) : m;
     release();
     if (m == 0) return hipSuccess;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), m * sizeof(T));
-    if (e == hipSuccess) n = m;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();  // the failed attempt must not surface later
+      p = nullptr;
+      e = hipMalloc(reinterpret_cast<void**>(&p), m * sizeof(T));
+      if (e == hipSuccess) n = m;
+      return e;
+    }
+    n = want;
     return e;
   }
   hipError_t upload(const T* h, size_t m, hipStream_t s) {
@@ -262,6 +273,7 @@ struct bsrg_ctx {
   bool orb_ok = false;
   int nbf = 0, nb = 0, npair = 0;
   std::vector<double> pbs, qbs, obs;
+  std::vector<double> Pbound;  // coefficients of the bound orbitals the bases were built for
   std::vector<int> iech, lbs, ibort, bound, bidx;
   double eps_ovl = 1e-8;
   DevVec<double> P, dens;
@@ -290,7 +302,6 @@ struct bsrg_ctx {
   int nwork = 0;                             // worker parts used by the last prepare
   std::unique_ptr<Pool> pool;                // host workers of the list pass
   std::vector<Terms*> partsS, partsH;        // term parts of S and H in list order
-  std::map<std::tuple<int, int, int>, int> tuned;  // GEMM shape -> engine
   int last_engine = -1, forced_engine = -1;
   DevVec<double> dtmp, dtmp2;
 };
@@ -731,16 +742,26 @@ int run_gemm(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, int eng
                     engine == 1 ? 1 : (engine == 2 ? 2 : 4), c->stream);
 }
 
-// Autotuner: time every FP64 engine once on this GEMM shape (overwriting
-// the output, which the real launch rewrites) and keep the fastest; the
-// choice is cached per shape.  capture_safe excludes the library GEMM.
+// Autotuner: time every FP64 engine once per GEMM shape class (overwriting
+// the output, which the real launch rewrites) and keep the fastest.  The
+// choice is shared by all contexts of the process; the class is (device,
+// rows, inner size, column count rounded to a 1.25x geometric bucket,
+// capture-safe), so jobs of similar size reuse it.  capture_safe excludes
+// the library GEMM.
+std::mutex g_tuned_mu;
+std::map<std::tuple<int, int, int, int, int>, int> g_tuned;
+
 int autotune_gemm(bsrg_ctx* c, Matrices& M, Staged& st, bool capture_safe) {
   if (!st.has_gemm || c->precision_mode == 1) return 0;
   if (c->forced_engine >= 0) { st.engine = c->forced_engine; return 0; }
   const int nr = (st.gemm_khi - st.gemm_klo + 1) * c->nb * c->nb;
-  const auto key = std::make_tuple(c->G.ns, static_cast<int>(M.nblk()), nr * (capture_safe ? -1 : 1));
-  auto it = c->tuned.find(key);
-  if (it != c->tuned.end()) { st.engine = it->second; return 0; }
+  const int bucket = static_cast<int>(std::floor(std::log(static_cast<double>(M.nblk())) / std::log(1.25)));
+  const auto key = std::make_tuple(c->device, c->G.ns, nr, bucket, capture_safe ? 1 : 0);
+  {
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) { st.engine = it->second; return 0; }
+  }
   float best = 1e30f;
   int best_e = 2;
   for (int e = capture_safe ? 1 : 0; e <= 3; ++e) {
@@ -753,7 +774,10 @@ int autotune_gemm(bsrg_ctx* c, Matrices& M, Staged& st, bool capture_safe) {
     (void)hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
     if (ms < best) { best = ms; best_e = e; }
   }
-  c->tuned[key] = best_e;
+  {
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    g_tuned[key] = best_e;
+  }
   st.engine = best_e;
   return 0;
 }
@@ -1121,6 +1145,12 @@ int bsrg_set_orbitals(bsrg_ctx* c, int nbf, const double* pbs, const int* iech, 
     return 0;  // same orbitals: the bases stay resident
   HIPCHK(hipSetDevice(c->device));
   const int ns = c->G.ns, ks = c->G.ks;
+  // the bases depend on the grid and the bound orbitals only: they stay
+  // resident when a wave brings new channel orbitals for the same target
+  std::vector<double> Pnew;
+  for (int i = 0; i < nbf; ++i)
+    if (iech[i] == 0) Pnew.insert(Pnew.end(), pbs + static_cast<size_t>(ns) * i, pbs + static_cast<size_t>(ns) * (i + 1));
+  const bool same_bound = c->orb_ok && Pnew == c->Pbound;
   c->nbf = nbf;
   c->pbs.assign(pbs, pbs + static_cast<size_t>(ns) * nbf);
   c->iech.assign(iech, iech + nbf);
@@ -1149,21 +1179,23 @@ int bsrg_set_orbitals(bsrg_ctx* c, int nbf, const double* pbs, const int* iech, 
       c->obs[j + static_cast<size_t>(nbf) * i] = s;
     }
   }
-  std::vector<double> P(static_cast<size_t>(ns) * std::max(c->nb, 1), 0.0);
-  for (int b = 0; b < c->nb; ++b)
-    std::memcpy(&P[static_cast<size_t>(ns) * b], &c->pbs[static_cast<size_t>(ns) * (c->bound[b] - 1)], sizeof(double) * ns);
-  HIPCHK(c->P.upload(P.data(), P.size(), c->stream));
-  HIPCHK(c->dens.resize(std::max<size_t>(static_cast<size_t>(c->npair) * ns * ks, 1)));
-  if (c->npair) launch_pair_dens(ns, ks, c->nb, c->P.p, c->dens.p, c->stream);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!same_bound) {
+    std::vector<double> P(static_cast<size_t>(ns) * std::max(c->nb, 1), 0.0);
+    if (!Pnew.empty()) std::memcpy(P.data(), Pnew.data(), Pnew.size() * sizeof(double));
+    HIPCHK(c->P.upload(P.data(), P.size(), c->stream));
+    HIPCHK(c->dens.resize(std::max<size_t>(static_cast<size_t>(c->npair) * ns * ks, 1)));
+    if (c->npair) launch_pair_dens(ns, ks, c->nb, c->P.p, c->dens.p, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->Pbound.swap(Pnew);
+    c->basis_nk = 0;
+    c->exch_nk = 0;
+  }
   c->Q.clear();
   c->Qok.clear();
   c->wtab.assign(static_cast<size_t>(nbf) * nbf, -1);
   c->wvec.clear();
   c->wvec_uploaded = -1;
-  c->basis_nk = 0;
-  c->exch_nk = 0;
   c->orb_ok = true;
   return 0;
 }
@@ -1599,6 +1631,41 @@ int bsrg_graph_sync(bsrg_graph* g, double* wave_ms, double* total_ms) {
     HIPCHK(hipEventElapsedTime(&ms, g->t0, g->t1));
     *total_ms = ms;
   }
+  return 0;
+}
+
+int bsrg_wave_diff(bsrg_ctx* a, bsrg_ctx* ref, int which, double floor_frac, double* rel) {
+  if (int rc = check_ctx(a)) return rc;
+  if (int rc = check_ctx(ref)) return rc;
+  if (!a->wave_ok || !ref->wave_ok) return fail(-8, "no assembled wave (bsrg_wave_assemble)");
+  if (a->device != ref->device) return fail(-5, "bsrg_wave_diff: contexts on different devices");
+  const Matrices& A = which == 0 ? a->S : a->H;
+  const Matrices& B = which == 0 ? ref->S : ref->H;
+  if (A.ns != B.ns || A.nch != B.nch || A.ncp != B.ncp) return fail(-1, "bsrg_wave_diff: different shapes");
+  HIPCHK(hipSetDevice(ref->device));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  hipStream_t s = ref->stream;
+  HIPCHK(ref->dtmp.resize(2 * kReduceBlocks));
+  std::vector<double> part(2 * kReduceBlocks);
+  // scale of the reference: largest element of its channel blocks, vectors and hbb
+  launch_absmax(B.hcc_n(), B.hcc.p, ref->dtmp.p, s);
+  launch_absmax(B.hcb_n(), B.hcb.p, ref->dtmp.p + kReduceBlocks, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(part.data(), ref->dtmp.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double mx = 0.0;
+  for (double v : part) mx = std::max(mx, v);
+  for (double v : B.hbb) mx = std::max(mx, std::fabs(v));
+  const double floor = std::max(floor_frac * mx, 1e-300);
+  launch_reldiff(B.hcc_n(), A.hcc.p, B.hcc.p, floor, ref->dtmp.p, s);
+  launch_reldiff(B.hcb_n(), A.hcb.p, B.hcb.p, floor, ref->dtmp.p + kReduceBlocks, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(part.data(), ref->dtmp.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double r = 0.0;
+  for (double v : part) r = std::max(r, v);
+  for (size_t q = 0; q < B.hbb.size(); ++q) r = std::max(r, std::fabs(A.hbb[q] - B.hbb[q]) / std::max(std::fabs(B.hbb[q]), floor));
+  *rel = r;
   return 0;
 }
 

@@ -605,6 +605,32 @@ __global__ void k_symmetrize_diag(int ns, int nch, double* hcc) {
   }
 }
 
+// ---- comparison of two results (per-job error statement) ------------------
+// partial[blockIdx.x] = max |x| over a grid-stride slice; block 256
+__global__ void k_absmax(size_t n, const double* __restrict__ x, double* __restrict__ partial) {
+  __shared__ double red[4];
+  double m = 0.0;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * 256)
+    m = fmax(m, fabs(x[i]));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// partial[blockIdx.x] = max |a - b| / max(|b|, floor) over a slice
+__global__ void k_reldiff(size_t n, const double* __restrict__ a, const double* __restrict__ b, double floor,
+                          double* __restrict__ partial) {
+  __shared__ double red[4];
+  double m = 0.0;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * 256)
+    m = fmax(m, fabs(a[i] - b[i]) / fmax(fabs(b[i]), floor));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
 __global__ void k_scale(size_t n, double a, double* x) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) x[i] *= a;
@@ -743,6 +769,14 @@ void launch_symmetrize_diag(int ns, int nch, double* hcc, hipStream_t s) {
   if (nch <= 0) return;
   const int nt = (ns + 31) / 32;
   hipLaunchKernelGGL(k_symmetrize_diag, dim3(nt * (nt + 1) / 2, nch), dim3(256), 0, s, ns, nch, hcc);
+}
+
+void launch_absmax(size_t n, const double* x, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_absmax, dim3(kReduceBlocks), dim3(256), 0, s, n, x, partial);
+}
+
+void launch_reldiff(size_t n, const double* a, const double* b, double floor, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_reldiff, dim3(kReduceBlocks), dim3(256), 0, s, n, a, b, floor, partial);
 }
 
 void launch_scale(size_t n, double a, double* x, hipStream_t s) {

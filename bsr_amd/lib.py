@@ -70,6 +70,7 @@ SIGNATURES = {
     "bsrg_wave_timings": [C.c_void_p, _dp],
     "bsrg_set_precision": [C.c_void_p, C.c_int, C.c_int],
     "bsrg_drop_caches": [C.c_void_p],
+    "bsrg_wave_diff": [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)],
     "bsrg_wave_prepare": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
                           C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, _dp, _ip],
     "bsrg_wave_launch": [C.c_void_p],

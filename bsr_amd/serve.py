@@ -42,7 +42,7 @@ class Job:
 
 
 class Server:
-    def __init__(self, nslots: int = 4, hbm_budget: float = 200e9, precision: int = 0, nslices: int = 5,
+    def __init__(self, nslots: int = 4, hbm_budget: float = 200e9, precision: int = 0, nslices: int = 0,
                  device: int = 0):
         self.slots = [Context(device) for _ in range(nslots)]
         for c in self.slots:
@@ -147,5 +147,7 @@ def summarize(jobs) -> dict:
     lat = np.array([j.latency for j in jobs]) * 1e3
     span = max(j.done for j in jobs) - min(j.arrival for j in jobs)
     elems = sum(j.wave.n_elements for j in jobs)
+    service = np.array([j.done - j.start for j in jobs]) * 1e3
     return dict(jobs=len(jobs), p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
-                elements_per_s=elems / span, span_s=span, failed=sum(j.error is not None for j in jobs))
+                elements_per_s=elems / span, span_s=span, failed=sum(j.error is not None for j in jobs),
+                service_ms_mean=float(service.mean()), device_ms_mean=float(np.mean([j.device_ms for j in jobs])))

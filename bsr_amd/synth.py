@@ -285,8 +285,10 @@ def c3_specs(seed: int = 3, nwaves: int = 48):
 
 
 def c3_wave(spec, ncp: int = 0) -> PartialWave:
-    return make_wave(spec["seed"], 8, spec["ns"], 18.0, 40.0, nch=spec["nch"], ncp=ncp, mk=8, nb=5, ntarg=5,
-                     fill=0.05, name=f"c3-{spec['seed']}")
+    w = make_wave(spec["seed"], 8, spec["ns"], 18.0, 40.0, nch=spec["nch"], ncp=ncp, mk=8, nb=5, ntarg=5,
+                  fill=0.05, name=f"c3-{spec['seed']}")
+    w.concat()  # a job arrives with its lists packed
+    return w
 
 
 def c4_stream(seed: int = 4, njobs: int = 60, rate: float = 20.0):
@@ -301,5 +303,6 @@ def c4_stream(seed: int = 4, njobs: int = 60, rate: float = 20.0):
         w = make_wave(seed * 100000 + j, 8, 200, 18.0, 40.0, nch=int(rng.integers(40, 81)),
                       ncp=int(rng.integers(200, 401)), mk=10, nb=5, ntarg=5, fill=0.05,
                       name=f"c4-{j}", target_seed=seed)
+        w.concat()  # a request arrives with its lists packed
         out.append((float(t[j]), w))
     return out

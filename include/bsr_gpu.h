@@ -194,6 +194,12 @@ int bsrg_graph_launch(bsrg_graph* graph);
 int bsrg_graph_sync(bsrg_graph* graph, double* wave_ms, double* total_ms);
 int bsrg_graph_destroy(bsrg_graph* graph);
 
+/* Error of one assembled result against a reference result of the same
+ * wave (e.g. split precision against FP64), S (which=0) or H (which=1):
+ * rel = max over hcc, hcb, hbb of |a - r| / max(|r|, floor_frac * max|r|).
+ * Both contexts must be on one device.  No counterpart in the reference. */
+int bsrg_wave_diff(bsrg_ctx* a, bsrg_ctx* ref, int which, double floor_frac, double* rel);
+
 /* Forget the resident R^k bands and orbital bases so the next call
  * rebuilds them (bsrg_set_grid / bsrg_set_orbitals keep them when called
  * again with identical inputs).  No counterpart in the reference. */
