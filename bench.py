@@ -9,8 +9,9 @@ k, orbital contractions, list processing, and the S and H blocks complete on
 the device, in plain FP64.  value = unique H elements nhm(nhm+1)/2 per second.
 
 Beside the headline line's required fields, the JSON line carries:
-  split    the same wave with the exchange contraction in split precision,
-           and its largest relative element error against the FP64 result;
+  split    the same wave with the exchange contraction in split precision
+           (int8 Ozaki slices; split_bf16: bf16 slices), and its largest
+           relative element error against the FP64 result;
   serving  a Poisson stream (20 jobs/s) of waves of the same target
            (nch ~ U[40,80], ncp ~ U[200,400]) through the serving loop, in
            FP64 and in split precision: p50/p99 latency at that rate, the
@@ -103,7 +104,7 @@ def split_errors(stream):
     relative element error of S and H (floor 1e-3 of the largest element)."""
     from bsr_amd import Context
     ref, spl = Context(0), Context(0)
-    spl.set_precision(1, 0)
+    spl.set_precision(2, 0)
     errs = []
     try:
         for _, w in stream:
@@ -188,21 +189,22 @@ def main():
                      "flops_per_launch": flops},
     }
     if not a.no_split:
-        sctx = Context(0)
-        sctx.set_precision(1, 0)
-        sms, sdev, _ = run_steps(sctx, w, a.warmup, a.steps)
-        err = max(sctx.wave_diff(ctx, 0), sctx.wave_diff(ctx, 1))
-        log(f"split: {sms:.3f} ms/step, max rel element error vs FP64 {err:.2e}")
-        out["split"] = {"value": w.n_elements / (sms * 1e-3), "ms_per_step": sms, "gemm_ms": sdev["gemm"],
-                        "engine": sctx.wave_engine(), "max_rel_err_vs_fp64": err,
-                        "vs_fp64_plain": ms / sms}
-        sctx.close()
+        # int8 slices (the split form used below) and bf16 slices
+        for key, mode in (("split", 2), ("split_bf16", 1)):
+            sctx = Context(0)
+            sctx.set_precision(mode, 0)
+            sms, sdev, _ = run_steps(sctx, w, a.warmup, a.steps)
+            err = max(sctx.wave_diff(ctx, 0), sctx.wave_diff(ctx, 1))
+            log(f"{key}: {sms:.3f} ms/step, GEMM {sdev['gemm']:.3f} ms, max rel element error vs FP64 {err:.2e}")
+            out[key] = {"value": w.n_elements / (sms * 1e-3), "ms_per_step": sms, "gemm_ms": sdev["gemm"],
+                        "engine": sctx.wave_engine(), "max_rel_err_vs_fp64": err, "vs_fp64_plain": ms / sms}
+            sctx.close()
     ctx.close()
     if not a.no_serve:
         log("generating served stream")
         stream = synth.c4_stream(njobs=a.serve_jobs)
         sv = {"jobs": len(stream)}
-        for name, prec in (("fp64", 0), ("split", 1)):
+        for name, prec in (("fp64", 0), ("split", 2)):
             sv[name] = serving(stream, prec)
             log(f"serving {name}: " + json.dumps(sv[name]))
         errs = split_errors(stream)

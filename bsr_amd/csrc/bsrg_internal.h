@@ -93,6 +93,13 @@ void launch_convol_exchange(int ns, int ks, const double* rkb, const double* d, 
 int mfma_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B, size_t ldb, double* C,
                size_t ldc, bool accumulate, int tm, hipStream_t s);
 // split-precision FP64 GEMM (ozaki.hip): C = A*B (+C), column-major
+// int8 split GEMM: A-side slices prepared once (kept while A is unchanged),
+// B side sliced per call into bbuf; C (+)= A*B.
+size_t ozaki8_a_bytes(int m, int K, int S);
+size_t ozaki8_b_bytes(int n, int K, int S);
+int ozaki8_prepare_a(int m, int K, const double* A, size_t lda, int S, void* abuf, hipStream_t s);
+int ozaki8_dgemm(int m, int n, int K, const void* abuf, const double* B, size_t ldb, double* C, size_t ldc,
+                 bool accumulate, int S, void* bbuf, hipStream_t s);
 size_t ozaki_workspace_bytes(int m, int n, int K, int S);
 int ozaki_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B, size_t ldb, double* C,
                 size_t ldc, bool accumulate, int S, void* work, hipStream_t s);

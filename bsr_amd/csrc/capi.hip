@@ -294,7 +294,12 @@ struct bsrg_ctx {
 
   // staged term uploads: S and H of the fused wave, and the per-call path
   Staged stS, stH, stC;
-  DevVec<char> ozk;  // split-precision workspace
+  DevVec<char> ozk;  // split-precision workspace (bf16 slices)
+  // int8 split precision: slices of the exchange bases, kept while the
+  // bases are unchanged (e_ver counts their rebuilds), and of the coefficients
+  DevVec<char> oz8a, oz8b;
+  long e_ver = 0, oz8a_ver = -1;
+  int oz8a_S = 0;
   Terms TS, TH;      // term sets of the prepared wave (host scalars at finish)
   Terms TC;          // term set of the per-call path
   Terms THt;         // H terms after the lists: (E_t - ec) * sb on the diagonal
@@ -386,6 +391,7 @@ int ensure_bases(bsrg_ctx* c, int nk, bool exchange) {
     launch_exchange_basis(ns, ks, nk, nb, c->rkb.p, c->P.p, c->E.p, c->stream);
     HIPCHK(hipGetLastError());
     c->exch_nk = nk;
+    ++c->e_ver;
   }
   return 0;
 }
@@ -706,6 +712,15 @@ int upload_terms(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
       const int nr = (T.gemm_khi - T.gemm_klo + 1) * nb * nb;
       HIPCHK(c->ozk.resize(ozaki_workspace_bytes(static_cast<int>(nss), static_cast<int>(M.nblk()), nr,
                                                  c->nslices)));
+    } else if (c->precision_mode == 2) {
+      const int nss = c->G.ns * c->G.ns, K = c->exch_nk * c->nb * c->nb;
+      if (c->oz8a_ver != c->e_ver || c->oz8a_S != c->nslices) {
+        HIPCHK(c->oz8a.resize(ozaki8_a_bytes(nss, K, c->nslices)));
+        if (int rc = ozaki8_prepare_a(nss, K, c->E.p, nss, c->nslices, c->oz8a.p, s)) return rc;
+        c->oz8a_ver = c->e_ver;
+        c->oz8a_S = c->nslices;
+      }
+      HIPCHK(c->oz8b.resize(ozaki8_b_bytes(static_cast<int>(M.nblk()), K, c->nslices)));
     }
   }
   if (st.bytes) {
@@ -731,6 +746,9 @@ int run_gemm(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, int eng
   if (c->precision_mode == 1)
     return ozaki_dgemm(static_cast<int>(nss), static_cast<int>(nblk), nr, A, nss, B, nt, M.hcc.p, nss, !overwrite,
                        c->nslices, c->ozk.p, c->stream);
+  if (c->precision_mode == 2)  // all multipoles of the bases: their slices are prepared once
+    return ozaki8_dgemm(static_cast<int>(nss), static_cast<int>(nblk), c->exch_nk * nb * nb, c->oz8a.p, st.gemm.p,
+                        nt, M.hcc.p, nss, !overwrite, c->nslices, c->oz8b.p, c->stream);
   if (engine == 0) {
     const double alpha = 1.0, beta = overwrite ? 0.0 : 1.0;
     rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, static_cast<int>(nss),
@@ -752,7 +770,7 @@ std::mutex g_tuned_mu;
 std::map<std::tuple<int, int, int, int, int>, int> g_tuned;
 
 int autotune_gemm(bsrg_ctx* c, Matrices& M, Staged& st, bool capture_safe) {
-  if (!st.has_gemm || c->precision_mode == 1) return 0;
+  if (!st.has_gemm || c->precision_mode != 0) return 0;
   if (c->forced_engine >= 0) { st.engine = c->forced_engine; return 0; }
   const int nr = (st.gemm_khi - st.gemm_klo + 1) * c->nb * c->nb;
   const int bucket = static_cast<int>(std::floor(std::log(static_cast<double>(M.nblk())) / std::log(1.25)));
@@ -783,25 +801,25 @@ int autotune_gemm(bsrg_ctx* c, Matrices& M, Staged& st, bool capture_safe) {
 }
 
 // Device-only work of an uploaded term set (no host synchronisation, no
-// allocation): the part of a wave that a hipGraph captures.
+// allocation): the part of a wave that a hipGraph captures.  The term pass
+// goes first -- on an overwrite it writes every block (no read) -- and the
+// exchange GEMM then accumulates into the blocks: one read-modify-write of
+// the channel blocks, inside the GEMM, instead of a second pass.
 int enqueue_terms(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, hipEvent_t e0, hipEvent_t e1) {
   const int ns = c->G.ns, ks = c->G.ks;
   hipStream_t s = c->stream;
-  if (e0) (void)hipEventRecord(e0, s);
-  if (st.has_gemm) {
-    if (int rc = run_gemm(c, M, st, overwrite, st.engine)) return rc;
-    c->last_engine = c->precision_mode == 1 ? 10 + c->nslices : st.engine;
-  }
-  if (e1) (void)hipEventRecord(e1, s);
-  // the GEMM wrote (or added to) hcc: the term pass adds to it; without a
-  // GEMM an overwrite pass writes every block
-  const int base = (st.has_gemm || !overwrite) ? 1 : 0;
-  launch_assemble(ns, ks, st.nblk, base, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid), st.dp<double>(st.o_bc),
-                  c->bands.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw), st.dp<int>(st.o_rs),
-                  st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p, s);
+  launch_assemble(ns, ks, st.nblk, overwrite ? 0 : 1, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
+                  st.dp<double>(st.o_bc), c->bands.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
+                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p, s);
   launch_vec_assemble(ns, st.nq, overwrite ? 0 : 1, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid),
                       st.dp<int>(st.o_vs), st.dp<double>(st.o_vc), c->St.p, c->VT.p, M.hcb.p, s);
   HIPCHK(hipGetLastError());
+  if (e0) (void)hipEventRecord(e0, s);
+  if (st.has_gemm) {
+    if (int rc = run_gemm(c, M, st, false, st.engine)) return rc;
+    c->last_engine = c->precision_mode == 0 ? st.engine : 10 * c->precision_mode + c->nslices;
+  }
+  if (e1) (void)hipEventRecord(e1, s);
   return 0;
 }
 
@@ -1679,9 +1697,9 @@ int bsrg_drop_caches(bsrg_ctx* c) {
 
 int bsrg_set_precision(bsrg_ctx* c, int mode, int nslices) {
   if (int rc = check_ctx(c)) return rc;
-  if (mode != 0 && mode != 1) return fail(-9, "precision mode must be 0 (FP64) or 1 (split)");
-  if (mode == 1 && nslices == 0) nslices = 6;
-  if (mode == 1 && (nslices < 2 || nslices > 8)) return fail(-9, "split precision: slices must be 2..8");
+  if (mode < 0 || mode > 2) return fail(-9, "precision mode must be 0 (FP64), 1 (bf16 split) or 2 (int8 split)");
+  if (mode != 0 && nslices == 0) nslices = 6;
+  if (mode != 0 && (nslices < 2 || nslices > 8)) return fail(-9, "split precision: slices must be 2..8");
   c->precision_mode = mode;
   c->nslices = nslices;
   return 0;

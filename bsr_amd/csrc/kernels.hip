@@ -467,8 +467,8 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
   __shared__ double su[16][256];
   __shared__ double sw[16][kAsmCols];
   __shared__ double sband[(kAsmCols + kMaxKs - 1) * kMaxKs];
-  __shared__ int tbid[64];
-  __shared__ double tbc[64];
+  __shared__ int tbid[64], tu[64], tw[64];
+  __shared__ double tbc[64], tf[64];
   // band entries (i, ith) for i in [yt-ks+1, yt+kAsmCols): sband[(i-ilo)*ks + ith]
   const int ilo = yt - (ks - 1), nent = (kAsmCols + ks - 1) * ks;
   const bool has_band = b1 > b0;
@@ -511,28 +511,40 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
       acc[a][b] = has_base ? h[x + static_cast<size_t>(ns) * y] : 0.0;
     }
   }
-  for (int tb = r0; tb < r1; tb += 16) {
-    const int nt = min(16, r1 - tb);
+  // rank-1 terms: metadata and scale factors of 64 terms at a time into LDS
+  // (one round of loads), then their vectors 16 at a time
+  for (int tc = r0; tc < r1; tc += 64) {
+    const int nc = min(64, r1 - tc);
     __syncthreads();
-    for (int r = 0; r < nt; ++r) {
-      const double* src = V + static_cast<size_t>(ru[tb + r]) * ns + x0;
-      for (int x = threadIdx.x; x < nrow; x += 256) su[r][x] = src[x];
+    if (threadIdx.x < nc) {
+      const int t = tc + threadIdx.x;
+      const int sx = rs[t];
+      tf[threadIdx.x] = rc[t] * (sx >= 0 ? stab[sx] : 1.0);
+      tu[threadIdx.x] = ru[t];
+      tw[threadIdx.x] = rw[t];
     }
-    for (int q = threadIdx.x; q < nt * kAsmCols; q += 256) {
-      const int r = q / kAsmCols, col = q % kAsmCols, t = tb + r, y = yt + col;
-      const double f = rc[t] * (rs[t] >= 0 ? stab[rs[t]] : 1.0);
-      sw[r][col] = y < ns ? f * V[static_cast<size_t>(rw[t]) * ns + y] : 0.0;
-    }
-    __syncthreads();
-    for (int r = 0; r < nt; ++r) {
-      double u[4];
+    for (int tb = 0; tb < nc; tb += 16) {
+      const int nt = min(16, nc - tb);
+      __syncthreads();
+      for (int q = threadIdx.x; q < nt * 256; q += 256) {
+        const int r = q >> 8, x = q & 255;
+        if (x < nrow) su[r][x] = V[static_cast<size_t>(tu[tb + r]) * ns + x0 + x];
+      }
+      for (int q = threadIdx.x; q < nt * kAsmCols; q += 256) {
+        const int r = q / kAsmCols, col = q % kAsmCols, y = yt + col;
+        sw[r][col] = y < ns ? tf[tb + r] * V[static_cast<size_t>(tw[tb + r]) * ns + y] : 0.0;
+      }
+      __syncthreads();
+      for (int r = 0; r < nt; ++r) {
+        double u[4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) u[a] = su[r][(lane + 64 * a) & 255];
+        for (int a = 0; a < 4; ++a) u[a] = su[r][(lane + 64 * a) & 255];
 #pragma unroll
-      for (int b = 0; b < kAsmCpw; ++b) {
-        const double w = sw[r][wv * kAsmCpw + b];
+        for (int b = 0; b < kAsmCpw; ++b) {
+          const double w = sw[r][wv * kAsmCpw + b];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) acc[a][b] += u[a] * w;
+          for (int a = 0; a < 4; ++a) acc[a][b] += u[a] * w;
+        }
       }
     }
   }

@@ -216,6 +216,158 @@ __global__ __launch_bounds__(256) void k_dgemm(int m, int n, int K, const double
     }
 }
 
+// ---- int8 slices on int8 MFMA, exact int32 accumulation -------------------
+// Scaling by 2^(e+1) (max|row| = f 2^e, f in [1/2,1)) leaves |x| < 1/2, so
+// every slice x_s = rint(y 2^7) of the running remainder is an integer in
+// [-64, 64]: int8.  Slice products are <= 2^12; a level (at most S products
+// per K index) summed over Kp <= 2^13 indices stays below 2^31, so the
+// int32 MFMA accumulation of each level is exact over the whole K.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void k_row_scale8(int m, int K, const double* A, size_t lda, double* sig) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double mx = 0.0;
+  for (int t = 0; t < K; ++t) mx = fmax(mx, fabs(A[i + lda * t]));
+  int e = 0;
+  if (mx > 0.0) (void)frexp(mx, &e);
+  sig[i] = ldexp(1.0, e + 1);
+}
+
+__global__ void k_col_scale8(int n, int K, const double* B, size_t ldb, double* tau) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double mx = 0.0;
+  for (int t = 0; t < K; ++t) mx = fmax(mx, fabs(B[t + ldb * j]));
+  int e = 0;
+  if (mx > 0.0) (void)frexp(mx, &e);
+  tau[j] = ldexp(1.0, e + 1);
+}
+
+__device__ __forceinline__ void cut8(double x, int S, int8_t* out, size_t stride) {
+  for (int s = 0; s < S; ++s) {
+    const double y = x * 128.0;
+    const double d = rint(y);
+    x = y - d;
+    out[s * stride] = static_cast<int8_t>(static_cast<int>(d));
+  }
+}
+
+// A slices As[s][row][t] (t fastest, Kp padded, rows padded to mp): 64 rows
+// x 64 t per workgroup, transposed through LDS.
+__global__ void k_slice8_rows(int m, int K, int mp, int Kp, int S, const double* A, size_t lda, const double* sig,
+                              int8_t* As) {
+  __shared__ double tile[64][65];
+  const int r0 = blockIdx.x * 64, t0 = blockIdx.y * 64;
+  for (int q = threadIdx.x; q < 64 * 64; q += blockDim.x) {
+    const int r = q % 64, t = q / 64;
+    const int i = r0 + r, tt = t0 + t;
+    tile[t][r] = (i < m && tt < K) ? A[i + lda * tt] / sig[i] : 0.0;
+  }
+  __syncthreads();
+  const size_t stride = static_cast<size_t>(mp) * Kp;
+  for (int q = threadIdx.x; q < 64 * 64; q += blockDim.x) {
+    const int t = q % 64, r = q / 64;
+    cut8(tile[t][r], S, As + static_cast<size_t>(r0 + r) * Kp + t0 + t, stride);
+  }
+}
+
+// B slices Bs[s][col][t] (t fastest)
+__global__ void k_slice8_cols(int n, int K, int np, int Kp, int S, const double* B, size_t ldb, const double* tau,
+                              int8_t* Bs) {
+  const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= static_cast<size_t>(np) * Kp) return;
+  const int t = static_cast<int>(q % Kp), j = static_cast<int>(q / Kp);
+  const double x = (j < n && t < K) ? B[t + ldb * j] / tau[j] : 0.0;
+  cut8(x, S, Bs + q, static_cast<size_t>(np) * Kp);
+}
+
+// C(m x n) (+)= A B computed as its transpose: MFMA rows are the n columns
+// of C (B slices), MFMA columns the m rows of C (A slices), so a wave's
+// output lanes run along the rows of C and the FP64 stores are coalesced.
+// Workgroup tile 64 (C columns) x 64 (C rows), 4 waves of 32 x 32
+// (2 x 2 MFMA 16x16x64 i8), K in chunks of 64 staged in LDS (rows padded to
+// 80 bytes: conflict-free 16-byte reads).  Every level L = s + r <= S-1 has
+// its own int32 accumulators; the epilogue recombines them in FP64.
+constexpr int kPad8 = 80;
+template <int S>
+__global__ __launch_bounds__(256) void k_ozaki8(int m, int n, int mp, int np, int Kp, const int8_t* __restrict__ As,
+                                                 const int8_t* __restrict__ Bs, const double* __restrict__ sig,
+                                                 const double* __restrict__ tau, double* __restrict__ C, size_t ldc,
+                                                 int accumulate) {
+  __shared__ __attribute__((aligned(16))) int8_t sB[S][64][kPad8];  // C columns (blocks)
+  __shared__ __attribute__((aligned(16))) int8_t sA[S][64][kPad8];  // C rows (positions)
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;   // C column / row tile origins
+  const int wc = (wave & 1) * 32, wr = (wave >> 1) * 32;  // wave tile within: columns, rows
+  const int li = lane & 15, lk = (lane >> 4) * 16;
+  i32x4 acc[S][2][2];
+#pragma unroll
+  for (int L = 0; L < S; ++L)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[L][a][b] = i32x4{0, 0, 0, 0};
+  const size_t sa = static_cast<size_t>(mp) * Kp, sb = static_cast<size_t>(np) * Kp;
+  const int pr = threadIdx.x >> 2, pk = (threadIdx.x & 3) * 16;  // staging: row, 16-byte piece
+  for (int kc = 0; kc < Kp; kc += 64) {
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      *reinterpret_cast<i32x4*>(&sB[s][pr][pk]) =
+          *reinterpret_cast<const i32x4*>(Bs + s * sb + static_cast<size_t>(c0 + pr) * Kp + kc + pk);
+      *reinterpret_cast<i32x4*>(&sA[s][pr][pk]) =
+          *reinterpret_cast<const i32x4*>(As + s * sa + static_cast<size_t>(r0 + pr) * Kp + kc + pk);
+    }
+    __syncthreads();
+    i32x4 av[S][2];
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int t = 0; t < 2; ++t) av[s][t] = *reinterpret_cast<const i32x4*>(&sA[s][wr + t * 16 + li][lk]);
+#pragma unroll
+    for (int r = 0; r < S; ++r) {
+      i32x4 bv[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) bv[t] = *reinterpret_cast<const i32x4*>(&sB[r][wc + t * 16 + li][lk]);
+#pragma unroll
+      for (int s = 0; s < S - r; ++s)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            acc[s + r][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bv[a], av[s][b], acc[s + r][a][b], 0, 0, 0);
+    }
+  }
+  // tile (a, b): MFMA row i = C column c0+wc+16a+i, MFMA column j = C row
+  // r0+wr+16b+j; D[i][j] in lane j + 16*(i/4), register i%4
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int row = r0 + wr + b * 16 + li;
+      if (row >= m) continue;
+      const double sr = sig[row];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int col = c0 + wc + a * 16 + (lane >> 4) * 4 + e;
+        if (col >= n) continue;
+        double v = 0.0;
+#pragma unroll
+        for (int L = S - 1; L >= 0; --L) v += ldexp(static_cast<double>(acc[L][a][b][e]), -kBeta * (L + 2));
+        v *= sr * tau[col];
+        double* p = C + row + ldc * col;
+        *p = accumulate ? *p + v : v;
+      }
+    }
+}
+
+template <int S>
+void launch_k8(dim3 g, hipStream_t s, int m, int n, int mp, int np, int Kp, const int8_t* As, const int8_t* Bs,
+               const double* sig, const double* tau, double* C, size_t ldc, int acc) {
+  hipLaunchKernelGGL(k_ozaki8<S>, g, dim3(256), 0, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc);
+}
+
 template <int S>
 void launch_k(dim3 g, hipStream_t s, int m, int n, int mp, int np, int Kp, const __bf16* As, const __bf16* Bs,
               const double* sig, const double* tau, double* C, size_t ldc, int acc) {
@@ -273,6 +425,53 @@ int ozaki_dgemm(int m, int n, int K, const double* A, size_t lda, const double* 
     case 6: launch_k<6>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
     case 7: launch_k<7>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
     default: launch_k<8>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : fail(-100, "split-precision GEMM launch failed");
+}
+
+// A side of the int8 split GEMM: slices and row scales, kept while A does
+// not change.  Layout: As[S][mp][Kp] int8, then sig[mp] (256-byte aligned).
+size_t ozaki8_a_bytes(int m, int K, int S) {
+  const size_t mp = (static_cast<size_t>(m) + 63) / 64 * 64, Kp = (static_cast<size_t>(K) + 63) / 64 * 64;
+  return (S * mp * Kp + 255) / 256 * 256 + mp * sizeof(double);
+}
+
+size_t ozaki8_b_bytes(int n, int K, int S) { return ozaki8_a_bytes(n, K, S); }
+
+int ozaki8_prepare_a(int m, int K, const double* A, size_t lda, int S, void* abuf, hipStream_t s) {
+  if (S < 2 || S > 8) return fail(-9, "split precision: slices must be 2..8");
+  if (m <= 0 || K <= 0) return 0;
+  const int mp = (m + 63) / 64 * 64, Kp = (K + 63) / 64 * 64;
+  int8_t* As = static_cast<int8_t*>(abuf);
+  double* sig = reinterpret_cast<double*>(As + (static_cast<size_t>(S) * mp * Kp + 255) / 256 * 256);
+  hipLaunchKernelGGL(k_row_scale8, dim3((m + 255) / 256), dim3(256), 0, s, m, K, A, lda, sig);
+  hipLaunchKernelGGL(k_slice8_rows, dim3(mp / 64, Kp / 64), dim3(256), 0, s, m, K, mp, Kp, S, A, lda, sig, As);
+  return hipGetLastError() == hipSuccess ? 0 : fail(-100, "split-precision slicing launch failed");
+}
+
+int ozaki8_dgemm(int m, int n, int K, const void* abuf, const double* B, size_t ldb, double* C, size_t ldc,
+                 bool accumulate, int S, void* bbuf, hipStream_t s) {
+  if (S < 2 || S > 8) return fail(-9, "split precision: slices must be 2..8");
+  if (m <= 0 || n <= 0 || K <= 0) return 0;
+  const int mp = (m + 63) / 64 * 64, np = (n + 63) / 64 * 64, Kp = (K + 63) / 64 * 64;
+  const int8_t* As = static_cast<const int8_t*>(abuf);
+  const double* sig = reinterpret_cast<const double*>(As + (static_cast<size_t>(S) * mp * Kp + 255) / 256 * 256);
+  int8_t* Bs = static_cast<int8_t*>(bbuf);
+  double* tau = reinterpret_cast<double*>(Bs + (static_cast<size_t>(S) * np * Kp + 255) / 256 * 256);
+  hipLaunchKernelGGL(k_col_scale8, dim3((n + 255) / 256), dim3(256), 0, s, n, K, B, ldb, tau);
+  const size_t nb = static_cast<size_t>(np) * Kp;
+  hipLaunchKernelGGL(k_slice8_cols, dim3(static_cast<unsigned>((nb + 255) / 256)), dim3(256), 0, s, n, K, np, Kp, S,
+                     B, ldb, tau, Bs);
+  const dim3 g(np / 64, mp / 64);
+  const int acc = accumulate ? 1 : 0;
+  switch (S) {
+    case 2: launch_k8<2>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
+    case 3: launch_k8<3>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
+    case 4: launch_k8<4>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
+    case 5: launch_k8<5>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
+    case 6: launch_k8<6>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
+    case 7: launch_k8<7>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
+    default: launch_k8<8>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
   }
   return hipGetLastError() == hipSuccess ? 0 : fail(-100, "split-precision GEMM launch failed");
 }

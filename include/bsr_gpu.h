@@ -165,7 +165,8 @@ int bsrg_wave_launch(bsrg_ctx* ctx);
 int bsrg_wave_finish(bsrg_ctx* ctx);
 
 /* Engine of the last exchange contraction: 0 library FP64 GEMM, 1-3 FP64
- * MFMA kernel (32/64/128-row tiles), 10+S split precision with S slices. */
+ * MFMA kernel (32/64/128-row tiles), 10+S bf16 split precision and 20+S
+ * int8 split precision with S slices. */
 int bsrg_wave_engine(bsrg_ctx* ctx);
 
 /* Force the FP64 exchange-contraction engine (0..3 as above) or -1 to let
@@ -206,8 +207,10 @@ int bsrg_wave_diff(bsrg_ctx* a, bsrg_ctx* ref, int which, double floor_frac, dou
 int bsrg_drop_caches(bsrg_ctx* ctx);
 
 /* Precision of the exchange contraction for later bsrg_wave_assemble
- * calls: 0 = plain FP64 (default), 1 = split precision (Ozaki slices on
- * bf16 MFMA, FP32 accumulation, FP64 recombination) with nslices slices. */
+ * calls: 0 = plain FP64 (default); 1 = split precision, Ozaki slices on
+ * bf16 MFMA with FP32 accumulation and FP64 recombination; 2 = split
+ * precision, the same slices on int8 MFMA with exact int32 accumulation
+ * and FP64 recombination.  nslices = 0 chooses 6 (2..8 allowed). */
 int bsrg_set_precision(bsrg_ctx* ctx, int mode, int nslices);
 
 #ifdef __cplusplus

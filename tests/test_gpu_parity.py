@@ -252,22 +252,24 @@ def test_fortran_host_driver(oracle_lib, tmp_path):
     assert g["ec"] == o["ec"]
 
 
-@pytest.mark.parametrize("nslices", [6, 7])
-def test_split_precision_wave(ctx, oracle_lib, nslices):
-    """Exchange contraction as Ozaki slices on bf16 MFMA: every element of H
-    within 1e-10 (floor 1e-3 of max|H|) of the FP64 oracle; S unaffected."""
+@pytest.mark.parametrize("mode,nslices", [(1, 6), (1, 7), (2, 6), (2, 7)])
+def test_split_precision_wave(ctx, oracle_lib, mode, nslices):
+    """Exchange contraction as Ozaki slices on bf16 (mode 1) or int8 (mode 2)
+    MFMA: every element of H within 1e-10 (floor 1e-3 of max|H|) of the FP64
+    oracle; S unaffected."""
     O = oracle_lib
     from bsr_amd import synth
     w = synth.make_wave(71, 8, 120, 18.0, 40.0, nch=6, ncp=10, mk=6, nb=5, ntarg=5, fill=0.1, name="split")
     try:
-        ctx.set_precision(1, nslices)
+        ctx.set_precision(mode, nslices)
         ctx.wave_assemble(w)
         S, H = ctx.wave_get(0), ctx.wave_get(1)
+        assert ctx.wave_engine() == 10 * mode + nslices
     finally:
         ctx.set_precision(0, 0)
     oS, oH = O.run_wave(w)
     e = rel_err(H.hcc, oH.hcc)
-    print(f"[split] S={nslices} max rel err H.hcc {e:.2e}", flush=True)
+    print(f"[split] mode={mode} S={nslices} max rel err H.hcc {e:.2e}", flush=True)
     assert e <= 1e-10
     assert rel_err(S.hcc, oS.hcc) < TOL
     assert rel_err(H.hcb, oH.hcb) < TOL
