@@ -147,20 +147,29 @@ __global__ __launch_bounds__(256) void k_ozaki(int m, int n, int mp, int np, int
       }
     }
   }
-  // D[i][j] sits in lane j + 16*(i/4), register i%4 of each 16x16 tile
+  // D[i][j] sits in lane j + 16*(i/4), register i%4 of each 16x16 tile;
+  // all loads of the old values before any store
+#pragma unroll
+  for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < 2; ++tj) {
+      const int col = min(col0 + tj * 16 + lr, n - 1);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = min(row0 + ti * 16 + (lane >> 4) * 4 + e, m - 1);
+        acc64[ti][tj][e] *= sig[row] * tau[col];
+        if (accumulate) acc64[ti][tj][e] += C[row + ldc * col];
+      }
+    }
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
     for (int tj = 0; tj < 2; ++tj) {
       const int col = col0 + tj * 16 + lr;
-      if (col >= n) continue;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int row = row0 + ti * 16 + (lane >> 4) * 4 + e;
-        if (row >= m) continue;
-        const double v = acc64[ti][tj][e] * sig[row] * tau[col];
-        double* c = C + row + ldc * col;
-        *c = accumulate ? *c + v : v;
+        if (row < m && col < n) C[row + ldc * col] = acc64[ti][tj][e];
       }
     }
 }
@@ -199,19 +208,30 @@ __global__ __launch_bounds__(256) void k_dgemm(int m, int n, int K, const double
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
   }
+  // FP64 16x16x4 output: D[i][j] in lane j + 16*(i%4), register i/4; all
+  // loads of the old values before any store
+  if (accumulate) {
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int col = min(col0 + b * 16 + lr, n - 1);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = min(row0 + a * 16 + lk + 4 * e, m - 1);
+          acc[a][b][e] += C[row + ldc * col];
+        }
+      }
+  }
 #pragma unroll
   for (int a = 0; a < TM; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int col = col0 + b * 16 + lr;
-      if (col >= n) continue;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        // FP64 16x16x4 output: D[i][j] in lane j + 16*(i%4), register i/4
         const int row = row0 + a * 16 + lk + 4 * e;
-        if (row >= m) continue;
-        double* c = C + row + ldc * col;
-        *c = accumulate ? *c + acc[a][b][e] : acc[a][b][e];
+        if (row < m && col < n) C[row + ldc * col] = acc[a][b][e];
       }
     }
 }
@@ -234,14 +254,18 @@ __global__ void k_row_scale8(int m, int K, const double* A, size_t lda, double* 
   sig[i] = ldexp(1.0, e + 1);
 }
 
+// one wavefront per column: coalesced reads along K, shuffle reduction
 __global__ void k_col_scale8(int n, int K, const double* B, size_t ldb, double* tau) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (j >= n) return;
   double mx = 0.0;
-  for (int t = 0; t < K; ++t) mx = fmax(mx, fabs(B[t + ldb * j]));
-  int e = 0;
-  if (mx > 0.0) (void)frexp(mx, &e);
-  tau[j] = ldexp(1.0, e + 1);
+  for (int t = lane; t < K; t += 64) mx = fmax(mx, fabs(B[t + ldb * j]));
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if (lane == 0) {
+    int e = 0;
+    if (mx > 0.0) (void)frexp(mx, &e);
+    tau[j] = ldexp(1.0, e + 1);
+  }
 }
 
 __device__ __forceinline__ void cut8(double x, int S, int8_t* out, size_t stride) {
@@ -253,8 +277,15 @@ __device__ __forceinline__ void cut8(double x, int S, int8_t* out, size_t stride
   }
 }
 
-// A slices As[s][row][t] (t fastest, Kp padded, rows padded to mp): 64 rows
-// x 64 t per workgroup, transposed through LDS.
+// Slices are stored chunk-major: X[s][c][row][64] for K chunk c = t/64, so
+// the 64 rows x 64 bytes of one chunk that a workgroup stages are one
+// contiguous 4 KB block.
+__device__ __forceinline__ size_t at8(int c, int row, int t, int rows_p) {
+  return (static_cast<size_t>(c) * rows_p + row) * 64 + t;
+}
+
+// A slices (rows padded to mp, K to Kp): 64 rows x 64 t per workgroup,
+// transposed through LDS.
 __global__ void k_slice8_rows(int m, int K, int mp, int Kp, int S, const double* A, size_t lda, const double* sig,
                               int8_t* As) {
   __shared__ double tile[64][65];
@@ -268,18 +299,18 @@ __global__ void k_slice8_rows(int m, int K, int mp, int Kp, int S, const double*
   const size_t stride = static_cast<size_t>(mp) * Kp;
   for (int q = threadIdx.x; q < 64 * 64; q += blockDim.x) {
     const int t = q % 64, r = q / 64;
-    cut8(tile[t][r], S, As + static_cast<size_t>(r0 + r) * Kp + t0 + t, stride);
+    cut8(tile[t][r], S, As + at8(blockIdx.y, r0 + r, t, mp), stride);
   }
 }
 
-// B slices Bs[s][col][t] (t fastest)
+// B slices (columns padded to np, K to Kp), same chunk-major layout
 __global__ void k_slice8_cols(int n, int K, int np, int Kp, int S, const double* B, size_t ldb, const double* tau,
                               int8_t* Bs) {
   const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= static_cast<size_t>(np) * Kp) return;
   const int t = static_cast<int>(q % Kp), j = static_cast<int>(q / Kp);
   const double x = (j < n && t < K) ? B[t + ldb * j] / tau[j] : 0.0;
-  cut8(x, S, Bs + q, static_cast<size_t>(np) * Kp);
+  cut8(x, S, Bs + at8(t / 64, j, t % 64, np), static_cast<size_t>(np) * Kp);
 }
 
 // C(m x n) (+)= A B computed as its transpose: MFMA rows are the n columns
@@ -310,16 +341,31 @@ __global__ __launch_bounds__(256) void k_ozaki8(int m, int n, int mp, int np, in
       for (int b = 0; b < 2; ++b) acc[L][a][b] = i32x4{0, 0, 0, 0};
   const size_t sa = static_cast<size_t>(mp) * Kp, sb = static_cast<size_t>(np) * Kp;
   const int pr = threadIdx.x >> 2, pk = (threadIdx.x & 3) * 16;  // staging: row, 16-byte piece
+  // software pipeline: the next chunk's global loads are in flight while
+  // the current chunk is multiplied out of LDS; the 64 rows of a chunk are
+  // one contiguous 4 KB block per slice
+  i32x4 gb[S], ga[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    gb[s] = *reinterpret_cast<const i32x4*>(Bs + s * sb + at8(0, c0, 0, np) + threadIdx.x * 16);
+    ga[s] = *reinterpret_cast<const i32x4*>(As + s * sa + at8(0, r0, 0, mp) + threadIdx.x * 16);
+  }
   for (int kc = 0; kc < Kp; kc += 64) {
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      *reinterpret_cast<i32x4*>(&sB[s][pr][pk]) =
-          *reinterpret_cast<const i32x4*>(Bs + s * sb + static_cast<size_t>(c0 + pr) * Kp + kc + pk);
-      *reinterpret_cast<i32x4*>(&sA[s][pr][pk]) =
-          *reinterpret_cast<const i32x4*>(As + s * sa + static_cast<size_t>(r0 + pr) * Kp + kc + pk);
+      *reinterpret_cast<i32x4*>(&sB[s][pr][pk]) = gb[s];
+      *reinterpret_cast<i32x4*>(&sA[s][pr][pk]) = ga[s];
     }
     __syncthreads();
+    if (kc + 64 < Kp) {
+      const int c = kc / 64 + 1;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        gb[s] = *reinterpret_cast<const i32x4*>(Bs + s * sb + at8(c, c0, 0, np) + threadIdx.x * 16);
+        ga[s] = *reinterpret_cast<const i32x4*>(As + s * sa + at8(c, r0, 0, mp) + threadIdx.x * 16);
+      }
+    }
     i32x4 av[S][2];
 #pragma unroll
     for (int s = 0; s < S; ++s)
@@ -340,24 +386,52 @@ __global__ __launch_bounds__(256) void k_ozaki8(int m, int n, int mp, int np, in
     }
   }
   // tile (a, b): MFMA row i = C column c0+wc+16a+i, MFMA column j = C row
-  // r0+wr+16b+j; D[i][j] in lane j + 16*(i/4), register i%4
+  // r0+wr+16b+j; D[i][j] in lane j + 16*(i/4), register i%4.  All loads of
+  // the old values are issued before any store.
+  double v[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int row = min(r0 + wr + b * 16 + li, m - 1);
+      const double sr = sig[row];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int col = min(c0 + wc + a * 16 + (lane >> 4) * 4 + e, n - 1);
+        double x = 0.0;
+#pragma unroll
+        for (int L = S - 1; L >= 0; --L) x += ldexp(static_cast<double>(acc[L][a][b][e]), -kBeta * (L + 2));
+        v[a][b][e] = x * (sr * tau[col]);
+      }
+    }
+  if (accumulate) {
+    double old[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = min(r0 + wr + b * 16 + li, m - 1);
+          const int col = min(c0 + wc + a * 16 + (lane >> 4) * 4 + e, n - 1);
+          old[a][b][e] = C[row + ldc * col];
+        }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[a][b][e] += old[a][b][e];
+  }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int row = r0 + wr + b * 16 + li;
-      if (row >= m) continue;
-      const double sr = sig[row];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int col = c0 + wc + a * 16 + (lane >> 4) * 4 + e;
-        if (col >= n) continue;
-        double v = 0.0;
-#pragma unroll
-        for (int L = S - 1; L >= 0; --L) v += ldexp(static_cast<double>(acc[L][a][b][e]), -kBeta * (L + 2));
-        v *= sr * tau[col];
-        double* p = C + row + ldc * col;
-        *p = accumulate ? *p + v : v;
+        if (row < m && col < n) C[row + ldc * col] = v[a][b][e];
       }
     }
 }
@@ -458,7 +532,7 @@ int ozaki8_dgemm(int m, int n, int K, const void* abuf, const double* B, size_t 
   const double* sig = reinterpret_cast<const double*>(As + (static_cast<size_t>(S) * mp * Kp + 255) / 256 * 256);
   int8_t* Bs = static_cast<int8_t*>(bbuf);
   double* tau = reinterpret_cast<double*>(Bs + (static_cast<size_t>(S) * np * Kp + 255) / 256 * 256);
-  hipLaunchKernelGGL(k_col_scale8, dim3((n + 255) / 256), dim3(256), 0, s, n, K, B, ldb, tau);
+  hipLaunchKernelGGL(k_col_scale8, dim3((n + 3) / 4), dim3(256), 0, s, n, K, B, ldb, tau);
   const size_t nb = static_cast<size_t>(np) * Kp;
   hipLaunchKernelGGL(k_slice8_cols, dim3(static_cast<unsigned>((nb + 255) / 256)), dim3(256), 0, s, n, K, np, Kp, S,
                      B, ldb, tau, Bs);
