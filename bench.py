@@ -53,6 +53,26 @@ def cpu_baseline():
                 sample=f"nch=10 ncp=50 ns=200 k<=10 (nhm={w.nhm}); {dt:.1f} s")
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes by the HIP runtime's copy kernel:
+    the reference reading for the HBM byte counters (FETCH_SIZE and
+    WRITE_SIZE of a kernel that reads and writes nbytes)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+
+
 def mean_dict(ds):
     return {k: round(sum(d[k] for d in ds) / len(ds), 4) for k in ds[0]}
 
@@ -149,6 +169,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=60)
     ap.add_argument("--batch-waves", type=int, default=48)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="finish with one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); partial waves shard as whole jobs "
@@ -219,6 +241,9 @@ def main():
     if not a.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()
+    if a.copy_mb:
+        copy_calibration(a.copy_mb << 20)
+        log(f"copied {a.copy_mb} MiB device to device")
     print(json.dumps(out), flush=True)
 
 

@@ -352,7 +352,12 @@ __global__ void k_direct_reduce(int ns, int ks, int npair, int nk, const double*
 template <int NB>
 __global__ void k_exchange_basis(int ns, int ks, const double* __restrict__ rkb,
                                  const double* __restrict__ P, double* __restrict__ E) {
-  const int y = blockIdx.x, kk = blockIdx.y;
+  // 1-D grid over (kk, y).  Neighbouring columns read the same band slabs;
+  // workgroups are dealt to the 8 XCDs round robin, so renumber them to put
+  // runs of consecutive columns on one XCD (its L2).
+  const int nwg = static_cast<int>(gridDim.x), full = nwg - nwg % 8, hw = static_cast<int>(blockIdx.x);
+  const int id = hw < full ? (hw % 8) * (full / 8) + hw / 8 : hw;
+  const int y = id % ns, kk = id / ns;
   const size_t band = static_cast<size_t>(ns) * ns * ks * ks;
   const double* R = rkb + kk * band;
   __shared__ double pw[NB][2 * kMaxKs];
@@ -738,7 +743,7 @@ void launch_direct_basis(int ns, int ks, int nk, int npair, const double* rkb, c
 
 void launch_exchange_basis(int ns, int ks, int nk, int nb, const double* rkb, const double* P, double* E,
                            hipStream_t s) {
-  const dim3 grid(ns, nk), block(ns <= 64 ? 64 : (ns <= 128 ? 128 : 256));
+  const dim3 grid(ns * nk), block(ns <= 64 ? 64 : (ns <= 128 ? 128 : 256));
   switch (nb) {
 #define NB_CASE(N) \
   case N: hipLaunchKernelGGL(k_exchange_basis<N>, grid, block, 0, s, ns, ks, rkb, P, E); break;

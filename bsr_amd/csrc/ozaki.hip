@@ -329,7 +329,13 @@ __global__ __launch_bounds__(256) void k_ozaki8(int m, int n, int mp, int np, in
   __shared__ __attribute__((aligned(16))) int8_t sB[S][64][kPad8];  // C columns (blocks)
   __shared__ __attribute__((aligned(16))) int8_t sA[S][64][kPad8];  // C rows (positions)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int c0 = blockIdx.x * 64, r0 = blockIdx.y * 64;   // C column / row tile origins
+  // 1-D grid over (row tile, column tile).  Workgroups are dealt to the 8
+  // XCDs round robin; renumber them so that the column tiles of one row
+  // tile, which share its A slices, run on one XCD and hit its L2.
+  const int nct = np / 64, nwg = static_cast<int>(gridDim.x), full = nwg - nwg % 8;
+  const int hw = static_cast<int>(blockIdx.x);
+  const int id = hw < full ? (hw % 8) * (full / 8) + hw / 8 : hw;
+  const int c0 = (id % nct) * 64, r0 = (id / nct) * 64;  // C column / row tile origins
   const int wc = (wave & 1) * 32, wr = (wave >> 1) * 32;  // wave tile within: columns, rows
   const int li = lane & 15, lk = (lane >> 4) * 16;
   i32x4 acc[S][2][2];
@@ -536,7 +542,7 @@ int ozaki8_dgemm(int m, int n, int K, const void* abuf, const double* B, size_t 
   const size_t nb = static_cast<size_t>(np) * Kp;
   hipLaunchKernelGGL(k_slice8_cols, dim3(static_cast<unsigned>((nb + 255) / 256)), dim3(256), 0, s, n, K, np, Kp, S,
                      B, ldb, tau, Bs);
-  const dim3 g(np / 64, mp / 64);
+  const dim3 g((np / 64) * (mp / 64));
   const int acc = accumulate ? 1 : 0;
   switch (S) {
     case 2: launch_k8<2>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
