@@ -77,6 +77,13 @@ def lib():
             "orc_wave_update_hv": (i, [vp, i, i, _dp, d]),
             "orc_wave_update_hb": (i, [vp, i, i, d]),
             "orc_wave_update_cf": (i, [vp, i, i, i, d]),
+            "orc_gauleg": (None, [d, d, _dp, _dp, i]),
+            "orc_dbs_new": (vp, [i, i, i, i, _dp]),
+            "orc_dbs_free": (None, [vp]),
+            "orc_dbs_tables": (None, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+            "orc_dbs_moments": (None, [vp, i, i, _dp]),
+            "orc_dbs_triang": (None, [vp, i, i, i, _dp]),
+            "orc_dbs_mrk": (None, [vp, i, i, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -124,6 +131,61 @@ def gauss(k):
     w = np.zeros(k, order="F")
     lib().orc_gauss(k, x, w)
     return x, w
+
+
+FAMILIES = ("pppp", "qqqq", "pqpq", "qpqp")
+
+
+def gauleg(n, x1=0.0, x2=1.0):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().orc_gauleg(x1, x2, x, w, n)
+    return x, w
+
+
+class DbsGrid:
+    """Dirac B-spline grid: knots t of order ks, P functions of order ksp
+    and Q functions of order ksq on the same intervals (DBS_gauss.f90)."""
+
+    def __init__(self, ks, ns, ksp, ksq, t):
+        self.ks, self.ns, self.ksp, self.ksq = ks, ns, ksp, ksq
+        self.nv = ns - ks + 1
+        self.nsp, self.nsq = self.nv + ksp - 1, self.nv + ksq - 1
+        self.t = np.asfortranarray(t, dtype=np.float64)
+        self._p = lib().orc_dbs_new(ks, ns, ksp, ksq, self.t)
+
+    def __del__(self):
+        try:
+            lib().orc_dbs_free(self._p)
+        except Exception:
+            pass
+
+    def tables(self):
+        nv, ks = self.nv, self.ks
+        gr, grw = np.zeros((nv, ks), order="F"), np.zeros((nv, ks), order="F")
+        pb = np.zeros((nv + 1, ks, self.ksp), order="F")
+        qb = np.zeros((nv + 1, ks, self.ksq), order="F")
+        tp, tq = np.zeros(self.nsp + self.ksp), np.zeros(self.nsq + self.ksq)
+        lib().orc_dbs_tables(self._p, gr, grw, pb, qb, tp, tq)
+        return dict(gr=gr, grw=grw, pbsp=pb, qbsp=qb, tp=tp, tq=tq)
+
+    def moments(self, q, k):
+        """rkm(ks*ks, nv) of basis q (0 P, 1 Q) for r^k (moment_pp/qq)."""
+        out = np.zeros((self.ks * self.ks, self.nv), order="F")
+        lib().orc_dbs_moments(self._p, q, k, out)
+        return out
+
+    def triang(self, family, k, iv):
+        kk = self.ks * self.ks
+        out = np.zeros((kk, kk), order="F")
+        lib().orc_dbs_triang(self._p, family, k, iv, out)
+        return out
+
+    def mrk(self, family, k):
+        """rkb(ns,ns,ks,ks) of family 0 pppp, 1 qqqq, 2 pqpq, 3 qpqp."""
+        ns, ks = self.ns, self.ks
+        out = np.zeros((ns, ns, ks, ks), order="F")
+        lib().orc_dbs_mrk(self._p, family, k, out)
+        return out
 
 
 class Grid:

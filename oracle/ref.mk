@@ -27,7 +27,31 @@ SRCS := \
 
 OBJS := $(addprefix $(OUT)/,$(notdir $(SRCS:.f90=.o)))
 
-all: $(OUT)/libbsr_ref.so
+# Dirac B-spline routines: a library of their own (routine names such as
+# convol and density repeat those of the BS library)
+DSRCS := \
+  $(REF)/libs/ZCOM/MOD_zconst.f90 $(REF)/libs/ZCOM/read_par.f90 $(REF)/libs/ZCOM/rrtc.f90 \
+  $(REF)/libs/ZCOM/check_file.f90 $(REF)/libs/ZCOM/gauss_points.f90 \
+  $(REF)/libs/DBS/DBS_nuclear.f90 $(REF)/libs/DBS/DBS_grid.f90 \
+  $(REF)/libs/DBS/DBS_gauss.f90 $(REF)/libs/DBS/DBS_moments.f90 \
+  $(REF)/libs/DBS/DBS_integrals.f90 $(REF)/libs/DBS/DBS_debug.f90 \
+  $(REF)/libs/DBS/zbsplvd.f90 $(REF)/libs/DBS/zinty.f90 \
+  $(REF)/libs/DBS/mrk_pppp.f90 $(REF)/libs/DBS/mrk_qqqq.f90 \
+  $(REF)/libs/DBS/mrk_pqpq.f90 $(REF)/libs/DBS/mrk_qpqp.f90 \
+  $(REF)/libs/DBS/convol.f90 $(REF)/libs/DBS/density.f90
+DOUT := $(OUT)/dbs
+DOBJS := $(addprefix $(DOUT)/,$(notdir $(DSRCS:.f90=.o)))
+
+all: $(OUT)/libbsr_ref.so $(OUT)/libdbs_ref.so
+
+$(OUT)/libdbs_ref.so: $(DSRCS) $(HERE)ref_dbs_driver.f90 $(HERE)ref.mk
+	mkdir -p $(DOUT)/mod
+	set -e; for f in $(DSRCS); do \
+	  $(FC) -O2 -fPIC -flto -module-dir $(DOUT)/mod -c $$f -o $(DOUT)/$$(basename $$f .f90).o; done
+	$(FC) -O2 -fPIC -flto -module-dir $(DOUT)/mod -c $(HERE)ref_dbs_driver.f90 -o $(DOUT)/ref_dbs_driver.o
+	echo '{ global: ref_*; local: *; };' > $(DOUT)/exports.map
+	$(FC) -flto -fuse-ld=lld -shared -o $@ $(DOBJS) $(DOUT)/ref_dbs_driver.o \
+	  -Wl,--version-script=$(DOUT)/exports.map
 
 $(OUT)/libbsr_ref.so: $(SRCS) $(HERE)ref_driver.f90 $(HERE)ref.mk
 	mkdir -p $(OUT)/mod

@@ -204,3 +204,65 @@ class RefMatrix:
         lib().ref_get_matrix(hcc.reshape(-1, order="F"), hcb.reshape(-1, order="F"),
                              hbb, acf.reshape(-1, order="F"))
         return hcc, hcb, hbb, acf
+
+
+# ---- Dirac B-spline routines: oracle/_ref/libdbs_ref.so ---------------------
+DBS_PATH = os.path.join(HERE, "_ref", "libdbs_ref.so")
+_D = None
+
+
+def dbs_available() -> bool:
+    return os.path.exists(DBS_PATH)
+
+
+def dbs_lib():
+    global _D
+    if _D is None:
+        L = C.CDLL(DBS_PATH)
+        i = C.c_int
+        sig = {
+            "ref_dbs_init": (None, [i, i, i, i, _dp]),
+            "ref_dbs_tables": (None, [_dp, _dp, _dp, _dp]),
+            "ref_dbs_moments": (None, [i, i, _dp]),
+            "ref_dbs_mrk": (None, [i, i, _dp]),
+            "ref_dbs_convol": (None, [i, i, i, _dp, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _D = L
+    return _D
+
+
+class RefDbs:
+    """The reference's DBS module state for one grid (one active at a time)."""
+
+    def __init__(self, ks, ns, ksp, ksq, t):
+        self.ks, self.ns, self.ksp, self.ksq = ks, ns, ksp, ksq
+        self.nv = ns - ks + 1
+        dbs_lib().ref_dbs_init(ks, ns, ksp, ksq, np.asfortranarray(t, dtype=np.float64))
+
+    def tables(self):
+        nv, ks = self.nv, self.ks
+        gr, grw = np.zeros((nv, ks), order="F"), np.zeros((nv, ks), order="F")
+        pb = np.zeros((nv + 1, ks, self.ksp), order="F")
+        qb = np.zeros((nv + 1, ks, self.ksq), order="F")
+        dbs_lib().ref_dbs_tables(gr, grw, pb, qb)
+        return dict(gr=gr, grw=grw, pbsp=pb, qbsp=qb)
+
+    def moments(self, q, k):
+        out = np.zeros((self.ks * self.ks, self.nv), order="F")
+        dbs_lib().ref_dbs_moments(q, k, out)
+        return out
+
+    def mrk(self, family, k):
+        out = np.zeros((self.ns, self.ns, self.ks, self.ks), order="F")
+        dbs_lib().ref_dbs_mrk(family, k, out)
+        return out
+
+    def convol(self, family, k, icase, d):
+        ns, ks = self.ns, self.ks
+        a = np.zeros((ns, ks if icase <= 2 else ns), order="F")
+        dbs_lib().ref_dbs_convol(family, k, icase, np.asfortranarray(d), a)
+        return a
