@@ -74,6 +74,40 @@ class Context:
         L.check(L.lib().bsrg_convol(self._p, k, icase, d, a))
         return a
 
+    # ---- Dirac B-spline bands (four families) ---------------------------------
+    def dbs_set_grid(self, ks, ns, ksp, ksq, t):
+        """Knots t(ns+ks) of order ks; P functions of order ksp, Q of order
+        ksq (DBS_grid / DBS_gauss module state)."""
+        L.check(L.lib().bsrg_dbs_set_grid(self._p, ks, ns, ksp, ksq, f64(t)))
+        self.dbs = (ks, ns, ksp, ksq)
+
+    def dbs_mrk(self, family, k):
+        """rkb(ns,ns,ks,ks) of family 0 pppp, 1 qqqq, 2 pqpq, 3 qpqp
+        (mrk_pppp.f90 and siblings)."""
+        ks, ns = self.dbs[0], self.dbs[1]
+        out = np.zeros((ns, ns, ks, ks), order="F")
+        L.check(L.lib().bsrg_dbs_mrk(self._p, family, k, out))
+        return out
+
+    def dbs_moments(self, q, k):
+        """Moments of basis q (0 P, 1 Q), packed pairs x cells: (r^k, r^-(k+1))."""
+        ks, ns, ksp, ksq = self.dbs
+        kb = ksq if q else ksp
+        nv, ikb = ns - ks + 1, kb * (kb + 1) // 2
+        a, b = np.zeros((ikb, nv), order="F"), np.zeros((ikb, nv), order="F")
+        L.check(L.lib().bsrg_dbs_moments(self._p, q, k, a, b))
+        return a, b
+
+    def dbs_convol(self, family, k, d, icase):
+        ks, ns = self.dbs[0], self.dbs[1]
+        shape = (ns, ks) if icase <= 2 else (ns, ns)
+        d = f64(d)
+        if d.shape != shape:
+            raise ValueError(f"density must have shape {shape}")
+        a = np.zeros(shape, order="F")
+        L.check(L.lib().bsrg_dbs_convol(self._p, family, k, icase, d, a))
+        return a
+
     # ---- one partial wave, call by call --------------------------------------
     def set_orbitals(self, pbs, iech, lbs, ibort, eps_ovl=1e-8):
         pbs = f64(pbs)

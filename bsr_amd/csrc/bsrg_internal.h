@@ -35,6 +35,16 @@ struct GridTables {
   std::vector<double> mrm(int mm) const;
 };
 
+// Dirac B-spline tables: Gauss points of the intervals (gauleg rule), the
+// P (q=0, order kb[0]=ksp) and Q (q=1, order kb[1]=ksq) bases: knots tb[q],
+// values val[q][(iv*ks+m)*kb[q]+ith] of spline iv+ith at point m of interval iv.
+void gauss_legendre_nr(int n, double x1, double x2, double* x, double* w);
+struct DbsTables {
+  int ks = 0, ns = 0, nv = 0, kb[2] = {0, 0}, nsb[2] = {0, 0};
+  std::vector<double> t, gx, gw, gr, grm, grw, tb[2], val[2];
+  void build(int ks, int ns, int ksp, int ksq, const double* t);
+};
+
 // ---- kernel launchers (kernels.hip) -------------------------------------
 struct DevGrid {
   int ks, ns, nv, ml, me;
@@ -88,6 +98,24 @@ void launch_convol_direct(int ns, int ks, int jcase, const double* rkb, const do
                           hipStream_t s);
 void launch_convol_exchange(int ns, int ks, const double* rkb, const double* d, double* a,
                             hipStream_t s);
+
+// cell-major copy: in[(b*nv+iv)*w+e] -> out[(b*w+e)*nv+iv] for b < nb
+void launch_cells_transpose(int nv, int w, int nb, const double* in, double* out, hipStream_t s);
+
+// ---- Dirac B-spline bands (dbs.hip) --------------------------------------
+struct DevDbs {
+  int ks, ns, nv, kb[2], nsb[2];
+  const double *t, *gx, *gw, *gr, *grm, *grw, *tb[2], *val[2];
+};
+// moments of basis q (0 P, 1 Q) for k < nk: mom[((kk*2+s)*nv+iv)*ikb+ii]
+// (s=0 r^k, s=1 r^-(k+1)) and their cell-major copy momT
+void launch_dbs_moments(const DevDbs& g, int q, int nk, double* mom, double* momT, hipStream_t s);
+// bands of family f (0 pppp, 1 qqqq, 2 pqpq, 3 qpqp) for k < nk: band kk at
+// rkb + kk*ns*ns*ks*ks.  mom1/momT1 of the r1 basis, mom2/momT2 of the r2
+// basis; scratch msum1/2 (nk*2*kb*ns), rkd and rdT (nk*nv*ik1*ik2).
+void launch_dbs_band(const DevDbs& g, int f, int nk, const double* mom1, const double* mom2, const double* momT1,
+                     const double* momT2, double* msum1, double* msum2, double* rkd, double* rdT, double* rkb,
+                     hipStream_t s);
 
 // FP64 MFMA GEMM (ozaki.hip): C = A*B (+C), column-major; tm = 1, 2, 4
 int mfma_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B, size_t ldb, double* C,

@@ -309,6 +309,17 @@ struct bsrg_ctx {
   std::vector<Terms*> partsS, partsH;        // term parts of S and H in list order
   int last_engine = -1, forced_engine = -1;
   DevVec<double> dtmp, dtmp2;
+
+  // Dirac B-spline grid and the resident bands of the four families
+  struct Dbs {
+    bool ok = false;
+    DbsTables T;
+    DevVec<double> t, gx, gw, gr, grm, grw, tb[2], val[2];
+    int mom_nk = 0;                       // moments of both bases for k < mom_nk
+    DevVec<double> mom[2], momT[2], msum[2][2];
+    int band_nk[4] = {0, 0, 0, 0};       // bands of family f for k < band_nk[f]
+    DevVec<double> rkd, rdT, rkb[4];
+  } dbs;
 };
 
 namespace {
@@ -1684,6 +1695,127 @@ int bsrg_wave_diff(bsrg_ctx* a, bsrg_ctx* ref, int which, double floor_frac, dou
   for (double v : part) r = std::max(r, v);
   for (size_t q = 0; q < B.hbb.size(); ++q) r = std::max(r, std::fabs(A.hbb[q] - B.hbb[q]) / std::max(std::fabs(B.hbb[q]), floor));
   *rel = r;
+  return 0;
+}
+
+namespace {
+
+DevDbs devdbs(bsrg_ctx* c) {
+  auto& D = c->dbs;
+  DevDbs g;
+  g.ks = D.T.ks; g.ns = D.T.ns; g.nv = D.T.nv;
+  for (int q = 0; q < 2; ++q) { g.kb[q] = D.T.kb[q]; g.nsb[q] = D.T.nsb[q]; g.tb[q] = D.tb[q].p; g.val[q] = D.val[q].p; }
+  g.t = D.t.p; g.gx = D.gx.p; g.gw = D.gw.p; g.gr = D.gr.p; g.grm = D.grm.p; g.grw = D.grw.p;
+  return g;
+}
+
+// moments of both bases for k < nk, then the bands of family f for k < nk
+int ensure_dbs_band(bsrg_ctx* c, int f, int nk) {
+  auto& D = c->dbs;
+  if (!D.ok) return fail(-3, "Dirac grid not set (bsrg_dbs_set_grid)");
+  if (f < 0 || f > 3) return fail(-1, "family must be 0 (pppp), 1 (qqqq), 2 (pqpq) or 3 (qpqp)");
+  if (nk <= D.band_nk[f]) return 0;
+  const int ks = D.T.ks, ns = D.T.ns, nv = D.T.nv;
+  const DevDbs g = devdbs(c);
+  if (D.mom_nk < nk) {
+    for (int q = 0; q < 2; ++q) {
+      const size_t ikb = static_cast<size_t>(D.T.kb[q]) * (D.T.kb[q] + 1) / 2;
+      HIPCHK(D.mom[q].resize(static_cast<size_t>(nk) * 2 * nv * ikb));
+      HIPCHK(D.momT[q].resize(D.mom[q].n));
+      launch_dbs_moments(g, q, nk, D.mom[q].p, D.momT[q].p, c->stream);
+    }
+    D.mom_nk = nk;
+  }
+  const int q1 = (f == 1 || f == 3) ? 1 : 0, q2 = (f == 1 || f == 2) ? 1 : 0;
+  const size_t ik1 = static_cast<size_t>(D.T.kb[q1]) * (D.T.kb[q1] + 1) / 2;
+  const size_t ik2 = static_cast<size_t>(D.T.kb[q2]) * (D.T.kb[q2] + 1) / 2;
+  HIPCHK(D.rkd.resize(static_cast<size_t>(nk) * nv * ik1 * ik2));
+  HIPCHK(D.rdT.resize(D.rkd.n));
+  for (int q = 0; q < 2; ++q)
+    for (int side = 0; side < 2; ++side) HIPCHK(D.msum[q][side].resize(static_cast<size_t>(nk) * 2 * ks * ns));
+  HIPCHK(D.rkb[f].resize(static_cast<size_t>(nk) * ns * ns * ks * ks));
+  launch_dbs_band(g, f, nk, D.mom[q1].p, D.mom[q2].p, D.momT[q1].p, D.momT[q2].p, D.msum[q1][0].p, D.msum[q2][1].p,
+                  D.rkd.p, D.rdT.p, D.rkb[f].p, c->stream);
+  HIPCHK(hipGetLastError());
+  D.band_nk[f] = nk;
+  return 0;
+}
+
+}  // namespace
+
+int bsrg_dbs_set_grid(bsrg_ctx* c, int ks, int ns, int ksp, int ksq, const double* t) {
+  if (int rc = check_ctx(c)) return rc;
+  if (ks < 2 || ks > 12 || ns < ks || !t) return fail(-1, "bsrg_dbs_set_grid: bad ks/ns (2 <= ks <= 12)");
+  if (ksp < 2 || ksp > ks || ksq < 2 || ksq > ks) return fail(-1, "bsrg_dbs_set_grid: ksp, ksq must be in 2..ks");
+  for (int i = 1; i < ns + ks; ++i)
+    if (t[i] < t[i - 1]) return fail(-1, "bsrg_dbs_set_grid: knots must be non-decreasing");
+  auto& D = c->dbs;
+  if (D.ok && D.T.ks == ks && D.T.ns == ns && D.T.kb[0] == ksp && D.T.kb[1] == ksq &&
+      std::equal(D.T.t.begin(), D.T.t.end(), t))
+    return 0;  // same grid: bands stay resident
+  HIPCHK(hipSetDevice(c->device));
+  D.T.build(ks, ns, ksp, ksq, t);
+  hipStream_t s = c->stream;
+  HIPCHK(D.t.upload(D.T.t.data(), D.T.t.size(), s));
+  HIPCHK(D.gx.upload(D.T.gx.data(), D.T.gx.size(), s));
+  HIPCHK(D.gw.upload(D.T.gw.data(), D.T.gw.size(), s));
+  HIPCHK(D.gr.upload(D.T.gr.data(), D.T.gr.size(), s));
+  HIPCHK(D.grm.upload(D.T.grm.data(), D.T.grm.size(), s));
+  HIPCHK(D.grw.upload(D.T.grw.data(), D.T.grw.size(), s));
+  for (int q = 0; q < 2; ++q) {
+    HIPCHK(D.tb[q].upload(D.T.tb[q].data(), D.T.tb[q].size(), s));
+    HIPCHK(D.val[q].upload(D.T.val[q].data(), D.T.val[q].size(), s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  D.ok = true;
+  D.mom_nk = 0;
+  for (int& n : D.band_nk) n = 0;
+  return 0;
+}
+
+int bsrg_dbs_mrk(bsrg_ctx* c, int family, int k, double* rkb) {
+  if (int rc = check_ctx(c)) return rc;
+  if (k < 0) return fail(-1, "bsrg_dbs_mrk: k < 0");
+  HIPCHK(hipSetDevice(c->device));
+  if (int rc = ensure_dbs_band(c, family, k + 1)) return rc;
+  const auto& D = c->dbs;
+  const size_t n = static_cast<size_t>(D.T.ns) * D.T.ns * D.T.ks * D.T.ks;
+  if (rkb) HIPCHK(hipMemcpyAsync(rkb, D.rkb[family].p + k * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int bsrg_dbs_moments(bsrg_ctx* c, int q, int k, double* rk, double* rkm) {
+  if (int rc = check_ctx(c)) return rc;
+  if (q < 0 || q > 1 || k < 0) return fail(-1, "bsrg_dbs_moments: q must be 0 (P) or 1 (Q), k >= 0");
+  HIPCHK(hipSetDevice(c->device));
+  if (int rc = ensure_dbs_band(c, 0, k + 1)) return rc;  // builds the moments of both bases
+  const auto& D = c->dbs;
+  const size_t ikb = static_cast<size_t>(D.T.kb[q]) * (D.T.kb[q] + 1) / 2, nv = D.T.nv;
+  const double* m = D.mom[q].p + static_cast<size_t>(k) * 2 * nv * ikb;
+  if (rk) HIPCHK(hipMemcpyAsync(rk, m, nv * ikb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (rkm) HIPCHK(hipMemcpyAsync(rkm, m + nv * ikb, nv * ikb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int bsrg_dbs_convol(bsrg_ctx* c, int family, int k, int icase, const double* d, double* a) {
+  if (int rc = check_ctx(c)) return rc;
+  if (icase < 1 || icase > 4) return fail(-2, " CONVOL:  unknown case ");
+  if (k < 0) return fail(-1, "bsrg_dbs_convol: k < 0");
+  HIPCHK(hipSetDevice(c->device));
+  if (int rc = ensure_dbs_band(c, family, k + 1)) return rc;
+  const auto& D = c->dbs;
+  const int ns = D.T.ns, ks = D.T.ks;
+  const size_t nin = icase <= 2 ? static_cast<size_t>(ns) * ks : static_cast<size_t>(ns) * ns;
+  HIPCHK(c->dtmp.upload(d, nin, c->stream));
+  HIPCHK(c->dtmp2.resize(nin));
+  const double* R = D.rkb[family].p + static_cast<size_t>(k) * ns * ns * ks * ks;
+  if (icase <= 2) launch_convol_direct(ns, ks, icase, R, c->dtmp.p, c->dtmp2.p, c->stream);
+  else launch_convol_exchange(ns, ks, R, c->dtmp.p, c->dtmp2.p, c->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(a, c->dtmp2.p, nin * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
 

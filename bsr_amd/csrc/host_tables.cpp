@@ -146,6 +146,69 @@ void bspline_values(const double* t, int ks, int left, double x, double* b) {
   }
 }
 
+// Gauss-Legendre rule of the Dirac B-spline library: the Numerical Recipes
+// method (gauleg, src/libs/ZCOM/gauss_points.f90:2-51) -- Newton from the
+// same first guess, stopped at |dz| <= 1e-14, the weight taken from the
+// derivative of the last Newton step.  The DBS integrals are defined with
+// exactly these nodes and weights.
+void gauss_legendre_nr(int n, double x1, double x2, double* x, double* w) {
+  const int m = (n + 1) / 2;
+  const double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+  for (int i = 1; i <= m; ++i) {
+    double z = std::cos(3.141592654 * (i - 0.25) / (n + 0.5)), z1, pp;
+    do {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / pp;
+    } while (std::fabs(z - z1) > 1.0e-14);
+    x[i - 1] = xm - xl * z;
+    x[n - i] = xm + xl * z;
+    w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+    w[n - i] = w[i - 1];
+  }
+}
+
+// P (q=0, order ksp) and Q (q=1, order ksq) bases on the knots of order ks
+// (Def_pbsp / Def_qbsp, src/libs/DBS/DBS_gauss.f90:153-252): knots
+// tb(1:kb) = t(1), tb(kb+1:nsb) = t(ks+1:ns), tb(nsb+1:nsb+kb) = t(ns+1:ns+kb),
+// nsb = nv+kb-1; values at the ks Gauss points of every interval.
+void DbsTables::build(int ks_, int ns_, int ksp, int ksq, const double* t_) {
+  ks = ks_; ns = ns_; nv = ns - ks + 1;
+  kb[0] = ksp; kb[1] = ksq;
+  t.assign(t_, t_ + ns + ks);
+  gx.resize(ks); gw.resize(ks);
+  gauss_legendre_nr(ks, 0.0, 1.0, gx.data(), gw.data());
+  gr.assign(static_cast<size_t>(nv) * ks, 0.0);
+  grm.assign(gr.size(), 0.0);
+  grw.assign(gr.size(), 0.0);
+  for (int iv = 0; iv < nv; ++iv) {
+    const double tl = t[iv + ks - 1], dt = t[iv + ks] - t[iv + ks - 1];
+    for (int m = 0; m < ks; ++m) {
+      gr[iv * ks + m] = dt * gx[m] + tl;
+      grm[iv * ks + m] = 1.0 / gr[iv * ks + m];
+      grw[iv * ks + m] = dt * gw[m];
+    }
+  }
+  for (int q = 0; q < 2; ++q) {
+    const int k = kb[q];
+    nsb[q] = nv + k - 1;
+    tb[q].assign(static_cast<size_t>(nsb[q]) + k, 0.0);
+    for (int i = 0; i < k; ++i) tb[q][i] = t[0];
+    for (int i = k; i < nsb[q]; ++i) tb[q][i] = t[i - k + ks];
+    for (int i = 0; i < k; ++i) tb[q][nsb[q] + i] = t[ns + i];
+    val[q].assign(static_cast<size_t>(nv) * ks * k, 0.0);
+    for (int iv = 0; iv < nv; ++iv)
+      for (int m = 0; m < ks; ++m)
+        bspline_values(tb[q].data(), k, iv + k, gr[iv * ks + m], &val[q][(static_cast<size_t>(iv) * ks + m) * k]);
+  }
+}
+
 void GridTables::build(int ks_, int ns_, int ml_, int me_, double h_, const double* t_) {
   ks = ks_; ns = ns_; nv = ns - ks + 1; ml = ml_; me = me_; h = h_;
   t.assign(t_, t_ + ns + ks);

@@ -712,6 +712,10 @@ void launch_moments(const DevGrid& g, int k0, int nk, double* mom, double* rkd, 
   }
 }
 
+void launch_cells_transpose(int nv, int w, int nb, const double* in, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_cells_transpose, dim3((w + 31) / 32, (nv + 31) / 32, nb), dim3(256), 0, s, nv, w, in, out);
+}
+
 void launch_band(const DevGrid& g, int nk, const double* mom, const double* rkd, double* msum, double* momT,
                  double* rdT, double* rkb, hipStream_t s) {
   const int ik = g.ks * (g.ks + 1) / 2;

@@ -71,6 +71,10 @@ SIGNATURES = {
     "bsrg_set_precision": [C.c_void_p, C.c_int, C.c_int],
     "bsrg_drop_caches": [C.c_void_p],
     "bsrg_wave_diff": [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)],
+    "bsrg_dbs_set_grid": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp],
+    "bsrg_dbs_mrk": [C.c_void_p, C.c_int, C.c_int, _dp],
+    "bsrg_dbs_moments": [C.c_void_p, C.c_int, C.c_int, _dp, _dp],
+    "bsrg_dbs_convol": [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp],
     "bsrg_wave_prepare": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
                           C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, _dp, _ip],
     "bsrg_wave_launch": [C.c_void_p],

@@ -73,6 +73,29 @@ int bsrg_rk_moments(bsrg_ctx* ctx, int k, double* rkd1, double* rkd2, double* rk
  * Replaces convol(ns,ks,a,d,icase,'s','s'), src/libs/BS/convol.f90:2-330. */
 int bsrg_convol(bsrg_ctx* ctx, int k, int icase, const double* d, double* a);
 
+/* ---- Dirac B-spline bands (dbs_mat) ---------------------------------------
+ * Knots t(ns+ks) of order ks (ks <= 12); P functions are B-splines of order
+ * ksp, Q functions of order ksq, on the same intervals.  Families:
+ * 0 PPPP, 1 QQQQ, 2 PQPQ (P pair on r1, Q pair on r2), 3 QPQP.  Bands stay
+ * resident while the grid is unchanged.
+ * Replaces the grid/spline module state of DBS_grid and DBS_gauss,
+ * src/libs/DBS/DBS_gauss.f90:72-252 (alloc_DBS_gauss, Def_pbsp, Def_qbsp). */
+int bsrg_dbs_set_grid(bsrg_ctx* ctx, int ks, int ns, int ksp, int ksq, const double* t);
+
+/* rkb(ns,ns,ks,ks) of family f for multipole k (entries beyond the orders
+ * are zero).  Replaces mrk_pppp / mrk_qqqq / mrk_pqpq / mrk_qpqp,
+ * src/libs/DBS/mrk_pppp.f90:2-77, mrk_qqqq.f90, mrk_pqpq.f90, mrk_qpqp.f90. */
+int bsrg_dbs_mrk(bsrg_ctx* ctx, int family, int k, double* rkb);
+
+/* Moments of basis q (0 P, 1 Q) over every cell, pairs i<=j packed:
+ * rk(ikb,nv) for r^k and rkm(ikb,nv) for r^-(k+1), ikb = kb(kb+1)/2.
+ * Replaces moments_pp / moments_qq, src/libs/DBS/DBS_moments.f90:146-244. */
+int bsrg_dbs_moments(bsrg_ctx* ctx, int q, int k, double* rk, double* rkm);
+
+/* convol(ns,ks,a,d,icase,'s','s') with the band of family f, multipole k.
+ * Replaces src/libs/DBS/convol.f90:2-268 for the symmetric cases. */
+int bsrg_dbs_convol(bsrg_ctx* ctx, int family, int k, int icase, const double* d, double* a);
+
 /* ---- one partial wave (a11-a15) ---------------------------------------- */
 
 /* Orbitals of the wave: pbs(ns,nbf); iech(i)=channel of orbital i (0 for
