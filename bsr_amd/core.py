@@ -98,6 +98,50 @@ class Context:
         L.check(L.lib().bsrg_dbs_moments(self._p, q, k, a, b))
         return a, b
 
+    def dirac_set_orbitals(self, pq, ipbs, i_ch, j_ch, v_ch):
+        """Two-component orbitals pq(ns,2,nbf), channel pointers ipbs, and the
+        <.|p> vectors V_ch(2ns,nv_ch) keyed by sorted (i_ch, j_ch)."""
+        ns = self.dbs[1]
+        pq = f64(pq)
+        nv = len(i_ch)
+        v = f64(np.asarray(v_ch, np.float64).reshape(2 * ns, -1)) if nv else np.zeros((2 * ns, 1), order="F")
+        L.check(L.lib().bsrg_dirac_set_orbitals(self._p, pq.shape[2], pq, i32(ipbs), nv, i32(i_ch if nv else [0]),
+                                                i32(j_ch if nv else [0]), v))
+
+    def dirac_assemble(self, nch, npert, mk, ntarg, iptar, itype, kpol, k1, k2, k3, k4, c):
+        """R^k part of a Dirac partial wave (R_data over the four families)."""
+        n = len(c)
+        z = np.zeros(1, np.int32)
+        L.check(L.lib().bsrg_dirac_assemble(
+            self._p, nch, npert, mk, ntarg, i32(iptar), n, *(i32(a) if n else z for a in (itype, kpol, k1, k2, k3, k4)),
+            f64(c) if n else np.zeros(1)))
+        self.dirac_dims = (nch, npert, mk, ntarg)
+
+    def dirac_wave(self, w):
+        """Grid, orbitals and lists of a synthetic Dirac wave (synth_dirac.DiracWave)."""
+        self.dbs_set_grid(w.ks, w.ns, w.ksp, w.ksq, w.t)
+        self.dirac_set_orbitals(w.pq, w.ipbs, w.i_ch, w.j_ch, w.v_ch)
+        self.dirac_assemble(w.nch, w.npert, w.mk, w.ntarg, w.iptar, *w.concat())
+
+    def dirac_get(self):
+        """hch(ms,ms,nblk), hcp(ms,nch*npert), hp, acf(nblk,0:mk), htarg."""
+        ns = self.dbs[1]
+        nch, npert, mk, ntarg = self.dirac_dims
+        ms, nblk = 2 * ns, nch * (nch + 1) // 2
+        hch = np.zeros((ms, ms, nblk), order="F")
+        hcp = np.zeros((ms, max(nch * npert, 1)), order="F")
+        hp = np.zeros(max(npert * (npert + 1) // 2, 1))
+        acf = np.zeros((nblk, mk + 1), order="F")
+        htarg = np.zeros(ntarg * (ntarg + 1) // 2)
+        L.check(L.lib().bsrg_dirac_get(self._p, hch, hcp, hp, acf, htarg))
+        return dict(hch=hch, hcp=hcp[:, : nch * npert], hp=hp[: npert * (npert + 1) // 2], acf=acf, htarg=htarg)
+
+    def dirac_timings(self):
+        """Device ms: bands+bases, terms, exchange GEMMs, output layout, total."""
+        out = np.zeros(5)
+        L.check(L.lib().bsrg_dirac_timings(self._p, out))
+        return out
+
     def dbs_convol(self, family, k, d, icase):
         ks, ns = self.dbs[0], self.dbs[1]
         shape = (ns, ks) if icase <= 2 else (ns, ns)

@@ -66,9 +66,10 @@ void launch_pair_dens(int ns, int ks, int nb, const double* P, double* dens, hip
 // part: scratch of nk*npair*ks*ns*ks doubles
 void launch_direct_basis(int ns, int ks, int nk, int npair, const double* rkb, const double* dens,
                          double* part, double* D, hipStream_t s);
-// exchange basis E[kk][a][b](x,y) = sum K(x,z,y,w) p_a(z) p_b(w)
+// exchange basis E[kk][a][b](x,y) = sum K(x,z,y,w) p_a(z) q_b(w), p = P (r1
+// pair), q = P2 (r2 pair)
 void launch_exchange_basis(int ns, int ks, int nk, int nb, const double* rkb, const double* P,
-                           double* E, hipStream_t s);
+                           const double* P2, double* E, hipStream_t s);
 // scalar table St[kk][p1][p2] = sum_band D[kk][p1]*dens[p2]
 void launch_stab(int ns, int ks, int nk, int npair, const double* D, const double* dens, double* St,
                  hipStream_t s);

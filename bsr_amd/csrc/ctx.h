@@ -1,0 +1,365 @@
+// Internal state of the C ABI: device/pinned buffers, term sets, staging
+// arenas, the host worker pool and the context struct.  Shared by capi.hip
+// (BS partial waves) and dirac.hip (Dirac partial waves).
+#pragma once
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <utility>
+#include <vector>
+
+#include "bsrg_internal.h"
+
+namespace bsrg {
+
+int fail(int code, const std::string& msg);
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail(-100, std::string("HIP error: ") + hipGetErrorString(e_) + " at " #x); \
+  } while (0)
+
+template <typename T>
+struct DevVec {
+  T* p = nullptr;
+  size_t n = 0;
+  DevVec() = default;
+  DevVec(const DevVec&) = delete;
+  DevVec& operator=(const DevVec&) = delete;
+  ~DevVec() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // Grows only; a regrowth allocates 25% headroom so that a run of jobs of
+  // slowly increasing size does not free and allocate (both synchronise)
+  // every time.
+  hipError_t resize(size_t m) {
+    if (m <= n && p) return hipSuccess;
+    const size_t want = n ? std::max(m, n + n / 4) : m;
+    release();
+    if (m == 0) return hipSuccess;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();  // the failed attempt must not surface later
+      p = nullptr;
+      e = hipMalloc(reinterpret_cast<void**>(&p), m * sizeof(T));
+      if (e == hipSuccess) n = m;
+      return e;
+    }
+    n = want;
+    return e;
+  }
+  hipError_t upload(const T* h, size_t m, hipStream_t s) {
+    hipError_t e = resize(m);
+    if (e != hipSuccess || m == 0) return e;
+    return hipMemcpyAsync(p, h, m * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+};
+
+struct Matrices {
+  int nch = 0, ncp = 0, mk = 0, ns = 0;
+  DevVec<double> hcc, hcb;
+  std::vector<double> hbb, acf, htarg, otarg;
+  size_t nblk() const { return static_cast<size_t>(nch) * (nch + 1) / 2; }
+  size_t hcc_n() const { return nblk() * ns * ns; }
+  size_t hcb_n() const { return static_cast<size_t>(ns) * nch * ncp; }
+  hipError_t alloc(int ns_, int nch_, int ncp_, int mk_) {
+    ns = ns_; nch = nch_; ncp = ncp_; mk = mk_;
+    hipError_t e = hcc.resize(std::max<size_t>(hcc_n(), 1));
+    if (e != hipSuccess) return e;
+    e = hcb.resize(std::max<size_t>(hcb_n(), 1));
+    hbb.assign(ncp > 0 ? static_cast<size_t>(ncp) * (ncp + 1) / 2 : 0, 0.0);
+    acf.assign(static_cast<size_t>(nch) * nch * (mk + 1), 0.0);
+    htarg.assign(nblk(), 0.0);
+    otarg.assign(nblk(), 0.0);
+    return e;
+  }
+};
+
+// Page-locked host buffer: uploads from it are asynchronous.  The owner
+// makes sure no copy from the old buffer is pending before it grows.
+template <typename T>
+struct PinnedVec {
+  T* p = nullptr;
+  size_t cap = 0;
+  PinnedVec() = default;
+  PinnedVec(const PinnedVec&) = delete;
+  PinnedVec& operator=(const PinnedVec&) = delete;
+  ~PinnedVec() { if (p) (void)hipHostFree(p); }
+  hipError_t reserve(size_t m) {
+    if (m <= cap) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max(m, cap + cap / 2);
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), want * sizeof(T), hipHostMallocDefault);
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+};
+
+// Term lists for one matrix set.
+struct Terms {
+  struct Band { int blk, bid; double c; };
+  struct R1 { int blk, u, w, sidx; double c; };
+  struct Vec { int q, v, sidx; double c; };
+  struct Scal { int kind, idx, sidx; double c; };  // kind 0 hbb, 1 htarg, 2 otarg
+  PinnedVec<double> gemm;    // nt x nblk, t = kk*nb*nb + a*nb + b (pinned: uploaded as is)
+  size_t gemm_n = 0;         // 0 while no exchange term was added
+  int gemm_klo = 1 << 30, gemm_khi = -1;
+  std::vector<Band> band;
+  std::vector<R1> r1;
+  std::vector<Vec> vec;
+  std::vector<Scal> scal;
+  std::vector<double> acf_add;  // same shape as acf
+  // A worker part of a parallel pass lists its exchange coefficients and
+  // ACF additions (index, value) instead of applying them; they are applied
+  // afterwards in part order, i.e. in list order.
+  bool deferred = false;
+  std::vector<std::pair<size_t, double>> gx, ax;
+  int err = 0;
+  std::string msg;
+  void clear() {
+    gemm_n = 0;
+    gemm_klo = 1 << 30;
+    gemm_khi = -1;
+    band.clear(); r1.clear(); vec.clear(); scal.clear(); acf_add.clear();
+    gx.clear(); ax.clear();
+    err = 0;
+    msg.clear();
+  }
+};
+
+// CSR arrays of one term set, sorted by output (list order kept within an
+// output), row pointers over every output, in one pinned host arena that
+// goes to the device in one copy.
+struct Staged {
+  PinnedVec<char> h;
+  DevVec<char> d;
+  DevVec<double> gemm;
+  hipEvent_t done = nullptr;  // the last upload from h / the term set's gemm has completed
+  bool pending = false;
+  std::vector<int> cnt[3];    // counting-sort scratch (band, rank-1, vector sets)
+  int nblk = 0, nq = 0;
+  size_t nband = 0, nr1 = 0, nvec = 0, bytes = 0;
+  // byte offsets of the arrays in the arena
+  size_t o_bc = 0, o_rc = 0, o_vc = 0, o_bptr = 0, o_bid = 0, o_rptr = 0, o_ru = 0, o_rw = 0, o_rs = 0, o_vptr = 0,
+         o_vid = 0, o_vs = 0;
+  bool has_gemm = false;
+  int gemm_klo = 0, gemm_khi = -1, engine = 2;
+  template <typename T>
+  T* hp(size_t o) { return reinterpret_cast<T*>(h.p + o); }
+  template <typename T>
+  const T* dp(size_t o) const { return reinterpret_cast<const T*>(d.p + o); }
+};
+
+// Fixed set of host worker threads owned by a context: run(n, f) calls
+// f(0..n-1), each index once, on the workers and the calling thread (indices
+// are taken from a shared counter), and returns when all are done.
+class Pool {
+ public:
+  explicit Pool(int nthreads) {
+    for (int t = 1; t < nthreads; ++t) th_.emplace_back([this] { loop(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& x : th_) x.join();
+  }
+  int size() const { return static_cast<int>(th_.size()) + 1; }
+  void run(int n, const std::function<void(int)>& f) {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      f_ = &f;
+      n_ = n;
+      next_.store(0);
+      busy_ = static_cast<int>(th_.size());
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (int i; (i = next_.fetch_add(1)) < n;) f(i);
+    std::unique_lock<std::mutex> lk(m_);
+    done_.wait(lk, [this] { return busy_ == 0; });
+    f_ = nullptr;
+  }
+
+ private:
+  void loop() {
+    unsigned seen = 0;
+    for (;;) {
+      const std::function<void(int)>* f;
+      int n;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        f = f_;
+        n = n_;
+      }
+      for (int i; (i = next_.fetch_add(1)) < n;) (*f)(i);
+      std::lock_guard<std::mutex> g(m_);
+      if (--busy_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* f_ = nullptr;
+  std::atomic<int> next_{0};
+  int n_ = 0, busy_ = 0;
+  unsigned gen_ = 0;
+  bool stop_ = false;
+};
+
+// A Dirac partial wave (R^k part of DBSR_MAT3).  Channel blocks are ms x ms
+// (ms = 2 ns: large then small component); on the device they are kept
+// quadrant-major, hq[q][blk] with q = 2a+b (a, b = 0 P, 1 Q) an ns x ns
+// block, so that the exchange GEMM of each family writes plain columns; the
+// output hch[blk] is the reference's ms x ms layout.  Blocks are numbered as
+// alloc_dbsr_matrix numbers them on one process: diagonal blocks first.
+struct DiracWave {
+  // orbitals
+  bool orb_ok = false;
+  int nbf = 0, nb = 0, npair = 0, nvch = 0;
+  std::vector<double> pq;      // (ns,2,nbf)
+  std::vector<int> ipbs, bidx; // bidx: orbital -> bound index, -1 for channel orbitals
+  std::vector<int> kcol;       // (nbf*nbf): 1-based V_ch column of (i,j), 0 if none
+  std::vector<double> vch;     // (ms, nvch)
+  DevVec<double> P;            // [c][b](x): component c of bound orbital b
+  DevVec<double> dens;         // [c][p](ns*ks): pair densities of component c
+  std::vector<std::vector<double>> Q;  // Q[k][a*nb+b] = quadr_pq(a,b,k), bound orbitals
+  // bases of the four families for k < basis_nk (exchange for k < exch_nk)
+  int basis_nk = 0, exch_nk = 0;
+  long e_ver = 0;
+  DevVec<double> D, St, VT, E, dpart;
+  PinnedVec<double> St_host;
+  // matrices
+  int nch = 0, npert = 0, mk = 0, ntarg = 0;
+  size_t nblk = 0;
+  std::vector<int> iptar;      // target of each channel
+  DevVec<double> hq, hv, hch, hcp, coef;
+  std::vector<double> hp, acf, htarg;
+  // lists of the last wave
+  Terms T;
+  Staged st;
+  struct XC { int blk, t; double c; };
+  std::vector<XC> xc;          // exchange coefficients (block, basis row, value) in list order
+  PinnedVec<char> xh;          // CSR of xc over blocks, uploaded in one copy
+  DevVec<char> xd;
+  size_t xbytes = 0, o_xptr = 0, o_xt = 0, o_xc = 0;
+  std::vector<int> xcnt;
+  int klo[4] = {0, 0, 0, 0}, khi[4] = {-1, -1, -1, -1};  // multipole range of each family's exchange terms
+  int engine = -1;
+  bool ok = false;
+  double ms[5] = {0, 0, 0, 0, 0};
+};
+
+}  // namespace bsrg
+
+using namespace bsrg;
+
+struct bsrg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  hipEvent_t ev[8] = {};
+  double timings[5] = {0, 0, 0, 0, 0};
+  // host ms of the last prepare: [0] checks and band/basis launches, [1] S and
+  // the overlap terms of H, [2] registry of GET_V vectors, [3] parallel list
+  // decoding, [4] merge of the parts, [5] sort and staging, [6] uploads and
+  // GEMM autotune, [7] total, [8] decoding time summed over the parts (cpu
+  // ms), [9] the longest part
+  double host_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int precision_mode = 0, nslices = 0;
+
+  GridTables G;
+  bool grid_ok = false;
+  DevVec<double> t, gx, gw, gr, grm, grw, bsp, sb;
+  int band_nk = 0;  // bands built for k = 0..band_nk-1
+  DevVec<double> rkb, mom, rkd, msum, momT, rdT;
+
+  // orbitals
+  bool orb_ok = false;
+  int nbf = 0, nb = 0, npair = 0;
+  std::vector<double> pbs, qbs, obs;
+  std::vector<double> Pbound;  // coefficients of the bound orbitals the bases were built for
+  std::vector<int> iech, lbs, ibort, bound, bidx;
+  double eps_ovl = 1e-8;
+  DevVec<double> P, dens;
+  // bases for k = 0..basis_nk-1
+  int basis_nk = 0, exch_nk = 0;
+  DevVec<double> bands;  // [sbu | D(kk,p)] each ns*ks
+  DevVec<double> E, St, VT, dpart;
+  PinnedVec<double> St_host;           // scalar table, read back for the host scalars
+  std::vector<std::vector<double>> Q;  // Q[k][(i-1)+nbf*(j-1)] = <p_i|r^k|p_j>, computed lazily
+  std::vector<char> Qok;
+  std::vector<int> wtab;                // GET_V registry (i1,i2) -> id, -1 if not yet made
+  std::vector<double> wvec;             // nW x ns
+  int wvec_uploaded = -1;
+
+  Matrices cur, S, H;
+  bool wave_ok = false, wave_prepared = false, capture_safe = false, capturing = false;
+  int wave_ntarg = 0;
+
+  // staged term uploads: S and H of the fused wave, and the per-call path
+  Staged stS, stH, stC;
+  DevVec<char> ozk;  // split-precision workspace (bf16 slices)
+  // int8 split precision: slices of the exchange bases, kept while the
+  // bases are unchanged (e_ver counts their rebuilds), and of the coefficients
+  DevVec<char> oz8a, oz8b;
+  long e_ver = 0, oz8a_ver = -1;
+  int oz8a_S = 0;
+  Terms TS, TH;      // term sets of the prepared wave (host scalars at finish)
+  Terms TC;          // term set of the per-call path
+  Terms THt;         // H terms after the lists: (E_t - ec) * sb on the diagonal
+  std::vector<std::unique_ptr<Terms>> work;  // worker parts of the parallel list pass
+  int nwork = 0;                             // worker parts used by the last prepare
+  std::unique_ptr<Pool> pool;                // host workers of the list pass
+  std::vector<Terms*> partsS, partsH;        // term parts of S and H in list order
+  int last_engine = -1, forced_engine = -1;
+  DevVec<double> dtmp, dtmp2;
+
+  // Dirac B-spline grid and the resident bands of the four families
+  struct Dbs {
+    bool ok = false;
+    DbsTables T;
+    DevVec<double> t, gx, gw, gr, grm, grw, tb[2], val[2];
+    int mom_nk = 0;                       // moments of both bases for k < mom_nk
+    DevVec<double> mom[2], momT[2], msum[2][2];
+    int band_nk[4] = {0, 0, 0, 0};       // bands of family f for k < band_nk[f]
+    DevVec<double> rkd, rdT, rkb[4];
+  } dbs;
+  DiracWave dw;
+};
+
+namespace bsrg {
+// Before a term set or its arena is rewritten: the last upload from them
+// has completed.
+int wait_staged(Staged& st);
+// Sort the terms of all parts by output block (nblk blocks) and vector
+// output (nq vectors), list order kept within an output, into the pinned
+// arena of st (CSR arrays; see Staged).
+int stage_terms(const std::vector<Terms*>& parts, Staged& st, int nblk, int nq, Pool* pool);
+}  // namespace bsrg
+
+namespace bsrg {
+// Dirac grid on the device and the bands of family f for k < nk (built on
+// the context's stream, kept resident).
+DevDbs devdbs(bsrg_ctx* c);
+int ensure_dbs_band(bsrg_ctx* c, int f, int nk);
+}  // namespace bsrg
+

@@ -346,12 +346,14 @@ __global__ void k_direct_reduce(int ns, int ks, int npair, int nk, const double*
 }
 
 // ---- a9 exchange basis ---------------------------------------------------
-// E[kk][a][b](x,y) = sum_{z,w} K(x,z,y,w) p_a(z) p_b(w),
-// K(x,z,y,w) = rkb(min(x,z), min(y,w), |x-z|, |y-w|).  One workgroup per
-// output column y; threads over rows x (coalesced band reads and writes).
+// E[kk][a][b](x,y) = sum_{z,w} K(x,z,y,w) p_a(z) q_b(w),
+// K(x,z,y,w) = rkb(min(x,z), min(y,w), |x-z|, |y-w|): p on the r1 pair (P1),
+// q on the r2 pair (P2; the same orbitals as P1 for the BS bands).  One
+// workgroup per output column y; threads over rows x (coalesced band reads
+// and writes).
 template <int NB>
-__global__ void k_exchange_basis(int ns, int ks, const double* __restrict__ rkb,
-                                 const double* __restrict__ P, double* __restrict__ E) {
+__global__ void k_exchange_basis(int ns, int ks, const double* __restrict__ rkb, const double* __restrict__ P,
+                                 const double* __restrict__ P2, double* __restrict__ E) {
   // 1-D grid over (kk, y).  Neighbouring columns read the same band slabs;
   // workgroups are dealt to the 8 XCDs round robin, so renumber them to put
   // runs of consecutive columns on one XCD (its L2).
@@ -363,7 +365,7 @@ __global__ void k_exchange_basis(int ns, int ks, const double* __restrict__ rkb,
   __shared__ double pw[NB][2 * kMaxKs];
   for (int q = threadIdx.x; q < NB * (2 * ks - 1); q += blockDim.x) {
     const int b = q / (2 * ks - 1), dw = q % (2 * ks - 1) - (ks - 1), w = y + dw;
-    pw[b][dw + ks - 1] = (w >= 0 && w < ns) ? P[static_cast<size_t>(b) * ns + w] : 0.0;
+    pw[b][dw + ks - 1] = (w >= 0 && w < ns) ? P2[static_cast<size_t>(b) * ns + w] : 0.0;
   }
   __syncthreads();
   const size_t s_ip = static_cast<size_t>(ns) * ns, s_jp = s_ip * ks;
@@ -745,12 +747,12 @@ void launch_direct_basis(int ns, int ks, int nk, int npair, const double* rkb, c
   hipLaunchKernelGGL(k_direct_reduce, dim3(nblocks(n, 256)), dim3(256), 0, s, ns, ks, npair, nk, part, D);
 }
 
-void launch_exchange_basis(int ns, int ks, int nk, int nb, const double* rkb, const double* P, double* E,
-                           hipStream_t s) {
+void launch_exchange_basis(int ns, int ks, int nk, int nb, const double* rkb, const double* P, const double* P2,
+                           double* E, hipStream_t s) {
   const dim3 grid(ns * nk), block(ns <= 64 ? 64 : (ns <= 128 ? 128 : 256));
   switch (nb) {
 #define NB_CASE(N) \
-  case N: hipLaunchKernelGGL(k_exchange_basis<N>, grid, block, 0, s, ns, ks, rkb, P, E); break;
+  case N: hipLaunchKernelGGL(k_exchange_basis<N>, grid, block, 0, s, ns, ks, rkb, P, P2, E); break;
     NB_CASE(1) NB_CASE(2) NB_CASE(3) NB_CASE(4) NB_CASE(5) NB_CASE(6) NB_CASE(7) NB_CASE(8)
 #undef NB_CASE
     default: break;

@@ -75,6 +75,11 @@ SIGNATURES = {
     "bsrg_dbs_mrk": [C.c_void_p, C.c_int, C.c_int, _dp],
     "bsrg_dbs_moments": [C.c_void_p, C.c_int, C.c_int, _dp, _dp],
     "bsrg_dbs_convol": [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp],
+    "bsrg_dirac_set_orbitals": [C.c_void_p, C.c_int, _dp, _ip, C.c_int, _ip, _ip, _dp],
+    "bsrg_dirac_assemble": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _ip, _ip, _ip, _ip,
+                            _ip, _ip, _dp],
+    "bsrg_dirac_get": [C.c_void_p, _odp, _odp, _odp, _odp, _odp],
+    "bsrg_dirac_timings": [C.c_void_p, _dp],
     "bsrg_wave_prepare": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
                           C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, _dp, _ip],
     "bsrg_wave_launch": [C.c_void_p],

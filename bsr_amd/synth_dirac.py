@@ -1,0 +1,239 @@
+"""Seeded synthetic Dirac partial waves in the reference's input formats.
+
+A Dirac partial wave is what DBSR_MAT3's SUB1 holds for the R^k pass: the
+knot grid with the orders of the large (P, order ksp) and small (Q, order
+ksq) component B-splines, the two-component orbitals pq(ns,2,nbf) with their
+channel pointers ipbs (0 for bound orbitals), the <.|p> vectors V_ch(ms,nv_ch)
+keyed by sorted (i_ch, j_ch) = (channel orbital, bound orbital), the target
+pointers iptar, and the sorted R^k lists of module c_data for itype 1..4 and
+every multipole k.  K1..K4 are packed as src/dbsr/DBSR_MAT3_MPI/
+idef_type.f90:168-186 describes (ibo = 2**15):
+
+  itype 1  K1=(i1,i3) K2=(i2,i4) bound pairs; K3,K4 = -ic,-jc (perturber
+           pair), io,-ic (overlap, perturber), io,jo (two overlaps) or io,0
+           (channel-orbital pair, target structure)
+  itype 2  K1=(j1,j2) bound pair, K2 bound orbital, K3 channel, K4 = io
+           (overlap) or -ic (perturber)
+  itype 3  K1=(ich,jch) ich>=jch, K2<=K3 bound orbitals    (direct)
+  itype 4  K1=(ich,jch) ich>=jch, K2,K3 bound orbitals      (exchange)
+
+Workload definition (one draw of numpy.random.default_rng(seed)):
+  grid        ks-fold knots at 0 and rmax, nv-1 interior knots geometric
+              from r0 = 0.05/z to rmax
+  bound       nb orbitals; P coefficients N(0,1) times a bump, Q = 0.1 x
+              N(0,1) times the same bump; coefficient 1 and the coefficients
+              past each component's function count are 0
+  channels    one channel orbital per channel (ipbs = ich), random pq;
+              target of a channel ~ U{1..ntarg}
+  V_ch        each (channel orbital, bound orbital) pair w.p. 0.7, a
+              vector 0.1 x N(0,1) of length ms = 2 ns
+  itype 3/4   every channel pair (i>=j) x bound pair (a<=b for itype 3,
+              ordered for itype 4) x k, w.p. fill, c ~ U[-1,1]
+  itype 1/2   records per structure w.p. fill with random bound orbitals
+              (perturber structures only when npert > 0)
+Duplicates (equal K1..K4) are merged by summing, as Merge_cdata does.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .synth import IBO, RecordList
+
+
+@dataclass
+class DiracList:
+    itype: int
+    kpol: int
+    k1: np.ndarray
+    k2: np.ndarray
+    k3: np.ndarray
+    k4: np.ndarray
+    c: np.ndarray
+
+    def __len__(self):
+        return len(self.c)
+
+
+@dataclass
+class DiracWave:
+    name: str
+    seed: int
+    ks: int
+    ns: int
+    ksp: int
+    ksq: int
+    t: np.ndarray              # (ns+ks,)
+    nb: int
+    nch: int
+    npert: int
+    mk: int
+    ntarg: int
+    pq: np.ndarray             # (ns, 2, nbf) Fortran order
+    ipbs: np.ndarray           # (nbf,)
+    i_ch: np.ndarray           # (nv_ch,) channel orbital (1-based)
+    j_ch: np.ndarray           # (nv_ch,) bound orbital (1-based)
+    v_ch: np.ndarray           # (ms, nv_ch) Fortran order
+    iptar: np.ndarray          # (nch,)
+    lists: list = field(default_factory=list)
+    _cat: tuple | None = None
+
+    @property
+    def ms(self): return 2 * self.ns
+
+    @property
+    def nbf(self): return self.pq.shape[2]
+
+    @property
+    def nhm(self): return self.nch * self.ms + self.npert
+
+    @property
+    def n_elements(self):
+        """Unique elements of the symmetric H of the R^k part."""
+        return self.nhm * (self.nhm + 1) // 2
+
+    @property
+    def n_records(self):
+        return sum(len(L) for L in self.lists)
+
+    def concat(self):
+        """All lists as one record set: itype, kpol, K1..K4, c (int32/float64)."""
+        if self._cat is None:
+            ls = [L for L in self.lists if len(L)]
+            if not ls:
+                z = np.zeros(0, np.int32)
+                self._cat = (z, z, z, z, z, z, np.zeros(0))
+            else:
+                self._cat = (
+                    np.concatenate([np.full(len(L), L.itype, np.int32) for L in ls]),
+                    np.concatenate([np.full(len(L), L.kpol, np.int32) for L in ls]),
+                    *(np.ascontiguousarray(np.concatenate([getattr(L, a) for L in ls]), dtype=np.int32)
+                      for a in ("k1", "k2", "k3", "k4")),
+                    np.ascontiguousarray(np.concatenate([L.c for L in ls]), dtype=np.float64),
+                )
+        return self._cat
+
+    def footprint_bytes(self):
+        """Device memory of one assembly: bands of four families, exchange
+        bases, channel blocks (quadrant-major and in the output layout),
+        channel-perturber vectors and the exchange coefficients."""
+        ns, ks, nk, nb = self.ns, self.ks, self.mk + 1, self.nb
+        nblk = self.nch * (self.nch + 1) // 2
+        bands = 4 * nk * ns * ns * ks * ks
+        exch = 4 * nk * nb * nb * ns * ns
+        blocks = 2 * nblk * self.ms * self.ms
+        vecs = 2 * self.ms * self.nch * self.npert
+        coef = 4 * nk * nb * nb * nblk
+        return 8 * (bands + exch + blocks + vecs + coef)
+
+
+def dirac_grid(ks: int, ns: int, z: float, rmax: float) -> np.ndarray:
+    nv = ns - ks + 1
+    inner = np.geomspace(0.05 / z, rmax, nv)[:-1]
+    return np.concatenate([np.zeros(ks), inner, np.full(ks, rmax)])
+
+
+def make_dirac_wave(seed: int, ks: int, ns: int, ksp: int, ksq: int, z: float, rmax: float, nb: int, nch: int,
+                    npert: int, mk: int, fill: float, ntarg: int = 3, name: str = "dirac") -> DiracWave:
+    rng = np.random.default_rng(seed)
+    nv = ns - ks + 1
+    nsp, nsq = nv + ksp - 1, nv + ksq - 1
+    t = dirac_grid(ks, ns, z, rmax)
+    nbf = nb + nch
+    pq = np.zeros((ns, 2, nbf), order="F")
+    x = np.linspace(0.0, 1.0, ns)
+    for o in range(nbf):
+        bump = np.exp(-((x - rng.uniform(0.05, 0.6)) / rng.uniform(0.05, 0.3)) ** 2)
+        pq[:, 0, o] = rng.standard_normal(ns) * bump
+        pq[:, 1, o] = 0.1 * rng.standard_normal(ns) * bump
+        pq[0, :, o] = 0.0
+        pq[nsp:, 0, o] = 0.0
+        pq[nsq:, 1, o] = 0.0
+    ipbs = np.zeros(nbf, np.int32)
+    ipbs[nb:] = np.arange(1, nch + 1)
+    iptar = rng.integers(1, ntarg + 1, nch).astype(np.int32)
+    ich, jch = [], []
+    for c in range(nch):
+        for b in range(nb):
+            if rng.random() < 0.7:
+                ich.append(nb + c + 1)
+                jch.append(b + 1)
+    nvch = len(ich)
+    v_ch = np.asfortranarray(0.1 * rng.standard_normal((2 * ns, nvch)))
+    ovl = list(zip(ich, jch))
+    bound = np.arange(1, nb + 1)
+    w = DiracWave(name, seed, ks, ns, ksp, ksq, t, nb, nch, npert, mk, ntarg, pq, ipbs,
+                  np.asarray(ich, np.int32), np.asarray(jch, np.int32), v_ch, iptar)
+
+    def pair():
+        a, b = sorted(rng.choice(bound, 2))
+        return int(a) * IBO + int(b)
+
+    def code(o):
+        return int(o[0]) * IBO + int(o[1])
+
+    for k in range(mk + 1):
+        for itype in (1, 2, 3, 4):
+            K1, K2, K3, K4, C = [], [], [], [], []
+
+            def add(a, b, c_, d):
+                K1.append(a); K2.append(b); K3.append(c_); K4.append(d)
+                C.append(rng.uniform(-1.0, 1.0))
+
+            if itype in (3, 4):
+                # every channel pair x bound pair, w.p. fill (vectorised draw)
+                ii, jj = np.tril_indices(nch)
+                if itype == 3:
+                    aa, bb = np.triu_indices(nb)
+                else:
+                    aa, bb = np.meshgrid(np.arange(nb), np.arange(nb), indexing="ij")
+                    aa, bb = aa.ravel(), bb.ravel()
+                keep = rng.random((len(ii), len(aa))) < fill
+                r, s = np.nonzero(keep)
+                k1 = (ii[r] + 1) * IBO + (jj[r] + 1)
+                rl = RecordList.build(k1, aa[s] + 1, bb[s] + 1, np.zeros(len(r)), rng.uniform(-1.0, 1.0, len(r)))
+                w.lists.append(DiracList(itype, k, rl.k1, rl.k2, rl.k3, rl.k4, rl.c))
+                continue
+            if itype == 1:
+                if npert:
+                    for ic in range(1, npert + 1):
+                        for jc in range(1, ic + 1):
+                            if rng.random() < fill:
+                                add(pair(), pair(), -ic, -jc)
+                    for o in ovl:
+                        for ic in range(1, npert + 1):
+                            if rng.random() < fill:
+                                add(pair(), pair(), code(o), -ic)
+                for p, o in enumerate(ovl):
+                    for q in range(p + 1):
+                        if rng.random() < fill:
+                            add(pair(), pair(), code(o), code(ovl[q]))
+                for c1 in range(nch):
+                    for c2 in range(c1 + 1):
+                        if rng.random() < fill:
+                            add(pair(), pair(), (nb + c1 + 1) * IBO + (nb + c2 + 1), 0)
+            else:
+                for c in range(1, nch + 1):
+                    for ic in range(1, npert + 1):
+                        if rng.random() < fill:
+                            add(pair(), int(rng.choice(bound)), c, -ic)
+                    for o in ovl:
+                        if rng.random() < fill:
+                            add(pair(), int(rng.choice(bound)), c, code(o))
+            rl = RecordList.build(K1, K2, K3, K4, C)
+            w.lists.append(DiracList(itype, k, rl.k1, rl.k2, rl.k3, rl.k4, rl.c))
+    return w
+
+
+def small(seed: int = 11, npert: int = 3) -> DiracWave:
+    """Parity case: ks=6, ns=24 (P order 5, Q order 6), 3 bound orbitals,
+    5 channels, k=0..2, fill 0.3."""
+    return make_dirac_wave(seed, 6, 24, 5, 6, 10.0, 20.0, 3, 5, npert, 2, 0.3, name="dirac_small")
+
+
+def c5(seed: int = 5, nch: int = 150, nb: int = 5, fill: float = 0.03) -> DiracWave:
+    """Dirac wave of the C5 shape: ks=9, ns=150 per component (P order 8,
+    Q order 9), Z=54, rmax=30, k=0..12, nch=150 (ms=300, nhm=45 000), no
+    perturbers, 5 bound orbitals, fill 3%."""
+    return make_dirac_wave(seed, 9, 150, 8, 9, 54.0, 30.0, nb, nch, 0, 12, fill, name="dirac_c5")

@@ -98,6 +98,45 @@ int bsrg_dbs_convol(bsrg_ctx* ctx, int family, int k, int icase, const double* d
 
 /* ---- one partial wave (a11-a15) ---------------------------------------- */
 
+/* ---- Dirac partial waves (DBSR_MAT3 R^k part) ------------------------------
+ * Grid from bsrg_dbs_set_grid.  Orbitals: pq(ns,2,nbf) large (1) and small
+ * (2) components, ipbs(i) = channel of orbital i (0 for bound orbitals), the
+ * <.|p> vectors V_ch(2ns,nv_ch) with their keys (i_ch, j_ch) = (channel
+ * orbital, bound orbital), strictly increasing as KBORT requires.  Replaces
+ * the module state R_data reads: DBS_orbitals_pq (pq, ipbs, V_ch, i_ch,
+ * j_ch; src/libs/DBS/DBS_orbitals_pq.f90:2-38, Get_v_ch :383-408). */
+int bsrg_dirac_set_orbitals(bsrg_ctx* ctx, int nbf, const double* pq, const int* ipbs, int nv_ch,
+                            const int* i_ch, const int* j_ch, const double* v_ch);
+
+/* The R^k part of one Dirac partial wave: every (itype, kpol) list through
+ * R_data for the four families PPPP, QQQQ, PQPQ, QPQP, as Gen_matrix does
+ * for icase 2 (src/dbsr/DBSR_MAT3_MPI/gen_matrix.f90:11-14, R_data.f90:2-290),
+ * accumulated from zero into hch(ms,ms,nblk), hcp(ms,nch*npert), hp,
+ * acf(nblk,0:mk), htarg with the UPDATE_* rules of updates.f90:79-289 and
+ * Target_h (target_subs.f90:2-31); ms = 2 ns, nblk = nch(nch+1)/2 numbered as
+ * alloc_dbsr_matrix does on one process (diagonal blocks first).  Records as
+ * in module c_data: record r has structure itype[r] (1..4) and multipole
+ * kpol[r]; records of one list are contiguous and in IPT order.  iptar(nch)
+ * maps channels to targets 1..ntarg.  iitar = 0 (target-structure records
+ * only update htarg).  Exchange records (itype 4) with ich < jch are
+ * rejected: the reference's list builder never writes them
+ * (idef_type.f90:268-296) and its UPDATE_HS would place their transpose in
+ * the untransposed quadrant. */
+int bsrg_dirac_assemble(bsrg_ctx* ctx, int nch, int npert, int mk, int ntarg, const int* iptar, int nrec,
+                        const int* itype, const int* kpol, const int* k1, const int* k2, const int* k3,
+                        const int* k4, const double* cdata);
+
+/* After bsrg_dirac_assemble: copy the accumulators to host arrays in the
+ * reference layout (any pointer may be NULL): hch(ms,ms,nblk),
+ * hcp(ms,nch*npert) with column (ich-1)*npert+ic, hp(npert(npert+1)/2),
+ * acf(nblk,0:mk), htarg(ntarg(ntarg+1)/2). */
+int bsrg_dirac_get(bsrg_ctx* ctx, double* hch, double* hcp, double* hp, double* acf, double* htarg);
+
+/* Device time (ms) of the last bsrg_dirac_assemble: [0] bands and bases,
+ * [1] band/rank-1/vector terms, [2] exchange GEMMs, [3] output layout,
+ * [4] total (list decoding and uploads included). */
+int bsrg_dirac_timings(bsrg_ctx* ctx, double* ms5);
+
 /* Orbitals of the wave: pbs(ns,nbf); iech(i)=channel of orbital i (0 for
  * bound orbitals); lbs(i)=l; ibort(nbf,nbf) nonzero where the overlap
  * <kl|nl> of channel orbital kl and bound orbital nl is kept.

@@ -29,9 +29,11 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "dbs_oracle.h"
+
 /* integer power by repeated squaring (Fortran real**integer) */
-static double dpow(double x, int n) {
-  if (n < 0) return 1.0 / dpow(x, -n);
+double orc_dpow(double x, int n) {
+  if (n < 0) return 1.0 / orc_dpow(x, -n);
   double r = 1.0, b = x;
   while (n) {
     if (n & 1) r *= b;
@@ -87,10 +89,6 @@ static void dbs_bvalues(const double* tb, int kb, int left, double x, double* b)
   }
 }
 
-typedef struct {
-  int ks, ns, nv, ksp, ksq, nsp, nsq;
-  double *t, *tp, *tq, *gx, *gw, *gr, *grm, *grw, *pbsp, *qbsp;
-} OrcDbs;
 
 void orc_dbs_free(OrcDbs* d) {
   if (!d) return;
@@ -176,8 +174,8 @@ void orc_dbs_moments(const OrcDbs* d, int q, int k, double* rkm) {
     for (int m = 0; m < ks; m++) {
       const int c = iv + nv * m;
       gw[m] = d->grw[c];
-      if (k > 0) gw[m] = d->grw[c] * dpow(d->gr[c], k);
-      if (k < 0) gw[m] = d->grw[c] * dpow(d->grm[c], -k);
+      if (k > 0) gw[m] = d->grw[c] * orc_dpow(d->gr[c], k);
+      if (k < 0) gw[m] = d->grw[c] * orc_dpow(d->grm[c], -k);
     }
     for (int j = 0; j < kb; j++)
       for (int m = 0; m < ks; m++) {
@@ -217,7 +215,7 @@ static void inner_ints(const OrcDbs* d, int q, int k, int iv, const double* x, c
       gx[i] = (r - xbase) * x[i] + xbase;
       dbs_bvalues(tb, kb, left, gx[i], bt[i]);
       gwv[i] = (r - xbase) * w[i];
-      gx[i] = gwv[i] * dpow(gx[i], k);
+      gx[i] = gwv[i] * orc_dpow(gx[i], k);
     }
     int jj = 0;
     for (int j = 0; j < kb; j++) {
@@ -238,7 +236,7 @@ static void outer_pairs(const OrcDbs* d, int q, int k, int iv, double outer[][32
   const double* tb; const double* v; int kb;
   basis_of(d, q, &tb, &v, &kb);
   double g[32];
-  for (int m = 0; m < ks; m++) g[m] = d->grw[iv + nv * m] * dpow(d->grm[iv + nv * m], k + 1);
+  for (int m = 0; m < ks; m++) g[m] = d->grw[iv + nv * m] * orc_dpow(d->grm[iv + nv * m], k + 1);
   int ii = 0;
   for (int i = 0; i < kb; i++)
     for (int ip = i; ip < kb; ip++) {

@@ -84,6 +84,13 @@ def lib():
             "orc_dbs_moments": (None, [vp, i, i, _dp]),
             "orc_dbs_triang": (None, [vp, i, i, i, _dp]),
             "orc_dbs_mrk": (None, [vp, i, i, _dp]),
+            "orc_dbsr_new": (vp, [vp, i, _dp, _ip, i, _ip, _ip, _dp, i, i, i, i, _ip]),
+            "orc_dbsr_free": (None, [vp]),
+            "orc_dbsr_sizes": (None, [vp, _ip]),
+            "orc_dbsr_get": (None, [vp, _dp, _dp, _dp, _dp, _dp]),
+            "orc_dbsr_rdata": (i, [vp, i, i, i, _ip, _ip, _ip, _ip, _dp]),
+            "orc_dbsr_quadr_pq": (d, [vp, i, i, i]),
+            "orc_dbsr_last_error": (C.c_char_p, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -186,6 +193,69 @@ class DbsGrid:
         out = np.zeros((ns, ns, ks, ks), order="F")
         lib().orc_dbs_mrk(self._p, family, k, out)
         return out
+
+
+def _i32(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.int32).reshape(-1))
+
+
+class DbsrWave:
+    """Dirac R^k assembly (R_data over the four families) on one grid:
+    orbitals pq(ns,2,nbf), channel pointers ipbs, <.|p> vectors V_ch(ms,nv_ch)
+    keyed by sorted (i_ch, j_ch), target pointers iptar.  Accumulates
+    hch(ms,ms,iicc), hcp(ms,iicb), hp, acf(iicc,0:mk), htarg in the
+    reference's numbering (dbsr_oracle.c)."""
+
+    def __init__(self, grid: DbsGrid, pq, ipbs, i_ch, j_ch, v_ch, nch, npert, mk, ntarg, iptar):
+        self.grid, self.mk, self.ntarg = grid, mk, ntarg
+        pq = np.asfortranarray(pq, dtype=np.float64)
+        v = np.asfortranarray(np.asarray(v_ch, dtype=np.float64).reshape(2 * grid.ns, -1))
+        if v.size == 0:
+            v = np.zeros((2 * grid.ns, 1), order="F")
+        self._p = lib().orc_dbsr_new(grid._p, pq.shape[2], pq, _i32(ipbs), len(i_ch), _i32(list(i_ch) or [0]),
+                                     _i32(list(j_ch) or [0]), v, nch, npert, mk, ntarg, _i32(iptar))
+        sz = np.zeros(4, dtype=np.int32)
+        lib().orc_dbsr_sizes(self._p, sz)
+        self.iicc, self.iicb, self.iibb, self.ms = (int(x) for x in sz)
+
+    def __del__(self):
+        try:
+            lib().orc_dbsr_free(self._p)
+        except Exception:
+            pass
+
+    def rdata(self, itype, kpol, k1, k2, k3, k4, c):
+        rc = lib().orc_dbsr_rdata(self._p, itype, kpol, len(c), _i32(k1), _i32(k2), _i32(k3), _i32(k4),
+                                  np.asfortranarray(c, dtype=np.float64))
+        if rc:
+            raise RuntimeError(f"orc_dbsr_rdata {rc}: {lib().orc_dbsr_last_error().decode()}")
+
+    def quadr_pq(self, io, jo, m):
+        return lib().orc_dbsr_quadr_pq(self._p, io, jo, m)
+
+    def get(self):
+        ms = self.ms
+        hch = np.zeros((ms, ms, max(self.iicc, 1)), order="F")
+        hcp = np.zeros((ms, max(self.iicb, 1)), order="F")
+        hp = np.zeros(max(self.iibb, 1), order="F")
+        acf = np.zeros((max(self.iicc, 1), self.mk + 1), order="F")
+        htarg = np.zeros(max(self.ntarg * (self.ntarg + 1) // 2, 1), order="F")
+        lib().orc_dbsr_get(self._p, hch.reshape(-1, order="F"), hcp.reshape(-1, order="F"), hp,
+                           acf.reshape(-1, order="F"), htarg)
+        return dict(hch=hch[:, :, : self.iicc], hcp=hcp[:, : self.iicb], hp=hp[: self.iibb],
+                    acf=acf[: self.iicc], htarg=htarg[: self.ntarg * (self.ntarg + 1) // 2])
+
+
+def dirac_assemble(w):
+    """Oracle run of a synthetic Dirac wave (bsr_amd.synth.DiracWave): every
+    (itype, kpol) list through R_data for the four families, in list order."""
+    g = DbsGrid(w.ks, w.ns, w.ksp, w.ksq, w.t)
+    o = DbsrWave(g, w.pq, w.ipbs, w.i_ch, w.j_ch, w.v_ch, w.nch, w.npert, w.mk, w.ntarg, w.iptar)
+    for L in w.lists:
+        o.rdata(L.itype, L.kpol, L.k1, L.k2, L.k3, L.k4, L.c)
+    out = o.get()
+    out["_grid"] = g
+    return out
 
 
 class Grid:

@@ -27,8 +27,9 @@ SRCS := \
 
 OBJS := $(addprefix $(OUT)/,$(notdir $(SRCS:.f90=.o)))
 
-# Dirac B-spline routines: a library of their own (routine names such as
-# convol and density repeat those of the BS library)
+# Dirac B-spline routines and the Dirac R^k assembly of DBSR_MAT3 (R_data,
+# updates, Target_h with the modules they read): a library of their own
+# (routine names such as convol and density repeat those of the BS library)
 DSRCS := \
   $(REF)/libs/ZCOM/MOD_zconst.f90 $(REF)/libs/ZCOM/read_par.f90 $(REF)/libs/ZCOM/rrtc.f90 \
   $(REF)/libs/ZCOM/check_file.f90 $(REF)/libs/ZCOM/gauss_points.f90 \
@@ -38,19 +39,27 @@ DSRCS := \
   $(REF)/libs/DBS/zbsplvd.f90 $(REF)/libs/DBS/zinty.f90 \
   $(REF)/libs/DBS/mrk_pppp.f90 $(REF)/libs/DBS/mrk_qqqq.f90 \
   $(REF)/libs/DBS/mrk_pqpq.f90 $(REF)/libs/DBS/mrk_qpqp.f90 \
-  $(REF)/libs/DBS/convol.f90 $(REF)/libs/DBS/density.f90
+  $(REF)/libs/DBS/convol.f90 $(REF)/libs/DBS/density.f90 \
+  $(REF)/libs/DBS/DBS_orbitals_pq.f90 $(REF)/libs/DBS/quadr_pq.f90 \
+  $(REF)/libs/DBS/sum_amb.f90 $(REF)/libs/DBS/bav.f90 \
+  $(REF)/libs/ZCONFJJ/mod_target_jj.f90 $(REF)/libs/ZCONFJJ/mod_orb_jj.f90 \
+  $(REF)/libs/ZCONFJJ/mod_conf_jj.f90 $(REF)/libs/ZCONFJJ/mod_channel_jj.f90 \
+  $(REF)/dbsr/DBSR_MAT3_MPI/mod_dbsr_mat.f90 $(REF)/dbsr/DBSR_MAT3_MPI/MOD_c_data.f90 \
+  $(REF)/dbsr/DBSR_MAT3_MPI/updates.f90 $(REF)/dbsr/DBSR_MAT3_MPI/target_subs.f90 \
+  $(REF)/dbsr/DBSR_MAT3_MPI/R_data.f90
 DOUT := $(OUT)/dbs
 DOBJS := $(addprefix $(DOUT)/,$(notdir $(DSRCS:.f90=.o)))
 
 all: $(OUT)/libbsr_ref.so $(OUT)/libdbs_ref.so
 
-$(OUT)/libdbs_ref.so: $(DSRCS) $(HERE)ref_dbs_driver.f90 $(HERE)ref.mk
+$(OUT)/libdbs_ref.so: $(DSRCS) $(HERE)ref_dbs_driver.f90 $(HERE)ref_dbsr_driver.f90 $(HERE)ref.mk
 	mkdir -p $(DOUT)/mod
 	set -e; for f in $(DSRCS); do \
 	  $(FC) -O2 -fPIC -flto -module-dir $(DOUT)/mod -c $$f -o $(DOUT)/$$(basename $$f .f90).o; done
 	$(FC) -O2 -fPIC -flto -module-dir $(DOUT)/mod -c $(HERE)ref_dbs_driver.f90 -o $(DOUT)/ref_dbs_driver.o
+	$(FC) -O2 -fPIC -flto -module-dir $(DOUT)/mod -c $(HERE)ref_dbsr_driver.f90 -o $(DOUT)/ref_dbsr_driver.o
 	echo '{ global: ref_*; local: *; };' > $(DOUT)/exports.map
-	$(FC) -flto -fuse-ld=lld -shared -o $@ $(DOBJS) $(DOUT)/ref_dbs_driver.o \
+	$(FC) -flto -fuse-ld=lld -shared -o $@ $(DOBJS) $(DOUT)/ref_dbs_driver.o $(DOUT)/ref_dbsr_driver.o \
 	  -Wl,--version-script=$(DOUT)/exports.map
 
 $(OUT)/libbsr_ref.so: $(SRCS) $(HERE)ref_driver.f90 $(HERE)ref.mk

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PATH = os.path.join(HERE, "_ref", "libbsr_ref.so")
 _L = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
 
 
 def build() -> bool:
@@ -226,6 +227,9 @@ def dbs_lib():
             "ref_dbs_moments": (None, [i, i, _dp]),
             "ref_dbs_mrk": (None, [i, i, _dp]),
             "ref_dbs_convol": (None, [i, i, i, _dp, _dp]),
+            "ref_dbsr_setup": (None, [i, _dp, _ip, i, _ip, _ip, _dp, i, i, i, i, _ip, _ip]),
+            "ref_dbsr_rdata": (None, [i, i, i, _ip, _ip, _ip, _ip, _dp]),
+            "ref_dbsr_get": (None, [_dp, _dp, _dp, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -266,3 +270,50 @@ class RefDbs:
         a = np.zeros((ns, ks if icase <= 2 else ns), order="F")
         dbs_lib().ref_dbs_convol(family, k, icase, np.asfortranarray(d), a)
         return a
+
+
+def _i32(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.int32).reshape(-1))
+
+
+class RefDbsr:
+    """The reference's R_data (DBSR_MAT3) on the active DBS grid: module
+    state filled by ref_dbsr_setup, lists run through R_data(itype,kpol,0..3)
+    as Gen_matrix does, accumulators copied out by ref_dbsr_get."""
+
+    def __init__(self, pq, ipbs, i_ch, j_ch, v_ch, nch, npert, mk, ntarg, iptar):
+        pq = np.asfortranarray(pq, dtype=np.float64)
+        self.ns = pq.shape[0]
+        self.mk, self.ntarg, self.npert = mk, ntarg, npert
+        v = np.asfortranarray(np.asarray(v_ch, dtype=np.float64).reshape(2 * self.ns, -1))
+        if v.size == 0:
+            v = np.zeros((2 * self.ns, 1), order="F")
+        sz = np.zeros(4, dtype=np.int32)
+        dbs_lib().ref_dbsr_setup(pq.shape[2], pq, _i32(ipbs), len(i_ch), _i32(list(i_ch) or [0]),
+                                 _i32(list(j_ch) or [0]), v, nch, npert, mk, ntarg, _i32(iptar), sz)
+        self.iicc, self.iicb, self.iibb, self.ms = (int(x) for x in sz)
+
+    def rdata(self, itype, kpol, k1, k2, k3, k4, c):
+        dbs_lib().ref_dbsr_rdata(itype, kpol, len(c), _i32(k1), _i32(k2), _i32(k3), _i32(k4),
+                                 np.asfortranarray(c, dtype=np.float64))
+
+    def get(self):
+        ms = self.ms
+        hch = np.zeros((ms, ms, max(self.iicc, 1)), order="F")
+        hcp = np.zeros((ms, max(self.iicb, 1)), order="F")
+        hp = np.zeros(max(self.iibb, 1), order="F")
+        acf = np.zeros((max(self.iicc, 1), self.mk + 1), order="F")
+        htarg = np.zeros(max(self.ntarg * (self.ntarg + 1) // 2, 1), order="F")
+        dbs_lib().ref_dbsr_get(hch.reshape(-1, order="F"), hcp.reshape(-1, order="F"), hp,
+                               acf.reshape(-1, order="F"), htarg)
+        return dict(hch=hch[:, :, : self.iicc], hcp=hcp[:, : self.iicb], hp=hp[: self.iibb],
+                    acf=acf[: self.iicc], htarg=htarg[: self.ntarg * (self.ntarg + 1) // 2])
+
+
+def dirac_assemble(w):
+    """Reference run of a synthetic Dirac wave (bsr_amd.synth.DiracWave)."""
+    RefDbs(w.ks, w.ns, w.ksp, w.ksq, w.t)
+    r = RefDbsr(w.pq, w.ipbs, w.i_ch, w.j_ch, w.v_ch, w.nch, w.npert, w.mk, w.ntarg, w.iptar)
+    for L in w.lists:
+        r.rdata(L.itype, L.kpol, L.k1, L.k2, L.k3, L.k4, L.c)
+    return r.get()
