@@ -18,7 +18,15 @@ Beside the headline line's required fields, the JSON line carries:
            saturated capacity (all jobs queued at once), and the error of
            every split-precision job against its FP64 result;
   batch    48 uneven waves (nch ~ U[10,120], ns ~ U[100,250], k=0..8) as one
-           hipGraph: total elements/s of a replay, per-wave kernel time.
+           hipGraph: total elements/s of a replay, per-wave kernel time;
+  dirac    the Dirac (DBSR) R^k assembly of one C5-shaped wave: ks=9, ns=150
+           per component (P order 8, Q order 9), Z=54, k=0..12, 150 channels
+           (ms=300, nhm=45 000), 5 bound orbitals, lists at 3% fill: the four
+           R^k band families, their orbital contractions, list processing and
+           hch/acf/htarg complete on the device, FP64; elements/s of H, the
+           device phases, the exchange GEMM against the FP64 MFMA peak, and
+           the oracle on an 8-channel sample of the same shape (time and the
+           largest element error of the GPU result on that sample).
 """
 from __future__ import annotations
 
@@ -156,6 +164,64 @@ def batch_c3(nwaves):
         b.close()
 
 
+def rel_err(a, b, floor_frac=1e-3):
+    import numpy as np
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    if b.size == 0:
+        return 0.0
+    scale = np.abs(b).max()
+    if scale == 0.0:
+        return float(np.abs(a).max())
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor_frac * scale)))
+
+
+def dirac_c5(steps, warmup, with_cpu):
+    from bsr_amd import Context, synth_dirac as SD
+    w = SD.c5()
+    ctx = Context(0)
+    try:
+        def step():
+            # a new Dirac wave: bands of the four families, bases, lists
+            ctx.drop_caches()
+            ctx.dirac_wave(w)
+
+        for _ in range(warmup):
+            step()
+        dev = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+            dev.append(dict(zip(("bases", "terms", "gemm", "layout", "total", "host_decode", "host_stage", "call"),
+                                map(float, ctx.dirac_timings()))))
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        dv = mean_dict(dev)
+        nblk = w.nch * (w.nch + 1) // 2
+        flops = 4 * 2.0 * w.ns * w.ns * nblk * (w.mk + 1) * w.nb * w.nb
+        tf = flops / (dv["gemm"] * 1e-3) / 1e12 if dv["gemm"] > 0 else 0.0
+        out = {"workload": f"Dirac wave ks=9 ns=150 (ksp=8 ksq=9) k<=12 nch={w.nch} ms={w.ms} nhm={w.nhm} "
+                           f"nb={w.nb} fill=3%",
+               "records": w.n_records, "elements": w.n_elements, "ms_per_wave": ms,
+               "elements_per_s": w.n_elements / (ms * 1e-3), "device_ms": dv,
+               "gemm_tflops": tf, "gemm_frac_fp64_peak": tf / PEAK_FP64_MFMA, "gemm_flops": flops,
+               "device_bytes": w.footprint_bytes()}
+        if with_cpu:
+            from oracle import oracle as O
+            ws = SD.c5(nch=8)
+            ctx.dirac_wave(ws)
+            got = ctx.dirac_get()
+            t0 = time.perf_counter()
+            ref = O.dirac_assemble(ws)
+            dt = time.perf_counter() - t0
+            out["sample_max_rel_err_vs_oracle"] = max(rel_err(got[k], ref[k]) for k in
+                                                      ("hch", "hcp", "hp", "acf", "htarg"))
+            out["cpu_baseline"] = dict(value=ws.n_elements / dt, unit="elements/s", cores=1, kind="port",
+                                       sample=f"nch=8 of the same shape (nhm={ws.nhm}, {ws.n_records} records); "
+                                              f"{dt:.1f} s")
+        return out
+    finally:
+        ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +235,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=60)
     ap.add_argument("--batch-waves", type=int, default=48)
+    ap.add_argument("--no-dirac", action="store_true")
+    ap.add_argument("--dirac-only", action="store_true", help="only the Dirac leg (printed as its own JSON line)")
+    ap.add_argument("--dirac-steps", type=int, default=3)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="finish with one device-to-device copy of this many MiB (calibrates HBM byte counters)")
     a = ap.parse_args()
@@ -177,6 +246,11 @@ def main():
               "per GPU, so the per-GPU figure is the N=1 line.", file=sys.stderr)
         sys.exit(2)
     from bsr_amd import Context, synth
+    if a.dirac_only:
+        d = dirac_c5(a.dirac_steps, 1, not a.no_cpu)
+        log("dirac: " + json.dumps(d))
+        print(json.dumps({"dirac": d}), flush=True)
+        return
     log("generating workload")
     w = synth.c2(nch=a.nch, ncp=a.ncp)
     ctx = Context(0)
@@ -238,6 +312,10 @@ def main():
         log("batch of uneven waves")
         out["batch"] = batch_c3(a.batch_waves)
         log("batch: " + json.dumps(out["batch"]))
+    if not a.no_dirac:
+        log("dirac wave")
+        out["dirac"] = dirac_c5(a.dirac_steps, 1, not a.no_cpu)
+        log("dirac: " + json.dumps(out["dirac"]))
     if not a.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline()

@@ -137,8 +137,9 @@ class Context:
         return dict(hch=hch, hcp=hcp[:, : nch * npert], hp=hp[: npert * (npert + 1) // 2], acf=acf, htarg=htarg)
 
     def dirac_timings(self):
-        """Device ms: bands+bases, terms, exchange GEMMs, output layout, total."""
-        out = np.zeros(5)
+        """ms: device bands+bases, terms, exchange GEMMs, output layout, first
+        to last event; host decoding, staging+uploads, whole call."""
+        out = np.zeros(8)
         L.check(L.lib().bsrg_dirac_timings(self._p, out))
         return out
 

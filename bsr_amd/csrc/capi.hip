@@ -1563,6 +1563,11 @@ int bsrg_drop_caches(bsrg_ctx* c) {
   c->band_nk = 0;
   c->basis_nk = 0;
   c->exch_nk = 0;
+  // Dirac: moments and bands of the four families, bases, quadr_pq tables
+  c->dbs.mom_nk = 0;
+  for (int& n : c->dbs.band_nk) n = 0;
+  c->dw.basis_nk = c->dw.exch_nk = 0;
+  c->dw.Q.clear();
   return 0;
 }
 

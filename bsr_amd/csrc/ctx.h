@@ -245,7 +245,11 @@ struct DiracWave {
   // bases of the four families for k < basis_nk (exchange for k < exch_nk)
   int basis_nk = 0, exch_nk = 0;
   long e_ver = 0;
-  DevVec<double> D, St, VT, E, dpart;
+  DevVec<double> D, St, Vf, E, dpart;  // per family
+  // families summed where a record uses them with one coefficient: direct
+  // bands per quadrant Dc[h][k][p] (h = r1 component), scalars Sc[k][pB][pA]
+  // over the four families, vectors VT = [Vc[h][k][p][b] | V_ch halves]
+  DevVec<double> Dc, Sc, VT;
   PinnedVec<double> St_host;
   // matrices
   int nch = 0, npert = 0, mk = 0, ntarg = 0;
@@ -253,12 +257,21 @@ struct DiracWave {
   std::vector<int> iptar;      // target of each channel
   DevVec<double> hq, hv, hch, hcp, coef;
   std::vector<double> hp, acf, htarg;
-  // lists of the last wave
-  Terms T;
-  Staged st;
+  // lists of the last wave, decoded in parts (contiguous record ranges, in
+  // list order) on the host workers
   struct XC { int blk, t; double c; };
-  std::vector<XC> xc;          // exchange coefficients (block, basis row, value) in list order
-  PinnedVec<char> xh;          // CSR of xc over blocks, uploaded in one copy
+  struct Part {
+    Terms T;
+    std::vector<XC> xc;                         // exchange coefficients (block, basis row, value)
+    std::vector<std::pair<size_t, double>> ax;  // ACF additions (index, value)
+    int klo[4], khi[4];
+    int err = 0;
+    std::string msg;
+  };
+  std::vector<std::unique_ptr<Part>> parts;
+  int nparts = 0;
+  Staged st;
+  PinnedVec<char> xh;          // CSR of the exchange coefficients over blocks, uploaded in one copy
   DevVec<char> xd;
   size_t xbytes = 0, o_xptr = 0, o_xt = 0, o_xc = 0;
   std::vector<int> xcnt;
@@ -266,6 +279,7 @@ struct DiracWave {
   int engine = -1;
   bool ok = false;
   double ms[5] = {0, 0, 0, 0, 0};
+  double host_ms[3] = {0, 0, 0};  // list decoding, staging and uploads, whole call
 };
 
 }  // namespace bsrg

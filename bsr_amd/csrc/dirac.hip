@@ -18,12 +18,17 @@
 //   itype 2  rank-1 c*V (x) V_ch half, quadrants (b1, h)     (UPDATE_HW, HV)
 //   itype 1  S*c with S from St: V_ch (x) V_ch halves (HW), V_ch halves into
 //            hcp (HV), hp (HB), htarg (Target_h)
+// A record applies one coefficient to all four families, so the families
+// that land in the same place are summed once per orbital set: the direct
+// bands and vectors of PPPP + PQPQ (r1 = P) and QQQQ + QPQP (r1 = Q), and the
+// scalars of all four.
 // Diagonal channel blocks collect the raw sums and are symmetrised at the
 // end, (X + X^T)/2, which is what the halved updates of the reference
 // (updates.f90:166-173, :204-207) add up to.  Every output element is summed
 // by one thread in list order (no atomics).
 #include <rocblas/rocblas.h>
 
+#include <chrono>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -64,13 +69,39 @@ __global__ __launch_bounds__(256) void k_dirac_coef(int nt, const int* __restric
   }
 }
 
-// hch[blk] (ms x ms, column-major) from the quadrant-major hq; the diagonal
-// channel blocks (blk < nch) are symmetrised, (X + X^T)/2, with the
-// transposed tile staged in LDS.  grid (nblk, ceil(ms/32), ceil(ms/32)),
-// block 256 = 32 rows x 8 column groups.
+// Output layout, off-diagonal channel blocks: quadrant q = 2a+b of block
+// nch+bo (ns x ns, contiguous) to rows a*ns.., columns b*ns.. of the ms x ms
+// block.  Reads are contiguous; writes are contiguous within a column.
+// grid (4*(nblk-nch), ceil(ns*ns/1024)), block 256, 4 elements a thread.
+__global__ __launch_bounds__(256) void k_dirac_layout(int ns, int nch, size_t nblk, const double* __restrict__ hq,
+                                                      double* __restrict__ hch) {
+  const int ms = 2 * ns, nss = ns * ns;
+  const size_t noff = nblk - nch;
+  const int q = static_cast<int>(blockIdx.x / noff);
+  const size_t blk = nch + blockIdx.x % noff;
+  const int a = q >> 1, b = q & 1;
+  const double* src = hq + (static_cast<size_t>(q) * nblk + blk) * nss;
+  double* dst = hch + blk * ms * ms + a * ns + static_cast<size_t>(b) * ns * ms;
+  const int e0 = blockIdx.y * 1024 + threadIdx.x;
+  double v[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = e0 + 256 * u;
+    v[u] = e < nss ? src[e] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int e = e0 + 256 * u;
+    if (e < nss) dst[(e % ns) + static_cast<size_t>(e / ns) * ms] = v[u];
+  }
+}
+
+// Output layout, diagonal channel blocks (blk < nch), symmetrised:
+// (X + X^T)/2, the transposed tile staged in LDS.  grid (nch, ceil(ms/32),
+// ceil(ms/32)), block 256 = 32 rows x 8 column groups.
 constexpr int kFt = 32;
-__global__ __launch_bounds__(256) void k_dirac_finalize(int ns, int nch, size_t nblk, const double* __restrict__ hq,
-                                                        double* __restrict__ hch) {
+__global__ __launch_bounds__(256) void k_dirac_diag(int ns, size_t nblk, const double* __restrict__ hq,
+                                                    double* __restrict__ hch) {
   const int ms = 2 * ns;
   const size_t blk = blockIdx.x;
   const int r0 = blockIdx.z * kFt, c0 = blockIdx.y * kFt;
@@ -82,13 +113,6 @@ __global__ __launch_bounds__(256) void k_dirac_finalize(int ns, int nch, size_t 
   };
   double* out = hch + blk * ms * ms;
   const int r = r0 + tx;
-  if (blk >= static_cast<size_t>(nch)) {
-    for (int j = ty; j < kFt; j += 8) {
-      const int c = c0 + j;
-      if (r < ms && c < ms) out[r + static_cast<size_t>(ms) * c] = at(r, c);
-    }
-    return;
-  }
   __shared__ double tile[kFt][kFt + 1];
   for (int j = ty; j < kFt; j += 8) {  // tile[j][tx] = X(c0 + tx, r0 + j)
     const int rr = c0 + tx, cc = r0 + j;
@@ -99,6 +123,18 @@ __global__ __launch_bounds__(256) void k_dirac_finalize(int ns, int nch, size_t 
     const int c = c0 + j;
     if (r < ms && c < ms) out[r + static_cast<size_t>(ms) * c] = 0.5 * (at(r, c) + tile[tx][j]);
   }
+}
+
+// out = a + b (n doubles)
+__global__ void k_sum2(size_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + b[i];
+}
+
+// out = ((x0 + x1) + x2) + x3 with xf = in + f*stride (n doubles)
+__global__ void k_sum4(size_t n, size_t stride, const double* __restrict__ in, double* __restrict__ out) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = ((in[i] + in[i + stride]) + in[i + 2 * stride]) + in[i + 3 * stride];
 }
 
 // hcp(:, q) (ms) from the component-major hv[a][q](ns).  grid (nq), block 256
@@ -179,35 +215,44 @@ int dirac_bases(bsrg_ctx* c, int nk, bool exch) {
     if (int rc = ensure_dbs_band(c, f, nk)) return rc;
   const size_t nsk = static_cast<size_t>(ns) * ks;
   if (W.basis_nk < nk) {
-    HIPCHK(W.D.resize(4 * static_cast<size_t>(nk) * np * nsk + 1));
+    const size_t nD = static_cast<size_t>(nk) * np * nsk, nS = static_cast<size_t>(nk) * np * np;
+    const size_t nV = static_cast<size_t>(nk) * np * nb * ns;
+    HIPCHK(W.D.resize(4 * nD + 1));
+    HIPCHK(W.Dc.resize(2 * nD + 1));
+    HIPCHK(W.VT.resize(2 * nV + (2 * static_cast<size_t>(W.nvch) + 1) * ns));
     if (np > 0) {
       HIPCHK(W.dpart.resize(static_cast<size_t>(nk) * np * ks * nsk));
-      HIPCHK(W.St.resize(4 * static_cast<size_t>(nk) * np * np));
+      HIPCHK(W.St.resize(4 * nS));
+      HIPCHK(W.Sc.resize(nS));
+      HIPCHK(W.Vf.resize(4 * nV));
       for (int f = 0; f < 4; ++f) {
-        double* Df = W.D.p + static_cast<size_t>(f) * nk * np * nsk;
+        double* Df = W.D.p + f * nD;
         launch_direct_basis(ns, ks, nk, np, c->dbs.rkb[f].p, W.dens.p + kB2[f] * np * nsk, W.dpart.p, Df, s);
-        launch_stab(ns, ks, nk, np, Df, W.dens.p + kB1[f] * np * nsk, W.St.p + static_cast<size_t>(f) * nk * np * np, s);
+        launch_stab(ns, ks, nk, np, Df, W.dens.p + kB1[f] * np * nsk, W.St.p + f * nS, s);
+        launch_vtab(ns, ks, nk, np, nb, Df, W.P.p + static_cast<size_t>(kB1[f]) * nb * ns, W.Vf.p + f * nV, s);
       }
+      // families with the same r1 component land in the same quadrant with
+      // one coefficient: PPPP + PQPQ (r1 = P), QQQQ + QPQP (r1 = Q)
+      for (int h = 0; h < 2; ++h) {
+        hipLaunchKernelGGL(k_sum2, dim3(static_cast<unsigned>((nD + 255) / 256)), dim3(256), 0, s, nD,
+                           W.D.p + h * nD, W.D.p + (h + 2) * nD, W.Dc.p + h * nD);
+        hipLaunchKernelGGL(k_sum2, dim3(static_cast<unsigned>((nV + 255) / 256)), dim3(256), 0, s, nV,
+                           W.Vf.p + h * nV, W.Vf.p + (h + 2) * nV, W.VT.p + h * nV);
+      }
+      hipLaunchKernelGGL(k_sum4, dim3(static_cast<unsigned>((nS + 255) / 256)), dim3(256), 0, s, nS, nS, W.St.p,
+                         W.Sc.p);
     }
-    const size_t nVf = 4 * static_cast<size_t>(nk) * np * nb;
-    HIPCHK(W.VT.resize((nVf + 2 * static_cast<size_t>(W.nvch) + 1) * ns));
-    if (np > 0)
-      for (int f = 0; f < 4; ++f)
-        launch_vtab(ns, ks, nk, np, nb, W.D.p + static_cast<size_t>(f) * nk * np * nsk,
-                    W.P.p + static_cast<size_t>(kB1[f]) * nb * ns,
-                    W.VT.p + static_cast<size_t>(f) * nk * np * nb * ns, s);
     // the <.|p> vectors after the family vectors; half h of column m is
-    // vector nVf + 2m + h, which is V_ch's own layout
+    // vector 2*nk*np*nb + 2m + h, which is V_ch's own layout
     if (W.nvch)
-      HIPCHK(hipMemcpyAsync(W.VT.p + nVf * ns, W.vch.data(), W.vch.size() * sizeof(double), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemcpyAsync(W.VT.p + 2 * nV, W.vch.data(), W.vch.size() * sizeof(double), hipMemcpyHostToDevice, s));
     HIPCHK(hipGetLastError());
-    const size_t nst = 4 * static_cast<size_t>(nk) * np * np;
     if (np > 0) {
-      if (nst > W.St_host.cap) {
+      if (nS > W.St_host.cap) {
         HIPCHK(hipStreamSynchronize(s));
-        HIPCHK(W.St_host.reserve(nst));
+        HIPCHK(W.St_host.reserve(nS));
       }
-      HIPCHK(hipMemcpyAsync(W.St_host.p, W.St.p, nst * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipMemcpyAsync(W.St_host.p, W.Sc.p, nS * sizeof(double), hipMemcpyDeviceToHost, s));
     }
     W.basis_nk = nk;
   }
@@ -249,17 +294,17 @@ int chk_pair(const DiracWave& W, int ich, int jch, const char* who) {
 }
 
 // One (itype, kpol) list -> terms, for the four families (R_data.f90:88-281).
-int dirac_decode(bsrg_ctx* c, int itype, int k, int n, const int* K1, const int* K2, const int* K3, const int* K4,
-                 const double* cd) {
+int dirac_decode(bsrg_ctx* c, DiracWave::Part& P, int itype, int k, int n, const int* K1, const int* K2,
+                 const int* K3, const int* K4, const double* cd) {
   auto& W = c->dw;
-  Terms& T = W.T;
+  Terms& T = P.T;
   const int nb = W.nb, np = W.npair, nk = W.basis_nk, nkE = W.exch_nk, nch = W.nch, npert = W.npert;
   const size_t nblk = W.nblk;
   const int nq1 = nch * npert;
-  const int nVf = 4 * nk * np * nb;
+  const int nVc = 2 * nk * np * nb;  // combined vectors before the V_ch halves
   auto qblk = [nblk](int a, int b, size_t bl) { return static_cast<int>((2 * a + b) * nblk + bl); };
-  auto vid = [=](int f, int p, int b) { return ((f * nk + k) * np + p) * nb + b; };
-  auto sid = [=](int f, int pB, int pA) { return ((f * nk + k) * np + pB) * np + pA; };
+  auto vid = [=](int h, int p, int b) { return ((h * nk + k) * np + p) * nb + b; };
+  auto sid = [=](int pB, int pA) { return (k * np + pB) * np + pA; };
   int rc = 0;
   if (itype == 1) {  // R_data.f90:92-165
     for (int q = 0; q < n; ++q) {
@@ -271,16 +316,15 @@ int dirac_decode(bsrg_ctx* c, int itype, int k, int n, const int* K1, const int*
       if (ic > 0 && jc > 0) {
         if (ic > npert || jc > npert) return fail(-12, "UPDATE_HB: index is out of range");
         const int i = std::max(ic, jc), j = std::min(ic, jc);
-        for (int f = 0; f < 4; ++f) T.scal.push_back({0, i * (i - 1) / 2 + j - 1, sid(f, pB, pA), cc});
+        T.scal.push_back({0, i * (i - 1) / 2 + j - 1, sid(pB, pA), cc});
       } else if (ic < 0 && jc > 0) {
         const int io1 = io / kIbo, io2 = io % kIbo, m = vcol(W, io1, io2);
         if (m == 0) return fail(-19, "RK: KBORT =0");
         const int ich = chan_of(W, io1);
         if (ich < 1 || ich > nch) return fail(-13, "UPDATE_HV: channel index out is of range");
         if (jc > npert) return fail(-14, "UPDATE_HV: bound index out is of range");
-        for (int f = 0; f < 4; ++f)
-          for (int h = 0; h < 2; ++h)
-            T.vec.push_back({h * nq1 + (ich - 1) * npert + (jc - 1), nVf + 2 * (m - 1) + h, sid(f, pB, pA), cc});
+        for (int h = 0; h < 2; ++h)
+          T.vec.push_back({h * nq1 + (ich - 1) * npert + (jc - 1), nVc + 2 * (m - 1) + h, sid(pB, pA), cc});
       } else if (ic < 0 && jc < 0) {
         const int io1 = io / kIbo, io2 = io % kIbo, jo1 = jo / kIbo, jo2 = jo % kIbo;
         const int mv = vcol(W, io1, io2), mw = vcol(W, jo1, jo2);
@@ -288,13 +332,12 @@ int dirac_decode(bsrg_ctx* c, int itype, int k, int n, const int* K1, const int*
         const int ich = chan_of(W, io1), jch = chan_of(W, jo1);
         if ((rc = chk_pair(W, ich, jch, "UPDATE_HW"))) return rc;
         const size_t bl = dblk(nch, ich, jch);
-        const int u0 = nVf + 2 * (mv - 1), w0 = nVf + 2 * (mw - 1);
-        for (int f = 0; f < 4; ++f)
-          for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b) {
-              if (ich >= jch) T.r1.push_back({qblk(a, b, bl), u0 + a, w0 + b, sid(f, pB, pA), cc});
-              else T.r1.push_back({qblk(a, b, bl), w0 + a, u0 + b, sid(f, pB, pA), cc});
-            }
+        const int u0 = nVc + 2 * (mv - 1), w0 = nVc + 2 * (mw - 1);
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b) {
+            if (ich >= jch) T.r1.push_back({qblk(a, b, bl), u0 + a, w0 + b, sid(pB, pA), cc});
+            else T.r1.push_back({qblk(a, b, bl), w0 + a, u0 + b, sid(pB, pA), cc});
+          }
       } else if (ic < 0 && jc == 0) {
         const int io1 = io / kIbo, io2 = io % kIbo;
         const int ich = chan_of(W, io1), jch = chan_of(W, io2);
@@ -302,7 +345,7 @@ int dirac_decode(bsrg_ctx* c, int itype, int k, int n, const int* K1, const int*
           return fail(-10, "target_h: channel index out of range");
         const int it = W.iptar[ich - 1], jt = W.iptar[jch - 1];
         const int i = std::max(it, jt), j = std::min(it, jt);
-        for (int f = 0; f < 4; ++f) T.scal.push_back({1, (i - 1) * i / 2 + j - 1, sid(f, pB, pA), cc});
+        T.scal.push_back({1, (i - 1) * i / 2 + j - 1, sid(pB, pA), cc});
       } else {
         return fail(-22, " R_data: unknown structure for itype=1");
       }
@@ -321,9 +364,9 @@ int dirac_decode(bsrg_ctx* c, int itype, int k, int n, const int* K1, const int*
         const int jch = chan_of(W, io1);
         if ((rc = chk_pair(W, ich, jch, "UPDATE_HW"))) return rc;
         const size_t bl = dblk(nch, ich, jch);
-        const int w0 = nVf + 2 * (m - 1);
-        for (int f = 0; f < 4; ++f) {
-          const int a = kB1[f], v = vid(f, pB, bi);
+        const int w0 = nVc + 2 * (m - 1);
+        for (int a = 0; a < 2; ++a) {  // r1 component: P from PPPP + PQPQ, Q from QQQQ + QPQP
+          const int v = vid(a, pB, bi);
           for (int h = 0; h < 2; ++h) {
             if (ich >= jch) T.r1.push_back({qblk(a, h, bl), v, w0 + h, -1, cc});
             else T.r1.push_back({qblk(h, a, bl), w0 + h, v, -1, cc});
@@ -332,8 +375,7 @@ int dirac_decode(bsrg_ctx* c, int itype, int k, int n, const int* K1, const int*
       } else if (io < 0) {
         if (ich < 1 || ich > nch) return fail(-13, "UPDATE_HV: channel index out is of range");
         if (-io > npert) return fail(-14, "UPDATE_HV: bound index out is of range");
-        for (int f = 0; f < 4; ++f)
-          T.vec.push_back({kB1[f] * nq1 + (ich - 1) * npert + (-io - 1), vid(f, pB, bi), -1, cc});
+        for (int a = 0; a < 2; ++a) T.vec.push_back({a * nq1 + (ich - 1) * npert + (-io - 1), vid(a, pB, bi), -1, cc});
       } else {
         return fail(-23, " R_data: uknown structure for itype=2 ");
       }
@@ -347,14 +389,14 @@ int dirac_decode(bsrg_ctx* c, int itype, int k, int n, const int* K1, const int*
       int p;
       if ((rc = pair_of(W, K2[q], K3[q], p))) return rc;
       const size_t bl = dblk(nch, ich, jch);
-      for (int f = 0; f < 4; ++f) T.band.push_back({qblk(kB1[f], kB1[f], bl), (f * nk + k) * np + p, cd[q]});
-      if (!Qk) Qk = &quadr_table(c, k);
+      for (int h = 0; h < 2; ++h) T.band.push_back({qblk(h, h, bl), (h * nk + k) * np + p, cd[q]});
+      if (!Qk) Qk = &W.Q[k];  // built before the parallel pass
       CA += cd[q] * (*Qk)[W.bidx[K2[q] - 1] * nb + W.bidx[K3[q] - 1]];
       if (q == n - 1 || K1[q + 1] != K1[q]) {
         if (k > W.mk) return fail(-17, "Update_cf: k > kmk ");
-        double& a = W.acf[bl + static_cast<size_t>(W.nblk) * k];
-        a += CA;  // atype 0
-        a += CA;  // atype 1
+        const size_t at = bl + static_cast<size_t>(W.nblk) * k;
+        P.ax.push_back({at, CA});  // atype 0
+        P.ax.push_back({at, CA});  // atype 1
         CA = 0.0;
       }
     }
@@ -372,9 +414,9 @@ int dirac_decode(bsrg_ctx* c, int itype, int k, int n, const int* K1, const int*
                          "idef_type.f90:268-296)");
       const int bl = static_cast<int>(dblk(nch, ich, jch));
       for (int f = 0; f < 4; ++f) {
-        W.xc.push_back({bl, (f * nkE + k) * nb2 + a * nb + b, cd[q]});
-        W.klo[f] = std::min(W.klo[f], k);
-        W.khi[f] = std::max(W.khi[f], k);
+        P.xc.push_back({bl, (f * nkE + k) * nb2 + a * nb + b, cd[q]});
+        P.klo[f] = std::min(P.klo[f], k);
+        P.khi[f] = std::max(P.khi[f], k);
       }
     }
   } else {
@@ -387,7 +429,9 @@ int dirac_decode(bsrg_ctx* c, int itype, int k, int n, const int* K1, const int*
 // block), in one pinned arena, uploaded in one copy.
 int stage_exchange(bsrg_ctx* c) {
   auto& W = c->dw;
-  const size_t nblk = W.nblk, n = W.xc.size();
+  size_t n = 0;
+  for (int t = 0; t < W.nparts; ++t) n += W.parts[t]->xc.size();
+  const size_t nblk = W.nblk;
   size_t o = 0;
   auto put = [&o](size_t count, size_t elem) { const size_t at = o; o += ((count * elem + 255) / 256) * 256; return at; };
   W.o_xc = put(n, 8);
@@ -400,14 +444,16 @@ int stage_exchange(bsrg_ctx* c) {
   int* tt = reinterpret_cast<int*>(W.xh.p + W.o_xt);
   double* cv = reinterpret_cast<double*>(W.xh.p + W.o_xc);
   W.xcnt.assign(nblk + 1, 0);
-  for (const auto& x : W.xc) ++W.xcnt[x.blk + 1];
+  for (int t = 0; t < W.nparts; ++t)
+    for (const auto& x : W.parts[t]->xc) ++W.xcnt[x.blk + 1];
   for (size_t b = 0; b < nblk; ++b) W.xcnt[b + 1] += W.xcnt[b];
   for (size_t b = 0; b <= nblk; ++b) ptr[b] = W.xcnt[b];
-  for (const auto& x : W.xc) {
-    const int at = W.xcnt[x.blk]++;
-    tt[at] = x.t;
-    cv[at] = x.c;
-  }
+  for (int t = 0; t < W.nparts; ++t)
+    for (const auto& x : W.parts[t]->xc) {
+      const int at = W.xcnt[x.blk]++;
+      tt[at] = x.t;
+      cv[at] = x.c;
+    }
   return 0;
 }
 
@@ -501,13 +547,15 @@ int bsrg_dirac_set_orbitals(bsrg_ctx* c, int nbf, const double* pq, const int* i
   W.kcol.assign(static_cast<size_t>(nbf) * nbf, 0);
   for (int m = 0; m < nv_ch; ++m) W.kcol[(i_ch[m] - 1) + static_cast<size_t>(nbf) * (j_ch[m] - 1)] = m + 1;
   if (same_bound) {
-    if (W.basis_nk > 0) {  // new V_ch into the vector table
-      const size_t nVf = 4 * static_cast<size_t>(W.basis_nk) * W.npair * W.nb;
-      HIPCHK(W.VT.resize((nVf + 2 * static_cast<size_t>(nv_ch) + 1) * ns));
-      if (nv_ch)
-        HIPCHK(hipMemcpyAsync(W.VT.p + nVf * ns, W.vch.data(), W.vch.size() * sizeof(double), hipMemcpyHostToDevice,
+    if (W.basis_nk > 0) {  // new V_ch into the vector table, after the combined vectors
+      const size_t nVc = 2 * static_cast<size_t>(W.basis_nk) * W.npair * W.nb;
+      if ((nVc + 2 * static_cast<size_t>(nv_ch) + 1) * ns > W.VT.n) {
+        W.basis_nk = 0;  // no room: the next assembly rebuilds the table
+      } else if (nv_ch) {
+        HIPCHK(hipMemcpyAsync(W.VT.p + nVc * ns, W.vch.data(), W.vch.size() * sizeof(double), hipMemcpyHostToDevice,
                               c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+      }
     }
     W.orb_ok = true;
     return 0;
@@ -547,6 +595,7 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   for (int i = 0; i < nch; ++i)
     if (iptar[i] < 1 || iptar[i] > ntarg) return fail(-1, "bsrg_dirac_assemble: iptar out of 1..ntarg");
   HIPCHK(hipSetDevice(c->device));
+  const auto hs = std::chrono::steady_clock::now();
   const auto& T = c->dbs.T;
   const int ns = T.ns, ks = T.ks, ms = 2 * ns;
   hipStream_t s = c->stream;
@@ -578,19 +627,61 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   W.hp.assign(static_cast<size_t>(npert) * (npert + 1) / 2, 0.0);
   W.acf.assign(W.nblk * (mk + 1), 0.0);
   W.htarg.assign(static_cast<size_t>(ntarg) * (ntarg + 1) / 2, 0.0);
-  // lists -> terms
+  // lists -> terms: records cut into ranges at list boundaries or where K1
+  // changes (the ACF group sums of itype 3 stay whole); each host worker
+  // decodes its ranges into its own part; parts stay in list order
+  const auto h0 = std::chrono::steady_clock::now();
   if (int rc = wait_staged(W.st)) return rc;
-  W.T.clear();
-  W.xc.clear();
-  for (int f = 0; f < 4; ++f) { W.klo[f] = 1 << 30; W.khi[f] = -1; }
-  for (int r = 0; r < nrec;) {
-    int e = r + 1;
-    while (e < nrec && itype[e] == itype[r] && kpol[e] == kpol[r]) ++e;
-    if (int rc = dirac_decode(c, itype[r], kpol[r], e - r, k1 + r, k2 + r, k3 + r, k4 + r, cdata + r)) return rc;
-    r = e;
+  for (int r = 0; r < nrec; ++r)
+    if (itype[r] == 3) (void)quadr_table(c, kpol[r]);  // tables the workers read
+  if (!c->pool) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    c->pool.reset(new Pool(static_cast<int>(std::min(8u, hw))));
   }
-  const std::vector<Terms*> one{&W.T};
-  if (int rc = stage_terms(one, W.st, static_cast<int>(4 * W.nblk), static_cast<int>(2 * nq1), c->pool.get()))
+  const int nth = static_cast<int>(std::min<long>(4L * c->pool->size(), nrec / 8192L + 1));
+  std::vector<int> cuts{0};
+  for (int t = 1; t < nth; ++t) {
+    int b = static_cast<int>(static_cast<long>(nrec) * t / nth);
+    if (b <= cuts.back()) continue;
+    while (b < nrec && itype[b] == itype[b - 1] && kpol[b] == kpol[b - 1] && k1[b] == k1[b - 1]) ++b;
+    if (b < nrec) cuts.push_back(b);
+  }
+  cuts.push_back(nrec);
+  W.nparts = static_cast<int>(cuts.size()) - 1;
+  while (static_cast<int>(W.parts.size()) < W.nparts) W.parts.emplace_back(new DiracWave::Part());
+  const std::function<void(int)> work = [&](int t) {
+    DiracWave::Part& P = *W.parts[t];
+    P.T.clear();
+    P.xc.clear();
+    P.ax.clear();
+    P.err = 0;
+    for (int f = 0; f < 4; ++f) { P.klo[f] = 1 << 30; P.khi[f] = -1; }
+    for (int r = cuts[t]; r < cuts[t + 1];) {
+      int e = r + 1;
+      while (e < cuts[t + 1] && itype[e] == itype[r] && kpol[e] == kpol[r]) ++e;
+      if (int rc = dirac_decode(c, P, itype[r], kpol[r], e - r, k1 + r, k2 + r, k3 + r, k4 + r, cdata + r)) {
+        P.err = rc;
+        P.msg = bsrg_last_error();
+        return;
+      }
+      r = e;
+    }
+  };
+  c->pool->run(W.nparts, work);
+  for (int t = 0; t < W.nparts; ++t)
+    if (W.parts[t]->err) return fail(W.parts[t]->err, W.parts[t]->msg);
+  bool has_x = false;
+  for (int f = 0; f < 4; ++f) { W.klo[f] = 1 << 30; W.khi[f] = -1; }
+  std::vector<Terms*> parts;
+  for (int t = 0; t < W.nparts; ++t) {
+    DiracWave::Part& P = *W.parts[t];
+    for (const auto& a : P.ax) W.acf[a.first] += a.second;
+    for (int f = 0; f < 4; ++f) { W.klo[f] = std::min(W.klo[f], P.klo[f]); W.khi[f] = std::max(W.khi[f], P.khi[f]); }
+    has_x = has_x || !P.xc.empty();
+    parts.push_back(&P.T);
+  }
+  const auto h1 = std::chrono::steady_clock::now();
+  if (int rc = stage_terms(parts, W.st, static_cast<int>(4 * W.nblk), static_cast<int>(2 * nq1), c->pool.get()))
     return rc;
   if (int rc = stage_exchange(c)) return rc;
   Staged& st = W.st;
@@ -598,7 +689,6 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
     HIPCHK(st.d.resize(st.bytes));
     HIPCHK(hipMemcpyAsync(st.d.p, st.h.p, st.bytes, hipMemcpyHostToDevice, s));
   }
-  const bool has_x = !W.xc.empty();
   const size_t nt = 4 * static_cast<size_t>(W.exch_nk) * W.nb * W.nb;
   if (has_x) {
     HIPCHK(W.xd.resize(W.xbytes));
@@ -611,6 +701,7 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   }
   HIPCHK(hipEventRecord(st.done, s));
   st.pending = true;
+  const auto h2 = std::chrono::steady_clock::now();
   int eng[4] = {-1, -1, -1, -1};
   for (int f = 0; f < 4; ++f)
     if (W.khi[f] >= 0) {
@@ -621,10 +712,10 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   // output layout
   (void)hipEventRecord(c->ev[2], s);
   launch_assemble(ns, ks, static_cast<int>(4 * W.nblk), 0, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
-                  st.dp<double>(st.o_bc), W.D.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
-                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), W.St.p, W.VT.p, W.hq.p, s);
+                  st.dp<double>(st.o_bc), W.Dc.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
+                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), W.Sc.p, W.VT.p, W.hq.p, s);
   launch_vec_assemble(ns, static_cast<int>(2 * nq1), 0, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid),
-                      st.dp<int>(st.o_vs), st.dp<double>(st.o_vc), W.St.p, W.VT.p, W.hv.p, s);
+                      st.dp<int>(st.o_vs), st.dp<double>(st.o_vc), W.Sc.p, W.VT.p, W.hv.p, s);
   HIPCHK(hipGetLastError());
   (void)hipEventRecord(c->ev[3], s);
   for (int f = 0; f < 4; ++f)
@@ -632,19 +723,23 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
       if (int rc = dirac_gemm(c, f, eng[f], true)) return rc;
   W.engine = eng[0] >= 0 ? eng[0] : eng[1];
   (void)hipEventRecord(c->ev[4], s);
-  hipLaunchKernelGGL(k_dirac_finalize, dim3(static_cast<unsigned>(W.nblk), (ms + kFt - 1) / kFt, (ms + kFt - 1) / kFt),
-                     dim3(256), 0, s, ns, nch, W.nblk, W.hq.p, W.hch.p);
+  if (W.nblk > static_cast<size_t>(nch))
+    hipLaunchKernelGGL(k_dirac_layout, dim3(static_cast<unsigned>(4 * (W.nblk - nch)), (ns * ns + 1023) / 1024),
+                       dim3(256), 0, s, ns, nch, W.nblk, W.hq.p, W.hch.p);
+  hipLaunchKernelGGL(k_dirac_diag, dim3(static_cast<unsigned>(nch), (ms + kFt - 1) / kFt, (ms + kFt - 1) / kFt),
+                     dim3(256), 0, s, ns, W.nblk, W.hq.p, W.hch.p);
   if (nq1) hipLaunchKernelGGL(k_dirac_vfinalize, dim3(static_cast<unsigned>(nq1)), dim3(256), 0, s, ns, static_cast<int>(nq1), W.hv.p, W.hcp.p);
   HIPCHK(hipGetLastError());
   (void)hipEventRecord(c->ev[5], s);
   HIPCHK(hipEventSynchronize(c->ev[5]));
   st.pending = false;
   // host scalars (HB, Target_h) in list order
-  for (const auto& x : W.T.scal) {
-    const double v = x.c * (x.sidx >= 0 ? W.St_host.p[x.sidx] : 1.0);
-    if (x.kind == 0) W.hp[x.idx] += v;
-    else W.htarg[x.idx] += v;
-  }
+  for (int t = 0; t < W.nparts; ++t)
+    for (const auto& x : W.parts[t]->T.scal) {
+      const double v = x.c * (x.sidx >= 0 ? W.St_host.p[x.sidx] : 1.0);
+      if (x.kind == 0) W.hp[x.idx] += v;
+      else W.htarg[x.idx] += v;
+    }
   float e01 = 0, e23 = 0, e34 = 0, e45 = 0, e05 = 0;
   (void)hipEventElapsedTime(&e01, c->ev[0], c->ev[1]);
   (void)hipEventElapsedTime(&e23, c->ev[2], c->ev[3]);
@@ -652,6 +747,12 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   (void)hipEventElapsedTime(&e45, c->ev[4], c->ev[5]);
   (void)hipEventElapsedTime(&e05, c->ev[0], c->ev[5]);
   W.ms[0] = e01; W.ms[1] = e23; W.ms[2] = e34; W.ms[3] = e45; W.ms[4] = e05;
+  auto msd = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
+  W.host_ms[0] = msd(h0, h1);
+  W.host_ms[1] = msd(h1, h2);
+  W.host_ms[2] = msd(hs, std::chrono::steady_clock::now());
   W.ok = true;
   return 0;
 }
@@ -671,8 +772,9 @@ int bsrg_dirac_get(bsrg_ctx* c, double* hch, double* hcp, double* hp, double* ac
   return 0;
 }
 
-int bsrg_dirac_timings(bsrg_ctx* c, double* ms5) {
-  if (!c || !ms5) return fail(-2, "null argument");
-  std::copy(c->dw.ms, c->dw.ms + 5, ms5);
+int bsrg_dirac_timings(bsrg_ctx* c, double* ms8) {
+  if (!c || !ms8) return fail(-2, "null argument");
+  std::copy(c->dw.ms, c->dw.ms + 5, ms8);
+  std::copy(c->dw.host_ms, c->dw.host_ms + 3, ms8 + 5);
   return 0;
 }

@@ -162,66 +162,61 @@ def make_dirac_wave(seed: int, ks: int, ns: int, ksp: int, ksq: int, z: float, r
     nvch = len(ich)
     v_ch = np.asfortranarray(0.1 * rng.standard_normal((2 * ns, nvch)))
     ovl = list(zip(ich, jch))
-    bound = np.arange(1, nb + 1)
     w = DiracWave(name, seed, ks, ns, ksp, ksq, t, nb, nch, npert, mk, ntarg, pq, ipbs,
                   np.asarray(ich, np.int32), np.asarray(jch, np.int32), v_ch, iptar)
 
-    def pair():
-        a, b = sorted(rng.choice(bound, 2))
-        return int(a) * IBO + int(b)
+    def pairs(n):
+        """n random bound pairs a <= b, packed a*ibo + b"""
+        x = rng.integers(1, nb + 1, (2, n))
+        return np.minimum(x[0], x[1]) * IBO + np.maximum(x[0], x[1])
 
-    def code(o):
-        return int(o[0]) * IBO + int(o[1])
-
+    ov_code = np.asarray([int(i) * IBO + int(j) for i, j in ovl], np.int64)
+    nov = len(ovl)
     for k in range(mk + 1):
         for itype in (1, 2, 3, 4):
-            K1, K2, K3, K4, C = [], [], [], [], []
+            parts = []  # (k1, k2, k3, k4) arrays of the records kept
 
-            def add(a, b, c_, d):
-                K1.append(a); K2.append(b); K3.append(c_); K4.append(d)
-                C.append(rng.uniform(-1.0, 1.0))
+            def keep(n):
+                return np.nonzero(rng.random(n) < fill)[0]
 
             if itype in (3, 4):
-                # every channel pair x bound pair, w.p. fill (vectorised draw)
+                # every channel pair x bound pair, w.p. fill
                 ii, jj = np.tril_indices(nch)
                 if itype == 3:
                     aa, bb = np.triu_indices(nb)
                 else:
                     aa, bb = np.meshgrid(np.arange(nb), np.arange(nb), indexing="ij")
                     aa, bb = aa.ravel(), bb.ravel()
-                keep = rng.random((len(ii), len(aa))) < fill
-                r, s = np.nonzero(keep)
-                k1 = (ii[r] + 1) * IBO + (jj[r] + 1)
-                rl = RecordList.build(k1, aa[s] + 1, bb[s] + 1, np.zeros(len(r)), rng.uniform(-1.0, 1.0, len(r)))
-                w.lists.append(DiracList(itype, k, rl.k1, rl.k2, rl.k3, rl.k4, rl.c))
-                continue
-            if itype == 1:
+                r, s_ = np.nonzero(rng.random((len(ii), len(aa))) < fill)
+                parts.append(((ii[r] + 1) * IBO + (jj[r] + 1), aa[s_] + 1, bb[s_] + 1, np.zeros(len(r), np.int64)))
+            elif itype == 1:
                 if npert:
-                    for ic in range(1, npert + 1):
-                        for jc in range(1, ic + 1):
-                            if rng.random() < fill:
-                                add(pair(), pair(), -ic, -jc)
-                    for o in ovl:
-                        for ic in range(1, npert + 1):
-                            if rng.random() < fill:
-                                add(pair(), pair(), code(o), -ic)
-                for p, o in enumerate(ovl):
-                    for q in range(p + 1):
-                        if rng.random() < fill:
-                            add(pair(), pair(), code(o), code(ovl[q]))
-                for c1 in range(nch):
-                    for c2 in range(c1 + 1):
-                        if rng.random() < fill:
-                            add(pair(), pair(), (nb + c1 + 1) * IBO + (nb + c2 + 1), 0)
+                    ic, jc = np.tril_indices(npert)
+                    q = keep(len(ic))
+                    parts.append((pairs(len(q)), pairs(len(q)), -(ic[q] + 1), -(jc[q] + 1)))
+                    o, ic = np.meshgrid(np.arange(nov), np.arange(npert), indexing="ij")
+                    o, ic = o.ravel(), ic.ravel()
+                    q = keep(len(o))
+                    parts.append((pairs(len(q)), pairs(len(q)), ov_code[o[q]], -(ic[q] + 1)))
+                p1, p2 = np.tril_indices(nov)
+                q = keep(len(p1))
+                parts.append((pairs(len(q)), pairs(len(q)), ov_code[p1[q]], ov_code[p2[q]]))
+                c1, c2 = np.tril_indices(nch)
+                q = keep(len(c1))
+                parts.append((pairs(len(q)), pairs(len(q)), (nb + c1[q] + 1) * IBO + (nb + c2[q] + 1),
+                              np.zeros(len(q), np.int64)))
             else:
-                for c in range(1, nch + 1):
-                    for ic in range(1, npert + 1):
-                        if rng.random() < fill:
-                            add(pair(), int(rng.choice(bound)), c, -ic)
-                    for o in ovl:
-                        if rng.random() < fill:
-                            add(pair(), int(rng.choice(bound)), c, code(o))
-            rl = RecordList.build(K1, K2, K3, K4, C)
+                if npert:
+                    ch, ic = np.meshgrid(np.arange(nch), np.arange(npert), indexing="ij")
+                    ch, ic = ch.ravel(), ic.ravel()
+                    q = keep(len(ch))
+                    parts.append((pairs(len(q)), rng.integers(1, nb + 1, len(q)), ch[q] + 1, -(ic[q] + 1)))
+                ch, o = np.meshgrid(np.arange(nch), np.arange(nov), indexing="ij")
+                ch, o = ch.ravel(), o.ravel()
+                q = keep(len(ch))
+                parts.append((pairs(len(q)), rng.integers(1, nb + 1, len(q)), ch[q] + 1, ov_code[o[q]]))
+            cols = [np.concatenate([p_[i] for p_ in parts]) if parts else np.zeros(0, np.int64) for i in range(4)]
+            rl = RecordList.build(*cols, rng.uniform(-1.0, 1.0, len(cols[0])))
             w.lists.append(DiracList(itype, k, rl.k1, rl.k2, rl.k3, rl.k4, rl.c))
     return w
 

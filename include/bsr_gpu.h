@@ -132,10 +132,12 @@ int bsrg_dirac_assemble(bsrg_ctx* ctx, int nch, int npert, int mk, int ntarg, co
  * acf(nblk,0:mk), htarg(ntarg(ntarg+1)/2). */
 int bsrg_dirac_get(bsrg_ctx* ctx, double* hch, double* hcp, double* hp, double* acf, double* htarg);
 
-/* Device time (ms) of the last bsrg_dirac_assemble: [0] bands and bases,
- * [1] band/rank-1/vector terms, [2] exchange GEMMs, [3] output layout,
- * [4] total (list decoding and uploads included). */
-int bsrg_dirac_timings(bsrg_ctx* ctx, double* ms5);
+/* Times (ms) of the last bsrg_dirac_assemble.  Device, from HIP events on
+ * the context's stream: [0] bands and bases, [1] band/rank-1/vector terms,
+ * [2] exchange GEMMs, [3] output layout, [4] first to last event (host list
+ * work in between included).  Host: [5] list decoding on the host workers,
+ * [6] staging and uploads, [7] the whole call. */
+int bsrg_dirac_timings(bsrg_ctx* ctx, double* ms8);
 
 /* Orbitals of the wave: pbs(ns,nbf); iech(i)=channel of orbital i (0 for
  * bound orbitals); lbs(i)=l; ibort(nbf,nbf) nonzero where the overlap
@@ -263,9 +265,11 @@ int bsrg_graph_destroy(bsrg_graph* graph);
  * Both contexts must be on one device.  No counterpart in the reference. */
 int bsrg_wave_diff(bsrg_ctx* a, bsrg_ctx* ref, int which, double floor_frac, double* rel);
 
-/* Forget the resident R^k bands and orbital bases so the next call
- * rebuilds them (bsrg_set_grid / bsrg_set_orbitals keep them when called
- * again with identical inputs).  No counterpart in the reference. */
+/* Forget the resident R^k bands and orbital bases (of the BS and of the
+ * Dirac partial waves) so the next call rebuilds them (bsrg_set_grid,
+ * bsrg_set_orbitals, bsrg_dbs_set_grid and bsrg_dirac_set_orbitals keep them
+ * when called again with identical inputs).  No counterpart in the
+ * reference. */
 int bsrg_drop_caches(bsrg_ctx* ctx);
 
 /* Precision of the exchange contraction for later bsrg_wave_assemble
