@@ -191,7 +191,7 @@ def dirac_c5(steps, warmup, with_cpu):
         t0 = time.perf_counter()
         for _ in range(steps):
             step()
-            dev.append(dict(zip(("bases", "terms", "gemm", "layout", "total", "host_decode", "host_stage", "call"),
+            dev.append(dict(zip(("bases", "gemm", "terms_layout", "symm_vec", "total", "host_decode", "host_stage", "call"),
                                 map(float, ctx.dirac_timings()))))
         ms = (time.perf_counter() - t0) * 1e3 / steps
         dv = mean_dict(dev)
@@ -201,9 +201,34 @@ def dirac_c5(steps, warmup, with_cpu):
         out = {"workload": f"Dirac wave ks=9 ns=150 (ksp=8 ksq=9) k<=12 nch={w.nch} ms={w.ms} nhm={w.nhm} "
                            f"nb={w.nb} fill=3%",
                "records": w.n_records, "elements": w.n_elements, "ms_per_wave": ms,
-               "elements_per_s": w.n_elements / (ms * 1e-3), "device_ms": dv,
+               "elements_per_s": w.n_elements / (ms * 1e-3), "device_ms": dv, "engine": ctx.dirac_engine(),
                "gemm_tflops": tf, "gemm_frac_fp64_peak": tf / PEAK_FP64_MFMA, "gemm_flops": flops,
                "device_bytes": w.footprint_bytes()}
+        # the same wave with the exchange GEMMs in split precision (int8
+        # slices; 7 of them keep Dirac waves within 1e-10), and its largest
+        # element error against the FP64 result
+        sctx = Context(0)
+        try:
+            sctx.set_precision(2, 7)
+
+            def sstep():
+                sctx.drop_caches()
+                sctx.dirac_wave(w)
+
+            for _ in range(warmup):
+                sstep()
+            sdev = []
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                sstep()
+                sdev.append(dict(zip(("bases", "gemm", "terms_layout", "symm_vec", "total", "host_decode",
+                                      "host_stage", "call"), map(float, sctx.dirac_timings()))))
+            sms = (time.perf_counter() - t0) * 1e3 / steps
+            out["split"] = {"engine": sctx.dirac_engine(), "ms_per_wave": sms,
+                            "elements_per_s": w.n_elements / (sms * 1e-3), "device_ms": mean_dict(sdev),
+                            "max_rel_err_vs_fp64": sctx.dirac_diff(ctx), "vs_fp64_plain": ms / sms}
+        finally:
+            sctx.close()
         if with_cpu:
             from oracle import oracle as O
             ws = SD.c5(nch=8)

@@ -136,9 +136,20 @@ class Context:
         L.check(L.lib().bsrg_dirac_get(self._p, hch, hcp, hp, acf, htarg))
         return dict(hch=hch, hcp=hcp[:, : nch * npert], hp=hp[: npert * (npert + 1) // 2], acf=acf, htarg=htarg)
 
+    def dirac_engine(self):
+        return int(L.lib().bsrg_dirac_engine(self._p))
+
+    def dirac_diff(self, ref, floor_frac=1e-3):
+        """Largest relative element error of this context's Dirac wave
+        against ref's (same wave), floor floor_frac * max|ref|."""
+        r = C.c_double()
+        L.check(L.lib().bsrg_dirac_diff(self._p, ref._p, floor_frac, C.byref(r)))
+        return r.value
+
     def dirac_timings(self):
-        """ms: device bands+bases, terms, exchange GEMMs, output layout, first
-        to last event; host decoding, staging+uploads, whole call."""
+        """ms: device bands+bases, exchange GEMMs, terms with the output
+        layout, symmetrisation+vectors, first to last event; host decoding,
+        staging+uploads, whole call."""
         out = np.zeros(8)
         L.check(L.lib().bsrg_dirac_timings(self._p, out))
         return out

@@ -80,10 +80,15 @@ void launch_vtab(int ns, int ks, int nk, int npair, int nb, const double* D, con
 void launch_sb_upper(int ns, int ks, const double* sb, double* sbu, hipStream_t s);
 // Channel blocks in one pass: hcc[blk] = base + sum_t c_t * band_t (symmetric band)
 // + sum_r c_r * sfac_r * V[u_r] (x) V[w_r]; CSR term lists over all nblk blocks;
-// base = current hcc if has_base, else 0 (then every block is written)
+// base = current hcc if has_base, else 0 (then every block is written).
+// qnblk > 0 (Dirac): the nblk = 4*qnblk blocks are quadrants q = 2a+b of
+// channel blocks of order 2ns; base[blk] (ns x ns, quadrant-major) is read
+// where bit q of qmask is set, and the sum is written to rows a*ns..,
+// columns b*ns.. of hcc block blk % qnblk.
 void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, const int* bid, const double* bc,
                      const double* bands, const int* rptr, const int* ru, const int* rw, const int* rs,
-                     const double* rc, const double* stab, const double* V, double* hcc, hipStream_t s);
+                     const double* rc, const double* stab, const double* V, double* hcc, hipStream_t s,
+                     const double* base = nullptr, int qnblk = 0, int qmask = 0);
 // hcb[q] = base + sum_r c_r * sfac_r * V[v_r]; CSR over all nq positions
 void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,
                          const double* c, const double* stab, const double* V, double* hcb, hipStream_t s);

@@ -276,6 +276,12 @@ struct DiracWave {
   size_t xbytes = 0, o_xptr = 0, o_xt = 0, o_xc = 0;
   std::vector<int> xcnt;
   int klo[4] = {0, 0, 0, 0}, khi[4] = {-1, -1, -1, -1};  // multipole range of each family's exchange terms
+  // split precision: int8 slices of each family's exchange matrices, kept
+  // while the matrices are unchanged (e_ver), the coefficient-side and bf16
+  // workspaces
+  DevVec<char> oz8a[4], oz8b, ozk;
+  long oz8a_ver[4] = {-1, -1, -1, -1};
+  int oz8a_S[4] = {0, 0, 0, 0};
   int engine = -1;
   bool ok = false;
   double ms[5] = {0, 0, 0, 0, 0};

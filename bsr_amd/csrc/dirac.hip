@@ -69,59 +69,28 @@ __global__ __launch_bounds__(256) void k_dirac_coef(int nt, const int* __restric
   }
 }
 
-// Output layout, off-diagonal channel blocks: quadrant q = 2a+b of block
-// nch+bo (ns x ns, contiguous) to rows a*ns.., columns b*ns.. of the ms x ms
-// block.  Reads are contiguous; writes are contiguous within a column.
-// grid (4*(nblk-nch), ceil(ns*ns/1024)), block 256, 4 elements a thread.
-__global__ __launch_bounds__(256) void k_dirac_layout(int ns, int nch, size_t nblk, const double* __restrict__ hq,
-                                                      double* __restrict__ hch) {
-  const int ms = 2 * ns, nss = ns * ns;
-  const size_t noff = nblk - nch;
-  const int q = static_cast<int>(blockIdx.x / noff);
-  const size_t blk = nch + blockIdx.x % noff;
-  const int a = q >> 1, b = q & 1;
-  const double* src = hq + (static_cast<size_t>(q) * nblk + blk) * nss;
-  double* dst = hch + blk * ms * ms + a * ns + static_cast<size_t>(b) * ns * ms;
-  const int e0 = blockIdx.y * 1024 + threadIdx.x;
-  double v[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int e = e0 + 256 * u;
-    v[u] = e < nss ? src[e] : 0.0;
-  }
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int e = e0 + 256 * u;
-    if (e < nss) dst[(e % ns) + static_cast<size_t>(e / ns) * ms] = v[u];
-  }
-}
-
-// Output layout, diagonal channel blocks (blk < nch), symmetrised:
-// (X + X^T)/2, the transposed tile staged in LDS.  grid (nch, ceil(ms/32),
-// ceil(ms/32)), block 256 = 32 rows x 8 column groups.
-constexpr int kFt = 32;
-__global__ __launch_bounds__(256) void k_dirac_diag(int ns, size_t nblk, const double* __restrict__ hq,
-                                                    double* __restrict__ hch) {
-  const int ms = 2 * ns;
-  const size_t blk = blockIdx.x;
-  const int r0 = blockIdx.z * kFt, c0 = blockIdx.y * kFt;
+// Diagonal channel blocks (the first nch blocks of hch, order ms) ->
+// (X + X^T)/2 in place, through 32 x 32 LDS tiles: the workgroup of tile
+// pair (ti >= tj) reads both tiles before it writes them.  grid
+// (nt*(nt+1)/2, nch), block 256 = 32 rows x 8 column groups.
+__global__ __launch_bounds__(256) void k_dirac_symm(int ms, double* __restrict__ hch) {
+  int p = blockIdx.x, ti = 0;
+  while (p > ti) { p -= ti + 1; ++ti; }
+  const int tj = p;
+  double* X = hch + static_cast<size_t>(blockIdx.y) * ms * ms;
+  __shared__ double A[32][33], B[32][33];
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  const size_t nss = static_cast<size_t>(ns) * ns;
-  auto at = [&](int r, int c) {
-    const int a = r >= ns, b = c >= ns;
-    return hq[(static_cast<size_t>(2 * a + b) * nblk + blk) * nss + (r - a * ns) + static_cast<size_t>(c - b * ns) * ns];
-  };
-  double* out = hch + blk * ms * ms;
-  const int r = r0 + tx;
-  __shared__ double tile[kFt][kFt + 1];
-  for (int j = ty; j < kFt; j += 8) {  // tile[j][tx] = X(c0 + tx, r0 + j)
-    const int rr = c0 + tx, cc = r0 + j;
-    tile[j][tx] = (rr < ms && cc < ms) ? at(rr, cc) : 0.0;
+  const int ri = ti * 32, cj = tj * 32;
+  for (int j = ty; j < 32; j += 8) {
+    const int r = ri + tx, c = cj + j, r2 = cj + tx, c2 = ri + j;
+    A[j][tx] = (r < ms && c < ms) ? X[r + static_cast<size_t>(ms) * c] : 0.0;      // X(ri+tx, cj+j)
+    B[j][tx] = (r2 < ms && c2 < ms) ? X[r2 + static_cast<size_t>(ms) * c2] : 0.0;  // X(cj+tx, ri+j)
   }
   __syncthreads();
-  for (int j = ty; j < kFt; j += 8) {
-    const int c = c0 + j;
-    if (r < ms && c < ms) out[r + static_cast<size_t>(ms) * c] = 0.5 * (at(r, c) + tile[tx][j]);
+  for (int j = ty; j < 32; j += 8) {
+    const int r = ri + tx, c = cj + j, r2 = cj + tx, c2 = ri + j;
+    if (r < ms && c < ms) X[r + static_cast<size_t>(ms) * c] = 0.5 * (A[j][tx] + B[tx][j]);
+    if (ti != tj && r2 < ms && c2 < ms) X[r2 + static_cast<size_t>(ms) * c2] = 0.5 * (B[j][tx] + A[tx][j]);
   }
 }
 
@@ -460,6 +429,10 @@ int stage_exchange(bsrg_ctx* c) {
 std::mutex g_dtuned_mu;
 std::map<std::tuple<int, int, int, int>, int> g_dtuned;
 
+// Exchange GEMM of family f into its quadrant: hq[b1,b2] (+)= E_f * coef_f.
+// engine 0 library FP64, 1-3 FP64 MFMA kernel, 10+S bf16 and 20+S int8 split
+// precision with S slices (the int8 slices of E_f prepared once per version
+// of the exchange matrices, over every multipole).
 int dirac_gemm(bsrg_ctx* c, int f, int engine, bool accumulate) {
   auto& W = c->dw;
   const int ns = c->dbs.T.ns, nb = W.nb, nkE = W.exch_nk;
@@ -471,6 +444,24 @@ int dirac_gemm(bsrg_ctx* c, int f, int engine, bool accumulate) {
   const double* B = W.coef.p + r0;
   double* C = W.hq.p + static_cast<size_t>(2 * kB1[f] + kB2[f]) * W.nblk * nss;
   const int m = static_cast<int>(nss), n = static_cast<int>(W.nblk);
+  if (engine >= 20) {
+    const int S = engine - 20, K = nkE * static_cast<int>(nb2);
+    const double* Af = W.E.p + static_cast<size_t>(f) * nkE * nb2 * nss;
+    if (W.oz8a_ver[f] != W.e_ver || W.oz8a_S[f] != S) {
+      HIPCHK(W.oz8a[f].resize(ozaki8_a_bytes(m, K, S)));
+      if (int rc = ozaki8_prepare_a(m, K, Af, nss, S, W.oz8a[f].p, c->stream)) return rc;
+      W.oz8a_ver[f] = W.e_ver;
+      W.oz8a_S[f] = S;
+    }
+    HIPCHK(W.oz8b.resize(ozaki8_b_bytes(n, K, S)));
+    return ozaki8_dgemm(m, n, K, W.oz8a[f].p, W.coef.p + static_cast<size_t>(f) * nkE * nb2, nt, C, nss, accumulate, S,
+                        W.oz8b.p, c->stream);
+  }
+  if (engine >= 10) {
+    const int S = engine - 10;
+    HIPCHK(W.ozk.resize(ozaki_workspace_bytes(m, n, Kt, S)));
+    return ozaki_dgemm(m, n, Kt, A, nss, B, nt, C, nss, accumulate, S, W.ozk.p, c->stream);
+  }
   if (engine == 0) {
     const double alpha = 1.0, beta = accumulate ? 1.0 : 0.0;
     rocblas_status rs = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, m, n, Kt, &alpha, A,
@@ -483,6 +474,7 @@ int dirac_gemm(bsrg_ctx* c, int f, int engine, bool accumulate) {
 // fastest FP64 engine for this GEMM shape, timed once per shape class
 int dirac_engine(bsrg_ctx* c, int f) {
   auto& W = c->dw;
+  if (c->precision_mode != 0) return 10 * c->precision_mode + c->nslices;
   if (c->forced_engine >= 0) return c->forced_engine;
   const int ns = c->dbs.T.ns;
   const int Kt = (W.khi[f] - W.klo[f] + 1) * W.nb * W.nb;
@@ -708,26 +700,31 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
       eng[f] = dirac_engine(c, f);
       if (eng[f] < 0) return eng[f];
     }
-  // device work: terms (every block written), exchange GEMMs (accumulate),
-  // output layout
+  // device work: the exchange GEMMs write their quadrants of hq; one pass
+  // then adds the band and rank-1 terms and writes the output layout; the
+  // diagonal channel blocks are symmetrised in place
   (void)hipEventRecord(c->ev[2], s);
+  int qmask = 0;
+  for (int f = 0; f < 4; ++f)
+    if (eng[f] >= 0) {
+      if (int rc = dirac_gemm(c, f, eng[f], false)) return rc;
+      qmask |= 1 << (2 * kB1[f] + kB2[f]);
+    }
+  W.engine = -1;
+  for (int f = 0; f < 4 && W.engine < 0; ++f) W.engine = eng[f];
+  (void)hipEventRecord(c->ev[3], s);
   launch_assemble(ns, ks, static_cast<int>(4 * W.nblk), 0, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
                   st.dp<double>(st.o_bc), W.Dc.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
-                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), W.Sc.p, W.VT.p, W.hq.p, s);
+                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), W.Sc.p, W.VT.p, W.hch.p, s, W.hq.p,
+                  static_cast<int>(W.nblk), qmask);
+  HIPCHK(hipGetLastError());
+  (void)hipEventRecord(c->ev[4], s);
+  {
+    const int nt = (ms + 31) / 32;
+    hipLaunchKernelGGL(k_dirac_symm, dim3(nt * (nt + 1) / 2, nch), dim3(256), 0, s, ms, W.hch.p);
+  }
   launch_vec_assemble(ns, static_cast<int>(2 * nq1), 0, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid),
                       st.dp<int>(st.o_vs), st.dp<double>(st.o_vc), W.Sc.p, W.VT.p, W.hv.p, s);
-  HIPCHK(hipGetLastError());
-  (void)hipEventRecord(c->ev[3], s);
-  for (int f = 0; f < 4; ++f)
-    if (eng[f] >= 0)
-      if (int rc = dirac_gemm(c, f, eng[f], true)) return rc;
-  W.engine = eng[0] >= 0 ? eng[0] : eng[1];
-  (void)hipEventRecord(c->ev[4], s);
-  if (W.nblk > static_cast<size_t>(nch))
-    hipLaunchKernelGGL(k_dirac_layout, dim3(static_cast<unsigned>(4 * (W.nblk - nch)), (ns * ns + 1023) / 1024),
-                       dim3(256), 0, s, ns, nch, W.nblk, W.hq.p, W.hch.p);
-  hipLaunchKernelGGL(k_dirac_diag, dim3(static_cast<unsigned>(nch), (ms + kFt - 1) / kFt, (ms + kFt - 1) / kFt),
-                     dim3(256), 0, s, ns, W.nblk, W.hq.p, W.hch.p);
   if (nq1) hipLaunchKernelGGL(k_dirac_vfinalize, dim3(static_cast<unsigned>(nq1)), dim3(256), 0, s, ns, static_cast<int>(nq1), W.hv.p, W.hcp.p);
   HIPCHK(hipGetLastError());
   (void)hipEventRecord(c->ev[5], s);
@@ -740,7 +737,7 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
       if (x.kind == 0) W.hp[x.idx] += v;
       else W.htarg[x.idx] += v;
     }
-  float e01 = 0, e23 = 0, e34 = 0, e45 = 0, e05 = 0;
+  float e01 = 0, e23 = 0, e34 = 0, e45 = 0, e05 = 0;  // bases, GEMMs, terms + layout, symmetrise + vectors
   (void)hipEventElapsedTime(&e01, c->ev[0], c->ev[1]);
   (void)hipEventElapsedTime(&e23, c->ev[2], c->ev[3]);
   (void)hipEventElapsedTime(&e34, c->ev[3], c->ev[4]);
@@ -776,5 +773,59 @@ int bsrg_dirac_timings(bsrg_ctx* c, double* ms8) {
   if (!c || !ms8) return fail(-2, "null argument");
   std::copy(c->dw.ms, c->dw.ms + 5, ms8);
   std::copy(c->dw.host_ms, c->dw.host_ms + 3, ms8 + 5);
+  return 0;
+}
+
+int bsrg_dirac_engine(bsrg_ctx* c) { return c ? c->dw.engine : -1; }
+
+int bsrg_dirac_diff(bsrg_ctx* a, bsrg_ctx* ref, double floor_frac, double* rel) {
+  if (!a || !ref || !rel) return fail(-2, "null argument");
+  const auto& A = a->dw;
+  const auto& B = ref->dw;
+  if (!A.ok || !B.ok) return fail(-6, "no assembled Dirac wave (bsrg_dirac_assemble)");
+  if (a->device != ref->device) return fail(-5, "bsrg_dirac_diff: contexts on different devices");
+  if (a->dbs.T.ns != ref->dbs.T.ns || A.nch != B.nch || A.npert != B.npert || A.mk != B.mk || A.ntarg != B.ntarg)
+    return fail(-1, "bsrg_dirac_diff: different shapes");
+  HIPCHK(hipSetDevice(ref->device));
+  HIPCHK(hipStreamSynchronize(a->stream));
+  hipStream_t s = ref->stream;
+  const size_t ms = 2 * static_cast<size_t>(ref->dbs.T.ns);
+  const size_t nh = B.nblk * ms * ms, nv = static_cast<size_t>(B.nch) * B.npert * ms;
+  HIPCHK(ref->dtmp.resize(2 * kReduceBlocks));
+  std::vector<double> part(2 * kReduceBlocks, 0.0);
+  // scale of the reference: largest element of its channel blocks, vectors and scalars
+  launch_absmax(nh, B.hch.p, ref->dtmp.p, s);
+  if (nv) launch_absmax(nv, B.hcp.p, ref->dtmp.p + kReduceBlocks, s);
+  else launch_fill(kReduceBlocks, 0.0, ref->dtmp.p + kReduceBlocks, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(part.data(), ref->dtmp.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  // one scale for the matrix parts (hch, hcp, hp); acf and htarg are other
+  // quantities with scales of their own
+  double mx = 0.0;
+  for (double v : part) mx = std::max(mx, v);
+  for (double v : B.hp) mx = std::max(mx, std::fabs(v));
+  const double floor = std::max(floor_frac * mx, 1e-300);
+  launch_reldiff(nh, A.hch.p, B.hch.p, floor, ref->dtmp.p, s);
+  if (nv) launch_reldiff(nv, A.hcp.p, B.hcp.p, floor, ref->dtmp.p + kReduceBlocks, s);
+  else launch_fill(kReduceBlocks, 0.0, ref->dtmp.p + kReduceBlocks, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(part.data(), ref->dtmp.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  double r = 0.0;
+  for (double v : part) r = std::max(r, v);
+  auto host_rel = [floor_frac](const std::vector<double>& a, const std::vector<double>& b, double fl) {
+    double m = 0.0, e = 0.0;
+    if (fl <= 0.0) {
+      for (double v : b) m = std::max(m, std::fabs(v));
+      fl = std::max(floor_frac * m, 1e-300);
+    }
+    for (size_t i = 0; i < b.size(); ++i) e = std::max(e, std::fabs(a[i] - b[i]) / std::max(std::fabs(b[i]), fl));
+    return e;
+  };
+  r = std::max(r, host_rel(A.hp, B.hp, floor));
+  r = std::max(r, host_rel(A.acf, B.acf, 0.0));
+  r = std::max(r, host_rel(A.htarg, B.htarg, 0.0));
+  *rel = r;
   return 0;
 }

@@ -455,23 +455,45 @@ __global__ void k_sb_upper(int ns, int ks, const double* sb, double* sbu) {
 // The band terms of the block are summed once per workgroup into the band
 // entries (i, ith) its columns touch, staged in LDS; rank-1 terms go 16 at a
 // time through LDS (u over the rows, f*w over the columns).
+// Output map: with qnblk = 0 block blk is hcc[blk] (ns x ns, in place);
+// with qnblk > 0 the blocks are the quadrants q = 2a+b of qnblk channel
+// blocks of order 2ns (Dirac): base[blk] is read from the quadrant-major
+// input and the sum goes to rows a*ns.., columns b*ns.. of output block
+// blk % qnblk (leading dimension 2ns); base bit q of qmask says whether
+// quadrant q has an input.
 constexpr int kAsmCpw = 8;                // columns per wavefront
 constexpr int kAsmCols = 4 * kAsmCpw;     // columns per workgroup
+// A = row groups of 64 per workgroup (rows per tile 64*A, A = ceil(ns/64)
+// up to 4), so that short blocks do not idle lanes.
+template <int A>
 __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, const int* __restrict__ bptr,
                                                   const int* __restrict__ bid, const double* __restrict__ bc,
                                                   const double* __restrict__ bands, const int* __restrict__ rptr,
                                                   const int* __restrict__ ru, const int* __restrict__ rw,
                                                   const int* __restrict__ rs, const double* __restrict__ rc,
                                                   const double* __restrict__ stab, const double* __restrict__ V,
-                                                  double* __restrict__ hcc) {
+                                                  double* __restrict__ hcc, const double* __restrict__ base,
+                                                  int qnblk, int qmask) {
   const int blk = blockIdx.x;
   const int b0 = bptr[blk], b1 = bptr[blk + 1], r0 = rptr[blk], r1 = rptr[blk + 1];
-  if (has_base && b0 == b1 && r0 == r1) return;
+  if (qnblk == 0 && has_base && b0 == b1 && r0 == r1) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int x0 = blockIdx.z * 256, yt = blockIdx.y * kAsmCols, y0 = yt + wv * kAsmCpw;
-  const int nrow = min(256, ns - x0);
-  double* h = hcc + static_cast<size_t>(blk) * ns * ns;
-  __shared__ double su[16][256];
+  constexpr int R = 64 * A;  // rows per tile
+  const int x0 = blockIdx.z * R, yt = blockIdx.y * kAsmCols, y0 = yt + wv * kAsmCpw;
+  const int nrow = min(R, ns - x0);
+  const double* hin = nullptr;
+  double* h;
+  size_t ldo = ns;
+  if (qnblk > 0) {
+    const int q = blk / qnblk, cb = blk % qnblk;
+    if ((qmask >> q) & 1) hin = base + static_cast<size_t>(blk) * ns * ns;
+    ldo = 2 * static_cast<size_t>(ns);
+    h = hcc + static_cast<size_t>(cb) * ldo * ldo + (q >> 1) * ns + static_cast<size_t>(q & 1) * ns * ldo;
+  } else {
+    h = hcc + static_cast<size_t>(blk) * ns * ns;
+    if (has_base) hin = h;
+  }
+  __shared__ double su[16][R];
   __shared__ double sw[16][kAsmCols];
   __shared__ double sband[(kAsmCols + kMaxKs - 1) * kMaxKs];
   __shared__ int tbid[64], tu[64], tw[64];
@@ -508,14 +530,14 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
   }
   // the sum starts from the base values, loaded before the rank-1 work so
   // their latency overlaps it: out = (base + rank-1 terms) + band terms
-  double acc[4][kAsmCpw];
+  double acc[A][kAsmCpw];
 #pragma unroll
   for (int b = 0; b < kAsmCpw; ++b) {
     const int y = min(y0 + b, ns - 1);
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int a = 0; a < A; ++a) {
       const int x = min(x0 + lane + 64 * a, ns - 1);
-      acc[a][b] = has_base ? h[x + static_cast<size_t>(ns) * y] : 0.0;
+      acc[a][b] = hin ? hin[x + static_cast<size_t>(ns) * y] : 0.0;
     }
   }
   // rank-1 terms: metadata and scale factors of 64 terms at a time into LDS
@@ -533,8 +555,8 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
     for (int tb = 0; tb < nc; tb += 16) {
       const int nt = min(16, nc - tb);
       __syncthreads();
-      for (int q = threadIdx.x; q < nt * 256; q += 256) {
-        const int r = q >> 8, x = q & 255;
+      for (int q = threadIdx.x; q < nt * R; q += 256) {
+        const int r = q / R, x = q % R;
         if (x < nrow) su[r][x] = V[static_cast<size_t>(tu[tb + r]) * ns + x0 + x];
       }
       for (int q = threadIdx.x; q < nt * kAsmCols; q += 256) {
@@ -543,14 +565,14 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
       }
       __syncthreads();
       for (int r = 0; r < nt; ++r) {
-        double u[4];
+        double u[A];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) u[a] = su[r][(lane + 64 * a) & 255];
+        for (int a = 0; a < A; ++a) u[a] = su[r][lane + 64 * a];
 #pragma unroll
         for (int b = 0; b < kAsmCpw; ++b) {
           const double w = sw[r][wv * kAsmCpw + b];
 #pragma unroll
-          for (int a = 0; a < 4; ++a) acc[a][b] += u[a] * w;
+          for (int a = 0; a < A; ++a) acc[a][b] += u[a] * w;
         }
       }
     }
@@ -561,7 +583,7 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
     const int y = y0 + b;
     if (y >= ns) continue;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int a = 0; a < A; ++a) {
       const int x = x0 + lane + 64 * a;
       if (x >= ns) continue;
       double v = acc[a][b];
@@ -569,7 +591,7 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
         const int ith = x > y ? x - y : y - x, i = min(x, y);
         if (ith < ks && i + ith < ns) v += sband[(i - ilo) * ks + ith];
       }
-      h[x + static_cast<size_t>(ns) * y] = v;
+      h[x + ldo * y] = v;
     }
   }
 }
@@ -774,11 +796,20 @@ void launch_sb_upper(int ns, int ks, const double* sb, double* sbu, hipStream_t 
 
 void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, const int* bid, const double* bc,
                      const double* bands, const int* rptr, const int* ru, const int* rw, const int* rs,
-                     const double* rc, const double* stab, const double* V, double* hcc, hipStream_t s) {
+                     const double* rc, const double* stab, const double* V, double* hcc, hipStream_t s,
+                     const double* base, int qnblk, int qmask) {
   if (nblk <= 0) return;
-  hipLaunchKernelGGL(k_assemble, dim3(nblk, (ns + kAsmCols - 1) / kAsmCols, (ns + 255) / 256), dim3(256), 0, s, ns, ks,
-                     has_base,
-                     bptr, bid, bc, bands, rptr, ru, rw, rs, rc, stab, V, hcc);
+  const int A = ns <= 64 ? 1 : (ns <= 128 ? 2 : (ns <= 192 ? 3 : 4));
+  const dim3 grid(nblk, (ns + kAsmCols - 1) / kAsmCols, (ns + 64 * A - 1) / (64 * A));
+  switch (A) {
+#define ASM_CASE(N)                                                                                             \
+  case N:                                                                                                       \
+    hipLaunchKernelGGL(k_assemble<N>, grid, dim3(256), 0, s, ns, ks, has_base, bptr, bid, bc, bands, rptr, ru, rw, \
+                       rs, rc, stab, V, hcc, base, qnblk, qmask);                                               \
+    break;
+    ASM_CASE(1) ASM_CASE(2) ASM_CASE(3) ASM_CASE(4)
+#undef ASM_CASE
+  }
 }
 
 void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,

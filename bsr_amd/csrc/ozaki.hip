@@ -319,10 +319,13 @@ __global__ void k_slice8_cols(int n, int K, int np, int Kp, int S, const double*
 // Workgroup tile 64 (C columns) x 64 (C rows), 4 waves of 32 x 32
 // (2 x 2 MFMA 16x16x64 i8), K in chunks of 64 staged in LDS (rows padded to
 // 80 bytes: conflict-free 16-byte reads).  Every level L = s + r <= S-1 has
-// its own int32 accumulators; the epilogue recombines them in FP64.
+// its own int32 accumulators; the epilogue recombines them in FP64.  Two
+// waves per SIMD are asked for up to S = 7 (the accumulators then live in
+// VGPRs; at S = 7 the default allocation would leave one wave per SIMD);
+// S = 8 keeps one (its LDS tiles allow one workgroup per CU anyway).
 constexpr int kPad8 = 80;
 template <int S>
-__global__ __launch_bounds__(256) void k_ozaki8(int m, int n, int mp, int np, int Kp, const int8_t* __restrict__ As,
+__global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, int mp, int np, int Kp, const int8_t* __restrict__ As,
                                                  const int8_t* __restrict__ Bs, const double* __restrict__ sig,
                                                  const double* __restrict__ tau, double* __restrict__ C, size_t ldc,
                                                  int accumulate) {

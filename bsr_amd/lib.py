@@ -80,6 +80,8 @@ SIGNATURES = {
                             _ip, _ip, _dp],
     "bsrg_dirac_get": [C.c_void_p, _odp, _odp, _odp, _odp, _odp],
     "bsrg_dirac_timings": [C.c_void_p, _dp],
+    "bsrg_dirac_engine": [C.c_void_p],
+    "bsrg_dirac_diff": [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)],
     "bsrg_wave_prepare": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
                           C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, _dp, _ip],
     "bsrg_wave_launch": [C.c_void_p],

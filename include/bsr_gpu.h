@@ -133,11 +133,23 @@ int bsrg_dirac_assemble(bsrg_ctx* ctx, int nch, int npert, int mk, int ntarg, co
 int bsrg_dirac_get(bsrg_ctx* ctx, double* hch, double* hcp, double* hp, double* acf, double* htarg);
 
 /* Times (ms) of the last bsrg_dirac_assemble.  Device, from HIP events on
- * the context's stream: [0] bands and bases, [1] band/rank-1/vector terms,
- * [2] exchange GEMMs, [3] output layout, [4] first to last event (host list
- * work in between included).  Host: [5] list decoding on the host workers,
- * [6] staging and uploads, [7] the whole call. */
+ * the context's stream: [0] bands and bases, [1] exchange GEMMs, [2] band
+ * and rank-1 terms with the output layout, [3] symmetrisation of the
+ * diagonal blocks and the channel-perturber vectors, [4] first to last event
+ * (host list work in between included).  Host: [5] list decoding on the
+ * host workers, [6] staging and uploads, [7] the whole call. */
 int bsrg_dirac_timings(bsrg_ctx* ctx, double* ms8);
+
+/* Engine of the last Dirac exchange contraction, as bsrg_wave_engine (the
+ * precision set by bsrg_set_precision applies to Dirac waves too). */
+int bsrg_dirac_engine(bsrg_ctx* ctx);
+
+/* Error of one assembled Dirac wave against a reference result of the same
+ * wave (e.g. split precision against FP64): rel = max over the elements of
+ * |a - r| / max(|r|, floor_frac * scale), scale = max|r| over hch, hcp and
+ * hp together, over acf, over htarg.  Both contexts on one device.  No
+ * counterpart in the reference. */
+int bsrg_dirac_diff(bsrg_ctx* a, bsrg_ctx* ref, double floor_frac, double* rel);
 
 /* Orbitals of the wave: pbs(ns,nbf); iech(i)=channel of orbital i (0 for
  * bound orbitals); lbs(i)=l; ibort(nbf,nbf) nonzero where the overlap

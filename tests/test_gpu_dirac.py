@@ -77,3 +77,27 @@ def test_dirac_rejects(ctx):
         ctx.dirac_assemble(w.nch, w.npert, w.mk, w.ntarg, w.iptar, [3], [0], [2 * ibo + 1], [1], [w.nb + 1],
                            [0], [0.5])
     assert e.value.code == -31
+
+
+@pytest.mark.parametrize("mode,nslices", [(2, 7), (2, 8), (1, 7), (1, 8)])
+def test_dirac_split_precision(ctx, mode, nslices):
+    """The exchange GEMMs of the four families in split precision (int8 or
+    bf16 Ozaki slices, FP64 recombination): every element within 1e-10 of
+    the FP64 result of the same wave (floor 1e-3 of the largest matrix
+    element).  Dirac waves need 7 slices for that (6 slices measure 3-4e-10:
+    the rows of the exchange matrices span all multipoles and bound pairs)."""
+    from bsr_amd import Context
+    w = CASES[2][1]()
+    print(f"[dirac-split] mode={mode} S={nslices} ns={w.ns} nch={w.nch}", flush=True)
+    ctx.dirac_wave(w)
+    sc = Context(0)
+    try:
+        sc.set_precision(mode, nslices)
+        sc.dirac_wave(w)
+        assert sc.dirac_engine() == 10 * mode + nslices
+        err = sc.dirac_diff(ctx)
+        print(f"[dirac-split] max rel element error vs FP64 {err:.2e}", flush=True)
+        assert err < 1e-10
+        assert err > 0.0  # the split path really ran
+    finally:
+        sc.close()
