@@ -1088,7 +1088,7 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   if (int rc = ensure_bands(c, nk)) return rc;
   (void)hipEventRecord(c->ev[1], c->stream);
   if (int rc = ensure_bases(c, nk, any_exch)) return rc;
-  (void)hipEventRecord(c->ev[2], c->stream);
+  (void)hipEventRecord(c->ev[8], c->stream);
   HIPCHK(c->S.alloc(c->G.ns, nch, ncp, mk));
   HIPCHK(c->H.alloc(c->G.ns, nch, ncp, mk));
   const auto ha = clk::now();
@@ -1161,7 +1161,7 @@ int bsrg_wave_finish(bsrg_ctx* c) {
     // overlap pass, so it carries the otarg accumulated there
     c->H.otarg = c->S.otarg;
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]); c->timings[0] = ms;
-    (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[2]); c->timings[1] = ms;
+    (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[8]); c->timings[1] = ms;
   }
   (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); c->timings[2] = ms;
   (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);

@@ -296,7 +296,7 @@ struct bsrg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[10] = {};  // 0-5 phases of a wave, 6-7 autotuning, 8 end of the bases
   double timings[5] = {0, 0, 0, 0, 0};
   // host ms of the last prepare: [0] checks and band/basis launches, [1] S and
   // the overlap terms of H, [2] registry of GET_V vectors, [3] parallel list

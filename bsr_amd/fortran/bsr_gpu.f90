@@ -9,6 +9,13 @@
 !    gpu_i_data(jtype, jpol, ...)  I_data(jtype,jpol) src/bsr/BSR_MAT3/I_data.f90:2
 !    gpu_o_data(...)               O_data             src/bsr/BSR_MAT3/O_data.f90:2
 !    gpu_update_hl(ich, jch, c)    UPDATE_HL(...,sb,c) src/bsr/BSR_MAT3/mod_intmat.f90:132
+!  and, for Dirac partial waves (DBSR_MAT3):
+!    gpu_dbs_grid(ks,ns,ksp,ksq,t) DBS_grid / DBS_gauss module state
+!    gpu_mrk_dirac(family, k, rkb) mrk_pppp/qqqq/pqpq/qpqp(k)  src/libs/DBS/mrk_pppp.f90:2
+!    gpu_dirac_orbitals(...)       DBS_orbitals_pq state (pq, ipbs, V_ch)
+!    gpu_dirac_r_data(...)         Gen_matrix icase 2 = R_data(itype,kpol,0..3)
+!                                  over every list   src/dbsr/DBSR_MAT3_MPI/R_data.f90:2
+!    gpu_dirac_get(...)            the accumulators hch, hcp, hp, acf, htarg
 !  A non-zero status stops the program with the library's message, as the
 !  reference's Stop does.
 !======================================================================
@@ -123,6 +130,39 @@ module bsr_gpu
       integer(c_int), value :: ntarg
       real(c_double) :: etarg(*)
     end function
+    integer(c_int) function bsrg_dbs_set_grid(ctx, ks, ns, ksp, ksq, t) bind(C, name="bsrg_dbs_set_grid")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ks, ns, ksp, ksq
+      real(c_double) :: t(*)
+    end function
+    integer(c_int) function bsrg_dbs_mrk(ctx, family, k, rkb) bind(C, name="bsrg_dbs_mrk")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: family, k
+      real(c_double) :: rkb(*)
+    end function
+    integer(c_int) function bsrg_dirac_set_orbitals(ctx, nbf, pq, ipbs, nv_ch, i_ch, j_ch, v_ch) &
+        bind(C, name="bsrg_dirac_set_orbitals")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nbf, nv_ch
+      real(c_double) :: pq(*), v_ch(*)
+      integer(c_int) :: ipbs(*), i_ch(*), j_ch(*)
+    end function
+    integer(c_int) function bsrg_dirac_assemble(ctx, nch, npert, mk, ntarg, iptar, nrec, itype, kpol, &
+        k1, k2, k3, k4, c) bind(C, name="bsrg_dirac_assemble")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nch, npert, mk, ntarg, nrec
+      integer(c_int) :: iptar(*), itype(*), kpol(*), k1(*), k2(*), k3(*), k4(*)
+      real(c_double) :: c(*)
+    end function
+    integer(c_int) function bsrg_dirac_get(ctx, hch, hcp, hp, acf, htarg) bind(C, name="bsrg_dirac_get")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: hch(*), hcp(*), hp(*), acf(*), htarg(*)
+    end function
   end interface
 
 contains
@@ -209,6 +249,39 @@ contains
     real(c_double), intent(in) :: cdata(*)
     real(8), intent(in) :: eps_c
     call gpu_check(bsrg_odata(gpu_ctx, ncdata, ipt, k3, k4, cdata, eps_c))
+  end subroutine
+
+  ! ---- Dirac partial waves ----------------------------------------------
+  subroutine gpu_dbs_grid(ks, ns, ksp, ksq, t)
+    integer, intent(in) :: ks, ns, ksp, ksq
+    real(8), intent(in) :: t(*)
+    call gpu_check(bsrg_dbs_set_grid(gpu_ctx, ks, ns, ksp, ksq, t))
+  end subroutine
+
+  subroutine gpu_mrk_dirac(family, k, rkb)
+    integer, intent(in) :: family, k   ! 0 pppp, 1 qqqq, 2 pqpq, 3 qpqp
+    real(8), intent(out) :: rkb(*)
+    call gpu_check(bsrg_dbs_mrk(gpu_ctx, family, k, rkb))
+  end subroutine
+
+  subroutine gpu_dirac_orbitals(nbf, pq, ipbs, nv_ch, i_ch, j_ch, v_ch)
+    integer, intent(in) :: nbf, nv_ch
+    real(8), intent(in) :: pq(*), v_ch(*)
+    integer, intent(in) :: ipbs(*), i_ch(*), j_ch(*)
+    call gpu_check(bsrg_dirac_set_orbitals(gpu_ctx, nbf, pq, ipbs, nv_ch, i_ch, j_ch, v_ch))
+  end subroutine
+
+  subroutine gpu_dirac_r_data(nch, npert, mk, ntarg, iptar, nrec, itype, kpol, k1, k2, k3, k4, c)
+    integer, intent(in) :: nch, npert, mk, ntarg, nrec
+    integer, intent(in) :: iptar(*), itype(*), kpol(*), k1(*), k2(*), k3(*), k4(*)
+    real(8), intent(in) :: c(*)
+    call gpu_check(bsrg_dirac_assemble(gpu_ctx, nch, npert, mk, ntarg, iptar, nrec, itype, kpol, &
+                                       k1, k2, k3, k4, c))
+  end subroutine
+
+  subroutine gpu_dirac_get(hch, hcp, hp, acf, htarg)
+    real(8), intent(out) :: hch(*), hcp(*), hp(*), acf(*), htarg(*)
+    call gpu_check(bsrg_dirac_get(gpu_ctx, hch, hcp, hp, acf, htarg))
   end subroutine
 
 end module bsr_gpu
