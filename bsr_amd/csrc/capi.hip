@@ -469,6 +469,8 @@ int upload_terms(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
     HIPCHK(st.d.resize(st.bytes));
     HIPCHK(hipMemcpyAsync(st.d.p, st.h.p, st.bytes, hipMemcpyHostToDevice, s));
   }
+  HIPCHK(c->bsum.resize(M.nblk() * c->G.ns * c->G.ks));
+  HIPCHK(c->rfac.resize(std::max<size_t>(st.nr1, 1)));
   HIPCHK(hipEventRecord(st.done, s));
   st.pending = true;
   return 0;
@@ -552,7 +554,7 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, hi
   hipStream_t s = c->stream;
   launch_assemble(ns, ks, st.nblk, overwrite ? 0 : 1, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
                   st.dp<double>(st.o_bc), c->bands.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
-                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p, s);
+                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p, c->bsum.p, c->rfac.p, s);
   launch_vec_assemble(ns, st.nq, overwrite ? 0 : 1, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid),
                       st.dp<int>(st.o_vs), st.dp<double>(st.o_vc), c->St.p, c->VT.p, M.hcb.p, s);
   HIPCHK(hipGetLastError());

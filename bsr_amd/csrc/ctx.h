@@ -352,6 +352,7 @@ struct bsrg_ctx {
   std::vector<Terms*> partsS, partsH;        // term parts of S and H in list order
   int last_engine = -1, forced_engine = -1;
   DevVec<double> dtmp, dtmp2;
+  DevVec<double> bsum, rfac;  // term-pass scratch: summed band of each block, factor of each rank-1 term
 
   // Dirac B-spline grid and the resident bands of the four families
   struct Dbs {

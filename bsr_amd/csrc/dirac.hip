@@ -691,6 +691,8 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
                        reinterpret_cast<const double*>(W.xd.p + W.o_xc), W.coef.p);
     HIPCHK(hipGetLastError());
   }
+  HIPCHK(c->bsum.resize(4 * W.nblk * ns * ks));
+  HIPCHK(c->rfac.resize(std::max<size_t>(st.nr1, 1)));
   HIPCHK(hipEventRecord(st.done, s));
   st.pending = true;
   const auto h2 = std::chrono::steady_clock::now();
@@ -715,7 +717,7 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   (void)hipEventRecord(c->ev[3], s);
   launch_assemble(ns, ks, static_cast<int>(4 * W.nblk), 0, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
                   st.dp<double>(st.o_bc), W.Dc.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
-                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), W.Sc.p, W.VT.p, W.hch.p, s, W.hq.p,
+                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), W.Sc.p, W.VT.p, W.hch.p, c->bsum.p, c->rfac.p, s, W.hq.p,
                   static_cast<int>(W.nblk), qmask);
   HIPCHK(hipGetLastError());
   (void)hipEventRecord(c->ev[4], s);

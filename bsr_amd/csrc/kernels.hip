@@ -452,9 +452,29 @@ __global__ void k_sb_upper(int ns, int ks, const double* sb, double* sbu) {
 // written.  Term lists are CSR over all blocks (ptr has nblk+1 entries).
 // grid (nblk, ceil(ns/kAsmCols), ceil(ns/256)), block 256: lane -> rows
 // x0+lane+64a (a<4), wave -> kAsmCpw columns yt+kAsmCpw*wave+b.
-// The band terms of the block are summed once per workgroup into the band
-// entries (i, ith) its columns touch, staged in LDS; rank-1 terms go 16 at a
+// The band entries (i, ith) the workgroup's columns touch come from the
+// block's summed band (k_term_prep) through LDS; rank-1 terms go 16 at a
 // time through LDS (u over the rows, f*w over the columns).
+// Per-block preparation of the term pass: the summed band of every block
+// with band terms, bsum[blk](i, ith) = sum of c_t * bands[bid_t](i, ith) over
+// the block's terms in list order, and the factor of every rank-1 term,
+// rf[t] = c_t * (stab[sidx_t] if sidx_t >= 0 else 1).
+// grid (nblk, ceil(ns*ks/256)), block 256.
+__global__ void k_term_prep(int ns, int ks, const int* __restrict__ bptr, const int* __restrict__ bid,
+                            const double* __restrict__ bc, const double* __restrict__ bands, double* __restrict__ bsum,
+                            const int* __restrict__ rptr, const int* __restrict__ rs, const double* __restrict__ rc,
+                            const double* __restrict__ stab, double* __restrict__ rf) {
+  const int blk = blockIdx.x;
+  if (blockIdx.y == 0)
+    for (int t = rptr[blk] + threadIdx.x; t < rptr[blk + 1]; t += 256) rf[t] = rc[t] * (rs[t] >= 0 ? stab[rs[t]] : 1.0);
+  const int b0 = bptr[blk], b1 = bptr[blk + 1];
+  const int nsk = ns * ks, e = blockIdx.y * 256 + threadIdx.x;
+  if (b0 == b1 || e >= nsk) return;
+  double s = 0.0;
+  for (int t = b0; t < b1; ++t) s += bc[t] * bands[static_cast<size_t>(bid[t]) * nsk + e];
+  bsum[static_cast<size_t>(blk) * nsk + e] = s;
+}
+
 // Output map: with qnblk = 0 block blk is hcc[blk] (ns x ns, in place);
 // with qnblk > 0 the blocks are the quadrants q = 2a+b of qnblk channel
 // blocks of order 2ns (Dirac): base[blk] is read from the quadrant-major
@@ -470,17 +490,15 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
                                                   const int* __restrict__ bid, const double* __restrict__ bc,
                                                   const double* __restrict__ bands, const int* __restrict__ rptr,
                                                   const int* __restrict__ ru, const int* __restrict__ rw,
-                                                  const int* __restrict__ rs, const double* __restrict__ rc,
-                                                  const double* __restrict__ stab, const double* __restrict__ V,
+                                                  const double* __restrict__ rf, const double* __restrict__ V,
                                                   double* __restrict__ hcc, const double* __restrict__ base,
-                                                  int qnblk, int qmask) {
+                                                  int qnblk, int qmask, const double* __restrict__ bsum) {
   const int blk = blockIdx.x;
   const int b0 = bptr[blk], b1 = bptr[blk + 1], r0 = rptr[blk], r1 = rptr[blk + 1];
   if (qnblk == 0 && has_base && b0 == b1 && r0 == r1) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr int R = 64 * A;  // rows per tile
   const int x0 = blockIdx.z * R, yt = blockIdx.y * kAsmCols, y0 = yt + wv * kAsmCpw;
-  const int nrow = min(R, ns - x0);
   const double* hin = nullptr;
   double* h;
   size_t ldo = ns;
@@ -496,36 +514,17 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
   __shared__ double su[16][R];
   __shared__ double sw[16][kAsmCols];
   __shared__ double sband[(kAsmCols + kMaxKs - 1) * kMaxKs];
-  __shared__ int tbid[64], tu[64], tw[64];
-  __shared__ double tbc[64], tf[64];
-  // band entries (i, ith) for i in [yt-ks+1, yt+kAsmCols): sband[(i-ilo)*ks + ith]
+  __shared__ int tu[64], tw[64];
+  __shared__ double tf[64];
+  // band entries (i, ith) for i in [yt-ks+1, yt+kAsmCols): sband[(i-ilo)*ks + ith],
+  // from the block's summed band (k_term_prep)
   const int ilo = yt - (ks - 1), nent = (kAsmCols + ks - 1) * ks;
   const bool has_band = b1 > b0;
   if (has_band) {
-    double e[3] = {0.0, 0.0, 0.0};  // nent <= 47*16 = 752 <= 3*256
-    for (int tb = b0; tb < b1; tb += 64) {
-      const int nt = min(64, b1 - tb);
-      __syncthreads();
-      if (threadIdx.x < nt) {
-        tbid[threadIdx.x] = bid[tb + threadIdx.x];
-        tbc[threadIdx.x] = bc[tb + threadIdx.x];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        const int q = threadIdx.x + 256 * m;
-        if (q >= nent) break;
-        const int i = ilo + q / ks, ith = q % ks;
-        if (i < 0 || i + ith >= ns) continue;
-        double s = e[m];
-        for (int t = 0; t < nt; ++t) s += tbc[t] * bands[static_cast<size_t>(tbid[t]) * ns * ks + i + ns * ith];
-        e[m] = s;
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const int q = threadIdx.x + 256 * m;
-      if (q < nent) sband[q] = e[m];
+    const double* bs = bsum + static_cast<size_t>(blk) * ns * ks;
+    for (int q = threadIdx.x; q < nent; q += 256) {
+      const int i = ilo + q / ks, ith = q % ks;
+      sband[q] = (i >= 0 && i + ith < ns) ? bs[i + ns * ith] : 0.0;
     }
   }
   // the sum starts from the base values, loaded before the rank-1 work so
@@ -540,15 +539,15 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
       acc[a][b] = hin ? hin[x + static_cast<size_t>(ns) * y] : 0.0;
     }
   }
-  // rank-1 terms: metadata and scale factors of 64 terms at a time into LDS
-  // (one round of loads), then their vectors 16 at a time
+  // rank-1 terms: metadata and factors of 64 terms at a time into LDS (one
+  // round of loads), then their vectors 16 at a time
+  const int nrow = min(R, ns - x0);
   for (int tc = r0; tc < r1; tc += 64) {
     const int nc = min(64, r1 - tc);
     __syncthreads();
     if (threadIdx.x < nc) {
       const int t = tc + threadIdx.x;
-      const int sx = rs[t];
-      tf[threadIdx.x] = rc[t] * (sx >= 0 ? stab[sx] : 1.0);
+      tf[threadIdx.x] = rf[t];
       tu[threadIdx.x] = ru[t];
       tw[threadIdx.x] = rw[t];
     }
@@ -578,41 +577,77 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
     }
   }
   __syncthreads();  // sband complete
+  // a wave's 64 rows of group a meet the band of its columns only when they
+  // lie within ks-1 of them (a wave-uniform test)
+  bool band_a[A];
+#pragma unroll
+  for (int a = 0; a < A; ++a)
+    band_a[a] = has_band && x0 + 64 * a + 63 > y0 - ks && x0 + 64 * a < y0 + kAsmCpw + ks - 1;
 #pragma unroll
   for (int b = 0; b < kAsmCpw; ++b) {
     const int y = y0 + b;
     if (y >= ns) continue;
+    double* hy = h + ldo * y;
 #pragma unroll
     for (int a = 0; a < A; ++a) {
       const int x = x0 + lane + 64 * a;
       if (x >= ns) continue;
       double v = acc[a][b];
-      if (has_band) {
-        const int ith = x > y ? x - y : y - x, i = min(x, y);
-        if (ith < ks && i + ith < ns) v += sband[(i - ilo) * ks + ith];
+      if (band_a[a]) {
+        const int d = x - y;
+        if (d > -ks && d < ks) v += sband[(min(x, y) - ilo) * ks + (d < 0 ? -d : d)];
       }
-      h[x + ldo * y] = v;
+      hy[x] = v;
     }
   }
 }
 
 // Channel-perturber vectors: hcb(:, q) = base + sum of c * f * V(:, id), list
-// order; CSR over all nq = nch*ncp positions.  grid (nq), block 64*ceil(ns/64)<=256
-__global__ void k_vec_assemble(int ns, int has_base, const int* __restrict__ ptr, const int* __restrict__ vid,
-                               const int* __restrict__ sidx, const double* __restrict__ c,
-                               const double* __restrict__ stab, const double* __restrict__ V,
-                               double* __restrict__ hcb) {
-  const int q = blockIdx.x;
+// order; CSR over all nq = nch*ncp positions.  One wavefront per vector,
+// four vectors per workgroup: the lanes load the factors and vector ids of
+// up to 64 terms at once and broadcast them; each lane keeps the elements
+// x0 + lane + 64j (j < 4) of a 256-element chunk in registers.
+// grid (ceil(nq/4)), block 256.
+__global__ __launch_bounds__(256) void k_vec_assemble(int ns, int nq, int has_base, const int* __restrict__ ptr,
+                                                      const int* __restrict__ vid, const int* __restrict__ sidx,
+                                                      const double* __restrict__ c, const double* __restrict__ stab,
+                                                      const double* __restrict__ V, double* __restrict__ hcb) {
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (q >= nq) return;
   const int t0 = ptr[q], t1 = ptr[q + 1];
   if (has_base && t0 == t1) return;
-  for (int x = threadIdx.x; x < ns; x += blockDim.x) {
-    double acc = 0.0;
-    for (int t = t0; t < t1; ++t) {
-      const double f = c[t] * (sidx[t] >= 0 ? stab[sidx[t]] : 1.0);
-      acc += f * V[static_cast<size_t>(vid[t]) * ns + x];
+  double* p = hcb + static_cast<size_t>(q) * ns;
+  for (int x0 = 0; x0 < ns; x0 += 256) {
+    double acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int x = x0 + lane + 64 * j;
+      acc[j] = (has_base && x < ns) ? p[x] : 0.0;
     }
-    double* p = hcb + static_cast<size_t>(q) * ns + x;
-    *p = has_base ? *p + acc : acc;
+    for (int tc = t0; tc < t1; tc += 64) {
+      const int n = min(64, t1 - tc);
+      double fl = 0.0;
+      int vl = 0;
+      if (lane < n) {
+        const int t = tc + lane;
+        fl = c[t] * (sidx[t] >= 0 ? stab[sidx[t]] : 1.0);
+        vl = vid[t];
+      }
+      for (int tt = 0; tt < n; ++tt) {
+        const double f = __shfl(fl, tt, 64);
+        const double* v = V + static_cast<size_t>(__shfl(vl, tt, 64)) * ns;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int x = x0 + lane + 64 * j;
+          if (x < ns) acc[j] += f * v[x];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int x = x0 + lane + 64 * j;
+      if (x < ns) p[x] = acc[j];
+    }
   }
 }
 
@@ -796,16 +831,18 @@ void launch_sb_upper(int ns, int ks, const double* sb, double* sbu, hipStream_t 
 
 void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, const int* bid, const double* bc,
                      const double* bands, const int* rptr, const int* ru, const int* rw, const int* rs,
-                     const double* rc, const double* stab, const double* V, double* hcc, hipStream_t s,
-                     const double* base, int qnblk, int qmask) {
+                     const double* rc, const double* stab, const double* V, double* hcc, double* bsum,
+                     double* rf, hipStream_t s, const double* base, int qnblk, int qmask) {
   if (nblk <= 0) return;
+  hipLaunchKernelGGL(k_term_prep, dim3(nblk, (ns * ks + 255) / 256), dim3(256), 0, s, ns, ks, bptr, bid, bc, bands,
+                     bsum, rptr, rs, rc, stab, rf);
   const int A = ns <= 64 ? 1 : (ns <= 128 ? 2 : (ns <= 192 ? 3 : 4));
   const dim3 grid(nblk, (ns + kAsmCols - 1) / kAsmCols, (ns + 64 * A - 1) / (64 * A));
   switch (A) {
 #define ASM_CASE(N)                                                                                             \
   case N:                                                                                                       \
     hipLaunchKernelGGL(k_assemble<N>, grid, dim3(256), 0, s, ns, ks, has_base, bptr, bid, bc, bands, rptr, ru, rw, \
-                       rs, rc, stab, V, hcc, base, qnblk, qmask);                                               \
+                       rf, V, hcc, base, qnblk, qmask, bsum);                                                   \
     break;
     ASM_CASE(1) ASM_CASE(2) ASM_CASE(3) ASM_CASE(4)
 #undef ASM_CASE
@@ -815,8 +852,8 @@ void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, co
 void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,
                          const double* c, const double* stab, const double* V, double* hcb, hipStream_t s) {
   if (nq <= 0) return;
-  const int bs = std::min(256, 64 * ((ns + 63) / 64));
-  hipLaunchKernelGGL(k_vec_assemble, dim3(nq), dim3(bs), 0, s, ns, has_base, ptr, vid, sidx, c, stab, V, hcb);
+  hipLaunchKernelGGL(k_vec_assemble, dim3((nq + 3) / 4), dim3(256), 0, s, ns, nq, has_base, ptr, vid, sidx, c, stab,
+                     V, hcb);
 }
 
 void launch_symmetrize_diag(int ns, int nch, double* hcc, hipStream_t s) {

@@ -15,6 +15,7 @@
 // 2^-7S * max|row| * max|col| * K.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -317,30 +318,62 @@ __global__ void k_slice8_cols(int n, int K, int np, int Kp, int S, const double*
 // of C (B slices), MFMA columns the m rows of C (A slices), so a wave's
 // output lanes run along the rows of C and the FP64 stores are coalesced.
 // Workgroup tile 64 (C columns) x 64 (C rows), 4 waves of 32 x 32
-// (2 x 2 MFMA 16x16x64 i8), K in chunks of 64 staged in LDS (rows padded to
-// 80 bytes: conflict-free 16-byte reads).  Every level L = s + r <= S-1 has
+// (2 x 2 MFMA 16x16x64 i8), K in chunks of 64 staged in LDS (swizzled
+// 64-byte rows, sw8).  Every level L = s + r <= S-1 has
 // its own int32 accumulators; the epilogue recombines them in FP64.  Two
 // waves per SIMD are asked for up to S = 7 (the accumulators then live in
 // VGPRs; at S = 7 the default allocation would leave one wave per SIMD);
 // S = 8 keeps one (its LDS tiles allow one workgroup per CU anyway).
-constexpr int kPad8 = 80;
+// LDS tiles: 64 rows of 64 bytes; the 16-byte piece q of row r sits at
+// piece q ^ ((r >> 2) & 3), so that the staging writes (4 lanes a row) and
+// the fragment reads (16 rows, one piece) both spread over all banks.
+__device__ __forceinline__ int sw8(int r, int byte) { return r * 64 + ((((byte >> 4) ^ (r >> 2)) & 3) << 4); }
 template <int S>
 __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, int mp, int np, int Kp, const int8_t* __restrict__ As,
                                                  const int8_t* __restrict__ Bs, const double* __restrict__ sig,
                                                  const double* __restrict__ tau, double* __restrict__ C, size_t ldc,
-                                                 int accumulate) {
-  __shared__ __attribute__((aligned(16))) int8_t sB[S][64][kPad8];  // C columns (blocks)
-  __shared__ __attribute__((aligned(16))) int8_t sA[S][64][kPad8];  // C rows (positions)
+                                                 int accumulate, int sc) {
+  __shared__ __attribute__((aligned(16))) int8_t sB[S][64 * 64];  // C columns (blocks)
+  __shared__ __attribute__((aligned(16))) int8_t sA[S][64 * 64];  // C rows (positions)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // 1-D grid over (row tile, column tile).  Workgroups are dealt to the 8
-  // XCDs round robin; renumber them so that the column tiles of one row
-  // tile, which share its A slices, run on one XCD and hit its L2.
-  const int nct = np / 64, nwg = static_cast<int>(gridDim.x), full = nwg - nwg % 8;
+  // XCDs round robin; renumber them so that each XCD gets a contiguous run.
+  // The run is ordered in column supertiles of sc column tiles (their B
+  // slices stay in the XCD's L2) and, within one, row tile by row tile, so
+  // the sc workgroups of a row tile share its A slices too.
+  const int nct = np / 64, nrt = mp / 64, nwg = static_cast<int>(gridDim.x), full = nwg - nwg % 8;
   const int hw = static_cast<int>(blockIdx.x);
-  const int id = hw < full ? (hw % 8) * (full / 8) + hw / 8 : hw;
-  const int c0 = (id % nct) * 64, r0 = (id / nct) * 64;  // C column / row tile origins
+  int id = hw < full ? (hw % 8) * (full / 8) + hw / 8 : hw;
+  const int nfull = nct / sc, per = nrt * sc;
+  int cs0, cw;
+  if (id < nfull * per) {
+    cs0 = (id / per) * sc;
+    id %= per;
+    cw = sc;
+  } else {
+    id -= nfull * per;
+    cs0 = nfull * sc;
+    cw = nct - cs0;
+  }
+  const int c0 = (cs0 + id % cw) * 64, r0 = (id / cw) * 64;  // C column / row tile origins
   const int wc = (wave & 1) * 32, wr = (wave >> 1) * 32;  // wave tile within: columns, rows
   const int li = lane & 15, lk = (lane >> 4) * 16;
+  // up to S = 6 the old values of C are loaded before the K loop (their
+  // latency hides behind it; 32 more VGPRs still leave two waves per SIMD)
+  constexpr bool kPre = S <= 6;
+  double old[2][2][4];
+  if (kPre && accumulate) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = min(r0 + wr + b * 16 + li, m - 1);
+          const int col = min(c0 + wc + a * 16 + (lane >> 4) * 4 + e, n - 1);
+          old[a][b][e] = C[row + ldc * col];
+        }
+  }
   i32x4 acc[S][2][2];
 #pragma unroll
   for (int L = 0; L < S; ++L)
@@ -363,8 +396,8 @@ __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, in
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      *reinterpret_cast<i32x4*>(&sB[s][pr][pk]) = gb[s];
-      *reinterpret_cast<i32x4*>(&sA[s][pr][pk]) = ga[s];
+      *reinterpret_cast<i32x4*>(&sB[s][sw8(pr, pk)]) = gb[s];
+      *reinterpret_cast<i32x4*>(&sA[s][sw8(pr, pk)]) = ga[s];
     }
     __syncthreads();
     if (kc + 64 < Kp) {
@@ -379,12 +412,12 @@ __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, in
 #pragma unroll
     for (int s = 0; s < S; ++s)
 #pragma unroll
-      for (int t = 0; t < 2; ++t) av[s][t] = *reinterpret_cast<const i32x4*>(&sA[s][wr + t * 16 + li][lk]);
+      for (int t = 0; t < 2; ++t) av[s][t] = *reinterpret_cast<const i32x4*>(&sA[s][sw8(wr + t * 16 + li, lk)]);
 #pragma unroll
     for (int r = 0; r < S; ++r) {
       i32x4 bv[2];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) bv[t] = *reinterpret_cast<const i32x4*>(&sB[r][wc + t * 16 + li][lk]);
+      for (int t = 0; t < 2; ++t) bv[t] = *reinterpret_cast<const i32x4*>(&sB[r][sw8(wc + t * 16 + li, lk)]);
 #pragma unroll
       for (int s = 0; s < S - r; ++s)
 #pragma unroll
@@ -414,17 +447,18 @@ __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, in
       }
     }
   if (accumulate) {
-    double old[2][2][4];
+    if (!kPre) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int row = min(r0 + wr + b * 16 + li, m - 1);
-          const int col = min(c0 + wc + a * 16 + (lane >> 4) * 4 + e, n - 1);
-          old[a][b][e] = C[row + ldc * col];
-        }
+          for (int e = 0; e < 4; ++e) {
+            const int row = min(r0 + wr + b * 16 + li, m - 1);
+            const int col = min(c0 + wc + a * 16 + (lane >> 4) * 4 + e, n - 1);
+            old[a][b][e] = C[row + ldc * col];
+          }
+    }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -448,7 +482,10 @@ __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, in
 template <int S>
 void launch_k8(dim3 g, hipStream_t s, int m, int n, int mp, int np, int Kp, const int8_t* As, const int8_t* Bs,
                const double* sig, const double* tau, double* C, size_t ldc, int acc) {
-  hipLaunchKernelGGL(k_ozaki8<S>, g, dim3(256), 0, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc);
+  // column supertile: the B slices of sc column tiles in about 1.5 MB
+  const size_t tile = static_cast<size_t>(S) * 64 * Kp;
+  const int sc = std::max(1, std::min(np / 64, static_cast<int>((3u << 19) / tile)));
+  hipLaunchKernelGGL(k_ozaki8<S>, g, dim3(256), 0, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, sc);
 }
 
 template <int S>

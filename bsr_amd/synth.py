@@ -136,13 +136,14 @@ class PartialWave:
         return tuple(np.concatenate(p) for p in parts)
 
     def footprint_bytes(self):
-        """Device bytes the wave needs (matrices S and H, bands, bases)."""
+        """Device bytes the wave needs (matrices S and H, bands, bases, the
+        summed band of each block)."""
         ns, ks = self.ns, self.ks
         nblk = self.nch * (self.nch + 1) // 2
         mats = 2 * 8 * (nblk * ns * ns + ns * self.nch * self.ncp)
         bands = 8 * (self.mk + 1) * ns * ns * ks * ks
         basis = 8 * (self.mk + 1) * self.nb * self.nb * ns * ns
-        return mats + bands + basis
+        return mats + bands + basis + 8 * nblk * ns * ks
 
 
 def _bound_orbitals(rng, g: G.Grid, nb: int):

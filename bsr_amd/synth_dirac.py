@@ -117,7 +117,8 @@ class DiracWave:
     def footprint_bytes(self):
         """Device memory of one assembly: bands of four families, exchange
         bases, channel blocks (quadrant-major and in the output layout),
-        channel-perturber vectors and the exchange coefficients."""
+        channel-perturber vectors, the exchange coefficients and the summed
+        bands of the quadrant blocks."""
         ns, ks, nk, nb = self.ns, self.ks, self.mk + 1, self.nb
         nblk = self.nch * (self.nch + 1) // 2
         bands = 4 * nk * ns * ns * ks * ks
@@ -125,7 +126,8 @@ class DiracWave:
         blocks = 2 * nblk * self.ms * self.ms
         vecs = 2 * self.ms * self.nch * self.npert
         coef = 4 * nk * nb * nb * nblk
-        return 8 * (bands + exch + blocks + vecs + coef)
+        bsum = 4 * nblk * ns * ks
+        return 8 * (bands + exch + blocks + vecs + coef + bsum)
 
 
 def dirac_grid(ks: int, ns: int, z: float, rmax: float) -> np.ndarray:
