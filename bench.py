@@ -9,9 +9,12 @@ k, orbital contractions, list processing, and the S and H blocks complete on
 the device, in plain FP64.  value = unique H elements nhm(nhm+1)/2 per second.
 
 Beside the headline line's required fields, the JSON line carries:
+  eigen    the lowest 10 eigenvalues of H x = E S x of the wave (FP64 on
+           the device) and the time of that solve;
   split    the same wave with the exchange contraction in split precision
-           (int8 Ozaki slices; split_bf16: bf16 slices), and its largest
-           relative element error against the FP64 result;
+           (int8 Ozaki slices; split_bf16: bf16 slices), its largest
+           relative element error against the FP64 result and (int8, the
+           served form) its largest lowest-10 eigenvalue error;
   serving  a Poisson stream (20 jobs/s) of waves of the same target
            (nch ~ U[40,80], ncp ~ U[200,400]) through the serving loop, in
            FP64 and in split precision: p50/p99 latency at that rate, the
@@ -258,6 +261,7 @@ def main():
     ap.add_argument("--no-split", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue check of the split forms")
     ap.add_argument("--serve-jobs", type=int, default=60)
     ap.add_argument("--batch-waves", type=int, default=48)
     ap.add_argument("--no-dirac", action="store_true")
@@ -270,6 +274,7 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); partial waves shard as whole jobs "
               "per GPU, so the per-GPU figure is the N=1 line.", file=sys.stderr)
         sys.exit(2)
+    import numpy as np
     from bsr_amd import Context, synth
     if a.dirac_only:
         d = dirac_c5(a.dirac_steps, 1, not a.no_cpu)
@@ -309,6 +314,15 @@ def main():
                      "kernel": "exchange contraction GEMM (FP64)", "kernel_ms": gemm_ms,
                      "flops_per_launch": flops},
     }
+    e_ref = None
+    if not a.no_eigen:
+        # lowest 10 eigenvalues of H x = E S x (FP64 on the device), the
+        # reference point of the split forms' eigenvalue error
+        t0 = time.perf_counter()
+        e_ref = ctx.wave_eigen(10)
+        eig_s = time.perf_counter() - t0
+        log(f"eigen: lowest {e_ref[:3]} ... in {eig_s:.2f} s (nhm={w.nhm})")
+        out["eigen"] = {"lowest10": [float(x) for x in e_ref], "solve_s": eig_s}
     if not a.no_split:
         # int8 slices (the split form used below) and bf16 slices
         for key, mode in (("split", 2), ("split_bf16", 1)):
@@ -319,6 +333,10 @@ def main():
             log(f"{key}: {sms:.3f} ms/step, GEMM {sdev['gemm']:.3f} ms, max rel element error vs FP64 {err:.2e}")
             out[key] = {"value": w.n_elements / (sms * 1e-3), "ms_per_step": sms, "gemm_ms": sdev["gemm"],
                         "engine": sctx.wave_engine(), "max_rel_err_vs_fp64": err, "vs_fp64_plain": ms / sms}
+            if e_ref is not None and key == "split":
+                de = float(np.max(np.abs(sctx.wave_eigen(10) - e_ref)))
+                out[key]["max_eig_err_vs_fp64"] = de
+                log(f"{key}: lowest-10 eigenvalue error vs FP64 {de:.2e}")
             sctx.close()
     ctx.close()
     if not a.no_serve:

@@ -277,6 +277,14 @@ class Context:
         L.check(L.lib().bsrg_wave_diff(self._p, ref._p, which, floor_frac, C.byref(out)))
         return out.value
 
+    def wave_eigen(self, nev: int = 10) -> np.ndarray:
+        """The nev lowest eigenvalues of H x = E S x of the last wave (FP64,
+        on the device; the closing steps of diag_hd,
+        src/bsr/BSR_HD3/diag_hd.f90:271-312, on the full basis)."""
+        e = np.zeros(nev)
+        L.check(L.lib().bsrg_wave_eigen(self._p, nev, e))
+        return e
+
     def drop_caches(self):
         """Forget resident bands and bases (the next wave rebuilds them)."""
         L.check(L.lib().bsrg_drop_caches(self._p))

@@ -130,6 +130,12 @@ module bsr_gpu
       integer(c_int), value :: ntarg
       real(c_double) :: etarg(*)
     end function
+    integer(c_int) function bsrg_wave_eigen(ctx, nev, e) bind(C, name="bsrg_wave_eigen")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nev
+      real(c_double) :: e(*)
+    end function
     integer(c_int) function bsrg_dbs_set_grid(ctx, ks, ns, ksp, ksq, t) bind(C, name="bsrg_dbs_set_grid")
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx

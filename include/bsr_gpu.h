@@ -277,6 +277,15 @@ int bsrg_graph_destroy(bsrg_graph* graph);
  * Both contexts must be on one device.  No counterpart in the reference. */
 int bsrg_wave_diff(bsrg_ctx* a, bsrg_ctx* ref, int which, double floor_frac, double* rel);
 
+/* After bsrg_wave_assemble: the nev lowest eigenvalues e[0..nev) of the
+ * generalized problem H x = E S x of the wave, FP64 on the device: the final
+ * DPOTRF / DSYGST / DSYEV steps of BSR_HD3's diag_hd
+ * (src/bsr/BSR_HD3/diag_hd.f90:271-312) on the full basis (nhm = nch*ns +
+ * ncp, no per-channel truncation as diag_hd.f90:78-190 makes first).  Uses
+ * 16 nhm^2 bytes of device memory during the call.  -40: S not positive
+ * definite; -41: no convergence. */
+int bsrg_wave_eigen(bsrg_ctx* ctx, int nev, double* e);
+
 /* Forget the resident R^k bands and orbital bases (of the BS and of the
  * Dirac partial waves) so the next call rebuilds them (bsrg_set_grid,
  * bsrg_set_orbitals, bsrg_dbs_set_grid and bsrg_dirac_set_orbitals keep them

@@ -168,6 +168,10 @@ def test_fused_wave_c1(ctx, oracle_lib, tmp_path):
     e_o = sl.eigh(O.dense_from_wave(oH), O.dense_from_wave(oS), eigvals_only=True, subset_by_index=[0, 9])
     print("[c1] eig", e_o[:3], np.abs(e_g - e_o).max(), flush=True)
     assert np.abs(e_g - e_o).max() <= 1e-9
+    # the same eigenvalues from the device solver (wave_eigen, rocSOLVER)
+    e_d = ctx.wave_eigen(10)
+    print("[c1] device eig err", np.abs(e_d - e_o).max(), flush=True)
+    assert np.abs(e_d - e_o).max() <= 1e-9
 
 
 def test_fused_wave_with_perturbers(ctx, oracle_lib):
@@ -182,6 +186,12 @@ def test_fused_wave_with_perturbers(ctx, oracle_lib):
     oS, oH = O.run_wave(w)
     _cmp_mats(S, oS, "S")
     _cmp_mats(H, oH, "H")
+    # lowest 10 eigenvalues with the perturber rows and columns in the basis
+    import scipy.linalg as sl
+    e_o = sl.eigh(O.dense_from_wave(oH), O.dense_from_wave(oS), eigvals_only=True, subset_by_index=[0, 9])
+    e_d = ctx.wave_eigen(10)
+    print("[c2-small] device eig err", np.abs(e_d - e_o).max(), flush=True)
+    assert np.abs(e_d - e_o).max() <= 1e-9
 
 
 def test_fused_equals_call_by_call_and_is_deterministic(ctx):
@@ -214,6 +224,10 @@ def test_errors_are_reported(ctx):
         ctx.i_data(10, 0, lst)
     with pytest.raises(BsrGpuError, match="unknown case"):
         ctx.convol(0, np.zeros((w.ns, w.ns)), 5)
+    from bsr_amd import Context
+    with Context(0) as fresh:
+        with pytest.raises(BsrGpuError, match="no assembled wave"):
+            fresh.wave_eigen(10)
 
 
 def test_fortran_host_driver(oracle_lib, tmp_path):
@@ -256,7 +270,8 @@ def test_fortran_host_driver(oracle_lib, tmp_path):
 def test_split_precision_wave(ctx, oracle_lib, mode, nslices):
     """Exchange contraction as Ozaki slices on bf16 (mode 1) or int8 (mode 2)
     MFMA: every element of H within 1e-10 (floor 1e-3 of max|H|) of the FP64
-    oracle; S unaffected."""
+    oracle, S unaffected, and the lowest 10 eigenvalues of H x = E S x
+    within 1e-9 of the oracle's (LAPACK)."""
     O = oracle_lib
     from bsr_amd import synth
     w = synth.make_wave(71, 8, 120, 18.0, 40.0, nch=6, ncp=10, mk=6, nb=5, ntarg=5, fill=0.1, name="split")
@@ -273,6 +288,11 @@ def test_split_precision_wave(ctx, oracle_lib, mode, nslices):
     assert e <= 1e-10
     assert rel_err(S.hcc, oS.hcc) < TOL
     assert rel_err(H.hcb, oH.hcb) < TOL
+    import scipy.linalg as sl
+    e_o = sl.eigh(O.dense_from_wave(oH), O.dense_from_wave(oS), eigvals_only=True, subset_by_index=[0, 9])
+    de = np.abs(ctx.wave_eigen(10) - e_o).max()
+    print(f"[split] mode={mode} S={nslices} max eigenvalue error {de:.2e}", flush=True)
+    assert de <= 1e-9
 
 
 @pytest.mark.parametrize("engine", [0, 1, 2, 3])
