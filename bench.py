@@ -286,6 +286,8 @@ def main():
     ctx = Context(0)
     ms, dev, host = run_steps(ctx, w, a.warmup, a.steps)
     log(f"fp64: {a.steps} steps, {ms:.3f} ms/step")
+    terms = {"S": ctx.wave_term_counts(0), "H": ctx.wave_term_counts(1)}
+    log("  terms: " + json.dumps(terms))
     log("  device ms: " + json.dumps(dev))
     log("  host ms:   " + json.dumps(host))
     gemm_ms = dev["gemm"]
@@ -308,7 +310,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"partial wave ks=8 ns=200 k<=10 nch={w.nch} ncp={w.ncp} nhm={w.nhm}",
-                   "device_ms_per_step": dev["total"], "device_ms": dev, "host_ms": host},
+                   "device_ms_per_step": dev["total"], "device_ms": dev, "host_ms": host, "terms": terms},
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
                      "frac": ach / PEAK_FP64_MFMA, "traffic": None,
                      "kernel": "exchange contraction GEMM (FP64)", "kernel_ms": gemm_ms,

@@ -269,6 +269,14 @@ class Context:
         return dict(launch=t[0], overlaps=t[1], registry=t[2], decode=t[3], merge=t[4], sort=t[5],
                     upload=t[6], total=t[7], decode_cpu=t[8], decode_longest=t[9])
 
+    def wave_term_counts(self, which: int = 1) -> dict:
+        """Sizes of the staged term sets of S (which=0) or H (which=1)."""
+        n = np.zeros(8, np.int64)
+        L.check(L.lib().bsrg_wave_term_counts(self._p, which, n))
+        keys = ("blocks", "band_blocks", "band_terms", "rank1_blocks", "rank1_terms", "vec_positions", "vec_terms",
+                "live_blocks")
+        return {k: int(v) for k, v in zip(keys, n)}
+
     def wave_diff(self, ref: "Context", which: int = 1, floor_frac: float = 1e-3) -> float:
         """Largest relative element error of this context's last wave against
         the same wave assembled by `ref` (S for which=0, H for which=1), with

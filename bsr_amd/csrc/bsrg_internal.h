@@ -86,11 +86,14 @@ void launch_sb_upper(int ns, int ks, const double* sb, double* sbu, hipStream_t 
 // where bit q of qmask is set, and the sum is written to rows a*ns..,
 // columns b*ns.. of hcc block blk % qnblk.
 // Scratch: bsum of nblk*ns*ks doubles (the summed band of each block), rf
-// of one double per rank-1 term (its factor).
+// of one double per rank-1 term (its factor).  live (optional, one byte per
+// block, qnblk = 0 and has_base = 0 only): 0 marks a block that already
+// holds zeros, which is then not rewritten when it has no terms.
 void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, const int* bid, const double* bc,
                      const double* bands, const int* rptr, const int* ru, const int* rw, const int* rs,
                      const double* rc, const double* stab, const double* V, double* hcc, double* bsum,
-                     double* rf, hipStream_t s, const double* base = nullptr, int qnblk = 0, int qmask = 0);
+                     double* rf, hipStream_t s, const double* base = nullptr, int qnblk = 0, int qmask = 0,
+                     const unsigned char* live = nullptr);
 // hcb[q] = base + sum_r c_r * sfac_r * V[v_r]; CSR over all nq positions
 void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,
                          const double* c, const double* stab, const double* V, double* hcb, hipStream_t s);

@@ -356,7 +356,8 @@ void counting_sort(const std::vector<Terms*>& parts, Vec vec, int nkeys, Key key
 // pinned arena: doubles (coefficients) first, then ints.  The band, rank-1
 // and vector sets are independent and are staged concurrently when a pool
 // is given.
-int stage_terms(const std::vector<Terms*>& parts, Staged& st, int nblk, int nq, Pool* pool) {
+int stage_terms(const std::vector<Terms*>& parts, Staged& st, int nblk, int nq, Pool* pool,
+                const std::vector<unsigned char>* live) {
   if (int rc = wait_staged(st)) return rc;
   st.nband = st.nr1 = st.nvec = 0;
   for (const Terms* P : parts) {
@@ -372,9 +373,12 @@ int stage_terms(const std::vector<Terms*>& parts, Staged& st, int nblk, int nq, 
   st.o_bptr = put(st.nblk + 1, 4); st.o_bid = put(st.nband, 4);
   st.o_rptr = put(st.nblk + 1, 4); st.o_ru = put(st.nr1, 4); st.o_rw = put(st.nr1, 4); st.o_rs = put(st.nr1, 4);
   st.o_vptr = put(st.nq + 1, 4); st.o_vid = put(st.nvec, 4); st.o_vs = put(st.nvec, 4);
+  st.has_live = live && live->size() >= static_cast<size_t>(nblk);
+  st.o_live = put(st.has_live ? nblk : 0, 1);
   st.bytes = o;  // the position scratch below is not uploaded
   const size_t o_pb = put(st.nband, 4), o_pr = put(st.nr1, 4), o_pv = put(st.nvec, 4);
   if (o > st.h.cap) HIPCHK(st.h.reserve(o));
+  if (st.has_live) std::memcpy(st.hp<unsigned char>(st.o_live), live->data(), nblk);
   const std::function<void(int)> task = [&](int which) {
     if (which == 0) {
       int* pos = st.hp<int>(o_pb);
@@ -552,9 +556,13 @@ int autotune_gemm(bsrg_ctx* c, Matrices& M, Staged& st, bool capture_safe) {
 int enqueue_terms(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, hipEvent_t e0, hipEvent_t e1) {
   const int ns = c->G.ns, ks = c->G.ks;
   hipStream_t s = c->stream;
+  // without an exchange GEMM (which reads every block), empty blocks that
+  // already hold zeros are left alone
+  const bool skip = overwrite && st.has_live && !st.has_gemm && !c->capturing;
   launch_assemble(ns, ks, st.nblk, overwrite ? 0 : 1, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
                   st.dp<double>(st.o_bc), c->bands.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
-                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p, c->bsum.p, c->rfac.p, s);
+                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p, c->bsum.p, c->rfac.p, s,
+                  nullptr, 0, 0, skip ? st.dp<unsigned char>(st.o_live) : nullptr);
   launch_vec_assemble(ns, st.nq, overwrite ? 0 : 1, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid),
                       st.dp<int>(st.o_vs), st.dp<double>(st.o_vc), c->St.p, c->VT.p, M.hcb.p, s);
   HIPCHK(hipGetLastError());
@@ -565,6 +573,19 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, hi
   }
   if (e1) (void)hipEventRecord(e1, s);
   return 0;
+}
+
+// After an overwriting term pass of M (and its GEMM) is enqueued: which
+// blocks may now hold nonzeros.  The pass wrote zeros into the empty
+// blocks it visited; a GEMM may add to any block; a captured pass may never
+// run, so its blocks are unknown.
+void update_live(bsrg_ctx* c, Matrices& M, Staged& st) {
+  if (M.live.size() < static_cast<size_t>(st.nblk)) M.live.resize(st.nblk, 1);
+  const int* bp = st.hp<int>(st.o_bptr);
+  const int* rp = st.hp<int>(st.o_rptr);
+  const bool unknown = st.has_gemm || c->capturing;
+  for (int b = 0; b < st.nblk; ++b)
+    M.live[b] = unknown || bp[b + 1] > bp[b] || rp[b + 1] > rp[b] ? 1 : 0;
 }
 
 // Host scalars (HB, Target_h, ACF) of the parts in list order; needs
@@ -1117,8 +1138,12 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   for (int t = 0; t < c->nwork; ++t) c->partsH.push_back(c->work[t].get());
   c->partsH.push_back(&THt);
   const auto h1 = clk::now();
-  if (int rc = stage_terms(c->partsS, c->stS, static_cast<int>(c->S.nblk()), c->S.nch * c->S.ncp, nullptr)) return rc;
-  if (int rc = stage_terms(c->partsH, c->stH, static_cast<int>(c->H.nblk()), c->H.nch * c->H.ncp, c->pool.get())) return rc;
+  if (int rc = stage_terms(c->partsS, c->stS, static_cast<int>(c->S.nblk()), c->S.nch * c->S.ncp, nullptr,
+                          &c->S.live))
+    return rc;
+  if (int rc = stage_terms(c->partsH, c->stH, static_cast<int>(c->H.nblk()), c->H.nch * c->H.ncp, c->pool.get(),
+                          &c->H.live))
+    return rc;
   const auto h2 = clk::now();
   if (int rc = upload_terms(c, c->S, TS, c->stS)) return rc;
   if (int rc = upload_terms(c, c->H, TH, c->stH)) return rc;
@@ -1140,9 +1165,11 @@ int bsrg_wave_launch(bsrg_ctx* c) {
   const bool timed = !c->capturing;  // no event records inside a graph capture
   if (timed) (void)hipEventRecord(c->ev[2], c->stream);
   if (int rc = enqueue_terms(c, c->S, c->stS, true, nullptr, nullptr)) return rc;
+  update_live(c, c->S, c->stS);
   launch_symmetrize_diag(c->G.ns, c->S.nch, c->S.hcc.p, c->stream);
   if (int rc = enqueue_terms(c, c->H, c->stH, true, timed ? c->ev[3] : nullptr, timed ? c->ev[4] : nullptr))
     return rc;
+  update_live(c, c->H, c->stH);
   launch_symmetrize_diag(c->G.ns, c->H.nch, c->H.hcc.p, c->stream);
   HIPCHK(hipGetLastError());
   if (timed) (void)hipEventRecord(c->ev[5], c->stream);
@@ -1190,6 +1217,28 @@ int bsrg_wave_engine(bsrg_ctx* c) { return c ? c->last_engine : -1; }
 int bsrg_wave_host_timings(bsrg_ctx* c, double* ms10) {
   if (int rc = check_ctx(c)) return rc;
   for (int i = 0; i < 10; ++i) ms10[i] = c->host_ms[i];
+  return 0;
+}
+
+int bsrg_wave_term_counts(bsrg_ctx* c, int which, long long* n8) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->wave_prepared) return fail(-8, "no prepared wave (bsrg_wave_prepare)");
+  if (!n8 || which < 0 || which > 1) return fail(-1, "bsrg_wave_term_counts: bad arguments");
+  Staged& st = which == 0 ? c->stS : c->stH;
+  const Matrices& M = which == 0 ? c->S : c->H;
+  const int* bp = st.hp<int>(st.o_bptr);
+  const int* rp = st.hp<int>(st.o_rptr);
+  const int* vp = st.hp<int>(st.o_vptr);
+  long long bb = 0, rb = 0, vq = 0, live = 0;
+  for (int b = 0; b < st.nblk; ++b) {
+    bb += bp[b + 1] > bp[b];
+    rb += rp[b + 1] > rp[b];
+    live += b < static_cast<int>(M.live.size()) && M.live[b];
+  }
+  for (int q = 0; q < st.nq; ++q) vq += vp[q + 1] > vp[q];
+  const long long v[8] = {st.nblk, bb, static_cast<long long>(st.nband), rb, static_cast<long long>(st.nr1),
+                          vq, static_cast<long long>(st.nvec), live};
+  std::copy(v, v + 8, n8);
   return 0;
 }
 

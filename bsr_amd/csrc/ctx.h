@@ -70,13 +70,20 @@ struct Matrices {
   int nch = 0, ncp = 0, mk = 0, ns = 0;
   DevVec<double> hcc, hcb;
   std::vector<double> hbb, acf, htarg, otarg;
+  // live[blk] = 0: channel block blk of hcc is known to hold zeros (blocks
+  // keep their numbering when nch changes, so the map outlives a wave)
+  std::vector<unsigned char> live;
   size_t nblk() const { return static_cast<size_t>(nch) * (nch + 1) / 2; }
   size_t hcc_n() const { return nblk() * ns * ns; }
   size_t hcb_n() const { return static_cast<size_t>(ns) * nch * ncp; }
   hipError_t alloc(int ns_, int nch_, int ncp_, int mk_) {
+    const double* before = hcc.p;
+    const int ns_before = ns;
     ns = ns_; nch = nch_; ncp = ncp_; mk = mk_;
     hipError_t e = hcc.resize(std::max<size_t>(hcc_n(), 1));
     if (e != hipSuccess) return e;
+    if (hcc.p != before || ns != ns_before) live.assign(nblk(), 1);
+    if (live.size() < nblk()) live.resize(nblk(), 1);
     e = hcb.resize(std::max<size_t>(hcb_n(), 1));
     hbb.assign(ncp > 0 ? static_cast<size_t>(ncp) * (ncp + 1) / 2 : 0, 0.0);
     acf.assign(static_cast<size_t>(nch) * nch * (mk + 1), 0.0);
@@ -154,8 +161,8 @@ struct Staged {
   size_t nband = 0, nr1 = 0, nvec = 0, bytes = 0;
   // byte offsets of the arrays in the arena
   size_t o_bc = 0, o_rc = 0, o_vc = 0, o_bptr = 0, o_bid = 0, o_rptr = 0, o_ru = 0, o_rw = 0, o_rs = 0, o_vptr = 0,
-         o_vid = 0, o_vs = 0;
-  bool has_gemm = false;
+         o_vid = 0, o_vs = 0, o_live = 0;
+  bool has_gemm = false, has_live = false;
   int gemm_klo = 0, gemm_khi = -1, engine = 2;
   template <typename T>
   T* hp(size_t o) { return reinterpret_cast<T*>(h.p + o); }
@@ -374,7 +381,8 @@ int wait_staged(Staged& st);
 // Sort the terms of all parts by output block (nblk blocks) and vector
 // output (nq vectors), list order kept within an output, into the pinned
 // arena of st (CSR arrays; see Staged).
-int stage_terms(const std::vector<Terms*>& parts, Staged& st, int nblk, int nq, Pool* pool);
+int stage_terms(const std::vector<Terms*>& parts, Staged& st, int nblk, int nq, Pool* pool,
+                const std::vector<unsigned char>* live = nullptr);
 }  // namespace bsrg
 
 namespace bsrg {

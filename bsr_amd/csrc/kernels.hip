@@ -492,10 +492,13 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
                                                   const int* __restrict__ ru, const int* __restrict__ rw,
                                                   const double* __restrict__ rf, const double* __restrict__ V,
                                                   double* __restrict__ hcc, const double* __restrict__ base,
-                                                  int qnblk, int qmask, const double* __restrict__ bsum) {
+                                                  int qnblk, int qmask, const double* __restrict__ bsum,
+                                                  const unsigned char* __restrict__ live) {
   const int blk = blockIdx.x;
   const int b0 = bptr[blk], b1 = bptr[blk + 1], r0 = rptr[blk], r1 = rptr[blk + 1];
-  if (qnblk == 0 && has_base && b0 == b1 && r0 == r1) return;
+  // a block without terms keeps its input (has_base), or, overwritten, is
+  // skipped when it already holds zeros (live[blk] == 0)
+  if (qnblk == 0 && b0 == b1 && r0 == r1 && (has_base || (live && !live[blk]))) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr int R = 64 * A;  // rows per tile
   const int x0 = blockIdx.z * R, yt = blockIdx.y * kAsmCols, y0 = yt + wv * kAsmCpw;
@@ -832,7 +835,7 @@ void launch_sb_upper(int ns, int ks, const double* sb, double* sbu, hipStream_t 
 void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, const int* bid, const double* bc,
                      const double* bands, const int* rptr, const int* ru, const int* rw, const int* rs,
                      const double* rc, const double* stab, const double* V, double* hcc, double* bsum,
-                     double* rf, hipStream_t s, const double* base, int qnblk, int qmask) {
+                     double* rf, hipStream_t s, const double* base, int qnblk, int qmask, const unsigned char* live) {
   if (nblk <= 0) return;
   hipLaunchKernelGGL(k_term_prep, dim3(nblk, (ns * ks + 255) / 256), dim3(256), 0, s, ns, ks, bptr, bid, bc, bands,
                      bsum, rptr, rs, rc, stab, rf);
@@ -842,7 +845,7 @@ void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, co
 #define ASM_CASE(N)                                                                                             \
   case N:                                                                                                       \
     hipLaunchKernelGGL(k_assemble<N>, grid, dim3(256), 0, s, ns, ks, has_base, bptr, bid, bc, bands, rptr, ru, rw, \
-                       rf, V, hcc, base, qnblk, qmask, bsum);                                                   \
+                       rf, V, hcc, base, qnblk, qmask, bsum, live);                                             \
     break;
     ASM_CASE(1) ASM_CASE(2) ASM_CASE(3) ASM_CASE(4)
 #undef ASM_CASE

@@ -91,6 +91,7 @@ SIGNATURES = {
     "bsrg_set_capture_safe": [C.c_void_p, C.c_int],
     "bsrg_set_gemm_engine": [C.c_void_p, C.c_int],
     "bsrg_wave_host_timings": [C.c_void_p, _dp],
+    "bsrg_wave_term_counts": [C.c_void_p, C.c_int, np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")],
     "bsrg_graph_capture": [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
     "bsrg_graph_launch": [C.c_void_p],
     "bsrg_graph_sync": [C.c_void_p, _odp, C.POINTER(C.c_double)],

@@ -257,6 +257,13 @@ int bsrg_set_gemm_engine(bsrg_ctx* ctx, int engine);
  * [9] the longest part. */
 int bsrg_wave_host_timings(bsrg_ctx* ctx, double* ms10);
 
+/* After bsrg_wave_prepare: sizes of the staged term sets of S (which=0) or
+ * H (which=1): n8 = {channel blocks, blocks with band terms, band terms,
+ * blocks with rank-1 terms, rank-1 terms, channel-perturber vectors with
+ * terms, vector terms, blocks that may hold nonzeros after the last
+ * launch}.  No counterpart in the reference. */
+int bsrg_wave_term_counts(bsrg_ctx* ctx, int which, long long* n8);
+
 /* 1: the autotuner of later prepares only considers engines that can be
  * captured in a hipGraph (the hand-written kernels). */
 int bsrg_set_capture_safe(bsrg_ctx* ctx, int on);

@@ -208,6 +208,31 @@ def test_fused_equals_call_by_call_and_is_deterministic(ctx):
         assert np.array_equal(b, b.T)
 
 
+def test_wave_sequence_on_one_context(ctx, oracle_lib):
+    """Waves of different channel counts in a row on one context (as the
+    serving loop runs them): S blocks that hold zeros are not rewritten when
+    a wave leaves them empty, so each wave must still match the oracle in
+    every block, including blocks a previous wave had filled."""
+    O = oracle_lib
+    from bsr_amd import synth
+    skipped = 0
+    for seed, nch in ((91, 7), (92, 4), (93, 7), (94, 6)):
+        w = synth.make_wave(seed, 8, 40, 6.0, 15.0, nch=nch, ncp=3, mk=3, nb=3, ntarg=3, fill=0.3, name="seq")
+        print(f"[seq] seed={seed} nch={nch}", flush=True)
+        ctx.wave_assemble(w)
+        cnt = ctx.wave_term_counts(0)
+        assert cnt["blocks"] == nch * (nch + 1) // 2
+        # after the launch exactly the S blocks with terms may hold nonzeros
+        assert max(cnt["band_blocks"], cnt["rank1_blocks"]) <= cnt["live_blocks"]
+        assert cnt["live_blocks"] <= min(cnt["blocks"], cnt["band_blocks"] + cnt["rank1_blocks"])
+        skipped += cnt["blocks"] - cnt["live_blocks"]
+        S, H = ctx.wave_get(0), ctx.wave_get(1)
+        oS, oH = O.run_wave(w)
+        _cmp_mats(S, oS, "S")
+        _cmp_mats(H, oH, "H")
+    assert skipped > 0  # the sequence exercises empty S blocks
+
+
 def test_errors_are_reported(ctx):
     from bsr_amd import synth, BsrGpuError
     w = synth.make_wave(41, 8, 30, 4.0, 10.0, nch=3, ncp=2, mk=1, nb=2, ntarg=2, fill=0.5, name="err")
