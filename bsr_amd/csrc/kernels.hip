@@ -286,15 +286,22 @@ __global__ void k_pair_dens(int ns, int ks, int nb, const double* P, double* den
 // chunk of 16 pairs) sums over j; the partials part[kk][p][jp](i,ip) are
 // then added in jp order by k_direct_reduce (deterministic).  The pair
 // densities of the current jp are staged in LDS.
-__global__ void k_direct_partial(int ns, int ks, int npair, const double* __restrict__ rkb,
-                                 const double* __restrict__ dens, double* __restrict__ part) {
-  extern __shared__ double dl[];  // [16][ns]
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int ip = blockIdx.y % ks, jp = blockIdx.y / ks;
+// Per (ip, jp) slab of the band and up to 16 pair densities:
+//   part[kk][p][jp](i, ip) = sum_j dens[p](j, jp) * rkb[kk](i, j, ip, jp).
+// 512 threads: wave w covers rows 64 (w & 3) + 256 blockIdx.z + lane and the
+// half w >> 2 of the j range; the densities of the slab's jp sit in LDS
+// (broadcast reads), the band streams through coalesced row reads, and the
+// two j halves are added in a fixed order through LDS.
+__global__ __launch_bounds__(512) void k_direct_partial(int ns, int ks, int npair, const double* __restrict__ rkb,
+                                                        const double* __restrict__ dens, double* __restrict__ part) {
+  extern __shared__ double dl[];  // [16][ns], then reused as [4][64][16]
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.z * 256 + (wave & 3) * 64 + lane, jh = wave >> 2;
+  const int ip = blockIdx.x % ks, jp = blockIdx.x / ks;
   const int nchunk = (npair + 15) / 16;
-  const int kk = blockIdx.z / nchunk, p0 = (blockIdx.z % nchunk) * 16;
+  const int kk = blockIdx.y / nchunk, p0 = (blockIdx.y % nchunk) * 16;
   const int np = min(16, npair - p0);
-  for (int q = threadIdx.x; q < 16 * ns; q += 64) {
+  for (int q = threadIdx.x; q < 16 * ns; q += 512) {
     const int p = q / ns, j = q % ns;
     dl[q] = p < np ? dens[static_cast<size_t>(p0 + p) * ns * ks + j + static_cast<size_t>(ns) * jp] : 0.0;
   }
@@ -302,12 +309,12 @@ __global__ void k_direct_partial(int ns, int ks, int npair, const double* __rest
   double acc[16];
 #pragma unroll
   for (int p = 0; p < 16; ++p) acc[p] = 0.0;
+  const int nj = ns - jp, half = (nj + 1) / 2, j0 = jh * half, j1 = min(nj, j0 + half);
   if (i < ns - ip) {
     const double* Rc = rkb + static_cast<size_t>(kk) * ns * ns * ks * ks + i +
                        static_cast<size_t>(ns) * ns * (ip + static_cast<size_t>(ks) * jp);
-    const int nj = ns - jp;
-    int j = 0;
-    for (; j + 4 <= nj; j += 4) {
+    int j = j0;
+    for (; j + 4 <= j1; j += 4) {
       const double r0 = Rc[static_cast<size_t>(ns) * j], r1 = Rc[static_cast<size_t>(ns) * (j + 1)];
       const double r2 = Rc[static_cast<size_t>(ns) * (j + 2)], r3 = Rc[static_cast<size_t>(ns) * (j + 3)];
 #pragma unroll
@@ -319,16 +326,23 @@ __global__ void k_direct_partial(int ns, int ks, int npair, const double* __rest
         acc[p] += d[3] * r3;
       }
     }
-    for (; j < nj; ++j) {
+    for (; j < j1; ++j) {
       const double r = Rc[static_cast<size_t>(ns) * j];
 #pragma unroll
       for (int p = 0; p < 16; ++p) acc[p] += dl[p * ns + j] * r;
     }
   }
-  if (i < ns) {
+  __syncthreads();  // the densities are no longer read: the space takes the second halves
+  double* red = dl + ((wave & 3) * 64 + lane) * 16;
+  if (jh == 1) {
+#pragma unroll
+    for (int p = 0; p < 16; ++p) red[p] = acc[p];
+  }
+  __syncthreads();
+  if (jh == 0 && i < ns) {
     for (int p = 0; p < np; ++p)
       part[((static_cast<size_t>(kk) * npair + p0 + p) * ks + jp) * ns * ks + i + static_cast<size_t>(ns) * ip] =
-          acc[p];
+          acc[p] + red[p];
   }
 }
 
@@ -351,9 +365,14 @@ __global__ void k_direct_reduce(int ns, int ks, int npair, int nk, const double*
 // q on the r2 pair (P2; the same orbitals as P1 for the BS bands).  One
 // workgroup per output column y; threads over rows x (coalesced band reads
 // and writes).
-template <int NB>
-__global__ void k_exchange_basis(int ns, int ks, const double* __restrict__ rkb, const double* __restrict__ P,
-                                 const double* __restrict__ P2, double* __restrict__ E) {
+template <int NB, int KS>
+__global__ __launch_bounds__(256) void k_exchange_basis(int ns, int ks_rt, const double* __restrict__ rkb,
+                                                        const double* __restrict__ P, const double* __restrict__ P2,
+                                                        double* __restrict__ E) {
+  // KS > 0: the order at compile time, so that the 2 KS - 1 band reads of
+  // one dz are issued together (out-of-range neighbours read a clamped
+  // element with weight 0)
+  const int ks = KS > 0 ? KS : ks_rt;
   // 1-D grid over (kk, y).  Neighbouring columns read the same band slabs;
   // workgroups are dealt to the 8 XCDs round robin, so renumber them to put
   // runs of consecutive columns on one XCD (its L2).
@@ -369,6 +388,7 @@ __global__ void k_exchange_basis(int ns, int ks, const double* __restrict__ rkb,
   }
   __syncthreads();
   const size_t s_ip = static_cast<size_t>(ns) * ns, s_jp = s_ip * ks;
+  constexpr int ND = KS > 0 ? 2 * KS - 1 : 2 * kMaxKs - 1;
   for (int x = threadIdx.x; x < ns; x += blockDim.x) {
     double e[NB * NB];
 #pragma unroll
@@ -377,16 +397,29 @@ __global__ void k_exchange_basis(int ns, int ks, const double* __restrict__ rkb,
       const int z = x + dz;
       if (z < 0 || z >= ns) continue;
       const int i0 = min(x, z), ip0 = dz < 0 ? -dz : dz;
+      const double* Ri = R + i0 + s_ip * ip0;
       double tb[NB];
 #pragma unroll
       for (int b = 0; b < NB; ++b) tb[b] = 0.0;
-      for (int dw = -(ks - 1); dw <= ks - 1; ++dw) {
-        const int w = y + dw;
-        if (w < 0 || w >= ns) continue;
-        const int j0 = min(y, w), jp0 = dw < 0 ? -dw : dw;
-        const double r = R[i0 + static_cast<size_t>(ns) * j0 + s_ip * ip0 + s_jp * jp0];
+      if (KS > 0) {
+        double r[ND];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) tb[b] += r * pw[b][dw + ks - 1];
+        for (int t = 0; t < ND; ++t) {
+          const int dw = t - (ks - 1), w = min(max(y + dw, 0), ns - 1);
+          r[t] = Ri[static_cast<size_t>(ns) * min(y, w) + s_jp * (dw < 0 ? -dw : dw)];
+        }
+#pragma unroll
+        for (int t = 0; t < ND; ++t)
+#pragma unroll
+          for (int b = 0; b < NB; ++b) tb[b] += r[t] * pw[b][t];
+      } else {
+        for (int dw = -(ks - 1); dw <= ks - 1; ++dw) {
+          const int w = y + dw;
+          if (w < 0 || w >= ns) continue;
+          const double rr = Ri[static_cast<size_t>(ns) * min(y, w) + s_jp * (dw < 0 ? -dw : dw)];
+#pragma unroll
+          for (int b = 0; b < NB; ++b) tb[b] += rr * pw[b][dw + ks - 1];
+        }
       }
 #pragma unroll
       for (int a = 0; a < NB; ++a) {
@@ -801,8 +834,9 @@ void launch_pair_dens(int ns, int ks, int nb, const double* P, double* dens, hip
 void launch_direct_basis(int ns, int ks, int nk, int npair, const double* rkb, const double* dens, double* part,
                          double* D, hipStream_t s) {
   const int nchunk = (npair + 15) / 16;
-  hipLaunchKernelGGL(k_direct_partial, dim3((ns + 63) / 64, ks * ks, nk * nchunk), dim3(64),
-                     sizeof(double) * 16 * ns, s, ns, ks, npair, rkb, dens, part);
+  const size_t lds = sizeof(double) * std::max(16 * ns, 4 * 64 * 16);
+  hipLaunchKernelGGL(k_direct_partial, dim3(ks * ks, nk * nchunk, (ns + 255) / 256), dim3(512), lds, s, ns, ks,
+                     npair, rkb, dens, part);
   const size_t n = static_cast<size_t>(ns) * ks * npair * nk;
   hipLaunchKernelGGL(k_direct_reduce, dim3(nblocks(n, 256)), dim3(256), 0, s, ns, ks, npair, nk, part, D);
 }
@@ -811,8 +845,15 @@ void launch_exchange_basis(int ns, int ks, int nk, int nb, const double* rkb, co
                            double* E, hipStream_t s) {
   const dim3 grid(ns * nk), block(ns <= 64 ? 64 : (ns <= 128 ? 128 : 256));
   switch (nb) {
-#define NB_CASE(N) \
-  case N: hipLaunchKernelGGL(k_exchange_basis<N>, grid, block, 0, s, ns, ks, rkb, P, P2, E); break;
+#define NB_CASE(N)                                                                                   \
+  case N:                                                                                            \
+    if (ks == 8)                                                                                     \
+      hipLaunchKernelGGL((k_exchange_basis<N, 8>), grid, block, 0, s, ns, ks, rkb, P, P2, E);        \
+    else if (ks == 9)                                                                                \
+      hipLaunchKernelGGL((k_exchange_basis<N, 9>), grid, block, 0, s, ns, ks, rkb, P, P2, E);        \
+    else                                                                                             \
+      hipLaunchKernelGGL((k_exchange_basis<N, 0>), grid, block, 0, s, ns, ks, rkb, P, P2, E);        \
+    break;
     NB_CASE(1) NB_CASE(2) NB_CASE(3) NB_CASE(4) NB_CASE(5) NB_CASE(6) NB_CASE(7) NB_CASE(8)
 #undef NB_CASE
     default: break;
