@@ -314,16 +314,16 @@ __global__ __launch_bounds__(512) void k_direct_partial(int ns, int ks, int npai
     const double* Rc = rkb + static_cast<size_t>(kk) * ns * ns * ks * ks + i +
                        static_cast<size_t>(ns) * ns * (ip + static_cast<size_t>(ks) * jp);
     int j = j0;
-    for (; j + 4 <= j1; j += 4) {
-      const double r0 = Rc[static_cast<size_t>(ns) * j], r1 = Rc[static_cast<size_t>(ns) * (j + 1)];
-      const double r2 = Rc[static_cast<size_t>(ns) * (j + 2)], r3 = Rc[static_cast<size_t>(ns) * (j + 3)];
+    // eight band reads in flight per lane
+    for (; j + 8 <= j1; j += 8) {
+      double r[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) r[u] = Rc[static_cast<size_t>(ns) * (j + u)];
 #pragma unroll
       for (int p = 0; p < 16; ++p) {
         const double* d = dl + p * ns + j;
-        acc[p] += d[0] * r0;
-        acc[p] += d[1] * r1;
-        acc[p] += d[2] * r2;
-        acc[p] += d[3] * r3;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[p] += d[u] * r[u];
       }
     }
     for (; j < j1; ++j) {
@@ -432,6 +432,78 @@ __global__ __launch_bounds__(256) void k_exchange_basis(int ns, int ks_rt, const
 #pragma unroll
     for (int q = 0; q < NB * NB; ++q)
       E[(static_cast<size_t>(kk) * NB * NB + q) * ns * ns + pos] = e[q];
+  }
+}
+
+// Exchange bases with the contraction over w done first, per band row:
+//   T_b(i, ip) = sum_dw rkb(i, min(y, y+dw), ip, |dw|) q_b(y + dw)
+//   E_ab(x, y) = sum_dz p_a(x + dz) T_b(min(x, x+dz), |dz|)
+// One workgroup per (64-row block of x, column y, multipole): phase 1 has
+// one thread per (row i, ip) of the block and its halo (2 KS - 1 band reads
+// in flight per thread), T lands in LDS; phase 2 has one thread per (x, a).
+// Same sums as k_exchange_basis, grouped differently (FP64 rounding aside).
+template <int NB, int KS>
+__global__ __launch_bounds__(((64 + KS - 1) * KS + 63) / 64 * 64) void k_exchange_basis_t(
+    int ns, const double* __restrict__ rkb, const double* __restrict__ P, const double* __restrict__ P2,
+    double* __restrict__ E) {
+  constexpr int H = KS - 1, NR = 64 + H, ND = 2 * KS - 1;
+  __shared__ double sT[KS][NB][NR];
+  __shared__ double pw[NB][ND];
+  __shared__ double sp[NB][64 + 2 * H];
+  const int nxb = (ns + 63) / 64;
+  const int nwg = static_cast<int>(gridDim.x), full = nwg - nwg % 8, hw = static_cast<int>(blockIdx.x);
+  const int id = hw < full ? (hw % 8) * (full / 8) + hw / 8 : hw;
+  const int xb = id % nxb, y = (id / nxb) % ns, kk = id / (nxb * ns);
+  const int x0 = xb * 64;
+  const size_t s_ip = static_cast<size_t>(ns) * ns, s_jp = s_ip * KS;
+  const double* R = rkb + static_cast<size_t>(kk) * s_jp * KS;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int q = tid; q < NB * ND; q += nt) {
+    const int b = q / ND, t = q % ND, w = y + t - H;
+    pw[b][t] = (w >= 0 && w < ns) ? P2[static_cast<size_t>(b) * ns + w] : 0.0;
+  }
+  for (int q = tid; q < NB * (64 + 2 * H); q += nt) {
+    const int a = q / (64 + 2 * H), z = x0 - H + q % (64 + 2 * H);
+    sp[a][q % (64 + 2 * H)] = (z >= 0 && z < ns) ? P[static_cast<size_t>(a) * ns + z] : 0.0;
+  }
+  __syncthreads();
+  // phase 1: thread (i, ip); rows outside [0, ns) only ever meet weight 0
+  if (tid < NR * KS) {
+    const int ri = tid % NR, ip = tid / NR, i = min(max(x0 - H + ri, 0), ns - 1);
+    const double* Ri = R + i + s_ip * ip;
+    double r[ND];
+#pragma unroll
+    for (int t = 0; t < ND; ++t) {
+      const int dw = t - H, w = min(max(y + dw, 0), ns - 1);
+      r[t] = Ri[static_cast<size_t>(ns) * min(y, w) + s_jp * (dw < 0 ? -dw : dw)];
+    }
+#pragma unroll 1
+    for (int b = 0; b < NB; ++b) {
+      double v = 0.0;
+#pragma unroll
+      for (int t = 0; t < ND; ++t) v += r[t] * pw[b][t];
+      sT[ip][b][ri] = v;
+    }
+  }
+  __syncthreads();
+  // phase 2: thread (x, a)
+  for (int q = tid; q < 64 * NB; q += nt) {
+    const int xl = q % 64, a = q / 64, x = x0 + xl;
+    if (x >= ns) continue;
+    double e[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) e[b] = 0.0;
+#pragma unroll 1
+    for (int t = 0; t < ND; ++t) {
+      const int dz = t - H;
+      const double pa = sp[a][xl + t];  // p_a(x + dz), 0 outside [0, ns)
+      const int ri = xl + H + (dz < 0 ? dz : 0), ip = dz < 0 ? -dz : dz;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) e[b] += pa * sT[ip][b][ri];
+    }
+    const size_t pos = x + static_cast<size_t>(ns) * y;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) E[((static_cast<size_t>(kk) * NB + a) * NB + b) * ns * ns + pos] = e[b];
   }
 }
 
@@ -844,15 +916,18 @@ void launch_direct_basis(int ns, int ks, int nk, int npair, const double* rkb, c
 void launch_exchange_basis(int ns, int ks, int nk, int nb, const double* rkb, const double* P, const double* P2,
                            double* E, hipStream_t s) {
   const dim3 grid(ns * nk), block(ns <= 64 ? 64 : (ns <= 128 ? 128 : 256));
+  const dim3 gridt(static_cast<unsigned>((ns + 63) / 64) * ns * nk);
   switch (nb) {
-#define NB_CASE(N)                                                                                   \
-  case N:                                                                                            \
-    if (ks == 8)                                                                                     \
-      hipLaunchKernelGGL((k_exchange_basis<N, 8>), grid, block, 0, s, ns, ks, rkb, P, P2, E);        \
-    else if (ks == 9)                                                                                \
-      hipLaunchKernelGGL((k_exchange_basis<N, 9>), grid, block, 0, s, ns, ks, rkb, P, P2, E);        \
-    else                                                                                             \
-      hipLaunchKernelGGL((k_exchange_basis<N, 0>), grid, block, 0, s, ns, ks, rkb, P, P2, E);        \
+#define NB_CASE(N)                                                                                      \
+  case N:                                                                                               \
+    if (ks == 8)                                                                                        \
+      hipLaunchKernelGGL((k_exchange_basis_t<N, 8>), gridt, dim3(((64 + 7) * 8 + 63) / 64 * 64), 0, s, ns, \
+                         rkb, P, P2, E);                                                                \
+    else if (ks == 9)                                                                                   \
+      hipLaunchKernelGGL((k_exchange_basis_t<N, 9>), gridt, dim3(((64 + 8) * 9 + 63) / 64 * 64), 0, s, ns, \
+                         rkb, P, P2, E);                                                                \
+    else                                                                                                \
+      hipLaunchKernelGGL((k_exchange_basis<N, 0>), grid, block, 0, s, ns, ks, rkb, P, P2, E);           \
     break;
     NB_CASE(1) NB_CASE(2) NB_CASE(3) NB_CASE(4) NB_CASE(5) NB_CASE(6) NB_CASE(7) NB_CASE(8)
 #undef NB_CASE
