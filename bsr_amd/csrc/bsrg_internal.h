@@ -137,8 +137,11 @@ int mfma_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B
 size_t ozaki8_a_bytes(int m, int K, int S);
 size_t ozaki8_b_bytes(int n, int K, int S);
 int ozaki8_prepare_a(int m, int K, const double* A, size_t lda, int S, void* abuf, hipStream_t s);
+// qns > 0: C points at the origin of a qns x qns submatrix of block 0 of a
+// sequence of blocks of order ldo (ldo^2 apart), m = qns^2, and row
+// x + qns y of column col lands at (x, y) of that submatrix in block col.
 int ozaki8_dgemm(int m, int n, int K, const void* abuf, const double* B, size_t ldb, double* C, size_t ldc,
-                 bool accumulate, int S, void* bbuf, hipStream_t s);
+                 bool accumulate, int S, void* bbuf, hipStream_t s, int qns = 0, int ldo = 0);
 size_t ozaki_workspace_bytes(int m, int n, int K, int S);
 int ozaki_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B, size_t ldb, double* C,
                 size_t ldc, bool accumulate, int S, void* work, hipStream_t s);

@@ -433,7 +433,7 @@ std::map<std::tuple<int, int, int, int>, int> g_dtuned;
 // engine 0 library FP64, 1-3 FP64 MFMA kernel, 10+S bf16 and 20+S int8 split
 // precision with S slices (the int8 slices of E_f prepared once per version
 // of the exchange matrices, over every multipole).
-int dirac_gemm(bsrg_ctx* c, int f, int engine, bool accumulate) {
+int dirac_gemm(bsrg_ctx* c, int f, int engine, bool accumulate, bool into_output = false) {
   auto& W = c->dw;
   const int ns = c->dbs.T.ns, nb = W.nb, nkE = W.exch_nk;
   const size_t nss = static_cast<size_t>(ns) * ns, nb2 = static_cast<size_t>(nb) * nb;
@@ -454,6 +454,13 @@ int dirac_gemm(bsrg_ctx* c, int f, int engine, bool accumulate) {
       W.oz8a_S[f] = S;
     }
     HIPCHK(W.oz8b.resize(ozaki8_b_bytes(n, K, S)));
+    if (into_output) {
+      // straight into quadrant (b1, b2) of the channel blocks, adding to them
+      const size_t ms = 2 * static_cast<size_t>(ns);
+      double* Q = W.hch.p + kB1[f] * static_cast<size_t>(ns) + kB2[f] * static_cast<size_t>(ns) * ms;
+      return ozaki8_dgemm(m, n, K, W.oz8a[f].p, W.coef.p + static_cast<size_t>(f) * nkE * nb2, nt, Q, nss, true, S,
+                          W.oz8b.p, c->stream, ns, static_cast<int>(ms));
+    }
     return ozaki8_dgemm(m, n, K, W.oz8a[f].p, W.coef.p + static_cast<size_t>(f) * nkE * nb2, nt, C, nss, accumulate, S,
                         W.oz8b.p, c->stream);
   }
@@ -612,7 +619,9 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   W.iptar.assign(iptar, iptar + nch);
   W.nblk = static_cast<size_t>(nch) * (nch + 1) / 2;
   const size_t nss = static_cast<size_t>(ns) * ns, nq1 = static_cast<size_t>(nch) * npert;
-  HIPCHK(W.hq.resize(4 * W.nblk * nss));
+  // the quadrant-major GEMM output; int8 split GEMMs write the output layout
+  // directly and need none
+  if (c->precision_mode != 2) HIPCHK(W.hq.resize(4 * W.nblk * nss));
   HIPCHK(W.hv.resize(std::max<size_t>(2 * nq1 * ns, 1)));
   HIPCHK(W.hch.resize(W.nblk * ms * ms));
   HIPCHK(W.hcp.resize(std::max<size_t>(nq1 * ms, 1)));
@@ -702,23 +711,41 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
       eng[f] = dirac_engine(c, f);
       if (eng[f] < 0) return eng[f];
     }
-  // device work: the exchange GEMMs write their quadrants of hq; one pass
-  // then adds the band and rank-1 terms and writes the output layout; the
-  // diagonal channel blocks are symmetrised in place
+  // device work.  FP64 (and bf16) GEMMs write their quadrants of hq; one
+  // pass then adds the band and rank-1 terms and writes the output layout.
+  // int8 split GEMMs place their results themselves: the term pass writes
+  // the output layout first and the GEMMs add into its quadrants, which
+  // saves reading and writing hq.  The diagonal channel blocks are
+  // symmetrised in place at the end.
+  bool fused = true;
+  for (int f = 0; f < 4; ++f) fused = fused && (eng[f] < 0 || eng[f] >= 20);
+  W.fused = fused;
+  auto terms = [&](const double* base, int qmask) {
+    launch_assemble(ns, ks, static_cast<int>(4 * W.nblk), 0, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
+                    st.dp<double>(st.o_bc), W.Dc.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
+                    st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), W.Sc.p, W.VT.p, W.hch.p, c->bsum.p, c->rfac.p, s, base,
+                    static_cast<int>(W.nblk), qmask);
+  };
   (void)hipEventRecord(c->ev[2], s);
-  int qmask = 0;
-  for (int f = 0; f < 4; ++f)
-    if (eng[f] >= 0) {
-      if (int rc = dirac_gemm(c, f, eng[f], false)) return rc;
-      qmask |= 1 << (2 * kB1[f] + kB2[f]);
-    }
+  if (fused) {
+    terms(nullptr, 0);
+    HIPCHK(hipGetLastError());
+    (void)hipEventRecord(c->ev[3], s);
+    for (int f = 0; f < 4; ++f)
+      if (eng[f] >= 0)
+        if (int rc = dirac_gemm(c, f, eng[f], true, true)) return rc;
+  } else {
+    int qmask = 0;
+    for (int f = 0; f < 4; ++f)
+      if (eng[f] >= 0) {
+        if (int rc = dirac_gemm(c, f, eng[f], false)) return rc;
+        qmask |= 1 << (2 * kB1[f] + kB2[f]);
+      }
+    (void)hipEventRecord(c->ev[3], s);
+    terms(W.hq.p, qmask);
+  }
   W.engine = -1;
   for (int f = 0; f < 4 && W.engine < 0; ++f) W.engine = eng[f];
-  (void)hipEventRecord(c->ev[3], s);
-  launch_assemble(ns, ks, static_cast<int>(4 * W.nblk), 0, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
-                  st.dp<double>(st.o_bc), W.Dc.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
-                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), W.Sc.p, W.VT.p, W.hch.p, c->bsum.p, c->rfac.p, s, W.hq.p,
-                  static_cast<int>(W.nblk), qmask);
   HIPCHK(hipGetLastError());
   (void)hipEventRecord(c->ev[4], s);
   {
@@ -745,7 +772,7 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   (void)hipEventElapsedTime(&e34, c->ev[3], c->ev[4]);
   (void)hipEventElapsedTime(&e45, c->ev[4], c->ev[5]);
   (void)hipEventElapsedTime(&e05, c->ev[0], c->ev[5]);
-  W.ms[0] = e01; W.ms[1] = e23; W.ms[2] = e34; W.ms[3] = e45; W.ms[4] = e05;
+  W.ms[0] = e01; W.ms[1] = W.fused ? e34 : e23; W.ms[2] = W.fused ? e23 : e34; W.ms[3] = e45; W.ms[4] = e05;
   auto msd = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
   };

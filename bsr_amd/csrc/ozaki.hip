@@ -328,11 +328,19 @@ __global__ void k_slice8_cols(int n, int K, int np, int Kp, int S, const double*
 // piece q ^ ((r >> 2) & 3), so that the staging writes (4 lanes a row) and
 // the fragment reads (16 rows, one piece) both spread over all banks.
 __device__ __forceinline__ int sw8(int r, int byte) { return r * 64 + ((((byte >> 4) ^ (r >> 2)) & 3) << 4); }
+// Place of C(row, col): column-major with leading dimension ldc, or, when
+// qns > 0, inside block col of order ldo (blocks ldo^2 apart) as the qns x qns
+// submatrix whose origin C points at: row = x + qns y -> x + ldo y.
+__device__ __forceinline__ size_t out_at(int row, int col, size_t ldc, int qns, int ldo) {
+  if (qns <= 0) return row + ldc * col;
+  const int y = row / qns, x = row - y * qns;
+  return static_cast<size_t>(col) * ldo * ldo + x + static_cast<size_t>(ldo) * y;
+}
 template <int S>
 __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, int mp, int np, int Kp, const int8_t* __restrict__ As,
                                                  const int8_t* __restrict__ Bs, const double* __restrict__ sig,
                                                  const double* __restrict__ tau, double* __restrict__ C, size_t ldc,
-                                                 int accumulate, int sc) {
+                                                 int accumulate, int sc, int qns, int ldo) {
   __shared__ __attribute__((aligned(16))) int8_t sB[S][64 * 64];  // C columns (blocks)
   __shared__ __attribute__((aligned(16))) int8_t sA[S][64 * 64];  // C rows (positions)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -371,7 +379,7 @@ __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, in
         for (int e = 0; e < 4; ++e) {
           const int row = min(r0 + wr + b * 16 + li, m - 1);
           const int col = min(c0 + wc + a * 16 + (lane >> 4) * 4 + e, n - 1);
-          old[a][b][e] = C[row + ldc * col];
+          old[a][b][e] = C[out_at(row, col, ldc, qns, ldo)];
         }
   }
   i32x4 acc[S][2][2];
@@ -456,7 +464,7 @@ __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, in
           for (int e = 0; e < 4; ++e) {
             const int row = min(r0 + wr + b * 16 + li, m - 1);
             const int col = min(c0 + wc + a * 16 + (lane >> 4) * 4 + e, n - 1);
-            old[a][b][e] = C[row + ldc * col];
+            old[a][b][e] = C[out_at(row, col, ldc, qns, ldo)];
           }
     }
 #pragma unroll
@@ -474,18 +482,18 @@ __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, in
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int col = c0 + wc + a * 16 + (lane >> 4) * 4 + e;
-        if (row < m && col < n) C[row + ldc * col] = v[a][b][e];
+        if (row < m && col < n) C[out_at(row, col, ldc, qns, ldo)] = v[a][b][e];
       }
     }
 }
 
 template <int S>
 void launch_k8(dim3 g, hipStream_t s, int m, int n, int mp, int np, int Kp, const int8_t* As, const int8_t* Bs,
-               const double* sig, const double* tau, double* C, size_t ldc, int acc) {
+               const double* sig, const double* tau, double* C, size_t ldc, int acc, int qns, int ldo) {
   // column supertile: the B slices of sc column tiles in about 1.5 MB
   const size_t tile = static_cast<size_t>(S) * 64 * Kp;
   const int sc = std::max(1, std::min(np / 64, static_cast<int>((3u << 19) / tile)));
-  hipLaunchKernelGGL(k_ozaki8<S>, g, dim3(256), 0, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, sc);
+  hipLaunchKernelGGL(k_ozaki8<S>, g, dim3(256), 0, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, sc, qns, ldo);
 }
 
 template <int S>
@@ -570,7 +578,8 @@ int ozaki8_prepare_a(int m, int K, const double* A, size_t lda, int S, void* abu
 }
 
 int ozaki8_dgemm(int m, int n, int K, const void* abuf, const double* B, size_t ldb, double* C, size_t ldc,
-                 bool accumulate, int S, void* bbuf, hipStream_t s) {
+                 bool accumulate, int S, void* bbuf, hipStream_t s, int qns, int ldo) {
+  if (qns > 0 && (static_cast<long long>(qns) * qns != m || ldo < qns)) return fail(-1, "split-precision GEMM: bad block layout");
   if (S < 2 || S > 8) return fail(-9, "split precision: slices must be 2..8");
   if (m <= 0 || n <= 0 || K <= 0) return 0;
   const int mp = (m + 63) / 64 * 64, np = (n + 63) / 64 * 64, Kp = (K + 63) / 64 * 64;
@@ -585,13 +594,13 @@ int ozaki8_dgemm(int m, int n, int K, const void* abuf, const double* B, size_t 
   const dim3 g((np / 64) * (mp / 64));
   const int acc = accumulate ? 1 : 0;
   switch (S) {
-    case 2: launch_k8<2>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    case 3: launch_k8<3>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    case 4: launch_k8<4>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    case 5: launch_k8<5>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    case 6: launch_k8<6>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    case 7: launch_k8<7>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    default: launch_k8<8>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
+    case 2: launch_k8<2>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    case 3: launch_k8<3>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    case 4: launch_k8<4>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    case 5: launch_k8<5>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    case 6: launch_k8<6>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    case 7: launch_k8<7>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    default: launch_k8<8>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
   }
   return hipGetLastError() == hipSuccess ? 0 : fail(-100, "split-precision GEMM launch failed");
 }

@@ -115,10 +115,12 @@ class DiracWave:
         return self._cat
 
     def footprint_bytes(self):
-        """Device memory of one assembly: bands of four families, exchange
-        bases, channel blocks (quadrant-major and in the output layout),
-        channel-perturber vectors, the exchange coefficients and the summed
-        bands of the quadrant blocks."""
+        """Device memory of one assembly in FP64: bands of four families,
+        exchange bases, channel blocks (quadrant-major and in the output
+        layout), channel-perturber vectors, the exchange coefficients and the
+        summed bands of the quadrant blocks.  The int8 split form needs no
+        quadrant-major copy (half of `blocks`) but its slices of the exchange
+        bases instead."""
         ns, ks, nk, nb = self.ns, self.ks, self.mk + 1, self.nb
         nblk = self.nch * (self.nch + 1) // 2
         bands = 4 * nk * ns * ns * ks * ks
