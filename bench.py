@@ -44,6 +44,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_MFMA = 78.6   # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0       # GB/s
+# HBM bytes of one launch of the library FP64 exchange GEMM at the headline
+# shape (read 2.132 GB + write 0.599 GB): two rocprofv3 counter runs
+# (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB device copy in the same process
+GEMM_TRAFFIC_BYTES = 2.731e9
 
 
 def log(msg):
@@ -291,6 +295,7 @@ def main():
     log("  device ms: " + json.dumps(dev))
     log("  host ms:   " + json.dumps(host))
     gemm_ms = dev["gemm"]
+    engine = ctx.wave_engine()
     nss = w.ns * w.ns
     nblk = w.nch * (w.nch + 1) // 2
     nt = (w.mk + 1) * w.nb * w.nb
@@ -312,8 +317,11 @@ def main():
         "config": {"workload": f"partial wave ks=8 ns=200 k<=10 nch={w.nch} ncp={w.ncp} nhm={w.nhm}",
                    "device_ms_per_step": dev["total"], "device_ms": dev, "host_ms": host, "terms": terms},
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                     "frac": ach / PEAK_FP64_MFMA, "traffic": None,
-                     "kernel": "exchange contraction GEMM (FP64)", "kernel_ms": gemm_ms,
+                     "frac": ach / PEAK_FP64_MFMA,
+                     "traffic": GEMM_TRAFFIC_BYTES if engine == 0 and (w.nch, w.ncp) == (60, 400) else None,
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of the library GEMM at this shape, "
+                                       "copy-calibrated: profiles/wave_c2_hbm_traffic.md",
+                     "kernel": "exchange contraction GEMM (FP64)", "engine": engine, "kernel_ms": gemm_ms,
                      "flops_per_launch": flops},
     }
     e_ref = None
