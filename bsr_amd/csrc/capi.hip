@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -334,6 +335,17 @@ int wait_staged(Staged& st) {
     st.pending = false;
   }
   return 0;
+}
+
+// Host worker threads per context: BSRG_HOST_THREADS (1..64) when set, else
+// the hardware threads up to 8 (more made the list decoding of a C2-sized
+// wave no faster on a 16-core share: its CPU time grows with the count).
+int host_workers() {
+  if (const char* e = std::getenv("BSRG_HOST_THREADS")) {
+    const int n = std::atoi(e);
+    if (n >= 1 && n <= 64) return n;
+  }
+  return static_cast<int>(std::min(8u, std::max(1u, std::thread::hardware_concurrency())));
 }
 
 // Stable counting sort of the terms of all parts (in part order) by key in
@@ -654,10 +666,7 @@ int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, con
       break;
     }
   const auto t2 = std::chrono::steady_clock::now();
-  if (!c->pool) {
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    c->pool.reset(new Pool(static_cast<int>(std::min(8u, hw))));
-  }
+  if (!c->pool) c->pool.reset(new Pool(host_workers()));
   // more ranges than workers, taken dynamically: records differ in cost
   const int nth = static_cast<int>(std::min<long>(4L * c->pool->size(), nrec / 4096L + 1));
   std::vector<int> cuts{0};

@@ -379,6 +379,8 @@ namespace bsrg {
 // Before a term set or its arena is rewritten: the last upload from them
 // has completed.
 int wait_staged(Staged& st);
+// Host worker threads of a context's pool.
+int host_workers();
 // Sort the terms of all parts by output block (nblk blocks) and vector
 // output (nq vectors), list order kept within an output, into the pinned
 // arena of st (CSR arrays; see Staged).

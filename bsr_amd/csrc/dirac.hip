@@ -636,8 +636,7 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   for (int r = 0; r < nrec; ++r)
     if (itype[r] == 3) (void)quadr_table(c, kpol[r]);  // tables the workers read
   if (!c->pool) {
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    c->pool.reset(new Pool(static_cast<int>(std::min(8u, hw))));
+    c->pool.reset(new Pool(host_workers()));
   }
   const int nth = static_cast<int>(std::min<long>(4L * c->pool->size(), nrec / 8192L + 1));
   std::vector<int> cuts{0};
