@@ -16,7 +16,8 @@ Beside the headline line's required fields, the JSON line carries:
            relative element error against the FP64 result and (int8, the
            served form) its largest lowest-10 eigenvalue error;
   serving  a Poisson stream (20 jobs/s) of waves of the same target
-           (nch ~ U[40,80], ncp ~ U[200,400]) through the serving loop, in
+           (nch ~ U[40,80], ncp ~ U[200,400]) through the serving loop (6
+           slots), in
            FP64 and in split precision: p50/p99 latency at that rate, the
            saturated capacity (all jobs queued at once), and the error of
            every split-precision job against its FP64 result;
@@ -113,7 +114,7 @@ def run_steps(ctx, w, warmup, steps):
     return ms, mean_dict(dev), mean_dict(host)
 
 
-def serving(stream, precision, nslots=4):
+def serving(stream, precision, nslots=6):
     """One pass of the stream at its arrival times, then the same jobs all
     queued at once (capacity)."""
     from bsr_amd.serve import Server, summarize

@@ -455,9 +455,10 @@ int upload_wvec(bsrg_ctx* c) {
 
 // Upload a staged term set (and the GET_V vectors) for a later enqueue:
 // one copy of the arena, one of the exchange coefficients.
-int upload_terms(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
+// The exchange coefficients of a term set (and the split-precision
+// workspaces its GEMM needs): all a GEMM launch reads besides the bases.
+int upload_gemm(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
   hipStream_t s = c->stream;
-  if (int rc = upload_wvec(c)) return rc;
   st.has_gemm = T.gemm_n > 0 && T.gemm_khi >= 0;
   if (st.has_gemm) {
     HIPCHK(st.gemm.resize(T.gemm_n));
@@ -481,6 +482,14 @@ int upload_terms(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
       HIPCHK(c->oz8b.resize(ozaki8_b_bytes(static_cast<int>(M.nblk()), K, c->nslices)));
     }
   }
+  return 0;
+}
+
+// The staged arena of a term set (and the GET_V vectors) for a later
+// enqueue, after upload_gemm.
+int upload_arena(bsrg_ctx* c, const Matrices& M, Staged& st) {
+  hipStream_t s = c->stream;
+  if (int rc = upload_wvec(c)) return rc;
   if (st.bytes) {
     HIPCHK(st.d.resize(st.bytes));
     HIPCHK(hipMemcpyAsync(st.d.p, st.h.p, st.bytes, hipMemcpyHostToDevice, s));
@@ -490,6 +499,13 @@ int upload_terms(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
   HIPCHK(hipEventRecord(st.done, s));
   st.pending = true;
   return 0;
+}
+
+// Upload a staged term set for a later enqueue: one copy of the exchange
+// coefficients, one of the arena.
+int upload_terms(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
+  if (int rc = upload_gemm(c, M, T, st)) return rc;
+  return upload_arena(c, M, st);
 }
 
 // The exchange-contraction GEMM hcc = E^T C (+hcc): split precision in
@@ -561,29 +577,41 @@ int autotune_gemm(bsrg_ctx* c, Matrices& M, Staged& st, bool capture_safe) {
 }
 
 // Device-only work of an uploaded term set (no host synchronisation, no
-// allocation): the part of a wave that a hipGraph captures.  The term pass
-// goes first -- on an overwrite it writes every block (no read) -- and the
-// exchange GEMM then accumulates into the blocks: one read-modify-write of
-// the channel blocks, inside the GEMM, instead of a second pass.
-int enqueue_terms(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, hipEvent_t e0, hipEvent_t e1) {
+// allocation): the part of a wave that a hipGraph captures.  The exchange
+// GEMM goes first (writing every block on an overwrite), then the term pass
+// adds the band and rank-1 terms to what the blocks hold; without a GEMM the
+// term pass writes the blocks itself.  gemm_done: the GEMM of this term set
+// was already enqueued (enqueue_gemm).
+int enqueue_gemm(bsrg_ctx* c, Matrices& M, Staged& st, bool overwrite, hipEvent_t e0, hipEvent_t e1) {
+  if (e0) (void)hipEventRecord(e0, c->stream);
+  if (int rc = run_gemm(c, M, st, overwrite, st.engine)) return rc;
+  c->last_engine = c->precision_mode == 0 ? st.engine : 10 * c->precision_mode + c->nslices;
+  if (e1) (void)hipEventRecord(e1, c->stream);
+  return 0;
+}
+
+int enqueue_terms(bsrg_ctx* c, Matrices& M, Staged& st, bool overwrite, hipEvent_t e0, hipEvent_t e1,
+                  bool gemm_done = false) {
   const int ns = c->G.ns, ks = c->G.ks;
   hipStream_t s = c->stream;
-  // without an exchange GEMM (which reads every block), empty blocks that
-  // already hold zeros are left alone
-  const bool skip = overwrite && st.has_live && !st.has_gemm && !c->capturing;
-  launch_assemble(ns, ks, st.nblk, overwrite ? 0 : 1, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
+  if (st.has_gemm && !gemm_done) {
+    if (int rc = enqueue_gemm(c, M, st, overwrite, e0, e1)) return rc;
+  } else if (!st.has_gemm) {
+    if (e0) (void)hipEventRecord(e0, s);  // an empty GEMM phase
+    if (e1) (void)hipEventRecord(e1, s);
+  }
+  // after a GEMM, or when accumulating, the pass adds to the blocks and
+  // leaves blocks without terms alone; an overwriting pass without a GEMM
+  // writes every block, except empty ones that already hold zeros
+  const bool add = st.has_gemm || !overwrite;
+  const bool skip = !add && st.has_live && !c->capturing;
+  launch_assemble(ns, ks, st.nblk, add ? 1 : 0, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
                   st.dp<double>(st.o_bc), c->bands.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
                   st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p, c->bsum.p, c->rfac.p, s,
                   nullptr, 0, 0, skip ? st.dp<unsigned char>(st.o_live) : nullptr);
   launch_vec_assemble(ns, st.nq, overwrite ? 0 : 1, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid),
                       st.dp<int>(st.o_vs), st.dp<double>(st.o_vc), c->St.p, c->VT.p, M.hcb.p, s);
   HIPCHK(hipGetLastError());
-  if (e0) (void)hipEventRecord(e0, s);
-  if (st.has_gemm) {
-    if (int rc = run_gemm(c, M, st, false, st.engine)) return rc;
-    c->last_engine = c->precision_mode == 0 ? st.engine : 10 * c->precision_mode + c->nslices;
-  }
-  if (e1) (void)hipEventRecord(e1, s);
   return 0;
 }
 
@@ -1147,6 +1175,18 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   for (int t = 0; t < c->nwork; ++t) c->partsH.push_back(c->work[t].get());
   c->partsH.push_back(&THt);
   const auto h1 = clk::now();
+  // one-shot waves start the exchange GEMM of H now: it needs only the
+  // coefficients, and runs while the terms are sorted and uploaded (a
+  // captured launch enqueues it itself)
+  c->gemm_enqueued = false;
+  if (c->early_gemm) {
+    if (int rc = upload_gemm(c, c->H, TH, c->stH)) return rc;
+    if (int rc = autotune_gemm(c, c->H, c->stH, c->capture_safe)) return rc;
+    if (c->stH.has_gemm) {
+      if (int rc = enqueue_gemm(c, c->H, c->stH, true, c->ev[3], c->ev[4])) return rc;
+      c->gemm_enqueued = true;
+    }
+  }
   if (int rc = stage_terms(c->partsS, c->stS, static_cast<int>(c->S.nblk()), c->S.nch * c->S.ncp, nullptr,
                           &c->S.live))
     return rc;
@@ -1155,8 +1195,12 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
     return rc;
   const auto h2 = clk::now();
   if (int rc = upload_terms(c, c->S, TS, c->stS)) return rc;
-  if (int rc = upload_terms(c, c->H, TH, c->stH)) return rc;
-  if (int rc = autotune_gemm(c, c->H, c->stH, c->capture_safe)) return rc;
+  if (c->early_gemm) {
+    if (int rc = upload_arena(c, c->H, c->stH)) return rc;
+  } else {
+    if (int rc = upload_terms(c, c->H, TH, c->stH)) return rc;
+    if (int rc = autotune_gemm(c, c->H, c->stH, c->capture_safe)) return rc;
+  }
   const auto h3 = clk::now();
   auto ms = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
   c->host_ms[0] = ms(h0, ha);
@@ -1176,8 +1220,11 @@ int bsrg_wave_launch(bsrg_ctx* c) {
   if (int rc = enqueue_terms(c, c->S, c->stS, true, nullptr, nullptr)) return rc;
   update_live(c, c->S, c->stS);
   launch_symmetrize_diag(c->G.ns, c->S.nch, c->S.hcc.p, c->stream);
-  if (int rc = enqueue_terms(c, c->H, c->stH, true, timed ? c->ev[3] : nullptr, timed ? c->ev[4] : nullptr))
+  if (int rc = enqueue_terms(c, c->H, c->stH, true, timed ? c->ev[3] : nullptr, timed ? c->ev[4] : nullptr,
+                             c->gemm_enqueued))
     return rc;
+  c->gemm_was_early = c->gemm_enqueued;
+  c->gemm_enqueued = false;
   update_live(c, c->H, c->stH);
   launch_symmetrize_diag(c->G.ns, c->H.nch, c->H.hcc.p, c->stream);
   HIPCHK(hipGetLastError());
@@ -1202,8 +1249,12 @@ int bsrg_wave_finish(bsrg_ctx* c) {
     (void)hipEventElapsedTime(&ms, c->ev[1], c->ev[8]); c->timings[1] = ms;
   }
   (void)hipEventElapsedTime(&ms, c->ev[3], c->ev[4]); c->timings[2] = ms;
-  (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
-  (void)hipEventElapsedTime(&ms2, c->ev[4], c->ev[5]); c->timings[3] = ms + ms2;
+  if (c->gemm_was_early) {  // the GEMM ran before the launch: the launch is all term passes
+    (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[5]); c->timings[3] = ms;
+  } else {
+    (void)hipEventElapsedTime(&ms, c->ev[2], c->ev[3]);
+    (void)hipEventElapsedTime(&ms2, c->ev[4], c->ev[5]); c->timings[3] = ms + ms2;
+  }
   (void)hipEventElapsedTime(&ms, c->ev[c->wave_ok ? 2 : 0], c->ev[5]); c->timings[4] = ms;
   c->wave_ok = true;
   c->wave_ntarg = c->H.nch > 0 ? c->wave_ntarg : 0;
@@ -1214,10 +1265,13 @@ int bsrg_wave_assemble(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const in
                        const int* k1, const int* k2, const int* k3, const int* k4, const double* cd, int novl,
                        const int* ok3, const int* ok4, const double* ocd, double eps_c, double ec, int ntarg,
                        const double* etarg, const int* iptar) {
-  if (int rc = bsrg_wave_prepare(c, nch, ncp, mk, nrec, jtype, kpol, k1, k2, k3, k4, cd, novl, ok3, ok4, ocd, eps_c,
-                                 ec, ntarg, etarg, iptar))
-    return rc;
-  if (int rc = bsrg_wave_launch(c)) return rc;
+  if (int rc = check_ctx(c)) return rc;
+  c->early_gemm = true;
+  const int rc = bsrg_wave_prepare(c, nch, ncp, mk, nrec, jtype, kpol, k1, k2, k3, k4, cd, novl, ok3, ok4, ocd, eps_c,
+                                   ec, ntarg, etarg, iptar);
+  c->early_gemm = false;
+  if (rc) return rc;
+  if (int rc2 = bsrg_wave_launch(c)) return rc2;
   return bsrg_wave_finish(c);
 }
 

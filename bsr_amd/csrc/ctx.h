@@ -341,6 +341,9 @@ struct bsrg_ctx {
 
   Matrices cur, S, H;
   bool wave_ok = false, wave_prepared = false, capture_safe = false, capturing = false;
+  // one-shot waves (bsrg_wave_assemble) enqueue the H exchange GEMM during
+  // the prepare, before the terms are staged
+  bool early_gemm = false, gemm_enqueued = false, gemm_was_early = false;
   int wave_ntarg = 0;
 
   // staged term uploads: S and H of the fused wave, and the per-call path
