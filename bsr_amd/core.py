@@ -293,6 +293,13 @@ class Context:
         L.check(L.lib().bsrg_wave_eigen(self._p, nev, e))
         return e
 
+    def matfile_eigen(self, path, nev: int = 10) -> np.ndarray:
+        """The nev lowest eigenvalues of H x = E S x of a bsr_mat.nnn file
+        (read as BSR_HD3 reads it, solved on the device)."""
+        e = np.zeros(nev)
+        L.check(L.lib().bsrg_matfile_eigen(self._p, str(path).encode(), nev, e))
+        return e
+
     def drop_caches(self):
         """Forget resident bands and bases (the next wave rebuilds them)."""
         L.check(L.lib().bsrg_drop_caches(self._p))

@@ -13,6 +13,10 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/bsr_gpu.h"
@@ -57,21 +61,15 @@ int densify(const Matrices& M, double* A, size_t lda, DevVec<double>& hbb, hipSt
   return 0;
 }
 
-}  // namespace
-
-int bsrg_wave_eigen(bsrg_ctx* c, int nev, double* e) {
-  if (!c) return fail(-2, "null context");
-  if (!c->wave_ok) return fail(-8, "no assembled wave (bsrg_wave_assemble)");
-  if (!e) return fail(-2, "null argument");
-  const Matrices& H = c->H;
-  const Matrices& S = c->S;
+// The nev lowest eigenvalues of H x = E S x for matrices in the block form.
+int solve_lowest(bsrg_ctx* c, const Matrices& H, const Matrices& S, int nev, double* e) {
   const size_t n = static_cast<size_t>(H.nch) * H.ns + H.ncp;
-  if (nev < 1 || static_cast<size_t>(nev) > n) return fail(-1, "bsrg_wave_eigen: nev must be in 1..nhm");
-  if (n > 46000) return fail(-1, "bsrg_wave_eigen: nhm above 46000 (32-bit sizes of the solver)");
+  if (nev < 1 || static_cast<size_t>(nev) > n) return fail(-1, "eigenvalues: nev must be in 1..nhm");
+  if (n > 46000) return fail(-1, "eigenvalues: nhm above 46000 (32-bit sizes of the solver)");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
   // dense work memory lives for this call only: 16 nhm^2 bytes
-  DevVec<double> A, B, W, Z, hbb;
+  DevVec<double> A, B, W, Z, hbb, hbbS;
   DevVec<int> info;
   HIPCHK(A.resize(n * n));
   HIPCHK(B.resize(n * n));
@@ -79,7 +77,6 @@ int bsrg_wave_eigen(bsrg_ctx* c, int nev, double* e) {
   HIPCHK(Z.resize(n * static_cast<size_t>(nev)));
   HIPCHK(info.resize(2));
   if (int rc = densify(H, A.p, n, hbb, c->stream)) return rc;
-  DevVec<double> hbbS;
   if (int rc = densify(S, B.p, n, hbbS, c->stream)) return rc;
   const rocblas_int ni = static_cast<rocblas_int>(n);
   rocblas_status st = rocsolver_dsygvdx(c->blas, rocblas_eform_ax, rocblas_evect_none, rocblas_erange_index,
@@ -96,4 +93,91 @@ int bsrg_wave_eigen(bsrg_ctx* c, int nev, double* e) {
   if (hinfo[1] != 0) return fail(-41, "eigenvalues did not converge");
   if (hinfo[0] != nev) return fail(-41, "eigen-solve returned " + std::to_string(hinfo[0]) + " eigenvalues");
   return 0;
+}
+
+// Fortran unformatted sequential records (4-byte length markers).
+struct RecordReader {
+  std::FILE* f = nullptr;
+  std::vector<char> buf;
+  ~RecordReader() { if (f) std::fclose(f); }
+  bool next() {
+    std::uint32_t len = 0, len2 = 0;
+    if (std::fread(&len, 4, 1, f) != 1) return false;
+    buf.resize(len);
+    if (len && std::fread(buf.data(), 1, len, f) != len) return false;
+    if (std::fread(&len2, 4, 1, f) != 1) return false;
+    return len == len2;
+  }
+};
+
+// One matrix (S or H) of the file as BSR_HD3 reads it (diag_hd.f90:42-63):
+// kch diagonal blocks, then (ic, jc) headers each followed by an idim x jdim
+// record (ns for a channel index, 1 for a perturber), up to ic <= 0.
+int read_matrix(RecordReader& r, int ns, int nch, int ncp, Matrices& M, std::vector<double>& hcc,
+                std::vector<double>& hcb) {
+  const size_t nss = static_cast<size_t>(ns) * ns;
+  M.ns = ns; M.nch = nch; M.ncp = ncp;
+  hcc.assign(M.nblk() * nss, 0.0);
+  hcb.assign(static_cast<size_t>(ns) * nch * ncp, 0.0);
+  M.hbb.assign(static_cast<size_t>(ncp) * (ncp + 1) / 2, 0.0);
+  for (int i = 1; i <= nch; ++i) {
+    if (!r.next() || r.buf.size() != nss * 8) return fail(-50, "bsr_mat file: bad diagonal block record");
+    std::memcpy(&hcc[(static_cast<size_t>(i) * (i + 1) / 2 - 1) * nss], r.buf.data(), nss * 8);
+  }
+  for (;;) {
+    if (!r.next() || r.buf.size() < 8) return fail(-50, "bsr_mat file: bad block header");
+    int ic, jc;
+    std::memcpy(&ic, r.buf.data(), 4);
+    std::memcpy(&jc, r.buf.data() + 4, 4);
+    if (ic <= 0) return 0;
+    if (ic > nch + ncp || jc < 1 || jc > nch + ncp) return fail(-50, "bsr_mat file: block index out of range");
+    const size_t idim = ic <= nch ? ns : 1, jdim = jc <= nch ? ns : 1;
+    if (!r.next() || r.buf.size() != idim * jdim * 8) return fail(-50, "bsr_mat file: bad block record");
+    const double* v = reinterpret_cast<const double*>(r.buf.data());
+    if (ic <= nch && jc <= nch) {  // channel block (ic, jc); the lower one is kept
+      const int i = std::max(ic, jc), j = std::min(ic, jc);
+      double* b = &hcc[(static_cast<size_t>(i) * (i - 1) / 2 + j - 1) * nss];
+      for (int y = 0; y < ns; ++y)
+        for (int x = 0; x < ns; ++x) b[x + static_cast<size_t>(ns) * y] = ic >= jc ? v[x + ns * y] : v[y + ns * x];
+    } else if (ic <= nch || jc <= nch) {  // channel-perturber vector
+      const int ich = std::min(ic, jc), ip = std::max(ic, jc) - nch;
+      std::memcpy(&hcb[static_cast<size_t>(ns) * (ich - 1 + static_cast<size_t>(nch) * (ip - 1))], v, ns * 8);
+    } else {
+      const int ip = std::max(ic, jc) - nch, jp = std::min(ic, jc) - nch;
+      M.hbb[static_cast<size_t>(ip) * (ip - 1) / 2 + jp - 1] = v[0];
+    }
+  }
+}
+
+}  // namespace
+
+int bsrg_wave_eigen(bsrg_ctx* c, int nev, double* e) {
+  if (!c) return fail(-2, "null context");
+  if (!c->wave_ok) return fail(-8, "no assembled wave (bsrg_wave_assemble)");
+  if (!e) return fail(-2, "null argument");
+  return solve_lowest(c, c->H, c->S, nev, e);
+}
+
+int bsrg_matfile_eigen(bsrg_ctx* c, const char* path, int nev, double* e) {
+  if (!c) return fail(-2, "null context");
+  if (!path || !e) return fail(-2, "null argument");
+  RecordReader r;
+  r.f = std::fopen(path, "rb");
+  if (!r.f) return fail(-51, std::string("cannot open ") + path);
+  if (!r.next() || r.buf.size() != 12) return fail(-50, "bsr_mat file: bad header record");
+  int hdr[3];
+  std::memcpy(hdr, r.buf.data(), 12);
+  const int ns = hdr[0], nch = hdr[1], ncp = hdr[2];
+  if (ns < 1 || nch < 1 || ncp < 0) return fail(-50, "bsr_mat file: bad sizes");
+  HIPCHK(hipSetDevice(c->device));
+  Matrices S, H;
+  std::vector<double> hcc, hcb;
+  for (Matrices* M : {&S, &H}) {
+    if (int rc = read_matrix(r, ns, nch, ncp, *M, hcc, hcb)) return rc;
+    HIPCHK(M->hcc.upload(hcc.data(), hcc.size(), c->stream));
+    HIPCHK(M->hcb.resize(std::max<size_t>(hcb.size(), 1)));
+    if (!hcb.empty()) HIPCHK(M->hcb.upload(hcb.data(), hcb.size(), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return solve_lowest(c, H, S, nev, e);
 }

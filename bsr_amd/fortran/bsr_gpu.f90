@@ -16,6 +16,8 @@
 !    gpu_dirac_r_data(...)         Gen_matrix icase 2 = R_data(itype,kpol,0..3)
 !                                  over every list   src/dbsr/DBSR_MAT3_MPI/R_data.f90:2
 !    gpu_dirac_get(...)            the accumulators hch, hcp, hp, acf, htarg
+!  and the closing eigenvalue step of BSR_HD3 (diag_hd.f90:271-312, full
+!  basis): bsrg_wave_eigen (assembled wave), bsrg_matfile_eigen (bsr_mat.nnn).
 !  A non-zero status stops the program with the library's message, as the
 !  reference's Stop does.
 !======================================================================
@@ -133,6 +135,13 @@ module bsr_gpu
     integer(c_int) function bsrg_wave_eigen(ctx, nev, e) bind(C, name="bsrg_wave_eigen")
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
+      integer(c_int), value :: nev
+      real(c_double) :: e(*)
+    end function
+    integer(c_int) function bsrg_matfile_eigen(ctx, path, nev, e) bind(C, name="bsrg_matfile_eigen")
+      import :: c_ptr, c_int, c_double, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: path(*)
       integer(c_int), value :: nev
       real(c_double) :: e(*)
     end function

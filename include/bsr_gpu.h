@@ -293,6 +293,12 @@ int bsrg_wave_diff(bsrg_ctx* a, bsrg_ctx* ref, int which, double floor_frac, dou
  * definite; -41: no convergence. */
 int bsrg_wave_eigen(bsrg_ctx* ctx, int nev, double* e);
 
+/* The same eigenvalues for the S and H blocks of a bsr_mat.nnn file, read
+ * as BSR_HD3 reads it (src/bsr/BSR_HD3/diag_hd.f90:37-63: kch diagonal
+ * blocks, then (ic, jc) headers with ns x ns, ns or scalar records, up to
+ * ic <= 0, for S then H).  -50: malformed file; -51: cannot open. */
+int bsrg_matfile_eigen(bsrg_ctx* ctx, const char* path, int nev, double* e);
+
 /* Forget the resident R^k bands and orbital bases (of the BS and of the
  * Dirac partial waves) so the next call rebuilds them (bsrg_set_grid,
  * bsrg_set_orbitals, bsrg_dbs_set_grid and bsrg_dirac_set_orbitals keep them

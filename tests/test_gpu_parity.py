@@ -172,6 +172,11 @@ def test_fused_wave_c1(ctx, oracle_lib, tmp_path):
     e_d = ctx.wave_eigen(10)
     print("[c1] device eig err", np.abs(e_d - e_o).max(), flush=True)
     assert np.abs(e_d - e_o).max() <= 1e-9
+    # and from the file, read as BSR_HD3 reads it
+    e_f = ctx.matfile_eigen(p1, 10)
+    print("[c1] file eig err", np.abs(e_f - e_o).max(), flush=True)
+    assert np.abs(e_f - e_o).max() <= 1e-9
+    assert np.abs(e_f - e_d).max() <= 1e-12
 
 
 def test_fused_wave_with_perturbers(ctx, oracle_lib):
@@ -253,6 +258,8 @@ def test_errors_are_reported(ctx):
     with Context(0) as fresh:
         with pytest.raises(BsrGpuError, match="no assembled wave"):
             fresh.wave_eigen(10)
+        with pytest.raises(BsrGpuError, match="cannot open"):
+            fresh.matfile_eigen("/nonexistent/bsr_mat.001", 10)
 
 
 def test_fortran_host_driver(oracle_lib, tmp_path):
