@@ -46,9 +46,9 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_MFMA = 78.6   # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0       # GB/s
 # HBM bytes of one launch of the library FP64 exchange GEMM at the headline
-# shape (read 2.132 GB + write 0.599 GB): two rocprofv3 counter runs
+# shape (read 1.51 GB + write 0.60 GB): two rocprofv3 counter runs
 # (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB device copy in the same process
-GEMM_TRAFFIC_BYTES = 2.731e9
+GEMM_TRAFFIC_BYTES = 2.105e9
 
 
 def log(msg):

@@ -291,7 +291,7 @@ struct DiracWave {
   int oz8a_S[4] = {0, 0, 0, 0};
   int engine = -1;
   bool ok = false;
-  bool fused = false;  // the last wave's GEMMs added into the output layout (int8 split)
+  bool fused = false;  // the last wave's GEMMs wrote the output layout directly (int8 split)
   double ms[5] = {0, 0, 0, 0, 0};
   double host_ms[3] = {0, 0, 0};  // list decoding, staging and uploads, whole call
 };

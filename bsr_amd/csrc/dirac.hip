@@ -455,11 +455,11 @@ int dirac_gemm(bsrg_ctx* c, int f, int engine, bool accumulate, bool into_output
     }
     HIPCHK(W.oz8b.resize(ozaki8_b_bytes(n, K, S)));
     if (into_output) {
-      // straight into quadrant (b1, b2) of the channel blocks, adding to them
+      // straight into quadrant (b1, b2) of the channel blocks
       const size_t ms = 2 * static_cast<size_t>(ns);
       double* Q = W.hch.p + kB1[f] * static_cast<size_t>(ns) + kB2[f] * static_cast<size_t>(ns) * ms;
-      return ozaki8_dgemm(m, n, K, W.oz8a[f].p, W.coef.p + static_cast<size_t>(f) * nkE * nb2, nt, Q, nss, true, S,
-                          W.oz8b.p, c->stream, ns, static_cast<int>(ms));
+      return ozaki8_dgemm(m, n, K, W.oz8a[f].p, W.coef.p + static_cast<size_t>(f) * nkE * nb2, nt, Q, nss,
+                          accumulate, S, W.oz8b.p, c->stream, ns, static_cast<int>(ms));
     }
     return ozaki8_dgemm(m, n, K, W.oz8a[f].p, W.coef.p + static_cast<size_t>(f) * nkE * nb2, nt, C, nss, accumulate, S,
                         W.oz8b.p, c->stream);
@@ -712,10 +712,9 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
     }
   // device work.  FP64 (and bf16) GEMMs write their quadrants of hq; one
   // pass then adds the band and rank-1 terms and writes the output layout.
-  // int8 split GEMMs place their results themselves: the term pass writes
-  // the output layout first and the GEMMs add into its quadrants, which
-  // saves reading and writing hq.  The diagonal channel blocks are
-  // symmetrised in place at the end.
+  // int8 split GEMMs place their results in the output layout themselves
+  // and the pass adds the terms in place, so no quadrant-major copy exists.
+  // The diagonal channel blocks are symmetrised in place at the end.
   bool fused = true;
   for (int f = 0; f < 4; ++f) fused = fused && (eng[f] < 0 || eng[f] >= 20);
   W.fused = fused;
@@ -727,12 +726,14 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   };
   (void)hipEventRecord(c->ev[2], s);
   if (fused) {
-    terms(nullptr, 0);
-    HIPCHK(hipGetLastError());
-    (void)hipEventRecord(c->ev[3], s);
+    int qmask = 0;
     for (int f = 0; f < 4; ++f)
-      if (eng[f] >= 0)
-        if (int rc = dirac_gemm(c, f, eng[f], true, true)) return rc;
+      if (eng[f] >= 0) {
+        if (int rc = dirac_gemm(c, f, eng[f], false, true)) return rc;
+        qmask |= 1 << (2 * kB1[f] + kB2[f]);
+      }
+    (void)hipEventRecord(c->ev[3], s);
+    terms(W.hch.p, qmask);  // in place on the quadrants the GEMMs wrote
   } else {
     int qmask = 0;
     for (int f = 0; f < 4; ++f)
@@ -771,7 +772,7 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   (void)hipEventElapsedTime(&e34, c->ev[3], c->ev[4]);
   (void)hipEventElapsedTime(&e45, c->ev[4], c->ev[5]);
   (void)hipEventElapsedTime(&e05, c->ev[0], c->ev[5]);
-  W.ms[0] = e01; W.ms[1] = W.fused ? e34 : e23; W.ms[2] = W.fused ? e23 : e34; W.ms[3] = e45; W.ms[4] = e05;
+  W.ms[0] = e01; W.ms[1] = e23; W.ms[2] = e34; W.ms[3] = e45; W.ms[4] = e05;
   auto msd = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
   };

@@ -609,12 +609,19 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
   const int x0 = blockIdx.z * R, yt = blockIdx.y * kAsmCols, y0 = yt + wv * kAsmCpw;
   const double* hin = nullptr;
   double* h;
-  size_t ldo = ns;
+  size_t ldo = ns, ldb = ns;
   if (qnblk > 0) {
     const int q = blk / qnblk, cb = blk % qnblk;
-    if ((qmask >> q) & 1) hin = base + static_cast<size_t>(blk) * ns * ns;
     ldo = 2 * static_cast<size_t>(ns);
     h = hcc + static_cast<size_t>(cb) * ldo * ldo + (q >> 1) * ns + static_cast<size_t>(q & 1) * ns * ldo;
+    if ((qmask >> q) & 1) {
+      if (base == hcc) {  // in place: the quadrant already holds its input
+        hin = h;
+        ldb = ldo;
+      } else {
+        hin = base + static_cast<size_t>(blk) * ns * ns;
+      }
+    }
   } else {
     h = hcc + static_cast<size_t>(blk) * ns * ns;
     if (has_base) hin = h;
@@ -644,7 +651,7 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
 #pragma unroll
     for (int a = 0; a < A; ++a) {
       const int x = min(x0 + lane + 64 * a, ns - 1);
-      acc[a][b] = hin ? hin[x + static_cast<size_t>(ns) * y] : 0.0;
+      acc[a][b] = hin ? hin[x + ldb * y] : 0.0;
     }
   }
   // rank-1 terms: metadata and factors of 64 terms at a time into LDS (one
