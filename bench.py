@@ -30,7 +30,9 @@ Beside the headline line's required fields, the JSON line carries:
            hch/acf/htarg complete on the device, FP64; elements/s of H, the
            device phases, the exchange GEMM against the FP64 MFMA peak, and
            the oracle on an 8-channel sample of the same shape (time and the
-           largest element error of the GPU result on that sample).
+           largest element error of the GPU result on that sample); and a
+           stream of 6 C5-shaped waves served on 2 slots, split precision
+           (7 int8 slices) and FP64, against the plain FP64 wave.
 """
 from __future__ import annotations
 
@@ -183,6 +185,26 @@ def rel_err(a, b, floor_frac=1e-3):
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor_frac * scale)))
 
 
+def dirac_serving(plain_elements_per_s, njobs=6, nslots=2):
+    from bsr_amd import synth_dirac as SD
+    from bsr_amd.serve import Server, summarize
+    waves = [SD.c5(seed=100 + j) for j in range(njobs)]
+    out = {"jobs": njobs, "slots": nslots}
+    for name, prec, nsl in (("split", 2, 7), ("fp64", 0, 0)):
+        srv = Server(nslots=nslots, precision=prec, nslices=nsl)
+        try:
+            warm = [srv.submit(w) for w in waves[:nslots]]
+            for j in warm:
+                j.event.wait()
+            sm = summarize(srv.run_stream([(0.0, w) for w in waves]))
+        finally:
+            srv.close()
+        out[name] = {"capacity_elements_per_s": sm["elements_per_s"], "ms_per_job": sm["span_s"] * 1e3 / njobs,
+                     "failed": sm["failed"]}
+    out["split_capacity_vs_fp64_plain"] = out["split"]["capacity_elements_per_s"] / plain_elements_per_s
+    return out
+
+
 def dirac_c5(steps, warmup, with_cpu):
     from bsr_amd import Context, synth_dirac as SD
     w = SD.c5()
@@ -237,6 +259,10 @@ def dirac_c5(steps, warmup, with_cpu):
                             "max_rel_err_vs_fp64": sctx.dirac_diff(ctx), "vs_fp64_plain": ms / sms}
         finally:
             sctx.close()
+        # served: a stream of C5-shaped waves (own seeds and orbitals, the
+        # same grid) queued at once on two slots, in split precision and in
+        # FP64; capacity against the plain FP64 wave above
+        out["serving"] = dirac_serving(out["elements_per_s"])
         if with_cpu:
             from oracle import oracle as O
             ws = SD.c5(nch=8)

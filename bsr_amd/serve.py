@@ -1,4 +1,5 @@
-"""Continuously batched serving of partial-wave assembly jobs.
+"""Continuously batched serving of partial-wave assembly jobs (BSR waves,
+and Dirac waves of DBSR_MAT3 through the same slots).
 
 A Server owns `nslots` device contexts.  Each slot has its own HIP stream
 and keeps the knot grid, B-spline tables, R^k bands and orbital bases of
@@ -20,6 +21,11 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .core import Context
+
+
+def is_dirac(wave) -> bool:
+    """Dirac (DBSR) partial waves carry two-component orbitals pq."""
+    return hasattr(wave, "pq") and hasattr(wave, "ipbs")
 
 
 @dataclass
@@ -105,10 +111,17 @@ class Server:
                 job = self.inbox[s].popleft()
             job.start = self.now()
             try:
-                ctx.wave_assemble(job.wave)
-                job.device_ms = ctx.wave_timings()["total"]
-                if job.keep:
-                    job.result = (ctx.wave_get(0), ctx.wave_get(1))
+                if is_dirac(job.wave):
+                    # a Dirac (DBSR) wave: grid, orbitals and lists of the four families
+                    ctx.dirac_wave(job.wave)
+                    job.device_ms = float(ctx.dirac_timings()[4])
+                    if job.keep:
+                        job.result = ctx.dirac_get()
+                else:
+                    ctx.wave_assemble(job.wave)
+                    job.device_ms = ctx.wave_timings()["total"]
+                    if job.keep:
+                        job.result = (ctx.wave_get(0), ctx.wave_get(1))
             except Exception as e:  # a failed job does not poison the slot
                 job.error = str(e)
             job.done = self.now()

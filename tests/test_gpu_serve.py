@@ -46,6 +46,26 @@ def test_alone_equals_in_mix(server):
         assert np.array_equal(a.hbb, b.hbb) and np.array_equal(a.acf, b.acf)
 
 
+def test_dirac_alone_equals_in_mix(server):
+    """A Dirac wave served alone and in a mix with BS waves on the same
+    slots gives identical bits."""
+    from bsr_amd import synth_dirac as SD
+    d = SD.small(13, npert=2)
+    alone = server.submit(d, keep=True)
+    alone.event.wait()
+    assert alone.error is None, alone.error
+    ws = _waves(3, seed0=700)
+    jobs = [server.submit(w) for w in ws[:2]]
+    mixed = server.submit(d, keep=True)
+    jobs.append(server.submit(ws[2]))
+    for j in jobs + [mixed]:
+        j.event.wait()
+        assert j.error is None, j.error
+    for key in ("hch", "hcp", "hp", "acf", "htarg"):
+        assert np.array_equal(alone.result[key], mixed.result[key]), key
+    assert np.abs(alone.result["hch"]).max() > 0
+
+
 def test_admission_by_footprint():
     if not HAS_GPU:
         pytest.skip("no HIP device")
