@@ -615,6 +615,20 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, Staged& st, bool overwrite, hipEvent
   return 0;
 }
 
+// One-shot waves (bsrg_wave_assemble) start the exchange GEMM of H as soon
+// as the exchange records are decoded: it needs only the coefficients, and
+// runs while the other records are decoded and the terms sorted and
+// uploaded.  A captured launch enqueues the GEMM itself.
+int start_early_gemm(bsrg_ctx* c) {
+  if (int rc = upload_gemm(c, c->H, c->TH, c->stH)) return rc;
+  if (int rc = autotune_gemm(c, c->H, c->stH, c->capture_safe)) return rc;
+  if (c->stH.has_gemm) {
+    if (int rc = enqueue_gemm(c, c->H, c->stH, true, c->ev[3], c->ev[4])) return rc;
+    c->gemm_enqueued = true;
+  }
+  return 0;
+}
+
 // After an overwriting term pass of M (and its GEMM) is enqueued: which
 // blocks may now hold nonzeros.  The pass wrote zeros into the empty
 // blocks it visited; a GEMM may add to any block; a captured pass may never
@@ -682,6 +696,9 @@ void register_vectors(bsrg_ctx* c, int nrec, const int* jtype, const int* K3, co
 // order.  The workers only read the GET_V registry; vectors missing from it
 // (the first wave of new orbitals) are registered in list order and the
 // pass is repeated.
+// Exchange records (jtype 8, 9) only add exchange coefficients.
+inline bool exch_type(int jtype) { return jtype >= 8; }
+
 int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, const int* k1, const int* k2,
                    const int* k3, const int* k4, const double* cd, bool any_exch) {
   const auto t1 = std::chrono::steady_clock::now();
@@ -695,7 +712,8 @@ int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, con
     }
   const auto t2 = std::chrono::steady_clock::now();
   if (!c->pool) c->pool.reset(new Pool(host_workers()));
-  // more ranges than workers, taken dynamically: records differ in cost
+  // more ranges than workers, taken dynamically: records differ in cost;
+  // a range holds exchange records only or none
   const int nth = static_cast<int>(std::min<long>(4L * c->pool->size(), nrec / 4096L + 1));
   std::vector<int> cuts{0};
   for (int t = 1; t < nth; ++t) {
@@ -705,31 +723,36 @@ int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, con
     if (b < nrec) cuts.push_back(b);
   }
   cuts.push_back(nrec);
+  {
+    std::vector<int> split{0};
+    for (size_t q = 0; q + 1 < cuts.size(); ++q) {
+      for (int r = cuts[q] + 1; r < cuts[q + 1]; ++r)
+        if (exch_type(jtype[r]) != exch_type(jtype[r - 1])) split.push_back(r);
+      split.push_back(cuts[q + 1]);
+    }
+    cuts.swap(split);
+  }
   const int np = static_cast<int>(cuts.size()) - 1;
   while (static_cast<int>(c->work.size()) < np) c->work.emplace_back(new Terms());
   c->nwork = np;
-  // the exchange coefficient matrix, zeroed by the workers in slices
+  std::vector<int> pexch, pother;  // part indices by kind, in list order
+  for (int t = 0; t < np; ++t) (nrec && exch_type(jtype[cuts[t]]) ? pexch : pother).push_back(t);
   const size_t nt = static_cast<size_t>(c->basis_nk) * c->nb * c->nb;
   if (any_exch && c->nb > 0) {
     TH.gemm_n = nt * M.nblk();
     HIPCHK(TH.gemm.reserve(TH.gemm_n));
   }
-  bool zeroed = false;
   std::vector<double> part_ms(np, 0.0);
-  const std::function<void(int)> work = [&](int t) {
+  auto decode = [&](int t) {
     const auto p0 = std::chrono::steady_clock::now();
     struct Stamp {
       std::chrono::steady_clock::time_point t0;
       double* out;
-      ~Stamp() { *out = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
+      ~Stamp() { *out += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(); }
     } stamp{p0, &part_ms[t]};
     Terms& P = *c->work[t];
     P.clear();
     P.deferred = true;
-    if (TH.gemm_n && !zeroed) {
-      const size_t lo = TH.gemm_n * t / np, hi = TH.gemm_n * (t + 1) / np;
-      std::memset(TH.gemm.p + lo, 0, (hi - lo) * sizeof(double));
-    }
     int r = cuts[t];
     while (r < cuts[t + 1]) {
       int e = r + 1;
@@ -742,30 +765,48 @@ int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, con
       r = e;
     }
   };
-  c->pool->run(np, work);
-  // GET_V vectors not yet in the registry (new orbitals): register them in
-  // list order and decode again
+  // phase 1: the exchange parts, while the coefficient matrix is zeroed in
+  // slices; their coefficients are summed in list order, and a one-shot
+  // wave starts its exchange GEMM now
+  const int nx = static_cast<int>(pexch.size()), nz = std::max(nx, 1);
+  const std::function<void(int)> phase1 = [&](int i) {
+    if (TH.gemm_n) {
+      const size_t lo = TH.gemm_n * i / nz, hi = TH.gemm_n * (i + 1) / nz;
+      std::memset(TH.gemm.p + lo, 0, (hi - lo) * sizeof(double));
+    }
+    if (i < nx) decode(pexch[i]);
+  };
+  c->pool->run(nz, phase1);
+  for (int t : pexch) {
+    const Terms& P = *c->work[t];
+    if (P.err) return fail(P.err, P.msg);
+    for (const auto& g : P.gx) TH.gemm.p[g.first] += g.second;
+    TH.gemm_klo = std::min(TH.gemm_klo, P.gemm_klo);
+    TH.gemm_khi = std::max(TH.gemm_khi, P.gemm_khi);
+  }
+  if (TH.gemm_khi < 0) TH.gemm_n = 0;
+  if (c->early_gemm)
+    if (int rc = start_early_gemm(c)) return rc;
+  // phase 2: everything else; GET_V vectors not yet in the registry (new
+  // orbitals) are registered in list order and the phase is repeated
+  const std::function<void(int)> phase2 = [&](int i) { decode(pother[i]); };
+  c->pool->run(static_cast<int>(pother.size()), phase2);
   bool missing = false;
-  for (int t = 0; t < np; ++t) missing = missing || c->work[t]->err == kNotRegistered;
+  for (int t : pother) missing = missing || c->work[t]->err == kNotRegistered;
   if (missing) {
     register_vectors(c, nrec, jtype, k3, k4);
-    zeroed = true;
-    c->pool->run(np, work);
+    c->pool->run(static_cast<int>(pother.size()), phase2);
   }
   const auto t3 = std::chrono::steady_clock::now();
-  for (int t = 0; t < np; ++t) {
+  for (int t : pother) {
     const Terms& P = *c->work[t];
     if (P.err) return fail(P.err, P.msg);
   }
   for (int t = 0; t < np; ++t) {
     const Terms& P = *c->work[t];
-    for (const auto& g : P.gx) TH.gemm.p[g.first] += g.second;
     if (!P.ax.empty() && TH.acf_add.empty()) TH.acf_add.assign(M.acf.size(), 0.0);
     for (const auto& a : P.ax) TH.acf_add[a.first] += a.second;
-    TH.gemm_klo = std::min(TH.gemm_klo, P.gemm_klo);
-    TH.gemm_khi = std::max(TH.gemm_khi, P.gemm_khi);
   }
-  if (TH.gemm_khi < 0) TH.gemm_n = 0;
   const auto t4 = std::chrono::steady_clock::now();
   auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
@@ -1168,6 +1209,7 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   for (int ich = 1; ich <= nch; ++ich) band_hl(TH, ich, ich, ec);
   if (int rc = add_odata(c, c->H, TH, novl, nullptr, ok3, ok4, ocd, eps_c, ec, false)) return rc;
   const auto hb = clk::now();
+  c->gemm_enqueued = false;
   if (int rc = idata_parallel(c, nrec, jtype, kpol, k1, k2, k3, k4, cd, any_exch)) return rc;
   for (int ich = 1; ich <= nch; ++ich) band_hl(THt, ich, ich, etarg[iptar[ich - 1] - 1] - ec);
   c->partsS.assign(1, &TS);
@@ -1175,18 +1217,6 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   for (int t = 0; t < c->nwork; ++t) c->partsH.push_back(c->work[t].get());
   c->partsH.push_back(&THt);
   const auto h1 = clk::now();
-  // one-shot waves start the exchange GEMM of H now: it needs only the
-  // coefficients, and runs while the terms are sorted and uploaded (a
-  // captured launch enqueues it itself)
-  c->gemm_enqueued = false;
-  if (c->early_gemm) {
-    if (int rc = upload_gemm(c, c->H, TH, c->stH)) return rc;
-    if (int rc = autotune_gemm(c, c->H, c->stH, c->capture_safe)) return rc;
-    if (c->stH.has_gemm) {
-      if (int rc = enqueue_gemm(c, c->H, c->stH, true, c->ev[3], c->ev[4])) return rc;
-      c->gemm_enqueued = true;
-    }
-  }
   if (int rc = stage_terms(c->partsS, c->stS, static_cast<int>(c->S.nblk()), c->S.nch * c->S.ncp, nullptr,
                           &c->S.live))
     return rc;
