@@ -901,6 +901,7 @@ int bsrg_create(bsrg_ctx** out, int device) {
   std::unique_ptr<bsrg_ctx> c(new bsrg_ctx());
   c->device = device;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
   if (rocblas_create_handle(&c->blas) != rocblas_status_success) return fail(-101, "rocblas_create_handle failed");
   rocblas_set_stream(c->blas, c->stream);
   for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
@@ -913,12 +914,14 @@ int bsrg_destroy(bsrg_ctx* c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
   if (c->blas) rocblas_destroy_handle(c->blas);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   for (Staged* st : {&c->stS, &c->stH, &c->stC, &c->dw.st}) if (st->done) (void)hipEventDestroy(st->done);
-  hipStream_t s = c->stream;
+  hipStream_t s = c->stream, cs = c->copy_stream;
   delete c;
   if (s) (void)hipStreamDestroy(s);
+  if (cs) (void)hipStreamDestroy(cs);
   return 0;
 }
 

@@ -303,6 +303,9 @@ using namespace bsrg;
 struct bsrg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // uploads that may run beside the compute stream (the Dirac term arena
+  // while the exchange GEMMs compute); default priority like the other
+  hipStream_t copy_stream = nullptr;
   rocblas_handle blas = nullptr;
   hipEvent_t ev[10] = {};  // 0-5 phases of a wave, 6-7 autotuning, 8 end of the bases
   double timings[5] = {0, 0, 0, 0, 0};

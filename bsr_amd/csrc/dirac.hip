@@ -681,14 +681,9 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
     parts.push_back(&P.T);
   }
   const auto h1 = std::chrono::steady_clock::now();
-  if (int rc = stage_terms(parts, W.st, static_cast<int>(4 * W.nblk), static_cast<int>(2 * nq1), c->pool.get()))
-    return rc;
+  // the exchange coefficients first: the GEMMs need only them, and compute
+  // while the other terms are sorted on the host and uploaded beside them
   if (int rc = stage_exchange(c)) return rc;
-  Staged& st = W.st;
-  if (st.bytes) {
-    HIPCHK(st.d.resize(st.bytes));
-    HIPCHK(hipMemcpyAsync(st.d.p, st.h.p, st.bytes, hipMemcpyHostToDevice, s));
-  }
   const size_t nt = 4 * static_cast<size_t>(W.exch_nk) * W.nb * W.nb;
   if (has_x) {
     HIPCHK(W.xd.resize(W.xbytes));
@@ -699,11 +694,10 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
                        reinterpret_cast<const double*>(W.xd.p + W.o_xc), W.coef.p);
     HIPCHK(hipGetLastError());
   }
+  size_t nr1 = 0;
+  for (const Terms* P : parts) nr1 += P->r1.size();
   HIPCHK(c->bsum.resize(4 * W.nblk * ns * ks));
-  HIPCHK(c->rfac.resize(std::max<size_t>(st.nr1, 1)));
-  HIPCHK(hipEventRecord(st.done, s));
-  st.pending = true;
-  const auto h2 = std::chrono::steady_clock::now();
+  HIPCHK(c->rfac.resize(std::max<size_t>(nr1, 1)));
   int eng[4] = {-1, -1, -1, -1};
   for (int f = 0; f < 4; ++f)
     if (W.khi[f] >= 0) {
@@ -718,32 +712,31 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   bool fused = true;
   for (int f = 0; f < 4; ++f) fused = fused && (eng[f] < 0 || eng[f] >= 20);
   W.fused = fused;
-  auto terms = [&](const double* base, int qmask) {
-    launch_assemble(ns, ks, static_cast<int>(4 * W.nblk), 0, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
-                    st.dp<double>(st.o_bc), W.Dc.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
-                    st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), W.Sc.p, W.VT.p, W.hch.p, c->bsum.p, c->rfac.p, s, base,
-                    static_cast<int>(W.nblk), qmask);
-  };
   (void)hipEventRecord(c->ev[2], s);
-  if (fused) {
-    int qmask = 0;
-    for (int f = 0; f < 4; ++f)
-      if (eng[f] >= 0) {
-        if (int rc = dirac_gemm(c, f, eng[f], false, true)) return rc;
-        qmask |= 1 << (2 * kB1[f] + kB2[f]);
-      }
-    (void)hipEventRecord(c->ev[3], s);
-    terms(W.hch.p, qmask);  // in place on the quadrants the GEMMs wrote
-  } else {
-    int qmask = 0;
-    for (int f = 0; f < 4; ++f)
-      if (eng[f] >= 0) {
-        if (int rc = dirac_gemm(c, f, eng[f], false)) return rc;
-        qmask |= 1 << (2 * kB1[f] + kB2[f]);
-      }
-    (void)hipEventRecord(c->ev[3], s);
-    terms(W.hq.p, qmask);
+  int qmask = 0;
+  for (int f = 0; f < 4; ++f)
+    if (eng[f] >= 0) {
+      if (int rc = dirac_gemm(c, f, eng[f], false, fused)) return rc;
+      qmask |= 1 << (2 * kB1[f] + kB2[f]);
+    }
+  (void)hipEventRecord(c->ev[3], s);
+  // band, rank-1 and vector terms: sorted and uploaded while the GEMMs run
+  if (int rc = stage_terms(parts, W.st, static_cast<int>(4 * W.nblk), static_cast<int>(2 * nq1), c->pool.get()))
+    return rc;
+  Staged& st = W.st;
+  if (st.bytes) {
+    HIPCHK(st.d.resize(st.bytes));
+    HIPCHK(hipMemcpyAsync(st.d.p, st.h.p, st.bytes, hipMemcpyHostToDevice, c->copy_stream));
   }
+  HIPCHK(hipEventRecord(st.done, c->copy_stream));
+  st.pending = true;
+  HIPCHK(hipStreamWaitEvent(s, st.done, 0));
+  const auto h2 = std::chrono::steady_clock::now();
+  // in place on the quadrants the int8 GEMMs wrote, else from hq
+  launch_assemble(ns, ks, static_cast<int>(4 * W.nblk), 0, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
+                  st.dp<double>(st.o_bc), W.Dc.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
+                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), W.Sc.p, W.VT.p, W.hch.p, c->bsum.p, c->rfac.p, s,
+                  fused ? W.hch.p : W.hq.p, static_cast<int>(W.nblk), qmask);
   W.engine = -1;
   for (int f = 0; f < 4 && W.engine < 0; ++f) W.engine = eng[f];
   HIPCHK(hipGetLastError());

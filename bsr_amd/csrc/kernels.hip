@@ -558,8 +558,9 @@ __global__ void k_sb_upper(int ns, int ks, const double* sb, double* sbu) {
 // grid (nblk, ceil(ns/kAsmCols), ceil(ns/256)), block 256: lane -> rows
 // x0+lane+64a (a<4), wave -> kAsmCpw columns yt+kAsmCpw*wave+b.
 // The band entries (i, ith) the workgroup's columns touch come from the
-// block's summed band (k_term_prep) through LDS; rank-1 terms go 16 at a
-// time through LDS (u over the rows, f*w over the columns).
+// block's summed band (k_term_prep) through LDS; rank-1 terms go kAsmR1 at
+// a time through LDS (u over the rows, f*w over the columns), a small batch
+// that keeps the LDS per workgroup low enough for 4-5 waves per SIMD.
 // Per-block preparation of the term pass: the summed band of every block
 // with band terms, bsum[blk](i, ith) = sum of c_t * bands[bid_t](i, ith) over
 // the block's terms in list order, and the factor of every rank-1 term,
@@ -588,6 +589,7 @@ __global__ void k_term_prep(int ns, int ks, const int* __restrict__ bptr, const 
 // quadrant q has an input.
 constexpr int kAsmCpw = 8;                // columns per wavefront
 constexpr int kAsmCols = 4 * kAsmCpw;     // columns per workgroup
+constexpr int kAsmR1 = 8;                 // rank-1 terms staged in LDS at a time
 // A = row groups of 64 per workgroup (rows per tile 64*A, A = ceil(ns/64)
 // up to 4), so that short blocks do not idle lanes.
 template <int A>
@@ -626,8 +628,8 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
     h = hcc + static_cast<size_t>(blk) * ns * ns;
     if (has_base) hin = h;
   }
-  __shared__ double su[16][R];
-  __shared__ double sw[16][kAsmCols];
+  __shared__ double su[kAsmR1][R];
+  __shared__ double sw[kAsmR1][kAsmCols];
   __shared__ double sband[(kAsmCols + kMaxKs - 1) * kMaxKs];
   __shared__ int tu[64], tw[64];
   __shared__ double tf[64];
@@ -666,8 +668,8 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
       tu[threadIdx.x] = ru[t];
       tw[threadIdx.x] = rw[t];
     }
-    for (int tb = 0; tb < nc; tb += 16) {
-      const int nt = min(16, nc - tb);
+    for (int tb = 0; tb < nc; tb += kAsmR1) {
+      const int nt = min(kAsmR1, nc - tb);
       __syncthreads();
       for (int q = threadIdx.x; q < nt * R; q += 256) {
         const int r = q / R, x = q % R;
