@@ -168,6 +168,7 @@ class Context:
     def set_orbitals(self, pbs, iech, lbs, ibort, eps_ovl=1e-8):
         pbs = f64(pbs)
         L.check(L.lib().bsrg_set_orbitals(self._p, pbs.shape[1], pbs, i32(iech), i32(lbs), i32(ibort), eps_ovl))
+        self._nbf = pbs.shape[1]
 
     def allocate_matrix(self, nch, ncp, mk):
         L.check(L.lib().bsrg_alloc_matrix(self._p, nch, ncp, mk))
@@ -186,6 +187,29 @@ class Context:
         n = len(lst.c)
         ipt = np.arange(1, n + 1) if ipt is None else ipt
         L.check(L.lib().bsrg_odata(self._p, n, i32(ipt), i32(lst.k3), i32(lst.k4), f64(lst.c), eps_c))
+
+    def gen_lval(self, z, kclosd):
+        """Gen_Lval with INT_core (l_core.f90:92-257): the one-electron
+        operator with the closed core (orbitals 1..kclosd) for every l."""
+        L.check(L.lib().bsrg_gen_lval(self._p, float(z), int(kclosd)))
+
+    def lval(self):
+        """(mlh, hl_full (ns,ns,mlh+1), L_int (nbf,nbf), L_vec (ns,nbf))."""
+        mlh = C.c_int()
+        L.check(L.lib().bsrg_lval_get(self._p, mlh, None, None, None))
+        ns, nbf = self.ns, self._nbf
+        hl = np.zeros((ns, ns, mlh.value + 1), order="F")
+        li = np.zeros((nbf, nbf), order="F")
+        lv = np.zeros((ns, nbf), order="F")
+        L.check(L.lib().bsrg_lval_get(self._p, mlh, hl, li, lv))
+        return mlh.value, hl, li, lv
+
+    def l_data(self, jtype, l, lst, iitar=0, eps_c=0.0, ipt=None):
+        """L_data(jtype, l) (L_data.f90:2-149) on the current matrix."""
+        n = len(lst.c)
+        ipt = np.arange(1, n + 1) if ipt is None else ipt
+        L.check(L.lib().bsrg_ldata(self._p, jtype, l, n, i32(ipt), i32(lst.k1), i32(lst.k2), i32(lst.k3),
+                                   i32(lst.k4), f64(lst.c), int(iitar), float(eps_c)))
 
     def symmetrize_diag(self):
         L.check(L.lib().bsrg_symmetrize_diag(self._p))

@@ -33,6 +33,8 @@ struct GridTables {
   std::vector<double> t, gx, gw, gr, grm, grw, bsp, sb;
   void build(int ks, int ns, int ml, int me, double h, const double* t);
   std::vector<double> mrm(int mm) const;
+  // <B_i|B''_j> in lower band storage, slot (0,0) holding <B_{ns-1}|B''_ns>
+  std::vector<double> db2() const;
 };
 
 // Dirac B-spline tables: Gauss points of the intervals (gauleg rule), the
@@ -98,6 +100,19 @@ void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, co
 void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,
                          const double* c, const double* stab, const double* V, double* hcb, hipStream_t s);
 void launch_symmetrize_diag(int ns, int nch, double* hcc, hipStream_t s);
+// One-electron operator of one l (Gen_Lval/INT_core): out (ns x ns) =
+// sum_t ec[t] E(eid[t]) + the band hl (lower storage) + sum_u dc[u]
+// bands(did[u]) (upper storage), the band region taken from the lower
+// element so that it is exactly symmetric.
+void launch_lcore(int ns, int ks, int nt, const int* eid, const double* ec, const double* E, int nd,
+                  const int* did, const double* dc, const double* bands, const double* hl, double* out,
+                  hipStream_t s);
+// out(:, o) = hl^T P(:, pcol[o]) for o < n.
+void launch_lvec(int ns, int n, const int* pcol, const double* P, const double* hl, double* out, hipStream_t s);
+// hcc[blk[b]] += sum over t in [ptr[b], ptr[b+1]) of c[t] * L(lid[t]) (its
+// transpose when tr[t]), L(l) = hlfull + l*ns*ns; nlisted blocks.
+void launch_dense_terms(int ns, int nlisted, const int* ptr, const int* blk, const int* lid, const int* tr,
+                        const double* c, const double* hlfull, double* hcc, hipStream_t s);
 // max-reductions over n doubles into kReduceBlocks partial results
 constexpr int kReduceBlocks = 1024;
 void launch_absmax(size_t n, const double* x, double* partial, hipStream_t s);

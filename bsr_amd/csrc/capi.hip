@@ -950,6 +950,7 @@ int bsrg_set_grid(bsrg_ctx* c, int ks, int ns, int ml, int me, double h, const d
   c->basis_nk = 0;
   c->exch_nk = 0;
   c->orb_ok = false;
+  c->lval_ok = false;
   c->wave_ok = false;
   return 0;
 }
@@ -1060,6 +1061,7 @@ int bsrg_set_orbitals(bsrg_ctx* c, int nbf, const double* pbs, const int* iech, 
   c->wtab.assign(static_cast<size_t>(nbf) * nbf, -1);
   c->wvec.clear();
   c->wvec_uploaded = -1;
+  c->lval_ok = false;
   c->orb_ok = true;
   return 0;
 }
@@ -1129,6 +1131,265 @@ int bsrg_odata(bsrg_ctx* c, int n, const int* ipt, const int* k3, const int* k4,
   T.clear();
   if (int rc = add_odata(c, c->cur, T, n, ipt, k3, k4, cd, eps_c, 1.0, true)) return rc;
   return apply_terms(c, c->cur, T, false);
+}
+
+namespace {
+
+// ZCB(k1,k2,k3) = (3j symbol (k1 k2 k3; 0 0 0))^2: with 2g = k1+k2+k3 even,
+// (2g-2k1)!(2g-2k2)!(2g-2k3)!/(2g+1)! * [g!/((g-k1)!(g-k2)!(g-k3)!)]^2;
+// zero for odd 2g or a broken triangle.  The factorial ratios are built as
+// running products so nothing overflows for the multipoles used here.
+double zcb(int k1, int k2, int k3) {
+  const int J = k1 + k2 + k3;
+  if (J % 2) return 0.0;
+  const int g = J / 2;
+  if (g < k1 || g < k2 || g < k3) return 0.0;
+  auto fact = [](int n) { long double f = 1.0L; for (int i = 2; i <= n; ++i) f *= i; return f; };
+  const long double a = fact(J - 2 * k1) * fact(J - 2 * k2) * fact(J - 2 * k3) / fact(J + 1);
+  const long double b = fact(g) / (fact(g - k1) * fact(g - k2) * fact(g - k3));
+  return static_cast<double>(a * b * b);
+}
+
+}  // namespace
+
+int bsrg_gen_lval(bsrg_ctx* c, double z, int kclosd) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->grid_ok) return fail(-3, "grid not set (bsrg_set_grid)");
+  if (!c->orb_ok) return fail(-4, "orbitals not set (bsrg_set_orbitals)");
+  const int nbf = c->nbf, ns = c->G.ns, ks = c->G.ks, nb = c->nb;
+  if (kclosd < 0 || kclosd > nbf) return fail(-60, "Gen_Lval: kclosd outside 0..nbf");
+  for (int ip = 0; ip < kclosd; ++ip)
+    if (c->iech[ip] != 0) return fail(-61, "Gen_Lval: a core orbital is a channel orbital");
+  HIPCHK(hipSetDevice(c->device));
+  c->lval_ok = false;
+  int mlh = 0, lcore = 0;
+  for (int i = 0; i < nbf; ++i) mlh = std::max(mlh, c->lbs[i]);
+  for (int ip = 0; ip < kclosd; ++ip) lcore = std::max(lcore, c->lbs[ip]);
+  // bases: the direct potential of the core needs k = 0, its exchange k up
+  // to min(l + l_core, 7)
+  if (kclosd > 0) {
+    const int nk = std::max(c->basis_nk, std::min(mlh + lcore, 7) + 1);
+    if (int rc = ensure_bases(c, nk, true)) return rc;
+  }
+  const size_t nss = static_cast<size_t>(ns) * ns, nsk = static_cast<size_t>(ns) * ks;
+  const std::vector<double> db2 = c->G.db2(), rm1 = c->G.mrm(-1), rm2 = c->G.mrm(-2);
+  // per l: the L band (hl, lower storage), the exchange list (E column,
+  // factor), k outer and core orbital inner; one direct list for all l
+  std::vector<double> hl(static_cast<size_t>(mlh + 1) * nsk, 0.0), ec, dc;
+  std::vector<int> eid, eptr{0}, did, lorb, lptr{0}, present(mlh + 1, 0);
+  for (int i = 0; i < nbf; ++i) present[c->lbs[i]] = 1;
+  for (int ip = 0; ip < kclosd; ++ip) {
+    int p;
+    if (int rc = pair_index(c, ip + 1, ip + 1, p)) return rc;
+    did.push_back(1 + p);  // D(k = 0, pair (ip, ip)) in the band table
+    dc.push_back(-2.0 * (4 * c->lbs[ip] + 2));
+  }
+  for (int l = 0; l <= mlh; ++l) {
+    if (present[l]) {
+      double* h = &hl[l * nsk];
+      const double c1 = static_cast<double>(l) * (l + 1), c2 = 2.0 * z;
+      for (size_t q = 0; q < nsk; ++q) h[q] = db2[q] - c1 * rm2[q] + c2 * rm1[q];
+      const double cb = db2[0] - db2[(ns - 1) + static_cast<size_t>(ns) * (ks - 2)];  // Bloch term
+      h[(ns - 1) + static_cast<size_t>(ns) * (ks - 2)] += cb;
+      h[(ns - 1) + static_cast<size_t>(ns) * (ks - 1)] -= cb;
+      if (kclosd > 0) {
+        int kmin = 1000, kmax = 0;
+        for (int ip = 0; ip < kclosd; ++ip) {
+          kmin = std::min(kmin, std::abs(l - c->lbs[ip]));
+          kmax = std::max(kmax, l + c->lbs[ip]);
+        }
+        kmax = std::min(kmax, 7);
+        for (int k = kmin; k <= kmax; ++k)
+          for (int ip = 0; ip < kclosd; ++ip) {
+            const double f = (4 * c->lbs[ip] + 2) * zcb(l, k, c->lbs[ip]);
+            if (f == 0.0) continue;
+            const int a = c->bidx[ip];
+            eid.push_back((k * nb + a) * nb + a);
+            ec.push_back(f);
+          }
+      }
+      for (int i = kclosd; i < nbf; ++i)
+        if (c->lbs[i] == l && c->iech[i] == 0) lorb.push_back(i);
+    }
+    eptr.push_back(static_cast<int>(eid.size()));
+    lptr.push_back(static_cast<int>(lorb.size()));
+  }
+  const int nlorb = static_cast<int>(lorb.size());
+  std::vector<int> pcol(std::max(nlorb, 1), 0);
+  for (int o = 0; o < nlorb; ++o) pcol[o] = c->bidx[lorb[o]];
+  // one upload of the doubles (bands, factors) and one of the ints
+  const size_t o_hl = 0, o_ec = hl.size(), o_dc = o_ec + ec.size(), nd_ = o_dc + dc.size();
+  std::vector<double> dbl(nd_ + 1, 0.0);
+  std::copy(hl.begin(), hl.end(), dbl.begin() + o_hl);
+  std::copy(ec.begin(), ec.end(), dbl.begin() + o_ec);
+  std::copy(dc.begin(), dc.end(), dbl.begin() + o_dc);
+  const size_t o_eid = 0, o_did = eid.size(), o_pc = o_did + did.size(), ni = o_pc + pcol.size();
+  std::vector<int> ints(ni + 1, 0);
+  std::copy(eid.begin(), eid.end(), ints.begin() + o_eid);
+  std::copy(did.begin(), did.end(), ints.begin() + o_did);
+  std::copy(pcol.begin(), pcol.end(), ints.begin() + o_pc);
+  hipStream_t s = c->stream;
+  HIPCHK(c->lwork.resize(dbl.size() + static_cast<size_t>(std::max(nlorb, 1)) * ns));
+  HIPCHK(c->lwork_i.resize(ints.size()));
+  HIPCHK(hipMemcpyAsync(c->lwork.p, dbl.data(), dbl.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->lwork_i.p, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(c->hlfull.resize(static_cast<size_t>(mlh + 1) * nss));
+  launch_fill(static_cast<size_t>(mlh + 1) * nss, 0.0, c->hlfull.p, s);
+  double* lv_dev = c->lwork.p + dbl.size();
+  for (int l = 0; l <= mlh; ++l) {
+    if (!present[l]) continue;
+    double* out = c->hlfull.p + l * nss;
+    launch_lcore(ns, ks, eptr[l + 1] - eptr[l], c->lwork_i.p + o_eid + eptr[l], c->lwork.p + o_ec + eptr[l], c->E.p,
+                 kclosd, c->lwork_i.p + o_did, c->lwork.p + o_dc, c->bands.p, c->lwork.p + o_hl + l * nsk, out, s);
+    launch_lvec(ns, lptr[l + 1] - lptr[l], c->lwork_i.p + o_pc + lptr[l], c->P.p, out, lv_dev + lptr[l] * ns, s);
+  }
+  HIPCHK(hipGetLastError());
+  std::vector<double> hf(static_cast<size_t>(mlh + 1) * nss), lvo(static_cast<size_t>(std::max(nlorb, 1)) * ns);
+  HIPCHK(hipMemcpyAsync(hf.data(), c->hlfull.p, hf.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(lvo.data(), lv_dev, lvo.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  // L_int(i, j) = <p_i|hl_core|p_j> from the lower triangle, as Gen_Lval
+  // sums it (l_core.f90:122-130); L_vec(:, i) joins the vector table
+  c->lint.assign(static_cast<size_t>(nbf) * nbf, 0.0);
+  c->lvec.assign(static_cast<size_t>(ns) * nbf, 0.0);
+  c->lvec_id.assign(nbf, -1);
+  for (int o = 0; o < nlorb; ++o) {
+    const int i = lorb[o], l = c->lbs[i];
+    const double* h = &hf[l * nss];
+    const double* pi = &c->pbs[static_cast<size_t>(ns) * i];
+    for (int q = o; q < lptr[l + 1]; ++q) {
+      const int j = lorb[q];
+      const double* pj = &c->pbs[static_cast<size_t>(ns) * j];
+      double v = 0.0;
+      for (int m = 0; m < ns; ++m)
+        for (int k = 0; k < m; ++k) v += h[m + static_cast<size_t>(ns) * k] * (pi[m] * pj[k] + pi[k] * pj[m]);
+      for (int m = 0; m < ns; ++m) v += h[m + static_cast<size_t>(ns) * m] * pi[m] * pj[m];
+      c->lint[i + static_cast<size_t>(nbf) * j] = v;
+      c->lint[j + static_cast<size_t>(nbf) * i] = v;
+    }
+    std::copy(&lvo[static_cast<size_t>(o) * ns], &lvo[static_cast<size_t>(o + 1) * ns],
+              &c->lvec[static_cast<size_t>(ns) * i]);
+    c->lvec_id[i] = static_cast<int>(c->wvec.size() / ns);
+    c->wvec.insert(c->wvec.end(), &lvo[static_cast<size_t>(o) * ns], &lvo[static_cast<size_t>(o + 1) * ns]);
+  }
+  c->mlh = mlh;
+  c->lval_ok = true;
+  return 0;
+}
+
+int bsrg_lval_get(bsrg_ctx* c, int* mlh, double* hl_full, double* l_int, double* l_vec) {
+  if (int rc = check_ctx(c)) return rc;
+  if (!c->lval_ok) return fail(-62, "L_data: Gen_Lval has not run (bsrg_gen_lval)");
+  HIPCHK(hipSetDevice(c->device));
+  if (mlh) *mlh = c->mlh;
+  const size_t nss = static_cast<size_t>(c->G.ns) * c->G.ns;
+  if (hl_full) {
+    HIPCHK(hipMemcpyAsync(hl_full, c->hlfull.p, (c->mlh + 1) * nss * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (l_int) std::copy(c->lint.begin(), c->lint.end(), l_int);
+  if (l_vec) std::copy(c->lvec.begin(), c->lvec.end(), l_vec);
+  return 0;
+}
+
+int bsrg_ldata(bsrg_ctx* c, int jtype, int l, int n, const int* ipt, const int* k1, const int* k2, const int* k3,
+               const int* k4, const double* cd, int iitar, double eps_c) {
+  if (int rc = check_ctx(c)) return rc;
+  if (n == 0) return 0;
+  if (!c->lval_ok) return fail(-62, "L_data: Gen_Lval has not run (bsrg_gen_lval)");
+  if (jtype < 1 || jtype > 9) return fail(-64, " L_data: uknown itype ");
+  if (l < 0 || l > c->mlh) return fail(-63, "L_data: l outside 0..max l of the orbitals");
+  if (n < 0 || !k1 || !k2 || !k3 || !k4 || !cd) return fail(-1, "bsrg_ldata: missing list arrays");
+  if (int rc = cur_ready(c)) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  if (c->basis_nk == 0)
+    if (int rc = ensure_bases(c, 1, false)) return rc;
+  Matrices& M = c->cur;
+  const int nbf = c->nbf, nV = c->basis_nk * c->npair * c->nb;
+  Terms& T = c->TC;
+  if (int rc = wait_staged(c->stC)) return rc;
+  T.clear();
+  struct Dense { int blk, tr; double c; };
+  std::vector<Dense> dense;
+  for (int q = 0; q < n; ++q) {
+    const int i = ipt ? ipt[q] - 1 : q;
+    if (i < 0 || i >= n) return fail(-1, "bsrg_ldata: ipt out of range");
+    const double cc = cd[i];
+    if (std::fabs(cc) < eps_c) continue;
+    if (jtype == 1) {
+      const int i1 = k1[i] / kIbi, i2 = k1[i] % kIbi;
+      if (i1 < 1 || i1 > nbf || i2 < 1 || i2 > nbf) return fail(-30, "orbital index out of range");
+      const double C = c->lint[(i1 - 1) + static_cast<size_t>(nbf) * (i2 - 1)] * cc;
+      const int ic = -k3[i], jc = -k4[i];
+      if (ic < 0 && jc == 0) {  // L(..) <i|j>
+        const int io = -ic, a1 = io / kIbo, a2 = io % kIbo;
+        if (a1 < 1 || a1 > nbf || a2 < 1 || a2 > nbf) return fail(-30, "orbital index out of range");
+        const int ich = c->iech[a1 - 1], jch = c->iech[a2 - 1];
+        if (int rc = chk_ch(M, ich, jch, "UPDATE_CF")) return rc;
+        if (iitar > 0 && ich != jch) band_hl(T, ich, jch, C);
+        T.scal.push_back({1, blk_index(ich, jch), -1, C});
+      } else if ((ic > 0 && jc > 0) || (ic < 0 && jc != 0)) {
+        if (int rc = add_struct1(c, M, T, k3[i], k4[i], -1, C, false, true)) return rc;
+      } else {
+        return fail(-65, " L_data: unknown structure ");
+      }
+    } else if (jtype <= 5) {
+      const int o = k2[i], ich = k3[i], io = k4[i];
+      if (o < 1 || o > nbf) return fail(-30, "orbital index out of range");
+      const int v = c->lvec_id[o - 1];  // -1: L_vec(:, o) is zero
+      if (io > 0) {
+        const int j1 = io / kIbo, j2 = io % kIbo;
+        int w;
+        if (int rc = get_v(c, j1, j2, w)) return rc;
+        const int jch = c->iech[j1 - 1];
+        if (int rc = chk_ch(M, ich, jch, "UPDATE_HW")) return rc;
+        if (v >= 0) {
+          if (ich >= jch) T.r1.push_back({blk_index(ich, jch), nV + v, nV + w, -1, cc});
+          else T.r1.push_back({blk_index(ich, jch), nV + w, nV + v, -1, cc});
+        }
+      } else if (io < 0) {
+        if (ich < 1 || ich > M.nch) return fail(-13, "UPDATE_HV: channel index out of range");
+        if (-io > M.ncp) return fail(-14, "UPDATE_HV: bound index out of range");
+        if (v >= 0) T.vec.push_back({(-io - 1) * M.nch + (ich - 1), nV + v, -1, cc});
+      } else {
+        return fail(-66, " L_data: uknown structure for itype 2-5 ");
+      }
+    } else {
+      const int ich = k1[i] / kIbi, jch = k1[i] % kIbi;
+      if (int rc = chk_ch(M, ich, jch, "UPDATE_HX")) return rc;
+      if (ich == jch || iitar > 1) dense.push_back({blk_index(ich, jch), ich < jch ? 1 : 0, cc});
+      else T.scal.push_back({2, blk_index(ich, jch), -1, cc});
+    }
+  }
+  if (int rc = apply_terms(c, M, T, false)) return rc;
+  if (dense.empty()) return 0;
+  // c * hl_full(l) (or its transpose) per block, in list order within a block
+  std::stable_sort(dense.begin(), dense.end(), [](const Dense& a, const Dense& b) { return a.blk < b.blk; });
+  std::vector<int> ints, ptr{0}, blk;
+  std::vector<double> cv;
+  for (size_t t = 0; t < dense.size(); ++t) {
+    if (t > 0 && dense[t].blk != dense[t - 1].blk) ptr.push_back(static_cast<int>(t));
+    if (t == 0 || dense[t].blk != dense[t - 1].blk) blk.push_back(dense[t].blk);
+    cv.push_back(dense[t].c);
+  }
+  ptr.push_back(static_cast<int>(dense.size()));
+  const int nl = static_cast<int>(blk.size()), nt = static_cast<int>(dense.size());
+  ints.insert(ints.end(), ptr.begin(), ptr.end());
+  ints.insert(ints.end(), blk.begin(), blk.end());
+  ints.insert(ints.end(), dense.size(), l);
+  for (const Dense& d : dense) ints.push_back(d.tr);
+  hipStream_t s = c->stream;
+  HIPCHK(c->lwork.resize(cv.size()));
+  HIPCHK(c->lwork_i.resize(ints.size()));
+  HIPCHK(hipMemcpyAsync(c->lwork.p, cv.data(), cv.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->lwork_i.p, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  const int* dp = c->lwork_i.p;
+  launch_dense_terms(c->G.ns, nl, dp, dp + nl + 1, dp + 2 * nl + 1, dp + 2 * nl + 1 + nt, c->lwork.p, c->hlfull.p,
+                     M.hcc.p, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
 }
 
 int bsrg_symmetrize_diag(bsrg_ctx* c) {

@@ -341,6 +341,15 @@ struct bsrg_ctx {
   std::vector<int> wtab;                // GET_V registry (i1,i2) -> id, -1 if not yet made
   std::vector<double> wvec;             // nW x ns
   int wvec_uploaded = -1;
+  // one-electron operator with the core (Gen_Lval): hl_full(:,:,l) for
+  // l = 0..mlh on the device, L_int on the host, and L_vec(:,i) as vectors
+  // of the W part of the vector table (lvec_id[i], -1 where L_vec is zero)
+  bool lval_ok = false;
+  int mlh = -1;
+  DevVec<double> hlfull, lwork;
+  DevVec<int> lwork_i;
+  std::vector<double> lint, lvec;
+  std::vector<int> lvec_id;
 
   Matrices cur, S, H;
   bool wave_ok = false, wave_prepared = false, capture_safe = false, capturing = false;

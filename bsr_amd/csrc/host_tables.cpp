@@ -257,6 +257,57 @@ std::vector<double> GridTables::mrm(int mm) const {
   return rm;
 }
 
+// Second derivatives of the ks B-splines nonzero on knot interval left
+// (1-based) at x.  Differentiating B_{i,k} = w_{i,k} B_{i,k-1} +
+// (1 - w_{i+1,k}) B_{i+1,k-1} gives
+// B'_{i,k} = (k-1) [B_{i,k-1}/(t_{i+k-1}-t_i) - B_{i+1,k-1}/(t_{i+k}-t_{i+1})];
+// applied to the first derivatives of order ks-1 (themselves from the
+// values of order ks-2) it gives the second derivatives of order ks.
+namespace {
+void spline_slope(const double* t, int k, int left, const double* f, double* out) {
+  for (int r = 0; r < k; ++r) {
+    const int i = left - k + 1 + r;  // 1-based index of the order-k spline
+    const double hl = r >= 1 ? t[i + k - 2] - t[i - 1] : 0.0, hr = r <= k - 2 ? t[i + k - 1] - t[i] : 0.0;
+    out[r] = (k - 1) * ((hl > 0.0 ? f[r - 1] / hl : 0.0) - (hr > 0.0 ? f[r] / hr : 0.0));
+  }
+}
+}  // namespace
+
+void bspline_second(const double* t, int ks, int left, double x, double* d2) {
+  double v[kMaxKs], d1[kMaxKs];
+  bspline_values(t, ks - 2, left, x, v);
+  spline_slope(t, ks - 1, left, v, d1);
+  spline_slope(t, ks, left, d1, d2);
+}
+
+// mdb2 (src/libs/BS/define_spline.f90:216-245): <B_i|B''_j> in the lower band
+// (i, jp) -> j = i+jp-ks, plus <B_{ns-1}|B''_ns> in the slot (0,0) that the
+// band does not use (the Bloch term of INT_core reads it).
+std::vector<double> GridTables::db2() const {
+  std::vector<double> out(static_cast<size_t>(ns) * ks, 0.0);
+  std::vector<double> d2(static_cast<size_t>(nv) * ks * ks);
+  for (int iv = 0; iv < nv; ++iv)
+    for (int m = 0; m < ks; ++m)
+      bspline_second(t.data(), ks, iv + ks, gr[iv * ks + m], &d2[(static_cast<size_t>(iv) * ks + m) * ks]);
+  for (int ith = 0; ith < ks; ++ith)
+    for (int jth = 0; jth <= ith; ++jth)
+      for (int iv = 0; iv < nv; ++iv) {
+        double s = 0.0;
+        for (int m = 0; m < ks; ++m) {
+          const size_t q = static_cast<size_t>(iv) * ks + m;
+          s += grw[q] * bsp[q * ks + ith] * d2[q * ks + jth];
+        }
+        out[(iv + ith) + static_cast<size_t>(ns) * (jth - ith + ks - 1)] += s;
+      }
+  double s = 0.0;
+  for (int m = 0; m < ks; ++m) {
+    const size_t q = static_cast<size_t>(nv - 1) * ks + m;
+    s += grw[q] * bsp[q * ks + ks - 2] * d2[q * ks + ks - 1];
+  }
+  out[0] += s;
+  return out;
+}
+
 // <x|A|y> for a symmetric lower band; bvmv.f90:2-67.
 double bvmv_sym(int n, int k, const double* a, const double* x, const double* y) {
   double s = 0.0;

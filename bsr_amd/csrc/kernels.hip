@@ -984,6 +984,90 @@ void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int
                      V, hcb);
 }
 
+// ---- one-electron operator with the core (Gen_Lval, L_data) ------------
+// INT_core (src/bsr/BSR_MAT3/l_core.f90:173-257) in full storage: the core
+// exchange sum (k outer, core orbital inner, the order the reference adds
+// them), then in the band region the L band and the summed direct core
+// potential, both read at the lower element (max, min) and written to both
+// halves.  grid (ceil(ns*ns/256)), block 256.
+__global__ __launch_bounds__(256) void k_lcore(int ns, int ks, int nt, const int* __restrict__ eid,
+                                               const double* __restrict__ ec, const double* __restrict__ E, int nd,
+                                               const int* __restrict__ did, const double* __restrict__ dc,
+                                               const double* __restrict__ bands, const double* __restrict__ hl,
+                                               double* __restrict__ out) {
+  const size_t nss = static_cast<size_t>(ns) * ns, e = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= nss) return;
+  const int x = static_cast<int>(e % ns), y = static_cast<int>(e / ns);
+  const int i = max(x, y), j = min(x, y), d = i - j;
+  const bool band = d < ks;
+  const size_t src = band ? i + static_cast<size_t>(ns) * j : e;
+  double v = 0.0;
+  for (int t = 0; t < nt; ++t) v += ec[t] * E[static_cast<size_t>(eid[t]) * nss + src];
+  if (band) {
+    v += hl[i + static_cast<size_t>(ns) * (ks - 1 - d)];
+    if (nd > 0) {
+      double h = 0.0;
+      for (int u = 0; u < nd; ++u) h += dc[u] * bands[static_cast<size_t>(did[u]) * ns * ks + j + static_cast<size_t>(ns) * d];
+      v += h;
+    }
+  }
+  out[e] = v;
+}
+
+// L_vec(m, o) = sum_k hl(k, m) P(k, pcol[o]), k ascending (l_core.f90:143).
+// grid (ceil(ns/256), n), block 256.
+__global__ __launch_bounds__(256) void k_lvec(int ns, const int* __restrict__ pcol, const double* __restrict__ P,
+                                              const double* __restrict__ hl, double* __restrict__ out) {
+  const int o = blockIdx.y, m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= ns) return;
+  const double* p = P + static_cast<size_t>(ns) * pcol[o];
+  const double* h = hl + static_cast<size_t>(ns) * m;
+  double s = 0.0;
+  for (int k = 0; k < ns; ++k) s += h[k] * p[k];
+  out[m + static_cast<size_t>(ns) * o] = s;
+}
+
+// Dense L terms (UPDATE_HX 'x' of C * hl_core, mod_intmat.f90:69-127): each
+// listed block adds its terms in list order, c * L(l) or its transpose.
+// grid (ceil(ns*ns/256), nlisted), block 256.
+__global__ __launch_bounds__(256) void k_dense_terms(int ns, const int* __restrict__ ptr, const int* __restrict__ blk,
+                                                     const int* __restrict__ lid, const int* __restrict__ tr,
+                                                     const double* __restrict__ c, const double* __restrict__ hlfull,
+                                                     double* __restrict__ hcc) {
+  const size_t nss = static_cast<size_t>(ns) * ns, e = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= nss) return;
+  const int b = blockIdx.y;
+  const size_t et = (e / ns) + static_cast<size_t>(ns) * (e % ns);
+  double* h = hcc + static_cast<size_t>(blk[b]) * nss;
+  double v = h[e];
+  for (int t = ptr[b]; t < ptr[b + 1]; ++t) {
+    const double* L = hlfull + static_cast<size_t>(lid[t]) * nss;
+    v += c[t] * L[tr[t] ? et : e];
+  }
+  h[e] = v;
+}
+
+void launch_lcore(int ns, int ks, int nt, const int* eid, const double* ec, const double* E, int nd,
+                  const int* did, const double* dc, const double* bands, const double* hl, double* out,
+                  hipStream_t s) {
+  const size_t nss = static_cast<size_t>(ns) * ns;
+  hipLaunchKernelGGL(k_lcore, dim3(nblocks(nss, 256)), dim3(256), 0, s, ns, ks, nt, eid, ec, E, nd, did, dc, bands,
+                     hl, out);
+}
+
+void launch_lvec(int ns, int n, const int* pcol, const double* P, const double* hl, double* out, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lvec, dim3((ns + 255) / 256, n), dim3(256), 0, s, ns, pcol, P, hl, out);
+}
+
+void launch_dense_terms(int ns, int nlisted, const int* ptr, const int* blk, const int* lid, const int* tr,
+                        const double* c, const double* hlfull, double* hcc, hipStream_t s) {
+  if (nlisted <= 0) return;
+  const size_t nss = static_cast<size_t>(ns) * ns;
+  hipLaunchKernelGGL(k_dense_terms, dim3(nblocks(nss, 256), nlisted), dim3(256), 0, s, ns, ptr, blk, lid, tr, c,
+                     hlfull, hcc);
+}
+
 void launch_symmetrize_diag(int ns, int nch, double* hcc, hipStream_t s) {
   if (nch <= 0) return;
   const int nt = (ns + 31) / 32;

@@ -59,6 +59,9 @@ SIGNATURES = {
     "bsrg_update_hl": [C.c_void_p, C.c_int, C.c_int, C.c_double],
     "bsrg_idata": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp],
     "bsrg_odata": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, C.c_double],
+    "bsrg_gen_lval": [C.c_void_p, C.c_double, C.c_int],
+    "bsrg_lval_get": [C.c_void_p, C.POINTER(C.c_int), _opt_dp(), _opt_dp(), _opt_dp()],
+    "bsrg_ldata": [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp, C.c_int, C.c_double],
     "bsrg_symmetrize_diag": [C.c_void_p],
     "bsrg_scale": [C.c_void_p, C.c_double],
     "bsrg_acf_finish": [C.c_void_p, C.c_int],

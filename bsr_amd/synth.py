@@ -265,6 +265,60 @@ def make_wave(seed: int, ks: int, ns: int, z: float, rmax: float, nch: int, ncp:
 
 # ---- the workloads ---------------------------------------------------------
 
+def l_lists(w: PartialWave, seed: int, fill: float = 0.2) -> dict:
+    """L-integral records (icase 6, L_data.f90:2-149) for a wave, keyed
+    (jtype, l) for l = 0..max l of the orbitals, with the structures of
+    L_data: jtype 1 a bound pair with HB/HV/HW/<i|j> parts, jtype 2-5 a
+    bound orbital with HV/HW parts, jtype 6-9 a channel pair (both orders).
+    Values U[-1,1], sorted and merged like the R^k lists."""
+    rng = np.random.default_rng(seed)
+    nb, nch, ncp, nbf = w.nb, w.nch, w.ncp, w.nbf
+    kept = [(i + 1, a + 1) for i in range(nb, nbf) for a in range(nb) if w.ibort[i, a]]
+    kept_io = np.array([kl * IBO + nl for kl, nl in kept], np.int64)
+    rand_pair = lambda n: (rng.integers(1, nb + 1, n) * IBI + rng.integers(1, nb + 1, n))
+    ch_i, ch_j = np.tril_indices(nch)
+    out = {}
+    for l in range(int(w.lbs.max()) + 1):
+        parts = []
+        if ncp > 0:
+            pi, pj = np.tril_indices(ncp)
+            sel = rng.random(len(pi)) < fill
+            parts.append((rand_pair(int(sel.sum())), -(pi[sel] + 1), -(pj[sel] + 1)))
+            if len(kept_io):
+                io, jc = np.nonzero(rng.random((len(kept_io), ncp)) < fill)
+                parts.append((rand_pair(len(io)), kept_io[io], -(jc + 1)))
+        if len(kept_io):
+            io, jo = np.nonzero(rng.random((len(kept_io), len(kept_io))) < fill)
+            parts.append((rand_pair(len(io)), kept_io[io], kept_io[jo]))
+        i1, i2 = np.nonzero(rng.random((nch, nch)) < fill)
+        parts.append((rand_pair(len(i1)), (nb + i1 + 1) * IBO + (nb + i2 + 1), np.zeros(len(i1))))
+        k1 = np.concatenate([p[0] for p in parts])
+        k3 = np.concatenate([p[1] for p in parts])
+        k4 = np.concatenate([p[2] for p in parts])
+        out[(1, l)] = RecordList.build(k1, np.zeros(len(k1)), k3, k4, rng.uniform(-1, 1, len(k1)))
+        for jt in (2, 3, 4, 5):
+            parts = []
+            if ncp > 0:
+                ich, ic = np.nonzero(rng.random((nch, ncp)) < fill)
+                parts.append((rng.integers(1, nb + 1, len(ich)), ich + 1, -(ic + 1)))
+            if len(kept_io):
+                ich, io = np.nonzero(rng.random((nch, len(kept_io))) < fill)
+                parts.append((rng.integers(1, nb + 1, len(ich)), ich + 1, kept_io[io]))
+            if parts:
+                k2 = np.concatenate([p[0] for p in parts])
+                k3 = np.concatenate([p[1] for p in parts])
+                k4 = np.concatenate([p[2] for p in parts])
+                out[(jt, l)] = RecordList.build(np.zeros(len(k2)), k2, k3, k4, rng.uniform(-1, 1, len(k2)))
+        for jt in (6, 7, 8, 9):
+            sel = rng.random(len(ch_i)) < fill
+            ci, cj = ch_i[sel] + 1, ch_j[sel] + 1
+            flip = rng.random(len(ci)) < 0.5
+            k1 = np.where(flip & (ci != cj), cj * IBI + ci, ci * IBI + cj)
+            z = np.zeros(len(k1))
+            out[(jt, l)] = RecordList.build(k1, z, z, z, rng.uniform(-1, 1, len(k1)))
+    return out
+
+
 def c1(seed: int = 1) -> PartialWave:
     """Plain FP64 small: ks=8, ns=60, Z=6, Rmax=20, k=0..4, 20 channels,
     5 target states, 10% fill, dim 1200."""

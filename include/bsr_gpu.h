@@ -180,6 +180,33 @@ int bsrg_idata(bsrg_ctx* ctx, int icase, int jtype, int jpol, int ncdata, const 
 int bsrg_odata(bsrg_ctx* ctx, int ncdata, const int* ipt, const int* k3, const int* k4,
                const double* cdata, double eps_c);
 
+/* One-electron operator with the closed core, for every l of the orbitals:
+ * hl_full(:,:,l) = d2/dr2 - l(l+1)/r^2 + 2z/r (with the Bloch term at the
+ * outer boundary) + the direct and exchange potential of the core orbitals
+ * 1..kclosd (R^k, k = 0 and |l-l_c| .. min(l+l_c, 7)); then
+ * L_int(i,j) = <p_i|hl_full|p_j> and L_vec(:,i) = hl_full p_i for the
+ * bound orbitals beyond the core.  Replaces Gen_Lval and INT_core,
+ * src/bsr/BSR_MAT3/l_core.f90:92-170, :173-257 (non-relativistic: the
+ * mass-velocity branch taken when rel is set is not reproduced).  Needs
+ * bsrg_set_orbitals; invalidated by a new grid or new orbitals. */
+int bsrg_gen_lval(bsrg_ctx* ctx, double z, int kclosd);
+
+/* Results of bsrg_gen_lval: mlh = largest l; hl_full (ns,ns,0:mlh),
+ * L_int (nbf,nbf), L_vec (ns,nbf), column-major (module L_core,
+ * l_core.f90:2-48); any output pointer may be NULL. */
+int bsrg_lval_get(bsrg_ctx* ctx, int* mlh, double* hl_full, double* l_int, double* l_vec);
+
+/* L-integral records of one (jtype, l) list into the current matrix.
+ * Replaces L_data(jtype,l), src/bsr/BSR_MAT3/L_data.f90:2-149: jtype 1
+ * scales L_int into HB/HV/HW/<i|j> (UPDATE_HL with sb when iitar > 0 and
+ * the channels differ, then Target_h), jtype 2-5 use L_vec (HW/HV), jtype
+ * 6-9 add c*hl_full(:,:,l) to block (ich,jch) when ich == jch or
+ * iitar > 1, else Target_h(ich,jch,0,c).  iitar as bsr_mat holds it
+ * (mod_param.f90:33); records with |c| < eps_c skipped. */
+int bsrg_ldata(bsrg_ctx* ctx, int jtype, int l, int ncdata, const int* ipt, const int* k1,
+               const int* k2, const int* k3, const int* k4, const double* cdata, int iitar,
+               double eps_c);
+
 /* Diagonal channel blocks -> (C+C^T)/2.  Replaces sub1.f90:80-84, :187-191. */
 int bsrg_symmetrize_diag(bsrg_ctx* ctx);
 

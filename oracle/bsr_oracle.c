@@ -632,6 +632,10 @@ typedef struct {
   double *hcc, *hcb, *hbb, *acf, *htarg, *otarg;
   double* qtab; /* quadr cache (nbf,nbf,0:mk) of m=k, computed on demand */
   int* qok;
+  /* one-electron operator of Gen_Lval: hl_full (ns,ns,0:mlh), L_int
+   * (nbf,nbf), L_vec (ns,nbf); lok once computed */
+  int mlh, lok;
+  double *hl_full, *lint, *lvec;
 } orc_wave;
 
 /* Orbital data as Read_data sets it up (read_data.f90:142-208):
@@ -680,7 +684,7 @@ void orc_wave_free(orc_wave* w) {
   if (!w) return;
   free(w->pbs); free(w->qbs); free(w->obs); free(w->iech); free(w->lbs); free(w->ibort);
   free(w->hcc); free(w->hcb); free(w->hbb); free(w->acf); free(w->htarg); free(w->otarg);
-  free(w->qtab); free(w->qok); free(w);
+  free(w->qtab); free(w->qok); free(w->hl_full); free(w->lint); free(w->lvec); free(w);
 }
 
 void orc_wave_get(const orc_wave* w, double* hcc, double* hcb, double* hbb, double* acf,
@@ -1033,3 +1037,300 @@ int orc_wave_update_hv(orc_wave* w, int ich, int ic, const double* v, double c) 
 }
 int orc_wave_update_hb(orc_wave* w, int ic, int jc, double c) { return update_hb(w, ic, jc, c); }
 int orc_wave_update_cf(orc_wave* w, int k, int ich, int jch, double c) { return update_cf(w, k, ich, jch, c); }
+
+/* ------------------------------------------------------------------ */
+/* One-electron operator with the core and the L-integral lists        */
+/* (BSR_MAT3 l_core.f90, L_data.f90), non-relativistic                 */
+/* ------------------------------------------------------------------ */
+
+/* Derivative of the order-k B-splines nonzero on interval left from a
+ * quantity f of the order-(k-1) ones (f[r] belongs to spline left-k+2+r):
+ * D[r] = (k-1) [f_i/(t_{i+k-1}-t_i) - f_{i+1}/(t_{i+k}-t_{i+1})] for the
+ * order-k spline i = left-k+1+r (1-based knots), a term with a zero knot
+ * span being zero. */
+static void bderiv_step(const double* t, int k, int left, const double* f, double* D) {
+  for (int r = 0; r < k; r++) {
+    const int i = left - k + 1 + r; /* 1-based */
+    double v = 0.0;
+    if (r >= 1) {
+      const double h = t[i + k - 2] - t[i - 1];
+      if (h > 0.0) v += f[r - 1] / h;
+    }
+    if (r <= k - 2) {
+      const double h = t[i + k - 1] - t[i];
+      if (h > 0.0) v -= f[r] / h;
+    }
+    D[r] = (k - 1) * v;
+  }
+}
+
+/* Values, first and second derivatives of the ks B-splines nonzero on knot
+ * interval left at x (the quantities vbsplvd returns for nderiv = 3,
+ * src/libs/BS/vbsplvd.f90:3-171), from the values of orders ks-2 and ks-1
+ * and the derivative formula applied once and twice. */
+static void bderivs(const double* t, int ks, int left, double x, double* b0, double* b1, double* b2) {
+  double v1[32], v2[32], d1[32];
+  bvalues(t, ks, left, x, b0);
+  bvalues(t, ks - 1, left, x, v1);
+  bvalues(t, ks - 2, left, x, v2);
+  bderiv_step(t, ks, left, v1, b1);
+  bderiv_step(t, ks - 1, left, v2, d1); /* first derivatives of order ks-1 */
+  bderiv_step(t, ks, left, d1, b2);
+}
+
+/* First and second derivatives at the Gauss points, [(iv*ks+m)*ks+ith]. */
+void orc_grid_bspd(const orc_grid* g, double* d1, double* d2) {
+  int ks = g->ks, nv = g->nv;
+  for (int iv = 0; iv < nv; iv++)
+    for (int m = 0; m < ks; m++) {
+      double b0[32], b1[32], b2[32];
+      bderivs(g->t, ks, iv + ks, g->gr[iv * ks + m], b0, b1, b2);
+      for (int ith = 0; ith < ks; ith++) {
+        if (d1) d1[(iv * ks + m) * ks + ith] = b1[ith];
+        if (d2) d2[(iv * ks + m) * ks + ith] = b2[ith];
+      }
+    }
+}
+
+/* <B_i|B''_j> in lower band storage (ns,ks), element (i,jp) for
+ * j = i+jp-ks, plus in the unused slot (1,1) the element
+ * <B_{ns-1}|B''_ns> of the last interval (mdb2,
+ * src/libs/BS/define_spline.f90:216-245). */
+void orc_db2(const orc_grid* g, double* db2) {
+  int ks = g->ks, ns = g->ns, nv = g->nv;
+  double* d2 = (double*)malloc(sizeof(double) * nv * ks * ks);
+  orc_grid_bspd(g, NULL, d2);
+  memset(db2, 0, sizeof(double) * ns * ks);
+  for (int ith = 0; ith < ks; ith++)
+    for (int jth = 0; jth <= ith; jth++) {
+      int jcol = jth - ith + ks - 1;
+      for (int iv = 0; iv < nv; iv++) {
+        double s = 0.0;
+        for (int m = 0; m < ks; m++) {
+          size_t q = (size_t)iv * ks + m;
+          s += g->grw[q] * g->bsp[q * ks + ith] * d2[q * ks + jth];
+        }
+        db2[(iv + ith) + (size_t)ns * jcol] += s;
+      }
+    }
+  {
+    double s = 0.0;
+    for (int m = 0; m < ks; m++) {
+      size_t q = (size_t)(nv - 1) * ks + m;
+      s += g->grw[q] * g->bsp[q * ks + ks - 2] * d2[q * ks + ks - 1];
+    }
+    db2[0] += s;
+  }
+  free(d2);
+}
+
+/* ZCB(k1,k2,k3) = (3j symbol (k1 k2 k3; 0 0 0))^2 from the closed form
+ * with 2g = k1+k2+k3 even:
+ * [(2g-2k1)!(2g-2k2)!(2g-2k3)!/(2g+1)!] [g!/((g-k1)!(g-k2)!(g-k3)!)]^2,
+ * zero when 2g is odd or a triangle condition fails. */
+double orc_zcb(int k1, int k2, int k3) {
+  int J = k1 + k2 + k3;
+  if (J % 2) return 0.0;
+  int g = J / 2;
+  if (g - k1 < 0 || g - k2 < 0 || g - k3 < 0) return 0.0;
+  long double f[128];
+  f[0] = 1.0L;
+  for (int i = 1; i < 128; i++) f[i] = f[i - 1] * i;
+  if (J + 1 >= 128) return 0.0;
+  long double a = f[J - 2 * k1] * f[J - 2 * k2] * f[J - 2 * k3] / f[J + 1];
+  long double b = f[g] / (f[g - k1] * f[g - k2] * f[g - k3]);
+  return (double)(a * b * b);
+}
+
+/* Gen_Lval with INT_core for every l present (l_core.f90:92-170,
+ * :173-257), non-relativistic: hl = db2 - l(l+1) rm2 + 2z rm1 with the
+ * Bloch term at the outer boundary, the direct core potential (k = 0) and
+ * the core exchange (k up to 7) from the R^k band, the sum expanded into
+ * the full hl_full(:,:,l); then L_int and L_vec of the bound orbitals
+ * beyond the core.  Orbitals 1..kclosd are the closed core. */
+int orc_wave_gen_lval(orc_wave* w, double z, int kclosd) {
+  orc_grid* g = w->g;
+  int ns = w->ns, ks = w->ks, nbf = w->nbf;
+  if (kclosd < 0 || kclosd > nbf) return fail(-60, "Gen_Lval: kclosd outside 0..nbf");
+  for (int ip = 0; ip < kclosd; ip++)
+    if (w->iech[ip] != 0) return fail(-61, "Gen_Lval: a core orbital is a channel orbital");
+  int mlh = 0;
+  for (int i = 0; i < nbf; i++) mlh = w->lbs[i] > mlh ? w->lbs[i] : mlh;
+  size_t nss = (size_t)ns * ns;
+  free(w->hl_full); free(w->lint); free(w->lvec);
+  w->mlh = mlh;
+  w->hl_full = (double*)calloc(nss * (mlh + 1), sizeof(double));
+  w->lint = (double*)calloc((size_t)nbf * nbf, sizeof(double));
+  w->lvec = (double*)calloc((size_t)ns * nbf, sizeof(double));
+  double *db2 = (double*)malloc(sizeof(double) * ns * ks), *rm1 = (double*)malloc(sizeof(double) * ns * ks),
+         *rm2 = (double*)malloc(sizeof(double) * ns * ks), *hl = (double*)malloc(sizeof(double) * ns * ks),
+         *hd = (double*)calloc((size_t)ns * ks, sizeof(double)), *d = (double*)malloc(sizeof(double) * nss),
+         *x = (double*)malloc(sizeof(double) * nss), *hc = (double*)malloc(sizeof(double) * nss);
+  orc_db2(g, db2);
+  orc_mrm(g, -1, rm1);
+  orc_mrm(g, -2, rm2);
+  int have_dir = 0, rc = 0;
+  for (int l = 0; l <= mlh && !rc; l++) {
+    int present = 0;
+    for (int i = 0; i < nbf; i++) present |= w->lbs[i] == l;
+    if (!present) continue;
+    /* INT_core(l) */
+    const double c1 = (double)l * (l + 1), c2 = 2.0 * z;
+    for (size_t q = 0; q < (size_t)ns * ks; q++) hl[q] = db2[q] - c1 * rm2[q] + c2 * rm1[q];
+    {
+      const double cb = db2[0] - db2[(ns - 1) + (size_t)ns * (ks - 2)];
+      hl[(ns - 1) + (size_t)ns * (ks - 2)] += cb;
+      hl[(ns - 1) + (size_t)ns * (ks - 1)] -= cb;
+    }
+    memset(hc, 0, sizeof(double) * nss);
+    if (kclosd > 0) {
+      if (!have_dir) {
+        mrk_cell_internal(g, 0);
+        for (int ip = 0; ip < kclosd; ip++) {
+          const double cc = -2.0 * (4 * w->lbs[ip] + 2);
+          const double* p = w->pbs + (size_t)ns * ip;
+          orc_density(ns, ks, d, p, p, 's');
+          if ((rc = orc_convol(ns, ks, x, d, 2, 's', 's', g->rkb))) break;
+          for (size_t q = 0; q < (size_t)ns * ks; q++) hd[q] += cc * x[q];
+        }
+        have_dir = 1;
+      }
+      int kmin = 1000, kmax = 0;
+      for (int ip = 0; ip < kclosd; ip++) {
+        int k = abs(l - w->lbs[ip]);
+        if (k < kmin) kmin = k;
+        k = l + w->lbs[ip];
+        if (k > kmax) kmax = k;
+      }
+      if (kmax > 7) kmax = 7;
+      for (int k = kmin; k <= kmax && !rc; k++) {
+        mrk_cell_internal(g, k);
+        for (int ip = 0; ip < kclosd; ip++) {
+          const double cc = (4 * w->lbs[ip] + 2) * orc_zcb(l, k, w->lbs[ip]);
+          if (cc == 0.0) continue;
+          const double* p = w->pbs + (size_t)ns * ip;
+          orc_density(ns, ks, d, p, p, 'x');
+          if ((rc = orc_convol(ns, ks, x, d, 3, 's', 's', g->rkb))) break;
+          for (size_t q = 0; q < nss; q++) hc[q] += cc * x[q];
+        }
+      }
+    }
+    for (int jp = 1; jp <= ks; jp++)
+      for (int i = 1 - jp + ks; i <= ns; i++) {
+        int j = i + jp - ks;
+        hc[(i - 1) + (size_t)ns * (j - 1)] += hl[(i - 1) + (size_t)ns * (jp - 1)];
+        hc[(j - 1) + (size_t)ns * (i - 1)] = hc[(i - 1) + (size_t)ns * (j - 1)];
+      }
+    if (kclosd > 0)
+      for (int jp = 1; jp <= ks; jp++)
+        for (int i = 1; i <= ns - jp + 1; i++) {
+          int j = i + jp - 1;
+          hc[(i - 1) + (size_t)ns * (j - 1)] += hd[(i - 1) + (size_t)ns * (jp - 1)];
+          hc[(j - 1) + (size_t)ns * (i - 1)] = hc[(i - 1) + (size_t)ns * (j - 1)];
+        }
+    /* L_int, L_vec of the bound orbitals with this l beyond the core */
+    for (int i = kclosd; i < nbf; i++) {
+      if (w->lbs[i] != l || w->iech[i] != 0) continue;
+      const double* pi = w->pbs + (size_t)ns * i;
+      for (int j = i; j < nbf; j++) {
+        if (w->lbs[j] != l || w->iech[j] != 0) continue;
+        const double* pj = w->pbs + (size_t)ns * j;
+        double c = 0.0;
+        for (int m = 0; m < ns; m++)
+          for (int k = 0; k < m; k++) c += hc[m + (size_t)ns * k] * (pi[m] * pj[k] + pi[k] * pj[m]);
+        for (int m = 0; m < ns; m++) c += hc[m + (size_t)ns * m] * pi[m] * pj[m];
+        w->lint[i + (size_t)nbf * j] = c;
+        w->lint[j + (size_t)nbf * i] = c;
+      }
+      for (int m = 0; m < ns; m++) {
+        double s = 0.0;
+        for (int k = 0; k < ns; k++) s += hc[k + (size_t)ns * m] * pi[k];
+        w->lvec[m + (size_t)ns * i] = s;
+      }
+    }
+    memcpy(w->hl_full + nss * l, hc, sizeof(double) * nss);
+  }
+  free(db2); free(rm1); free(rm2); free(hl); free(hd); free(d); free(x); free(hc);
+  w->lok = rc == 0;
+  return rc;
+}
+
+int orc_wave_lval_mlh(const orc_wave* w) { return w->lok ? w->mlh : -1; }
+
+void orc_wave_lval_get(const orc_wave* w, double* hl_full, double* lint, double* lvec) {
+  if (!w->lok) return;
+  size_t nss = (size_t)w->ns * w->ns;
+  if (hl_full) memcpy(hl_full, w->hl_full, sizeof(double) * nss * (w->mlh + 1));
+  if (lint) memcpy(lint, w->lint, sizeof(double) * w->nbf * w->nbf);
+  if (lvec) memcpy(lvec, w->lvec, sizeof(double) * w->ns * w->nbf);
+}
+
+/* L_data(jtype,l), src/bsr/BSR_MAT3/L_data.f90:2-149, over the records in
+ * the order ipt (1-based); iitar as the module bsr_mat holds it
+ * (mod_param.f90:33), records with |c| < eps_c skipped. */
+int orc_wave_ldata(orc_wave* w, int jtype, int l, int nc, const int* ipt, const int* K1, const int* K2,
+                   const int* K3, const int* K4, const double* cd, int iitar, double eps_c) {
+  int ns = w->ns, rc = 0;
+  if (!w->lok) return fail(-62, "L_data: Gen_Lval has not run");
+  if (l < 0 || l > w->mlh) return fail(-63, "L_data: l outside 0..max l");
+  if (jtype < 1 || jtype > 9) return fail(-64, " L_data: uknown itype ");
+  const double* hc = w->hl_full + (size_t)ns * ns * l;
+  double *v = (double*)malloc(sizeof(double) * ns), *u = (double*)malloc(sizeof(double) * ns),
+         *xx = (double*)malloc(sizeof(double) * ns * ns);
+  for (int q = 0; q < nc && !rc; q++) {
+    int i = ipt ? ipt[q] - 1 : q;
+    double c = cd[i];
+    if (fabs(c) < eps_c) continue;
+    if (jtype == 1) {
+      int i1 = K1[i] / IBI, i2 = K1[i] % IBI;
+      if (i1 < 1 || i1 > w->nbf || i2 < 1 || i2 > w->nbf) { rc = fail(-30, "orbital index out of range"); break; }
+      double C = w->lint[(i1 - 1) + (size_t)w->nbf * (i2 - 1)] * c;
+      int ic = -K3[i], jc = -K4[i];
+      if (ic > 0 && jc > 0) {
+        rc = update_hb(w, ic, jc, C);
+      } else if (ic < 0 && jc > 0) {
+        int io = -ic, j1 = io / IBO, j2 = io % IBO;
+        if (!(rc = get_v(w, j1, j2, v))) rc = update_hv(w, w->iech[j1 - 1], jc, v, C);
+      } else if (ic < 0 && jc < 0) {
+        int io = -ic, a1 = io / IBO, a2 = io % IBO, jo = -jc, b1 = jo / IBO, b2 = jo % IBO;
+        if (!(rc = get_v(w, a1, a2, v)) && !(rc = get_v(w, b1, b2, u))) {
+          for (int r = 0; r < ns; r++) v[r] *= C;
+          rc = update_hw(w, w->iech[a1 - 1], w->iech[b1 - 1], v, u);
+        }
+      } else if (ic < 0 && jc == 0) {
+        int io = -ic, a1 = io / IBO, a2 = io % IBO;
+        if (a1 < 1 || a1 > w->nbf || a2 < 1 || a2 > w->nbf) { rc = fail(-30, "orbital index out of range"); break; }
+        int ich = w->iech[a1 - 1], jch = w->iech[a2 - 1];
+        if (iitar > 0 && ich != jch) rc = orc_wave_update_hl(w, ich, jch, C);
+        if (!rc) rc = target_h(w, ich, jch, C, 0.0);
+      } else {
+        rc = fail(-65, " L_data: unknown structure ");
+      }
+    } else if (jtype <= 5) {
+      int o = K2[i], ich = K3[i], io = K4[i];
+      if (o < 1 || o > w->nbf) { rc = fail(-30, "orbital index out of range"); break; }
+      const double* lv = w->lvec + (size_t)ns * (o - 1);
+      if (io > 0) {
+        int j1 = io / IBO, j2 = io % IBO;
+        if (!(rc = get_v(w, j1, j2, u))) {
+          for (int r = 0; r < ns; r++) u[r] *= c;
+          rc = update_hw(w, ich, w->iech[j1 - 1], lv, u);
+        }
+      } else if (io < 0) {
+        rc = update_hv(w, ich, -io, lv, c);
+      } else {
+        rc = fail(-66, " L_data: uknown structure for itype 2-5 ");
+      }
+    } else {
+      int ich = K1[i] / IBI, jch = K1[i] % IBI;
+      if (ich == jch || iitar > 1) {
+        for (size_t e = 0; e < (size_t)ns * ns; e++) xx[e] = c * hc[e];
+        rc = update_hx(w, ich, jch, xx, 'x');
+      } else {
+        rc = target_h(w, ich, jch, 0.0, c);
+      }
+    }
+  }
+  free(v); free(u); free(xx);
+  return rc;
+}

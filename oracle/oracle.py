@@ -77,6 +77,13 @@ def lib():
             "orc_wave_update_hv": (i, [vp, i, i, _dp, d]),
             "orc_wave_update_hb": (i, [vp, i, i, d]),
             "orc_wave_update_cf": (i, [vp, i, i, i, d]),
+            "orc_grid_bspd": (None, [vp, _dp, _dp]),
+            "orc_db2": (None, [vp, _dp]),
+            "orc_zcb": (d, [i, i, i]),
+            "orc_wave_gen_lval": (i, [vp, d, i]),
+            "orc_wave_lval_mlh": (i, [vp]),
+            "orc_wave_lval_get": (None, [vp, _dp, _dp, _dp]),
+            "orc_wave_ldata": (i, [vp, i, i, i, _ip, _ip, _ip, _ip, _ip, _dp, i, d]),
             "orc_gauleg": (None, [d, d, _dp, _dp, i]),
             "orc_dbs_new": (vp, [i, i, i, i, _dp]),
             "orc_dbs_free": (None, [vp]),
@@ -131,6 +138,11 @@ def mkgrid0(ks, z, h, hmax, rmax):
     ns, ml, me, hh = C.c_int(), C.c_int(), C.c_int(), C.c_double()
     _check(lib().orc_mkgrid0(ks, z, h, hmax, rmax, cap, t, ns, ml, me, hh))
     return t[: ns.value + ks].copy(), ns.value, ml.value, me.value, hh.value
+
+
+def zcb(k1, k2, k3):
+    """(3j symbol (k1 k2 k3; 0 0 0))^2, the reference's ZCB."""
+    return lib().orc_zcb(k1, k2, k3)
 
 
 def gauss(k):
@@ -295,6 +307,22 @@ class Grid:
         lib().orc_mrm(self.ptr, m, rm)
         return rm
 
+    def bspd(self):
+        """First and second derivatives of the B-splines at the Gauss
+        points, [iv, m, ith] like bsp."""
+        ks, nv = self.ks, self.nv
+        d1 = np.zeros(nv * ks * ks)
+        d2 = np.zeros(nv * ks * ks)
+        lib().orc_grid_bspd(self.ptr, d1, d2)
+        return d1.reshape(nv, ks, ks), d2.reshape(nv, ks, ks)
+
+    def db2(self):
+        """<B_i|B''_j> in lower band storage (mdb2), slot (1,1) holding
+        <B_{ns-1}|B''_ns>."""
+        out = np.zeros((self.ns, self.ks), order="F")
+        lib().orc_db2(self.ptr, out)
+        return out
+
     def moments(self, k):
         rkm = np.zeros(self.ik * self.nv, order="F")
         lib().orc_moments(self.ptr, k, rkm)
@@ -426,6 +454,30 @@ class Wave:
         ipt = np.arange(1, n + 1, dtype=np.int32)
         _check(lib().orc_wave_odata(self.ptr, n, ipt, _f(lst.k3, np.int32), _f(lst.k4, np.int32),
                                     _f(lst.c)))
+
+    def gen_lval(self, z, kclosd):
+        """Gen_Lval / INT_core (l_core.f90:92-257), non-relativistic."""
+        _check(lib().orc_wave_gen_lval(self.ptr, float(z), int(kclosd)))
+
+    def lval(self):
+        """(mlh, hl_full (ns,ns,mlh+1), L_int (nbf,nbf), L_vec (ns,nbf))."""
+        j = self.job
+        mlh = lib().orc_wave_lval_mlh(self.ptr)
+        if mlh < 0:
+            raise OracleError("Gen_Lval has not run")
+        hl = np.zeros((j.ns, j.ns, mlh + 1), order="F")
+        li = np.zeros((j.nbf, j.nbf), order="F")
+        lv = np.zeros((j.ns, j.nbf), order="F")
+        lib().orc_wave_lval_get(self.ptr, hl, li, lv)
+        return mlh, hl, li, lv
+
+    def ldata(self, jtype, l, lst, iitar=0, eps_c=0.0):
+        """L_data(jtype, l) (L_data.f90:2-149)."""
+        n = len(lst.c)
+        ipt = np.arange(1, n + 1, dtype=np.int32)
+        _check(lib().orc_wave_ldata(self.ptr, jtype, l, n, ipt, _f(lst.k1, np.int32), _f(lst.k2, np.int32),
+                                    _f(lst.k3, np.int32), _f(lst.k4, np.int32), _f(lst.c), int(iitar),
+                                    float(eps_c)))
 
     def symmetrize_diag(self):
         lib().orc_wave_symmetrize_diag(self.ptr)

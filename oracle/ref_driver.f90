@@ -298,3 +298,61 @@ subroutine ref_get_matrix(hcco, hcbo, hbbo, acfo) bind(C, name="ref_get_matrix")
   end if
   acfo(1:size(acf)) = reshape(acf, [size(acf)])
 end subroutine
+
+!----------------------------------------------------------------------
+! One-electron operator with the core: Gen_Lval / INT_core of
+! src/bsr/BSR_MAT3/l_core.f90.  db2 is handed in: the reference computes
+! it in mdb2, an internal procedure of define_spline (which factorises sb
+! with LAPACK), so it cannot be called on its own; the tests pin db2
+! separately (exact integrals).
+!----------------------------------------------------------------------
+subroutine ref_bspd(d1o, d2o) bind(C, name="ref_bspd")
+  use iso_c_binding
+  use spline_param
+  use spline_grid
+  implicit none
+  real(c_double) :: d1o(nv,ks,ks), d2o(nv,ks,ks)
+  d1o = bspd(1:nv,:,:,1); d2o = bspd(1:nv,:,:,2)
+end subroutine
+
+subroutine ref_galerkin(rm1o, rm2o) bind(C, name="ref_galerkin")
+  use iso_c_binding
+  use spline_param
+  use spline_galerkin
+  implicit none
+  real(c_double) :: rm1o(ns,ks), rm2o(ns,ks)
+  rm1o = rm1; rm2o = rm2
+end subroutine
+
+subroutine ref_set_lcore(z_, kclosd_, nbf_, lbs_, iech_, db2in) bind(C, name="ref_set_lcore")
+  use iso_c_binding
+  use spline_param
+  use spline_atomic
+  use spline_galerkin
+  use spline_orbitals
+  use bsr_mat, only: nud
+  implicit none
+  real(c_double), value :: z_
+  integer(c_int), value :: kclosd_, nbf_
+  integer(c_int) :: lbs_(nbf_), iech_(nbf_)
+  real(c_double) :: db2in(ns,ks)
+  z = z_; kclosd = kclosd_; rel = .false.
+  lbs(1:nbf_) = lbs_; iech(1:nbf_) = iech_; nbs(1:nbf_) = 0
+  db2 = db2in
+  nud = 0
+end subroutine
+
+subroutine ref_gen_lval(mlh_o, hlo, lio, lvo) bind(C, name="ref_gen_lval")
+  use iso_c_binding
+  use spline_param
+  use spline_orbitals
+  use L_core
+  implicit none
+  integer(c_int) :: mlh_o
+  real(c_double) :: hlo(ns,ns,0:*), lio(nbf,nbf), lvo(ns,nbf)
+  call Alloc_Lcore(0,0,0)
+  call Gen_Lval
+  mlh_o = mlh
+  hlo(:,:,0:mlh) = hl_full(:,:,0:mlh)
+  lio = L_int; lvo = L_vec
+end subroutine

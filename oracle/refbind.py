@@ -61,6 +61,10 @@ def lib():
             "ref_update_hb": (None, [i, i, d]),
             "ref_update_cf": (None, [i, i, i, d]),
             "ref_get_matrix": (None, [_dp, _dp, _dp, _dp]),
+            "ref_bspd": (None, [_dp, _dp]),
+            "ref_galerkin": (None, [_dp, _dp]),
+            "ref_set_lcore": (None, [d, i, i, _ip, _ip, _dp]),
+            "ref_gen_lval": (None, [C.POINTER(i), _dp, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -139,6 +143,35 @@ class RefGrid:
 
     def quadr(self, i, j, m):
         return lib().ref_quadr(i, j, m)
+
+    def bspd(self):
+        ks, nv = self.ks, self.nv
+        d1 = np.zeros((nv, ks, ks), order="F")
+        d2 = np.zeros((nv, ks, ks), order="F")
+        lib().ref_bspd(d1, d2)
+        return d1, d2
+
+    def rm(self):
+        """rm1 = <B_i|1/r|B_j>, rm2 = <B_i|1/r^2|B_j> (lower band)."""
+        rm1 = np.zeros((self.ns, self.ks), order="F")
+        rm2 = np.zeros((self.ns, self.ks), order="F")
+        lib().ref_galerkin(rm1, rm2)
+        return rm1, rm2
+
+    def gen_lval(self, pbs, lbs, iech, z, kclosd, db2):
+        """The reference's Gen_Lval (l_core.f90:92-170) on orbitals pbs
+        with the given db2: (mlh, hl_full, L_int, L_vec)."""
+        nbf = pbs.shape[1]
+        self.set_orbitals(pbs)
+        lib().ref_set_lcore(float(z), int(kclosd), nbf, np.ascontiguousarray(lbs, dtype=np.int32),
+                            np.ascontiguousarray(iech, dtype=np.int32), _f(db2))
+        ns = self.ns
+        mlh = C.c_int()
+        hl = np.zeros((ns, ns, int(max(lbs)) + 1), order="F")
+        li = np.zeros((nbf, nbf), order="F")
+        lv = np.zeros((ns, nbf), order="F")
+        lib().ref_gen_lval(mlh, hl, li, lv)
+        return mlh.value, hl, li, lv
 
 
 def density(ns, ks, p1, p2, typ):
