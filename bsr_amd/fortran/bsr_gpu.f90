@@ -9,6 +9,8 @@
 !    gpu_i_data(jtype, jpol, ...)  I_data(jtype,jpol) src/bsr/BSR_MAT3/I_data.f90:2
 !    gpu_o_data(...)               O_data             src/bsr/BSR_MAT3/O_data.f90:2
 !    gpu_update_hl(ich, jch, c)    UPDATE_HL(...,sb,c) src/bsr/BSR_MAT3/mod_intmat.f90:132
+!    gpu_gen_lval(z, kclosd)       Gen_Lval + INT_core src/bsr/BSR_MAT3/l_core.f90:92
+!    gpu_l_data(jtype, l, ...)     L_data(jtype,l)    src/bsr/BSR_MAT3/L_data.f90:2
 !  and, for Dirac partial waves (DBSR_MAT3):
 !    gpu_dbs_grid(ks,ns,ksp,ksq,t) DBS_grid / DBS_gauss module state
 !    gpu_mrk_dirac(family, k, rkb) mrk_pppp/qqqq/pqpq/qpqp(k)  src/libs/DBS/mrk_pppp.f90:2
@@ -92,6 +94,21 @@ module bsr_gpu
       type(c_ptr), value :: ctx
       integer(c_int), value :: n
       integer(c_int) :: ipt(*), k3(*), k4(*)
+      real(c_double) :: c(*)
+      real(c_double), value :: eps_c
+    end function
+    integer(c_int) function bsrg_gen_lval(ctx, z, kclosd) bind(C, name="bsrg_gen_lval")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: z
+      integer(c_int), value :: kclosd
+    end function
+    integer(c_int) function bsrg_ldata(ctx, jtype, l, n, ipt, k1, k2, k3, k4, c, iitar, eps_c) &
+        bind(C, name="bsrg_ldata")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: jtype, l, n, iitar
+      integer(c_int) :: ipt(*), k1(*), k2(*), k3(*), k4(*)
       real(c_double) :: c(*)
       real(c_double), value :: eps_c
     end function
@@ -264,6 +281,23 @@ contains
     real(c_double), intent(in) :: cdata(*)
     real(8), intent(in) :: eps_c
     call gpu_check(bsrg_odata(gpu_ctx, ncdata, ipt, k3, k4, cdata, eps_c))
+  end subroutine
+
+  ! Gen_Lval (l_core.f90:92-170): the one-electron operator with the core
+  ! orbitals 1..kclosd, before the L-integral lists of a partial wave
+  subroutine gpu_gen_lval(z, kclosd)
+    real(8), intent(in) :: z
+    integer, intent(in) :: kclosd
+    call gpu_check(bsrg_gen_lval(gpu_ctx, z, kclosd))
+  end subroutine
+
+  ! L_data(jtype,l) (L_data.f90:2-149) for the records of module cmdata
+  subroutine gpu_l_data(jtype, l, ncdata, ipt, k1, k2, k3, k4, cdata, iitar, eps_c)
+    integer, intent(in) :: jtype, l, ncdata, iitar
+    integer(c_int), intent(in) :: ipt(*), k1(*), k2(*), k3(*), k4(*)
+    real(c_double), intent(in) :: cdata(*)
+    real(8), intent(in) :: eps_c
+    call gpu_check(bsrg_ldata(gpu_ctx, jtype, l, ncdata, ipt, k1, k2, k3, k4, cdata, iitar, eps_c))
   end subroutine
 
   ! ---- Dirac partial waves ----------------------------------------------
