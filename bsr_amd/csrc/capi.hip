@@ -494,8 +494,9 @@ int upload_arena(bsrg_ctx* c, const Matrices& M, Staged& st) {
     HIPCHK(st.d.resize(st.bytes));
     HIPCHK(hipMemcpyAsync(st.d.p, st.h.p, st.bytes, hipMemcpyHostToDevice, s));
   }
-  HIPCHK(c->bsum.resize(M.nblk() * c->G.ns * c->G.ks));
-  HIPCHK(c->rfac.resize(std::max<size_t>(st.nr1, 1)));
+  const bool isS = &M == &c->S;
+  HIPCHK((isS ? c->bsumS : c->bsum).resize(M.nblk() * c->G.ns * c->G.ks));
+  HIPCHK((isS ? c->rfacS : c->rfac).resize(std::max<size_t>(st.nr1, 1)));
   HIPCHK(hipEventRecord(st.done, s));
   st.pending = true;
   return 0;
@@ -607,8 +608,9 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, Staged& st, bool overwrite, hipEvent
   const bool skip = !add && st.has_live && !c->capturing;
   launch_assemble(ns, ks, st.nblk, add ? 1 : 0, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
                   st.dp<double>(st.o_bc), c->bands.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
-                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p, c->bsum.p, c->rfac.p, s,
-                  nullptr, 0, 0, skip ? st.dp<unsigned char>(st.o_live) : nullptr);
+                  st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p,
+                  (&M == &c->S ? c->bsumS : c->bsum).p, (&M == &c->S ? c->rfacS : c->rfac).p, s, nullptr, 0, 0,
+                  skip ? st.dp<unsigned char>(st.o_live) : nullptr);
   launch_vec_assemble(ns, st.nq, overwrite ? 0 : 1, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid),
                       st.dp<int>(st.o_vs), st.dp<double>(st.o_vc), c->St.p, c->VT.p, M.hcb.p, s);
   HIPCHK(hipGetLastError());
@@ -640,6 +642,14 @@ void update_live(bsrg_ctx* c, Matrices& M, Staged& st) {
   const bool unknown = st.has_gemm || c->capturing;
   for (int b = 0; b < st.nblk; ++b)
     M.live[b] = unknown || bp[b + 1] > bp[b] || rp[b + 1] > rp[b] ? 1 : 0;
+}
+
+// The term pass of S (overwriting) and its diagonal symmetrisation.
+int enqueue_s(bsrg_ctx* c) {
+  if (int rc = enqueue_terms(c, c->S, c->stS, true, nullptr, nullptr)) return rc;
+  update_live(c, c->S, c->stS);
+  launch_symmetrize_diag(c->G.ns, c->S.nch, c->S.hcc.p, c->stream);
+  return 0;
 }
 
 // Host scalars (HB, Target_h, ACF) of the parts in list order; needs
@@ -1472,23 +1482,36 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   // H: ec * S + R^k lists + (E_t - ec) * sb
   for (int ich = 1; ich <= nch; ++ich) band_hl(TH, ich, ich, ec);
   if (int rc = add_odata(c, c->H, TH, novl, nullptr, ok3, ok4, ocd, eps_c, ec, false)) return rc;
+  c->partsS.assign(1, &TS);
+  c->s_enqueued = false;
+  if (c->early_gemm) {
+    // a one-shot wave: S is complete, and its term pass runs behind the
+    // bases while the R^k lists are decoded
+    if (int rc = stage_terms(c->partsS, c->stS, static_cast<int>(c->S.nblk()), c->S.nch * c->S.ncp, nullptr,
+                            &c->S.live))
+      return rc;
+    if (int rc = upload_terms(c, c->S, TS, c->stS)) return rc;
+    if (int rc = enqueue_s(c)) return rc;
+    c->s_enqueued = true;
+  }
   const auto hb = clk::now();
   c->gemm_enqueued = false;
   if (int rc = idata_parallel(c, nrec, jtype, kpol, k1, k2, k3, k4, cd, any_exch)) return rc;
   for (int ich = 1; ich <= nch; ++ich) band_hl(THt, ich, ich, etarg[iptar[ich - 1] - 1] - ec);
-  c->partsS.assign(1, &TS);
   c->partsH.assign(1, &TH);
   for (int t = 0; t < c->nwork; ++t) c->partsH.push_back(c->work[t].get());
   c->partsH.push_back(&THt);
   const auto h1 = clk::now();
-  if (int rc = stage_terms(c->partsS, c->stS, static_cast<int>(c->S.nblk()), c->S.nch * c->S.ncp, nullptr,
-                          &c->S.live))
-    return rc;
+  if (!c->s_enqueued)
+    if (int rc = stage_terms(c->partsS, c->stS, static_cast<int>(c->S.nblk()), c->S.nch * c->S.ncp, nullptr,
+                            &c->S.live))
+      return rc;
   if (int rc = stage_terms(c->partsH, c->stH, static_cast<int>(c->H.nblk()), c->H.nch * c->H.ncp, c->pool.get(),
                           &c->H.live))
     return rc;
   const auto h2 = clk::now();
-  if (int rc = upload_terms(c, c->S, TS, c->stS)) return rc;
+  if (!c->s_enqueued)
+    if (int rc = upload_terms(c, c->S, TS, c->stS)) return rc;
   if (c->early_gemm) {
     if (int rc = upload_arena(c, c->H, c->stH)) return rc;
   } else {
@@ -1511,9 +1534,9 @@ int bsrg_wave_launch(bsrg_ctx* c) {
   if (!c->wave_prepared) return fail(-8, "no prepared wave (bsrg_wave_prepare)");
   const bool timed = !c->capturing;  // no event records inside a graph capture
   if (timed) (void)hipEventRecord(c->ev[2], c->stream);
-  if (int rc = enqueue_terms(c, c->S, c->stS, true, nullptr, nullptr)) return rc;
-  update_live(c, c->S, c->stS);
-  launch_symmetrize_diag(c->G.ns, c->S.nch, c->S.hcc.p, c->stream);
+  if (!c->s_enqueued)
+    if (int rc = enqueue_s(c)) return rc;
+  c->s_enqueued = false;
   if (int rc = enqueue_terms(c, c->H, c->stH, true, timed ? c->ev[3] : nullptr, timed ? c->ev[4] : nullptr,
                              c->gemm_enqueued))
     return rc;

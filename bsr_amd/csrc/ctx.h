@@ -356,6 +356,7 @@ struct bsrg_ctx {
   // one-shot waves (bsrg_wave_assemble) enqueue the H exchange GEMM during
   // the prepare, before the terms are staged
   bool early_gemm = false, gemm_enqueued = false, gemm_was_early = false;
+  bool s_enqueued = false;  // the S pass of a one-shot wave was enqueued during the prepare
   int wave_ntarg = 0;
 
   // staged term uploads: S and H of the fused wave, and the per-call path
@@ -376,6 +377,7 @@ struct bsrg_ctx {
   int last_engine = -1, forced_engine = -1;
   DevVec<double> dtmp, dtmp2;
   DevVec<double> bsum, rfac;  // term-pass scratch: summed band of each block, factor of each rank-1 term
+  DevVec<double> bsumS, rfacS;  // the same for S, whose pass may be queued while H's scratch grows
 
   // Dirac B-spline grid and the resident bands of the four families
   struct Dbs {
