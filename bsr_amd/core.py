@@ -211,6 +211,11 @@ class Context:
         L.check(L.lib().bsrg_ldata(self._p, jtype, l, n, i32(ipt), i32(lst.k1), i32(lst.k2), i32(lst.k3),
                                    i32(lst.k4), f64(lst.c), int(iitar), float(eps_c)))
 
+    def bs_orth(self, which=0):
+        """BS_ORTH (bs_orth.f90:2-92): which = 0 the matrix of
+        allocate_matrix, 1 the H of the last wave_assemble."""
+        L.check(L.lib().bsrg_bs_orth(self._p, int(which)))
+
     def symmetrize_diag(self):
         L.check(L.lib().bsrg_symmetrize_diag(self._p))
 

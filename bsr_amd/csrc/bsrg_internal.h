@@ -100,6 +100,12 @@ void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, co
 void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,
                          const double* c, const double* stab, const double* V, double* hcb, hipStream_t s);
 void launch_symmetrize_diag(int ns, int nch, double* hcc, hipStream_t s);
+// BS_ORTH on all channel blocks (nblk = nch(nch+1)/2) and channel-perturber
+// vectors: channel c is projected with the nz[c] columns of Z (and BZ = B Z)
+// starting at column zoff[c]; nzmax = max nz.  Returns -1 when nzmax is
+// outside 1..16, -2 when its LDS would not fit.
+int launch_bs_orth(int ns, int nblk, int nch, int ncp, int nzmax, const int* zoff, const int* nz, const double* Z,
+                   const double* BZ, double* hcc, double* hcb, hipStream_t s);
 // One-electron operator of one l (Gen_Lval/INT_core): out (ns x ns) =
 // sum_t ec[t] E(eid[t]) + the band hl (lower storage) + sum_u dc[u]
 // bands(did[u]) (upper storage), the band region taken from the lower

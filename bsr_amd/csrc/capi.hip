@@ -1402,6 +1402,60 @@ int bsrg_ldata(bsrg_ctx* c, int jtype, int l, int n, const int* ipt, const int* 
   return 0;
 }
 
+int bsrg_bs_orth(bsrg_ctx* c, int which) {
+  if (int rc = check_ctx(c)) return rc;
+  if (which != 0 && which != 1) return fail(-1, "bsrg_bs_orth: which must be 0 (current matrix) or 1 (the wave's H)");
+  if (!c->orb_ok) return fail(-4, "orbitals not set (bsrg_set_orbitals)");
+  if (which == 0) {
+    if (int rc = cur_ready(c)) return rc;
+  } else if (!c->wave_ok) {
+    return fail(-8, "no assembled wave (bsrg_wave_assemble)");
+  }
+  Matrices& M = which == 0 ? c->cur : c->H;
+  const int ns = c->G.ns, ks = c->G.ks, nbf = c->nbf, nch = M.nch;
+  HIPCHK(hipSetDevice(c->device));
+  // per channel: the bound orbitals of the channel orbital's l that IBORT
+  // does not exempt (bs_orth.f90:40-55)
+  std::vector<int> zoff(nch, 0), nz(nch, 0), cols;
+  int nzmax = 0;
+  for (int ich = 1; ich <= nch; ++ich) {
+    int i = 0;
+    for (int o = 1; o <= nbf && !i; ++o)
+      if (c->iech[o - 1] == ich) i = o;
+    if (!i) return fail(-70, "BS_ORTH: channel without a channel orbital");
+    zoff[ich - 1] = static_cast<int>(cols.size());
+    for (int j = 1; j <= nbf; ++j) {
+      if (c->iech[j - 1] > 0 || c->lbs[i - 1] != c->lbs[j - 1]) continue;
+      if (c->ibort[(std::max(i, j) - 1) + static_cast<size_t>(nbf) * (std::min(i, j) - 1)] != 0) continue;
+      cols.push_back(j);
+    }
+    nz[ich - 1] = static_cast<int>(cols.size()) - zoff[ich - 1];
+    nzmax = std::max(nzmax, nz[ich - 1]);
+  }
+  if (nzmax == 0) return 0;
+  if (nzmax > 16) return fail(-71, "BS_ORTH: more than 16 orthogonality conditions for one channel");
+  const size_t nc = cols.size();
+  std::vector<double> zb(2 * nc * ns);  // Z then B Z, one column per condition
+  for (size_t q = 0; q < nc; ++q) {
+    const double* p = &c->pbs[static_cast<size_t>(ns) * (cols[q] - 1)];
+    std::copy(p, p + ns, &zb[q * ns]);
+    bxv_sym(ks, ns, c->G.sb.data(), p, &zb[(nc + q) * ns]);
+  }
+  std::vector<int> ints(zoff);
+  ints.insert(ints.end(), nz.begin(), nz.end());
+  hipStream_t s = c->stream;
+  HIPCHK(c->lwork.resize(zb.size()));
+  HIPCHK(c->lwork_i.resize(ints.size()));
+  HIPCHK(hipMemcpyAsync(c->lwork.p, zb.data(), zb.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->lwork_i.p, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  const int lrc = launch_bs_orth(ns, static_cast<int>(M.nblk()), nch, M.ncp, nzmax, c->lwork_i.p, c->lwork_i.p + nch,
+                                 c->lwork.p, c->lwork.p + nc * ns, M.hcc.p, M.hcb.p, s);
+  if (lrc) return fail(-71, "BS_ORTH: the projectors of one channel do not fit the kernel's local memory");
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
 int bsrg_symmetrize_diag(bsrg_ctx* c) {
   if (int rc = check_ctx(c)) return rc;
   if (int rc = cur_ready(c)) return rc;

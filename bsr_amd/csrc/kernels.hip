@@ -1047,6 +1047,147 @@ __global__ __launch_bounds__(256) void k_dense_terms(int ns, const int* __restri
   h[e] = v;
 }
 
+// ---- BS_ORTH (src/bsr/BSR_MAT3/bs_orth.f90:2-92) -------------------------
+// Channel block (i, j), i >= j: H <- (1 - B Z_i Z_i^T) H (1 - Z_j Z_j^T B)
+// = H - BZ_i (T - U) - V BZ_j^T, with T = Z_i^T H, V = H Z_j,
+// U = (Z_i^T V) BZ_j^T; Z_c holds the nz[c] bound orbitals channel c must be
+// orthogonal to (columns zoff[c].. of Z, and BZ = B Z), nz <= kOrthMax.
+// One workgroup of 4 waves per block.  Pass 1: wave w takes the 64-row
+// chunks w, w+4, ... of the block and walks its columns (coalesced reads):
+// V rows of the chunk in registers, partial T (lane sums reduced across the
+// wave) per wave in LDS, summed over the waves in a fixed order.  Pass 2
+// writes the block element by element.
+// LDS (doubles): Tp[4][nz][ns], V[nz][ns], TU[nz][ns], W[nz][nz].
+constexpr int kOrthMax = 16;
+
+__device__ inline void blk_pair(int blk, int& i, int& j) {
+  int r = static_cast<int>((sqrt(8.0 * blk + 1.0) + 1.0) * 0.5);
+  while (r * (r - 1) / 2 > blk) --r;
+  while ((r + 1) * r / 2 <= blk) ++r;
+  i = r;
+  j = blk - r * (r - 1) / 2 + 1;
+}
+
+__global__ __launch_bounds__(256) void k_bs_orth(int ns, int nzmax, const int* __restrict__ zoff,
+                                                 const int* __restrict__ nz, const double* __restrict__ Z,
+                                                 const double* __restrict__ BZ, double* __restrict__ hcc) {
+  extern __shared__ double lds[];
+  const int blk = blockIdx.x;
+  int ci, cj;
+  blk_pair(blk, ci, cj);  // 1-based channels, ci >= cj
+  const int ni = nz[ci - 1], nj = nz[cj - 1];
+  if (ni == 0 && nj == 0) return;
+  const double* Zi = Z + static_cast<size_t>(zoff[ci - 1]) * ns;
+  const double* Zj = Z + static_cast<size_t>(zoff[cj - 1]) * ns;
+  const double* BZi = BZ + static_cast<size_t>(zoff[ci - 1]) * ns;
+  const double* BZj = BZ + static_cast<size_t>(zoff[cj - 1]) * ns;
+  double* Tp = lds;                                         // [4][nzmax][ns]
+  double* V = Tp + 4 * static_cast<size_t>(nzmax) * ns;     // [nzmax][ns]
+  double* TU = V + static_cast<size_t>(nzmax) * ns;         // [nzmax][ns]
+  double* W = TU + static_cast<size_t>(nzmax) * ns;         // [nzmax][nzmax]
+  double* h = hcc + static_cast<size_t>(blk) * ns * ns;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int q = threadIdx.x; q < 4 * nzmax * ns; q += 256) Tp[q] = 0.0;
+  __syncthreads();
+  for (int x0 = 64 * wv; x0 < ns; x0 += 256) {
+    const int x = x0 + lane;
+    const bool in = x < ns;
+    double zi[kOrthMax], vacc[kOrthMax];
+#pragma unroll
+    for (int a = 0; a < kOrthMax; ++a) {
+      zi[a] = (a < ni && in) ? Zi[static_cast<size_t>(a) * ns + x] : 0.0;
+      vacc[a] = 0.0;
+    }
+    for (int y = 0; y < ns; ++y) {
+      const double hv = in ? h[x + static_cast<size_t>(ns) * y] : 0.0;
+#pragma unroll
+      for (int b = 0; b < kOrthMax; ++b)
+        if (b < nj) vacc[b] += hv * Zj[static_cast<size_t>(b) * ns + y];
+#pragma unroll
+      for (int a = 0; a < kOrthMax; ++a) {
+        if (a >= ni) break;
+        double t = zi[a] * hv;
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if (lane == 0) Tp[(static_cast<size_t>(wv) * nzmax + a) * ns + y] += t;
+      }
+    }
+    if (in)
+      for (int b = 0; b < nj; ++b) V[static_cast<size_t>(b) * ns + x] = vacc[b];
+  }
+  __syncthreads();
+  // T = sum over the waves (fixed order); W = Z_i^T V
+  for (int q = threadIdx.x; q < ni * ns; q += 256) {
+    const int a = q / ns, y = q % ns;
+    double t = 0.0;
+    for (int w = 0; w < 4; ++w) t += Tp[(static_cast<size_t>(w) * nzmax + a) * ns + y];
+    TU[static_cast<size_t>(a) * ns + y] = t;
+  }
+  if (threadIdx.x < ni * nj) {
+    const int a = threadIdx.x / nj, b = threadIdx.x % nj;
+    double s = 0.0;
+    for (int x = 0; x < ns; ++x) s += Zi[static_cast<size_t>(a) * ns + x] * V[static_cast<size_t>(b) * ns + x];
+    W[a * nzmax + b] = s;
+  }
+  __syncthreads();
+  // TU = T - W BZ_j^T
+  for (int q = threadIdx.x; q < ni * ns; q += 256) {
+    const int a = q / ns, y = q % ns;
+    double u = 0.0;
+    for (int b = 0; b < nj; ++b) u += W[a * nzmax + b] * BZj[static_cast<size_t>(b) * ns + y];
+    TU[static_cast<size_t>(a) * ns + y] -= u;
+  }
+  __syncthreads();
+  const size_t nss = static_cast<size_t>(ns) * ns;
+  for (size_t e = threadIdx.x; e < nss; e += 256) {
+    const int x = static_cast<int>(e % ns), y = static_cast<int>(e / ns);
+    double v = h[e];
+    for (int a = 0; a < ni; ++a) v -= BZi[static_cast<size_t>(a) * ns + x] * TU[static_cast<size_t>(a) * ns + y];
+    for (int b = 0; b < nj; ++b) v -= V[static_cast<size_t>(b) * ns + x] * BZj[static_cast<size_t>(b) * ns + y];
+    h[e] = v;
+  }
+}
+
+// Channel-perturber vectors: hcb(:, ich, ip) <- (1 - B Z Z^T) v, one wave
+// per vector.  grid (ceil(nch*ncp/4)), block 256.
+__global__ __launch_bounds__(256) void k_bs_orth_vec(int ns, int nch, int nq, const int* __restrict__ zoff,
+                                                     const int* __restrict__ nz, const double* __restrict__ Z,
+                                                     const double* __restrict__ BZ, double* __restrict__ hcb) {
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (q >= nq) return;
+  const int ich = q % nch, n = nz[ich];
+  if (n == 0) return;
+  double* v = hcb + static_cast<size_t>(q) * ns;
+  const double* Zc = Z + static_cast<size_t>(zoff[ich]) * ns;
+  const double* BZc = BZ + static_cast<size_t>(zoff[ich]) * ns;
+  double s[kOrthMax];
+  for (int a = 0; a < n; ++a) {
+    double t = 0.0;
+    for (int x = lane; x < ns; x += 64) t += Zc[static_cast<size_t>(a) * ns + x] * v[x];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    s[a] = t;
+  }
+  for (int x = lane; x < ns; x += 64) {
+    double u = v[x];
+    for (int a = 0; a < n; ++a) u -= BZc[static_cast<size_t>(a) * ns + x] * s[a];
+    v[x] = u;
+  }
+}
+
+size_t bs_orth_lds_bytes(int ns, int nzmax) {
+  return (6 * static_cast<size_t>(nzmax) * ns + static_cast<size_t>(nzmax) * nzmax) * sizeof(double);
+}
+
+int launch_bs_orth(int ns, int nblk, int nch, int ncp, int nzmax, const int* zoff, const int* nz, const double* Z,
+                   const double* BZ, double* hcc, double* hcb, hipStream_t s) {
+  if (nzmax < 1 || nzmax > kOrthMax) return -1;
+  const size_t lds = bs_orth_lds_bytes(ns, nzmax);
+  if (lds > 160 * 1024) return -2;
+  if (nblk > 0) hipLaunchKernelGGL(k_bs_orth, dim3(nblk), dim3(256), lds, s, ns, nzmax, zoff, nz, Z, BZ, hcc);
+  const int nq = nch * ncp;
+  if (nq > 0) hipLaunchKernelGGL(k_bs_orth_vec, dim3((nq + 3) / 4), dim3(256), 0, s, ns, nch, nq, zoff, nz, Z, BZ, hcb);
+  return 0;
+}
+
 void launch_lcore(int ns, int ks, int nt, const int* eid, const double* ec, const double* E, int nd,
                   const int* did, const double* dc, const double* bands, const double* hl, double* out,
                   hipStream_t s) {

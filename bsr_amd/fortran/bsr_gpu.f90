@@ -11,6 +11,7 @@
 !    gpu_update_hl(ich, jch, c)    UPDATE_HL(...,sb,c) src/bsr/BSR_MAT3/mod_intmat.f90:132
 !    gpu_gen_lval(z, kclosd)       Gen_Lval + INT_core src/bsr/BSR_MAT3/l_core.f90:92
 !    gpu_l_data(jtype, l, ...)     L_data(jtype,l)    src/bsr/BSR_MAT3/L_data.f90:2
+!    bsrg_bs_orth(gpu_ctx, 0)      BS_ORTH            src/bsr/BSR_MAT3/bs_orth.f90:2
 !  and, for Dirac partial waves (DBSR_MAT3):
 !    gpu_dbs_grid(ks,ns,ksp,ksq,t) DBS_grid / DBS_gauss module state
 !    gpu_mrk_dirac(family, k, rkb) mrk_pppp/qqqq/pqpq/qpqp(k)  src/libs/DBS/mrk_pppp.f90:2
@@ -111,6 +112,11 @@ module bsr_gpu
       integer(c_int) :: ipt(*), k1(*), k2(*), k3(*), k4(*)
       real(c_double) :: c(*)
       real(c_double), value :: eps_c
+    end function
+    integer(c_int) function bsrg_bs_orth(ctx, which) bind(C, name="bsrg_bs_orth")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which
     end function
     integer(c_int) function bsrg_symmetrize_diag(ctx) bind(C, name="bsrg_symmetrize_diag")
       import :: c_ptr, c_int

@@ -207,6 +207,17 @@ int bsrg_ldata(bsrg_ctx* ctx, int jtype, int l, int ncdata, const int* ipt, cons
                const int* k2, const int* k3, const int* k4, const double* cdata, int iitar,
                double eps_c);
 
+/* Orthogonality of every channel orbital to the bound orbitals of its l
+ * that IBORT does not exempt: H(i,j) -> (1 - B zz_i) H(i,j) (1 - zz_j B)
+ * on the channel blocks and hcb(:,i,ip) -> (1 - B zz_i) hcb(:,i,ip), with
+ * zz_i = sum of p p^T over those orbitals and B the B-spline Gram matrix.
+ * Replaces BS_ORTH, src/bsr/BSR_MAT3/bs_orth.f90:2-92 (called by SUB1 after
+ * the diagonal symmetrisation, sub1.f90:195).  which = 0: the matrix of
+ * bsrg_alloc_matrix; 1: H of the last bsrg_wave_assemble.  At most 16
+ * conditions per channel; a channel without a channel orbital (iech) is an
+ * error. */
+int bsrg_bs_orth(bsrg_ctx* ctx, int which);
+
 /* Diagonal channel blocks -> (C+C^T)/2.  Replaces sub1.f90:80-84, :187-191. */
 int bsrg_symmetrize_diag(bsrg_ctx* ctx);
 

@@ -1334,3 +1334,77 @@ int orc_wave_ldata(orc_wave* w, int jtype, int l, int nc, const int* ipt, const 
   free(v); free(u); free(xx);
   return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* BS_ORTH, src/bsr/BSR_MAT3/bs_orth.f90:2-92: orthogonality of each    */
+/* channel orbital to the bound orbitals of its l that IBORT does not  */
+/* exempt, by H -> (1 - B zz') H (1 - zz' B) (Bentley).                */
+/* ------------------------------------------------------------------ */
+static void matmul_nn(int n, const double* a, const double* b, double* c) {
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) {
+      double s = 0.0;
+      for (int k = 0; k < n; k++) s += a[i + (size_t)n * k] * b[k + (size_t)n * j];
+      c[i + (size_t)n * j] = s;
+    }
+}
+
+int orc_wave_bs_orth(orc_wave* w) {
+  int ns = w->ns, ks = w->ks, nbf = w->nbf, nch = w->nch;
+  size_t nss = (size_t)ns * ns;
+  double *B = (double*)calloc(nss, sizeof(double)), *zz = (double*)malloc(sizeof(double) * nss),
+         *x = (double*)malloc(sizeof(double) * nss), *y = (double*)malloc(sizeof(double) * nss),
+         *xx = (double*)malloc(sizeof(double) * nss), *z = (double*)malloc(sizeof(double) * ns);
+  /* bb: the full Gram matrix from the lower band sb */
+  for (int i = 0; i < ns; i++)
+    for (int jp = 0; jp < ks; jp++) {
+      int j = i + jp - (ks - 1);
+      if (j < 0) continue;
+      B[i + (size_t)ns * j] = w->g->sb[i + (size_t)ns * jp];
+      B[j + (size_t)ns * i] = w->g->sb[i + (size_t)ns * jp];
+    }
+  int rc = 0;
+  for (int ich = 1; ich <= nch && !rc; ich++) {
+    int i = 0; /* ipch(ich): the channel orbital */
+    for (int o = 1; o <= nbf; o++)
+      if (w->iech[o - 1] == ich) { i = o; break; }
+    if (!i) { rc = fail(-70, "BS_ORTH: channel without a channel orbital"); break; }
+    memset(zz, 0, sizeof(double) * nss);
+    int nort = 0;
+    for (int j = 1; j <= nbf; j++) {
+      if (w->iech[j - 1] > 0) continue;
+      if (w->lbs[i - 1] != w->lbs[j - 1]) continue;
+      int a = i > j ? i : j, b = i > j ? j : i;
+      if (w->ibort[(a - 1) + (size_t)nbf * (b - 1)] != 0) continue;
+      nort++;
+      const double* p = w->pbs + (size_t)ns * (j - 1);
+      for (int ii = 0; ii < ns; ii++)
+        for (int jj = 0; jj < ns; jj++) zz[ii + (size_t)ns * jj] += p[ii] * p[jj];
+    }
+    if (nort == 0) continue;
+    matmul_nn(ns, B, zz, x);
+    matmul_nn(ns, zz, B, y);
+    for (int jch = 1; jch <= nch; jch++) {
+      double* h = w->hcc + blk(ich, jch) * nss;
+      if (ich >= jch) {
+        matmul_nn(ns, x, h, xx);
+        for (size_t q = 0; q < nss; q++) h[q] -= xx[q];
+      }
+      if (ich <= jch) {
+        matmul_nn(ns, h, y, xx);
+        for (size_t q = 0; q < nss; q++) h[q] -= xx[q];
+      }
+    }
+    for (int ip = 0; ip < w->ncp; ip++) {
+      double* v = w->hcb + (size_t)ip * ns * nch + (size_t)(ich - 1) * ns;
+      for (int c = 0; c < ns; c++) {
+        double s = 0.0;
+        for (int r = 0; r < ns; r++) s += v[r] * y[r + (size_t)ns * c];
+        z[c] = s;
+      }
+      for (int c = 0; c < ns; c++) v[c] -= z[c];
+    }
+  }
+  free(B); free(zz); free(x); free(y); free(xx); free(z);
+  return rc;
+}

@@ -84,6 +84,7 @@ def lib():
             "orc_wave_lval_mlh": (i, [vp]),
             "orc_wave_lval_get": (None, [vp, _dp, _dp, _dp]),
             "orc_wave_ldata": (i, [vp, i, i, i, _ip, _ip, _ip, _ip, _ip, _dp, i, d]),
+            "orc_wave_bs_orth": (i, [vp]),
             "orc_gauleg": (None, [d, d, _dp, _dp, i]),
             "orc_dbs_new": (vp, [i, i, i, i, _dp]),
             "orc_dbs_free": (None, [vp]),
@@ -479,6 +480,10 @@ class Wave:
                                     _f(lst.k3, np.int32), _f(lst.k4, np.int32), _f(lst.c), int(iitar),
                                     float(eps_c)))
 
+    def bs_orth(self):
+        """BS_ORTH (bs_orth.f90:2-92) on the accumulated matrix."""
+        _check(lib().orc_wave_bs_orth(self.ptr))
+
     def symmetrize_diag(self):
         lib().orc_wave_symmetrize_diag(self.ptr)
 
@@ -489,12 +494,12 @@ class Wave:
         lib().orc_wave_acf_finish(self.ptr, kcore)
 
 
-def run_wave(job, grid: Grid | None = None):
+def run_wave(job, grid: Grid | None = None, orth: bool = False):
     """One partial wave in the order of SUB1 (src/bsr/BSR_MAT3/sub1.f90:70-219):
     S = sb on the channel diagonals + O_data, symmetrised; H = EC*S + R^k
     contributions (jpol outer, jtype inner, state_res.f90:155-159) +
-    (E_t - EC)*sb, symmetrised; ACF finished.  Returns (S, H) as
-    WaveMatrices (S.htarg/acf unused)."""
+    (E_t - EC)*sb, symmetrised (then BS_ORTH when orth, sub1.f90:195); ACF
+    finished.  Returns (S, H) as WaveMatrices (S.htarg/acf unused)."""
     if grid is None:
         grid = Grid(job.ks, job.ns, job.ml, job.me, job.h, job.t)
     w = Wave(grid, job)
@@ -513,6 +518,8 @@ def run_wave(job, grid: Grid | None = None):
     for ich in range(1, job.nch + 1):
         w.update_hl(ich, ich, float(job.etarg[job.iptar[ich - 1] - 1] - job.ec))
     w.symmetrize_diag()
+    if orth:
+        w.bs_orth()
     w.acf_finish(0)
     H = w.get()
     return S, H
