@@ -21,6 +21,9 @@ Beside the headline line's required fields, the JSON line carries:
            FP64 and in split precision: p50/p99 latency at that rate, the
            saturated capacity (all jobs queued at once), and the error of
            every split-precision job against its FP64 result;
+  bs_orth  BS_ORTH (the channel orbitals' orthogonality conditions) on the
+           wave's H: time per call and the algorithmic HBM rate of the
+           blocks it rewrites;
   batch    48 uneven waves (nch ~ U[10,120], ns ~ U[100,250], k=0..8) as one
            hipGraph: total elements/s of a replay, per-wave kernel time;
   dirac    the Dirac (DBSR) R^k assembly of one C5-shaped wave: ks=9, ns=150
@@ -93,6 +96,23 @@ def copy_calibration(nbytes):
 
 def mean_dict(ds):
     return {k: round(sum(d[k] for d in ds) / len(ds), 4) for k in ds[0]}
+
+
+def _orth_conditions(w):
+    """Number of BS_ORTH conditions of each channel (bs_orth.f90:40-55):
+    bound orbitals of the channel orbital's l with IBORT = 0."""
+    import numpy as np
+    out = []
+    for ich in range(1, w.nch + 1):
+        i = int(np.nonzero(np.asarray(w.iech) == ich)[0][0]) + 1
+        n = 0
+        for j in range(1, w.nbf + 1):
+            if w.iech[j - 1] > 0 or w.lbs[j - 1] != w.lbs[i - 1]:
+                continue
+            if w.ibort[max(i, j) - 1, min(i, j) - 1] == 0:
+                n += 1
+        out.append(n)
+    return out if any(out) else []
 
 
 def run_steps(ctx, w, warmup, steps):
@@ -375,6 +395,22 @@ def main():
                 out[key]["max_eig_err_vs_fp64"] = de
                 log(f"{key}: lowest-10 eigenvalue error vs FP64 {de:.2e}")
             sctx.close()
+    # BS_ORTH on the wave's H (after the legs that compare against it): the
+    # channel orbitals' orthogonality conditions; the kernel reads every
+    # touched block twice and writes it once
+    conds = _orth_conditions(w)
+    if conds:
+        reps = 5
+        ctx.bs_orth(1)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.bs_orth(1)
+        oms = (time.perf_counter() - t0) * 1e3 / reps
+        touched = sum(1 for i in range(1, w.nch + 1) for j in range(1, i + 1) if conds[i - 1] or conds[j - 1])
+        gbs = 3 * touched * w.ns * w.ns * 8 / (oms * 1e-3) / 1e9
+        out["bs_orth"] = {"ms": oms, "channels_with_conditions": sum(1 for q in conds if q),
+                          "blocks": touched, "algorithmic_GB_s": gbs}
+        log(f"bs_orth: {oms:.3f} ms per call, {touched} blocks, {gbs:.0f} GB/s (2 reads + 1 write per block)")
     ctx.close()
     if not a.no_serve:
         log("generating served stream")

@@ -1052,12 +1052,11 @@ __global__ __launch_bounds__(256) void k_dense_terms(int ns, const int* __restri
 // = H - BZ_i (T - U) - V BZ_j^T, with T = Z_i^T H, V = H Z_j,
 // U = (Z_i^T V) BZ_j^T; Z_c holds the nz[c] bound orbitals channel c must be
 // orthogonal to (columns zoff[c].. of Z, and BZ = B Z), nz <= kOrthMax.
-// One workgroup of 4 waves per block.  Pass 1: wave w takes the 64-row
-// chunks w, w+4, ... of the block and walks its columns (coalesced reads):
-// V rows of the chunk in registers, partial T (lane sums reduced across the
-// wave) per wave in LDS, summed over the waves in a fixed order.  Pass 2
-// writes the block element by element.
-// LDS (doubles): Tp[4][nz][ns], V[nz][ns], TU[nz][ns], W[nz][nz].
+// One workgroup of 4 waves per block.  Pass 1 reads the block twice (the
+// second time mostly from cache): V with lanes over rows, T with lanes over
+// columns, each sum in index order in one register; no cross-lane sums.
+// Pass 2 writes the block element by element.
+// LDS (doubles): V[nz][ns], TU[nz][ns], W[nz][nz].
 constexpr int kOrthMax = 16;
 
 __device__ inline void blk_pair(int blk, int& i, int& j) {
@@ -1081,47 +1080,56 @@ __global__ __launch_bounds__(256) void k_bs_orth(int ns, int nzmax, const int* _
   const double* Zj = Z + static_cast<size_t>(zoff[cj - 1]) * ns;
   const double* BZi = BZ + static_cast<size_t>(zoff[ci - 1]) * ns;
   const double* BZj = BZ + static_cast<size_t>(zoff[cj - 1]) * ns;
-  double* Tp = lds;                                         // [4][nzmax][ns]
-  double* V = Tp + 4 * static_cast<size_t>(nzmax) * ns;     // [nzmax][ns]
-  double* TU = V + static_cast<size_t>(nzmax) * ns;         // [nzmax][ns]
-  double* W = TU + static_cast<size_t>(nzmax) * ns;         // [nzmax][nzmax]
+  double* V = lds;                                       // [nzmax][ns]
+  double* TU = V + static_cast<size_t>(nzmax) * ns;      // [nzmax][ns]
+  double* W = TU + static_cast<size_t>(nzmax) * ns;      // [nzmax][nzmax]
   double* h = hcc + static_cast<size_t>(blk) * ns * ns;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int q = threadIdx.x; q < 4 * nzmax * ns; q += 256) Tp[q] = 0.0;
-  __syncthreads();
-  for (int x0 = 64 * wv; x0 < ns; x0 += 256) {
-    const int x = x0 + lane;
-    const bool in = x < ns;
-    double zi[kOrthMax], vacc[kOrthMax];
+  // V = H Z_j: lanes over rows (coalesced), eight columns per round
+  if (nj > 0)
+    for (int x0 = 64 * wv; x0 < ns; x0 += 256) {
+      const int x = x0 + lane;
+      const bool in = x < ns;
+      double vacc[kOrthMax];
 #pragma unroll
-    for (int a = 0; a < kOrthMax; ++a) {
-      zi[a] = (a < ni && in) ? Zi[static_cast<size_t>(a) * ns + x] : 0.0;
-      vacc[a] = 0.0;
-    }
-    for (int y = 0; y < ns; ++y) {
-      const double hv = in ? h[x + static_cast<size_t>(ns) * y] : 0.0;
+      for (int q = 0; q < kOrthMax; ++q) vacc[q] = 0.0;
+      for (int y0 = 0; y0 < ns; y0 += 8) {
+        double hv[8];
 #pragma unroll
-      for (int b = 0; b < kOrthMax; ++b)
-        if (b < nj) vacc[b] += hv * Zj[static_cast<size_t>(b) * ns + y];
+        for (int u = 0; u < 8; ++u) hv[u] = (in && y0 + u < ns) ? h[x + static_cast<size_t>(ns) * (y0 + u)] : 0.0;
+        const int nu = min(8, ns - y0);
+        for (int u = 0; u < nu; ++u)
 #pragma unroll
-      for (int a = 0; a < kOrthMax; ++a) {
-        if (a >= ni) break;
-        double t = zi[a] * hv;
-        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-        if (lane == 0) Tp[(static_cast<size_t>(wv) * nzmax + a) * ns + y] += t;
+          for (int q = 0; q < kOrthMax; ++q)
+            if (q < nj) vacc[q] += hv[u] * Zj[static_cast<size_t>(q) * ns + y0 + u];
       }
+      if (in)
+        for (int q = 0; q < nj; ++q) V[static_cast<size_t>(q) * ns + x] = vacc[q];
     }
-    if (in)
-      for (int b = 0; b < nj; ++b) V[static_cast<size_t>(b) * ns + x] = vacc[b];
-  }
+  // T = Z_i^T H: lanes over columns (each lane walks its column; a cache
+  // line serves eight rows), eight rows per round
+  if (ni > 0)
+    for (int yc = 64 * wv; yc < ns; yc += 256) {
+      const int y = yc + lane;
+      const bool in = y < ns;
+      double tacc[kOrthMax];
+#pragma unroll
+      for (int q = 0; q < kOrthMax; ++q) tacc[q] = 0.0;
+      for (int x0 = 0; x0 < ns; x0 += 8) {
+        double hv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) hv[u] = (in && x0 + u < ns) ? h[x0 + u + static_cast<size_t>(ns) * y] : 0.0;
+        const int nu = min(8, ns - x0);
+        for (int u = 0; u < nu; ++u)
+#pragma unroll
+          for (int q = 0; q < kOrthMax; ++q)
+            if (q < ni) tacc[q] += Zi[static_cast<size_t>(q) * ns + x0 + u] * hv[u];
+      }
+      if (in)
+        for (int q = 0; q < ni; ++q) TU[static_cast<size_t>(q) * ns + y] = tacc[q];
+    }
   __syncthreads();
-  // T = sum over the waves (fixed order); W = Z_i^T V
-  for (int q = threadIdx.x; q < ni * ns; q += 256) {
-    const int a = q / ns, y = q % ns;
-    double t = 0.0;
-    for (int w = 0; w < 4; ++w) t += Tp[(static_cast<size_t>(w) * nzmax + a) * ns + y];
-    TU[static_cast<size_t>(a) * ns + y] = t;
-  }
+  // W = Z_i^T V
   if (threadIdx.x < ni * nj) {
     const int a = threadIdx.x / nj, b = threadIdx.x % nj;
     double s = 0.0;
@@ -1138,12 +1146,20 @@ __global__ __launch_bounds__(256) void k_bs_orth(int ns, int nzmax, const int* _
   }
   __syncthreads();
   const size_t nss = static_cast<size_t>(ns) * ns;
-  for (size_t e = threadIdx.x; e < nss; e += 256) {
-    const int x = static_cast<int>(e % ns), y = static_cast<int>(e / ns);
-    double v = h[e];
-    for (int a = 0; a < ni; ++a) v -= BZi[static_cast<size_t>(a) * ns + x] * TU[static_cast<size_t>(a) * ns + y];
-    for (int b = 0; b < nj; ++b) v -= V[static_cast<size_t>(b) * ns + x] * BZj[static_cast<size_t>(b) * ns + y];
-    h[e] = v;
+  for (size_t e0 = threadIdx.x; e0 < nss; e0 += 4 * 256) {
+    double hv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) hv[u] = e0 + 256 * u < nss ? h[e0 + 256 * u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t e = e0 + 256 * u;
+      if (e >= nss) break;
+      const int x = static_cast<int>(e % ns), y = static_cast<int>(e / ns);
+      double v = hv[u];
+      for (int a = 0; a < ni; ++a) v -= BZi[static_cast<size_t>(a) * ns + x] * TU[static_cast<size_t>(a) * ns + y];
+      for (int b = 0; b < nj; ++b) v -= V[static_cast<size_t>(b) * ns + x] * BZj[static_cast<size_t>(b) * ns + y];
+      h[e] = v;
+    }
   }
 }
 
@@ -1174,7 +1190,7 @@ __global__ __launch_bounds__(256) void k_bs_orth_vec(int ns, int nch, int nq, co
 }
 
 size_t bs_orth_lds_bytes(int ns, int nzmax) {
-  return (6 * static_cast<size_t>(nzmax) * ns + static_cast<size_t>(nzmax) * nzmax) * sizeof(double);
+  return (2 * static_cast<size_t>(nzmax) * ns + static_cast<size_t>(nzmax) * nzmax) * sizeof(double);
 }
 
 int launch_bs_orth(int ns, int nblk, int nch, int ncp, int nzmax, const int* zoff, const int* nz, const double* Z,
