@@ -1504,11 +1504,30 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   c->wave_prepared = false;
   using clk = std::chrono::steady_clock;
   const auto h0 = clk::now();
+  // record checks on the host workers (the first bad record in list order
+  // is the one reported)
+  if (!c->pool) c->pool.reset(new Pool(host_workers()));
+  const int nchk = static_cast<int>(std::min<long>(c->pool->size(), nrec / 16384L + 1));
+  std::vector<int> bad(nchk, -1), exch(nchk, 0);
+  const std::function<void(int)> check = [&](int t) {
+    const int r0 = static_cast<int>(static_cast<long>(nrec) * t / nchk);
+    const int r1 = static_cast<int>(static_cast<long>(nrec) * (t + 1) / nchk);
+    int ex = 0;
+    for (int r = r0; r < r1; ++r) {
+      if (jtype[r] < 1 || jtype[r] > 9 || kpol[r] < 0 || kpol[r] > mk) { bad[t] = r; return; }
+      ex |= jtype[r] >= 8;
+    }
+    exch[t] = ex;
+  };
+  c->pool->run(nchk, check);
   bool any_exch = false;
-  for (int r = 0; r < nrec; ++r) {
-    if (jtype[r] < 1 || jtype[r] > 9) return fail(-21, " I_data: unknown itype ");
-    if (kpol[r] < 0 || kpol[r] > mk) return fail(-17, "UPDATE_CF: multipole index out of range");
-    if (jtype[r] >= 8) any_exch = true;
+  for (int t = 0; t < nchk; ++t) {
+    if (bad[t] >= 0) {
+      const int r = bad[t];
+      if (jtype[r] < 1 || jtype[r] > 9) return fail(-21, " I_data: unknown itype ");
+      return fail(-17, "UPDATE_CF: multipole index out of range");
+    }
+    any_exch = any_exch || exch[t];
   }
   for (int i = 0; i < nch; ++i)
     if (iptar[i] < 1 || iptar[i] > ntarg) return fail(-7, "iptar out of range");
