@@ -335,3 +335,28 @@ class Context:
 
     def set_precision(self, mode, nslices=0):
         L.check(L.lib().bsrg_set_precision(self._p, mode, nslices))
+
+
+def cdata_build(j1, j2, j3, j4, c):
+    """Add_cdata (add_cdata.f90:2-56) of every record into one empty block:
+    ((k1, k2, k3, k4), c) sorted on the key, equal keys summed."""
+    n = len(c)
+    k = [np.zeros(n, np.int32) for _ in range(4)]
+    co = np.zeros(n)
+    nout = C.c_int()
+    L.check(L.lib().bsrg_cdata_build(n, i32(j1), i32(j2), i32(j3), i32(j4), f64(c), *k, co, nout))
+    m = nout.value
+    return [x[:m] for x in k], co[:m]
+
+
+def cdata_merge(ip, jp, k1, k2, k3, k4, c, eps_c):
+    """Merge_CDATA (merge_cdata.f90:2-91) over the sorted blocks
+    [ip(m), jp(m)]: (ipt, merged coefficients)."""
+    n = len(c)
+    cc = f64(c).copy()
+    ipt = np.zeros(max(n, 1), np.int32)
+    nc = C.c_int()
+    L.check(L.lib().bsrg_cdata_merge(len(ip), i32(ip).copy(), i32(jp), i32(k1), i32(k2), i32(k3), i32(k4), cc,
+                                     float(eps_c), ipt, nc))
+    return ipt[: nc.value].copy(), cc
+

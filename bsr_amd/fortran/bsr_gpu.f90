@@ -12,6 +12,7 @@
 !    gpu_gen_lval(z, kclosd)       Gen_Lval + INT_core src/bsr/BSR_MAT3/l_core.f90:92
 !    gpu_l_data(jtype, l, ...)     L_data(jtype,l)    src/bsr/BSR_MAT3/L_data.f90:2
 !    bsrg_bs_orth(gpu_ctx, 0)      BS_ORTH            src/bsr/BSR_MAT3/bs_orth.f90:2
+!    bsrg_cdata_merge(...)         Merge_CDATA        src/bsr/BSR_MAT3/merge_cdata.f90:2
 !  and, for Dirac partial waves (DBSR_MAT3):
 !    gpu_dbs_grid(ks,ns,ksp,ksq,t) DBS_grid / DBS_gauss module state
 !    gpu_mrk_dirac(family, k, rkb) mrk_pppp/qqqq/pqpq/qpqp(k)  src/libs/DBS/mrk_pppp.f90:2
@@ -110,6 +111,14 @@ module bsr_gpu
       type(c_ptr), value :: ctx
       integer(c_int), value :: jtype, l, n, iitar
       integer(c_int) :: ipt(*), k1(*), k2(*), k3(*), k4(*)
+      real(c_double) :: c(*)
+      real(c_double), value :: eps_c
+    end function
+    integer(c_int) function bsrg_cdata_merge(nn, ip, jp, k1, k2, k3, k4, c, eps_c, ipt, nc) &
+        bind(C, name="bsrg_cdata_merge")
+      import :: c_int, c_double
+      integer(c_int), value :: nn
+      integer(c_int) :: ip(*), jp(*), k1(*), k2(*), k3(*), k4(*), ipt(*), nc
       real(c_double) :: c(*)
       real(c_double), value :: eps_c
     end function

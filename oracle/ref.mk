@@ -25,7 +25,8 @@ SRCS := \
   $(REF)/libs/ZCONFLS/MOD_target.f90 $(REF)/libs/ZCONFLS/MOD_channel.f90 \
   $(REF)/bsr/BSR_MAT3/mod_intmat.f90 \
   $(REF)/libs/BS/int_de.f90 $(REF)/libs/BS/azl.f90 $(REF)/libs/ZCOM/ipointer.f90 \
-  $(REF)/libs/RADIAL/zcb.f90 $(REF)/bsr/BSR_MAT3/pri_int.f90 $(REF)/bsr/BSR_MAT3/l_core.f90
+  $(REF)/libs/RADIAL/zcb.f90 $(REF)/bsr/BSR_MAT3/pri_int.f90 $(REF)/bsr/BSR_MAT3/l_core.f90 \
+  $(REF)/bsr/BSR_MAT3/mod_cmdata.f90 $(REF)/bsr/BSR_MAT3/add_cdata.f90 $(REF)/bsr/BSR_MAT3/merge_cdata.f90
 
 OBJS := $(addprefix $(OUT)/,$(notdir $(SRCS:.f90=.o)))
 

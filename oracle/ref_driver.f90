@@ -356,3 +356,45 @@ subroutine ref_gen_lval(mlh_o, hlo, lio, lvo) bind(C, name="ref_gen_lval")
   hlo(:,:,0:mlh) = hl_full(:,:,0:mlh)
   lio = L_int; lvo = L_vec
 end subroutine
+
+!----------------------------------------------------------------------
+! Coefficient lists of module cmdata: Add_cdata into one block, and
+! Merge_CDATA over sorted blocks (src/bsr/BSR_MAT3/add_cdata.f90,
+! merge_cdata.f90).
+!----------------------------------------------------------------------
+subroutine ref_add_cdata(n, j1, j2, j3, j4, c, k1o, k2o, k3o, k4o, co, nout) bind(C, name="ref_add_cdata")
+  use iso_c_binding
+  use cmdata
+  implicit none
+  integer(c_int), value :: n
+  integer(c_int) :: j1(n), j2(n), j3(n), j4(n), k1o(n), k2o(n), k3o(n), k4o(n), nout
+  real(c_double) :: c(n), co(n)
+  integer :: i, ip, jp
+  if (allocated(CDATA)) deallocate(CDATA, K1, K2, K3, K4, IPT)
+  allocate(CDATA(n + 1), K1(n + 1), K2(n + 1), K3(n + 1), K4(n + 1), IPT(n + 1))
+  ip = 1; jp = 0
+  do i = 1, n
+    call Add_cdata(ip, jp, c(i), j1(i), j2(i), j3(i), j4(i))
+  end do
+  nout = jp
+  k1o(1:jp) = K1(1:jp); k2o(1:jp) = K2(1:jp); k3o(1:jp) = K3(1:jp); k4o(1:jp) = K4(1:jp)
+  co(1:jp) = CDATA(1:jp)
+end subroutine
+
+subroutine ref_merge_cdata(nn, ip, jp, n, k1i, k2i, k3i, k4i, c, eps, ipto, nc) bind(C, name="ref_merge_cdata")
+  use iso_c_binding
+  use cmdata
+  implicit none
+  integer(c_int), value :: nn, n
+  integer(c_int) :: ip(nn), jp(nn), k1i(n), k2i(n), k3i(n), k4i(n), ipto(n), nc
+  real(c_double) :: c(n)
+  real(c_double), value :: eps
+  integer :: ncc
+  if (allocated(CDATA)) deallocate(CDATA, K1, K2, K3, K4, IPT)
+  allocate(CDATA(n), K1(n), K2(n), K3(n), K4(n), IPT(n))
+  K1 = k1i; K2 = k2i; K3 = k3i; K4 = k4i; CDATA = c
+  call Merge_cdata(nn, ip, jp, ncc, eps)
+  nc = ncc
+  ipto(1:ncc) = IPT(1:ncc)
+  c = CDATA
+end subroutine

@@ -62,6 +62,8 @@ def lib():
             "ref_update_cf": (None, [i, i, i, d]),
             "ref_get_matrix": (None, [_dp, _dp, _dp, _dp]),
             "ref_bspd": (None, [_dp, _dp]),
+            "ref_add_cdata": (None, [i, _ip, _ip, _ip, _ip, _dp, _ip, _ip, _ip, _ip, _dp, C.POINTER(i)]),
+            "ref_merge_cdata": (None, [i, _ip, _ip, i, _ip, _ip, _ip, _ip, _dp, d, _ip, C.POINTER(i)]),
             "ref_galerkin": (None, [_dp, _dp]),
             "ref_set_lcore": (None, [d, i, i, _ip, _ip, _dp]),
             "ref_gen_lval": (None, [C.POINTER(i), _dp, _dp, _dp]),
@@ -172,6 +174,29 @@ class RefGrid:
         lv = np.zeros((ns, nbf), order="F")
         lib().ref_gen_lval(mlh, hl, li, lv)
         return mlh.value, hl, li, lv
+
+
+def add_cdata(j1, j2, j3, j4, c):
+    """The reference's Add_cdata, record by record, into one block."""
+    n = len(c)
+    i32 = lambda a: np.asfortranarray(np.asarray(a, dtype=np.int32))
+    k = [np.zeros(n, np.int32) for _ in range(4)]
+    co = np.zeros(n, order="F")
+    nout = C.c_int()
+    lib().ref_add_cdata(n, i32(j1), i32(j2), i32(j3), i32(j4), _f(c), *k, co, nout)
+    m = nout.value
+    return [x[:m] for x in k], co[:m]
+
+
+def merge_cdata(ip, jp, k1, k2, k3, k4, c, eps):
+    """The reference's Merge_CDATA: (ipt, merged cdata)."""
+    i32 = lambda a: np.asfortranarray(np.asarray(a, dtype=np.int32))
+    n = len(c)
+    cc = _f(c).copy()
+    ipt = np.zeros(n, np.int32)
+    nc = C.c_int()
+    lib().ref_merge_cdata(len(ip), i32(ip), i32(jp), n, i32(k1), i32(k2), i32(k3), i32(k4), cc, float(eps), ipt, nc)
+    return ipt[: nc.value].copy(), cc
 
 
 def density(ns, ks, p1, p2, typ):
