@@ -356,7 +356,7 @@ def cdata_merge(ip, jp, k1, k2, k3, k4, c, eps_c):
     cc = f64(c).copy()
     ipt = np.zeros(max(n, 1), np.int32)
     nc = C.c_int()
-    L.check(L.lib().bsrg_cdata_merge(len(ip), i32(ip).copy(), i32(jp), i32(k1), i32(k2), i32(k3), i32(k4), cc,
+    L.check(L.lib().bsrg_cdata_merge(len(ip), i32(ip).copy(), i32(jp), n, i32(k1), i32(k2), i32(k3), i32(k4), cc,
                                      float(eps_c), ipt, nc))
     return ipt[: nc.value].copy(), cc
 

@@ -1173,7 +1173,10 @@ int bsrg_gen_lval(bsrg_ctx* c, double z, int kclosd) {
   HIPCHK(hipSetDevice(c->device));
   c->lval_ok = false;
   int mlh = 0, lcore = 0;
-  for (int i = 0; i < nbf; ++i) mlh = std::max(mlh, c->lbs[i]);
+  for (int i = 0; i < nbf; ++i) {
+    if (c->lbs[i] < 0 || c->lbs[i] > 100) return fail(-62, "Gen_Lval: orbital l outside 0..100");
+    mlh = std::max(mlh, c->lbs[i]);
+  }
   for (int ip = 0; ip < kclosd; ++ip) lcore = std::max(lcore, c->lbs[ip]);
   // bases: the direct potential of the core needs k = 0, its exchange k up
   // to min(l + l_core, 7)

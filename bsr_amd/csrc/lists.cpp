@@ -61,12 +61,15 @@ int bsrg_cdata_build(int n, const int* j1, const int* j2, const int* j3, const i
 // the heads kept in a map from key to the blocks whose head holds it, so a
 // step costs O(log nn) instead of a scan of all pairs; the sums, their
 // order and ipt are those of the reference.
-int bsrg_cdata_merge(int nn, int* ip, const int* jp, const int* k1, const int* k2, const int* k3, const int* k4,
-                     double* cdata, double eps_c, int* ipt, int* nc) {
-  if (nn < 0 || !nc) return bsrg::fail(-2, "bsrg_cdata_merge: bad arguments");
+int bsrg_cdata_merge(int nn, int* ip, const int* jp, int n, const int* k1, const int* k2, const int* k3,
+                     const int* k4, double* cdata, double eps_c, int* ipt, int* nc) {
+  if (nn < 0 || n < 0 || !nc) return bsrg::fail(-2, "bsrg_cdata_merge: bad arguments");
   *nc = 0;
   if (nn == 0) return 0;
   if (!ip || !jp || !k1 || !k2 || !k3 || !k4 || !cdata || !ipt) return bsrg::fail(-2, "bsrg_cdata_merge: null argument");
+  for (int m = 0; m < nn; ++m)
+    if (jp[m] > 0 && ip[m] > 0 && (ip[m] > n || jp[m] > n))
+      return bsrg::fail(-2, "bsrg_cdata_merge: block outside the n records");
   auto key = [&](int i) { return Key(k1[i - 1], k2[i - 1], k3[i - 1], k4[i - 1]); };
   std::map<Key, std::set<int>> heads;  // key -> blocks (0-based) whose head holds it
   std::set<std::pair<int, Key>> clash;  // (smallest block, key) of keys held by two or more heads
@@ -96,7 +99,7 @@ int bsrg_cdata_merge(int nn, int* ip, const int* jp, const int* k1, const int* k
     if (++ip[m] <= jp[m]) insert(m);
     else ip[m] = 0;
   };
-  int n = 0;
+  int cnt = 0;
   while (!heads.empty()) {
     if (!clash.empty()) {
       const auto& s = heads[clash.begin()->second];
@@ -109,9 +112,9 @@ int bsrg_cdata_merge(int nn, int* ip, const int* jp, const int* k1, const int* k
     }
     const int mm = *heads.begin()->second.begin();
     const int i = ip[mm];
-    if (std::abs(cdata[i - 1]) > eps_c) ipt[n++] = i;
+    if (std::abs(cdata[i - 1]) > eps_c) ipt[cnt++] = i;
     advance(mm);
   }
-  *nc = n;
+  *nc = cnt;
   return 0;
 }

@@ -114,10 +114,10 @@ module bsr_gpu
       real(c_double) :: c(*)
       real(c_double), value :: eps_c
     end function
-    integer(c_int) function bsrg_cdata_merge(nn, ip, jp, k1, k2, k3, k4, c, eps_c, ipt, nc) &
+    integer(c_int) function bsrg_cdata_merge(nn, ip, jp, n, k1, k2, k3, k4, c, eps_c, ipt, nc) &
         bind(C, name="bsrg_cdata_merge")
       import :: c_int, c_double
-      integer(c_int), value :: nn
+      integer(c_int), value :: nn, n
       integer(c_int) :: ip(*), jp(*), k1(*), k2(*), k3(*), k4(*), ipt(*), nc
       real(c_double) :: c(*)
       real(c_double), value :: eps_c

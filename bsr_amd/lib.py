@@ -61,7 +61,7 @@ SIGNATURES = {
     "bsrg_odata": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, C.c_double],
     "bsrg_gen_lval": [C.c_void_p, C.c_double, C.c_int],
     "bsrg_cdata_build": [C.c_int, _ip, _ip, _ip, _ip, _dp, _ip, _ip, _ip, _ip, _dp, C.POINTER(C.c_int)],
-    "bsrg_cdata_merge": [C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp, C.c_double, _ip, C.POINTER(C.c_int)],
+    "bsrg_cdata_merge": [C.c_int, _ip, _ip, C.c_int, _ip, _ip, _ip, _ip, _dp, C.c_double, _ip, C.POINTER(C.c_int)],
     "bsrg_bs_orth": [C.c_void_p, C.c_int],
     "bsrg_lval_get": [C.c_void_p, C.POINTER(C.c_int), _opt_dp(), _opt_dp(), _opt_dp()],
     "bsrg_ldata": [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _dp, C.c_int, C.c_double],

@@ -181,13 +181,14 @@ int bsrg_idata(bsrg_ctx* ctx, int icase, int jtype, int jpol, int ncdata, const 
  * (K1,K2,K3,K4), the coefficients of equal keys summed in arrival order;
  * outputs hold up to n entries, *nout of them used.
  * bsrg_cdata_merge: Merge_CDATA (merge_cdata.f90:2-91) over nn sorted
- * blocks [ip(m), jp(m)] (1-based) of k1..k4/cdata: equal keys of different
+ * blocks [ip(m), jp(m)] (1-based, within the n records) of k1..k4/cdata
+ * (ipt holds up to n entries): equal keys of different
  * blocks summed into the first block's entry (the others set to 0) as the
  * reference orders those sums, ipt(1..*nc) the surviving entries with
  * |cdata| > eps_c in key order; ip is consumed (set to 0), cdata updated. */
 int bsrg_cdata_build(int n, const int* j1, const int* j2, const int* j3, const int* j4, const double* c,
                      int* k1, int* k2, int* k3, int* k4, double* cout, int* nout);
-int bsrg_cdata_merge(int nn, int* ip, const int* jp, const int* k1, const int* k2, const int* k3,
+int bsrg_cdata_merge(int nn, int* ip, const int* jp, int n, const int* k1, const int* k2, const int* k3,
                      const int* k4, double* cdata, double eps_c, int* ipt, int* nc);
 
 /* Overlap records (structure 1 only).  Replaces O_data,
