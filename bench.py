@@ -21,6 +21,8 @@ Beside the headline line's required fields, the JSON line carries:
            FP64 and in split precision: p50/p99 latency at that rate, the
            saturated capacity (all jobs queued at once), and the error of
            every split-precision job against its FP64 result;
+  l_data   the one-electron operator with a closed core (Gen_Lval) and the
+           L-integral lists of the wave applied call by call (L_data);
   bs_orth  BS_ORTH (the channel orbitals' orthogonality conditions) on the
            wave's H: time per call and the algorithmic HBM rate of the
            blocks it rewrites;
@@ -411,6 +413,22 @@ def main():
         out["bs_orth"] = {"ms": oms, "channels_with_conditions": sum(1 for q in conds if q),
                           "blocks": touched, "algorithmic_GB_s": gbs}
         log(f"bs_orth: {oms:.3f} ms per call, {touched} blocks, {gbs:.0f} GB/s (2 reads + 1 write per block)")
+    # one-electron operator with a closed core of the first two bound
+    # orbitals (Gen_Lval) and the L-integral lists of the wave (L_data, call
+    # by call into a matrix of the wave's size)
+    llists = synth.l_lists(w, 6, fill=0.05)
+    nrec_l = sum(len(x) for x in llists.values())
+    ctx.allocate_matrix(w.nch, w.ncp, w.mk)
+    t0 = time.perf_counter()
+    ctx.gen_lval(18.0, 2)
+    gl_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    for (jt, l), lst in sorted(llists.items()):
+        if len(lst):
+            ctx.l_data(jt, l, lst, iitar=1)
+    ld_ms = (time.perf_counter() - t0) * 1e3
+    out["l_data"] = {"gen_lval_ms": gl_ms, "l_data_ms": ld_ms, "records": nrec_l, "lists": len(llists)}
+    log(f"gen_lval: {gl_ms:.2f} ms; l_data: {ld_ms:.2f} ms for {nrec_l} records in {len(llists)} lists")
     ctx.close()
     if not a.no_serve:
         log("generating served stream")
