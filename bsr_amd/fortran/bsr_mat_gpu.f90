@@ -13,8 +13,8 @@
 program bsr_mat_gpu
   use bsr_gpu
   implicit none
-  integer(c_int) :: ks, ns, ml, me, nbf, nch, ncp, mk, ntarg, nlists, novl
-  real(c_double) :: h, ec, eps_c, eps_ovl
+  integer(c_int) :: ks, ns, ml, me, nbf, nch, ncp, mk, ntarg, nlists, novl, nllists, kclosd, iitar, orth
+  real(c_double) :: h, ec, eps_c, eps_ovl, z
   real(c_double), allocatable :: t(:), pbs(:,:), etarg(:), oc(:), c(:)
   integer(c_int), allocatable :: iech(:), lbs(:), ibort(:,:), iptar(:), ok3(:), ok4(:), ipt(:)
   integer(c_int), allocatable :: k1(:), k2(:), k3(:), k4(:)
@@ -31,6 +31,7 @@ program bsr_mat_gpu
   read(nu) t, pbs, iech, lbs, ibort, iptar, etarg
   allocate(ok3(novl), ok4(novl), oc(novl))
   read(nu) ok3, ok4, oc
+  read(nu) nllists, kclosd, iitar, orth, z
 
   nij = nch*(nch+1)/2
   allocate(hcc(ns,ns,nij), hcb(ns,nch,max(ncp,1)), hbb(max(ncp*(ncp+1)/2,1)), &
@@ -55,8 +56,17 @@ program bsr_mat_gpu
   write(nw) ns, nch, ncp
   call record_matrix(nw)
 
-  ! interaction matrix
+  ! interaction matrix: EC*S, L-integrals (Gen_Lval, then icase 6), R^k
   call gpu_check(bsrg_scale(gpu_ctx, ec))
+  if (nllists > 0) call gpu_gen_lval(z, kclosd)
+  do il = 1, nllists
+    read(nu) jtype, kpol, n
+    allocate(k1(n), k2(n), k3(n), k4(n), c(n), ipt(n))
+    read(nu) k1, k2, k3, k4, c
+    ipt = [(i, i = 1, n)]
+    call gpu_l_data(jtype, kpol, n, ipt, k1, k2, k3, k4, c, iitar, 0.d0)
+    deallocate(k1, k2, k3, k4, c, ipt)
+  end do
   do il = 1, nlists
     read(nu) jtype, kpol, n
     allocate(k1(n), k2(n), k3(n), k4(n), c(n), ipt(n))
@@ -70,6 +80,7 @@ program bsr_mat_gpu
     call gpu_update_hl(ich, ich, etarg(iptar(ich)) - ec)
   end do
   call gpu_check(bsrg_symmetrize_diag(gpu_ctx))
+  if (orth /= 0) call gpu_check(bsrg_bs_orth(gpu_ctx, 0))
   call gpu_check(bsrg_acf_finish(gpu_ctx, 0))
   call gpu_check(bsrg_get_matrix(gpu_ctx, hcc, hcb, hbb, acf, htarg, otarg))
   call record_matrix(nw)

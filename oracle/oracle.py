@@ -494,12 +494,16 @@ class Wave:
         lib().orc_wave_acf_finish(self.ptr, kcore)
 
 
-def run_wave(job, grid: Grid | None = None, orth: bool = False):
+def run_wave(job, grid: Grid | None = None, orth: bool = False, l_lists: dict | None = None, z: float = 0.0,
+             kclosd: int = 0, iitar: int = 0):
     """One partial wave in the order of SUB1 (src/bsr/BSR_MAT3/sub1.f90:70-219):
     S = sb on the channel diagonals + O_data, symmetrised; H = EC*S + R^k
     contributions (jpol outer, jtype inner, state_res.f90:155-159) +
     (E_t - EC)*sb, symmetrised (then BS_ORTH when orth, sub1.f90:195); ACF
-    finished.  Returns (S, H) as WaveMatrices (S.htarg/acf unused)."""
+    finished.  With l_lists, Gen_Lval (nuclear charge z, closed core
+    1..kclosd) and the L-integral lists (keyed (jtype, l)) come between the
+    EC scaling and the R^k lists (sub1.f90:121-129).  Returns (S, H) as
+    WaveMatrices (S.htarg/acf unused)."""
     if grid is None:
         grid = Grid(job.ks, job.ns, job.ml, job.me, job.h, job.t)
     w = Wave(grid, job)
@@ -510,6 +514,11 @@ def run_wave(job, grid: Grid | None = None, orth: bool = False):
     w.symmetrize_diag()
     S = w.get()
     w.scale(job.ec)
+    if l_lists:
+        w.gen_lval(z, kclosd)
+        for (jt, l), lst in sorted(l_lists.items()):
+            if len(lst):
+                w.ldata(jt, l, lst, iitar=iitar)
     for k in range(0, job.mk + 1):
         for jtype in range(1, 10):
             lst = job.rk_lists.get((jtype, k))

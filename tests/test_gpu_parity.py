@@ -262,10 +262,13 @@ def test_errors_are_reported(ctx):
             fresh.matfile_eigen("/nonexistent/bsr_mat.001", 10)
 
 
-def test_fortran_host_driver(oracle_lib, tmp_path):
+@pytest.mark.parametrize("full", [False, True], ids=["rk", "l_orth"])
+def test_fortran_host_driver(oracle_lib, tmp_path, full):
     """The Fortran host (module bsr_gpu over the C ABI, program bsr_mat_gpu)
     runs a whole wave call by call and writes bsr_mat.nnn with the Fortran
-    runtime's own unformatted records; compared with the oracle's file."""
+    runtime's own unformatted records; compared with the oracle's file.
+    l_orth adds SUB1's one-electron part (Gen_Lval with a closed core, the
+    L-integral lists) and BS_ORTH."""
     import subprocess
     from tests.conftest import HAS_GPU
     if not HAS_GPU:
@@ -274,11 +277,15 @@ def test_fortran_host_driver(oracle_lib, tmp_path):
     from bsr_amd import synth, jobfile
     from bsr_amd.build import DRIVER
     w = synth.make_wave(61, 8, 40, 4.0, 12.0, nch=5, ncp=6, mk=3, nb=3, ntarg=2, fill=0.3, name="fortran")
+    extra = {}
+    if full:
+        w.lbs[:] = 1  # orthogonality conditions for the channels
+        extra = dict(l_lists=synth.l_lists(w, 8, fill=0.3), z=4.0, kclosd=1, iitar=1)
     job, out, ref = tmp_path / "wave.bin", tmp_path / "bsr_mat.001", tmp_path / "oracle.001"
-    jobfile.write(job, w)
+    jobfile.write(job, w, orth=full, **extra)
     r = subprocess.run([DRIVER, str(job), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    oS, oH = O.run_wave(w)
+    oS, oH = O.run_wave(w, orth=full, **extra)
     O.write_bsr_mat(ref, w, oS, oH)
     g, o = O.read_bsr_mat(out), O.read_bsr_mat(ref)
     assert (g["ns"], g["nch"], g["ncp"], g["mk"], g["ns2"]) == (o["ns"], o["nch"], o["ncp"], o["mk"], o["ns2"])
