@@ -11,7 +11,6 @@
 namespace bsrg {
 
 constexpr int kMaxKs = 16;    // largest spline order supported
-constexpr int kMaxNb = 8;     // largest number of bound orbitals per wave
 constexpr int kIbi = 32768;   // packing base of orbital pairs (cmdata ibi)
 constexpr int kIbo = 32768;   // packing base of overlaps (bsr_mat ibo)
 
@@ -102,10 +101,13 @@ void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int
 void launch_symmetrize_diag(int ns, int nch, double* hcc, hipStream_t s);
 // BS_ORTH on all channel blocks (nblk = nch(nch+1)/2) and channel-perturber
 // vectors: channel c is projected with the nz[c] columns of Z (and BZ = B Z)
-// starting at column zoff[c]; nzmax = max nz.  Returns -1 when nzmax is
-// outside 1..16, -2 when its LDS would not fit.
+// starting at column zoff[c]; nzmax = max nz (any number).  The work arrays
+// of a block go to LDS when they fit, else to scratch (nscratch doubles, at
+// least bs_orth_scratch_doubles(ns, nblk, nzmax)).  Returns -1 when nzmax < 1,
+// -2 when the scratch is too small.
+size_t bs_orth_scratch_doubles(int ns, int nblk, int nzmax);
 int launch_bs_orth(int ns, int nblk, int nch, int ncp, int nzmax, const int* zoff, const int* nz, const double* Z,
-                   const double* BZ, double* hcc, double* hcb, hipStream_t s);
+                   const double* BZ, double* hcc, double* hcb, double* scratch, size_t nscratch, hipStream_t s);
 // One-electron operator of one l (Gen_Lval/INT_core): out (ns x ns) =
 // sum_t ec[t] E(eid[t]) + the band hl (lower storage) + sum_u dc[u]
 // bands(did[u]) (upper storage), the band region taken from the lower

@@ -121,7 +121,6 @@ int ensure_bases(bsrg_ctx* c, int nk, bool exchange) {
     c->wvec_uploaded = -1;
   }
   if (exchange && c->exch_nk < nk && nb > 0) {
-    if (nb > kMaxNb) return fail(-32, "too many bound orbitals for the exchange basis");
     HIPCHK(c->E.resize(static_cast<size_t>(nk) * nb * nb * ns * ns));
     launch_exchange_basis(ns, ks, nk, nb, c->rkb.p, c->P.p, c->P.p, c->E.p, c->stream);
     HIPCHK(hipGetLastError());
@@ -1436,7 +1435,6 @@ int bsrg_bs_orth(bsrg_ctx* c, int which) {
     nzmax = std::max(nzmax, nz[ich - 1]);
   }
   if (nzmax == 0) return 0;
-  if (nzmax > 16) return fail(-71, "BS_ORTH: more than 16 orthogonality conditions for one channel");
   const size_t nc = cols.size();
   std::vector<double> zb(2 * nc * ns);  // Z then B Z, one column per condition
   for (size_t q = 0; q < nc; ++q) {
@@ -1451,9 +1449,12 @@ int bsrg_bs_orth(bsrg_ctx* c, int which) {
   HIPCHK(c->lwork_i.resize(ints.size()));
   HIPCHK(hipMemcpyAsync(c->lwork.p, zb.data(), zb.size() * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(c->lwork_i.p, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  const size_t nscr = bs_orth_scratch_doubles(ns, static_cast<int>(M.nblk()), nzmax);
+  if (nscr) HIPCHK(c->dtmp2.resize(nscr));
   const int lrc = launch_bs_orth(ns, static_cast<int>(M.nblk()), nch, M.ncp, nzmax, c->lwork_i.p, c->lwork_i.p + nch,
-                                 c->lwork.p, c->lwork.p + nc * ns, M.hcc.p, M.hcb.p, s);
-  if (lrc) return fail(-71, "BS_ORTH: the projectors of one channel do not fit the kernel's local memory");
+                                 c->lwork.p, c->lwork.p + nc * ns, M.hcc.p, M.hcb.p, nscr ? c->dtmp2.p : nullptr,
+                                 nscr, s);
+  if (lrc) return fail(-71, "BS_ORTH: no work space for the projectors");
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s));
   return 0;

@@ -226,7 +226,6 @@ int dirac_bases(bsrg_ctx* c, int nk, bool exch) {
     W.basis_nk = nk;
   }
   if (exch && W.exch_nk < nk && nb > 0) {
-    if (nb > kMaxNb) return fail(-32, "too many bound orbitals for the exchange basis");
     const size_t per = static_cast<size_t>(nk) * nb * nb * ns * ns;
     HIPCHK(W.E.resize(4 * per));
     for (int f = 0; f < 4; ++f)

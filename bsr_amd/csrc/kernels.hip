@@ -362,11 +362,15 @@ __global__ void k_direct_reduce(int ns, int ks, int npair, int nk, const double*
 // ---- a9 exchange basis ---------------------------------------------------
 // E[kk][a][b](x,y) = sum_{z,w} K(x,z,y,w) p_a(z) q_b(w),
 // K(x,z,y,w) = rkb(min(x,z), min(y,w), |x-z|, |y-w|): p on the r1 pair (P1),
-// q on the r2 pair (P2; the same orbitals as P1 for the BS bands).  One
+// q on the r2 pair (P2; the same orbitals as P1 for the BS bands).  The
+// nbt x nbt pairs are covered in tiles of at most NB orbitals on each side:
+// this launch makes a in [a0, a0+na), b in [b0, b0+nbc) (na, nbc <= NB;
+// orbitals past the counts get weight 0 and are not written).  One
 // workgroup per output column y; threads over rows x (coalesced band reads
 // and writes).
 template <int NB, int KS>
-__global__ __launch_bounds__(256) void k_exchange_basis(int ns, int ks_rt, const double* __restrict__ rkb,
+__global__ __launch_bounds__(256) void k_exchange_basis(int ns, int ks_rt, int nbt, int a0, int na, int b0, int nbc,
+                                                        const double* __restrict__ rkb,
                                                         const double* __restrict__ P, const double* __restrict__ P2,
                                                         double* __restrict__ E) {
   // KS > 0: the order at compile time, so that the 2 KS - 1 band reads of
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(256) void k_exchange_basis(int ns, int ks_rt, const
   __shared__ double pw[NB][2 * kMaxKs];
   for (int q = threadIdx.x; q < NB * (2 * ks - 1); q += blockDim.x) {
     const int b = q / (2 * ks - 1), dw = q % (2 * ks - 1) - (ks - 1), w = y + dw;
-    pw[b][dw + ks - 1] = (w >= 0 && w < ns) ? P2[static_cast<size_t>(b) * ns + w] : 0.0;
+    pw[b][dw + ks - 1] = (b < nbc && w >= 0 && w < ns) ? P2[static_cast<size_t>(b0 + b) * ns + w] : 0.0;
   }
   __syncthreads();
   const size_t s_ip = static_cast<size_t>(ns) * ns, s_jp = s_ip * ks;
@@ -423,29 +427,34 @@ __global__ __launch_bounds__(256) void k_exchange_basis(int ns, int ks_rt, const
       }
 #pragma unroll
       for (int a = 0; a < NB; ++a) {
-        const double pa = P[static_cast<size_t>(a) * ns + z];
+        const double pa = a < na ? P[static_cast<size_t>(a0 + a) * ns + z] : 0.0;
 #pragma unroll
         for (int b = 0; b < NB; ++b) e[a * NB + b] += pa * tb[b];
       }
     }
     const size_t pos = x + static_cast<size_t>(ns) * y;
 #pragma unroll
-    for (int q = 0; q < NB * NB; ++q)
-      E[(static_cast<size_t>(kk) * NB * NB + q) * ns * ns + pos] = e[q];
+    for (int q = 0; q < NB * NB; ++q) {
+      const int a = q / NB, b = q % NB;
+      if (a < na && b < nbc)
+        E[((static_cast<size_t>(kk) * nbt + a0 + a) * nbt + b0 + b) * ns * ns + pos] = e[q];
+    }
   }
 }
 
 // Exchange bases with the contraction over w done first, per band row:
 //   T_b(i, ip) = sum_dw rkb(i, min(y, y+dw), ip, |dw|) q_b(y + dw)
 //   E_ab(x, y) = sum_dz p_a(x + dz) T_b(min(x, x+dz), |dz|)
-// One workgroup per (64-row block of x, column y, multipole): phase 1 has
-// one thread per (row i, ip) of the block and its halo (2 KS - 1 band reads
-// in flight per thread), T lands in LDS; phase 2 has one thread per (x, a).
+// One workgroup per (64-row block of x, column y, multipole) and tile of NB
+// r2 orbitals b in [b0, b0+nbc): phase 1 has one thread per (row i, ip) of
+// the block and its halo (2 KS - 1 band reads in flight per thread), T
+// lands in LDS; phase 2 has one thread per (x, a) and runs over all nbt r1
+// orbitals in chunks of NB, so the band is read once per b tile.
 // Same sums as k_exchange_basis, grouped differently (FP64 rounding aside).
 template <int NB, int KS>
 __global__ __launch_bounds__(((64 + KS - 1) * KS + 63) / 64 * 64) void k_exchange_basis_t(
-    int ns, const double* __restrict__ rkb, const double* __restrict__ P, const double* __restrict__ P2,
-    double* __restrict__ E) {
+    int ns, int nbt, int b0, int nbc, const double* __restrict__ rkb, const double* __restrict__ P,
+    const double* __restrict__ P2, double* __restrict__ E) {
   constexpr int H = KS - 1, NR = 64 + H, ND = 2 * KS - 1;
   __shared__ double sT[KS][NB][NR];
   __shared__ double pw[NB][ND];
@@ -460,11 +469,7 @@ __global__ __launch_bounds__(((64 + KS - 1) * KS + 63) / 64 * 64) void k_exchang
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int q = tid; q < NB * ND; q += nt) {
     const int b = q / ND, t = q % ND, w = y + t - H;
-    pw[b][t] = (w >= 0 && w < ns) ? P2[static_cast<size_t>(b) * ns + w] : 0.0;
-  }
-  for (int q = tid; q < NB * (64 + 2 * H); q += nt) {
-    const int a = q / (64 + 2 * H), z = x0 - H + q % (64 + 2 * H);
-    sp[a][q % (64 + 2 * H)] = (z >= 0 && z < ns) ? P[static_cast<size_t>(a) * ns + z] : 0.0;
+    pw[b][t] = (b < nbc && w >= 0 && w < ns) ? P2[static_cast<size_t>(b0 + b) * ns + w] : 0.0;
   }
   __syncthreads();
   // phase 1: thread (i, ip); rows outside [0, ns) only ever meet weight 0
@@ -485,25 +490,33 @@ __global__ __launch_bounds__(((64 + KS - 1) * KS + 63) / 64 * 64) void k_exchang
       sT[ip][b][ri] = v;
     }
   }
-  __syncthreads();
-  // phase 2: thread (x, a)
-  for (int q = tid; q < 64 * NB; q += nt) {
-    const int xl = q % 64, a = q / 64, x = x0 + xl;
-    if (x >= ns) continue;
-    double e[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) e[b] = 0.0;
-#pragma unroll 1
-    for (int t = 0; t < ND; ++t) {
-      const int dz = t - H;
-      const double pa = sp[a][xl + t];  // p_a(x + dz), 0 outside [0, ns)
-      const int ri = xl + H + (dz < 0 ? dz : 0), ip = dz < 0 ? -dz : dz;
-#pragma unroll
-      for (int b = 0; b < NB; ++b) e[b] += pa * sT[ip][b][ri];
+  // phase 2: thread (x, a), r1 orbitals in chunks of NB
+  for (int a0 = 0; a0 < nbt; a0 += NB) {
+    __syncthreads();  // sT complete; the previous chunk no longer reads sp
+    for (int q = tid; q < NB * (64 + 2 * H); q += nt) {
+      const int a = q / (64 + 2 * H), z = x0 - H + q % (64 + 2 * H);
+      sp[a][q % (64 + 2 * H)] = (a0 + a < nbt && z >= 0 && z < ns) ? P[static_cast<size_t>(a0 + a) * ns + z] : 0.0;
     }
-    const size_t pos = x + static_cast<size_t>(ns) * y;
+    __syncthreads();
+    for (int q = tid; q < 64 * NB; q += nt) {
+      const int xl = q % 64, a = q / 64, x = x0 + xl;
+      if (x >= ns || a0 + a >= nbt) continue;
+      double e[NB];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) E[((static_cast<size_t>(kk) * NB + a) * NB + b) * ns * ns + pos] = e[b];
+      for (int b = 0; b < NB; ++b) e[b] = 0.0;
+#pragma unroll 1
+      for (int t = 0; t < ND; ++t) {
+        const int dz = t - H;
+        const double pa = sp[a][xl + t];  // p_a(x + dz), 0 outside [0, ns)
+        const int ri = xl + H + (dz < 0 ? dz : 0), ip = dz < 0 ? -dz : dz;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) e[b] += pa * sT[ip][b][ri];
+      }
+      const size_t pos = x + static_cast<size_t>(ns) * y;
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        if (b < nbc) E[((static_cast<size_t>(kk) * nbt + a0 + a) * nbt + b0 + b) * ns * ns + pos] = e[b];
+    }
   }
 }
 
@@ -924,23 +937,31 @@ void launch_direct_basis(int ns, int ks, int nk, int npair, const double* rkb, c
 
 void launch_exchange_basis(int ns, int ks, int nk, int nb, const double* rkb, const double* P, const double* P2,
                            double* E, hipStream_t s) {
+  if (nb <= 0) return;
   const dim3 grid(ns * nk), block(ns <= 64 ? 64 : (ns <= 128 ? 128 : 256));
   const dim3 gridt(static_cast<unsigned>((ns + 63) / 64) * ns * nk);
-  switch (nb) {
-#define NB_CASE(N)                                                                                      \
-  case N:                                                                                               \
-    if (ks == 8)                                                                                        \
+  // up to 8 orbitals a tile exactly; more in tiles of 8
+  const int tile = nb <= 8 ? nb : 8;
+  for (int b0 = 0; b0 < nb; b0 += tile) {
+    const int nbc = std::min(tile, nb - b0);
+    switch (tile) {
+#define NB_CASE(N)                                                                                        \
+  case N:                                                                                                 \
+    if (ks == 8)                                                                                          \
       hipLaunchKernelGGL((k_exchange_basis_t<N, 8>), gridt, dim3(((64 + 7) * 8 + 63) / 64 * 64), 0, s, ns, \
-                         rkb, P, P2, E);                                                                \
-    else if (ks == 9)                                                                                   \
+                         nb, b0, nbc, rkb, P, P2, E);                                                     \
+    else if (ks == 9)                                                                                     \
       hipLaunchKernelGGL((k_exchange_basis_t<N, 9>), gridt, dim3(((64 + 8) * 9 + 63) / 64 * 64), 0, s, ns, \
-                         rkb, P, P2, E);                                                                \
-    else                                                                                                \
-      hipLaunchKernelGGL((k_exchange_basis<N, 0>), grid, block, 0, s, ns, ks, rkb, P, P2, E);           \
+                         nb, b0, nbc, rkb, P, P2, E);                                                     \
+    else                                                                                                  \
+      for (int a0 = 0; a0 < nb; a0 += tile)                                                               \
+        hipLaunchKernelGGL((k_exchange_basis<N, 0>), grid, block, 0, s, ns, ks, nb, a0,                   \
+                           std::min(tile, nb - a0), b0, nbc, rkb, P, P2, E);                              \
     break;
-    NB_CASE(1) NB_CASE(2) NB_CASE(3) NB_CASE(4) NB_CASE(5) NB_CASE(6) NB_CASE(7) NB_CASE(8)
+      NB_CASE(1) NB_CASE(2) NB_CASE(3) NB_CASE(4) NB_CASE(5) NB_CASE(6) NB_CASE(7) NB_CASE(8)
 #undef NB_CASE
-    default: break;
+      default: break;
+    }
   }
 }
 
@@ -1051,12 +1072,16 @@ __global__ __launch_bounds__(256) void k_dense_terms(int ns, const int* __restri
 // Channel block (i, j), i >= j: H <- (1 - B Z_i Z_i^T) H (1 - Z_j Z_j^T B)
 // = H - BZ_i (T - U) - V BZ_j^T, with T = Z_i^T H, V = H Z_j,
 // U = (Z_i^T V) BZ_j^T; Z_c holds the nz[c] bound orbitals channel c must be
-// orthogonal to (columns zoff[c].. of Z, and BZ = B Z), nz <= kOrthMax.
+// orthogonal to (columns zoff[c].. of Z, and BZ = B Z).  T and V are linear
+// in the conditions: both are formed from the unmodified block in chunks of
+// kOrthMax conditions, so any number of conditions is handled.
 // One workgroup of 4 waves per block.  Pass 1 reads the block twice (the
 // second time mostly from cache): V with lanes over rows, T with lanes over
 // columns, each sum in index order in one register; no cross-lane sums.
 // Pass 2 writes the block element by element.
-// LDS (doubles): V[nz][ns], TU[nz][ns], W[nz][nz].
+// Work arrays (doubles): V[nz][ns], TU[nz][ns], W[nz][nz], in LDS when they
+// fit (GLB = false), else in a global scratch slice per workgroup (blocks
+// blk0 .. blk0 + gridDim.x - 1 in one launch).
 constexpr int kOrthMax = 16;
 
 __device__ inline void blk_pair(int blk, int& i, int& j) {
@@ -1067,11 +1092,13 @@ __device__ inline void blk_pair(int blk, int& i, int& j) {
   j = blk - r * (r - 1) / 2 + 1;
 }
 
-__global__ __launch_bounds__(256) void k_bs_orth(int ns, int nzmax, const int* __restrict__ zoff,
+template <bool GLB>
+__global__ __launch_bounds__(256) void k_bs_orth(int ns, int nzmax, int blk0, const int* __restrict__ zoff,
                                                  const int* __restrict__ nz, const double* __restrict__ Z,
-                                                 const double* __restrict__ BZ, double* __restrict__ hcc) {
+                                                 const double* __restrict__ BZ, double* __restrict__ hcc,
+                                                 double* __restrict__ scratch) {
   extern __shared__ double lds[];
-  const int blk = blockIdx.x;
+  const int blk = blk0 + static_cast<int>(blockIdx.x);
   int ci, cj;
   blk_pair(blk, ci, cj);  // 1-based channels, ci >= cj
   const int ni = nz[ci - 1], nj = nz[cj - 1];
@@ -1080,13 +1107,15 @@ __global__ __launch_bounds__(256) void k_bs_orth(int ns, int nzmax, const int* _
   const double* Zj = Z + static_cast<size_t>(zoff[cj - 1]) * ns;
   const double* BZi = BZ + static_cast<size_t>(zoff[ci - 1]) * ns;
   const double* BZj = BZ + static_cast<size_t>(zoff[cj - 1]) * ns;
-  double* V = lds;                                       // [nzmax][ns]
+  const size_t per = 2 * static_cast<size_t>(nzmax) * ns + static_cast<size_t>(nzmax) * nzmax;
+  double* V = GLB ? scratch + per * blockIdx.x : lds;    // [nzmax][ns]
   double* TU = V + static_cast<size_t>(nzmax) * ns;      // [nzmax][ns]
   double* W = TU + static_cast<size_t>(nzmax) * ns;      // [nzmax][nzmax]
   double* h = hcc + static_cast<size_t>(blk) * ns * ns;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   // V = H Z_j: lanes over rows (coalesced), eight columns per round
-  if (nj > 0)
+  for (int q0 = 0; q0 < nj; q0 += kOrthMax) {
+    const int nq = min(kOrthMax, nj - q0);
     for (int x0 = 64 * wv; x0 < ns; x0 += 256) {
       const int x = x0 + lane;
       const bool in = x < ns;
@@ -1101,14 +1130,16 @@ __global__ __launch_bounds__(256) void k_bs_orth(int ns, int nzmax, const int* _
         for (int u = 0; u < nu; ++u)
 #pragma unroll
           for (int q = 0; q < kOrthMax; ++q)
-            if (q < nj) vacc[q] += hv[u] * Zj[static_cast<size_t>(q) * ns + y0 + u];
+            if (q < nq) vacc[q] += hv[u] * Zj[static_cast<size_t>(q0 + q) * ns + y0 + u];
       }
       if (in)
-        for (int q = 0; q < nj; ++q) V[static_cast<size_t>(q) * ns + x] = vacc[q];
+        for (int q = 0; q < nq; ++q) V[static_cast<size_t>(q0 + q) * ns + x] = vacc[q];
     }
+  }
   // T = Z_i^T H: lanes over columns (each lane walks its column; a cache
   // line serves eight rows), eight rows per round
-  if (ni > 0)
+  for (int q0 = 0; q0 < ni; q0 += kOrthMax) {
+    const int nq = min(kOrthMax, ni - q0);
     for (int yc = 64 * wv; yc < ns; yc += 256) {
       const int y = yc + lane;
       const bool in = y < ns;
@@ -1123,15 +1154,16 @@ __global__ __launch_bounds__(256) void k_bs_orth(int ns, int nzmax, const int* _
         for (int u = 0; u < nu; ++u)
 #pragma unroll
           for (int q = 0; q < kOrthMax; ++q)
-            if (q < ni) tacc[q] += Zi[static_cast<size_t>(q) * ns + x0 + u] * hv[u];
+            if (q < nq) tacc[q] += Zi[static_cast<size_t>(q0 + q) * ns + x0 + u] * hv[u];
       }
       if (in)
-        for (int q = 0; q < ni; ++q) TU[static_cast<size_t>(q) * ns + y] = tacc[q];
+        for (int q = 0; q < nq; ++q) TU[static_cast<size_t>(q0 + q) * ns + y] = tacc[q];
     }
+  }
   __syncthreads();
   // W = Z_i^T V
-  if (threadIdx.x < ni * nj) {
-    const int a = threadIdx.x / nj, b = threadIdx.x % nj;
+  for (int t = threadIdx.x; t < ni * nj; t += 256) {
+    const int a = t / nj, b = t % nj;
     double s = 0.0;
     for (int x = 0; x < ns; ++x) s += Zi[static_cast<size_t>(a) * ns + x] * V[static_cast<size_t>(b) * ns + x];
     W[a * nzmax + b] = s;
@@ -1164,10 +1196,12 @@ __global__ __launch_bounds__(256) void k_bs_orth(int ns, int nzmax, const int* _
 }
 
 // Channel-perturber vectors: hcb(:, ich, ip) <- (1 - B Z Z^T) v, one wave
-// per vector.  grid (ceil(nch*ncp/4)), block 256.
-__global__ __launch_bounds__(256) void k_bs_orth_vec(int ns, int nch, int nq, const int* __restrict__ zoff,
+// per vector; the projections s_a = z_a . v of a wave sit in LDS
+// (4 x nzmax doubles).  grid (ceil(nch*ncp/4)), block 256.
+__global__ __launch_bounds__(256) void k_bs_orth_vec(int ns, int nch, int nq, int nzmax, const int* __restrict__ zoff,
                                                      const int* __restrict__ nz, const double* __restrict__ Z,
                                                      const double* __restrict__ BZ, double* __restrict__ hcb) {
+  extern __shared__ double sl[];
   const int q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (q >= nq) return;
   const int ich = q % nch, n = nz[ich];
@@ -1175,13 +1209,15 @@ __global__ __launch_bounds__(256) void k_bs_orth_vec(int ns, int nch, int nq, co
   double* v = hcb + static_cast<size_t>(q) * ns;
   const double* Zc = Z + static_cast<size_t>(zoff[ich]) * ns;
   const double* BZc = BZ + static_cast<size_t>(zoff[ich]) * ns;
-  double s[kOrthMax];
+  double* s = sl + static_cast<size_t>(threadIdx.x >> 6) * nzmax;
   for (int a = 0; a < n; ++a) {
     double t = 0.0;
     for (int x = lane; x < ns; x += 64) t += Zc[static_cast<size_t>(a) * ns + x] * v[x];
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-    s[a] = t;
+    if (lane == 0) s[a] = t;
   }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   for (int x = lane; x < ns; x += 64) {
     double u = v[x];
     for (int a = 0; a < n; ++a) u -= BZc[static_cast<size_t>(a) * ns + x] * s[a];
@@ -1193,14 +1229,35 @@ size_t bs_orth_lds_bytes(int ns, int nzmax) {
   return (2 * static_cast<size_t>(nzmax) * ns + static_cast<size_t>(nzmax) * nzmax) * sizeof(double);
 }
 
+size_t bs_orth_scratch_doubles(int ns, int nblk, int nzmax) {
+  if (bs_orth_lds_bytes(ns, nzmax) <= 64 * 1024) return 0;
+  const size_t per = bs_orth_lds_bytes(ns, nzmax) / sizeof(double);
+  const size_t batch = std::max<size_t>(1, std::min<size_t>(nblk, (size_t(32) << 20) / per));
+  return batch * per;
+}
+
 int launch_bs_orth(int ns, int nblk, int nch, int ncp, int nzmax, const int* zoff, const int* nz, const double* Z,
-                   const double* BZ, double* hcc, double* hcb, hipStream_t s) {
-  if (nzmax < 1 || nzmax > kOrthMax) return -1;
+                   const double* BZ, double* hcc, double* hcb, double* scratch, size_t nscratch, hipStream_t s) {
+  if (nzmax < 1) return -1;
   const size_t lds = bs_orth_lds_bytes(ns, nzmax);
-  if (lds > 160 * 1024) return -2;
-  if (nblk > 0) hipLaunchKernelGGL(k_bs_orth, dim3(nblk), dim3(256), lds, s, ns, nzmax, zoff, nz, Z, BZ, hcc);
+  if (nblk > 0) {
+    if (lds <= 64 * 1024) {
+      hipLaunchKernelGGL(k_bs_orth<false>, dim3(nblk), dim3(256), lds, s, ns, nzmax, 0, zoff, nz, Z, BZ, hcc,
+                         static_cast<double*>(nullptr));
+    } else {
+      // work arrays in global scratch, blocks in batches that fit it
+      const size_t per = lds / sizeof(double);
+      const int batch = static_cast<int>(std::min<size_t>(nblk, nscratch / per));
+      if (batch < 1 || !scratch) return -2;
+      for (int b0 = 0; b0 < nblk; b0 += batch)
+        hipLaunchKernelGGL(k_bs_orth<true>, dim3(std::min(batch, nblk - b0)), dim3(256), 0, s, ns, nzmax, b0, zoff,
+                           nz, Z, BZ, hcc, scratch);
+    }
+  }
   const int nq = nch * ncp;
-  if (nq > 0) hipLaunchKernelGGL(k_bs_orth_vec, dim3((nq + 3) / 4), dim3(256), 0, s, ns, nch, nq, zoff, nz, Z, BZ, hcb);
+  if (nq > 0)
+    hipLaunchKernelGGL(k_bs_orth_vec, dim3((nq + 3) / 4), dim3(256), 4 * nzmax * sizeof(double), s, ns, nch, nq,
+                       nzmax, zoff, nz, Z, BZ, hcb);
   return 0;
 }
 

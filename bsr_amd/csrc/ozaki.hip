@@ -583,6 +583,10 @@ int ozaki8_dgemm(int m, int n, int K, const void* abuf, const double* B, size_t 
   if (S < 2 || S > 8) return fail(-9, "split precision: slices must be 2..8");
   if (m <= 0 || n <= 0 || K <= 0) return 0;
   const int mp = (m + 63) / 64 * 64, np = (n + 63) / 64 * 64, Kp = (K + 63) / 64 * 64;
+  // a level holds at most S slice products of magnitude <= 2^12 per inner
+  // index: its int32 sum is exact while S 2^12 Kp < 2^31
+  if (static_cast<long long>(S) * 4096 * Kp >= (1LL << 31))
+    return fail(-9, "split precision: inner size too large for exact int32 levels");
   const int8_t* As = static_cast<const int8_t*>(abuf);
   const double* sig = reinterpret_cast<const double*>(As + (static_cast<size_t>(S) * mp * Kp + 255) / 256 * 256);
   int8_t* Bs = static_cast<int8_t*>(bbuf);

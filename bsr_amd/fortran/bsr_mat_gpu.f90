@@ -64,7 +64,7 @@ program bsr_mat_gpu
     allocate(k1(n), k2(n), k3(n), k4(n), c(n), ipt(n))
     read(nu) k1, k2, k3, k4, c
     ipt = [(i, i = 1, n)]
-    call gpu_l_data(jtype, kpol, n, ipt, k1, k2, k3, k4, c, iitar, 0.d0)
+    call gpu_l_data(jtype, kpol, n, ipt, k1, k2, k3, k4, c, iitar, eps_c)
     deallocate(k1, k2, k3, k4, c, ipt)
   end do
   do il = 1, nlists

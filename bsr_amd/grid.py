@@ -42,6 +42,13 @@ def mkgrid(kind: int, ks: int, z: float, h: float, hmax: float, rmax: float):
     return t[: ns.value + ks].copy(), ns.value, ml.value, me.value, hh.value
 
 
+def recipe_grid(kind: int, ks: int, z: float, h: float, hmax: float, rmax: float) -> Grid:
+    """The grid the reference's recipe itself gives (kind 0 mkgrid, 1
+    mkgrid1), e.g. the knot.dat parameters of a BSR example."""
+    t, ns, ml, me, hh = mkgrid(kind, ks, z, h, hmax, rmax)
+    return Grid(ks, ns, ml, me, hh, t, z, rmax, hmax)
+
+
 def fixed_ns_grid(ks: int, ns: int, z: float, rmax: float, h: float = 0.05) -> Grid:
     """mkgrid1-shaped grid with exactly ns splines.  Tries ml = round(1/h)
     first (the requested near-origin step), then coarser ml, bisecting on

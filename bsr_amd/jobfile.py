@@ -10,8 +10,8 @@ bsr_amd/fortran/bsr_mat_gpu.f90 (little endian, no record markers):
   nlists x { int32 jtype kpol n; int32 k1(n) k2(n) k3(n) k4(n); f64 c(n) }
 
 The L-integral lists (SUB1's icase 6, after Gen_Lval with the closed core
-1..kclosd and nuclear charge z) come before the R^k lists, which appear in
-call order (k outer, jtype inner); orth = 1 applies BS_ORTH after the final
+1..kclosd and nuclear charge z) come before the R^k lists; both appear in
+call order (l or k outer, jtype inner, state_res.f90:155-159); orth = 1 applies BS_ORTH after the final
 symmetrisation.
 """
 from __future__ import annotations
@@ -21,7 +21,8 @@ import numpy as np
 
 def write(path, w, l_lists: dict | None = None, z: float = 0.0, kclosd: int = 0, iitar: int = 0,
           orth: bool = False) -> None:
-    llists = sorted((key, lst) for key, lst in (l_lists or {}).items() if len(lst))
+    # State_res generates jpol (here l) outer, jtype inner (state_res.f90:155-159)
+    llists = sorted(((key, lst) for key, lst in (l_lists or {}).items() if len(lst)), key=lambda kv: (kv[0][1], kv[0][0]))
     lists = [((jt, k), w.rk_lists[(jt, k)]) for k in range(w.mk + 1) for jt in range(1, 10)
              if (jt, k) in w.rk_lists and len(w.rk_lists[(jt, k)])]
     i4 = lambda a: np.asarray(a, "<i4").tobytes(order="F")

@@ -79,6 +79,8 @@ class Server:
         if job.footprint > self.budget:
             raise ValueError(f"job needs {job.footprint} bytes, more than the HBM budget {self.budget:.0f}")
         with self.cv:
+            if self.stopping:
+                raise RuntimeError("server is closing")
             self.queue.append(job)
             self.cv.notify_all()
         return job
@@ -86,8 +88,8 @@ class Server:
     def _dispatch(self):
         while True:
             with self.cv:
-                while not self.stopping and not (self.queue and self.free and
-                                                 self.used + self.queue[0].footprint <= self.budget):
+                while not (self.queue and self.free and self.used + self.queue[0].footprint <= self.budget) \
+                        and not (self.stopping and not self.queue):
                     self.cv.wait()
                 if self.stopping and not self.queue:
                     return
@@ -104,7 +106,9 @@ class Server:
         ctx = self.slots[s]
         while True:
             with self.cv:
-                while not self.inbox[s] and not self.stopping:
+                # on close, a worker leaves only once no job is queued: the
+                # dispatcher may still hand queued jobs to this slot
+                while not self.inbox[s] and not (self.stopping and not self.queue):
                     self.cv.wait()
                 if not self.inbox[s]:
                     return
@@ -146,6 +150,8 @@ class Server:
         return jobs
 
     def close(self):
+        """Stop accepting jobs, run every queued job to completion, then
+        release the slots."""
         with self.cv:
             self.stopping = True
             self.cv.notify_all()

@@ -160,11 +160,12 @@ def _bound_orbitals(rng, g: G.Grid, nb: int):
 
 def make_wave(seed: int, ks: int, ns: int, z: float, rmax: float, nch: int, ncp: int, mk: int,
               nb: int = 5, ntarg: int = 5, fill: float = 0.05, h: float = 0.05,
-              name: str = "wave", target_seed: int | None = None) -> PartialWave:
+              name: str = "wave", target_seed: int | None = None, grid: G.Grid | None = None) -> PartialWave:
     """target_seed, when given, draws the bound (target) orbitals from their
-    own stream, so that waves of one target share them."""
+    own stream, so that waves of one target share them.  grid, when given,
+    replaces the fixed-ns grid (ks, ns, z, rmax and h are then taken from it)."""
     rng = np.random.default_rng(seed)
-    g = G.fixed_ns_grid(ks, ns, z, rmax, h)
+    g = G.fixed_ns_grid(ks, ns, z, rmax, h) if grid is None else grid
     P, lb = _bound_orbitals(rng if target_seed is None else np.random.default_rng(target_seed), g, nb)
     lch = rng.integers(0, 4, nch)
     nbf = nb + nch

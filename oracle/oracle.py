@@ -516,9 +516,9 @@ def run_wave(job, grid: Grid | None = None, orth: bool = False, l_lists: dict | 
     w.scale(job.ec)
     if l_lists:
         w.gen_lval(z, kclosd)
-        for (jt, l), lst in sorted(l_lists.items()):
+        for (jt, l), lst in sorted(l_lists.items(), key=lambda kv: (kv[0][1], kv[0][0])):  # l outer, jtype inner
             if len(lst):
-                w.ldata(jt, l, lst, iitar=iitar)
+                w.ldata(jt, l, lst, iitar=iitar, eps_c=job.eps_c)
     for k in range(0, job.mk + 1):
         for jtype in range(1, 10):
             lst = job.rk_lists.get((jtype, k))

@@ -90,7 +90,7 @@ def test_l_then_rk_on_one_matrix(ctx, oracle_lib):
     ctx.gen_lval(4.0, 1)
     ow = O.Wave(O.Grid(w.ks, w.ns, w.ml, w.me, w.h, w.t), w)
     ow.gen_lval(4.0, 1)
-    for (jt, l), lst in sorted(synth.l_lists(w, 9, fill=0.2).items()):
+    for (jt, l), lst in sorted(synth.l_lists(w, 9, fill=0.2).items(), key=lambda kv: (kv[0][1], kv[0][0])):
         if len(lst):
             ctx.l_data(jt, l, lst, iitar=1)
             ow.ldata(jt, l, lst, iitar=1)
