@@ -21,6 +21,7 @@ program bsr_mat_gpu
   real(c_double), allocatable :: hcc(:,:,:), hcb(:,:,:), hbb(:), acf(:,:,:), htarg(:), otarg(:)
   character(512) :: fin, fout
   integer :: nu, nw, il, jtype, kpol, n, ich, i, nij
+  integer(c_int) :: kcore
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -81,7 +82,12 @@ program bsr_mat_gpu
   end do
   call gpu_check(bsrg_symmetrize_diag(gpu_ctx))
   if (orth /= 0) call gpu_check(bsrg_bs_orth(gpu_ctx, 0))
-  call gpu_check(bsrg_acf_finish(gpu_ctx, 0))
+  ! core screening of ACF(i,i,0): 2(4l+2) per closed shell (sub1.f90:218-219)
+  kcore = 0
+  do i = 1, kclosd
+    kcore = kcore + 2 * (4 * lbs(i) + 2)
+  end do
+  call gpu_check(bsrg_acf_finish(gpu_ctx, kcore))
   call gpu_check(bsrg_get_matrix(gpu_ctx, hcc, hcb, hbb, acf, htarg, otarg))
   call record_matrix(nw)
   write(nw) mk

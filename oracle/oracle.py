@@ -494,13 +494,19 @@ class Wave:
         lib().orc_wave_acf_finish(self.ptr, kcore)
 
 
+def core_acf(lbs, kclosd: int) -> int:
+    """Core-screening term added to ACF(i,i,0): sum over the closed shells
+    1..kclosd of 2(4l+2) (sub1.f90:218-219)."""
+    return int(sum(2 * (4 * int(lbs[i]) + 2) for i in range(kclosd)))
+
+
 def run_wave(job, grid: Grid | None = None, orth: bool = False, l_lists: dict | None = None, z: float = 0.0,
              kclosd: int = 0, iitar: int = 0):
     """One partial wave in the order of SUB1 (src/bsr/BSR_MAT3/sub1.f90:70-219):
     S = sb on the channel diagonals + O_data, symmetrised; H = EC*S + R^k
     contributions (jpol outer, jtype inner, state_res.f90:155-159) +
     (E_t - EC)*sb, symmetrised (then BS_ORTH when orth, sub1.f90:195); ACF
-    finished.  With l_lists, Gen_Lval (nuclear charge z, closed core
+    finished with the core term of the closed shells 1..kclosd.  With l_lists, Gen_Lval (nuclear charge z, closed core
     1..kclosd) and the L-integral lists (keyed (jtype, l)) come between the
     EC scaling and the R^k lists (sub1.f90:121-129).  Returns (S, H) as
     WaveMatrices (S.htarg/acf unused)."""
@@ -529,7 +535,7 @@ def run_wave(job, grid: Grid | None = None, orth: bool = False, l_lists: dict | 
     w.symmetrize_diag()
     if orth:
         w.bs_orth()
-    w.acf_finish(0)
+    w.acf_finish(core_acf(job.lbs, kclosd))
     H = w.get()
     return S, H
 
