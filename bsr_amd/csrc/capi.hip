@@ -872,6 +872,36 @@ int get_matrices(bsrg_ctx* c, Matrices& M, double* hcc, double* hcb, double* hbb
 
 }  // namespace
 
+// ---- R^k band of one multipole into caller memory, allocating nothing ----
+// The work arrays of the band build (moments, diagonal cells, their pair
+// sums and cell-major copies) live in one file-scope device array; each
+// device holds its own copy, whose address is looked up once per device
+// (bsrg_create) and kept in g_scratch_dev.  Every element of the work
+// arrays is written before it is read, so nothing needs clearing.
+constexpr size_t kScratchDoubles = size_t(2) << 20;  // 16 MB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[kMaxDevices] = {};
+
+// With the device current: the address of its copy of scratch.
+static int resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(-5, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_dev[device]) return 0;
+  void* ptr = nullptr;
+  HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) return fail(-100, "HIP error: no device address for the scratch array");
+  g_scratch_dev[device] = static_cast<double*>(ptr);
+  return 0;
+}
+
+static size_t band_work_doubles(int ks, int ns) {
+  const size_t nv = static_cast<size_t>(ns - ks + 1), ik = static_cast<size_t>(ks) * (ks + 1) / 2;
+  return 4 * nv * ik + 2 * nv * ik * ik + 2 * static_cast<size_t>(ks) * ns;
+}
+
 extern "C" {
 
 const char* bsrg_last_error(void) { return g_err.c_str(); }
@@ -907,6 +937,7 @@ int bsrg_create(bsrg_ctx** out, int device) {
   HIPCHK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(-5, "no such HIP device");
   HIPCHK(hipSetDevice(device));
+  if (int rc = resolve_scratch(device)) return rc;
   std::unique_ptr<bsrg_ctx> c(new bsrg_ctx());
   c->device = device;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -972,6 +1003,31 @@ int bsrg_mrk_cell(bsrg_ctx* c, int k, double* rkb) {
   const size_t n = static_cast<size_t>(c->G.ns) * c->G.ns * c->G.ks * c->G.ks;
   if (rkb) HIPCHK(hipMemcpyAsync(rkb, c->rkb.p + k * n, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int bsrg_mrk_cell_noalloc(bsrg_ctx* c, int k, double* rkb_dev, void* stream) {
+  if (int rc = check_ctx(c)) return rc;
+  if (k < 0) return fail(-1, "bsrg_mrk_cell_noalloc: k < 0");
+  if (!rkb_dev) return fail(-1, "bsrg_mrk_cell_noalloc: null output");
+  if (!c->grid_ok) return fail(-3, "grid not set (bsrg_set_grid)");
+  const int ks = c->G.ks, ns = c->G.ns, nv = c->G.nv, ik = ks * (ks + 1) / 2;
+  if (band_work_doubles(ks, ns) > kScratchDoubles)
+    return fail(-6, "bsrg_mrk_cell_noalloc: the grid needs more work space than the 16 MB scratch array");
+  HIPCHK(hipSetDevice(c->device));
+  double* w = g_scratch_dev[c->device];
+  if (!w) return fail(-100, "HIP error: scratch array not resolved for this device");
+  double* mom = w;                                         // 2 nv ik
+  double* rkd = mom + static_cast<size_t>(2) * nv * ik;    // nv ik^2
+  double* msum = rkd + static_cast<size_t>(nv) * ik * ik;  // 2 ks ns
+  double* momT = msum + static_cast<size_t>(2) * ks * ns;  // 2 nv ik
+  double* rdT = momT + static_cast<size_t>(2) * nv * ik;   // nv ik^2
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  DevGrid g = devgrid(c);
+  launch_moments(g, k, 1, mom, rkd, s);
+  launch_band(g, 1, mom, rkd, msum, momT, rdT, rkb_dev, s);
+  HIPCHK(hipGetLastError());
+  if (!stream) HIPCHK(hipStreamSynchronize(s));
   return 0;
 }
 

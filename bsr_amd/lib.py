@@ -52,6 +52,7 @@ SIGNATURES = {
     "bsrg_destroy": [C.c_void_p],
     "bsrg_set_grid": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp],
     "bsrg_mrk_cell": [C.c_void_p, C.c_int, _odp],
+    "bsrg_mrk_cell_noalloc": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
     "bsrg_rk_moments": [C.c_void_p, C.c_int, _odp, _odp, _odp],
     "bsrg_convol": [C.c_void_p, C.c_int, C.c_int, _dp, _dp],
     "bsrg_set_orbitals": [C.c_void_p, C.c_int, _dp, _ip, _ip, _ip, C.c_double],

@@ -63,6 +63,20 @@ int bsrg_set_grid(bsrg_ctx* ctx, int ks, int ns, int ml, int me, double h, const
  * src/libs/BS/mrk_cell.f90:2-82 (with rk_moments, rk_moments.f90:2-101). */
 int bsrg_mrk_cell(bsrg_ctx* ctx, int k, double* rkb);
 
+/* The R^k band of multipole k, as bsrg_mrk_cell computes it, written to
+ * device memory rkb_dev (ns*ns*ks*ks doubles, the same layout), allocating
+ * nothing: no hipMalloc, hipFree or allocating library call.  No
+ * counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops).  The grid must be set
+ * (bsrg_set_grid) beforehand.  Work arrays (moments, diagonal cells) live
+ * in a 16 MB device array of the library, one per GPU, shared by every call
+ * on that GPU: calls that overlap on one GPU race.  Grids whose work arrays
+ * exceed it (4 nv ik + 2 nv ik^2 + 2 ks ns doubles, ik = ks(ks+1)/2) are
+ * rejected (status -6).  stream: a hipStream_t to enqueue on (the call
+ * then returns without synchronising), or NULL for the context's stream,
+ * synchronised before return. */
+int bsrg_mrk_cell_noalloc(bsrg_ctx* ctx, int k, double* rkb_dev, void* stream);
+
 /* Moments and diagonal-cell integrals of multipole k, packed as the
  * reference's first ik=ks(ks+1)/2 rows: rkd1,rkd2 (ik,nv), rkd (ik,ik,nv).
  * Replaces rk_moments(k), src/libs/BS/rk_moments.f90:2-50. */
