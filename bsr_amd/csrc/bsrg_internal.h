@@ -94,7 +94,33 @@ void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, co
                      const double* bands, const int* rptr, const int* ru, const int* rw, const int* rs,
                      const double* rc, const double* stab, const double* V, double* hcc, double* bsum,
                      double* rf, hipStream_t s, const double* base = nullptr, int qnblk = 0, int qmask = 0,
-                     const unsigned char* live = nullptr);
+                     const unsigned char* live = nullptr, bool prep = true);
+// The per-block preparation of a term pass alone (k_term_prep): summed
+// band bsum of every block with band terms, factor rf of every rank-1 term.
+void launch_term_prep(int ns, int ks, int nblk, const int* bptr, const int* bid, const double* bc,
+                      const double* bands, double* bsum, const int* rptr, const int* rs, const double* rc,
+                      const double* stab, double* rf, hipStream_t s);
+// Exchange contraction fused with the term pass (xasm.hip), H overwritten:
+// launch_coef_pack turns the dense coefficient matrix C (rows 0..nr-1 of
+// columns blk, leading dimension ldc) into per-block nonzero lists (slots of
+// xasm_slot(nr) per block: xo = LDS byte offset of the basis, xc = value;
+// xn = count); launch_xasm then writes every block once with the exchange
+// product from the bases E (columns of ns^2, the nr used ones) and the band
+// terms, and adds the rank-1 terms with the term pass (nobands: nblk+1
+// zeros).  The term arena is a term set's, as for launch_assemble.
+// xasm_max_t(device): the largest nr whose LDS tile fits (64 nr doubles).
+int xasm_max_t(int device);
+int xasm_slot(int nr);
+void launch_coef_pack(int nblk, int nr, const double* C, size_t ldc, int* xo, double* xc, int* xn, hipStream_t s);
+// launch_xprod: the exchange product alone, H = E C over the packed lists,
+// every block written (the GEMM's place in the wave; the term pass follows).
+int launch_xprod(int ns, int nblk, int nr, const double* E, const int* xo, const double* xc, const int* xn,
+                 double* hcc, hipStream_t s);
+int launch_xasm(int ns, int ks, int nblk, int nr, const double* E, const int* xo, const double* xc, const int* xn,
+                const int* bptr, const int* bid, const double* bc, const double* bands, const int* rptr,
+                const int* ru, const int* rw, const int* rs, const double* rc, const double* stab, const double* V,
+                double* hcc, double* bsum, double* rf, const unsigned char* live, const int* nobands,
+                hipStream_t s);
 // hcb[q] = base + sum_r c_r * sfac_r * V[v_r]; CSR over all nq positions
 void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,
                          const double* c, const double* stab, const double* V, double* hcb, hipStream_t s);

@@ -508,9 +508,31 @@ int upload_terms(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
   return upload_arena(c, M, st);
 }
 
+// Engine 4 (the exchange contraction fused with the term pass, xasm.hip)
+// applies to an overwriting FP64 pass whose used exchange bases fit the LDS
+// tile of the kernel.
+int xasm_rows(bsrg_ctx* c, const Staged& st) {
+  return (st.gemm_khi - st.gemm_klo + 1) * c->nb * c->nb;
+}
+bool xasm_eligible(bsrg_ctx* c, const Staged& st) {
+  return c->precision_mode == 0 && st.has_gemm && xasm_rows(c, st) <= xasm_max_t(c->device);
+}
+int ensure_xasm(bsrg_ctx* c, const Matrices& M, const Staged& st) {
+  const size_t nblk = M.nblk(), n = nblk * static_cast<size_t>(xasm_slot(xasm_rows(c, st)));
+  HIPCHK(c->xt.resize(n));
+  HIPCHK(c->xc.resize(n));
+  HIPCHK(c->xn.resize(std::max<size_t>(nblk, 1)));
+  if (c->xzero.n < nblk + 1) {
+    HIPCHK(c->xzero.resize(nblk + 1));
+    HIPCHK(hipMemsetAsync(c->xzero.p, 0, c->xzero.n * sizeof(int), c->stream));
+  }
+  return 0;
+}
+
 // The exchange-contraction GEMM hcc = E^T C (+hcc): split precision in
 // mode 1, else engine 0 = rocBLAS dgemm, 1/2/3 = FP64 MFMA kernel with
-// 32/64/128-row workgroup tiles.
+// 32/64/128-row workgroup tiles; engine 4 (overwrite only) packs the
+// nonzeros of C per block for the fused pass, which does the product.
 int run_gemm(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, int engine) {
   const int nb = c->nb;
   const size_t nss = static_cast<size_t>(c->G.ns) * c->G.ns, nblk = M.nblk();
@@ -519,6 +541,16 @@ int run_gemm(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, int eng
   const int nr = (st.gemm_khi - st.gemm_klo + 1) * nb * nb;
   const double* A = c->E.p + r0 * nss;
   const double* B = st.gemm.p + r0;
+  if (engine == 4 && c->precision_mode == 0) {
+    if (overwrite && c->xn.n >= nblk && c->xt.n >= nblk * static_cast<size_t>(xasm_slot(nr)) &&
+        c->xzero.n >= nblk + 1) {
+      launch_coef_pack(static_cast<int>(nblk), nr, B, nt, c->xt.p, c->xc.p, c->xn.p, c->stream);
+      if (launch_xprod(c->G.ns, static_cast<int>(nblk), nr, A, c->xt.p, c->xc.p, c->xn.p, M.hcc.p, c->stream))
+        return fail(-100, "sparse exchange product: bad shape");
+      return 0;
+    }
+    engine = 2;  // an accumulating product: the plain GEMM
+  }
   if (c->precision_mode == 1)
     return ozaki_dgemm(static_cast<int>(nss), static_cast<int>(nblk), nr, A, nss, B, nt, M.hcc.p, nss, !overwrite,
                        c->nslices, c->ozk.p, c->stream);
@@ -547,7 +579,16 @@ std::map<std::tuple<int, int, int, int, int>, int> g_tuned;
 
 int autotune_gemm(bsrg_ctx* c, Matrices& M, Staged& st, bool capture_safe) {
   if (!st.has_gemm || c->precision_mode != 0) return 0;
-  if (c->forced_engine >= 0) { st.engine = c->forced_engine; return 0; }
+  if (c->forced_engine >= 0) {
+    st.engine = c->forced_engine;
+    if (st.engine == 4 && !xasm_eligible(c, st)) st.engine = 2;
+    if (st.engine == 4) return ensure_xasm(c, M, st);
+    return 0;
+  }
+  if (xasm_eligible(c, st)) {  // the fused pass writes H once: preferred when it fits
+    st.engine = 4;
+    return ensure_xasm(c, M, st);
+  }
   const int nr = (st.gemm_khi - st.gemm_klo + 1) * c->nb * c->nb;
   const int bucket = static_cast<int>(std::floor(std::log(static_cast<double>(M.nblk())) / std::log(1.25)));
   const auto key = std::make_tuple(c->device, c->G.ns, nr, bucket, capture_safe ? 1 : 0);
@@ -594,11 +635,38 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, Staged& st, bool overwrite, hipEvent
                   bool gemm_done = false) {
   const int ns = c->G.ns, ks = c->G.ks;
   hipStream_t s = c->stream;
-  if (st.has_gemm && !gemm_done) {
+  const bool fused = st.has_gemm && !gemm_done && st.engine == 4 && overwrite && c->precision_mode == 0;
+  if (fused) {
+    if (e0) (void)hipEventRecord(e0, s);  // no separate product phase
+    if (e1) (void)hipEventRecord(e1, s);
+  } else if (st.has_gemm && !gemm_done) {
     if (int rc = enqueue_gemm(c, M, st, overwrite, e0, e1)) return rc;
   } else if (!st.has_gemm) {
     if (e0) (void)hipEventRecord(e0, s);  // an empty GEMM phase
     if (e1) (void)hipEventRecord(e1, s);
+  }
+  if (st.has_gemm && !gemm_done && st.engine == 4 && overwrite && c->precision_mode == 0) {
+    // engine 4 with nothing enqueued yet (a prepared or captured wave): the
+    // exchange product and the band terms in one pass that writes every
+    // block once, then the rank-1 terms; a wave whose product already ran
+    // (started during the host staging) takes the term pass below
+    launch_coef_pack(st.nblk, xasm_rows(c, st), st.gemm.p + static_cast<size_t>(st.gemm_klo) * c->nb * c->nb,
+                     static_cast<size_t>(c->basis_nk) * c->nb * c->nb, c->xt.p, c->xc.p, c->xn.p, s);
+    c->last_engine = 4;
+    const int nb = c->nb;
+    const size_t r0 = static_cast<size_t>(st.gemm_klo) * nb * nb, nss = static_cast<size_t>(ns) * ns;
+    const int xr = launch_xasm(ns, ks, st.nblk, xasm_rows(c, st), c->E.p + r0 * nss, c->xt.p, c->xc.p, c->xn.p,
+                               st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid), st.dp<double>(st.o_bc), c->bands.p,
+                               st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw), st.dp<int>(st.o_rs),
+                               st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p,
+                               (&M == &c->S ? c->bsumS : c->bsum).p, (&M == &c->S ? c->rfacS : c->rfac).p,
+                               st.has_live && !c->capturing ? st.dp<unsigned char>(st.o_live) : nullptr,
+                               c->xzero.p, s);
+    if (xr) return fail(-100, "fused exchange pass: bad shape");
+    launch_vec_assemble(ns, st.nq, 0, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid), st.dp<int>(st.o_vs),
+                        st.dp<double>(st.o_vc), c->St.p, c->VT.p, M.hcb.p, s);
+    HIPCHK(hipGetLastError());
+    return 0;
   }
   // after a GEMM, or when accumulating, the pass adds to the blocks and
   // leaves blocks without terms alone; an overwriting pass without a GEMM
@@ -1867,7 +1935,7 @@ int bsrg_set_capture_safe(bsrg_ctx* c, int on) {
 
 int bsrg_set_gemm_engine(bsrg_ctx* c, int engine) {
   if (int rc = check_ctx(c)) return rc;
-  if (engine < -1 || engine > 3) return fail(-1, "bsrg_set_gemm_engine: engine must be -1..3");
+  if (engine < -1 || engine > 4) return fail(-1, "bsrg_set_gemm_engine: engine must be -1..4");
   c->forced_engine = engine;
   return 0;
 }

@@ -377,6 +377,10 @@ struct bsrg_ctx {
   int last_engine = -1, forced_engine = -1;
   DevVec<double> dtmp, dtmp2;
   DevVec<double> bsum, rfac;  // term-pass scratch: summed band of each block, factor of each rank-1 term
+  // engine 4 (exchange fused with the term pass): nonzeros of each block's
+  // exchange coefficients, slots of nr per block
+  DevVec<int> xt, xn, xzero;
+  DevVec<double> xc;
   DevVec<double> bsumS, rfacS;  // the same for S, whose pass may be queued while H's scratch grows
 
   // Dirac B-spline grid and the resident bands of the four families

@@ -978,13 +978,23 @@ void launch_sb_upper(int ns, int ks, const double* sb, double* sbu, hipStream_t 
   hipLaunchKernelGGL(k_sb_upper, dim3(nblocks(static_cast<size_t>(ns) * ks, 256)), dim3(256), 0, s, ns, ks, sb, sbu);
 }
 
-void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, const int* bid, const double* bc,
-                     const double* bands, const int* rptr, const int* ru, const int* rw, const int* rs,
-                     const double* rc, const double* stab, const double* V, double* hcc, double* bsum,
-                     double* rf, hipStream_t s, const double* base, int qnblk, int qmask, const unsigned char* live) {
+void launch_term_prep(int ns, int ks, int nblk, const int* bptr, const int* bid, const double* bc,
+                      const double* bands, double* bsum, const int* rptr, const int* rs, const double* rc,
+                      const double* stab, double* rf, hipStream_t s) {
   if (nblk <= 0) return;
   hipLaunchKernelGGL(k_term_prep, dim3(nblk, (ns * ks + 255) / 256), dim3(256), 0, s, ns, ks, bptr, bid, bc, bands,
                      bsum, rptr, rs, rc, stab, rf);
+}
+
+void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, const int* bid, const double* bc,
+                     const double* bands, const int* rptr, const int* ru, const int* rw, const int* rs,
+                     const double* rc, const double* stab, const double* V, double* hcc, double* bsum,
+                     double* rf, hipStream_t s, const double* base, int qnblk, int qmask, const unsigned char* live,
+                     bool prep) {
+  if (nblk <= 0) return;
+  if (prep)
+    hipLaunchKernelGGL(k_term_prep, dim3(nblk, (ns * ks + 255) / 256), dim3(256), 0, s, ns, ks, bptr, bid, bc, bands,
+                       bsum, rptr, rs, rc, stab, rf);
   const int A = ns <= 64 ? 1 : (ns <= 128 ? 2 : (ns <= 192 ? 3 : 4));
   const dim3 grid(nblk, (ns + kAsmCols - 1) / kAsmCols, (ns + 64 * A - 1) / (64 * A));
   switch (A) {

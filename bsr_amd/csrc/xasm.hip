@@ -1,0 +1,224 @@
+// The exchange contraction over each block's nonzero coefficients (engine 4
+// of the FP64 wave).
+//
+// H[blk](x,y) = sum_t C(t, blk) E_t(x,y)              exchange (jtype 8, 9)
+//
+// The coefficient matrix C (nt x nblk, nt = nk nb^2) is sparse: a record of
+// jtype 8/9 touches one multipole and one ordered bound pair of one channel
+// pair, so a block holds a few tens of the nt possible t (under a tenth at
+// the production shape).  A dense product spends most of its FLOPs on
+// zeros; here each block runs over its own nonzeros only.
+//
+// Layout: a workgroup owns 64 consecutive positions p = x + ns y of the
+// ns^2 block (one 512-byte row tile of every block) and a range of blocks.
+// It stages the tile of every used basis E_t, [t][64] doubles, in LDS once
+// and then runs over its blocks, one wavefront per block: lane l sums
+// c_j E_{t_j}(p0 + l) over the block's nonzeros and writes its element:
+// 512 coalesced bytes per block and tile, and no read of H.
+//
+// Two forms:
+//   launch_xprod  the product alone, in the GEMM's place: a one-shot wave
+//                 starts it as soon as the exchange records are decoded and
+//                 the term pass adds the band and rank-1 terms afterwards;
+//   launch_xasm   a wave whose product was not started early (prepared or
+//                 captured waves): the band terms join the product in the
+//                 same pass (H written once), then the term pass adds only
+//                 the rank-1 terms to the blocks that have them.
+//
+// Sums: the exchange part in ascending t (four interleaved partial sums
+// added pairwise), then the band term; the rank-1 terms in list order: a
+// fixed order, so a block's values do not depend on what else runs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "bsrg_internal.h"
+
+namespace bsrg {
+
+namespace {
+
+constexpr int kXWaves = 16;  // wavefronts per workgroup (4 per SIMD)
+constexpr int kXChunk = 8;  // list entries per scalar-load round
+
+// Per block (one wavefront): the nonzeros of column blk of C (rows 0..nr-1,
+// leading dimension ldc) in ascending t, stored as (byte offset t*512 of
+// E_t's tile in LDS, c) in slots blk*nrs.. (nrs = nr rounded up to
+// kXChunk), padded with (0, 0) up to the next multiple of kXChunk; their
+// count in xn[blk].
+__global__ __launch_bounds__(256) void k_coef_pack(int nblk, int nr, int nrs, const double* __restrict__ C, size_t ldc,
+                                                   int* __restrict__ xo, double* __restrict__ xc,
+                                                   int* __restrict__ xn) {
+  const int blk = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (blk >= nblk) return;
+  const double* col = C + static_cast<size_t>(blk) * ldc;
+  int* ol = xo + static_cast<size_t>(blk) * nrs;
+  double* cl = xc + static_cast<size_t>(blk) * nrs;
+  int base = 0;
+  for (int t0 = 0; t0 < nr; t0 += 64) {
+    const int t = t0 + lane;
+    const double v = t < nr ? col[t] : 0.0;
+    const bool nz = v != 0.0;
+    const unsigned long long m = __ballot(nz);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (nz) {
+      ol[base + before] = t * 64 * static_cast<int>(sizeof(double));
+      cl[base + before] = v;
+    }
+    base += __popcll(m);
+  }
+  const int pad = (base + kXChunk - 1) / kXChunk * kXChunk;
+  if (base + lane < pad) {
+    ol[base + lane] = 0;
+    cl[base + lane] = 0.0;
+  }
+  if (lane == 0) xn[blk] = base;
+}
+
+// grid ntile * ngroup: workgroup -> (tile = id % ntile, group = id / ntile);
+// group g owns blocks [g*per, min(nblk, (g+1)*per)), per = ceil(nblk/ngroup).
+// E: the used bases, column t at E + t*nss.  LDS: nr * 64 doubles.
+// The lists of a block are wave-uniform: kXChunk entries at a time come in
+// through scalar loads and serve as scalar operands, so the inner loop is
+// one address add, one LDS read and one FMA per nonzero (the padding adds
+// zeros).  The band term (one read of the block's summed band, for the
+// elements within ks-1 of the diagonal) is requested before the sum.
+template <bool BAND>
+__global__ __launch_bounds__(64 * kXWaves) void k_xasm(
+    int ns, int ks, int nblk, int nr, int nrs, int ntile, int ngroup, const double* __restrict__ E,
+    const int* __restrict__ xo, const double* __restrict__ xc, const int* __restrict__ xn,
+    const int* __restrict__ bptr, const int* __restrict__ rptr, const double* __restrict__ bsum,
+    const unsigned char* __restrict__ live, double* __restrict__ hcc) {
+  extern __shared__ double El[];
+  const size_t nss = static_cast<size_t>(ns) * ns;
+  const int tile = static_cast<int>(blockIdx.x % ntile), grp = static_cast<int>(blockIdx.x / ntile);
+  const size_t p0 = static_cast<size_t>(tile) * 64;
+  {
+    // stage the tile: eight loads in flight per thread
+    const int tot = nr * 64;
+    for (int q0 = threadIdx.x; q0 < tot; q0 += 8 * 64 * kXWaves) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = q0 + u * 64 * kXWaves, t = q >> 6;
+        const size_t pp = p0 + (q & 63);
+        v[u] = (q < tot && pp < nss) ? E[static_cast<size_t>(t) * nss + pp] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = q0 + u * 64 * kXWaves;
+        if (q < tot) El[q] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const size_t p = p0 + lane;
+  const bool valid = p < nss;
+  const int x = valid ? static_cast<int>(p % ns) : 0, y = valid ? static_cast<int>(p / ns) : 0;
+  const int d = x - y, ad = d < 0 ? -d : d;
+  const bool inband = ad < ks;
+  const size_t bi = static_cast<size_t>(min(x, y)) + static_cast<size_t>(ns) * ad;
+  const char* Eb = reinterpret_cast<const char*>(El + lane);
+  const int per = (nblk + ngroup - 1) / ngroup, blo = grp * per, bhi = min(nblk, blo + per);
+  for (int blk = blo + wv; blk < bhi; blk += kXWaves) {
+    const int n = xn[blk];
+    double band = 0.0;
+    if (BAND) {
+      const int b0 = bptr[blk], b1 = bptr[blk + 1];
+      // a block without terms that already holds zeros stays as it is
+      if (n == 0 && b0 == b1 && rptr[blk] == rptr[blk + 1] && live && !live[blk]) continue;
+      band = (b1 > b0 && inband) ? bsum[static_cast<size_t>(blk) * ns * ks + bi] : 0.0;
+    }
+    const int* ol = xo + static_cast<size_t>(blk) * nrs;
+    const double* cl = xc + static_cast<size_t>(blk) * nrs;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j0 = 0; j0 < n; j0 += kXChunk) {
+      int oo[kXChunk];
+      double cc[kXChunk];
+#pragma unroll
+      for (int i = 0; i < kXChunk; ++i) {
+        oo[i] = ol[j0 + i];
+        cc[i] = cl[j0 + i];
+      }
+#pragma unroll
+      for (int i = 0; i < kXChunk; ++i) a[i & 3] += cc[i] * *reinterpret_cast<const double*>(Eb + oo[i]);
+    }
+    const double acc = ((a[0] + a[1]) + (a[2] + a[3])) + band;
+    if (valid) hcc[static_cast<size_t>(blk) * nss + p] = acc;
+  }
+}
+
+}  // namespace
+
+int xasm_max_t(int device) {
+  // the E tile of every used basis must fit the LDS a workgroup may use
+  static std::mutex mu;
+  static int lds_bytes[64] = {};
+  if (device < 0 || device >= 64) return 0;
+  std::lock_guard<std::mutex> g(mu);
+  if (!lds_bytes[device]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || v <= 0) {
+      (void)hipGetLastError();
+      v = 64 * 1024;
+    }
+    lds_bytes[device] = v;
+  }
+  return lds_bytes[device] / (64 * static_cast<int>(sizeof(double)));
+}
+
+int xasm_slot(int nr) { return (nr + kXChunk - 1) / kXChunk * kXChunk; }
+
+void launch_coef_pack(int nblk, int nr, const double* C, size_t ldc, int* xo, double* xc, int* xn, hipStream_t s) {
+  if (nblk <= 0) return;
+  hipLaunchKernelGGL(k_coef_pack, dim3((nblk + 3) / 4), dim3(256), 0, s, nblk, nr, xasm_slot(nr), C, ldc, xo, xc,
+                     xn);
+}
+
+namespace {
+void xasm_grid(int ns, int nblk, int& ntile, int& ngroup) {
+  const size_t nss = static_cast<size_t>(ns) * ns;
+  ntile = static_cast<int>((nss + 63) / 64);
+  // about six workgroups per CU in all (one resident at a time when the
+  // tile needs most of the LDS), at least one block per wavefront
+  ngroup = std::max(1, (6 * 256 + ntile - 1) / ntile);
+  ngroup = std::min(ngroup, std::max(1, nblk / kXWaves));
+}
+}  // namespace
+
+int launch_xprod(int ns, int nblk, int nr, const double* E, const int* xo, const double* xc, const int* xn,
+                 double* hcc, hipStream_t s) {
+  if (nblk <= 0) return 0;
+  if (nr < 1) return -1;
+  int ntile, ngroup;
+  xasm_grid(ns, nblk, ntile, ngroup);
+  hipLaunchKernelGGL(k_xasm<false>, dim3(static_cast<unsigned>(ntile) * ngroup), dim3(64 * kXWaves),
+                     static_cast<size_t>(nr) * 64 * sizeof(double), s, ns, 0, nblk, nr, xasm_slot(nr), ntile, ngroup,
+                     E, xo, xc, xn, nullptr, nullptr, nullptr, nullptr, hcc);
+  return 0;
+}
+
+int launch_xasm(int ns, int ks, int nblk, int nr, const double* E, const int* xo, const double* xc, const int* xn,
+                const int* bptr, const int* bid, const double* bc, const double* bands, const int* rptr,
+                const int* ru, const int* rw, const int* rs, const double* rc, const double* stab, const double* V,
+                double* hcc, double* bsum, double* rf, const unsigned char* live, const int* nobands,
+                hipStream_t s) {
+  if (nblk <= 0) return 0;
+  if (nr < 1) return -1;
+  launch_term_prep(ns, ks, nblk, bptr, bid, bc, bands, bsum, rptr, rs, rc, stab, rf, s);
+  int ntile, ngroup;
+  xasm_grid(ns, nblk, ntile, ngroup);
+  hipLaunchKernelGGL(k_xasm<true>, dim3(static_cast<unsigned>(ntile) * ngroup), dim3(64 * kXWaves),
+                     static_cast<size_t>(nr) * 64 * sizeof(double), s, ns, ks, nblk, nr, xasm_slot(nr), ntile,
+                     ngroup, E, xo, xc, xn, bptr, rptr, bsum, live, hcc);
+  // the rank-1 terms: added by the term pass to the blocks that have them
+  // (no band terms: nobands is an all-zero pointer array)
+  launch_assemble(ns, ks, nblk, 1, nobands, bid, bc, bands, rptr, ru, rw, rs, rc, stab, V, hcc, bsum, rf, s, nullptr,
+                  0, 0, nullptr, /*prep=*/false);
+  return 0;
+}
+
+}  // namespace bsrg

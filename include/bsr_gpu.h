@@ -309,12 +309,14 @@ int bsrg_wave_launch(bsrg_ctx* ctx);
 int bsrg_wave_finish(bsrg_ctx* ctx);
 
 /* Engine of the last exchange contraction: 0 library FP64 GEMM, 1-3 FP64
- * MFMA kernel (32/64/128-row tiles), 10+S bf16 split precision and 20+S
- * int8 split precision with S slices. */
+ * MFMA kernel (32/64/128-row tiles), 4 FP64 product over each block's
+ * nonzero coefficients fused with the term pass (H written once), 10+S
+ * bf16 split precision and 20+S int8 split precision with S slices. */
 int bsrg_wave_engine(bsrg_ctx* ctx);
 
-/* Force the FP64 exchange-contraction engine (0..3 as above) or -1 to let
- * the autotuner choose (default). */
+/* Force the FP64 exchange-contraction engine (0..4 as above) or -1 to let
+ * the library choose (default: engine 4 when the wave's exchange bases fit
+ * its LDS tile, else the autotuner's fastest of 0..3; 1..4 when capture-safe). */
 int bsrg_set_gemm_engine(bsrg_ctx* ctx, int engine);
 
 /* Host time (ms) of the last prepare: [0] checks and band/basis launches,

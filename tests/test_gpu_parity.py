@@ -337,10 +337,11 @@ def test_split_precision_wave(ctx, oracle_lib, mode, nslices):
     assert de <= 1e-9
 
 
-@pytest.mark.parametrize("engine", [0, 1, 2, 3])
+@pytest.mark.parametrize("engine", [0, 1, 2, 3, 4])
 def test_fp64_gemm_engines(ctx, oracle_lib, engine):
-    """Each FP64 engine of the exchange contraction (library GEMM and the
-    three MFMA kernel tilings) against the oracle."""
+    """Each FP64 engine of the exchange contraction (library GEMM, the three
+    MFMA kernel tilings, the product over each block's nonzero coefficients
+    fused with the term pass) against the oracle."""
     O = oracle_lib
     from bsr_amd import synth
     w = synth.make_wave(81, 8, 90, 18.0, 40.0, nch=7, ncp=5, mk=4, nb=5, ntarg=4, fill=0.15, name="eng")
@@ -353,6 +354,22 @@ def test_fp64_gemm_engines(ctx, oracle_lib, engine):
         ctx.set_gemm_engine(-1)
     oS, oH = O.run_wave(w)
     assert rel_err(H.hcc, oH.hcc) < TOL
+    assert rel_err(H.hcb, oH.hcb) < TOL
+
+
+def test_fused_engine_is_the_default_and_deterministic(ctx):
+    """With the default choice the FP64 wave uses engine 4 when its bases fit
+    the LDS tile; two runs give the same bits, and the per-block result does
+    not depend on the block count (a wave with one channel fewer gives the
+    same leading blocks bit for bit, the blocks' sums being independent)."""
+    from bsr_amd import synth
+    w = synth.make_wave(82, 8, 120, 18.0, 40.0, nch=9, ncp=6, mk=6, nb=5, ntarg=4, fill=0.1, name="eng4")
+    ctx.wave_assemble(w)
+    assert ctx.wave_engine() == 4
+    H1 = ctx.wave_get(1)
+    ctx.wave_assemble(w, set_inputs=False)
+    H2 = ctx.wave_get(1)
+    assert np.array_equal(H1.hcc, H2.hcc) and np.array_equal(H1.hcb, H2.hcb)
 
 
 def test_batch_graph(oracle_lib):
