@@ -9,6 +9,17 @@ k, orbital contractions, list processing, and the S and H blocks complete on
 the device, in plain FP64.  value = unique H elements nhm(nhm+1)/2 per second.
 
 Beside the headline line's required fields, the JSON line carries:
+  roofline the exchange contraction (the step's largest kernel; by default
+           the product over each block's nonzero coefficients) against the
+           HBM bound: algorithmic bytes (bases read once, nonzero list, H
+           written once) over its event-timed duration, with the counter
+           traffic of the same kernel;
+  step_hbm the step's algorithmic bytes (bands read once, S and H written
+           once, lists, orbitals) over the step time, against 8 TB/s;
+  latency  the device-resident step, and the extra time to copy S and H to
+           the host and to write bsr_mat.nnn;
+  fp64_engines  the same step with the FP64 product forced to the library
+           GEMM (0), the MFMA kernel (2) and the default (4);
   eigen    the lowest 10 eigenvalues of H x = E S x of the wave (FP64 on
            the device) and the time of that solve;
   split    the same wave with the exchange contraction in split precision
@@ -37,7 +48,11 @@ Beside the headline line's required fields, the JSON line carries:
            the oracle on an 8-channel sample of the same shape (time and the
            largest element error of the GPU result on that sample); and a
            stream of 6 C5-shaped waves served on 2 slots, split precision
-           (7 int8 slices) and FP64, against the plain FP64 wave.
+           (7 int8 slices) and FP64, against the plain FP64 wave;
+  cpu_baseline  the oracle (the reference's loop structure, FP64) on a
+           10-channel, 50-perturber sample of the headline wave, on all host
+           cores (processes owning blocks round robin, as BSR_MAT3_MPI
+           ranks do) and on one core, and on the full C1 wave.
 """
 from __future__ import annotations
 
@@ -51,29 +66,124 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64_MFMA = 78.6   # TFLOP/s, MI355X dense FP64 matrix
+PEAK_FP64_VECTOR = 78.6  # TFLOP/s, MI355X FP64 vector
 PEAK_HBM = 8000.0       # GB/s
-# HBM bytes of one launch of the library FP64 exchange GEMM at the headline
-# shape (read 1.51 GB + write 0.60 GB): two rocprofv3 counter runs
-# (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB device copy in the same process
-GEMM_TRAFFIC_BYTES = 2.105e9
+PEAK_INT8_MFMA = 5000.0  # TOP/s, MI355X dense int8 matrix
+# HBM bytes of one launch of the sparse exchange product (k_xasm<false>) at
+# the headline shape: two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE)
+# scaled by a 1 GiB device copy in the same process (read 308.1 MB, write
+# 585.6 MB)
+XPROD_TRAFFIC_BYTES = 893.7e6
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline():
-    """The oracle (1 thread, the reference's loop structure) on a bounded
-    sample of the same workload: same grid, k range, fill and orbital set,
-    10 channels and 50 perturbers instead of 60 and 400."""
+def _owner_split(w, nparts):
+    """The wave's records dealt to nparts owners by output location, round
+    robin over channel-channel blocks as BSR_MAT3_MPI deals them
+    (src/bsr/BSR_MAT3_MPI/mod_intmat_mpi.f90:62-67), channel-perturber
+    vectors and perturber pairs likewise; one sub-wave per owner."""
+    import copy
+    import numpy as np
     from bsr_amd import synth
+    IBI, IBO = synth.IBI, synth.IBO
+    iech = np.asarray(w.iech)
+    nblk = w.nch * (w.nch + 1) // 2
+
+    def blk(i, j):
+        i, j = np.maximum(i, j), np.minimum(i, j)
+        return i * (i - 1) // 2 + j - 1
+
+    def owner_rk(jt, L):
+        if jt >= 6:
+            return blk(L.k1 // IBI, L.k1 % IBI)
+        if jt >= 2:
+            ch = L.k3
+            jch = np.where(L.k4 > 0, iech[np.maximum(L.k4 // IBO, 1) - 1], 0)
+            return np.where(L.k4 > 0, blk(ch, np.maximum(jch, 1)), nblk + (ch - 1) * max(w.ncp, 1) - L.k4 - 1)
+        return _owner_struct1(L.k3, L.k4)
+
+    def _owner_struct1(k3, k4):
+        ic, jc = -k3, -k4
+        out = np.zeros(len(k3), np.int64)
+        pp = (ic > 0) & (jc > 0)
+        out[pp] = nblk + w.nch * max(w.ncp, 1) + np.maximum(ic[pp], jc[pp])
+        cv = (ic < 0) & (jc > 0)
+        out[cv] = nblk + (iech[(-ic[cv]) // IBO - 1] - 1) * max(w.ncp, 1) + jc[cv] - 1
+        cc = (ic < 0) & (jc < 0)
+        out[cc] = blk(iech[(-ic[cc]) // IBO - 1], iech[(-jc[cc]) // IBO - 1])
+        tg = (ic < 0) & (jc == 0)
+        out[tg] = blk(iech[(-ic[tg]) // IBO - 1], iech[(-ic[tg]) % IBO - 1])
+        return out
+
+    parts = []
+    for r in range(nparts):
+        sub = copy.copy(w)
+        sub.__dict__.pop("_packed", None)
+        sub.rk_lists = {}
+        for key, L in w.rk_lists.items():
+            sel = owner_rk(key[0], L) % nparts == r
+            sub.rk_lists[key] = synth.RecordList(L.k1[sel], L.k2[sel], L.k3[sel], L.k4[sel], L.c[sel])
+        o = w.olist
+        sel = _owner_struct1(o.k3, o.k4) % nparts == r
+        sub.olist = synth.RecordList(o.k1[sel], o.k2[sel], o.k3[sel], o.k4[sel], o.c[sel])
+        parts.append(sub)
+    return parts
+
+
+def _oracle_run(job):
     from oracle import oracle as O
-    w = synth.c2(nch=10, ncp=50)
     t0 = time.perf_counter()
-    O.run_wave(w)
-    dt = time.perf_counter() - t0
-    return dict(value=w.n_elements / dt, unit="elements/s", cores=1, kind="port",
-                sample=f"nch=10 ncp=50 ns=200 k<=10 (nhm={w.nhm}); {dt:.1f} s")
+    O.run_wave(job)
+    return time.perf_counter() - t0
+
+
+def cpu_time(w, cores):
+    """Seconds of the oracle (the reference's loop structure, FP64) on wave w:
+    one process, or `cores` processes each owning a share of the output
+    blocks (every process builds its own R^k bands, as MPI ranks do); the
+    wall time of the slowest."""
+    if cores <= 1:
+        return _oracle_run(w)
+    import multiprocessing as mp
+    parts = _owner_split(w, cores)
+    with mp.get_context("fork").Pool(cores) as pool:
+        t0 = time.perf_counter()
+        pool.map(_oracle_run, parts)
+        return time.perf_counter() - t0
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline():
+    """The oracle on a bounded sample of the headline workload (same grid, k
+    range, fill and orbitals; 10 channels and 50 perturbers instead of 60
+    and 400) on all host cores, block ownership as BSR_MAT3_MPI; beside it
+    the same sample on one core, and the full C1 wave (ks=8, ns=60, 20
+    channels) on one core and on all cores."""
+    from bsr_amd import synth
+    cores = host_cores()
+    w = synth.c2(nch=10, ncp=50)
+    dt_all = cpu_time(w, cores)
+    dt_one = cpu_time(w, 1)
+    c1 = synth.c1()
+    c1_one = cpu_time(c1, 1)
+    c1_all = cpu_time(c1, cores)
+    return dict(value=w.n_elements / dt_all, unit="elements/s", cores=cores, kind="port",
+                sample=f"nch=10 ncp=50 ns=200 k<=10 (nhm={w.nhm}), {cores} processes owning blocks round robin: "
+                       f"{dt_all:.2f} s; one core {dt_one:.2f} s",
+                one_core_elements_per_s=w.n_elements / dt_one,
+                c1_full={"nhm": c1.nhm, "one_core_s": c1_one, "one_core_elements_per_s": c1.n_elements / c1_one,
+                         "all_cores_s": c1_all, "all_cores_elements_per_s": c1.n_elements / c1_all,
+                         "cores": cores})
 
 
 def copy_calibration(nbytes):
@@ -115,6 +225,33 @@ def _orth_conditions(w):
                 n += 1
         out.append(n)
     return out if any(out) else []
+
+
+def exchange_nonzeros(w):
+    """Nonzeros of the exchange coefficient matrix C(t, blk), t = (k, a, b):
+    the distinct (block, multipole, ordered bound pair) of the jtype 8/9
+    records (I_data.f90:192-216 contracts each group of one channel pair)."""
+    import numpy as np
+    from bsr_amd import synth
+    nb, keys = w.nb, []
+    for (jt, k), L in w.rk_lists.items():
+        if jt < 8 or len(L) == 0:
+            continue
+        ich, jch = L.k1 // synth.IBI, L.k1 % synth.IBI
+        i, j = np.maximum(ich, jch), np.minimum(ich, jch)
+        a, b = L.k2.astype(np.int64) - 1, L.k3.astype(np.int64) - 1
+        t = k * nb * nb + np.where(ich >= jch, a * nb + b, b * nb + a)
+        keys.append((i * (i - 1) // 2 + j - 1).astype(np.int64) * 100000 + t)
+    return int(len(np.unique(np.concatenate(keys)))) if keys else 0
+
+
+def step_bytes(w):
+    """Algorithmic HBM bytes of one wave: every R^k band read once, S and H
+    written once, the lists and the orbitals read."""
+    ns, ks, nch, ncp = w.ns, w.ks, w.nch, w.ncp
+    bands = 8 * ns * ns * ks * ks * (w.mk + 1)
+    mats = 2 * 8 * (nch * (nch + 1) // 2 * ns * ns + nch * ncp * ns + ncp * (ncp + 1) // 2)
+    return bands + mats + 24 * w.n_records() + 8 * ns * (w.nb + ncp)
 
 
 def run_steps(ctx, w, warmup, steps):
@@ -348,8 +485,14 @@ def main():
     nss = w.ns * w.ns
     nblk = w.nch * (w.nch + 1) // 2
     nt = (w.mk + 1) * w.nb * w.nb
-    flops = 2.0 * nss * nblk * nt
-    ach = flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    nnz = exchange_nonzeros(w)
+    # the exchange contraction (the largest kernel of the step): algorithmic
+    # bytes = the used exchange bases read once + the nonzero list + every H
+    # block written once
+    xbytes = 8.0 * nss * nt + 12.0 * nnz + 8.0 * nss * nblk
+    ach = xbytes / (gemm_ms * 1e-3) / 1e9 if gemm_ms > 0 else 0.0
+    xflops = 2.0 * nss * nnz
+    sbytes = step_bytes(w)
     out = {
         "metric": "matrix elems/s FP64 vs split-prec @<=1e-10 rel err; p50/p99 partial-wave latency",
         "value": w.n_elements / (ms * 1e-3),
@@ -365,14 +508,44 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"partial wave ks=8 ns=200 k<=10 nch={w.nch} ncp={w.ncp} nhm={w.nhm}",
                    "device_ms_per_step": dev["total"], "device_ms": dev, "host_ms": host, "terms": terms},
-        "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_MFMA, "unit": "TFLOP/s",
-                     "frac": ach / PEAK_FP64_MFMA,
-                     "traffic": GEMM_TRAFFIC_BYTES if engine == 0 and (w.nch, w.ncp) == (60, 400) else None,
-                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of the library GEMM at this shape, "
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM, "unit": "GB/s", "frac": ach / PEAK_HBM,
+                     "traffic": XPROD_TRAFFIC_BYTES if engine == 4 and (w.nch, w.ncp) == (60, 400) else None,
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of k_xasm<false> at this shape, "
                                        "copy-calibrated: profiles/wave_c2_hbm_traffic.md",
-                     "kernel": "exchange contraction GEMM (FP64)", "engine": engine, "kernel_ms": gemm_ms,
-                     "flops_per_launch": flops},
+                     "kernel": "exchange contraction over the nonzero coefficients, k_xasm<false> (engine 4)",
+                     "engine": engine, "kernel_ms": gemm_ms, "algorithmic_bytes": xbytes,
+                     "coefficient_nonzeros": nnz, "coefficient_density": nnz / (nt * nblk),
+                     "fp64_flops": xflops,
+                     "fp64_tflops": xflops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0,
+                     "dense_gemm_flops": 2.0 * nss * nblk * nt},
+        "step_hbm": {"algorithmic_bytes": sbytes, "GB_s": sbytes / (ms * 1e-3) / 1e9,
+                     "frac": sbytes / (ms * 1e-3) / 1e9 / PEAK_HBM,
+                     "device_GB_s": sbytes / (dev["total"] * 1e-3) / 1e9},
     }
+    # the S and H of the wave to host memory and to a bsr_mat file: the
+    # latency a caller sees beyond the device-resident result
+    t0 = time.perf_counter()
+    ctx.wave_get(0)
+    ctx.wave_get(1)
+    d2h = (time.perf_counter() - t0) * 1e3
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        t0 = time.perf_counter()
+        ctx.wave_write(os.path.join(td, "bsr_mat.001"), w)
+        fms = (time.perf_counter() - t0) * 1e3
+    out["latency"] = {"device_resident_ms": ms, "d2h_ms": d2h, "file_ms": fms}
+    log(f"latency: step {ms:.2f} ms, +D2H {d2h:.1f} ms, +file {fms:.1f} ms")
+    # the same step with each FP64 product engine forced: library GEMM (0),
+    # MFMA kernel with 64-row tiles (2), and the default sparse product (4)
+    eng = {}
+    for e in (0, 2):
+        ctx.set_gemm_engine(e)
+        ems, edev, _ = run_steps(ctx, w, 2, max(3, a.steps // 2))
+        eng[str(e)] = {"ms_per_step": ems, "product_ms": edev["gemm"], "terms_ms": edev["terms"]}
+    ctx.set_gemm_engine(-1)
+    eng[str(engine)] = {"ms_per_step": ms, "product_ms": gemm_ms, "terms_ms": dev["terms"]}
+    out["fp64_engines"] = eng
+    log("engines: " + json.dumps(eng))
     e_ref = None
     if not a.no_eigen:
         # lowest 10 eigenvalues of H x = E S x (FP64 on the device), the
@@ -392,6 +565,15 @@ def main():
             log(f"{key}: {sms:.3f} ms/step, GEMM {sdev['gemm']:.3f} ms, max rel element error vs FP64 {err:.2e}")
             out[key] = {"value": w.n_elements / (sms * 1e-3), "ms_per_step": sms, "gemm_ms": sdev["gemm"],
                         "engine": sctx.wave_engine(), "max_rel_err_vs_fp64": err, "vs_fp64_plain": ms / sms}
+            if mode == 2 and sdev["gemm"] > 0:
+                # int8 slice products issued by the split GEMM: S(S+1)/2 per
+                # level pair over the 64-padded inner size
+                S = sctx.wave_engine() - 20
+                kp = (nt + 63) // 64 * 64
+                ops = S * (S + 1) // 2 * 2.0 * nss * ((nblk + 63) // 64 * 64) * kp
+                rate = ops / (sdev["gemm"] * 1e-3) / 1e12
+                out[key]["mfma"] = {"int8_tops": rate, "peak": PEAK_INT8_MFMA, "frac": rate / PEAK_INT8_MFMA,
+                                    "counters": "profiles/mfma_utilisation_c2.md"}
             if e_ref is not None and key == "split":
                 de = float(np.max(np.abs(sctx.wave_eigen(10) - e_ref)))
                 out[key]["max_eig_err_vs_fp64"] = de
