@@ -154,6 +154,84 @@ class Context:
         L.check(L.lib().bsrg_dirac_timings(self._p, out))
         return out
 
+    # ---- channel-block steps of the Dirac wave (overlap, diag, transform) -----
+    def dirac_overlap(self, nch, npert, ntarg, iptar, itype, k1, k2, c, eps_c=1e-10, iitar=0):
+        """Overlap pass: fppqq on the channel diagonal blocks, then O_data
+        over the overlap records (itype per record)."""
+        n = len(c)
+        z = np.zeros(1, np.int32)
+        L.check(L.lib().bsrg_dirac_overlap(self._p, nch, npert, ntarg, i32(iptar), n,
+                                           *(i32(a) if n else z for a in (itype, k1, k2)),
+                                           f64(c) if n else np.zeros(1), float(eps_c), int(iitar)))
+        self.dirac_sdims = (nch, npert, ntarg)
+
+    def dirac_overlap_wave(self, w, eps_c=1e-10, iitar=0):
+        """Overlap pass of a synthetic Dirac wave (its olists)."""
+        ls = [L_ for L_ in w.olists if len(L_)]
+        it = np.concatenate([np.full(len(L_), L_.itype, np.int32) for L_ in ls]) if ls else np.zeros(0, np.int32)
+        cat = [np.concatenate([getattr(L_, a) for L_ in ls]) if ls else np.zeros(0) for a in ("k1", "k2", "c")]
+        self.dirac_overlap(w.nch, w.npert, w.ntarg, w.iptar, it, *cat, eps_c=eps_c, iitar=iitar)
+
+    def dirac_overlap_get(self):
+        """S: hch(ms,ms,nblk), hcp(ms,nch*npert), hp, otarg."""
+        ns = self.dbs[1]
+        nch, npert, ntarg = self.dirac_sdims
+        ms, nblk = 2 * ns, nch * (nch + 1) // 2
+        hch = np.zeros((ms, ms, nblk), order="F")
+        hcp = np.zeros((ms, max(nch * npert, 1)), order="F")
+        hp = np.zeros(max(npert * (npert + 1) // 2, 1))
+        otarg = np.zeros(ntarg * (ntarg + 1) // 2)
+        L.check(L.lib().bsrg_dirac_overlap_get(self._p, hch, hcp, hp, otarg))
+        return dict(hch=hch, hcp=hcp[:, : nch * npert], hp=hp[: npert * (npert + 1) // 2], otarg=otarg)
+
+    def dirac_update_hx(self, which, d, ich, jch, imat, c):
+        """UPDATE_HX records: block (ich,jch) of S (0) or H (1) += c * d[:,:,imat-1]."""
+        d = f64(np.asarray(d, np.float64).reshape(d.shape[0], d.shape[1], -1))
+        n = len(c)
+        z = np.zeros(1, np.int32)
+        L.check(L.lib().bsrg_dirac_update_hx(self._p, which, d.shape[2], d, n,
+                                             *(i32(a) if n else z for a in (ich, jch, imat)),
+                                             f64(c) if n else np.zeros(1)))
+
+    def dirac_energy_shift(self, ecore, etarg):
+        L.check(L.lib().bsrg_dirac_energy_shift(self._p, float(ecore), f64(etarg)))
+
+    def dirac_diag_channels(self, lch, kch, ilzero=-1, jlzero=-1, ibzero=1, jbzero=1,
+                            edmin=-2 * 137.035999084 ** 2, edmax=1e10, egap=1e-3):
+        """Diag_channels after Zero_cond; returns ipsol (nch,)."""
+        nch = len(lch)
+        ipsol = np.zeros(nch, np.int32)
+        ns = C.c_int()
+        L.check(L.lib().bsrg_dirac_diag_channels(self._p, i32(lch), i32(kch), ilzero, jlzero, ibzero, jbzero,
+                                                 float(edmin), float(edmax), float(egap), ipsol, C.byref(ns)))
+        self._ipsol = ipsol
+        return ipsol
+
+    def dirac_solutions(self):
+        """diag(ms,ms,nch), eval(nsol), iprm(nch*ms)."""
+        ms = 2 * self.dbs[1]
+        nch = len(self._ipsol)
+        diag = np.zeros((ms, ms, nch), order="F")
+        ev = np.zeros(max(int(self._ipsol.sum()), 1))
+        iprm = np.zeros(nch * ms, np.int32)
+        L.check(L.lib().bsrg_dirac_solutions(self._p, diag, ev, iprm))
+        return diag, ev[: int(self._ipsol.sum())], iprm
+
+    def dirac_transform(self, which):
+        L.check(L.lib().bsrg_dirac_transform(self._p, int(which)))
+
+    def dirac_overlaps(self):
+        nch, npert, _ = self.dirac_sdims
+        out = np.zeros((nch + npert, nch + npert), order="F")
+        L.check(L.lib().bsrg_dirac_overlaps(self._p, out))
+        return out
+
+    def dirac_diag_timings(self):
+        """Device ms: overlap pass, channel diagonalisation, S and H transforms."""
+        out = np.zeros(4)
+        L.check(L.lib().bsrg_dirac_diag_timings(self._p, out))
+        return out
+
     def dbs_convol(self, family, k, d, icase):
         ks, ns = self.dbs[0], self.dbs[1]
         shape = (ns, ks) if icase <= 2 else (ns, ns)

@@ -2141,6 +2141,7 @@ int bsrg_dbs_set_grid(bsrg_ctx* c, int ks, int ns, int ksp, int ksq, const doubl
   c->dw.orb_ok = false;
   c->dw.basis_nk = c->dw.exch_nk = 0;
   c->dw.ok = false;
+  c->dw.s_ok = c->dw.sol_ok = c->dw.ovl_ok = false;
   return 0;
 }
 

@@ -294,6 +294,28 @@ struct DiracWave {
   bool fused = false;  // the last wave's GEMMs wrote the output layout directly (int8 split)
   double ms[5] = {0, 0, 0, 0, 0};
   double host_ms[3] = {0, 0, 0};  // list decoding, staging and uploads, whole call
+  // overlap pass: S channel blocks, channel-perturber vectors (device),
+  // perturber and target overlaps (host); the B-spline overlaps fppqq and
+  // V_ch on the device; term lists of the block passes
+  bool s_ok = false;
+  int s_nch = 0, s_npert = 0, s_ntarg = 0;
+  std::vector<int> s_iptar;
+  DevVec<double> fpq, vchd, shch, shcp, dmat;
+  std::vector<double> shp, sotarg;
+  DevVec<int> ti, tb;
+  DevVec<int4> tk;
+  DevVec<double> tcd;
+  DevVec<const double*> mp;
+  // channel solutions diag(ms,ms,nch), their counts, the kept B-splines
+  bool sol_ok = false;
+  DevVec<double> sol, dwork, bmax;
+  DevVec<int> keepd, posd, ksold, isold, iwork;
+  std::vector<int> ipsol, iprm;
+  // largest transformed overlaps (nch+npert)^2 of the last S transform
+  bool ovl_ok = false;
+  int ovl_n = 0;
+  std::vector<double> ovl;
+  double dms[4] = {0, 0, 0, 0};  // overlap pass, channel diagonalisation, S and H transforms
 };
 
 }  // namespace bsrg

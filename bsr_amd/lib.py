@@ -25,9 +25,8 @@ class BsrGpuError(RuntimeError):
         self.code = code
 
 
-def _opt_dp():
+def _opt_dp(base=_dp):
     """ndpointer that also accepts None (NULL)."""
-    base = _dp
 
     class _Opt(base):
         @classmethod
@@ -40,6 +39,7 @@ def _opt_dp():
 
 
 _odp = _opt_dp()
+_oip = _opt_dp(_ip)
 
 # name -> argtypes (all return int status unless listed in _RESTYPE)
 SIGNATURES = {
@@ -91,6 +91,17 @@ SIGNATURES = {
     "bsrg_dirac_timings": [C.c_void_p, _dp],
     "bsrg_dirac_engine": [C.c_void_p],
     "bsrg_dirac_diff": [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double)],
+    "bsrg_dirac_overlap": [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _ip, _ip, _ip, _dp, C.c_double,
+                           C.c_int],
+    "bsrg_dirac_overlap_get": [C.c_void_p, _odp, _odp, _odp, _odp],
+    "bsrg_dirac_update_hx": [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, _ip, _ip, _ip, _dp],
+    "bsrg_dirac_energy_shift": [C.c_void_p, C.c_double, _dp],
+    "bsrg_dirac_diag_channels": [C.c_void_p, _ip, _ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                 C.c_double, _oip, C.POINTER(C.c_int)],
+    "bsrg_dirac_solutions": [C.c_void_p, _odp, _odp, _oip],
+    "bsrg_dirac_transform": [C.c_void_p, C.c_int],
+    "bsrg_dirac_overlaps": [C.c_void_p, _dp],
+    "bsrg_dirac_diag_timings": [C.c_void_p, _dp],
     "bsrg_wave_prepare": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
                           C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, _dp, _ip],
     "bsrg_wave_launch": [C.c_void_p],

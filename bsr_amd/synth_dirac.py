@@ -78,6 +78,17 @@ class DiracWave:
     iptar: np.ndarray          # (nch,)
     lists: list = field(default_factory=list)
     _cat: tuple | None = None
+    # the overlap pass (O_data lists, kpol 0, K3 = K4 = 0), the channel
+    # quantum numbers l and kappa, core and target energies
+    olists: list = field(default_factory=list)
+    lch: np.ndarray | None = None
+    kch: np.ndarray | None = None
+    ecore: float = 0.0
+    etarg: np.ndarray | None = None
+    # one-electron matrices hd(ms,ms,nkap) of the channel kappas and the
+    # L(ij) records (ich, jch, index into hd, c) that add them to H
+    hd: np.ndarray | None = None
+    lij: tuple | None = None
 
     @property
     def ms(self): return 2 * self.ns
@@ -222,7 +233,90 @@ def make_dirac_wave(seed: int, ks: int, ns: int, ksp: int, ksq: int, z: float, r
             cols = [np.concatenate([p_[i] for p_ in parts]) if parts else np.zeros(0, np.int64) for i in range(4)]
             rl = RecordList.build(*cols, rng.uniform(-1.0, 1.0, len(cols[0])))
             w.lists.append(DiracList(itype, k, rl.k1, rl.k2, rl.k3, rl.k4, rl.c))
+    add_overlap_pass(w, fill)
     return w
+
+
+def add_overlap_pass(w: DiracWave, fill: float) -> None:
+    """The overlap lists of the wave (O_data structures, O_data.f90:7-12),
+    the channel kappa and l, and the energies, from their own stream
+    (default_rng([seed, 1])):
+
+      kappa       ~ U{-4..-1, 1..4}, l = kappa for kappa > 0, -kappa-1 else
+      itype 1     every perturber pair ic >= jc w.p. fill, c ~ U[-0.05,0.05],
+                  1 on the diagonal
+      itype 2     every (overlap, perturber) pair w.p. fill, c ~ U[-0.05,0.05]
+      itype 3     every overlap io with itself w.p. 1/2, c ~ U(0, 0.05] (a
+                  positive rank-1 term keeps the channel's S block positive
+                  definite), and every ordered pair of overlaps of different
+                  channels w.p. fill, c ~ U[-0.05,0.05]
+      itype 4     every channel-orbital pair (ich >= jch): c = 1 on the
+                  diagonal, w.p. fill c ~ U[-0.05,0.05] off it
+    Ecore ~ U[-120,-80], Etarg(it) = Ecore + U[0,2]; per distinct kappa a
+    one-electron matrix hd: symmetric, N(0,1) within 8 of the diagonal (a
+    stand-in for the Dirac operator that spreads the channel spectra), and
+    an L(ij) record (ich, ich, hd of its kappa, 1) per channel."""
+    rng = np.random.default_rng([w.seed, 1])
+    nch, npert, nb = w.nch, w.npert, w.nb
+    kap = rng.integers(1, 5, nch) * rng.choice([-1, 1], nch)
+    w.kch = kap.astype(np.int32)
+    w.lch = np.where(kap > 0, kap, -kap - 1).astype(np.int32)
+    w.ecore = float(rng.uniform(-120.0, -80.0))
+    w.etarg = w.ecore + rng.uniform(0.0, 2.0, w.ntarg)
+    ov = [int(i) * IBO + int(j) for i, j in zip(w.i_ch, w.j_ch)]
+    och = [int(w.ipbs[int(i) - 1]) for i in w.i_ch]
+
+    def small(n):
+        return rng.uniform(-0.05, 0.05, n)
+
+    lists = []
+    # itype 1: perturber pairs
+    k1, k2, c = [], [], []
+    for ic in range(1, npert + 1):
+        for jc in range(1, ic + 1):
+            if ic == jc:
+                k1.append(ic), k2.append(jc), c.append(1.0)
+            elif rng.random() < fill:
+                k1.append(ic), k2.append(jc), c.append(float(small(1)[0]))
+    lists.append((1, np.asarray(k1, np.int64), np.asarray(k2, np.int64), np.asarray(c)))
+    # itype 2: overlap x perturber
+    k1, k2 = [], []
+    for o in ov:
+        for ic in range(1, npert + 1):
+            if rng.random() < fill:
+                k1.append(o), k2.append(ic)
+    lists.append((2, np.asarray(k1, np.int64), np.asarray(k2, np.int64), small(len(k1))))
+    # itype 3: overlap x overlap
+    k1, k2, c = [], [], []
+    for a, (o, ca) in enumerate(zip(ov, och)):
+        if rng.random() < 0.5:
+            k1.append(o), k2.append(o), c.append(float(rng.uniform(1e-3, 0.05)))
+        for o2, cb in zip(ov, och):
+            if cb != ca and rng.random() < fill:
+                k1.append(o), k2.append(o2), c.append(float(small(1)[0]))
+    lists.append((3, np.asarray(k1, np.int64), np.asarray(k2, np.int64), np.asarray(c)))
+    # itype 4: channel-orbital pairs (target structure)
+    k1, c = [], []
+    for ich in range(1, nch + 1):
+        for jch in range(1, ich + 1):
+            if ich == jch:
+                k1.append((nb + ich) * IBO + nb + jch), c.append(1.0)
+            elif rng.random() < fill:
+                k1.append((nb + ich) * IBO + nb + jch), c.append(float(small(1)[0]))
+    lists.append((4, np.asarray(k1, np.int64), np.zeros(len(k1), np.int64), np.asarray(c)))
+    kaps = sorted(set(int(k) for k in w.kch))
+    ms = w.ms
+    hd = np.zeros((ms, ms, len(kaps)), order="F")
+    band = np.abs(np.subtract.outer(np.arange(ms), np.arange(ms))) <= 8
+    for m in range(len(kaps)):
+        x = rng.standard_normal((ms, ms)) * band
+        hd[:, :, m] = (x + x.T) / 2
+    w.hd = hd
+    ich = np.arange(1, nch + 1, dtype=np.int32)
+    w.lij = (ich, ich.copy(), np.asarray([kaps.index(int(k)) + 1 for k in w.kch], np.int32), np.ones(nch))
+    for itype, k1, k2, c in lists:
+        rl = RecordList.build(k1, k2, np.zeros(len(c)), np.zeros(len(c)), c)
+        w.olists.append(DiracList(itype, 0, rl.k1, rl.k2, rl.k3, rl.k4, rl.c))
 
 
 def small(seed: int = 11, npert: int = 3) -> DiracWave:

@@ -165,6 +165,64 @@ int bsrg_dirac_engine(bsrg_ctx* ctx);
  * counterpart in the reference. */
 int bsrg_dirac_diff(bsrg_ctx* a, bsrg_ctx* ref, double floor_frac, double* rel);
 
+/* ---- the channel-block steps of DBSR_MAT3's SUB1 ----------------------------
+ * S (the overlap matrix) lives beside H in the context, in the same layout.
+ *
+ * Overlap pass: S from zero, fppqq (the P and Q B-spline overlaps,
+ * src/libs/DBS/DBS_gauss.f90:256-280) on every channel diagonal block
+ * (src/dbsr/DBSR_MAT3_MPI/sub1.f90:77-78), then O_data (O_data.f90:2-90)
+ * over the overlap records in list order: itype 1 <ic|jc> (K1=ic, K2=jc),
+ * 2 <i|.> jc (K1=io, K2=jc), 3 <i|.><.|j> (K1=io, K2=jo), 4 <i|j> (K1=io)
+ * with io = i1*2^15 + i2 as in module c_data; records with |c| < eps_c are
+ * skipped; iitar > 1 adds c*fppqq for itype 4 off the diagonal.  Errors as
+ * the reference's stops (-10 range, -12 KBORT=0, -21 unknown itype).
+ * Results: shch(ms,ms,nblk), shcp(ms,nch*npert), shp, otarg. */
+int bsrg_dirac_overlap(bsrg_ctx* ctx, int nch, int npert, int ntarg, const int* iptar, int nrec, const int* itype,
+                       const int* k1, const int* k2, const double* cdata, double eps_c, int iitar);
+/* Copy S to host arrays in the layout of bsrg_dirac_get (NULL skips one);
+ * otarg(ntarg(ntarg+1)/2). */
+int bsrg_dirac_overlap_get(bsrg_ctx* ctx, double* hch, double* hcp, double* hp, double* otarg);
+
+/* UPDATE_HX (updates.f90:79-106) for nrec records in order: block (ich,jch)
+ * of S (which = 0) or H (which = 1) += c * D_m (D_m^T when ich < jch),
+ * D = d(ms,ms,nmat), m = imat[r] (1-based).  The update L_data makes for
+ * its L(ij) records with the one-electron matrices hd(:,:,kappa)
+ * (L_data.f90:112-128). */
+int bsrg_dirac_update_hx(bsrg_ctx* ctx, int which, int nmat, const double* d, int nrec, const int* ich,
+                         const int* jch, const int* imat, const double* c);
+
+/* H_ii += Ecore S_ii + (Etarg(iptar(i)) - Ecore) fppqq for every channel:
+ * the core and target energies of the diagonal blocks (sub1.f90:96,
+ * :128-130).  etarg(ntarg). */
+int bsrg_dirac_energy_shift(bsrg_ctx* ctx, double ecore, const double* etarg);
+
+/* Diag_channels (diag.f90:2-81) after Zero_cond (zero_cond.f90:2-56): per
+ * channel, H_ii c = E S_ii c on the B-splines kept (lch, kch: l and kappa of
+ * each channel; ilzero, jlzero < 0 take the defaults from l and kappa;
+ * ibzero, jbzero deleted at the boundary); solutions with E in [edmin,
+ * edmax] and |E| >= egap, largest component made positive.  The reference's
+ * defaults: ilzero = jlzero = -1, ibzero = jbzero = 1, edmin = -2 c^2,
+ * edmax = 1e10, egap = 1e-3.  ipsol(nch) (may be NULL), *nsol = sum. */
+int bsrg_dirac_diag_channels(bsrg_ctx* ctx, const int* lch, const int* kch, int ilzero, int jlzero, int ibzero,
+                             int jbzero, double edmin, double edmax, double egap, int* ipsol, int* nsol);
+/* The solutions: diag(ms,ms,nch) as the reference holds it (column k <
+ * ipsol(ich) solution k, energies in column ms, the rest of S_ii), eval(nsol)
+ * in channel order (sub1.f90:148-152), iprm(nch*ms) of Zero_cond; any may
+ * be NULL. */
+int bsrg_dirac_solutions(bsrg_ctx* ctx, double* diag, double* eval, int* iprm);
+
+/* transform_matrix (transform_matrix.f90:2-66) of S (which = 0) or H
+ * (which = 1) with the last solutions: off-diagonal blocks with an element
+ * of at least 1e-20 become C_i^T H_ij C_j in their top-left isol x jsol
+ * corner and zeros elsewhere; vectors likewise C_i^T v.  For S also the
+ * overlaps matrix (nch+npert)^2 the reference's Check_mat reads. */
+int bsrg_dirac_transform(bsrg_ctx* ctx, int which);
+/* overlaps(nch+npert, nch+npert) of the last S transform. */
+int bsrg_dirac_overlaps(bsrg_ctx* ctx, double* ov);
+/* Device ms of the last overlap pass, channel diagonalisation, S and H
+ * transforms. */
+int bsrg_dirac_diag_timings(bsrg_ctx* ctx, double* ms4);
+
 /* Orbitals of the wave: pbs(ns,nbf); iech(i)=channel of orbital i (0 for
  * bound orbitals); lbs(i)=l; ibort(nbf,nbf) nonzero where the overlap
  * <kl|nl> of channel orbital kl and bound orbital nl is kept.

@@ -30,8 +30,9 @@ SRCS := \
 
 OBJS := $(addprefix $(OUT)/,$(notdir $(SRCS:.f90=.o)))
 
-# Dirac B-spline routines and the Dirac R^k assembly of DBSR_MAT3 (R_data,
-# updates, Target_h with the modules they read): a library of their own
+# Dirac B-spline routines and the Dirac assembly of DBSR_MAT3 (R_data,
+# O_data, updates, Target_h, Zero_cond, transform_matrix with the modules
+# they read): a library of their own
 # (routine names such as convol and density repeat those of the BS library)
 DSRCS := \
   $(REF)/libs/ZCOM/MOD_zconst.f90 $(REF)/libs/ZCOM/read_par.f90 $(REF)/libs/ZCOM/rrtc.f90 \
@@ -49,7 +50,8 @@ DSRCS := \
   $(REF)/libs/ZCONFJJ/mod_conf_jj.f90 $(REF)/libs/ZCONFJJ/mod_channel_jj.f90 \
   $(REF)/dbsr/DBSR_MAT3_MPI/mod_dbsr_mat.f90 $(REF)/dbsr/DBSR_MAT3_MPI/MOD_c_data.f90 \
   $(REF)/dbsr/DBSR_MAT3_MPI/updates.f90 $(REF)/dbsr/DBSR_MAT3_MPI/target_subs.f90 \
-  $(REF)/dbsr/DBSR_MAT3_MPI/R_data.f90
+  $(REF)/dbsr/DBSR_MAT3_MPI/R_data.f90 $(REF)/dbsr/DBSR_MAT3_MPI/O_data.f90 \
+  $(REF)/dbsr/DBSR_MAT3_MPI/zero_cond.f90 $(REF)/dbsr/DBSR_MAT3_MPI/transform_matrix.f90
 DOUT := $(OUT)/dbs
 DOBJS := $(addprefix $(DOUT)/,$(notdir $(DSRCS:.f90=.o)))
 

@@ -1,8 +1,9 @@
 !======================================================================
 !  ref_dbsr_driver.f90 -- TEST INFRASTRUCTURE ONLY.
 !
-!  C-callable entry points over the reference's own Dirac R^k assembly
-!  (src/dbsr/DBSR_MAT3_MPI: R_data, the UPDATE_* routines, Target_h), which
+!  C-callable entry points over the reference's own Dirac assembly
+!  (src/dbsr/DBSR_MAT3_MPI: R_data, O_data, the UPDATE_* routines,
+!  Target_h, Zero_cond, transform_matrix), which
 !  ref.mk compiles together with the DBS library into
 !  oracle/_ref/libdbs_ref.so.  This file holds no reference code: it fills
 !  the module state that R_data reads (orbitals pq, ipbs, the <.|p> vectors
@@ -84,4 +85,95 @@ subroutine ref_dbsr_get(hch_o, hcp_o, hp_o, acf_o, htarg_o) bind(C, name="ref_db
   end if
   nt = ntarg * (ntarg + 1) / 2
   if (nt > 0) htarg_o(1:nt) = htarg(1:nt)
+end subroutine
+
+subroutine ref_dbsr_odata(itype, n, k1in, k2in, cin, eps, iitar_) bind(C, name="ref_dbsr_odata")
+  ! the overlap pass of SUB1 (icase 0): O_data(itype) on one list
+  use iso_c_binding
+  use dbsr_mat
+  use c_data
+  implicit none
+  integer(c_int), value :: itype, n, iitar_
+  real(c_double), value :: eps
+  integer(c_int) :: k1in(n), k2in(n)
+  real(c_double) :: cin(n)
+  integer :: i
+  if (allocated(K1)) deallocate(K1, K2, K3, K4, IPT, CDATA)
+  allocate(K1(n), K2(n), K3(n), K4(n), IPT(n), CDATA(n))
+  K1 = k1in; K2 = k2in; K3 = 0; K4 = 0; CDATA = cin
+  do i = 1, n
+    IPT(i) = i
+  end do
+  ncdata = n
+  eps_c = eps
+  iitar = iitar_
+  call O_data(itype)
+end subroutine
+
+subroutine ref_dbsr_diag_fppqq() bind(C, name="ref_dbsr_diag_fppqq")
+  ! the B-spline overlaps of the channel diagonal blocks (sub1.f90:78)
+  use dbsr_mat
+  use DBS_gauss
+  implicit none
+  integer :: ich
+  do ich = 1, nch
+    call UPDATE_HX(ich, ich, fppqq, 1.d0)
+  end do
+end subroutine
+
+subroutine ref_dbsr_get_otarg(otarg_o) bind(C, name="ref_dbsr_get_otarg")
+  use iso_c_binding
+  use dbsr_mat
+  implicit none
+  real(c_double) :: otarg_o(*)
+  integer :: nt
+  nt = ntarg * (ntarg + 1) / 2
+  if (nt > 0) otarg_o(1:nt) = otarg(1:nt)
+end subroutine
+
+subroutine ref_dbsr_zero_cond(lch_in, kch_in, ilz, jlz, ibz, jbz, iprm_o, nhm_o) bind(C, name="ref_dbsr_zero_cond")
+  ! Zero_cond for the channels of the current wave, channel l and kappa given
+  use iso_c_binding
+  use dbsr_mat
+  use channel_jj, only: lch, kch
+  implicit none
+  integer(c_int) :: lch_in(*), kch_in(*), iprm_o(*), nhm_o
+  integer(c_int), value :: ilz, jlz, ibz, jbz
+  if (allocated(lch)) deallocate(lch)
+  if (allocated(kch)) deallocate(kch)
+  allocate(lch(nch), kch(nch))
+  lch = lch_in(1:nch); kch = kch_in(1:nch)
+  ilzero = ilz; jlzero = jlz; ibzero = ibz; jbzero = jbz
+  mhm = nch * ms + npert
+  call Zero_cond
+  iprm_o(1:nch*ms) = iprm(1:nch*ms)
+  nhm_o = nhm
+end subroutine
+
+subroutine ref_dbsr_transform(icase_, diag_in, ipsol_in) bind(C, name="ref_dbsr_transform")
+  ! transform_matrix on the module matrices with the given channel solutions
+  use iso_c_binding
+  use dbsr_mat
+  implicit none
+  integer(c_int), value :: icase_
+  real(c_double) :: diag_in(ms, ms, nch)
+  integer(c_int) :: ipsol_in(nch)
+  if (allocated(diag)) deallocate(diag)
+  allocate(diag(ms, ms, nch)); diag = diag_in
+  if (allocated(ipsol)) deallocate(ipsol)
+  allocate(ipsol(0:nch)); ipsol(0) = 0; ipsol(1:nch) = ipsol_in
+  icase = icase_
+  call transform_matrix
+end subroutine
+
+subroutine ref_dbsr_get_overlaps(n, ov_o) bind(C, name="ref_dbsr_get_overlaps")
+  ! the largest transformed overlaps transform_matrix records for icase 0,
+  ! overlaps(nch+npert, nch+npert)
+  use iso_c_binding
+  use dbsr_mat
+  implicit none
+  integer(c_int), value :: n
+  real(c_double) :: ov_o(n, n)
+  ov_o = 0.d0
+  if (allocated(overlaps)) ov_o(1:nch+npert, 1:nch+npert) = overlaps(1:nch+npert, 1:nch+npert)
 end subroutine

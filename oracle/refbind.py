@@ -288,6 +288,12 @@ def dbs_lib():
             "ref_dbsr_setup": (None, [i, _dp, _ip, i, _ip, _ip, _dp, i, i, i, i, _ip, _ip]),
             "ref_dbsr_rdata": (None, [i, i, i, _ip, _ip, _ip, _ip, _dp]),
             "ref_dbsr_get": (None, [_dp, _dp, _dp, _dp, _dp]),
+            "ref_dbsr_odata": (None, [i, i, _ip, _ip, _dp, C.c_double, i]),
+            "ref_dbsr_diag_fppqq": (None, []),
+            "ref_dbsr_get_otarg": (None, [_dp]),
+            "ref_dbsr_zero_cond": (None, [_ip, _ip, i, i, i, i, _ip, _ip]),
+            "ref_dbsr_transform": (None, [i, _dp, _ip]),
+            "ref_dbsr_get_overlaps": (None, [i, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -355,6 +361,41 @@ class RefDbsr:
         dbs_lib().ref_dbsr_rdata(itype, kpol, len(c), _i32(k1), _i32(k2), _i32(k3), _i32(k4),
                                  np.asfortranarray(c, dtype=np.float64))
 
+    def diag_fppqq(self):
+        """UPDATE_HX(ich,ich,fppqq,1) for every channel (sub1.f90:78)."""
+        dbs_lib().ref_dbsr_diag_fppqq()
+
+    def odata(self, itype, k1, k2, c, eps_c=1e-10, iitar=0):
+        """O_data(itype) on one overlap list (O_data.f90:2-90)."""
+        n = len(c)
+        if n == 0:
+            return
+        dbs_lib().ref_dbsr_odata(itype, n, _i32(k1), _i32(k2), np.asfortranarray(c, dtype=np.float64),
+                                 float(eps_c), int(iitar))
+
+    def otarg(self):
+        out = np.zeros(max(self.ntarg * (self.ntarg + 1) // 2, 1))
+        dbs_lib().ref_dbsr_get_otarg(out)
+        return out[: self.ntarg * (self.ntarg + 1) // 2]
+
+    def zero_cond(self, lch, kch, ilzero=-1, jlzero=-1, ibzero=1, jbzero=1):
+        """Zero_cond (zero_cond.f90:2-56): iprm(nch*ms) and nhm."""
+        nch = len(lch)
+        iprm = np.zeros(nch * self.ms, np.int32)
+        nhm = np.zeros(1, np.int32)
+        dbs_lib().ref_dbsr_zero_cond(_i32(lch), _i32(kch), ilzero, jlzero, ibzero, jbzero, iprm, nhm)
+        return iprm, int(nhm[0])
+
+    def transform(self, icase, diag, ipsol):
+        """transform_matrix (transform_matrix.f90:2-66) on the module matrices."""
+        dbs_lib().ref_dbsr_transform(int(icase), np.asfortranarray(diag, dtype=np.float64), _i32(ipsol))
+
+    def overlaps(self, nch):
+        n = nch + self.npert
+        out = np.zeros((n, n), order="F")
+        dbs_lib().ref_dbsr_get_overlaps(n, out)
+        return out
+
     def get(self):
         ms = self.ms
         hch = np.zeros((ms, ms, max(self.iicc, 1)), order="F")
@@ -375,3 +416,18 @@ def dirac_assemble(w):
     for L in w.lists:
         r.rdata(L.itype, L.kpol, L.k1, L.k2, L.k3, L.k4, L.c)
     return r.get()
+
+
+def dirac_overlap(w, eps_c=1e-10, iitar=0, keep=False):
+    """Reference overlap pass of a synthetic Dirac wave: fppqq on the channel
+    diagonal blocks, then O_data on every overlap list (sub1.f90:78-79).
+    keep=True also returns the RefDbsr that holds the result."""
+    RefDbs(w.ks, w.ns, w.ksp, w.ksq, w.t)
+    r = RefDbsr(w.pq, w.ipbs, w.i_ch, w.j_ch, w.v_ch, w.nch, w.npert, w.mk, w.ntarg, w.iptar)
+    r.diag_fppqq()
+    for L in w.olists:
+        r.odata(L.itype, L.k1, L.k2, L.c, eps_c, iitar)
+    out = r.get()
+    out.pop("acf"), out.pop("htarg")
+    out["otarg"] = r.otarg()
+    return (out, r) if keep else out
