@@ -102,6 +102,19 @@ SIGNATURES = {
     "bsrg_dirac_transform": [C.c_void_p, C.c_int],
     "bsrg_dirac_overlaps": [C.c_void_p, _dp],
     "bsrg_dirac_diag_timings": [C.c_void_p, _dp],
+    "bsrg_lb_create": [],
+    "bsrg_lb_destroy": [C.c_void_p],
+    "bsrg_lb_read_bank": [C.c_void_p, C.c_char_p, _ip],
+    "bsrg_lb_bank_get": [C.c_void_p, _dp, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip],
+    "bsrg_lb_set_records": [C.c_void_p, C.c_int, _dp, _ip, _ip, _ip],
+    "bsrg_lb_set_dets": [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _ip, C.c_int, C.c_int, _ip, _ip, _ip],
+    "bsrg_lb_set_orbitals": [C.c_void_p, C.c_int, _ip, _ip, _ip, _dp, C.c_int],
+    "bsrg_lb_set_channels": [C.c_void_p, C.c_int, _ip, C.c_int, _ip],
+    "bsrg_lb_set_states": [C.c_void_p, C.c_int, _ip, _ip, _dp, C.c_int, _ip, C.c_int, _ip, _ip, C.c_int, _ip],
+    "bsrg_lb_set_params": [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int],
+    "bsrg_lb_run": [C.c_void_p, C.c_int, C.c_int, _ip],
+    "bsrg_lb_expansion": [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip, _dp],
+    "bsrg_lb_lists": [C.c_void_p, _ip, _ip, _ip, _ip, _ip, _ip, _ip, _dp],
     "bsrg_wave_prepare": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
                           C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, _dp, _ip],
     "bsrg_wave_launch": [C.c_void_p],
@@ -116,7 +129,7 @@ SIGNATURES = {
     "bsrg_graph_sync": [C.c_void_p, _odp, C.POINTER(C.c_double)],
     "bsrg_graph_destroy": [C.c_void_p],
 }
-_RESTYPE = {"bsrg_last_error": C.c_char_p}
+_RESTYPE = {"bsrg_last_error": C.c_char_p, "bsrg_lb_create": C.c_void_p, "bsrg_lb_destroy": None}
 
 _LIB = None
 

@@ -263,6 +263,60 @@ int bsrg_cdata_build(int n, const int* j1, const int* j2, const int* j3, const i
 int bsrg_cdata_merge(int nn, int* ip, const int* jp, int n, const int* k1, const int* k2, const int* k3,
                      const int* k4, double* cdata, double eps_c, int* ipt, int* nc);
 
+/* ---- list builder: int_bnk records -> cmdata lists (host) -------------------
+ * The record loop of State_res (src/bsr/BSR_MAT3/state_res.f90:2-161) for one
+ * icase, on a builder object holding the module state that loop reads.  The
+ * relativistic J-dependent cases 7-10 (which read the term tables) are
+ * rejected (-44); the reference's stops come back as errors with its
+ * message (-45..-48). */
+typedef struct bsrg_lb bsrg_lb;
+bsrg_lb* bsrg_lb_create(void);
+void bsrg_lb_destroy(bsrg_lb* lb);
+/* Read an int_bnk file: the configuration headers (read_symc_LS,
+ * Read_symt_LS, Read_oper_LS, src/libs/ZCONFLS/MOD_symc.f90:175-195,
+ * MOD_symt.f90:224-336, skipped), the determinants and overlap factors
+ * (Read_dets, src/bsr/BSR_MAT3/mod_dets.f90:50-86), then the coefficient
+ * records (c, ijt, int, idf) to the end (state_res.f90:37-41).  Fortran
+ * unformatted sequential, 4-byte record markers.  sizes5 = ndet, kdet,
+ * ndef, kdef, nrec.  -40 malformed, -41 cannot open. */
+int bsrg_lb_read_bank(bsrg_lb* lb, const char* path, int* sizes5);
+/* The bank as read (any pointer may be NULL). */
+int bsrg_lb_bank_get(bsrg_lb* lb, double* c, int* ijt, int* intb, int* idf, int* kpd, int* ipd, int* npd, int* kpf,
+                     int* ipf, int* npf);
+/* The same data given directly (module dets: KPD, IPD, NPD, KPF, IPF, NPF,
+ * mod_dets.f90:2-34; 0-based pointers ipd/ipf as stored there). */
+int bsrg_lb_set_records(bsrg_lb* lb, int nrec, const double* c, const int* ijt, const int* intb, const int* idf);
+int bsrg_lb_set_dets(bsrg_lb* lb, int ndet, int kdet, const int* kpd, const int* ipd, const int* npd, int ndef,
+                     int kdef, const int* kpf, const int* ipf, const int* npf);
+/* Orbitals: iech (= ief, channel of a continuum orbital, 0 bound), l, the
+ * packed IBORT and OBS (nbf x nbf, as Read_data leaves them,
+ * read_data.f90:150-208) and the electron count nelc (module target). */
+int bsrg_lb_set_orbitals(bsrg_lb* lb, int nbf, const int* iech, const int* lef, const int* ibort, const double* obs,
+                         int nelc);
+/* Last configuration of each channel ipconf(nch) and of each perturber
+ * ippert(0:npert) (module channel, MOD_channel.f90:2-46; Ifind_channel). */
+int bsrg_lb_set_channels(bsrg_lb* lb, int nch, const int* ipconf, int npert, const int* ippert);
+/* States (module conf_LS): state is has the no_state(is) orbitals
+ * ip_orb(ip_state(is)+1 ..) and expansion coefficient wc(is); symmetry it
+ * covers the states ip_stat(it_state1(it) .. it_state2(it)) (it_state1 = 0:
+ * none). */
+int bsrg_lb_set_states(bsrg_lb* lb, int ncfg, const int* ip_state, const int* no_state, const double* wc, int lorb,
+                       const int* ip_orb, int nsym, const int* it_state1, const int* it_state2, int nstat,
+                       const int* ip_stat);
+/* eps_c, eps_det, eps_soo and the cmdata blocks nb, mb, kb (module bsr_mat,
+ * mod_param.f90:55-66; defaults 1e-10, 1e-10, 1e-3, 2000, 5000, 100). */
+int bsrg_lb_set_params(bsrg_lb* lb, double eps_c, double eps_det, double eps_soo, int nb, int mb, int kb);
+/* Run the loop for icase with multipoles up to mk.  sizes3 = coefficients
+ * passed to Add_coef, list records produced, lists (batches). */
+int bsrg_lb_run(bsrg_lb* lb, int icase, int mk, int* sizes3);
+/* The coefficients in the order State_res passes them to Add_coef
+ * (structure, multipole, K1..K4, c). */
+int bsrg_lb_expansion(bsrg_lb* lb, int* itype, int* kpol, int* k1, int* k2, int* k3, int* k4, double* c);
+/* The lists in generation order: each batch is what one Gen_matrix call
+ * hands to I_data/L_data/O_data (gen_matrix.f90:17-66), records in IPT
+ * order. */
+int bsrg_lb_lists(bsrg_lb* lb, int* batch, int* jtype, int* jpol, int* k1, int* k2, int* k3, int* k4, double* c);
+
 /* Overlap records (structure 1 only).  Replaces O_data,
  * src/bsr/BSR_MAT3/O_data.f90:2-76; records with |c| < eps_c skipped. */
 int bsrg_odata(bsrg_ctx* ctx, int ncdata, const int* ipt, const int* k3, const int* k4,

@@ -52,10 +52,36 @@ DSRCS := \
   $(REF)/dbsr/DBSR_MAT3_MPI/updates.f90 $(REF)/dbsr/DBSR_MAT3_MPI/target_subs.f90 \
   $(REF)/dbsr/DBSR_MAT3_MPI/R_data.f90 $(REF)/dbsr/DBSR_MAT3_MPI/O_data.f90 \
   $(REF)/dbsr/DBSR_MAT3_MPI/zero_cond.f90 $(REF)/dbsr/DBSR_MAT3_MPI/transform_matrix.f90
+# The BSR_MAT3 list builder's routines (Read_dets with the int_bnk header
+# readers, Decode_INT, Ifind_channel, Jsym_int, Det_fact, Check_det, VDET,
+# DET, Idef_itype) with the modules they read: a library of their own,
+# built with the libs_03 spline_orbitals module, whose IBORT and OBS arrays
+# check_det.f90 and vdet.f90 use
+LSRCS := \
+  $(REF)/libs_03/BS/spline_param.f90 $(REF)/libs_03/BS/spline_orbitals.f90 \
+  $(REF)/bsr/BSR_MAT3/mod_param.f90 $(REF)/libs/ZCONFLS/MOD_param_LS.f90 \
+  $(REF)/libs/ZCONFLS/MOD_symc.f90 $(REF)/libs/ZCONFLS/MOD_symt.f90 $(REF)/libs/ZCONFLS/MOD_orb_LS.f90 \
+  $(REF)/libs/ZCONFLS/MOD_target.f90 $(REF)/libs/ZCONFLS/MOD_channel.f90 $(REF)/libs/ZCONFLS/incode_int.f90 \
+  $(REF)/libs/ZCOM/read_par.f90 $(REF)/libs/ZCOM/stop_mp.f90 $(REF)/libs/ZCOM/det.f90 \
+  $(REF)/bsr/BSR_MAT3/mod_cmdata.f90 $(REF)/bsr/BSR_MAT3/mod_dets.f90 $(REF)/bsr/BSR_MAT3/mod_ndef.f90 \
+  $(REF)/bsr/BSR_MAT3/mod_ndet.f90 $(REF)/bsr/BSR_MAT3/det_fact.f90 $(REF)/bsr/BSR_MAT3/check_det.f90 \
+  $(REF)/bsr/BSR_MAT3/vdet.f90 $(REF)/bsr/BSR_MAT3/jsym_int.f90 $(REF)/bsr/BSR_MAT3/idef_type.f90
+LOUT := $(OUT)/lb
+LOBJS := $(addprefix $(LOUT)/,$(notdir $(LSRCS:.f90=.o)))
+
 DOUT := $(OUT)/dbs
 DOBJS := $(addprefix $(DOUT)/,$(notdir $(DSRCS:.f90=.o)))
 
-all: $(OUT)/libbsr_ref.so $(OUT)/libdbs_ref.so
+all: $(OUT)/libbsr_ref.so $(OUT)/libdbs_ref.so $(OUT)/liblb_ref.so
+
+$(OUT)/liblb_ref.so: $(LSRCS) $(HERE)ref_lb_driver.f90 $(HERE)ref.mk
+	mkdir -p $(LOUT)/mod
+	set -e; for f in $(LSRCS); do \
+	  $(FC) -O2 -fPIC -flto -module-dir $(LOUT)/mod -c $$f -o $(LOUT)/$$(basename $$f .f90).o; done
+	$(FC) -O2 -fPIC -flto -module-dir $(LOUT)/mod -c $(HERE)ref_lb_driver.f90 -o $(LOUT)/ref_lb_driver.o
+	echo '{ global: ref_*; local: *; };' > $(LOUT)/exports.map
+	$(FC) -flto -fuse-ld=lld -shared -o $@ $(LOBJS) $(LOUT)/ref_lb_driver.o \
+	  -Wl,--version-script=$(LOUT)/exports.map
 
 $(OUT)/libdbs_ref.so: $(DSRCS) $(HERE)ref_dbs_driver.f90 $(HERE)ref_dbsr_driver.f90 $(HERE)ref.mk
 	mkdir -p $(DOUT)/mod

@@ -431,3 +431,97 @@ def dirac_overlap(w, eps_c=1e-10, iitar=0, keep=False):
     out.pop("acf"), out.pop("htarg")
     out["otarg"] = r.otarg()
     return (out, r) if keep else out
+
+
+# ---- list builder routines: oracle/_ref/liblb_ref.so --------------------------
+LB_PATH = os.path.join(HERE, "_ref", "liblb_ref.so")
+_LB = None
+
+
+def lb_available() -> bool:
+    return os.path.exists(LB_PATH)
+
+
+def lb_lib():
+    global _LB
+    if _LB is None:
+        L = C.CDLL(LB_PATH)
+        i = C.c_int
+        sig = {
+            "ref_lb_read": (None, [i, _dp, _ip, _ip, _ip, C.POINTER(C.c_int), _ip]),
+            "ref_lb_get_dets": (None, [_ip, _ip, _ip, _ip, _ip, _ip]),
+            "ref_lb_set_dets": (None, [i, i, _ip, _ip, _ip, i, i, _ip, _ip, _ip]),
+            "ref_lb_setup": (None, [i, _ip, _ip, _ip, _dp, i, i, _ip, i, _ip]),
+            "ref_lb_decode": (None, [i, _ip, _ip]),
+            "ref_lb_jsym": (None, [i, _ip, _ip]),
+            "ref_lb_expand": (None, [i, i, C.c_double, C.c_double, C.c_double, i, _dp, _ip, _ip, _ip, i, _ip, _ip,
+                                     _dp, i, _ip, i, _ip, _ip, i, _ip, i, _ip, _ip, _ip, _dp,
+                                     C.POINTER(C.c_int)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LB = L
+    return _LB
+
+
+def lb_read(dirpath, cap=1 << 20):
+    """Read_dets on dirpath/int_bnk.tst and the coefficient records after it:
+    (c, ijt, int, idf), (kpd, ipd, npd, kpf, ipf, npf)."""
+    cwd = os.getcwd()
+    c = np.zeros(cap)
+    a = [np.zeros(cap, np.int32) for _ in range(3)]
+    n = C.c_int()
+    sz = np.zeros(4, np.int32)
+    try:
+        os.chdir(dirpath)
+        lb_lib().ref_lb_read(cap, c, *a, C.byref(n), sz)
+    finally:
+        os.chdir(cwd)
+    m = n.value
+    ndet, kdet, ndef, kdef = (int(x) for x in sz)
+    d = [np.zeros(max(x, 1), np.int32) for x in (ndet, ndet, kdet, ndef, ndef, kdef)]
+    lb_lib().ref_lb_get_dets(*d)
+    d = [x[:k] for x, k in zip(d, (ndet, ndet, kdet, ndef, ndef, kdef))]
+    return (c[:m], a[0][:m], a[1][:m], a[2][:m]), tuple(d)
+
+
+def lb_decode(ints):
+    ints = _i32(ints)
+    out = np.zeros((6, len(ints)), np.int32, order="F")
+    lb_lib().ref_lb_decode(len(ints), ints, out)
+    return out
+
+
+def lb_jsym(met, quads):
+    q = np.asfortranarray(np.asarray(quads, np.int32).reshape(4, -1))
+    lb_lib().ref_lb_jsym(q.shape[1], _i32(met), q)
+    return q
+
+
+def lb_expand(b, icase, mk, eps_c=1e-10, eps_det=1e-10, eps_soo=1e-3, cap=1 << 21):
+    """The reference's routines walked in State_res order over a synthetic
+    bank (bsr_amd.synth_bank.BankWave): (itype, kpol, k1, k2, k3, k4, c)."""
+    w = b.wave
+    L = lb_lib()
+    L.ref_lb_setup(w.nbf, _i32(w.iech), _i32(w.lbs), _i32(np.asarray(b.ibort).reshape(-1, order="F")),
+                   np.asfortranarray(np.asarray(b.obs, np.float64).reshape(-1, order="F")), b.nelc,
+                   len(b.ipconf), _i32(b.ipconf), len(b.ippert) - 1, _i32(b.ippert))
+    z = np.zeros(1, np.int32)
+    L.ref_lb_set_dets(len(b.kpd), len(b.npd), _i32(b.kpd) if len(b.kpd) else z, _i32(b.ipd) if len(b.ipd) else z,
+                      _i32(b.npd) if len(b.npd) else z, len(b.kpf), len(b.npf), _i32(b.kpf) if len(b.kpf) else z,
+                      _i32(b.ipf) if len(b.ipf) else z, _i32(b.npf) if len(b.npf) else z)
+    t = np.zeros(cap, np.int32)
+    kp = np.zeros(cap, np.int32)
+    kk = np.zeros((4, cap), np.int32, order="F")
+    cc = np.zeros(cap)
+    n = C.c_int()
+    L.ref_lb_expand(icase, mk, eps_c, eps_det, eps_soo, len(b.c), _f(b.c), _i32(b.ijt), _i32(b.intb), _i32(b.idf),
+                    len(b.ip_state), _i32(b.ip_state), _i32(b.no_state), _f(b.wc), len(b.ip_orb), _i32(b.ip_orb),
+                    len(b.it1), _i32(b.it1), _i32(b.it2), len(b.ip_stat), _i32(b.ip_stat), cap, t, kp,
+                    kk.reshape(-1, order="F"), cc, C.byref(n))
+    m = n.value
+    if m > cap:
+        raise RuntimeError("lb_expand: cap too small")
+    return t[:m], kp[:m], kk[0, :m], kk[1, :m], kk[2, :m], kk[3, :m], cc[:m]
