@@ -307,6 +307,18 @@ class Context:
         from .matrices import WaveMatrices
         return WaveMatrices(self.ns, nch, ncp, mk)
 
+    def matrix_record(self, path, interrupt, intercase, max_nbuf=0):
+        """The current matrix as an interrupted run leaves it (BSR_MAT4)."""
+        L.check(L.lib().bsrg_matrix_record(self._p, str(path).encode(), int(interrupt), int(intercase),
+                                           int(max_nbuf)))
+
+    def matrix_resume(self, path):
+        """Read an interrupted run's matrix into the current one: returns
+        (interrupt, intercase, max_nbuf)."""
+        a, b, m = C.c_int(), C.c_int(), C.c_int()
+        L.check(L.lib().bsrg_matrix_resume(self._p, str(path).encode(), C.byref(a), C.byref(b), C.byref(m)))
+        return a.value, b.value, m.value
+
     def get_matrix(self):
         m = self._alloc_out(self.nch, self.ncp, self.mk)
         L.check(L.lib().bsrg_get_matrix(self._p, m.hcc, m.hcb if self.ncp else None, m.hbb if self.ncp else None,

@@ -1921,6 +1921,131 @@ int bsrg_wave_write(bsrg_ctx* c, const char* path, double tmax, int ntarg, const
   return 0;
 }
 
+// ---- interrupted runs (BSR_MAT4) ----------------------------------------------
+// The partial matrix of an interrupted run, as BSR_MAT4 leaves it when its
+// time limit is reached (state_res.f90:187-196): the header (ns, nch,
+// npert), Record_matrix (record_matrix.f90:2-46: every nonzero channel
+// block (ich >= jch), channel-perturber vector and perturber scalar behind
+// an (i, j) header, perturbers as nch + ip, then (-1, -1)), mk, ACF,
+// htarg, otarg, and last (interrupt, intercase, max_nbuf), the record
+// read_data.f90:39-41 reads back first.
+int bsrg_matrix_record(bsrg_ctx* c, const char* path, int interrupt, int intercase, int max_nbuf) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = cur_ready(c)) return rc;
+  if (!path) return fail(-2, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  Matrices& M = c->cur;
+  std::vector<double> hcc(M.hcc_n()), hcb(M.hcb_n());
+  if (int rc = get_matrices(c, M, hcc.data(), hcb.data(), nullptr, nullptr, nullptr, nullptr)) return rc;
+  std::FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(-40, std::string("cannot open ") + path);
+  const int ns = M.ns, kch = M.nch, kcp = M.ncp;
+  const size_t nss = static_cast<size_t>(ns) * ns;
+  const int32_t head[3] = {ns, kch, kcp};
+  bool ok = put_record(f, head, 12);
+  for (int ich = 1; ich <= kch && ok; ++ich)
+    for (int jch = 1; jch <= ich && ok; ++jch) {
+      const double* b = &hcc[(static_cast<size_t>(ich) * (ich - 1) / 2 + jch - 1) * nss];
+      double s = 0.0;
+      for (size_t q = 0; q < nss; ++q) s += std::fabs(b[q]);
+      if (s == 0.0) continue;
+      const int32_t hdr[2] = {ich, jch};
+      ok = put_record(f, hdr, 8) && put_record(f, b, nss * 8);
+    }
+  for (int ich = 1; ich <= kch && ok; ++ich)
+    for (int ip = 1; ip <= kcp && ok; ++ip) {
+      const double* v = &hcb[(static_cast<size_t>(ip - 1) * kch + (ich - 1)) * ns];
+      double s = 0.0;
+      for (int q = 0; q < ns; ++q) s += std::fabs(v[q]);
+      if (s == 0.0) continue;
+      const int32_t hdr[2] = {ip + kch, ich};
+      ok = put_record(f, hdr, 8) && put_record(f, v, static_cast<size_t>(ns) * 8);
+    }
+  for (int ip = 1; ip <= kcp && ok; ++ip)
+    for (int jp = 1; jp <= ip && ok; ++jp) {
+      const double v = M.hbb[static_cast<size_t>(ip) * (ip - 1) / 2 + jp - 1];
+      if (v == 0.0) continue;
+      const int32_t hdr[2] = {ip + kch, jp + kch};
+      ok = put_record(f, hdr, 8) && put_record(f, &v, 8);
+    }
+  const int32_t end[2] = {-1, -1};
+  ok = ok && put_record(f, end, 8);
+  const int32_t mk = M.mk;
+  ok = ok && put_record(f, &mk, 4);
+  ok = ok && put_record(f, M.acf.data(), M.acf.size() * 8);
+  ok = ok && put_record(f, M.htarg.data(), M.htarg.size() * 8);
+  ok = ok && put_record(f, M.otarg.data(), M.otarg.size() * 8);
+  const int32_t tail[3] = {interrupt, intercase, max_nbuf};
+  ok = ok && put_record(f, tail, 12);
+  if (std::fclose(f) != 0) ok = false;
+  if (!ok) return fail(-41, std::string("write failed: ") + path);
+  return 0;
+}
+
+// Resume from such a file (Read_matrix, read_matrix.f90:2-33, and the
+// markers of read_data.f90:39-41): the blocks it holds replace those of the
+// current matrix (allocated beforehand with the file's sizes, as
+// alloc_dbsr_matrix precedes Read_matrix), the others stay as they are.
+int bsrg_matrix_resume(bsrg_ctx* c, const char* path, int* interrupt, int* intercase, int* max_nbuf) {
+  if (int rc = check_ctx(c)) return rc;
+  if (int rc = cur_ready(c)) return rc;
+  if (!path) return fail(-2, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  Matrices& M = c->cur;
+  std::FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(-51, std::string("cannot open ") + path);
+  std::vector<char> rec;
+  auto next = [&]() {
+    int32_t n1 = 0, n2 = 0;
+    if (std::fread(&n1, 4, 1, f) != 1 || n1 < 0) return false;
+    rec.resize(n1);
+    if (n1 && std::fread(rec.data(), 1, n1, f) != static_cast<size_t>(n1)) return false;
+    return std::fread(&n2, 4, 1, f) == 1 && n2 == n1;
+  };
+  auto bad = [&](const char* what) {
+    std::fclose(f);
+    return fail(-50, std::string("partial matrix file: ") + what);
+  };
+  auto ints = [&](int i) { int32_t v; std::memcpy(&v, rec.data() + 4 * i, 4); return v; };
+  if (!next() || rec.size() < 12) return bad("header");
+  if (ints(0) != M.ns || ints(1) != M.nch || ints(2) != M.ncp) return bad("sizes differ from the allocated matrix");
+  const int ns = M.ns, kch = M.nch;
+  const size_t nss = static_cast<size_t>(ns) * ns;
+  for (;;) {
+    if (!next() || rec.size() < 8) return bad("block header");
+    const int ic = ints(0), jc = ints(1);
+    if (ic <= 0) break;
+    if (ic > kch && jc > kch) {
+      const int ip = ic - kch, jp = jc - kch;
+      if (ip > M.ncp || jp > ip || !next() || rec.size() != 8) return bad("perturber scalar");
+      std::memcpy(&M.hbb[static_cast<size_t>(ip) * (ip - 1) / 2 + jp - 1], rec.data(), 8);
+    } else if (ic > kch) {
+      const int ip = ic - kch;
+      if (ip > M.ncp || jc < 1 || jc > kch || !next() || rec.size() != static_cast<size_t>(ns) * 8)
+        return bad("channel-perturber vector");
+      HIPCHK(hipMemcpy(M.hcb.p + (static_cast<size_t>(ip - 1) * kch + (jc - 1)) * ns, rec.data(), rec.size(),
+                       hipMemcpyHostToDevice));
+    } else {
+      if (jc < 1 || jc > ic || !next() || rec.size() != nss * 8) return bad("channel block");
+      HIPCHK(hipMemcpy(M.hcc.p + (static_cast<size_t>(ic) * (ic - 1) / 2 + jc - 1) * nss, rec.data(), rec.size(),
+                       hipMemcpyHostToDevice));
+    }
+  }
+  if (!next() || rec.size() != 4 || ints(0) != M.mk) return bad("mk differs from the allocated matrix");
+  if (!next() || rec.size() != M.acf.size() * 8) return bad("ACF");
+  std::memcpy(M.acf.data(), rec.data(), rec.size());
+  if (!next() || rec.size() != M.htarg.size() * 8) return bad("htarg");
+  std::memcpy(M.htarg.data(), rec.data(), rec.size());
+  if (!next() || rec.size() != M.otarg.size() * 8) return bad("otarg");
+  std::memcpy(M.otarg.data(), rec.data(), rec.size());
+  if (!next() || rec.size() != 12) return bad("interrupt record");
+  if (interrupt) *interrupt = ints(0);
+  if (intercase) *intercase = ints(1);
+  if (max_nbuf) *max_nbuf = ints(2);
+  std::fclose(f);
+  return 0;
+}
+
 int bsrg_wave_timings(bsrg_ctx* c, double* ms5) {
   if (int rc = check_ctx(c)) return rc;
   for (int i = 0; i < 5; ++i) ms5[i] = c->timings[i];

@@ -102,6 +102,8 @@ SIGNATURES = {
     "bsrg_dirac_transform": [C.c_void_p, C.c_int],
     "bsrg_dirac_overlaps": [C.c_void_p, _dp],
     "bsrg_dirac_diag_timings": [C.c_void_p, _dp],
+    "bsrg_matrix_record": [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int],
+    "bsrg_matrix_resume": [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "bsrg_lb_create": [],
     "bsrg_lb_destroy": [C.c_void_p],
     "bsrg_lb_read_bank": [C.c_void_p, C.c_char_p, _ip],

@@ -488,6 +488,21 @@ int bsrg_matfile_eigen(bsrg_ctx* ctx, const char* path, int nev, double* e);
  * reference. */
 int bsrg_drop_caches(bsrg_ctx* ctx);
 
+/* Interrupted runs (BSR_MAT4's time limit, src/bsr/BSR_MAT4/state_res.f90:
+ * 187-196): write the matrix of bsrg_alloc_matrix as it stands -- header
+ * (ns, nch, npert), Record_matrix (record_matrix.f90:2-46; nonzero channel
+ * blocks, channel-perturber vectors and perturber scalars behind (i, j)
+ * headers, perturbers as nch + ip, ended by (-1, -1)), mk, ACF, htarg,
+ * otarg, and last (interrupt, intercase, max_nbuf).  -40 cannot open, -41
+ * write failed. */
+int bsrg_matrix_record(bsrg_ctx* ctx, const char* path, int interrupt, int intercase, int max_nbuf);
+/* Resume: read such a file into the matrix of bsrg_alloc_matrix (allocated
+ * with the file's ns, nch, npert and mk), as Read_matrix does
+ * (read_matrix.f90:2-33), and return the markers (read_data.f90:39-41);
+ * the caller skips what was done and continues.  -50 malformed or other
+ * sizes, -51 cannot open. */
+int bsrg_matrix_resume(bsrg_ctx* ctx, const char* path, int* interrupt, int* intercase, int* max_nbuf);
+
 /* Precision of the exchange contraction for later bsrg_wave_assemble
  * calls: 0 = plain FP64 (default); 1 = split precision, Ozaki slices on
  * bf16 MFMA with FP32 accumulation and FP64 recombination; 2 = split
