@@ -360,7 +360,9 @@ class Context:
         return L.lib().bsrg_wave_engine(self._p)
 
     def set_gemm_engine(self, engine=-1):
-        """0 library FP64 GEMM, 1-3 FP64 MFMA kernel tiles, -1 autotune."""
+        """0 library FP64 GEMM, 1-3 FP64 MFMA kernel tiles, 4 sparse product
+        over the nonzero coefficients, -1 automatic (BS waves: autotuned,
+        4 preferred where its tile fits; Dirac waves: 4)."""
         L.check(L.lib().bsrg_set_gemm_engine(self._p, engine))
 
     def set_capture_safe(self, on=True):

@@ -114,6 +114,17 @@ int xasm_slot(int nr);
 void launch_coef_pack(int nblk, int nr, const double* C, size_t ldc, int* xo, double* xc, int* xn, hipStream_t s);
 // launch_xprod: the exchange product alone, H = E C over the packed lists,
 // every block written (the GEMM's place in the wave; the term pass follows).
+// The Dirac form: launch_xquad writes quadrant (qa, qb) of the channel
+// blocks (order 2ns) of hch with the product of the family's bases E
+// (columns of ns^2) and the lists launch_coef_pack_quad packed from rows
+// [0, nr) of C (slots of xasm_slot(nr) per block) for tiles of wt positions,
+// wt = xquad_width(nr, device) (64, 48 or 32; 0 when nr > xquad_max_t).
+int xquad_width(int nr, int device);
+int xquad_max_t(int device);
+void launch_coef_pack_quad(int nblk, int nr, int wt, const double* C, size_t ldc, int* xo, double* xc, int* xn,
+                           hipStream_t s);
+int launch_xquad(int ns, int nblk, int nr, int wt, const double* E, const int* xo, const double* xc, const int* xn,
+                 int qa, int qb, double* hch, hipStream_t s);
 int launch_xprod(int ns, int nblk, int nr, const double* E, const int* xo, const double* xc, const int* xn,
                  double* hcc, hipStream_t s);
 int launch_xasm(int ns, int ks, int nblk, int nr, const double* E, const int* xo, const double* xc, const int* xn,

@@ -289,9 +289,12 @@ struct DiracWave {
   DevVec<char> oz8a[4], oz8b, ozk;
   long oz8a_ver[4] = {-1, -1, -1, -1};
   int oz8a_S[4] = {0, 0, 0, 0};
+  // sparse FP64 product: per-block nonzero lists of the coefficients
+  DevVec<int> qxo, qxn;
+  DevVec<double> qxc;
   int engine = -1;
   bool ok = false;
-  bool fused = false;  // the last wave's GEMMs wrote the output layout directly (int8 split)
+  bool fused = false;  // the last wave's products wrote the output layout directly (sparse FP64, int8 split)
   double ms[5] = {0, 0, 0, 0, 0};
   double host_ms[3] = {0, 0, 0};  // list decoding, staging and uploads, whole call
   // overlap pass: S channel blocks, channel-perturber vectors (device),

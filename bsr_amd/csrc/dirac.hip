@@ -425,9 +425,6 @@ int stage_exchange(bsrg_ctx* c) {
   return 0;
 }
 
-std::mutex g_dtuned_mu;
-std::map<std::tuple<int, int, int, int>, int> g_dtuned;
-
 // Exchange GEMM of family f into its quadrant: hq[b1,b2] (+)= E_f * coef_f.
 // engine 0 library FP64, 1-3 FP64 MFMA kernel, 10+S bf16 and 20+S int8 split
 // precision with S slices (the int8 slices of E_f prepared once per version
@@ -443,6 +440,15 @@ int dirac_gemm(bsrg_ctx* c, int f, int engine, bool accumulate, bool into_output
   const double* B = W.coef.p + r0;
   double* C = W.hq.p + static_cast<size_t>(2 * kB1[f] + kB2[f]) * W.nblk * nss;
   const int m = static_cast<int>(nss), n = static_cast<int>(W.nblk);
+  if (engine == 4) {
+    // family f's quadrant (b1, b2) of every channel block from the packed
+    // nonzeros of the coefficients (identical for the four families)
+    const double* Ef = W.E.p + r0 * nss;
+    if (launch_xquad(ns, n, Kt, xquad_width(Kt, c->device), Ef, W.qxo.p, W.qxc.p, W.qxn.p, kB1[f], kB2[f], W.hch.p,
+                     c->stream))
+      return fail(-1, "launch_xquad: no coefficient rows");
+    return 0;
+  }
   if (engine >= 20) {
     const int S = engine - 20, K = nkE * static_cast<int>(nb2);
     const double* Af = W.E.p + static_cast<size_t>(f) * nkE * nb2 * nss;
@@ -477,35 +483,14 @@ int dirac_gemm(bsrg_ctx* c, int f, int engine, bool accumulate, bool into_output
   return mfma_dgemm(m, n, Kt, A, nss, B, nt, C, nss, accumulate, engine == 1 ? 1 : (engine == 2 ? 2 : 4), c->stream);
 }
 
-// fastest FP64 engine for this GEMM shape, timed once per shape class
+// FP64 engine of family f: the sparse exchange product (4) unless an engine
+// is forced (0 library GEMM, 1-3 FP64 MFMA GEMM kernel); split precision:
+// 10+S bf16, 20+S int8 slices
 int dirac_engine(bsrg_ctx* c, int f) {
-  auto& W = c->dw;
   if (c->precision_mode != 0) return 10 * c->precision_mode + c->nslices;
-  if (c->forced_engine >= 0) return c->forced_engine;
-  const int ns = c->dbs.T.ns;
-  const int Kt = (W.khi[f] - W.klo[f] + 1) * W.nb * W.nb;
-  const int bucket = static_cast<int>(std::floor(std::log(static_cast<double>(W.nblk)) / std::log(1.25)));
-  const auto key = std::make_tuple(c->device, ns, Kt, bucket);
-  {
-    std::lock_guard<std::mutex> g(g_dtuned_mu);
-    auto it = g_dtuned.find(key);
-    if (it != g_dtuned.end()) return it->second;
-  }
-  float best = 1e30f;
-  int best_e = 2;
-  for (int e = 0; e <= 3; ++e) {  // the output is rewritten by the term pass afterwards
-    if (int rc = dirac_gemm(c, f, e, false)) return rc;
-    (void)hipEventRecord(c->ev[6], c->stream);
-    if (int rc = dirac_gemm(c, f, e, false)) return rc;
-    (void)hipEventRecord(c->ev[7], c->stream);
-    HIPCHK(hipEventSynchronize(c->ev[7]));
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, c->ev[6], c->ev[7]);
-    if (ms < best) { best = ms; best_e = e; }
-  }
-  std::lock_guard<std::mutex> g(g_dtuned_mu);
-  g_dtuned[key] = best_e;
-  return best_e;
+  if (c->forced_engine >= 0 && c->forced_engine != 4) return c->forced_engine;
+  const int nr = (c->dw.khi[f] - c->dw.klo[f] + 1) * c->dw.nb * c->dw.nb;
+  return nr <= xquad_max_t(c->device) ? 4 : 0;  // the library GEMM when the tile does not fit
 }
 
 }  // namespace
@@ -618,9 +603,11 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   W.iptar.assign(iptar, iptar + nch);
   W.nblk = static_cast<size_t>(nch) * (nch + 1) / 2;
   const size_t nss = static_cast<size_t>(ns) * ns, nq1 = static_cast<size_t>(nch) * npert;
-  // the quadrant-major GEMM output; int8 split GEMMs write the output layout
-  // directly and need none
-  if (c->precision_mode != 2) HIPCHK(W.hq.resize(4 * W.nblk * nss));
+  // the quadrant-major GEMM output: only the library and MFMA GEMMs and
+  // the bf16 split form need it (the sparse product and the int8 split
+  // GEMMs write the output layout directly)
+  if (c->precision_mode == 1 || (c->precision_mode == 0 && c->forced_engine >= 0 && c->forced_engine <= 3))
+    HIPCHK(W.hq.resize(4 * W.nblk * nss));
   HIPCHK(W.hv.resize(std::max<size_t>(2 * nq1 * ns, 1)));
   HIPCHK(W.hch.resize(W.nblk * ms * ms));
   HIPCHK(W.hcp.resize(std::max<size_t>(nq1 * ms, 1)));
@@ -708,9 +695,26 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   // int8 split GEMMs place their results in the output layout themselves
   // and the pass adds the terms in place, so no quadrant-major copy exists.
   // The diagonal channel blocks are symmetrised in place at the end.
-  bool fused = true;
-  for (int f = 0; f < 4; ++f) fused = fused && (eng[f] < 0 || eng[f] >= 20);
+  bool fused = true, sparse = false;
+  for (int f = 0; f < 4; ++f) {
+    fused = fused && (eng[f] < 0 || eng[f] == 4 || eng[f] >= 20);
+    sparse = sparse || eng[f] == 4;
+  }
   W.fused = fused;
+  if (sparse) {
+    // per-block nonzero lists of the coefficients of family 0 (the others'
+    // are the same)
+    const size_t nb2 = static_cast<size_t>(W.nb) * W.nb;
+    const int nr = (W.khi[0] - W.klo[0] + 1) * static_cast<int>(nb2);
+    const size_t slots = W.nblk * static_cast<size_t>(xasm_slot(nr));
+    HIPCHK(W.qxo.resize(slots));
+    HIPCHK(W.qxc.resize(slots));
+    HIPCHK(W.qxn.resize(W.nblk));
+    launch_coef_pack_quad(static_cast<int>(W.nblk), nr, xquad_width(nr, c->device),
+                          W.coef.p + static_cast<size_t>(W.klo[0]) * nb2,
+                          4 * static_cast<size_t>(W.exch_nk) * nb2, W.qxo.p, W.qxc.p, W.qxn.p, s);
+    HIPCHK(hipGetLastError());
+  }
   (void)hipEventRecord(c->ev[2], s);
   int qmask = 0;
   for (int f = 0; f < 4; ++f)
@@ -731,7 +735,7 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
   st.pending = true;
   HIPCHK(hipStreamWaitEvent(s, st.done, 0));
   const auto h2 = std::chrono::steady_clock::now();
-  // in place on the quadrants the int8 GEMMs wrote, else from hq
+  // in place on the quadrants the sparse or int8 products wrote, else from hq
   launch_assemble(ns, ks, static_cast<int>(4 * W.nblk), 0, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
                   st.dp<double>(st.o_bc), W.Dc.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
                   st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), W.Sc.p, W.VT.p, W.hch.p, c->bsum.p, c->rfac.p, s,

@@ -568,8 +568,9 @@ __global__ void k_sb_upper(int ns, int ks, const double* sb, double* sbu) {
 // each sum in list order.  base is what hcc holds (the exchange GEMM result,
 // or earlier contributions) when has_base, else 0 and every block is
 // written.  Term lists are CSR over all blocks (ptr has nblk+1 entries).
-// grid (nblk, ceil(ns/kAsmCols), ceil(ns/256)), block 256: lane -> rows
-// x0+lane+64a (a<4), wave -> kAsmCpw columns yt+kAsmCpw*wave+b.
+// grid nblk * ceil(ns/kAsmCols) * ceil(ns/(64A)) (block, column tile, row
+// tile), block 256: lane -> rows x0+lane+64a (a<A), wave -> kAsmCpw columns
+// yt+kAsmCpw*wave+b.
 // The band entries (i, ith) the workgroup's columns touch come from the
 // block's summed band (k_term_prep) through LDS; rank-1 terms go kAsmR1 at
 // a time through LDS (u over the rows, f*w over the columns), a small batch
@@ -614,14 +615,24 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
                                                   double* __restrict__ hcc, const double* __restrict__ base,
                                                   int qnblk, int qmask, const double* __restrict__ bsum,
                                                   const unsigned char* __restrict__ live) {
-  const int blk = blockIdx.x;
+  // one-dimensional grid over (block, column tile, row tile), block-major;
+  // workgroups go to the eight XCDs in turn (id mod 8), so they are numbered
+  // XCD by XCD: the tiles of a block and neighbouring blocks share an L2
+  const int ny = (ns + kAsmCols - 1) / kAsmCols, nz = (ns + 64 * A - 1) / (64 * A);
+  unsigned g = blockIdx.x;
+  const unsigned per8 = gridDim.x / 8;
+  if (g < per8 * 8) g = (g % 8) * per8 + g / 8;
+  const int blk = static_cast<int>(g / static_cast<unsigned>(ny * nz));
+  const int tyz = static_cast<int>(g % static_cast<unsigned>(ny * nz)), ty = tyz % ny, tz = tyz / ny;
   const int b0 = bptr[blk], b1 = bptr[blk + 1], r0 = rptr[blk], r1 = rptr[blk + 1];
   // a block without terms keeps its input (has_base), or, overwritten, is
   // skipped when it already holds zeros (live[blk] == 0)
   if (qnblk == 0 && b0 == b1 && r0 == r1 && (has_base || (live && !live[blk]))) return;
+  // a quadrant updated in place that has no terms keeps its values
+  if (qnblk > 0 && base == hcc && ((qmask >> (blk / qnblk)) & 1) && b0 == b1 && r0 == r1) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr int R = 64 * A;  // rows per tile
-  const int x0 = blockIdx.z * R, yt = blockIdx.y * kAsmCols, y0 = yt + wv * kAsmCpw;
+  const int x0 = tz * R, yt = ty * kAsmCols, y0 = yt + wv * kAsmCpw;
   const double* hin = nullptr;
   double* h;
   size_t ldo = ns, ldb = ns;
@@ -996,7 +1007,7 @@ void launch_assemble(int ns, int ks, int nblk, int has_base, const int* bptr, co
     hipLaunchKernelGGL(k_term_prep, dim3(nblk, (ns * ks + 255) / 256), dim3(256), 0, s, ns, ks, bptr, bid, bc, bands,
                        bsum, rptr, rs, rc, stab, rf);
   const int A = ns <= 64 ? 1 : (ns <= 128 ? 2 : (ns <= 192 ? 3 : 4));
-  const dim3 grid(nblk, (ns + kAsmCols - 1) / kAsmCols, (ns + 64 * A - 1) / (64 * A));
+  const dim3 grid(static_cast<unsigned>(nblk) * ((ns + kAsmCols - 1) / kAsmCols) * ((ns + 64 * A - 1) / (64 * A)));
   switch (A) {
 #define ASM_CASE(N)                                                                                             \
   case N:                                                                                                       \

@@ -43,12 +43,12 @@ constexpr int kXWaves = 16;  // wavefronts per workgroup (4 per SIMD)
 constexpr int kXChunk = 8;  // list entries per scalar-load round
 
 // Per block (one wavefront): the nonzeros of column blk of C (rows 0..nr-1,
-// leading dimension ldc) in ascending t, stored as (byte offset t*512 of
+// leading dimension ldc) in ascending t, stored as (byte offset t*rowb of
 // E_t's tile in LDS, c) in slots blk*nrs.. (nrs = nr rounded up to
 // kXChunk), padded with (0, 0) up to the next multiple of kXChunk; their
 // count in xn[blk].
 __global__ __launch_bounds__(256) void k_coef_pack(int nblk, int nr, int nrs, const double* __restrict__ C, size_t ldc,
-                                                   int* __restrict__ xo, double* __restrict__ xc,
+                                                   int rowb, int* __restrict__ xo, double* __restrict__ xc,
                                                    int* __restrict__ xn) {
   const int blk = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (blk >= nblk) return;
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_coef_pack(int nblk, int nr, int nrs, co
     const unsigned long long m = __ballot(nz);
     const int before = __popcll(m & ((1ull << lane) - 1ull));
     if (nz) {
-      ol[base + before] = t * 64 * static_cast<int>(sizeof(double));
+      ol[base + before] = t * rowb;
       cl[base + before] = v;
     }
     base += __popcll(m);
@@ -151,7 +151,114 @@ __global__ __launch_bounds__(64 * kXWaves) void k_xasm(
   }
 }
 
+// The Dirac form (one family, one quadrant): H_q[blk](x,y) = sum_t C(t,blk)
+// E_f,t(x,y) written to quadrant (qa, qb) of channel block blk (order 2ns,
+// leading dimension 2ns).  A family may use more bases than a 64-position
+// tile of them fits in LDS (325 at ks 9, k <= 12, five bound orbitals: 166
+// KB), so the tile is wt positions, the widest of 64, 48 and 32 that fits
+// ([t][wt] doubles, all nr rows at once); lanes past wt idle.  As in
+// k_xasm, one wavefront per block with its list wave-uniform (scalar loads,
+// scalar operands): a lane's work per nonzero is one address add, one LDS
+// read and one FMA.  Lists from the slots of k_coef_pack (offsets t*wt*8).
+// grid ntile * ngroup, ntile = ceil(ns^2/wt).
+__global__ __launch_bounds__(64 * kXWaves) void k_xquad(int ns, int nblk, int nr, int nrs, int wt, int ntile,
+                                                        int ngroup, const double* __restrict__ E,
+                                                        const int* __restrict__ xo, const double* __restrict__ xc,
+                                                        const int* __restrict__ xn, int qa, int qb,
+                                                        double* __restrict__ hch) {
+  extern __shared__ double El[];
+  const size_t nss = static_cast<size_t>(ns) * ns, ms = 2 * static_cast<size_t>(ns);
+  // workgroups go to the eight XCDs in turn (id mod 8); numbering them XCD
+  // by XCD puts neighbouring tiles on one XCD, so the 128-byte lines that two
+  // tiles share (a tile starts at any 8-byte alignment in the quadrant) are
+  // merged in one L2 before they are written back
+  unsigned g = blockIdx.x;
+  const unsigned per8 = gridDim.x / 8;
+  if (g < per8 * 8) g = (g % 8) * per8 + g / 8;
+  const int tile = static_cast<int>(g % ntile), grp = static_cast<int>(g / ntile);
+  const size_t p0 = static_cast<size_t>(tile) * wt;
+  {
+    const int tot = nr * wt;
+    for (int q0 = threadIdx.x; q0 < tot; q0 += 8 * 64 * kXWaves) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        // clamped addresses, so all eight loads issue without branches
+        const int q = min(q0 + u * 64 * kXWaves, tot - 1), t = q / wt;
+        const size_t pp = p0 + (q - t * wt);
+        const double e = E[static_cast<size_t>(t) * nss + min(pp, nss - 1)];
+        v[u] = pp < nss ? e : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int q = q0 + u * 64 * kXWaves;
+        if (q < tot) El[q] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const size_t p = p0 + lane;
+  const bool valid = lane < wt && p < nss;
+  const size_t x = valid ? p % ns : 0, y = valid ? p / ns : 0;
+  const size_t off = (static_cast<size_t>(qa) * ns + x) + ms * (static_cast<size_t>(qb) * ns + y);
+  // idle lanes read the tile's last column (in bounds) and write nothing
+  const char* Eb = reinterpret_cast<const char*>(El + min(lane, wt - 1));
+  const int per = (nblk + ngroup - 1) / ngroup, blo = grp * per, bhi = min(nblk, blo + per);
+  for (int blk = blo + wv; blk < bhi; blk += kXWaves) {
+    const int n = xn[blk];
+    const int* ol = xo + static_cast<size_t>(blk) * nrs;
+    const double* cl = xc + static_cast<size_t>(blk) * nrs;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j0 = 0; j0 < n; j0 += kXChunk) {
+      int oo[kXChunk];
+      double cc[kXChunk];
+#pragma unroll
+      for (int i = 0; i < kXChunk; ++i) {
+        oo[i] = ol[j0 + i];
+        cc[i] = cl[j0 + i];
+      }
+#pragma unroll
+      for (int i = 0; i < kXChunk; ++i) a[i & 3] += cc[i] * *reinterpret_cast<const double*>(Eb + oo[i]);
+    }
+    if (valid) hch[static_cast<size_t>(blk) * ms * ms + off] = (a[0] + a[1]) + (a[2] + a[3]);
+  }
+}
+
 }  // namespace
+
+int xquad_width(int nr, int device) {
+  const int rows = xasm_max_t(device);  // rows of a 64-position tile
+  if (nr <= rows) return 64;
+  if (3 * nr <= 4 * rows) return 48;
+  if (nr <= 2 * rows) return 32;
+  return 0;
+}
+
+int xquad_max_t(int device) { return 2 * xasm_max_t(device); }
+
+void launch_coef_pack_quad(int nblk, int nr, int wt, const double* C, size_t ldc, int* xo, double* xc, int* xn,
+                           hipStream_t s) {
+  if (nblk <= 0 || nr <= 0) return;
+  hipLaunchKernelGGL(k_coef_pack, dim3((nblk + 3) / 4), dim3(256), 0, s, nblk, nr, xasm_slot(nr), C, ldc,
+                     wt * static_cast<int>(sizeof(double)), xo, xc, xn);
+}
+
+int launch_xquad(int ns, int nblk, int nr, int wt, const double* E, const int* xo, const double* xc, const int* xn,
+                 int qa, int qb, double* hch, hipStream_t s) {
+  if (nblk <= 0) return 0;
+  if (nr < 1 || (wt != 64 && wt != 48 && wt != 32)) return -1;
+  const size_t nss = static_cast<size_t>(ns) * ns;
+  const int ntile = static_cast<int>((nss + wt - 1) / wt);
+  // about eight workgroups per CU in all, at least one block per wavefront
+  int ngroup = std::max(1, (8 * 256 + ntile - 1) / ntile);
+  ngroup = std::min(ngroup, std::max(1, nblk / kXWaves));
+  hipLaunchKernelGGL(k_xquad, dim3(static_cast<unsigned>(ntile) * ngroup), dim3(64 * kXWaves),
+                     static_cast<size_t>(nr) * wt * sizeof(double), s, ns, nblk, nr, xasm_slot(nr), wt, ntile, ngroup,
+                     E, xo, xc, xn, qa, qb, hch);
+  return 0;
+}
 
 int xasm_max_t(int device) {
   // the E tile of every used basis must fit the LDS a workgroup may use
@@ -174,8 +281,8 @@ int xasm_slot(int nr) { return (nr + kXChunk - 1) / kXChunk * kXChunk; }
 
 void launch_coef_pack(int nblk, int nr, const double* C, size_t ldc, int* xo, double* xc, int* xn, hipStream_t s) {
   if (nblk <= 0) return;
-  hipLaunchKernelGGL(k_coef_pack, dim3((nblk + 3) / 4), dim3(256), 0, s, nblk, nr, xasm_slot(nr), C, ldc, xo, xc,
-                     xn);
+  hipLaunchKernelGGL(k_coef_pack, dim3((nblk + 3) / 4), dim3(256), 0, s, nblk, nr, xasm_slot(nr), C, ldc, 512, xo,
+                     xc, xn);
 }
 
 namespace {

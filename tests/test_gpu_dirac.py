@@ -47,6 +47,27 @@ def test_dirac_wave(ctx, oracle_lib, case):
     _check(ctx, oracle_lib, make(), name)
 
 
+@pytest.mark.parametrize("engine", [0, 2, 4])
+def test_dirac_fp64_engines(oracle_lib, engine):
+    """Every FP64 exchange engine against the oracle: the library GEMM (0)
+    and the MFMA GEMM kernel (2) into the quadrant-major buffer, then the
+    term pass into the output layout; the sparse product (4, the default)
+    straight into the output layout, then the term pass in place (quadrants
+    without terms left as the product wrote them)."""
+    from bsr_amd import Context
+    from tests.conftest import HAS_GPU
+    if not HAS_GPU:
+        pytest.skip("no HIP device")
+    c = Context(0)
+    try:
+        c.set_gemm_engine(engine)
+        for name, make in CASES[:3]:
+            _check(c, oracle_lib, make(), f"{name}_engine{engine}")
+            assert c.dirac_engine() == engine
+    finally:
+        c.close()
+
+
 def test_dirac_reuse(ctx, oracle_lib):
     """A larger wave, then a smaller one with other orbitals, then the first
     again: resident bases are rebuilt or kept as the orbitals require."""
