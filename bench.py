@@ -385,14 +385,54 @@ def dirac_c5(steps, warmup, with_cpu):
         ms = (time.perf_counter() - t0) * 1e3 / steps
         dv = mean_dict(dev)
         nblk = w.nch * (w.nch + 1) // 2
-        flops = 4 * 2.0 * w.ns * w.ns * nblk * (w.mk + 1) * w.nb * w.nb
-        tf = flops / (dv["gemm"] * 1e-3) / 1e12 if dv["gemm"] > 0 else 0.0
+        nss = w.ns * w.ns
+        nt = (w.mk + 1) * w.nb * w.nb
+        flops = 4 * 2.0 * nss * nblk * nt
+        eng = ctx.dirac_engine()
         out = {"workload": f"Dirac wave ks=9 ns=150 (ksp=8 ksq=9) k<=12 nch={w.nch} ms={w.ms} nhm={w.nhm} "
                            f"nb={w.nb} fill=3%",
                "records": w.n_records, "elements": w.n_elements, "ms_per_wave": ms,
-               "elements_per_s": w.n_elements / (ms * 1e-3), "device_ms": dv, "engine": ctx.dirac_engine(),
-               "gemm_tflops": tf, "gemm_frac_fp64_peak": tf / PEAK_FP64_MFMA, "gemm_flops": flops,
+               "elements_per_s": w.n_elements / (ms * 1e-3), "device_ms": dv, "engine": eng,
                "device_bytes": w.footprint_bytes()}
+        if eng == 4:
+            # the sparse product writes every quadrant of every channel block
+            # once and reads each family's bases once: its algorithmic bytes
+            nbytes = 8.0 * (4 * nblk * nss + 4 * nt * nss)
+            out["product"] = {"kind": "sparse exchange product over the nonzero coefficients (4 families)",
+                              "ms": dv["gemm"], "algorithmic_bytes": nbytes,
+                              "GB_s": nbytes / (dv["gemm"] * 1e-3) / 1e9,
+                              "frac_hbm_peak": nbytes / (dv["gemm"] * 1e-3) / 1e9 / PEAK_HBM,
+                              "dense_equivalent_flops": flops}
+        else:
+            tf = flops / (dv["gemm"] * 1e-3) / 1e12 if dv["gemm"] > 0 else 0.0
+            out["product"] = {"kind": "dense FP64 GEMM (4 families)", "ms": dv["gemm"], "tflops": tf,
+                              "frac_fp64_peak": tf / PEAK_FP64_MFMA, "flops": flops}
+        # the same wave with the library FP64 GEMM (the dense product) for comparison
+        lctx = Context(0)
+        try:
+            lctx.set_gemm_engine(0)
+
+            def lstep():
+                lctx.drop_caches()
+                lctx.dirac_wave(w)
+
+            for _ in range(warmup):
+                lstep()
+            ldev = []
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                lstep()
+                ldev.append(dict(zip(("bases", "gemm", "terms_layout", "symm_vec", "total", "host_decode",
+                                      "host_stage", "call"), map(float, lctx.dirac_timings()))))
+            lms = (time.perf_counter() - t0) * 1e3 / steps
+            ldv = mean_dict(ldev)
+            ltf = flops / (ldv["gemm"] * 1e-3) / 1e12 if ldv["gemm"] > 0 else 0.0
+            out["library_gemm"] = {"engine": lctx.dirac_engine(), "ms_per_wave": lms, "device_ms": ldv,
+                                   "gemm_tflops": ltf, "gemm_frac_fp64_peak": ltf / PEAK_FP64_MFMA,
+                                   "max_rel_err_vs_sparse": lctx.dirac_diff(ctx),
+                                   "sparse_speedup": lms / ms}
+        finally:
+            lctx.close()
         # the same wave with the exchange GEMMs in split precision (int8
         # slices; 7 of them keep Dirac waves within 1e-10), and its largest
         # element error against the FP64 result

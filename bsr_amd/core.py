@@ -9,6 +9,8 @@ from __future__ import annotations
 
 import ctypes as C
 
+import os
+
 import numpy as np
 
 from . import lib as L
@@ -231,6 +233,13 @@ class Context:
         out = np.zeros(4)
         L.check(L.lib().bsrg_dirac_diag_timings(self._p, out))
         return out
+
+    def dirac_record(self, path, which, nelc_core=0):
+        """Write (which=0: start the file with the solutions, then S) or
+        append (which=1: H, mk and the asymptotic coefficients) the
+        transformed matrix as DBSR's Record_matrix
+        (src/dbsr/DBSR_MAT3_MPI/record_matrix.f90, sub1.f90:154-182,271-284)."""
+        L.check(L.lib().bsrg_dirac_record(self._p, os.fsencode(str(path)), int(which), int(nelc_core)))
 
     def dbs_convol(self, family, k, d, icase):
         ks, ns = self.dbs[0], self.dbs[1]

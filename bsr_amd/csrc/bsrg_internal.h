@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdio>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -205,5 +206,9 @@ int ozaki8_dgemm(int m, int n, int K, const void* abuf, const double* B, size_t 
 size_t ozaki_workspace_bytes(int m, int n, int K, int S);
 int ozaki_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B, size_t ldb, double* C,
                 size_t ldc, bool accumulate, int S, void* work, hipStream_t s);
+
+// One Fortran unformatted sequential record (4-byte length, data, length);
+// false on a write error or a record of 2 GiB or more.
+bool put_record(std::FILE* f, const void* data, size_t bytes);
 
 }  // namespace bsrg

@@ -1833,15 +1833,18 @@ int bsrg_wave_get(bsrg_ctx* c, int which, double* hcc, double* hcb, double* hbb,
 
 }  // extern "C"
 
-namespace {
+namespace bsrg {
 
-// One Fortran unformatted sequential record: 4-byte length, data, length.
 bool put_record(std::FILE* f, const void* data, size_t bytes) {
   if (bytes > 0x7fffffffu) return false;
   const int32_t n = static_cast<int32_t>(bytes);
   return std::fwrite(&n, 4, 1, f) == 1 && (bytes == 0 || std::fwrite(data, 1, bytes, f) == bytes) &&
          std::fwrite(&n, 4, 1, f) == 1;
 }
+
+}  // namespace bsrg
+
+namespace {
 
 // Record_matrix, src/bsr/BSR_MAT3/record_matrix.f90:2-58: diagonal blocks,
 // then non-zero lower blocks with (i,j) headers, channel-perturber vectors,

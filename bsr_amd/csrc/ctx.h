@@ -319,6 +319,10 @@ struct DiracWave {
   int ovl_n = 0;
   std::vector<double> ovl;
   double dms[4] = {0, 0, 0, 0};  // overlap pass, channel diagonalisation, S and H transforms
+  bool tr[2] = {false, false};    // S / H transformed with the current solutions
+  DevVec<double> rec;             // Record_matrix staging: block corners, then their maxima
+  DevVec<int> reci;
+  DevVec<size_t> reco;
 };
 
 }  // namespace bsrg

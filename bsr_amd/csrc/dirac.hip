@@ -591,7 +591,7 @@ int bsrg_dirac_assemble(bsrg_ctx* c, int nch, int npert, int mk, int ntarg, cons
     exch = exch || itype[r] == 4;
   }
   if (nk > 64) return fail(-1, "bsrg_dirac_assemble: kpol > 63");
-  W.ok = false;
+  W.ok = W.tr[1] = false;
   (void)hipEventRecord(c->ev[0], s);
   if (int rc = dirac_bases(c, nk, exch)) return rc;
   (void)hipEventRecord(c->ev[1], s);

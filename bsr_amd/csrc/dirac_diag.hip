@@ -148,6 +148,34 @@ __global__ __launch_bounds__(256) void k_piece_max(size_t n, const double* __res
   if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
+// Record_matrix staging: off-diagonal block b (block number bk[b], rows ni
+// = isol[bi[b]-1], columns nj = isol[bj[b]-1] of its top-left corner) packed
+// column by column at out + off[b], its largest |element| in cmax[b].
+// grid (nb), block 256.
+__global__ __launch_bounds__(256) void k_corner_pack(int ms, const int* __restrict__ isol, const int* __restrict__ bi,
+                                                     const int* __restrict__ bj, const int* __restrict__ bk,
+                                                     const size_t* __restrict__ off, const double* __restrict__ H,
+                                                     double* __restrict__ out, double* __restrict__ cmax) {
+  const int b = blockIdx.x, ni = isol[bi[b] - 1], nj = isol[bj[b] - 1];
+  const double* h = H + static_cast<size_t>(bk[b]) * ms * ms;
+  double* o = out + off[b];
+  const size_t n = static_cast<size_t>(ni) * nj;
+  double m = 0.0;
+  for (size_t q = threadIdx.x; q < n; q += 256) {
+    const double v = h[q % ni + static_cast<size_t>(ms) * (q / ni)];
+    o[q] = v;
+    m = fmax(m, fabs(v));
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cmax[b] = red[0];
+}
+
 // The reduced problems of one group of channels (same kept count n): member
 // g is channel chans[g] (0-based), keep[ich*ms + r] its r-th kept B-spline;
 // A from the H diagonal block, B from the S diagonal block, both n x n.
@@ -424,7 +452,7 @@ int bsrg_dirac_overlap(bsrg_ctx* c, int nch, int npert, int ntarg, const int* ip
   const size_t mss = static_cast<size_t>(ms) * ms, nblk = static_cast<size_t>(nch) * (nch + 1) / 2;
   const size_t ncol = static_cast<size_t>(nch) * npert;
   hipStream_t s = c->stream;
-  W.s_ok = W.sol_ok = W.ovl_ok = false;
+  W.s_ok = W.sol_ok = W.ovl_ok = W.tr[0] = false;
   W.s_nch = nch;
   W.s_npert = npert;
   W.s_ntarg = ntarg;
@@ -532,6 +560,7 @@ int bsrg_dirac_update_hx(bsrg_ctx* c, int which, int nmat, const double* d, int 
     return fail(-6, which == 0 ? "no overlap matrix (bsrg_dirac_overlap)" : "no assembled Dirac wave (bsrg_dirac_assemble)");
   if (nmat < 1 || !d || nrec < 0 || (nrec > 0 && (!ich || !jch || !imat || !cc)))
     return fail(-1, "bsrg_dirac_update_hx: missing arguments");
+  W.tr[which] = false;
   const int nch = which == 0 ? W.s_nch : W.nch;
   const int ms = 2 * c->dbs.T.ns;
   const size_t mss = static_cast<size_t>(ms) * ms, nblk = static_cast<size_t>(nch) * (nch + 1) / 2;
@@ -558,6 +587,7 @@ int bsrg_dirac_energy_shift(bsrg_ctx* c, double ecore, const double* etarg) {
   if (!W.ok) return fail(-6, "no assembled Dirac wave (bsrg_dirac_assemble)");
   if (!W.s_ok) return fail(-6, "no overlap matrix (bsrg_dirac_overlap)");
   if (W.s_nch != W.nch) return fail(-1, "bsrg_dirac_energy_shift: S and H of different channel counts");
+  W.tr[1] = false;
   if (!etarg) return fail(-2, "null argument");
   HIPCHK(hipSetDevice(c->device));
   const int ms = 2 * c->dbs.T.ns, nch = W.nch;
@@ -677,6 +707,7 @@ int bsrg_dirac_diag_channels(bsrg_ctx* c, const int* lch, const int* kch, int il
   }
   if (nsol) *nsol = tot;
   W.sol_ok = true;
+  W.tr[0] = W.tr[1] = false;
   W.ovl_ok = false;
   return 0;
 }
@@ -793,6 +824,7 @@ int bsrg_dirac_transform(bsrg_ctx* c, int which) {
   }
   HIPCHK(hipStreamSynchronize(s));
   W.dms[2 + which] = elapsed(c->ev[0], c->ev[1]);
+  W.tr[which] = true;
   return 0;
 }
 
@@ -806,6 +838,135 @@ int bsrg_dirac_overlaps(bsrg_ctx* c, double* ov) {
 int bsrg_dirac_diag_timings(bsrg_ctx* c, double* ms4) {
   if (!c || !ms4) return fail(-2, "null argument");
   std::copy(c->dw.dms, c->dw.dms + 4, ms4);
+  return 0;
+}
+
+int bsrg_dirac_record(bsrg_ctx* c, const char* path, int which, int nelc_core) {
+  if (int rc = dcheck(c)) return rc;
+  if (!path) return fail(-2, "null argument");
+  auto& W = c->dw;
+  if (which != 0 && which != 1) return fail(-1, "bsrg_dirac_record: which must be 0 (S) or 1 (H)");
+  if (!W.sol_ok || !W.tr[which])
+    return fail(-6, which == 0 ? "no transformed overlap matrix (bsrg_dirac_transform(ctx, 0))"
+                               : "no transformed Hamiltonian matrix (bsrg_dirac_transform(ctx, 1))");
+  HIPCHK(hipSetDevice(c->device));
+  const auto& T = c->dbs.T;
+  const int ns = T.ns, ms = 2 * ns, nch = which == 0 ? W.s_nch : W.nch, npert = which == 0 ? W.s_npert : W.npert;
+  const std::vector<int>& isol = W.ipsol;
+  if (static_cast<int>(isol.size()) != nch) return fail(-1, "bsrg_dirac_record: solutions of another channel count");
+  hipStream_t s = c->stream;
+  // the off-diagonal blocks' corners and maxima, in the order they are written
+  const int nb = nch * (nch - 1) / 2;
+  std::vector<int> bi(std::max(nb, 1)), bj(std::max(nb, 1)), bk(std::max(nb, 1));
+  std::vector<size_t> off(std::max(nb, 1) + 1, 0);
+  {
+    int b = 0;
+    for (int ich = 2; ich <= nch; ++ich)
+      for (int jch = 1; jch < ich; ++jch, ++b) {
+        bi[b] = ich;
+        bj[b] = jch;
+        bk[b] = static_cast<int>(dblk(nch, ich, jch));
+        off[b + 1] = off[b] + static_cast<size_t>(isol[ich - 1]) * isol[jch - 1];
+      }
+  }
+  const size_t tot = off[nb];
+  std::vector<double> corners(tot), cmax(nb);
+  if (nb > 0) {
+    HIPCHK(W.rec.resize(tot + nb));
+    HIPCHK(W.reci.resize(3 * static_cast<size_t>(nb)));
+    HIPCHK(W.reco.resize(nb));
+    HIPCHK(W.isold.upload(isol.data(), isol.size(), s));
+    HIPCHK(hipMemcpyAsync(W.reci.p, bi.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(W.reci.p + nb, bj.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(W.reci.p + 2 * nb, bk.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(W.reco.p, off.data(), nb * sizeof(size_t), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_corner_pack, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s, ms, W.isold.p, W.reci.p,
+                       W.reci.p + nb, W.reci.p + 2 * nb, W.reco.p, which == 0 ? W.shch.p : W.hch.p, W.rec.p,
+                       W.rec.p + tot);
+    HIPCHK(hipGetLastError());
+    if (tot) HIPCHK(hipMemcpyAsync(corners.data(), W.rec.p, tot * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(cmax.data(), W.rec.p + tot, nb * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  const size_t ncol = static_cast<size_t>(nch) * npert;
+  std::vector<double> hcp(ncol * ms);
+  if (ncol)
+    HIPCHK(hipMemcpyAsync(hcp.data(), which == 0 ? W.shcp.p : W.hcp.p, hcp.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  std::vector<double> sol;
+  if (which == 0) {
+    sol.resize(static_cast<size_t>(nch) * ms * ms);
+    HIPCHK(hipMemcpyAsync(sol.data(), W.sol.p, sol.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  const std::vector<double>& hp = which == 0 ? W.shp : W.hp;
+  std::FILE* f = std::fopen(path, which == 0 ? "wb" : "ab");
+  if (!f) return fail(-40, std::string("cannot open ") + path);
+  bool ok = true;
+  if (which == 0) {
+    // sub1.f90:154-158: sizes, nsol, jpsol(0:nch), eval(nsol), the solution vectors
+    std::vector<int32_t> jpsol(nch + 1, 0);
+    for (int i = 1; i <= nch; ++i) jpsol[i] = jpsol[i - 1] + isol[i - 1];
+    const int32_t nsol = jpsol[nch];
+    std::vector<double> eval, vecs;
+    eval.reserve(nsol);
+    vecs.reserve(static_cast<size_t>(nsol) * ms);
+    for (int ich = 0; ich < nch; ++ich) {
+      const double* d = sol.data() + static_cast<size_t>(ich) * ms * ms;
+      for (int js = 0; js < isol[ich]; ++js) {
+        eval.push_back(d[js + static_cast<size_t>(ms) * (ms - 1)]);
+        vecs.insert(vecs.end(), d + static_cast<size_t>(js) * ms, d + static_cast<size_t>(js + 1) * ms);
+      }
+    }
+    const int32_t head[5] = {ns, nch, npert, T.nsb[0], T.nsb[1]};
+    ok = put_record(f, head, sizeof(head)) && put_record(f, &nsol, 4) &&
+         put_record(f, jpsol.data(), jpsol.size() * 4) && put_record(f, eval.data(), eval.size() * 8) &&
+         put_record(f, vecs.data(), vecs.size() * 8);
+  }
+  // record_matrix.f90:12-49: channel blocks with an element of 1e-20 or more,
+  // channel-perturber vectors likewise, non-zero perturber elements, (0,0)
+  for (int b = 0; b < nb && ok; ++b) {
+    if (cmax[b] < 1e-20) continue;
+    const int32_t hdr[2] = {bi[b], bj[b]};
+    ok = put_record(f, hdr, 8) && put_record(f, corners.data() + off[b], (off[b + 1] - off[b]) * 8);
+  }
+  for (int ich = 1; ich <= nch && ok; ++ich)
+    for (int ip = 1; ip <= npert && ok; ++ip) {
+      const double* v = hcp.data() + (static_cast<size_t>(ich - 1) * npert + ip - 1) * ms;
+      const int ni = isol[ich - 1];
+      double m = 0.0;
+      for (int q = 0; q < ni; ++q) m = std::max(m, std::fabs(v[q]));
+      if (m < 1e-20) continue;
+      const int32_t hdr[2] = {ip + nch, ich};
+      ok = put_record(f, hdr, 8) && put_record(f, v, static_cast<size_t>(ni) * 8);
+    }
+  for (int ip = 1; ip <= npert && ok; ++ip)
+    for (int jp = 1; jp <= ip && ok; ++jp) {
+      const double v = hp[static_cast<size_t>(ip) * (ip - 1) / 2 + jp - 1];
+      if (v == 0.0) continue;
+      const int32_t hdr[2] = {ip + nch, jp + nch};
+      ok = put_record(f, hdr, 8) && put_record(f, &v, 8);
+    }
+  const int32_t end[2] = {0, 0};
+  ok = ok && put_record(f, end, 8);
+  if (which == 1 && ok) {
+    // sub1.f90:249-253, 277-284: the core electrons on the diagonal k = 0
+    // coefficients, |acf| < 1e-5 cut to zero, then mk and the nch x nch
+    // matrix of every k
+    const size_t nblk = static_cast<size_t>(nch) * (nch + 1) / 2;
+    std::vector<double> a(W.acf);
+    for (int ich = 1; ich <= nch; ++ich) a[dblk(nch, ich, ich)] += 2.0 * nelc_core;
+    for (double& x : a)
+      if (std::fabs(x) < 0.00001) x = 0.0;
+    std::vector<double> full(static_cast<size_t>(nch) * nch * (W.mk + 1));
+    size_t q = 0;
+    for (int k = 0; k <= W.mk; ++k)
+      for (int jch = 1; jch <= nch; ++jch)
+        for (int ich = 1; ich <= nch; ++ich) full[q++] = a[dblk(nch, ich, jch) + nblk * k];
+    const int32_t mk = W.mk;
+    ok = put_record(f, &mk, 4) && put_record(f, full.data(), full.size() * 8);
+  }
+  if (std::fclose(f) != 0) ok = false;
+  if (!ok) return fail(-41, std::string("write failed: ") + path);
   return 0;
 }
 

@@ -102,6 +102,7 @@ SIGNATURES = {
     "bsrg_dirac_transform": [C.c_void_p, C.c_int],
     "bsrg_dirac_overlaps": [C.c_void_p, _dp],
     "bsrg_dirac_diag_timings": [C.c_void_p, _dp],
+    "bsrg_dirac_record": [C.c_void_p, C.c_char_p, C.c_int, C.c_int],
     "bsrg_matrix_record": [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.c_int],
     "bsrg_matrix_resume": [C.c_void_p, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "bsrg_lb_create": [],

@@ -222,6 +222,23 @@ int bsrg_dirac_overlaps(bsrg_ctx* ctx, double* ov);
 /* Device ms of the last overlap pass, channel diagonalisation, S and H
  * transforms. */
 int bsrg_dirac_diag_timings(bsrg_ctx* ctx, double* ms4);
+/* Record_matrix of DBSR (src/dbsr/DBSR_MAT3_MPI/record_matrix.f90:2-51)
+ * for the last transform of S (which = 0) or H (which = 1), as sub1.f90
+ * writes dbsr_mat.nnn (unformatted sequential records):
+ *   which = 0 starts the file (sub1.f90:154-158: ns, nch, npert, nsp, nsq;
+ *   nsol; jpsol(0:nch); eval(nsol); the solution vectors) and adds the
+ *   transformed S (sub1.f90:182);
+ *   which = 1 appends the transformed H (sub1.f90:271), then mk and the
+ *   asymptotic coefficients acf(icc(ich,jch),k) for all ich, jch, k
+ *   (sub1.f90:283-284) after the core term 2*nelc_core on the diagonal k=0
+ *   coefficients (:249-253) and the |acf| < 1e-5 cut (:277-279).
+ * Record_matrix writes (ich,jch) and the isol(ich) x isol(jch) corner of
+ * every off-diagonal channel block with an element of at least 1e-20, then
+ * (ip+nch,ich) and the corner of each channel-perturber vector likewise,
+ * (ip+nch,jp+nch) and each non-zero perturber element, and (0,0).
+ * -6 when that matrix has not been transformed with the current solutions
+ * (an update, a new assembly or new solutions since); -40/-41 file errors. */
+int bsrg_dirac_record(bsrg_ctx* ctx, const char* path, int which, int nelc_core);
 
 /* Orbitals of the wave: pbs(ns,nbf); iech(i)=channel of orbital i (0 for
  * bound orbitals); lbs(i)=l; ibort(nbf,nbf) nonzero where the overlap

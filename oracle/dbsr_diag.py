@@ -236,3 +236,77 @@ def energy_shift(hch, shch, fpq, iptar, ecore, etarg):
         c = etarg[int(iptar[ich]) - 1] - ecore
         h[:, :, ich] = h[:, :, ich] + ecore * shch[:, :, ich] + c * fpq
     return h
+
+
+def record_matrix(hch, hcp, hp, ipsol, nch, npert):
+    """Record_matrix (src/dbsr/DBSR_MAT3_MPI/record_matrix.f90:2-51) as the
+    list of records it writes: for every off-diagonal channel block whose
+    isol(ich) x isol(jch) corner has an element of at least 1e-20, (ich,
+    jch) and the corner (column-major); then (ip+nch, ich) and the corner of
+    each channel-perturber vector likewise; (ip+nch, jp+nch) and every
+    non-zero perturber element; (0, 0)."""
+    recs = []
+    for ich in range(2, nch + 1):
+        ni = ipsol[ich - 1]
+        for jch in range(1, ich):
+            nj = ipsol[jch - 1]
+            c = hch[:ni, :nj, dblk(nch, ich, jch)]
+            if _maxabs(c) < 1e-20:
+                continue
+            recs += [np.array([ich, jch], np.int32), np.asfortranarray(c).ravel(order="F")]
+    for ich in range(1, nch + 1):
+        ni = ipsol[ich - 1]
+        for ip in range(1, npert + 1):
+            v = hcp[:ni, (ich - 1) * npert + ip - 1]
+            if _maxabs(v) < 1e-20:
+                continue
+            recs += [np.array([ip + nch, ich], np.int32), np.array(v)]
+    for ip in range(1, npert + 1):
+        for jp in range(1, ip + 1):
+            x = hp[ip * (ip - 1) // 2 + jp - 1]
+            if x == 0.0:
+                continue
+            recs += [np.array([ip + nch, jp + nch], np.int32), np.array([x])]
+    recs.append(np.array([0, 0], np.int32))
+    return recs
+
+
+def solution_records(ns, nch, npert, nsp, nsq, diag, ipsol):
+    """The records sub1.f90:154-158 writes before the transformed overlap
+    matrix: (ns, nch, npert, nsp, nsq); nsol; jpsol(0:nch); eval(nsol) =
+    diag(is, ms, ich) in channel order; the solution vectors diag(:, js, ich)."""
+    ms = diag.shape[0]
+    jpsol = np.concatenate([[0], np.cumsum(ipsol)]).astype(np.int32)
+    ev = np.concatenate([diag[:k, ms - 1, i] for i, k in enumerate(ipsol)] + [np.zeros(0)])
+    vec = np.concatenate([diag[:, :k, i].ravel(order="F") for i, k in enumerate(ipsol)] + [np.zeros(0)])
+    return [np.array([ns, nch, npert, nsp, nsq], np.int32), np.array([jpsol[-1]], np.int32), jpsol, ev, vec]
+
+
+def acf_records(acf, nch, mk, nelc_core):
+    """sub1.f90:249-253 and :277-284: acf(i,0) += 2 nelc_core on the diagonal
+    blocks, |acf| < 1e-5 set to zero, then mk and acf(icc(ich,jch),k) for
+    ich fastest, then jch, then k; acf[blk + nblk*k] in the block numbering
+    of dblk."""
+    nblk = nch * (nch + 1) // 2
+    a = np.array(acf, dtype=np.float64, copy=True)
+    for ich in range(1, nch + 1):
+        a[dblk(nch, ich, ich)] += 2 * nelc_core
+    a[np.abs(a) < 0.00001] = 0.0
+    full = np.array([a[dblk(nch, ich, jch) + nblk * k] for k in range(mk + 1) for jch in range(1, nch + 1)
+                     for ich in range(1, nch + 1)])
+    return [np.array([mk], np.int32), full]
+
+
+def read_records(path):
+    """The records of a Fortran unformatted sequential file (4-byte
+    markers), as raw bytes."""
+    out = []
+    with open(path, "rb") as f:
+        data = f.read()
+    p = 0
+    while p < len(data):
+        n = int(np.frombuffer(data[p:p + 4], np.int32)[0])
+        out.append(data[p + 4:p + 4 + n])
+        assert int(np.frombuffer(data[p + 4 + n:p + 8 + n], np.int32)[0]) == n
+        p += n + 8
+    return out

@@ -31,7 +31,7 @@ SRCS := \
 OBJS := $(addprefix $(OUT)/,$(notdir $(SRCS:.f90=.o)))
 
 # Dirac B-spline routines and the Dirac assembly of DBSR_MAT3 (R_data,
-# O_data, updates, Target_h, Zero_cond, transform_matrix with the modules
+# O_data, updates, Target_h, Zero_cond, transform_matrix, Record_matrix with the modules
 # they read): a library of their own
 # (routine names such as convol and density repeat those of the BS library)
 DSRCS := \
@@ -51,7 +51,8 @@ DSRCS := \
   $(REF)/dbsr/DBSR_MAT3_MPI/mod_dbsr_mat.f90 $(REF)/dbsr/DBSR_MAT3_MPI/MOD_c_data.f90 \
   $(REF)/dbsr/DBSR_MAT3_MPI/updates.f90 $(REF)/dbsr/DBSR_MAT3_MPI/target_subs.f90 \
   $(REF)/dbsr/DBSR_MAT3_MPI/R_data.f90 $(REF)/dbsr/DBSR_MAT3_MPI/O_data.f90 \
-  $(REF)/dbsr/DBSR_MAT3_MPI/zero_cond.f90 $(REF)/dbsr/DBSR_MAT3_MPI/transform_matrix.f90
+  $(REF)/dbsr/DBSR_MAT3_MPI/zero_cond.f90 $(REF)/dbsr/DBSR_MAT3_MPI/transform_matrix.f90 \
+  $(REF)/dbsr/DBSR_MAT3_MPI/record_matrix.f90
 # The BSR_MAT3 list builder's routines (Read_dets with the int_bnk header
 # readers, Decode_INT, Ifind_channel, Jsym_int, Det_fact, Check_det, VDET,
 # DET, Idef_itype) with the modules they read: a library of their own,

@@ -177,3 +177,21 @@ subroutine ref_dbsr_get_overlaps(n, ov_o) bind(C, name="ref_dbsr_get_overlaps")
   ov_o = 0.d0
   if (allocated(overlaps)) ov_o(1:nch+npert, 1:nch+npert) = overlaps(1:nch+npert, 1:nch+npert)
 end subroutine
+
+subroutine ref_dbsr_record(path, n) bind(C, name="ref_dbsr_record")
+  ! Record_matrix (record_matrix.f90) of the module matrices, written to a
+  ! new file on the module's unit nui
+  use iso_c_binding
+  use dbsr_mat
+  implicit none
+  integer(c_int), value :: n
+  character(kind=c_char) :: path(n)
+  character(len=n) :: fn
+  integer :: i
+  do i = 1, n
+    fn(i:i) = path(i)
+  end do
+  open(nui, file=fn, form='unformatted', status='replace')
+  call Record_matrix
+  close(nui)
+end subroutine

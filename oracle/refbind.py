@@ -294,6 +294,7 @@ def dbs_lib():
             "ref_dbsr_zero_cond": (None, [_ip, _ip, i, i, i, i, _ip, _ip]),
             "ref_dbsr_transform": (None, [i, _dp, _ip]),
             "ref_dbsr_get_overlaps": (None, [i, _dp]),
+            "ref_dbsr_record": (None, [C.c_char_p, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -389,6 +390,11 @@ class RefDbsr:
     def transform(self, icase, diag, ipsol):
         """transform_matrix (transform_matrix.f90:2-66) on the module matrices."""
         dbs_lib().ref_dbsr_transform(int(icase), np.asfortranarray(diag, dtype=np.float64), _i32(ipsol))
+
+    def record(self, path):
+        """Record_matrix (record_matrix.f90:2-51) of the module matrices into a new file."""
+        b = os.fsencode(str(path))
+        dbs_lib().ref_dbsr_record(b, len(b))
 
     def overlaps(self, nch):
         n = nch + self.npert

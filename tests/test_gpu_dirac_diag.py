@@ -198,7 +198,7 @@ def _vec_check(got, ref, before, diag, ipsol, nch, npert, ms):
 
 
 @pytest.mark.parametrize("name", ["small", "mid", "c5"])
-def test_transform(ctx, name):
+def test_transform(ctx, name, tmp_path):
     from oracle import dbsr_diag as D
     w = _wave(name)
     H, S = _diag_inputs(ctx, w)
@@ -214,11 +214,46 @@ def test_transform(ctx, name):
     _vec_check(St["hcp"], v, S["hcp"], diag, ipsol, w.nch, w.npert, ms)
     assert np.array_equal(ov == 0, ovr == 0)
     assert np.allclose(ov, ovr, rtol=1e-10, atol=0)
+    path = tmp_path / "dbsr_mat.nnn"
+    ctx.dirac_record(path, 0)
+    nS = len(D.read_records(path))
     ctx.dirac_transform(1)
     Ht = ctx.dirac_get()
     h, v, _ = D.transform(H["hch"], H["hcp"], diag, ipsol, w.nch, w.npert)
     wh = _bound_check(Ht["hch"], h, H["hch"], diag, ipsol, w.nch, ms)
     _vec_check(Ht["hcp"], v, H["hcp"], diag, ipsol, w.nch, w.npert, ms)
+    ctx.dirac_record(path, 1, nelc_core=3)
+    _check_file(path, nS, w, diag, ipsol, St, Ht, h, v)
     t = ctx.dirac_diag_timings()
     _say(f"[transform] {name}: S {ws / EPS:.1f} eps, H {wh / EPS:.1f} eps of the bound; "
          f"ms: overlap {t[0]:.2f} diag {t[1]:.2f} S {t[2]:.2f} H {t[3]:.2f}")
+
+
+def _check_file(path, nS, w, diag, ipsol, St, Ht, h_ora, v_ora):
+    """The dbsr_mat.nnn records against the restatements (oracle/dbsr_diag.py,
+    Record_matrix pinned against the reference's own by
+    tests/test_oracle_dbsr_diag.py): byte for byte on the GPU's own
+    transformed matrices; against the oracle's transform the same records
+    (same skipped blocks), values within the transform's rounding bound."""
+    from oracle import dbsr_diag as D
+    got = D.read_records(path)
+    nv = w.ns - w.ks + 1
+    head = D.solution_records(w.ns, w.nch, w.npert, nv + w.ksp - 1, nv + w.ksq - 1, diag, ipsol)
+    recS = D.record_matrix(St["hch"], St["hcp"], St["hp"], ipsol, w.nch, w.npert)
+    recH = D.record_matrix(Ht["hch"], Ht["hcp"], Ht["hp"], ipsol, w.nch, w.npert)
+    tail = D.acf_records(Ht["acf"].ravel(order="F"), w.nch, w.mk, 3)
+    want = head + recS + recH + tail
+    assert nS == len(head) + len(recS)
+    assert len(got) == len(want)
+    for i, (a, b) in enumerate(zip(got, want)):
+        assert a == b.tobytes(), i
+    ora = D.record_matrix(h_ora, v_ora, Ht["hp"], ipsol, w.nch, w.npert)
+    assert len(ora) == len(recH)
+    for a, b in zip(recH, ora):
+        assert a.dtype == b.dtype and a.shape == b.shape
+        if a.dtype == np.int32:
+            assert np.array_equal(a, b)
+        else:
+            # the elements themselves are held to the rounding bound above
+            assert np.allclose(a, b, rtol=0, atol=1e-10 * float(np.abs(b).max()))
+    _say(f"[record] {w.name}: {len(got)} records, {len(recS) // 2} S and {len(recH) // 2} H pieces")

@@ -174,7 +174,7 @@ def test_diag_channels_pins(waves, name):
 
 @needs_ref
 @pytest.mark.parametrize("name", ["small", "mid"])
-def test_transform_vs_reference(waves, name):
+def test_transform_vs_reference(waves, name, tmp_path):
     w = waves[name]
     fpq = _fpq(w)
     H, _ = _diag_setup(w)
@@ -185,6 +185,7 @@ def test_transform_vs_reference(waves, name):
     diag, ipsol = D.diag_channels(hd, S["hch"][:, :, : w.nch], iprm, w.nch, -1e5, 2e5, 3e3)
     r.transform(0, diag, ipsol)
     St, ovr = r.get(), r.overlaps(w.nch)
+    _assert_record_matrix(r, tmp_path / "s.mat", St, ipsol, w)
     h, v, ov = D.transform(S["hch"], S["hcp"], diag, ipsol, w.nch, w.npert, S["hp"])
     assert np.array_equal(h[:, :, : w.nch], St["hch"][:, :, : w.nch])    # diagonal blocks untouched
     _assert_transform_close(h, St["hch"], S["hch"], diag, ipsol, w.nch)
@@ -197,9 +198,25 @@ def test_transform_vs_reference(waves, name):
     H0 = RH.get()
     RH.transform(1, diag, ipsol)
     Ht = RH.get()
+    _assert_record_matrix(RH, tmp_path / "h.mat", Ht, ipsol, w)
     h, v, _ = D.transform(H0["hch"], H0["hcp"], diag, ipsol, w.nch, w.npert)
     _assert_transform_close(h, Ht["hch"], H0["hch"], diag, ipsol, w.nch)
     _assert_vectors_close(v, Ht["hcp"], H0["hcp"], diag, ipsol, w.nch, w.npert)
+
+
+def _assert_record_matrix(r, path, M, ipsol, w):
+    """The reference's Record_matrix (record_matrix.f90) over its transformed
+    module matrices, byte for byte against the restatement on the same
+    matrices; the skipped blocks are the all-zero ones."""
+    r.record(path)
+    got = D.read_records(path)
+    ref = D.record_matrix(M["hch"], M["hcp"], M["hp"], ipsol, w.nch, w.npert)
+    assert len(got) == len(ref)
+    for a, b in zip(got, ref):
+        assert a == b.tobytes()
+    nblk = sum(1 for i in range(2, w.nch + 1) for j in range(1, i)
+               if np.abs(M["hch"][:, :, D.dblk(w.nch, i, j)]).max() >= 1e-20 and ipsol[i - 1] and ipsol[j - 1])
+    assert sum(1 for a in got if len(a) == 8 and 0 < np.frombuffer(a, np.int32)[0] <= w.nch) == nblk
 
 
 def _assert_vectors_close(got, ref, before, diag, ipsol, nch, npert):
