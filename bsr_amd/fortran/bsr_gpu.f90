@@ -20,6 +20,14 @@
 !    gpu_dirac_r_data(...)         Gen_matrix icase 2 = R_data(itype,kpol,0..3)
 !                                  over every list   src/dbsr/DBSR_MAT3_MPI/R_data.f90:2
 !    gpu_dirac_get(...)            the accumulators hch, hcp, hp, acf, htarg
+!    gpu_dirac_overlap(...)        UPDATE_HX(fppqq) + O_data  sub1.f90:77-79, O_data.f90:2
+!    gpu_dirac_diag(...)           Zero_cond + Diag_channels  diag.f90:2, zero_cond.f90:2
+!    bsrg_dirac_transform(ctx, w)  transform_matrix           transform_matrix.f90:2
+!    gpu_dirac_record(...)         Record_matrix + SUB1's records  record_matrix.f90:2
+!  the list builder (BSR_MAT3 State_res, src/bsr/BSR_MAT3/state_res.f90:2):
+!    gpu_lb_open(path) .. gpu_lb_run(icase, mk) .. bsrg_lb_lists
+!  and BSR_MAT4's interrupted runs: gpu_matrix_record, gpu_matrix_resume
+!  (src/bsr/BSR_MAT4/state_res.f90:187-196, read_matrix.f90:2).
 !  and the closing eigenvalue step of BSR_HD3 (diag_hd.f90:271-312, full
 !  basis): bsrg_wave_eigen (assembled wave), bsrg_matfile_eigen (bsr_mat.nnn).
 !  A non-zero status stops the program with the library's message, as the
@@ -210,6 +218,133 @@ module bsr_gpu
       type(c_ptr), value :: ctx
       real(c_double) :: hch(*), hcp(*), hp(*), acf(*), htarg(*)
     end function
+    integer(c_int) function bsrg_dirac_overlap(ctx, nch, npert, ntarg, iptar, nrec, itype, k1, k2, cdata, &
+        eps_c, iitar) bind(C, name="bsrg_dirac_overlap")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nch, npert, ntarg, nrec, iitar
+      integer(c_int) :: iptar(*), itype(*), k1(*), k2(*)
+      real(c_double) :: cdata(*)
+      real(c_double), value :: eps_c
+    end function
+    integer(c_int) function bsrg_dirac_overlap_get(ctx, hch, hcp, hp, otarg) bind(C, name="bsrg_dirac_overlap_get")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: hch(*), hcp(*), hp(*), otarg(*)
+    end function
+    integer(c_int) function bsrg_dirac_update_hx(ctx, which, nmat, d, nrec, ich, jch, imat, c) &
+        bind(C, name="bsrg_dirac_update_hx")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which, nmat, nrec
+      real(c_double) :: d(*), c(*)
+      integer(c_int) :: ich(*), jch(*), imat(*)
+    end function
+    integer(c_int) function bsrg_dirac_energy_shift(ctx, ecore, etarg) bind(C, name="bsrg_dirac_energy_shift")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: ecore
+      real(c_double) :: etarg(*)
+    end function
+    integer(c_int) function bsrg_dirac_diag_channels(ctx, lch, kch, ilzero, jlzero, ibzero, jbzero, edmin, edmax, &
+        egap, ipsol, nsol) bind(C, name="bsrg_dirac_diag_channels")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int) :: lch(*), kch(*), ipsol(*), nsol
+      integer(c_int), value :: ilzero, jlzero, ibzero, jbzero
+      real(c_double), value :: edmin, edmax, egap
+    end function
+    integer(c_int) function bsrg_dirac_solutions(ctx, diag, eval, iprm) bind(C, name="bsrg_dirac_solutions")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: diag(*), eval(*)
+      integer(c_int) :: iprm(*)
+    end function
+    integer(c_int) function bsrg_dirac_transform(ctx, which) bind(C, name="bsrg_dirac_transform")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: which
+    end function
+    integer(c_int) function bsrg_dirac_overlaps(ctx, ov) bind(C, name="bsrg_dirac_overlaps")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: ov(*)
+    end function
+    integer(c_int) function bsrg_dirac_record(ctx, path, which, nelc_core) bind(C, name="bsrg_dirac_record")
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: path(*)
+      integer(c_int), value :: which, nelc_core
+    end function
+    integer(c_int) function bsrg_matrix_record(ctx, path, interrupt, intercase, max_nbuf) &
+        bind(C, name="bsrg_matrix_record")
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: path(*)
+      integer(c_int), value :: interrupt, intercase, max_nbuf
+    end function
+    integer(c_int) function bsrg_matrix_resume(ctx, path, interrupt, intercase, max_nbuf) &
+        bind(C, name="bsrg_matrix_resume")
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: path(*)
+      integer(c_int) :: interrupt, intercase, max_nbuf
+    end function
+    function bsrg_lb_create() bind(C, name="bsrg_lb_create") result(p)
+      import :: c_ptr
+      type(c_ptr) :: p
+    end function
+    subroutine bsrg_lb_destroy(lb) bind(C, name="bsrg_lb_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: lb
+    end subroutine
+    integer(c_int) function bsrg_lb_read_bank(lb, path, sizes5) bind(C, name="bsrg_lb_read_bank")
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: lb
+      character(kind=c_char) :: path(*)
+      integer(c_int) :: sizes5(5)
+    end function
+    integer(c_int) function bsrg_lb_set_orbitals(lb, nbf, iech, lef, ibort, obs, nelc) &
+        bind(C, name="bsrg_lb_set_orbitals")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: lb
+      integer(c_int), value :: nbf, nelc
+      integer(c_int) :: iech(*), lef(*), ibort(*)
+      real(c_double) :: obs(*)
+    end function
+    integer(c_int) function bsrg_lb_set_channels(lb, nch, ipconf, npert, ippert) bind(C, name="bsrg_lb_set_channels")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: lb
+      integer(c_int), value :: nch, npert
+      integer(c_int) :: ipconf(*), ippert(*)
+    end function
+    integer(c_int) function bsrg_lb_set_states(lb, ncfg, ip_state, no_state, wc, lorb, ip_orb, nsym, it_state1, &
+        it_state2, nstat, ip_stat) bind(C, name="bsrg_lb_set_states")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: lb
+      integer(c_int), value :: ncfg, lorb, nsym, nstat
+      integer(c_int) :: ip_state(*), no_state(*), ip_orb(*), it_state1(*), it_state2(*), ip_stat(*)
+      real(c_double) :: wc(*)
+    end function
+    integer(c_int) function bsrg_lb_set_params(lb, eps_c, eps_det, eps_soo, nb, mb, kb) &
+        bind(C, name="bsrg_lb_set_params")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: lb
+      real(c_double), value :: eps_c, eps_det, eps_soo
+      integer(c_int), value :: nb, mb, kb
+    end function
+    integer(c_int) function bsrg_lb_run(lb, icase, mk, sizes3) bind(C, name="bsrg_lb_run")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: lb
+      integer(c_int), value :: icase, mk
+      integer(c_int) :: sizes3(3)
+    end function
+    integer(c_int) function bsrg_lb_lists(lb, batch, jtype, jpol, k1, k2, k3, k4, c) bind(C, name="bsrg_lb_lists")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: lb
+      integer(c_int) :: batch(*), jtype(*), jpol(*), k1(*), k2(*), k3(*), k4(*)
+      real(c_double) :: c(*)
+    end function
   end interface
 
 contains
@@ -346,6 +481,70 @@ contains
   subroutine gpu_dirac_get(hch, hcp, hp, acf, htarg)
     real(8), intent(out) :: hch(*), hcp(*), hp(*), acf(*), htarg(*)
     call gpu_check(bsrg_dirac_get(gpu_ctx, hch, hcp, hp, acf, htarg))
+  end subroutine
+
+  ! DBSR's overlap pass (sub1.f90:77-79, O_data.f90:2-90) into S
+  subroutine gpu_dirac_overlap(nch, npert, ntarg, iptar, nrec, itype, k1, k2, cdata, eps_c, iitar)
+    integer, intent(in) :: nch, npert, ntarg, nrec, iitar
+    integer, intent(in) :: iptar(*), itype(*), k1(*), k2(*)
+    real(8), intent(in) :: cdata(*), eps_c
+    call gpu_check(bsrg_dirac_overlap(gpu_ctx, nch, npert, ntarg, iptar, nrec, itype, k1, k2, cdata, eps_c, iitar))
+  end subroutine
+
+  ! Zero_cond + Diag_channels (diag.f90:2-81) with the reference's defaults
+  subroutine gpu_dirac_diag(lch, kch, ipsol, nsol)
+    integer, intent(in) :: lch(*), kch(*)
+    integer, intent(out) :: ipsol(*), nsol
+    real(8), parameter :: c_au = 137.035999084d0
+    call gpu_check(bsrg_dirac_diag_channels(gpu_ctx, lch, kch, -1, -1, 1, 1, -2.d0 * c_au * c_au, 1.d10, 1.d-3, &
+                                            ipsol, nsol))
+  end subroutine
+
+  ! Record_matrix of the transformed S (which = 0, starts the file) or H
+  ! (which = 1, appends H, mk and the asymptotic coefficients)
+  subroutine gpu_dirac_record(path, which, nelc_core)
+    character(*), intent(in) :: path
+    integer, intent(in) :: which, nelc_core
+    call gpu_check(bsrg_dirac_record(gpu_ctx, trim(path) // c_null_char, which, nelc_core))
+  end subroutine
+
+  ! ---- BSR_MAT4 interrupted runs -----------------------------------------
+  subroutine gpu_matrix_record(path, interrupt, intercase, max_nbuf)
+    character(*), intent(in) :: path
+    integer, intent(in) :: interrupt, intercase, max_nbuf
+    call gpu_check(bsrg_matrix_record(gpu_ctx, trim(path) // c_null_char, interrupt, intercase, max_nbuf))
+  end subroutine
+
+  subroutine gpu_matrix_resume(path, interrupt, intercase, max_nbuf)
+    character(*), intent(in) :: path
+    integer, intent(out) :: interrupt, intercase, max_nbuf
+    integer(c_int) :: a, b, c
+    call gpu_check(bsrg_matrix_resume(gpu_ctx, trim(path) // c_null_char, a, b, c))
+    interrupt = a; intercase = b; max_nbuf = c
+  end subroutine
+
+  ! ---- list builder (State_res) -------------------------------------------
+  ! a builder over the bank at path; sizes5 = ndet, kdet, ndef, kdef, nrec
+  function gpu_lb_open(path, sizes5) result(lb)
+    character(*), intent(in) :: path
+    integer, intent(out) :: sizes5(5)
+    type(c_ptr) :: lb
+    integer(c_int) :: sz(5)
+    lb = bsrg_lb_create()
+    if (.not. c_associated(lb)) call gpu_check(-1_c_int)
+    call gpu_check(bsrg_lb_read_bank(lb, trim(path) // c_null_char, sz))
+    sizes5 = sz
+  end function
+
+  ! the record loop of State_res for icase; sizes3 = coefficients, list
+  ! records, lists; then bsrg_lb_lists(lb, ...) with arrays of sizes3(2)
+  subroutine gpu_lb_run(lb, icase, mk, sizes3)
+    type(c_ptr), intent(in) :: lb
+    integer, intent(in) :: icase, mk
+    integer, intent(out) :: sizes3(3)
+    integer(c_int) :: sz(3)
+    call gpu_check(bsrg_lb_run(lb, icase, mk, sz))
+    sizes3 = sz
   end subroutine
 
 end module bsr_gpu
