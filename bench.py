@@ -69,6 +69,7 @@ PEAK_FP64_MFMA = 78.6   # TFLOP/s, MI355X dense FP64 matrix
 PEAK_FP64_VECTOR = 78.6  # TFLOP/s, MI355X FP64 vector
 PEAK_HBM = 8000.0       # GB/s
 PEAK_INT8_MFMA = 5000.0  # TOP/s, MI355X dense int8 matrix
+PEAK_BF16_MFMA = 2500.0  # TFLOP/s, MI355X dense bf16 matrix
 # HBM bytes of one launch of the sparse exchange product (k_xasm<false>) at
 # the headline shape: two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE)
 # scaled by a 1 GiB device copy in the same process (read 308.1 MB, write
@@ -605,15 +606,17 @@ def main():
             log(f"{key}: {sms:.3f} ms/step, GEMM {sdev['gemm']:.3f} ms, max rel element error vs FP64 {err:.2e}")
             out[key] = {"value": w.n_elements / (sms * 1e-3), "ms_per_step": sms, "gemm_ms": sdev["gemm"],
                         "engine": sctx.wave_engine(), "max_rel_err_vs_fp64": err, "vs_fp64_plain": ms / sms}
-            if mode == 2 and sdev["gemm"] > 0:
-                # int8 slice products issued by the split GEMM: S(S+1)/2 per
-                # level pair over the 64-padded inner size
-                S = sctx.wave_engine() - 20
-                kp = (nt + 63) // 64 * 64
+            if sdev["gemm"] > 0:
+                # slice products issued by the split GEMM: S(S+1)/2 per level
+                # pair over the padded inner size (64 for int8, 32 for bf16)
+                S = sctx.wave_engine() - (20 if mode == 2 else 10)
+                pad = 64 if mode == 2 else 32
+                kp = (nt + pad - 1) // pad * pad
                 ops = S * (S + 1) // 2 * 2.0 * nss * ((nblk + 63) // 64 * 64) * kp
                 rate = ops / (sdev["gemm"] * 1e-3) / 1e12
-                out[key]["mfma"] = {"int8_tops": rate, "peak": PEAK_INT8_MFMA, "frac": rate / PEAK_INT8_MFMA,
-                                    "counters": "profiles/mfma_utilisation_c2.md"}
+                peak = PEAK_INT8_MFMA if mode == 2 else PEAK_BF16_MFMA
+                out[key]["mfma"] = {("int8_tops" if mode == 2 else "bf16_tflops"): rate, "peak": peak,
+                                    "frac": rate / peak, "counters": "profiles/mfma_utilisation_c2.md"}
             if e_ref is not None and key == "split":
                 de = float(np.max(np.abs(sctx.wave_eigen(10) - e_ref)))
                 out[key]["max_eig_err_vs_fp64"] = de

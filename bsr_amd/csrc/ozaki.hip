@@ -1,23 +1,28 @@
-// Split-precision (Ozaki-style) FP64 GEMM on bf16 MFMA for gfx950.
+// Split-precision (Ozaki-style) FP64 GEMM on int8 or bf16 MFMA for gfx950,
+// and the plain FP64 MFMA GEMM.
 //
 //   C(m x n) = A(m x K) * B(K x n) [+ C],  FP64 in and out, column-major.
 //
-// Every row of A and column of B is scaled by a power of two so that its
-// largest entry is below 1, then cut into S slices of 7 bits:
-//   a = sigma * sum_s x_s 2^-7(s+1),   x_s integers, |x_1| <= 128, |x_s| <= 64,
-// exact in bf16.  A slice product is an integer of magnitude <= 2^14.  The
-// products with s+r <= S-1 are kept; level L = s+r (L+1 products per K
-// index) accumulates in FP32 over 128 values of K, i.e. at most 8*128
-// products of magnitude <= 2^14: the sum stays <= 2^24 and the bf16 MFMA
-// with FP32 accumulation is exact.  Each level is then converted to FP64,
-// scaled by 2^-7(L+2), summed in FP64, and the result is multiplied by
-// sigma_row * tau_col.  The truncation error per element is of order
-// 2^-7S * max|row| * max|col| * K.
+// Every row of A and column of B is scaled by a power of two, 2^(e+1) for a
+// largest entry f 2^e (f in [1/2, 1)), which leaves every entry below 1/2,
+// then cut into S slices of 7 bits:
+//   a = sigma * sum_s x_s 2^-7s,   x_s = rint(y 2^7) of the running
+// remainder y, integers in [-64, 64]: exact in int8 and in bf16.  A slice
+// product is an integer of magnitude <= 2^12.  The products with
+// s + r <= S-1 are kept; level L = s + r accumulates its L+1 products per
+// inner index over the whole K:
+//   int8 slices, int32 MFMA accumulation: exact while S 2^12 Kp < 2^31;
+//   bf16 slices, FP32 MFMA accumulation: every partial sum is an integer of
+//     magnitude <= S 2^12 Kp, exact while that is <= 2^24.
+// Each level is converted to FP64, scaled by 2^-7(L+2) and summed in FP64,
+// and the result is multiplied by sigma_row * tau_col.  The truncation
+// error per element is of order 2^-7S * max|row| * max|col| * K.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "bsrg_internal.h"
 
@@ -29,151 +34,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBeta = 7;
-
-// power of two above max |row i| (1 for an all-zero row)
-__global__ void k_row_scale(int m, int K, const double* A, size_t lda, double* sig) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  double mx = 0.0;
-  for (int t = 0; t < K; ++t) mx = fmax(mx, fabs(A[i + lda * t]));
-  int e = 0;
-  if (mx > 0.0) (void)frexp(mx, &e);
-  sig[i] = ldexp(1.0, e);
-}
-
-__global__ void k_col_scale(int n, int K, const double* B, size_t ldb, double* tau) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  double mx = 0.0;
-  for (int t = 0; t < K; ++t) mx = fmax(mx, fabs(B[t + ldb * j]));
-  int e = 0;
-  if (mx > 0.0) (void)frexp(mx, &e);
-  tau[j] = ldexp(1.0, e);
-}
-
-__device__ __forceinline__ void cut(double x, int S, __bf16* out, size_t stride) {
-  for (int s = 0; s < S; ++s) {
-    const double y = x * 128.0;
-    const double d = rint(y);
-    x = y - d;
-    out[s * stride] = static_cast<__bf16>(static_cast<float>(d));
-  }
-}
-
-// A slices, layout As[s][row][t] (t fastest, Kp padded, rows padded to mp):
-// 64 rows x 32 t per workgroup, transposed through LDS.
-__global__ void k_slice_rows(int m, int K, int mp, int Kp, int S, const double* A, size_t lda, const double* sig,
-                             __bf16* As) {
-  __shared__ double tile[32][65];
-  const int r0 = blockIdx.x * 64, t0 = blockIdx.y * 32;
-  for (int q = threadIdx.x; q < 64 * 32; q += blockDim.x) {
-    const int r = q % 64, t = q / 64;
-    const int i = r0 + r, tt = t0 + t;
-    tile[t][r] = (i < m && tt < K) ? A[i + lda * tt] / sig[i] : 0.0;
-  }
-  __syncthreads();
-  const size_t stride = static_cast<size_t>(mp) * Kp;
-  for (int q = threadIdx.x; q < 64 * 32; q += blockDim.x) {
-    const int t = q % 32, r = q / 32;
-    cut(tile[t][r], S, As + static_cast<size_t>(r0 + r) * Kp + t0 + t, stride);
-  }
-}
-
-// B slices, layout Bs[r][col][t] (t fastest).
-__global__ void k_slice_cols(int n, int K, int np, int Kp, int S, const double* B, size_t ldb, const double* tau,
-                             __bf16* Bs) {
-  const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q >= static_cast<size_t>(np) * Kp) return;
-  const int t = static_cast<int>(q % Kp), j = static_cast<int>(q / Kp);
-  const double x = (j < n && t < K) ? B[t + ldb * j] / tau[j] : 0.0;
-  cut(x, S, Bs + q, static_cast<size_t>(np) * Kp);
-}
-
-// 64 x 64 output tile per workgroup, 32 x 32 per wave (2 x 2 MFMA 16x16x32).
-template <int S>
-__global__ __launch_bounds__(256) void k_ozaki(int m, int n, int mp, int np, int Kp, const __bf16* __restrict__ As,
-                                                const __bf16* __restrict__ Bs, const double* __restrict__ sig,
-                                                const double* __restrict__ tau, double* __restrict__ C, size_t ldc,
-                                                int accumulate) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int row0 = blockIdx.x * 64 + (wave & 1) * 32, col0 = blockIdx.y * 64 + (wave >> 1) * 32;
-  const int lr = lane & 15, lk = (lane >> 4) * 8;
-  f32x4 acc[S][2][2];
-  double acc64[2][2][4];
-#pragma unroll
-  for (int L = 0; L < S; ++L)
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) acc[L][a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) acc64[a][b][e] = 0.0;
-  const size_t sa = static_cast<size_t>(mp) * Kp, sb = static_cast<size_t>(np) * Kp;
-  const __bf16* pa = As + static_cast<size_t>(row0 + lr) * Kp + lk;
-  const __bf16* pb = Bs + static_cast<size_t>(col0 + lr) * Kp + lk;
-  for (int kc = 0; kc < Kp; kc += 32) {
-    bf16x8 a[S][2], b[S][2];
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        a[s][t] = *reinterpret_cast<const bf16x8*>(pa + s * sa + static_cast<size_t>(t * 16) * Kp + kc);
-        b[s][t] = *reinterpret_cast<const bf16x8*>(pb + s * sb + static_cast<size_t>(t * 16) * Kp + kc);
-      }
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-#pragma unroll
-      for (int r = 0; r < S - s; ++r)
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-          for (int tj = 0; tj < 2; ++tj)
-            acc[s + r][ti][tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[s][ti], b[r][tj], acc[s + r][ti][tj], 0, 0, 0);
-    if (((kc >> 5) & 3) == 3 || kc + 32 >= Kp) {
-#pragma unroll
-      for (int L = 0; L < S; ++L) {
-        const double f = ldexp(1.0, -kBeta * (L + 2));
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-          for (int tj = 0; tj < 2; ++tj) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) acc64[ti][tj][e] += static_cast<double>(acc[L][ti][tj][e]) * f;
-            acc[L][ti][tj] = f32x4{0.f, 0.f, 0.f, 0.f};
-          }
-      }
-    }
-  }
-  // D[i][j] sits in lane j + 16*(i/4), register i%4 of each 16x16 tile;
-  // all loads of the old values before any store
-#pragma unroll
-  for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < 2; ++tj) {
-      const int col = min(col0 + tj * 16 + lr, n - 1);
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int row = min(row0 + ti * 16 + (lane >> 4) * 4 + e, m - 1);
-        acc64[ti][tj][e] *= sig[row] * tau[col];
-        if (accumulate) acc64[ti][tj][e] += C[row + ldc * col];
-      }
-    }
-#pragma unroll
-  for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-    for (int tj = 0; tj < 2; ++tj) {
-      const int col = col0 + tj * 16 + lr;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int row = row0 + ti * 16 + (lane >> 4) * 4 + e;
-        if (row < m && col < n) C[row + ldc * col] = acc64[ti][tj][e];
-      }
-    }
-}
 
 // Plain FP64 GEMM on FP64 MFMA (16x16x4): workgroup tile (TM*32) x 64,
 // 4 waves of (TM*16) x 32; A column-major (rows fastest), B column-major
@@ -237,12 +97,7 @@ __global__ __launch_bounds__(256) void k_dgemm(int m, int n, int K, const double
     }
 }
 
-// ---- int8 slices on int8 MFMA, exact int32 accumulation -------------------
-// Scaling by 2^(e+1) (max|row| = f 2^e, f in [1/2,1)) leaves |x| < 1/2, so
-// every slice x_s = rint(y 2^7) of the running remainder is an integer in
-// [-64, 64]: int8.  Slice products are <= 2^12; a level (at most S products
-// per K index) summed over Kp <= 2^13 indices stays below 2^31, so the
-// int32 MFMA accumulation of each level is exact over the whole K.
+// ---- slices on int8 or bf16 MFMA ---------------------------------------------
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 
 __global__ void k_row_scale8(int m, int K, const double* A, size_t lda, double* sig) {
@@ -269,58 +124,71 @@ __global__ void k_col_scale8(int n, int K, const double* B, size_t ldb, double* 
   }
 }
 
-__device__ __forceinline__ void cut8(double x, int S, int8_t* out, size_t stride) {
+template <typename T>
+__device__ __forceinline__ void cut8(double x, int S, T* out, size_t stride) {
   for (int s = 0; s < S; ++s) {
     const double y = x * 128.0;
     const double d = rint(y);
     x = y - d;
-    out[s * stride] = static_cast<int8_t>(static_cast<int>(d));
+    out[s * stride] = static_cast<T>(static_cast<float>(d));
   }
 }
 
-// Slices are stored chunk-major: X[s][c][row][64] for K chunk c = t/64, so
-// the 64 rows x 64 bytes of one chunk that a workgroup stages are one
-// contiguous 4 KB block.
+// K elements in a 64-byte chunk row: 64 int8 or 32 bf16
+template <typename T>
+constexpr int chunk_k() { return 64 / static_cast<int>(sizeof(T)); }
+
+// Slices are stored chunk-major: X[s][c][row][CK] for K chunk c = t/CK
+// (CK = 64 bytes of elements), so the 64 rows x 64 bytes of one chunk that a
+// workgroup stages are one contiguous 4 KB block.
+template <typename T>
 __device__ __forceinline__ size_t at8(int c, int row, int t, int rows_p) {
-  return (static_cast<size_t>(c) * rows_p + row) * 64 + t;
+  return (static_cast<size_t>(c) * rows_p + row) * chunk_k<T>() + t;
 }
 
-// A slices (rows padded to mp, K to Kp): 64 rows x 64 t per workgroup,
+// A slices (rows padded to mp, K to Kp): 64 rows x CK t per workgroup,
 // transposed through LDS.
+template <typename T>
 __global__ void k_slice8_rows(int m, int K, int mp, int Kp, int S, const double* A, size_t lda, const double* sig,
-                              int8_t* As) {
-  __shared__ double tile[64][65];
-  const int r0 = blockIdx.x * 64, t0 = blockIdx.y * 64;
-  for (int q = threadIdx.x; q < 64 * 64; q += blockDim.x) {
+                              T* As) {
+  constexpr int CK = chunk_k<T>();
+  __shared__ double tile[CK][65];
+  const int r0 = blockIdx.x * 64, t0 = blockIdx.y * CK;
+  for (int q = threadIdx.x; q < 64 * CK; q += blockDim.x) {
     const int r = q % 64, t = q / 64;
     const int i = r0 + r, tt = t0 + t;
     tile[t][r] = (i < m && tt < K) ? A[i + lda * tt] / sig[i] : 0.0;
   }
   __syncthreads();
   const size_t stride = static_cast<size_t>(mp) * Kp;
-  for (int q = threadIdx.x; q < 64 * 64; q += blockDim.x) {
-    const int t = q % 64, r = q / 64;
-    cut8(tile[t][r], S, As + at8(blockIdx.y, r0 + r, t, mp), stride);
+  for (int q = threadIdx.x; q < 64 * CK; q += blockDim.x) {
+    const int t = q % CK, r = q / CK;
+    cut8(tile[t][r], S, As + at8<T>(blockIdx.y, r0 + r, t, mp), stride);
   }
 }
 
 // B slices (columns padded to np, K to Kp), same chunk-major layout
+template <typename T>
 __global__ void k_slice8_cols(int n, int K, int np, int Kp, int S, const double* B, size_t ldb, const double* tau,
-                              int8_t* Bs) {
+                              T* Bs) {
+  constexpr int CK = chunk_k<T>();
   const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q >= static_cast<size_t>(np) * Kp) return;
   const int t = static_cast<int>(q % Kp), j = static_cast<int>(q / Kp);
   const double x = (j < n && t < K) ? B[t + ldb * j] / tau[j] : 0.0;
-  cut8(x, S, Bs + at8(t / 64, j, t % 64, np), static_cast<size_t>(np) * Kp);
+  cut8(x, S, Bs + at8<T>(t / CK, j, t % CK, np), static_cast<size_t>(np) * Kp);
 }
 
 // C(m x n) (+)= A B computed as its transpose: MFMA rows are the n columns
 // of C (B slices), MFMA columns the m rows of C (A slices), so a wave's
 // output lanes run along the rows of C and the FP64 stores are coalesced.
 // Workgroup tile 64 (C columns) x 64 (C rows), 4 waves of 32 x 32
-// (2 x 2 MFMA 16x16x64 i8), K in chunks of 64 staged in LDS (swizzled
-// 64-byte rows, sw8).  Every level L = s + r <= S-1 has
-// its own int32 accumulators; the epilogue recombines them in FP64.  Two
+// (2 x 2 MFMA 16x16x64 i8 or 16x16x32 bf16), K in chunks of 64 bytes
+// staged in LDS (swizzled 64-byte rows, sw8); the fragments of A and B are
+// the same 16 bytes of a chunk row per lane, so the two operands' K
+// elements correspond whatever order the instruction takes them in.  Every
+// level L = s + r <= S-1 has its own int32 or FP32 accumulators; the
+// epilogue recombines them in FP64.  Two
 // waves per SIMD are asked for up to S = 7 (the accumulators then live in
 // VGPRs; at S = 7 the default allocation would leave one wave per SIMD);
 // S = 8 keeps one (its LDS tiles allow one workgroup per CU anyway).
@@ -336,11 +204,14 @@ __device__ __forceinline__ size_t out_at(int row, int col, size_t ldc, int qns, 
   const int y = row / qns, x = row - y * qns;
   return static_cast<size_t>(col) * ldo * ldo + x + static_cast<size_t>(ldo) * y;
 }
-template <int S>
-__global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, int mp, int np, int Kp, const int8_t* __restrict__ As,
-                                                 const int8_t* __restrict__ Bs, const double* __restrict__ sig,
+template <typename T, int S>
+__global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, int mp, int np, int Kp, const T* __restrict__ As,
+                                                 const T* __restrict__ Bs, const double* __restrict__ sig,
                                                  const double* __restrict__ tau, double* __restrict__ C, size_t ldc,
                                                  int accumulate, int sc, int qns, int ldo) {
+  constexpr bool kI8 = sizeof(T) == 1;
+  constexpr int CK = chunk_k<T>();
+  using Acc = typename std::conditional<kI8, i32x4, f32x4>::type;
   __shared__ __attribute__((aligned(16))) int8_t sB[S][64 * 64];  // C columns (blocks)
   __shared__ __attribute__((aligned(16))) int8_t sA[S][64 * 64];  // C rows (positions)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -382,25 +253,26 @@ __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, in
           old[a][b][e] = C[out_at(row, col, ldc, qns, ldo)];
         }
   }
-  i32x4 acc[S][2][2];
+  Acc acc[S][2][2];
 #pragma unroll
   for (int L = 0; L < S; ++L)
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) acc[L][a][b] = i32x4{0, 0, 0, 0};
+      for (int b = 0; b < 2; ++b) acc[L][a][b] = Acc{0, 0, 0, 0};
   const size_t sa = static_cast<size_t>(mp) * Kp, sb = static_cast<size_t>(np) * Kp;
   const int pr = threadIdx.x >> 2, pk = (threadIdx.x & 3) * 16;  // staging: row, 16-byte piece
   // software pipeline: the next chunk's global loads are in flight while
   // the current chunk is multiplied out of LDS; the 64 rows of a chunk are
   // one contiguous 4 KB block per slice
+  constexpr int kPiece = 16 / static_cast<int>(sizeof(T));  // elements in a thread's 16 bytes
   i32x4 gb[S], ga[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    gb[s] = *reinterpret_cast<const i32x4*>(Bs + s * sb + at8(0, c0, 0, np) + threadIdx.x * 16);
-    ga[s] = *reinterpret_cast<const i32x4*>(As + s * sa + at8(0, r0, 0, mp) + threadIdx.x * 16);
+    gb[s] = *reinterpret_cast<const i32x4*>(Bs + s * sb + at8<T>(0, c0, 0, np) + threadIdx.x * kPiece);
+    ga[s] = *reinterpret_cast<const i32x4*>(As + s * sa + at8<T>(0, r0, 0, mp) + threadIdx.x * kPiece);
   }
-  for (int kc = 0; kc < Kp; kc += 64) {
+  for (int kc = 0; kc < Kp; kc += CK) {
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -408,12 +280,12 @@ __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, in
       *reinterpret_cast<i32x4*>(&sA[s][sw8(pr, pk)]) = ga[s];
     }
     __syncthreads();
-    if (kc + 64 < Kp) {
-      const int c = kc / 64 + 1;
+    if (kc + CK < Kp) {
+      const int c = kc / CK + 1;
 #pragma unroll
       for (int s = 0; s < S; ++s) {
-        gb[s] = *reinterpret_cast<const i32x4*>(Bs + s * sb + at8(c, c0, 0, np) + threadIdx.x * 16);
-        ga[s] = *reinterpret_cast<const i32x4*>(As + s * sa + at8(c, r0, 0, mp) + threadIdx.x * 16);
+        gb[s] = *reinterpret_cast<const i32x4*>(Bs + s * sb + at8<T>(c, c0, 0, np) + threadIdx.x * kPiece);
+        ga[s] = *reinterpret_cast<const i32x4*>(As + s * sa + at8<T>(c, r0, 0, mp) + threadIdx.x * kPiece);
       }
     }
     i32x4 av[S][2];
@@ -432,7 +304,12 @@ __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, in
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < 2; ++b)
-            acc[s + r][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bv[a], av[s][b], acc[s + r][a][b], 0, 0, 0);
+            if constexpr (kI8)
+              acc[s + r][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(bv[a], av[s][b], acc[s + r][a][b], 0, 0, 0);
+            else
+              acc[s + r][a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, bv[a]),
+                                                                        __builtin_bit_cast(bf16x8, av[s][b]),
+                                                                        acc[s + r][a][b], 0, 0, 0);
     }
   }
   // tile (a, b): MFMA row i = C column c0+wc+16a+i, MFMA column j = C row
@@ -487,19 +364,28 @@ __global__ __launch_bounds__(256, S <= 7 ? 2 : 1) void k_ozaki8(int m, int n, in
     }
 }
 
-template <int S>
-void launch_k8(dim3 g, hipStream_t s, int m, int n, int mp, int np, int Kp, const int8_t* As, const int8_t* Bs,
+template <typename T, int S>
+void launch_k8(dim3 g, hipStream_t s, int m, int n, int mp, int np, int Kp, const T* As, const T* Bs,
                const double* sig, const double* tau, double* C, size_t ldc, int acc, int qns, int ldo) {
   // column supertile: the B slices of sc column tiles in about 1.5 MB
-  const size_t tile = static_cast<size_t>(S) * 64 * Kp;
+  const size_t tile = static_cast<size_t>(S) * 64 * Kp * sizeof(T);
   const int sc = std::max(1, std::min(np / 64, static_cast<int>((3u << 19) / tile)));
-  hipLaunchKernelGGL(k_ozaki8<S>, g, dim3(256), 0, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, sc, qns, ldo);
+  hipLaunchKernelGGL((k_ozaki8<T, S>), g, dim3(256), 0, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, sc, qns,
+                     ldo);
 }
 
-template <int S>
-void launch_k(dim3 g, hipStream_t s, int m, int n, int mp, int np, int Kp, const __bf16* As, const __bf16* Bs,
-              const double* sig, const double* tau, double* C, size_t ldc, int acc) {
-  hipLaunchKernelGGL(k_ozaki<S>, g, dim3(256), 0, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc);
+template <typename T>
+void launch_slices(dim3 g, hipStream_t s, int S, int m, int n, int mp, int np, int Kp, const T* As, const T* Bs,
+                   const double* sig, const double* tau, double* C, size_t ldc, int acc, int qns, int ldo) {
+  switch (S) {
+    case 2: launch_k8<T, 2>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    case 3: launch_k8<T, 3>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    case 4: launch_k8<T, 4>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    case 5: launch_k8<T, 5>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    case 6: launch_k8<T, 6>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    case 7: launch_k8<T, 7>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+    default: launch_k8<T, 8>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
+  }
 }
 
 }  // namespace
@@ -521,39 +407,43 @@ int mfma_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B
   return hipGetLastError() == hipSuccess ? 0 : fail(-100, "FP64 MFMA GEMM launch failed");
 }
 
+// bf16 slices: A and B sliced per call into work (A slices, B slices, then
+// sig and tau, each 256-byte aligned)
 size_t ozaki_workspace_bytes(int m, int n, int K, int S) {
   const size_t mp = (static_cast<size_t>(m) + 63) / 64 * 64, np = (static_cast<size_t>(n) + 63) / 64 * 64;
   const size_t Kp = (static_cast<size_t>(K) + 31) / 32 * 32;
-  return S * (mp + np) * Kp * sizeof(__bf16) + (mp + np) * sizeof(double) + 256;
+  const auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  return al(S * mp * Kp * sizeof(__bf16)) + al(S * np * Kp * sizeof(__bf16)) + al(mp * sizeof(double)) +
+         np * sizeof(double);
 }
 
 int ozaki_dgemm(int m, int n, int K, const double* A, size_t lda, const double* B, size_t ldb, double* C, size_t ldc,
                 bool accumulate, int S, void* work, hipStream_t s) {
   if (S < 2 || S > 8) return fail(-9, "split precision: slices must be 2..8");
   if (m <= 0 || n <= 0 || K <= 0) return 0;
-  const int mp = (m + 63) / 64 * 64, np = (n + 63) / 64 * 64, Kp = (K + 31) / 32 * 32;
-  __bf16* As = static_cast<__bf16*>(work);
-  __bf16* Bs = As + static_cast<size_t>(S) * mp * Kp;
-  double* sig = reinterpret_cast<double*>(
-      (reinterpret_cast<uintptr_t>(Bs + static_cast<size_t>(S) * np * Kp) + 15) & ~static_cast<uintptr_t>(15));
-  double* tau = sig + mp;
-  hipLaunchKernelGGL(k_row_scale, dim3((m + 255) / 256), dim3(256), 0, s, m, K, A, lda, sig);
-  hipLaunchKernelGGL(k_col_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, K, B, ldb, tau);
-  hipLaunchKernelGGL(k_slice_rows, dim3(mp / 64, Kp / 32), dim3(256), 0, s, m, K, mp, Kp, S, A, lda, sig, As);
+  constexpr int CK = chunk_k<__bf16>();
+  const int mp = (m + 63) / 64 * 64, np = (n + 63) / 64 * 64, Kp = (K + CK - 1) / CK * CK;
+  // every partial sum of a level is an integer of magnitude <= S 2^12 Kp:
+  // exact in FP32 while that stays <= 2^24
+  if (static_cast<long long>(S) * 4096 * Kp > (1LL << 24))
+    return fail(-9, "split precision: inner size too large for exact FP32 levels (bf16 slices)");
+  const auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  char* w = static_cast<char*>(work);
+  __bf16* As = reinterpret_cast<__bf16*>(w);
+  w += al(static_cast<size_t>(S) * mp * Kp * sizeof(__bf16));
+  __bf16* Bs = reinterpret_cast<__bf16*>(w);
+  w += al(static_cast<size_t>(S) * np * Kp * sizeof(__bf16));
+  double* sig = reinterpret_cast<double*>(w);
+  w += al(static_cast<size_t>(mp) * sizeof(double));
+  double* tau = reinterpret_cast<double*>(w);
+  hipLaunchKernelGGL(k_row_scale8, dim3((m + 255) / 256), dim3(256), 0, s, m, K, A, lda, sig);
+  hipLaunchKernelGGL(k_slice8_rows<__bf16>, dim3(mp / 64, Kp / CK), dim3(256), 0, s, m, K, mp, Kp, S, A, lda, sig, As);
+  hipLaunchKernelGGL(k_col_scale8, dim3((n + 3) / 4), dim3(256), 0, s, n, K, B, ldb, tau);
   const size_t nb = static_cast<size_t>(np) * Kp;
-  hipLaunchKernelGGL(k_slice_cols, dim3(static_cast<unsigned>((nb + 255) / 256)), dim3(256), 0, s, n, K, np, Kp, S,
-                     B, ldb, tau, Bs);
-  const dim3 g(mp / 64, np / 64);
-  const int acc = accumulate ? 1 : 0;
-  switch (S) {
-    case 2: launch_k<2>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    case 3: launch_k<3>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    case 4: launch_k<4>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    case 5: launch_k<5>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    case 6: launch_k<6>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    case 7: launch_k<7>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-    default: launch_k<8>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc); break;
-  }
+  hipLaunchKernelGGL(k_slice8_cols<__bf16>, dim3(static_cast<unsigned>((nb + 255) / 256)), dim3(256), 0, s, n, K, np,
+                     Kp, S, B, ldb, tau, Bs);
+  launch_slices<__bf16>(dim3((np / 64) * (mp / 64)), s, S, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc,
+                        accumulate ? 1 : 0, 0, 0);
   return hipGetLastError() == hipSuccess ? 0 : fail(-100, "split-precision GEMM launch failed");
 }
 
@@ -573,7 +463,8 @@ int ozaki8_prepare_a(int m, int K, const double* A, size_t lda, int S, void* abu
   int8_t* As = static_cast<int8_t*>(abuf);
   double* sig = reinterpret_cast<double*>(As + (static_cast<size_t>(S) * mp * Kp + 255) / 256 * 256);
   hipLaunchKernelGGL(k_row_scale8, dim3((m + 255) / 256), dim3(256), 0, s, m, K, A, lda, sig);
-  hipLaunchKernelGGL(k_slice8_rows, dim3(mp / 64, Kp / 64), dim3(256), 0, s, m, K, mp, Kp, S, A, lda, sig, As);
+  hipLaunchKernelGGL(k_slice8_rows<int8_t>, dim3(mp / 64, Kp / 64), dim3(256), 0, s, m, K, mp, Kp, S, A, lda, sig,
+                     As);
   return hipGetLastError() == hipSuccess ? 0 : fail(-100, "split-precision slicing launch failed");
 }
 
@@ -593,19 +484,10 @@ int ozaki8_dgemm(int m, int n, int K, const void* abuf, const double* B, size_t 
   double* tau = reinterpret_cast<double*>(Bs + (static_cast<size_t>(S) * np * Kp + 255) / 256 * 256);
   hipLaunchKernelGGL(k_col_scale8, dim3((n + 3) / 4), dim3(256), 0, s, n, K, B, ldb, tau);
   const size_t nb = static_cast<size_t>(np) * Kp;
-  hipLaunchKernelGGL(k_slice8_cols, dim3(static_cast<unsigned>((nb + 255) / 256)), dim3(256), 0, s, n, K, np, Kp, S,
-                     B, ldb, tau, Bs);
-  const dim3 g((np / 64) * (mp / 64));
-  const int acc = accumulate ? 1 : 0;
-  switch (S) {
-    case 2: launch_k8<2>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
-    case 3: launch_k8<3>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
-    case 4: launch_k8<4>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
-    case 5: launch_k8<5>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
-    case 6: launch_k8<6>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
-    case 7: launch_k8<7>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
-    default: launch_k8<8>(g, s, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc, acc, qns, ldo); break;
-  }
+  hipLaunchKernelGGL(k_slice8_cols<int8_t>, dim3(static_cast<unsigned>((nb + 255) / 256)), dim3(256), 0, s, n, K, np,
+                     Kp, S, B, ldb, tau, Bs);
+  launch_slices<int8_t>(dim3((np / 64) * (mp / 64)), s, S, m, n, mp, np, Kp, As, Bs, sig, tau, C, ldc,
+                        accumulate ? 1 : 0, qns, ldo);
   return hipGetLastError() == hipSuccess ? 0 : fail(-100, "split-precision GEMM launch failed");
 }
 
