@@ -276,9 +276,9 @@ def run_steps(ctx, w, warmup, steps):
     return ms, mean_dict(dev), mean_dict(host)
 
 
-def serving(stream, precision, nslots=6):
-    """One pass of the stream at its arrival times, then the same jobs all
-    queued at once (capacity)."""
+def serving(stream, precision, window_s, nslots=6):
+    """One pass of the stream at its arrival times (window_s: the arrival
+    window), then the same jobs all queued at once (capacity)."""
     from bsr_amd.serve import Server, summarize
     srv = Server(nslots=nslots, precision=precision)
     try:
@@ -290,7 +290,7 @@ def serving(stream, precision, nslots=6):
         burst = summarize(srv.run_stream([(0.0, w) for _, w in stream]))
     finally:
         srv.close()
-    return dict(rate_jobs_per_s=round(len(stream) / stream[-1][0], 2), p50_ms=paced["p50_ms"],
+    return dict(rate_jobs_per_s=round(len(stream) / window_s, 2), p50_ms=paced["p50_ms"],
                 p99_ms=paced["p99_ms"], elements_per_s_at_rate=paced["elements_per_s"],
                 capacity_elements_per_s=burst["elements_per_s"], capacity_p50_ms=burst["p50_ms"],
                 capacity_p99_ms=burst["p99_ms"], service_ms_mean=paced["service_ms_mean"],
@@ -298,14 +298,19 @@ def serving(stream, precision, nslots=6):
 
 
 def split_errors(stream):
-    """Each job of the stream in FP64 and in split precision: the largest
-    relative element error of S and H (floor 1e-3 of the largest element)."""
+    """Each distinct wave of the stream in FP64 and in split precision: the
+    largest relative element error of S and H (floor 1e-3 of the largest
+    element)."""
     from bsr_amd import Context
     ref, spl = Context(0), Context(0)
     spl.set_precision(2, 0)
     errs = []
+    seen = set()
     try:
         for _, w in stream:
+            if id(w) in seen:
+                continue
+            seen.add(id(w))
             ref.wave_assemble(w)
             spl.wave_assemble(w)
             errs.append(max(spl.wave_diff(ref, 0), spl.wave_diff(ref, 1)))
@@ -493,7 +498,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue check of the split forms")
-    ap.add_argument("--serve-jobs", type=int, default=60)
+    ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--batch-waves", type=int, default=48)
     ap.add_argument("--no-dirac", action="store_true")
     ap.add_argument("--dirac-only", action="store_true", help="only the Dirac leg (printed as its own JSON line)")
@@ -657,10 +662,12 @@ def main():
     ctx.close()
     if not a.no_serve:
         log("generating served stream")
-        stream = synth.c4_stream(njobs=a.serve_jobs)
-        sv = {"jobs": len(stream)}
+        rate = 20.0
+        stream = synth.c4_stream(njobs=a.serve_jobs, rate=rate, ndistinct=60)
+        sv = {"jobs": len(stream), "distinct_waves": len({id(w) for _, w in stream}),
+              "arrivals": f"Poisson at {rate:g} jobs/s conditioned on {len(stream)} arrivals in {len(stream) / rate:g} s"}
         for name, prec in (("fp64", 0), ("split", 2)):
-            sv[name] = serving(stream, prec)
+            sv[name] = serving(stream, prec, len(stream) / rate)
             log(f"serving {name}: " + json.dumps(sv[name]))
         errs = split_errors(stream)
         sv["split"]["max_rel_err_vs_fp64"] = max(errs)

@@ -347,18 +347,23 @@ def c3_wave(spec, ncp: int = 0) -> PartialWave:
     return w
 
 
-def c4_stream(seed: int = 4, njobs: int = 60, rate: float = 20.0):
-    """Served stream: Poisson arrivals (mean `rate` jobs/s) of partial waves
-    of one target drawn around the production wave -- same grid (ks=8,
-    ns=200, Z=18, Rmax=40) and bound orbitals, k=0..10, 5% fill, nch ~
-    U[40,80], ncp ~ U[200,400].  Returns [(arrival_s, PartialWave)]."""
+def c4_stream(seed: int = 4, njobs: int = 60, rate: float = 20.0, ndistinct: int | None = None):
+    """Served stream: Poisson arrivals at `rate` jobs/s of partial waves of
+    one target drawn around the production wave -- same grid (ks=8, ns=200,
+    Z=18, Rmax=40) and bound orbitals, k=0..10, 5% fill, nch ~ U[40,80],
+    ncp ~ U[200,400].  The arrivals are a Poisson process conditioned on
+    njobs arrivals in the window [0, njobs/rate): njobs sorted uniform
+    times, so the realised rate is the specified one.  At most `ndistinct`
+    waves are generated and the stream cycles through them (every job is
+    assembled anew).  Returns [(arrival_s, PartialWave)]."""
     rng = np.random.default_rng(seed)
-    t = np.cumsum(rng.exponential(1.0 / rate, njobs))
-    out = []
-    for j in range(njobs):
+    t = np.sort(rng.uniform(0.0, njobs / rate, njobs))
+    nd = njobs if ndistinct is None else max(1, min(njobs, ndistinct))
+    waves = []
+    for j in range(nd):
         w = make_wave(seed * 100000 + j, 8, 200, 18.0, 40.0, nch=int(rng.integers(40, 81)),
                       ncp=int(rng.integers(200, 401)), mk=10, nb=5, ntarg=5, fill=0.05,
                       name=f"c4-{j}", target_seed=seed)
         w.concat()  # a request arrives with its lists packed
-        out.append((float(t[j]), w))
-    return out
+        waves.append(w)
+    return [(float(t[j]), waves[j % nd]) for j in range(njobs)]
