@@ -106,9 +106,9 @@ void launch_term_prep(int ns, int ks, int nblk, const int* bptr, const int* bid,
 // columns blk, leading dimension ldc) into per-block nonzero lists (slots of
 // xasm_slot(nr) per block: xo = LDS byte offset of the basis, xc = value;
 // xn = count); launch_xasm then writes every block once with the exchange
-// product from the bases E (columns of ns^2, the nr used ones) and the band
-// terms, and adds the rank-1 terms with the term pass (nobands: nblk+1
-// zeros).  The term arena is a term set's, as for launch_assemble.
+// product from the bases E (columns of ns^2, the nr used ones) and its band
+// terms (TermEpi), and adds the rank-1 terms with the term pass (nobands:
+// nblk+1 zeros).  The term arena is a term set's, as for launch_assemble.
 // xasm_max_t(device): the largest nr whose LDS tile fits (64 nr doubles).
 int xasm_max_t(int device);
 int xasm_slot(int nr);
@@ -133,6 +133,17 @@ int launch_xasm(int ns, int ks, int nblk, int nr, const double* E, const int* xo
                 const int* ru, const int* rw, const int* rs, const double* rc, const double* stab, const double* V,
                 double* hcc, double* bsum, double* rf, const unsigned char* live, const int* nobands,
                 hipStream_t s);
+// The band terms of an overwriting exchange product, added in the product
+// kernel's epilogue (term_epi.h): per element, the product, then its band
+// term -- the first step of k_assemble after a stored product, which then
+// adds the rank-1 terms; a pass with the bands off (bptr all zeros) adds the
+// rank-1 terms after such a product, so both routes give the same bits.
+// bsum comes from k_term_prep (launch_term_prep).
+struct TermEpi {
+  int ns, ks;
+  const int* bptr;      // band-term CSR over the blocks (only emptiness is read)
+  const double* bsum;   // summed band of each block with band terms, ns*ks per block
+};
 // hcb[q] = base + sum_r c_r * sfac_r * V[v_r]; CSR over all nq positions
 void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,
                          const double* c, const double* stab, const double* V, double* hcb, hipStream_t s);
@@ -165,6 +176,7 @@ void launch_absmax(size_t n, const double* x, double* partial, hipStream_t s);
 void launch_reldiff(size_t n, const double* a, const double* b, double floor, double* partial, hipStream_t s);
 void launch_scale(size_t n, double a, double* x, hipStream_t s);
 void launch_fill(size_t n, double v, double* x, hipStream_t s);
+void launch_fill_int(size_t n, int v, int* x, hipStream_t s);
 // leaf convolutions of one band with a general density
 void launch_convol_direct(int ns, int ks, int jcase, const double* rkb, const double* d, double* a,
                           hipStream_t s);

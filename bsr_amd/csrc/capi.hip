@@ -452,6 +452,16 @@ int upload_wvec(bsrg_ctx* c) {
   return 0;
 }
 
+// nblk+1 zeros: the band-term pointers of a rank-1-only term pass
+int ensure_nobands(bsrg_ctx* c, size_t nblk) {
+  if (c->xzero.n < nblk + 1) {
+    HIPCHK(c->xzero.resize(nblk + 1));
+    launch_fill_int(c->xzero.n, 0, c->xzero.p, c->stream);
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
 // Upload a staged term set (and the GET_V vectors) for a later enqueue:
 // one copy of the arena, one of the exchange coefficients.
 // The exchange coefficients of a term set (and the split-precision
@@ -522,11 +532,7 @@ int ensure_xasm(bsrg_ctx* c, const Matrices& M, const Staged& st) {
   HIPCHK(c->xt.resize(n));
   HIPCHK(c->xc.resize(n));
   HIPCHK(c->xn.resize(std::max<size_t>(nblk, 1)));
-  if (c->xzero.n < nblk + 1) {
-    HIPCHK(c->xzero.resize(nblk + 1));
-    HIPCHK(hipMemsetAsync(c->xzero.p, 0, c->xzero.n * sizeof(int), c->stream));
-  }
-  return 0;
+  return ensure_nobands(c, nblk);
 }
 
 // The exchange-contraction GEMM hcc = E^T C (+hcc): split precision in
@@ -542,8 +548,7 @@ int run_gemm(bsrg_ctx* c, Matrices& M, const Staged& st, bool overwrite, int eng
   const double* A = c->E.p + r0 * nss;
   const double* B = st.gemm.p + r0;
   if (engine == 4 && c->precision_mode == 0) {
-    if (overwrite && c->xn.n >= nblk && c->xt.n >= nblk * static_cast<size_t>(xasm_slot(nr)) &&
-        c->xzero.n >= nblk + 1) {
+    if (overwrite && c->xn.n >= nblk && c->xt.n >= nblk * static_cast<size_t>(xasm_slot(nr))) {
       launch_coef_pack(static_cast<int>(nblk), nr, B, nt, c->xt.p, c->xc.p, c->xn.p, c->stream);
       if (launch_xprod(c->G.ns, static_cast<int>(nblk), nr, A, c->xt.p, c->xc.p, c->xn.p, M.hcc.p, c->stream))
         return fail(-100, "sparse exchange product: bad shape");

@@ -408,7 +408,7 @@ struct bsrg_ctx {
   DevVec<double> bsum, rfac;  // term-pass scratch: summed band of each block, factor of each rank-1 term
   // engine 4 (exchange fused with the term pass): nonzeros of each block's
   // exchange coefficients, slots of nr per block
-  DevVec<int> xt, xn, xzero;
+  DevVec<int> xt, xn, xzero;  // xzero: nblk+1 zeros (band pointers of a rank-1-only pass)
   DevVec<double> xc;
   DevVec<double> bsumS, rfacS;  // the same for S, whose pass may be queued while H's scratch grows
 

@@ -564,8 +564,11 @@ __global__ void k_sb_upper(int ns, int ks, const double* sb, double* sbu) {
 }
 
 // ---- accumulation of term lists into the channel blocks ------------------
-// One pass over a channel block: out = (base + rank-1 terms) + band terms,
-// each sum in list order.  base is what hcc holds (the exchange GEMM result,
+// One pass over a channel block: out = (base + band terms) + rank-1 terms,
+// the rank-1 terms in list order, the band terms summed per block first
+// (k_term_prep).  The order matches a product kernel that adds the band term
+// in its epilogue and leaves the rank-1 terms to this pass (bands off), so
+// both routes give the same bits.  base is what hcc holds (the exchange GEMM result,
 // or earlier contributions) when has_base, else 0 and every block is
 // written.  Term lists are CSR over all blocks (ptr has nblk+1 entries).
 // grid nblk * ceil(ns/kAsmCols) * ceil(ns/(64A)) (block, column tile, row
@@ -668,8 +671,9 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
       sband[q] = (i >= 0 && i + ith < ns) ? bs[i + ns * ith] : 0.0;
     }
   }
-  // the sum starts from the base values, loaded before the rank-1 work so
-  // their latency overlaps it: out = (base + rank-1 terms) + band terms
+  // the sum starts from the base values, loaded before the rank-1 vectors
+  // are staged so their latency overlaps that: out = (base + band terms) +
+  // rank-1 terms
   double acc[A][kAsmCpw];
 #pragma unroll
   for (int b = 0; b < kAsmCpw; ++b) {
@@ -680,6 +684,22 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
       acc[a][b] = hin ? hin[x + ldb * y] : 0.0;
     }
   }
+  // a wave's 64 rows of group a meet the band of its columns only when they
+  // lie within ks-1 of them (a wave-uniform test); applied once sband is
+  // complete (after a barrier), before the first rank-1 term
+  bool banded = false;
+  const auto add_band = [&]() {
+#pragma unroll
+    for (int a = 0; a < A; ++a) {
+      if (!(has_band && x0 + 64 * a + 63 > y0 - ks && x0 + 64 * a < y0 + kAsmCpw + ks - 1)) continue;
+#pragma unroll
+      for (int b = 0; b < kAsmCpw; ++b) {
+        const int x = x0 + lane + 64 * a, y = y0 + b, d = x - y;
+        if (x < ns && y < ns && d > -ks && d < ks) acc[a][b] += sband[(min(x, y) - ilo) * ks + (d < 0 ? -d : d)];
+      }
+    }
+    banded = true;
+  };
   // rank-1 terms: metadata and factors of 64 terms at a time into LDS (one
   // round of loads), then their vectors 16 at a time
   const int nrow = min(R, ns - x0);
@@ -704,6 +724,7 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
         sw[r][col] = y < ns ? tf[tb + r] * V[static_cast<size_t>(tw[tb + r]) * ns + y] : 0.0;
       }
       __syncthreads();
+      if (!banded) add_band();
       for (int r = 0; r < nt; ++r) {
         double u[A];
 #pragma unroll
@@ -717,13 +738,10 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
       }
     }
   }
-  __syncthreads();  // sband complete
-  // a wave's 64 rows of group a meet the band of its columns only when they
-  // lie within ks-1 of them (a wave-uniform test)
-  bool band_a[A];
-#pragma unroll
-  for (int a = 0; a < A; ++a)
-    band_a[a] = has_band && x0 + 64 * a + 63 > y0 - ks && x0 + 64 * a < y0 + kAsmCpw + ks - 1;
+  if (!banded) {
+    __syncthreads();  // sband complete
+    add_band();
+  }
 #pragma unroll
   for (int b = 0; b < kAsmCpw; ++b) {
     const int y = y0 + b;
@@ -732,13 +750,7 @@ __global__ __launch_bounds__(256) void k_assemble(int ns, int ks, int has_base, 
 #pragma unroll
     for (int a = 0; a < A; ++a) {
       const int x = x0 + lane + 64 * a;
-      if (x >= ns) continue;
-      double v = acc[a][b];
-      if (band_a[a]) {
-        const int d = x - y;
-        if (d > -ks && d < ks) v += sband[(min(x, y) - ilo) * ks + (d < 0 ? -d : d)];
-      }
-      hy[x] = v;
+      if (x < ns) hy[x] = acc[a][b];
     }
   }
 }
@@ -854,6 +866,11 @@ __global__ void k_scale(size_t n, double a, double* x) {
 }
 
 __global__ void k_fill(size_t n, double v, double* x) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+__global__ void k_fill_int(size_t n, int v, int* x) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) x[i] = v;
 }
@@ -1325,6 +1342,11 @@ void launch_scale(size_t n, double a, double* x, hipStream_t s) {
 void launch_fill(size_t n, double v, double* x, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(k_fill, dim3(nblocks(n, 256)), dim3(256), 0, s, n, v, x);
+}
+
+void launch_fill_int(size_t n, int v, int* x, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_int, dim3(nblocks(n, 256)), dim3(256), 0, s, n, v, x);
 }
 
 void launch_convol_direct(int ns, int ks, int jcase, const double* rkb, const double* d, double* a, hipStream_t s) {

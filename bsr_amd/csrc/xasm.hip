@@ -26,14 +26,17 @@
 //                 the rank-1 terms to the blocks that have them.
 //
 // Sums: the exchange part in ascending t (four interleaved partial sums
-// added pairwise), then the band term; the rank-1 terms in list order: a
-// fixed order, so a block's values do not depend on what else runs.
+// added pairwise), then the band term, then the rank-1 terms in list order
+// (the order of the term pass after a separate product, so both forms give
+// the same bits): a fixed order, so a block's values do not depend on what
+// else runs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <mutex>
 
 #include "bsrg_internal.h"
+#include "term_epi.h"
 
 namespace bsrg {
 
@@ -83,13 +86,13 @@ __global__ __launch_bounds__(256) void k_coef_pack(int nblk, int nr, int nrs, co
 // through scalar loads and serve as scalar operands, so the inner loop is
 // one address add, one LDS read and one FMA per nonzero (the padding adds
 // zeros).  The band term (one read of the block's summed band, for the
-// elements within ks-1 of the diagonal) is requested before the sum.
-template <bool BAND>
+// elements within ks-1 of the diagonal) is requested before the sum and
+// added after it in the fused form.
+template <bool FUSED>
 __global__ __launch_bounds__(64 * kXWaves) void k_xasm(
-    int ns, int ks, int nblk, int nr, int nrs, int ntile, int ngroup, const double* __restrict__ E,
-    const int* __restrict__ xo, const double* __restrict__ xc, const int* __restrict__ xn,
-    const int* __restrict__ bptr, const int* __restrict__ rptr, const double* __restrict__ bsum,
-    const unsigned char* __restrict__ live, double* __restrict__ hcc) {
+    int ns, int nblk, int nr, int nrs, int ntile, int ngroup, const double* __restrict__ E,
+    const int* __restrict__ xo, const double* __restrict__ xc, const int* __restrict__ xn, TermEpi te,
+    const int* __restrict__ rptr, const unsigned char* __restrict__ live, double* __restrict__ hcc) {
   extern __shared__ double El[];
   const size_t nss = static_cast<size_t>(ns) * ns;
   const int tile = static_cast<int>(blockIdx.x % ntile), grp = static_cast<int>(blockIdx.x / ntile);
@@ -118,20 +121,15 @@ __global__ __launch_bounds__(64 * kXWaves) void k_xasm(
   const size_t p = p0 + lane;
   const bool valid = p < nss;
   const int x = valid ? static_cast<int>(p % ns) : 0, y = valid ? static_cast<int>(p / ns) : 0;
-  const int d = x - y, ad = d < 0 ? -d : d;
-  const bool inband = ad < ks;
-  const size_t bi = static_cast<size_t>(min(x, y)) + static_cast<size_t>(ns) * ad;
   const char* Eb = reinterpret_cast<const char*>(El + lane);
   const int per = (nblk + ngroup - 1) / ngroup, blo = grp * per, bhi = min(nblk, blo + per);
   for (int blk = blo + wv; blk < bhi; blk += kXWaves) {
     const int n = xn[blk];
+    // a block without terms that already holds zeros stays as it is
+    if (FUSED && n == 0 && te.bptr[blk] == te.bptr[blk + 1] && rptr[blk] == rptr[blk + 1] && live && !live[blk])
+      continue;
     double band = 0.0;
-    if (BAND) {
-      const int b0 = bptr[blk], b1 = bptr[blk + 1];
-      // a block without terms that already holds zeros stays as it is
-      if (n == 0 && b0 == b1 && rptr[blk] == rptr[blk + 1] && live && !live[blk]) continue;
-      band = (b1 > b0 && inband) ? bsum[static_cast<size_t>(blk) * ns * ks + bi] : 0.0;
-    }
+    const bool has_band = FUSED && epi_band(te, blk, x, y, band);
     const int* ol = xo + static_cast<size_t>(blk) * nrs;
     const double* cl = xc + static_cast<size_t>(blk) * nrs;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
@@ -146,7 +144,8 @@ __global__ __launch_bounds__(64 * kXWaves) void k_xasm(
 #pragma unroll
       for (int i = 0; i < kXChunk; ++i) a[i & 3] += cc[i] * *reinterpret_cast<const double*>(Eb + oo[i]);
     }
-    const double acc = ((a[0] + a[1]) + (a[2] + a[3])) + band;
+    double acc = (a[0] + a[1]) + (a[2] + a[3]);
+    if (has_band) acc += band;
     if (valid) hcc[static_cast<size_t>(blk) * nss + p] = acc;
   }
 }
@@ -303,8 +302,8 @@ int launch_xprod(int ns, int nblk, int nr, const double* E, const int* xo, const
   int ntile, ngroup;
   xasm_grid(ns, nblk, ntile, ngroup);
   hipLaunchKernelGGL(k_xasm<false>, dim3(static_cast<unsigned>(ntile) * ngroup), dim3(64 * kXWaves),
-                     static_cast<size_t>(nr) * 64 * sizeof(double), s, ns, 0, nblk, nr, xasm_slot(nr), ntile, ngroup,
-                     E, xo, xc, xn, nullptr, nullptr, nullptr, nullptr, hcc);
+                     static_cast<size_t>(nr) * 64 * sizeof(double), s, ns, nblk, nr, xasm_slot(nr), ntile, ngroup,
+                     E, xo, xc, xn, TermEpi{}, nullptr, nullptr, hcc);
   return 0;
 }
 
@@ -318,9 +317,10 @@ int launch_xasm(int ns, int ks, int nblk, int nr, const double* E, const int* xo
   launch_term_prep(ns, ks, nblk, bptr, bid, bc, bands, bsum, rptr, rs, rc, stab, rf, s);
   int ntile, ngroup;
   xasm_grid(ns, nblk, ntile, ngroup);
+  const TermEpi te{ns, ks, bptr, bsum};
   hipLaunchKernelGGL(k_xasm<true>, dim3(static_cast<unsigned>(ntile) * ngroup), dim3(64 * kXWaves),
-                     static_cast<size_t>(nr) * 64 * sizeof(double), s, ns, ks, nblk, nr, xasm_slot(nr), ntile,
-                     ngroup, E, xo, xc, xn, bptr, rptr, bsum, live, hcc);
+                     static_cast<size_t>(nr) * 64 * sizeof(double), s, ns, nblk, nr, xasm_slot(nr), ntile, ngroup, E,
+                     xo, xc, xn, te, rptr, live, hcc);
   // the rank-1 terms: added by the term pass to the blocks that have them
   // (no band terms: nobands is an all-zero pointer array)
   launch_assemble(ns, ks, nblk, 1, nobands, bid, bc, bands, rptr, ru, rw, rs, rc, stab, V, hcc, bsum, rf, s, nullptr,

@@ -372,6 +372,42 @@ def test_fused_engine_is_the_default_and_deterministic(ctx):
     assert np.array_equal(H1.hcc, H2.hcc) and np.array_equal(H1.hcb, H2.hcb)
 
 
+@pytest.mark.parametrize("mode,nslices", [(0, 0), (2, 6), (2, 7), (1, 6)])
+def test_fused_term_pass_equals_separate(ctx, oracle_lib, mode, nslices):
+    """A prepared wave runs the exchange product with the band terms in its
+    epilogue and adds the rank-1 terms in a pass of their own; a one-shot
+    wave starts the product early and adds the band and rank-1 terms in one
+    separate pass.  Both add the same terms in the same order, so S and H
+    agree bit for bit, in FP64 (the sparse product) and in the split forms;
+    and the fused result matches the oracle."""
+    O = oracle_lib
+    from bsr_amd import synth
+    # ns^2 = 8100 is not a multiple of the 64-position tiles, 45 blocks not a
+    # multiple of the split kernel's 64 columns; perturbers give rank-1 terms
+    w = synth.make_wave(83, 8, 90, 18.0, 40.0, nch=9, ncp=8, mk=5, nb=5, ntarg=4, fill=0.12, name="fuse")
+    try:
+        ctx.set_precision(mode, nslices)
+        ctx.wave_assemble(w)
+        S1, H1 = ctx.wave_get(0), ctx.wave_get(1)
+        counts = ctx.wave_term_counts(1)
+        assert counts["rank1_terms"] > 0 and counts["band_terms"] > 0, counts
+        ctx.wave_prepare(w, set_inputs=False)
+        ctx.wave_launch()
+        ctx.wave_finish()
+        S2, H2 = ctx.wave_get(0), ctx.wave_get(1)
+        assert ctx.wave_engine() == (4 if mode == 0 else 10 * mode + nslices)
+    finally:
+        ctx.set_precision(0, 0)
+    for a, b in ((S1, S2), (H1, H2)):
+        assert np.array_equal(a.hcc, b.hcc) and np.array_equal(a.hcb, b.hcb)
+        assert np.array_equal(a.hbb, b.hbb) and np.array_equal(a.acf, b.acf)
+    oS, oH = O.run_wave(w)
+    e = rel_err(H2.hcc, oH.hcc)
+    print(f"[fuse] mode={mode} S={nslices} H.hcc vs oracle {e:.2e}", flush=True)
+    assert e <= (TOL if mode == 0 else 1e-10)
+    assert rel_err(S2.hcc, oS.hcc) < TOL
+
+
 def test_batch_graph(oracle_lib):
     """Uneven waves captured into one hipGraph: results match the oracle and
     two replays give identical bits."""
