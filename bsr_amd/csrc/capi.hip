@@ -859,6 +859,7 @@ int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, con
     if (i < nx) decode(pexch[i]);
   };
   c->pool->run(nz, phase1);
+  const auto tp1 = std::chrono::steady_clock::now();
   for (int t : pexch) {
     const Terms& P = *c->work[t];
     if (P.err) return fail(P.err, P.msg);
@@ -867,8 +868,10 @@ int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, con
     TH.gemm_khi = std::max(TH.gemm_khi, P.gemm_khi);
   }
   if (TH.gemm_khi < 0) TH.gemm_n = 0;
+  const auto tp2 = std::chrono::steady_clock::now();
   if (c->early_gemm)
     if (int rc = start_early_gemm(c)) return rc;
+  const auto tp3 = std::chrono::steady_clock::now();
   // phase 2: everything else; GET_V vectors not yet in the registry (new
   // orbitals) are registered in list order and the phase is repeated
   const std::function<void(int)> phase2 = [&](int i) { decode(pother[i]); };
@@ -896,6 +899,10 @@ int idata_parallel(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, con
   c->host_ms[2] = ms(t1, t2);
   c->host_ms[3] = ms(t2, t3);
   c->host_ms[4] = ms(t3, t4);
+  c->host_ms[10] = ms(t2, tp1);   // exchange records
+  c->host_ms[11] = ms(tp1, tp2);  // their coefficients summed
+  c->host_ms[12] = ms(tp2, tp3);  // early product enqueued
+  c->host_ms[13] = ms(tp3, t3);   // the other records
   c->host_ms[8] = 0.0;
   c->host_ms[9] = 0.0;
   for (double x : part_ms) { c->host_ms[8] += x; c->host_ms[9] = std::max(c->host_ms[9], x); }
@@ -1731,6 +1738,13 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   c->host_ms[5] = ms(h1, h2);
   c->host_ms[6] = ms(h2, h3);
   c->host_ms[7] = ms(h0, h3);
+  if (std::getenv("BSRG_HOST_TRACE"))
+    std::fprintf(stderr,
+                 "bsrg host ms: checks+launch %.3f overlaps %.3f decode %.3f (exchange %.3f sum %.3f early %.3f "
+                 "other %.3f; cpu %.3f longest %.3f) acf %.3f sort %.3f upload %.3f total %.3f\n",
+                 c->host_ms[0], c->host_ms[1], c->host_ms[3], c->host_ms[10], c->host_ms[11], c->host_ms[12],
+                 c->host_ms[13], c->host_ms[8], c->host_ms[9], c->host_ms[4], c->host_ms[5], c->host_ms[6],
+                 c->host_ms[7]);
   c->wave_prepared = true;
   return 0;
 }

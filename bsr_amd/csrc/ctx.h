@@ -343,7 +343,7 @@ struct bsrg_ctx {
   // decoding, [4] merge of the parts, [5] sort and staging, [6] uploads and
   // GEMM autotune, [7] total, [8] decoding time summed over the parts (cpu
   // ms), [9] the longest part
-  double host_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double host_ms[16] = {};  // 0-9 as bsrg_wave_host_timings reports them; 10-13 the decode phases
   int precision_mode = 0, nslices = 0;
 
   GridTables G;
