@@ -665,7 +665,9 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, Staged& st, bool overwrite, hipEvent
                                st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw), st.dp<int>(st.o_rs),
                                st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p,
                                (&M == &c->S ? c->bsumS : c->bsum).p, (&M == &c->S ? c->rfacS : c->rfac).p,
-                               st.has_live && !c->capturing ? st.dp<unsigned char>(st.o_live) : nullptr,
+                               st.has_live && (!c->capturing || c->live_in_capture)
+                                   ? st.dp<unsigned char>(st.o_live)
+                                   : nullptr,
                                c->xzero.p, s);
     if (xr) return fail(-100, "fused exchange pass: bad shape");
     launch_vec_assemble(ns, st.nq, 0, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid), st.dp<int>(st.o_vs),
@@ -677,7 +679,7 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, Staged& st, bool overwrite, hipEvent
   // leaves blocks without terms alone; an overwriting pass without a GEMM
   // writes every block, except empty ones that already hold zeros
   const bool add = st.has_gemm || !overwrite;
-  const bool skip = !add && st.has_live && !c->capturing;
+  const bool skip = !add && st.has_live && (!c->capturing || c->live_in_capture);
   launch_assemble(ns, ks, st.nblk, add ? 1 : 0, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid),
                   st.dp<double>(st.o_bc), c->bands.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw),
                   st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->VT.p, M.hcc.p,
@@ -711,7 +713,7 @@ void update_live(bsrg_ctx* c, Matrices& M, Staged& st) {
   if (M.live.size() < static_cast<size_t>(st.nblk)) M.live.resize(st.nblk, 1);
   const int* bp = st.hp<int>(st.o_bptr);
   const int* rp = st.hp<int>(st.o_rptr);
-  const bool unknown = st.has_gemm || c->capturing;
+  const bool unknown = st.has_gemm || (c->capturing && !c->live_in_capture);
   for (int b = 0; b < st.nblk; ++b)
     M.live[b] = unknown || bp[b + 1] > bp[b] || rp[b + 1] > rp[b] ? 1 : 0;
 }
@@ -1642,6 +1644,7 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   HIPCHK(hipSetDevice(c->device));
   c->wave_ok = false;
   c->wave_prepared = false;
+  c->eager_done = false;
   using clk = std::chrono::steady_clock;
   const auto h0 = clk::now();
   // record checks on the host workers (the first bad record in list order
@@ -1766,6 +1769,7 @@ int bsrg_wave_launch(bsrg_ctx* c) {
   launch_symmetrize_diag(c->G.ns, c->H.nch, c->H.hcc.p, c->stream);
   HIPCHK(hipGetLastError());
   if (timed) (void)hipEventRecord(c->ev[5], c->stream);
+  if (!c->capturing) c->eager_done = true;
   return 0;
 }
 
@@ -2126,6 +2130,23 @@ int bsrg_graph_capture(int n, bsrg_ctx** ctxs, bsrg_graph** out) {
   for (int i = 0; i < n; ++i) HIPCHK(hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming));
   hipStream_t s0 = ctxs[0]->stream;
   for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(ctxs[i]->stream));
+  // a wave that already ran eagerly leaves exact live maps: the blocks they
+  // mark hold zeros, and every replay writes the same values into the
+  // others, so the captured passes may skip them (their maps go into the
+  // staged term sets the captured kernels read)
+  for (int i = 0; i < n; ++i) {
+    bsrg_ctx* c = ctxs[i];
+    if (!c->eager_done) continue;
+    for (Staged* st : {&c->stS, &c->stH}) {
+      Matrices& M = st == &c->stS ? c->S : c->H;
+      if (!st->has_live || M.live.size() < static_cast<size_t>(st->nblk)) continue;
+      std::memcpy(st->hp<unsigned char>(st->o_live), M.live.data(), st->nblk);
+      HIPCHK(hipMemcpyAsync(st->d.p + st->o_live, st->hp<unsigned char>(st->o_live), st->nblk,
+                            hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->live_in_capture = true;
+  }
   HIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeGlobal));
   int rc = 0;
   for (int i = 0; i < n; ++i) ctxs[i]->capturing = true;
@@ -2138,7 +2159,7 @@ int bsrg_graph_capture(int n, bsrg_ctx** ctxs, bsrg_graph** out) {
     if (hipStreamWaitEvent(s0, g->join[i], 0) != hipSuccess) rc = fail(-100, "graph join failed");
   }
   hipError_t e = hipStreamEndCapture(s0, &g->graph);
-  for (int i = 0; i < n; ++i) ctxs[i]->capturing = false;
+  for (int i = 0; i < n; ++i) ctxs[i]->capturing = ctxs[i]->live_in_capture = false;
   if (rc) return rc;
   if (e != hipSuccess) return fail(-100, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));

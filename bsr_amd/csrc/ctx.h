@@ -382,6 +382,10 @@ struct bsrg_ctx {
 
   Matrices cur, S, H;
   bool wave_ok = false, wave_prepared = false, capture_safe = false, capturing = false;
+  // the prepared wave ran eagerly (not captured) since bsrg_wave_prepare, so
+  // the live maps say exactly which blocks hold zeros; a capture may then
+  // skip them (live_in_capture, set for the duration of a capture)
+  bool eager_done = false, live_in_capture = false;
   // one-shot waves (bsrg_wave_assemble) enqueue the H exchange GEMM during
   // the prepare, before the terms are staged
   bool early_gemm = false, gemm_enqueued = false, gemm_was_early = false;
