@@ -327,14 +327,26 @@ def batch_c3(nwaves):
     b = Batch(waves)
     try:
         b.prepare()
-        b.capture()
-        b.run()
-        reps = [b.run() for _ in range(3)]
-        tot = min(r[1] for r in reps)
+        out = {}
+        # the exchange products of all waves in one persistent launch (work
+        # items from one counter, granularity from the autotuner), and one
+        # graph branch per wave
+        for key, persistent in (("persistent", True), ("per_wave", False)):
+            t0 = time.perf_counter()
+            b.capture(persistent=persistent)
+            cap_s = time.perf_counter() - t0
+            b.run()
+            reps = [b.run() for _ in range(3)]
+            tot = min(r[1] for r in reps)
+            out[key] = dict(graph_ms=tot, elements_per_s=b.n_elements() / (tot * 1e-3), capture_s=cap_s)
+            if persistent:
+                nb, bpi = b.batch_info()
+                out[key].update(waves_in_persistent_launch=nb, blocks_per_item=bpi)
+        best = min(out.values(), key=lambda d: d["graph_ms"])
         per = sorted(b.eager_ms)
-        return dict(waves=len(waves), graph_ms=tot, elements_per_s=b.n_elements() / (tot * 1e-3),
-                    wave_kernel_ms_p50=per[len(per) // 2], wave_kernel_ms_max=per[-1],
-                    engines=sorted(set(b.engines())))
+        return dict(waves=len(waves), graph_ms=best["graph_ms"], elements_per_s=best["elements_per_s"],
+                    graphs=out, wave_kernel_ms_p50=per[len(per) // 2], wave_kernel_ms_max=per[-1],
+                    eager_kernel_ms_sum=sum(per), engines=sorted(set(b.engines())))
     finally:
         b.close()
 

@@ -3,10 +3,14 @@
 Each wave gets its own device context (stream, grid tables, R^k bands,
 bases, matrices).  `prepare` does the host list processing and leaves every
 input resident; the device work of all waves is then captured once into a
-single hipGraph (one branch per wave) and every `run` replays the whole
-batch with one graph launch.  The GEMM engine of each wave's exchange
-contraction is chosen by the autotuner among the hand-written kernels (the
-capturable ones) for that wave's shape.
+single hipGraph and every `run` replays the whole batch with one graph
+launch.  With `persistent` (the default) the exchange products of all waves
+run in one persistent launch that takes work items of every wave from one
+counter, its granularity chosen by the library's autotuner on the first
+capture of the batch's shape; otherwise each wave's launch step is one
+branch of the graph.  Waves whose product is not the sparse FP64 one keep
+their own branch, their exchange engine chosen by the per-shape autotuner
+among the capturable kernels.
 """
 from __future__ import annotations
 
@@ -39,9 +43,20 @@ class Batch:
             t = c.wave_timings()
             self.eager_ms.append(t["gemm"] + t["terms"])
 
-    def capture(self):
+    def capture(self, persistent: bool = True):
+        if self._graph:
+            L.lib().bsrg_graph_destroy(self._graph)
+            self._graph = C.c_void_p()
         arr = (C.c_void_p * len(self.ctx))(*[c._p for c in self.ctx])
-        L.check(L.lib().bsrg_graph_capture(len(self.ctx), arr, C.byref(self._graph)))
+        f = L.lib().bsrg_graph_capture_batched if persistent else L.lib().bsrg_graph_capture
+        L.check(f(len(self.ctx), arr, C.byref(self._graph)))
+
+    def batch_info(self):
+        """(waves in the persistent launch, blocks per work item chosen by
+        the autotuner; 0 = the waves' own grids)."""
+        nb, bpi = C.c_int(), C.c_int()
+        L.check(L.lib().bsrg_graph_batch_info(self._graph, C.byref(nb), C.byref(bpi)))
+        return nb.value, bpi.value
 
     def run(self):
         """One replay of the whole batch; returns (per-wave ms, graph ms)."""

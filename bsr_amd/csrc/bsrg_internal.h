@@ -144,6 +144,30 @@ struct TermEpi {
   const int* bptr;      // band-term CSR over the blocks (only emptiness is read)
   const double* bsum;   // summed band of each block with band terms, ns*ks per block
 };
+// One wave's fused exchange product in a persistent batched launch
+// (launch_xasm_batch): the arguments of its own k_xasm launch and the first
+// of its work items (ntile * ngroup items: tile = i % ntile, group = i /
+// ntile).
+struct XWave {
+  const double* E;
+  const int* xo;
+  const double* xc;
+  const int* xn;
+  TermEpi te;
+  const int* rptr;
+  const unsigned char* live;
+  double* hcc;
+  int ns, nblk, nr, nrs, ntile, ngroup, item0;
+};
+// The 64-position tiles of an ns^2 block and the default block groups of a
+// wave's own launch.
+void xasm_grid(int ns, int nblk, int& ntile, int& ngroup);
+// The fused products of nw waves (device array `waves`, items numbered
+// consecutively in array order, `total` in all) by nwg persistent
+// workgroups pulling items from `counter` (one int of device memory,
+// cleared here by a kernel); max_nr: the largest nr (LDS tile rows, at
+// most xasm_max_t); nw at most 256.
+int launch_xasm_batch(const XWave* waves, int nw, int total, int max_nr, int nwg, int* counter, hipStream_t s);
 // hcb[q] = base + sum_r c_r * sfac_r * V[v_r]; CSR over all nq positions
 void launch_vec_assemble(int ns, int nq, int has_base, const int* ptr, const int* vid, const int* sidx,
                          const double* c, const double* stab, const double* V, double* hcb, hipStream_t s);

@@ -2095,8 +2095,13 @@ struct bsrg_graph {
   std::vector<bsrg_ctx*> ctx;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipEvent_t t0 = nullptr, t1 = nullptr, fork = nullptr;
-  std::vector<hipEvent_t> join;
+  hipEvent_t t0 = nullptr, t1 = nullptr, fork = nullptr, fork2 = nullptr;
+  std::vector<hipEvent_t> join, join2;
+  // batched capture: the waves' product descriptors and the work counter
+  DevVec<XWave> xw;
+  DevVec<int> counter;
+  int batch_bpi = 0;  // blocks per work item chosen by the autotuner (0: the waves' own grids)
+  int nbatched = 0;   // waves whose products run in the persistent launch
 };
 
 int bsrg_graph_destroy(bsrg_graph* g) {
@@ -2104,6 +2109,8 @@ int bsrg_graph_destroy(bsrg_graph* g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   for (hipEvent_t e : g->join) if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : g->join2) if (e) (void)hipEventDestroy(e);
+  if (g->fork2) (void)hipEventDestroy(g->fork2);
   if (g->t0) (void)hipEventDestroy(g->t0);
   if (g->t1) (void)hipEventDestroy(g->t1);
   if (g->fork) (void)hipEventDestroy(g->fork);
@@ -2164,6 +2171,257 @@ int bsrg_graph_capture(int n, bsrg_ctx** ctxs, bsrg_graph** out) {
   if (e != hipSuccess) return fail(-100, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
   *out = g.release();
+  return 0;
+}
+
+namespace {
+
+// A prepared FP64 wave whose H pass is the fused sparse product (engine 4,
+// nothing enqueued yet): its launch step splits into pre (S pass, H lists
+// and summed bands), the product (batched) and post (rank-1 and vector
+// terms, symmetrisation), which together enqueue what bsrg_wave_launch does.
+bool batchable(bsrg_ctx* c) {
+  const Staged& st = c->stH;
+  // the persistent kernel's tile rows share the LDS with about 1 KB of its
+  // own (the waves' first items)
+  return c->precision_mode == 0 && st.has_gemm && st.engine == 4 && !c->gemm_enqueued &&
+         xasm_rows(c, st) <= xasm_max_t(c->device) - 4 && c->xzero.n >= static_cast<size_t>(st.nblk) + 1;
+}
+
+int h_fused_pre(bsrg_ctx* c) {
+  if (!c->s_enqueued)
+    if (int rc = enqueue_s(c)) return rc;
+  c->s_enqueued = false;
+  Staged& st = c->stH;
+  const int ns = c->G.ns, ks = c->G.ks, nb = c->nb;
+  launch_coef_pack(st.nblk, xasm_rows(c, st), st.gemm.p + static_cast<size_t>(st.gemm_klo) * nb * nb,
+                   static_cast<size_t>(c->basis_nk) * nb * nb, c->xt.p, c->xc.p, c->xn.p, c->stream);
+  c->last_engine = 4;
+  launch_term_prep(ns, ks, st.nblk, st.dp<int>(st.o_bptr), st.dp<int>(st.o_bid), st.dp<double>(st.o_bc), c->bands.p,
+                   c->bsum.p, st.dp<int>(st.o_rptr), st.dp<int>(st.o_rs), st.dp<double>(st.o_rc), c->St.p, c->rfac.p,
+                   c->stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+XWave h_fused_desc(bsrg_ctx* c, int bpi) {
+  Staged& st = c->stH;
+  const int ns = c->G.ns, ks = c->G.ks, nb = c->nb, nr = xasm_rows(c, st);
+  XWave x{};
+  x.E = c->E.p + static_cast<size_t>(st.gemm_klo) * nb * nb * static_cast<size_t>(ns) * ns;
+  x.xo = c->xt.p;
+  x.xc = c->xc.p;
+  x.xn = c->xn.p;
+  x.te = TermEpi{ns, ks, st.dp<int>(st.o_bptr), c->bsum.p};
+  x.rptr = st.dp<int>(st.o_rptr);
+  x.live = st.has_live && (!c->capturing || c->live_in_capture) ? st.dp<unsigned char>(st.o_live) : nullptr;
+  x.hcc = c->H.hcc.p;
+  x.ns = ns;
+  x.nblk = st.nblk;
+  x.nr = nr;
+  x.nrs = xasm_slot(nr);
+  xasm_grid(ns, st.nblk, x.ntile, x.ngroup);
+  // bpi > 0: about bpi blocks per work item, at least one per wavefront
+  if (bpi > 0) x.ngroup = std::max(1, std::min((st.nblk + bpi - 1) / bpi, std::max(1, st.nblk / 16)));
+  return x;
+}
+
+int h_fused_post(bsrg_ctx* c) {
+  Staged& st = c->stH;
+  const int ns = c->G.ns, ks = c->G.ks;
+  launch_assemble(ns, ks, st.nblk, 1, c->xzero.p, st.dp<int>(st.o_bid), st.dp<double>(st.o_bc), c->bands.p,
+                  st.dp<int>(st.o_rptr), st.dp<int>(st.o_ru), st.dp<int>(st.o_rw), st.dp<int>(st.o_rs),
+                  st.dp<double>(st.o_rc), c->St.p, c->VT.p, c->H.hcc.p, c->bsum.p, c->rfac.p, c->stream, nullptr, 0,
+                  0, nullptr, /*prep=*/false);
+  launch_vec_assemble(ns, st.nq, 0, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid), st.dp<int>(st.o_vs),
+                      st.dp<double>(st.o_vc), c->St.p, c->VT.p, c->H.hcb.p, c->stream);
+  c->gemm_was_early = false;
+  c->gemm_enqueued = false;
+  update_live(c, c->H, c->stH);
+  launch_symmetrize_diag(ns, c->H.nch, c->H.hcc.p, c->stream);
+  HIPCHK(hipGetLastError());
+  if (!c->capturing) c->eager_done = true;
+  return 0;
+}
+
+// The products of the batchable waves `bw` (in descending order of work) as
+// descriptors with consecutive item numbers; returns the item count.
+int batch_descs(const std::vector<bsrg_ctx*>& bw, int bpi, std::vector<XWave>& xs, int& max_nr) {
+  xs.clear();
+  max_nr = 0;
+  int items = 0;
+  for (bsrg_ctx* c : bw) {
+    XWave x = h_fused_desc(c, bpi);
+    x.item0 = items;
+    items += x.ntile * x.ngroup;
+    max_nr = std::max(max_nr, x.nr);
+    xs.push_back(x);
+  }
+  return items;
+}
+
+// Autotuner of the batched product: blocks per work item, timed once per
+// batch shape (the waves' ns, used bases and block counts) and cached.
+std::mutex g_batch_mu;
+std::map<std::vector<long long>, int> g_batch_bpi;
+
+int tune_batch(bsrg_graph* g, const std::vector<bsrg_ctx*>& bw, int nwg) {
+  std::vector<long long> key{bw[0]->device};
+  for (bsrg_ctx* c : bw)
+    key.push_back((static_cast<long long>(c->G.ns) << 40) | (static_cast<long long>(xasm_rows(c, c->stH)) << 24) |
+                  c->stH.nblk);
+  {
+    std::lock_guard<std::mutex> l(g_batch_mu);
+    auto it = g_batch_bpi.find(key);
+    if (it != g_batch_bpi.end()) return it->second;
+  }
+  hipStream_t s0 = bw[0]->stream;
+  // every wave's lists and summed bands in place, then each candidate timed
+  // on the first stream (the products overwrite H with partial sums; the
+  // batch's launches recompute everything)
+  for (bsrg_ctx* c : bw) {
+    if (int rc = h_fused_pre(c)) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  const int cands[] = {0, 32, 64, 128, 256, 512};
+  float best = 1e30f;
+  int best_bpi = 0;
+  std::vector<XWave> xs;
+  for (int bpi : cands) {
+    int max_nr = 0;
+    const int items = batch_descs(bw, bpi, xs, max_nr);
+    HIPCHK(g->xw.resize(xs.size()));
+    HIPCHK(hipMemcpy(g->xw.p, xs.data(), xs.size() * sizeof(XWave), hipMemcpyHostToDevice));
+    float ms = 0;
+    for (int rep = 0; rep < 2; ++rep) {
+      (void)hipEventRecord(bw[0]->ev[6], s0);
+      if (launch_xasm_batch(g->xw.p, static_cast<int>(xs.size()), items, max_nr, nwg, g->counter.p, s0))
+        return fail(-100, "batched product: bad launch");
+      (void)hipEventRecord(bw[0]->ev[7], s0);
+      HIPCHK(hipEventSynchronize(bw[0]->ev[7]));
+      (void)hipEventElapsedTime(&ms, bw[0]->ev[6], bw[0]->ev[7]);
+    }
+    HIPCHK(hipGetLastError());
+    if (ms < best) {
+      best = ms;
+      best_bpi = bpi;
+    }
+  }
+  std::lock_guard<std::mutex> l(g_batch_mu);
+  g_batch_bpi[key] = best_bpi;
+  return best_bpi;
+}
+
+}  // namespace
+
+int bsrg_graph_capture_batched(int n, bsrg_ctx** ctxs, bsrg_graph** out) {
+  if (n < 1 || !ctxs || !out) return fail(-1, "bsrg_graph_capture_batched: bad arguments");
+  for (int i = 0; i < n; ++i) {
+    if (int rc = check_ctx(ctxs[i])) return rc;
+    if (!ctxs[i]->wave_prepared) return fail(-8, "bsrg_graph_capture_batched: wave not prepared");
+    if (ctxs[i]->device != ctxs[0]->device) return fail(-5, "bsrg_graph_capture_batched: contexts on different devices");
+    if (ctxs[i]->stH.has_gemm && ctxs[i]->precision_mode == 0 && ctxs[i]->stH.engine == 0)
+      return fail(-9, "bsrg_graph_capture_batched: prepare with bsrg_set_capture_safe(ctx,1)");
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return fail(-1, "bsrg_graph_capture_batched: a context appears twice");
+  }
+  HIPCHK(hipSetDevice(ctxs[0]->device));
+  std::unique_ptr<bsrg_graph, int (*)(bsrg_graph*)> g(new bsrg_graph(), bsrg_graph_destroy);
+  g->ctx.assign(ctxs, ctxs + n);
+  HIPCHK(hipEventCreate(&g->t0));
+  HIPCHK(hipEventCreate(&g->t1));
+  HIPCHK(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&g->fork2, hipEventDisableTiming));
+  g->join.assign(n, nullptr);
+  g->join2.assign(n, nullptr);
+  for (int i = 0; i < n; ++i) {
+    HIPCHK(hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&g->join2[i], hipEventDisableTiming));
+  }
+  hipStream_t s0 = ctxs[0]->stream;
+  for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(ctxs[i]->stream));
+  // the waves whose products join the persistent launch, most work first
+  std::vector<bsrg_ctx*> bw;
+  for (int i = 0; i < n; ++i)
+    if (batchable(ctxs[i]) && static_cast<int>(bw.size()) < 256) bw.push_back(ctxs[i]);
+  std::stable_sort(bw.begin(), bw.end(), [](bsrg_ctx* a, bsrg_ctx* b) {
+    return static_cast<double>(a->stH.nblk) * a->G.ns * a->G.ns > static_cast<double>(b->stH.nblk) * b->G.ns * b->G.ns;
+  });
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctxs[0]->device) != hipSuccess || ncu < 1) {
+    (void)hipGetLastError();
+    ncu = 256;
+  }
+  HIPCHK(g->counter.resize(1));
+  if (!bw.empty()) {
+    const int bpi = tune_batch(g.get(), bw, ncu);
+    if (bpi < 0) return bpi;
+    g->batch_bpi = bpi;
+  }
+  // exact live maps of waves that ran eagerly (as in bsrg_graph_capture)
+  for (int i = 0; i < n; ++i) {
+    bsrg_ctx* c = ctxs[i];
+    if (!c->eager_done) continue;
+    for (Staged* st : {&c->stS, &c->stH}) {
+      Matrices& M = st == &c->stS ? c->S : c->H;
+      if (!st->has_live || M.live.size() < static_cast<size_t>(st->nblk)) continue;
+      std::memcpy(st->hp<unsigned char>(st->o_live), M.live.data(), st->nblk);
+      HIPCHK(hipMemcpyAsync(st->d.p + st->o_live, st->hp<unsigned char>(st->o_live), st->nblk,
+                            hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->live_in_capture = true;
+  }
+  std::vector<XWave> xs;
+  int max_nr = 0, items = 0;
+  for (bsrg_ctx* c : bw) c->capturing = true;  // descriptors as the capture sees them
+  items = batch_descs(bw, g->batch_bpi, xs, max_nr);
+  for (bsrg_ctx* c : bw) c->capturing = false;
+  if (!xs.empty()) {
+    HIPCHK(g->xw.resize(xs.size()));
+    HIPCHK(hipMemcpy(g->xw.p, xs.data(), xs.size() * sizeof(XWave), hipMemcpyHostToDevice));
+  }
+  g->nbatched = static_cast<int>(xs.size());
+  auto in_batch = [&](bsrg_ctx* c) { return std::find(bw.begin(), bw.end(), c) != bw.end(); };
+  HIPCHK(hipStreamBeginCapture(s0, hipStreamCaptureModeGlobal));
+  int rc = 0;
+  for (int i = 0; i < n; ++i) ctxs[i]->capturing = true;
+  // branches: S passes and the H lists of the batched waves, whole launch
+  // steps of the others; then the persistent product; then the rank-1 and
+  // vector terms of the batched waves
+  (void)hipEventRecord(g->fork, s0);
+  for (int i = 1; i < n && !rc; ++i)
+    if (hipStreamWaitEvent(ctxs[i]->stream, g->fork, 0) != hipSuccess) rc = fail(-100, "graph fork failed");
+  for (int i = 0; i < n && !rc; ++i) rc = in_batch(ctxs[i]) ? h_fused_pre(ctxs[i]) : bsrg_wave_launch(ctxs[i]);
+  for (int i = 1; i < n && !rc; ++i) {
+    (void)hipEventRecord(g->join[i], ctxs[i]->stream);
+    if (hipStreamWaitEvent(s0, g->join[i], 0) != hipSuccess) rc = fail(-100, "graph join failed");
+  }
+  if (!rc && !xs.empty() &&
+      launch_xasm_batch(g->xw.p, static_cast<int>(xs.size()), items, max_nr, ncu, g->counter.p, s0))
+    rc = fail(-100, "batched product: bad launch");
+  (void)hipEventRecord(g->fork2, s0);
+  for (int i = 1; i < n && !rc; ++i)
+    if (hipStreamWaitEvent(ctxs[i]->stream, g->fork2, 0) != hipSuccess) rc = fail(-100, "graph fork failed");
+  for (int i = 0; i < n && !rc; ++i)
+    if (in_batch(ctxs[i])) rc = h_fused_post(ctxs[i]);
+  for (int i = 1; i < n && !rc; ++i) {
+    (void)hipEventRecord(g->join2[i], ctxs[i]->stream);
+    if (hipStreamWaitEvent(s0, g->join2[i], 0) != hipSuccess) rc = fail(-100, "graph join failed");
+  }
+  hipError_t e = hipStreamEndCapture(s0, &g->graph);
+  for (int i = 0; i < n; ++i) ctxs[i]->capturing = ctxs[i]->live_in_capture = false;
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(-100, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  HIPCHK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  *out = g.release();
+  return 0;
+}
+
+int bsrg_graph_batch_info(bsrg_graph* g, int* nbatched, int* blocks_per_item) {
+  if (!g) return fail(-1, "bsrg_graph_batch_info: no graph");
+  if (nbatched) *nbatched = g->nbatched;
+  if (blocks_per_item) *blocks_per_item = g->batch_bpi;
   return 0;
 }
 

@@ -43,7 +43,7 @@ namespace bsrg {
 namespace {
 
 constexpr int kXWaves = 16;  // wavefronts per workgroup (4 per SIMD)
-constexpr int kXChunk = 8;  // list entries per scalar-load round
+constexpr int kXChunk = 8;  // list entries per scalar-load round (lists are padded to a multiple)
 
 // Per block (one wavefront): the nonzeros of column blk of C (rows 0..nr-1,
 // leading dimension ldc) in ascending t, stored as (byte offset t*rowb of
@@ -79,24 +79,29 @@ __global__ __launch_bounds__(256) void k_coef_pack(int nblk, int nr, int nrs, co
   if (lane == 0) xn[blk] = base;
 }
 
-// grid ntile * ngroup: workgroup -> (tile = id % ntile, group = id / ntile);
-// group g owns blocks [g*per, min(nblk, (g+1)*per)), per = ceil(nblk/ngroup).
-// E: the used bases, column t at E + t*nss.  LDS: nr * 64 doubles.
-// The lists of a block are wave-uniform: kXChunk entries at a time come in
-// through scalar loads and serve as scalar operands, so the inner loop is
-// one address add, one LDS read and one FMA per nonzero (the padding adds
-// zeros).  The band term (one read of the block's summed band, for the
-// elements within ks-1 of the diagonal) is requested before the sum and
-// added after it in the fused form.
+// One work item: tile `tile` (64 positions) of the blocks of group `grp`
+// ([grp*per, min(nblk, (grp+1)*per)), per = ceil(nblk/ngroup)), by the
+// whole 1024-thread workgroup.  E: the used bases, column t at E + t*nss.
+// LDS: nr * 64 doubles.  The lists of a block are wave-uniform: kXChunk
+// entries at a time come in through scalar loads and serve as scalar
+// operands, so the inner loop is one address add, one LDS read and one FMA
+// per nonzero (the padding adds zeros).  The band term (one read of the
+// block's summed band, for the elements within ks-1 of the diagonal) is
+// requested before the sum and added after it in the fused form.
 template <bool FUSED>
-__global__ __launch_bounds__(64 * kXWaves) void k_xasm(
-    int ns, int nblk, int nr, int nrs, int ntile, int ngroup, const double* __restrict__ E,
-    const int* __restrict__ xo, const double* __restrict__ xc, const int* __restrict__ xn, TermEpi te,
-    const int* __restrict__ rptr, const unsigned char* __restrict__ live, double* __restrict__ hcc) {
-  extern __shared__ double El[];
+__device__ __forceinline__ void xasm_item(int ns, int nblk, int nr, int nrs, int ngroup, int tile, int grp,
+                                          const double* __restrict__ E, const int* __restrict__ xo,
+                                          const double* __restrict__ xc, const int* __restrict__ xn,
+                                          const TermEpi& te, const int* __restrict__ rptr,
+                                          const unsigned char* __restrict__ live, double* __restrict__ hcc,
+                                          double* El) {
   const size_t nss = static_cast<size_t>(ns) * ns;
-  const int tile = static_cast<int>(blockIdx.x % ntile), grp = static_cast<int>(blockIdx.x / ntile);
   const size_t p0 = static_cast<size_t>(tile) * 64;
+  const gptr<double> Eg = as_global(const_cast<double*>(E));
+  const gptr<double> hg = as_global(hcc);
+  const cptr<int> xoc = as_const(xo), xnc = as_const(xn), rpc = as_const(rptr), bpc = as_const(te.bptr);
+  const cptr<double> xcc = as_const(xc);
+  const cptr<unsigned char> livec = as_const(live);
   {
     // stage the tile: eight loads in flight per thread
     const int tot = nr * 64;
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(64 * kXWaves) void k_xasm(
       for (int u = 0; u < 8; ++u) {
         const int q = q0 + u * 64 * kXWaves, t = q >> 6;
         const size_t pp = p0 + (q & 63);
-        v[u] = (q < tot && pp < nss) ? E[static_cast<size_t>(t) * nss + pp] : 0.0;
+        v[u] = (q < tot && pp < nss) ? Eg[static_cast<size_t>(t) * nss + pp] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -124,16 +129,31 @@ __global__ __launch_bounds__(64 * kXWaves) void k_xasm(
   const char* Eb = reinterpret_cast<const char*>(El + lane);
   const int per = (nblk + ngroup - 1) / ngroup, blo = grp * per, bhi = min(nblk, blo + per);
   for (int blk = blo + wv; blk < bhi; blk += kXWaves) {
-    const int n = xn[blk];
+    const int n = xnc[blk];
     // a block without terms that already holds zeros stays as it is
-    if (FUSED && n == 0 && te.bptr[blk] == te.bptr[blk + 1] && rptr[blk] == rptr[blk + 1] && live && !live[blk])
+    if (FUSED && n == 0 && bpc[blk] == bpc[blk + 1] && rpc[blk] == rpc[blk + 1] && live && !livec[blk])
       continue;
     double band = 0.0;
     const bool has_band = FUSED && epi_band(te, blk, x, y, band);
-    const int* ol = xo + static_cast<size_t>(blk) * nrs;
-    const double* cl = xc + static_cast<size_t>(blk) * nrs;
+    const cptr<int> ol = xoc + static_cast<size_t>(blk) * nrs;
+    const cptr<double> cl = xcc + static_cast<size_t>(blk) * nrs;
     double a[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int j0 = 0; j0 < n; j0 += kXChunk) {
+    // two chunks (16 LDS reads in flight) per round over the padded list,
+    // then the last chunk
+    const int npad = (n + kXChunk - 1) / kXChunk * kXChunk;
+    int j0 = 0;
+    for (; j0 + 2 * kXChunk <= npad; j0 += 2 * kXChunk) {
+      int oo[2 * kXChunk];
+      double cc[2 * kXChunk];
+#pragma unroll
+      for (int i = 0; i < 2 * kXChunk; ++i) {
+        oo[i] = ol[j0 + i];
+        cc[i] = cl[j0 + i];
+      }
+#pragma unroll
+      for (int i = 0; i < 2 * kXChunk; ++i) a[i & 3] += cc[i] * *reinterpret_cast<const double*>(Eb + oo[i]);
+    }
+    if (j0 < npad) {
       int oo[kXChunk];
       double cc[kXChunk];
 #pragma unroll
@@ -146,7 +166,59 @@ __global__ __launch_bounds__(64 * kXWaves) void k_xasm(
     }
     double acc = (a[0] + a[1]) + (a[2] + a[3]);
     if (has_band) acc += band;
-    if (valid) hcc[static_cast<size_t>(blk) * nss + p] = acc;
+    if (valid) hg[static_cast<size_t>(blk) * nss + p] = acc;
+  }
+}
+
+// grid ntile * ngroup: workgroup -> (tile = id % ntile, group = id / ntile).
+template <bool FUSED>
+__global__ __launch_bounds__(64 * kXWaves) void k_xasm(
+    int ns, int nblk, int nr, int nrs, int ntile, int ngroup, const double* __restrict__ E,
+    const int* __restrict__ xo, const double* __restrict__ xc, const int* __restrict__ xn, TermEpi te,
+    const int* __restrict__ rptr, const unsigned char* __restrict__ live, double* __restrict__ hcc) {
+  extern __shared__ double El[];
+  xasm_item<FUSED>(ns, nblk, nr, nrs, ngroup, static_cast<int>(blockIdx.x % ntile),
+                   static_cast<int>(blockIdx.x / ntile), E, xo, xc, xn, te, rptr, live, hcc, El);
+}
+
+// The fused products of a batch of waves in one persistent launch: each
+// workgroup takes the next work item (a tile of a group of blocks of one
+// wave) from a counter until the counter passes the last item.  The waves'
+// items are numbered in order (wave w owns [item0, item0 + ntile*ngroup));
+// the host puts the waves with the most work first, so the small items of
+// the small waves fill the end.  Every element is computed by xasm_item as
+// in the wave's own launch, so the results are the same bits.  counter is
+// zero at the start (the launcher clears it with a kernel).
+constexpr int kXBatchMax = 256;  // waves in one batched launch
+__global__ __launch_bounds__(64 * kXWaves) void k_xasm_batch(const XWave* __restrict__ waves, int nw, int total,
+                                                             int* __restrict__ counter) {
+  extern __shared__ double El[];
+  __shared__ int s_it[2], s_first[kXBatchMax];
+  for (int w = threadIdx.x; w < nw; w += blockDim.x) s_first[w] = waves[w].item0;
+  if (threadIdx.x == 0) s_it[0] = atomicAdd(counter, 1);
+  __syncthreads();
+  // the item index is the same in every lane; taken as a scalar, everything
+  // derived from it (tile, group, block numbers, list addresses) stays in
+  // scalar registers and the lists come in through scalar loads
+  int it = __builtin_amdgcn_readfirstlane(s_it[0]);
+  // the next item's index is taken while the current item runs: item k
+  // reads slot k&1, and thread 0 fills slot (k+1)&1 before the barrier that
+  // ends item k (every thread read that slot before the barrier of item k-1)
+  for (int k = 0; it < total; ++k) {  // `it` is the same in every thread
+    if (threadIdx.x == 0) s_it[(k + 1) & 1] = atomicAdd(counter, 1);
+    // the wave of the item: the last one whose first item is <= it
+    int lo = 0, hi = nw - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_first[mid] <= it) lo = mid;
+      else hi = mid - 1;
+    }
+    const XWave X = waves[__builtin_amdgcn_readfirstlane(lo)];  // held in scalar registers for the item
+    const int local = it - X.item0;
+    xasm_item<true>(X.ns, X.nblk, X.nr, X.nrs, X.ngroup, local % X.ntile, local / X.ntile, X.E, X.xo, X.xc, X.xn,
+                    X.te, X.rptr, X.live, X.hcc, El);
+    __syncthreads();  // the tile is free, and the next index is in its slot
+    it = __builtin_amdgcn_readfirstlane(s_it[(k + 1) & 1]);
   }
 }
 
@@ -284,7 +356,6 @@ void launch_coef_pack(int nblk, int nr, const double* C, size_t ldc, int* xo, do
                      xc, xn);
 }
 
-namespace {
 void xasm_grid(int ns, int nblk, int& ntile, int& ngroup) {
   const size_t nss = static_cast<size_t>(ns) * ns;
   ntile = static_cast<int>((nss + 63) / 64);
@@ -293,7 +364,15 @@ void xasm_grid(int ns, int nblk, int& ntile, int& ngroup) {
   ngroup = std::max(1, (6 * 256 + ntile - 1) / ntile);
   ngroup = std::min(ngroup, std::max(1, nblk / kXWaves));
 }
-}  // namespace
+
+int launch_xasm_batch(const XWave* waves, int nw, int total, int max_nr, int nwg, int* counter, hipStream_t s) {
+  if (nw < 1 || total < 1) return 0;
+  if (nw > kXBatchMax || max_nr < 1 || nwg < 1) return -1;
+  launch_fill_int(1, 0, counter, s);
+  hipLaunchKernelGGL(k_xasm_batch, dim3(nwg), dim3(64 * kXWaves), static_cast<size_t>(max_nr) * 64 * sizeof(double),
+                     s, waves, nw, total, counter);
+  return 0;
+}
 
 int launch_xprod(int ns, int nblk, int nr, const double* E, const int* xo, const double* xc, const int* xn,
                  double* hcc, hipStream_t s) {

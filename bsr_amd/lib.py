@@ -131,6 +131,8 @@ SIGNATURES = {
     "bsrg_graph_launch": [C.c_void_p],
     "bsrg_graph_sync": [C.c_void_p, _odp, C.POINTER(C.c_double)],
     "bsrg_graph_destroy": [C.c_void_p],
+    "bsrg_graph_capture_batched": [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
+    "bsrg_graph_batch_info": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
 }
 _RESTYPE = {"bsrg_last_error": C.c_char_p, "bsrg_lb_create": C.c_void_p, "bsrg_lb_destroy": None}
 

@@ -477,6 +477,23 @@ int bsrg_graph_launch(bsrg_graph* graph);
 int bsrg_graph_sync(bsrg_graph* graph, double* wave_ms, double* total_ms);
 int bsrg_graph_destroy(bsrg_graph* graph);
 
+/* As bsrg_graph_capture, but the exchange products of the FP64 waves whose
+ * H pass is the fused sparse product (engine 4) run in ONE persistent
+ * launch: persistent workgroups (one per CU) take work items (a 64-position
+ * tile of a group of blocks of one wave) from a counter, the waves with the
+ * most work first.  Branches before it run the S passes and the waves'
+ * lists and summed bands, branches after it the rank-1 and vector terms;
+ * other waves keep their whole launch step in their branch.  The
+ * granularity (blocks per work item) is chosen by timing candidates on the
+ * first capture of a batch shape and cached; the timing leaves partial
+ * products in H until the graph runs.  Each element is summed as in the
+ * wave's own launch: the results equal bsrg_graph_capture's bit for bit.
+ * batch_info: the number of waves in the persistent launch and the chosen
+ * blocks per item (0: the waves' own grids).  No counterpart in the
+ * reference (its waves are separate program runs, bsr_mat.f90:82-91). */
+int bsrg_graph_capture_batched(int n, bsrg_ctx** ctxs, bsrg_graph** graph);
+int bsrg_graph_batch_info(bsrg_graph* graph, int* nbatched, int* blocks_per_item);
+
 /* Error of one assembled result against a reference result of the same
  * wave (e.g. split precision against FP64), S (which=0) or H (which=1):
  * rel = max over hcc, hcb, hbb of |a - r| / max(|r|, floor_frac * max|r|).

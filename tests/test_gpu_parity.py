@@ -408,9 +408,11 @@ def test_fused_term_pass_equals_separate(ctx, oracle_lib, mode, nslices):
     assert rel_err(S2.hcc, oS.hcc) < TOL
 
 
-def test_batch_graph(oracle_lib):
-    """Uneven waves captured into one hipGraph: results match the oracle and
-    two replays give identical bits."""
+@pytest.mark.parametrize("persistent", [True, False])
+def test_batch_graph(oracle_lib, persistent):
+    """Uneven waves captured into one hipGraph (their exchange products in
+    one persistent launch, or one branch per wave): results match the oracle
+    and two replays give identical bits."""
     from tests.conftest import HAS_GPU
     if not HAS_GPU:
         pytest.skip("no HIP device")
@@ -422,7 +424,7 @@ def test_batch_graph(oracle_lib):
     b = Batch(waves)
     try:
         b.prepare()
-        b.capture()
+        b.capture(persistent=persistent)
         per, tot = b.run()
         assert len(per) == 3 and tot > 0
         first = [c.wave_get(1).hcc.copy() for c in b.ctx]

@@ -120,7 +120,8 @@ def _digest(M):
 def test_c3_batch_graph_equals_eager():
     """The 48 uneven waves of C3 (nch 10..120, ns 100..250, k<=8): the
     graph replay of the batch gives, for every wave, the bits of its eager
-    run."""
+    run, with the exchange products of all 48 waves in one persistent launch
+    and with one branch per wave."""
     if not HAS_GPU:
         pytest.skip("no HIP device")
     from bsr_amd import synth
@@ -137,12 +138,17 @@ def test_c3_batch_graph_equals_eager():
             eager.append((_digest(c.wave_get(0)), _digest(c.wave_get(1))))
             if j % 12 == 11:
                 _say(f"[c3-48] eager digests {j + 1}/48")
-        b.capture()
-        per, tot = b.run()
-        _say(f"[c3-48] graph replay {tot:.2f} ms")
-        for j, (c, e) in enumerate(zip(b.ctx, eager)):
-            assert (_digest(c.wave_get(0)), _digest(c.wave_get(1))) == e, f"wave {j}"
-        _say("[c3-48] every wave bit-identical to its eager run")
+        for persistent in (True, False):
+            b.capture(persistent=persistent)
+            if persistent:
+                nb, bpi = b.batch_info()
+                _say(f"[c3-48] persistent launch of {nb} waves, {bpi} blocks per item")
+                assert nb == len(waves)
+            per, tot = b.run()
+            _say(f"[c3-48] graph replay ({'persistent' if persistent else 'per wave'}) {tot:.2f} ms")
+            for j, (c, e) in enumerate(zip(b.ctx, eager)):
+                assert (_digest(c.wave_get(0)), _digest(c.wave_get(1))) == e, f"wave {j}"
+            _say("[c3-48] every wave bit-identical to its eager run")
     finally:
         b.close()
 
