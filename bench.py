@@ -74,7 +74,7 @@ PEAK_BF16_MFMA = 2500.0  # TFLOP/s, MI355X dense bf16 matrix
 # the headline shape: two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE)
 # scaled by a 1 GiB device copy in the same process (read 308.1 MB, write
 # 585.6 MB)
-XPROD_TRAFFIC_BYTES = 893.7e6
+XPROD_TRAFFIC_BYTES = 893.6e6
 
 
 def log(msg):
