@@ -72,7 +72,7 @@ PEAK_INT8_MFMA = 5000.0  # TOP/s, MI355X dense int8 matrix
 PEAK_BF16_MFMA = 2500.0  # TFLOP/s, MI355X dense bf16 matrix
 # HBM bytes of one launch of the sparse exchange product (k_xasm<false>) at
 # the headline shape: two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE)
-# scaled by a 1 GiB device copy in the same process (read 308.1 MB, write
+# scaled by a 1 GiB device copy in the same process (read 308.0 MB, write
 # 585.6 MB)
 XPROD_TRAFFIC_BYTES = 893.6e6
 
@@ -510,6 +510,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-eigen", action="store_true", help="skip the eigenvalue check of the split forms")
+    ap.add_argument("--no-eigen-dense", action="store_true", help="skip the dense eigen-solve beside block Lanczos")
     ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--batch-waves", type=int, default=48)
     ap.add_argument("--no-dirac", action="store_true")
@@ -607,12 +608,26 @@ def main():
     e_ref = None
     if not a.no_eigen:
         # lowest 10 eigenvalues of H x = E S x (FP64 on the device), the
-        # reference point of the split forms' eigenvalue error
+        # reference point of the split forms' eigenvalue error: block
+        # Lanczos (the default at this size), then the dense solver on the
+        # same matrices as an independent check
         t0 = time.perf_counter()
         e_ref = ctx.wave_eigen(10)
         eig_s = time.perf_counter() - t0
-        log(f"eigen: lowest {e_ref[:3]} ... in {eig_s:.2f} s (nhm={w.nhm})")
-        out["eigen"] = {"lowest10": [float(x) for x in e_ref], "solve_s": eig_s}
+        info = ctx.eigen_info()
+        log(f"eigen: lowest {e_ref[:3]} ... in {eig_s:.2f} s (nhm={w.nhm}, {info})")
+        out["eigen"] = {"lowest10": [float(x) for x in e_ref], "solve_s": eig_s,
+                        "method": "block Lanczos" if info["method"] == 2 else "dense (sygvdx)",
+                        "krylov_dim": info["dim"], "residual_bound": info["resid"]}
+        if not a.no_eigen_dense:
+            ctx.set_eigen_method(1)
+            t0 = time.perf_counter()
+            e_dense = ctx.wave_eigen(10)
+            dense_s = time.perf_counter() - t0
+            ctx.set_eigen_method(0)
+            de = float(np.max(np.abs(e_dense - e_ref)))
+            out["eigen"]["dense"] = {"solve_s": dense_s, "max_abs_diff": de}
+            log(f"eigen dense: {dense_s:.2f} s, max |dE| against Lanczos {de:.2e}")
     if not a.no_split:
         # int8 slices (the split form used below) and bf16 slices
         for key, mode in (("split", 2), ("split_bf16", 1)):

@@ -423,6 +423,18 @@ class Context:
         L.check(L.lib().bsrg_wave_eigen(self._p, nev, e))
         return e
 
+    def set_eigen_method(self, method: int = 0):
+        """0 automatic (block Lanczos from nhm 2048 on), 1 dense (sygvdx), 2
+        block Lanczos (falls back to dense when it does not converge)."""
+        L.check(L.lib().bsrg_set_eigen_method(self._p, method))
+
+    def eigen_info(self) -> dict:
+        """Method the last eigenvalue call used (1 dense, 2 block Lanczos),
+        its Krylov dimension and its largest residual bound."""
+        m, d, r = C.c_int(), C.c_int(), C.c_double()
+        L.check(L.lib().bsrg_eigen_info(self._p, C.byref(m), C.byref(d), C.byref(r)))
+        return dict(method=m.value, dim=d.value, resid=r.value)
+
     def matfile_eigen(self, path, nev: int = 10) -> np.ndarray:
         """The nev lowest eigenvalues of H x = E S x of a bsr_mat.nnn file
         (read as BSR_HD3 reads it, solved on the device)."""

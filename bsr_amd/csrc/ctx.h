@@ -336,6 +336,11 @@ struct bsrg_ctx {
   // while the exchange GEMMs compute); default priority like the other
   hipStream_t copy_stream = nullptr;
   rocblas_handle blas = nullptr;
+  // eigenvalues: 0 automatic (block Lanczos from nhm 2048 on, else the dense
+  // solver), 1 dense (sygvdx), 2 block Lanczos; what the last call used, its
+  // Krylov dimension and largest residual bound
+  int eigen_method = 0, eigen_used = 0, eigen_dim = 0;
+  double eigen_resid = 0.0;
   hipEvent_t ev[10] = {};  // 0-5 phases of a wave, 6-7 autotuning, 8 end of the bases
   double timings[5] = {0, 0, 0, 0, 0};
   // host ms of the last prepare: [0] checks and band/basis launches, [1] S and

@@ -13,6 +13,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -61,6 +62,212 @@ int densify(const Matrices& M, double* A, size_t lda, DevVec<double>& hbb, hipSt
   return 0;
 }
 
+// Upper triangle of a symmetric matrix from its lower one, through 32 x 32
+// LDS tiles: tile (ti, tj), ti >= tj, reads the lower tile (rows ti,
+// columns tj) and writes its transpose to (rows tj, columns ti); on the
+// diagonal tiles only the strictly upper part is written.
+// grid nt*(nt+1)/2, block 256 (32 x 8).
+__global__ void k_sym_upper(int n, double* A, size_t lda) {
+  __shared__ double t[32][33];
+  int b = blockIdx.x, ti = 0;
+  while (b > ti) { b -= ti + 1; ++ti; }
+  const int tj = b;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int row = ti * 32 + tx, col = tj * 32 + r;
+    t[r][tx] = (row < n && col < n) ? A[row + lda * col] : 0.0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int row = tj * 32 + tx, col = ti * 32 + r;  // element (row, col) = A(col, row)
+    if (row < n && col < n && row < col) A[row + lda * col] = t[tx][r];
+  }
+}
+
+// A deterministic start block: V(i, j) uniform in [-1, 1) from a hash of (i, j).
+__global__ void k_start_block(int n, int p, double* V) {
+  const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= static_cast<size_t>(n) * p) return;
+  unsigned long long z = q * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  V[q] = static_cast<double>(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+// Cholesky factor R (upper, R^T R = G) of a small symmetric p x p matrix in
+// place (column-major); false when G is not numerically positive definite.
+bool chol_upper(int p, double* G) {
+  for (int j = 0; j < p; ++j) {
+    double d = G[j + p * j];
+    for (int k = 0; k < j; ++k) d -= G[k + p * j] * G[k + p * j];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    G[j + p * j] = d;
+    for (int i = j + 1; i < p; ++i) {
+      double v = G[j + p * i];
+      for (int k = 0; k < j; ++k) v -= G[k + p * j] * G[k + p * i];
+      G[j + p * i] = v / d;
+    }
+    for (int i = j + 1; i < p; ++i) G[i + p * j] = 0.0;
+  }
+  return true;
+}
+
+#define RBCHK(x)                                                                        \
+  do {                                                                                  \
+    rocblas_status rs_ = (x);                                                           \
+    if (rs_ != rocblas_status_success)                                                  \
+      return fail(-100, std::string(#x " failed: ") + rocblas_status_to_string(rs_)); \
+  } while (0)
+
+// The nev lowest eigenvalues of H x = E S x by block Lanczos.  S = L L^T
+// (potrf), C = L^-1 H L^-T (sygst) in full storage, then block Lanczos on C
+// with blocks of p vectors and full reorthogonalisation (two classical
+// Gram-Schmidt passes against every earlier block, Cholesky QR twice for
+// the new block): T = Q^T C Q is block tridiagonal with the symmetrised
+// diagonal blocks and the QR factors below and above the diagonal.  The
+// eigenvalues of T (rocSOLVER syevd on the device) are checked at growing
+// Krylov dimensions; Ritz pair (theta, Q s) has residual ||B s_last|| (B the
+// coupling to the next block, s_last the last block of s), and an
+// eigenvalue of C lies within that residual of theta, so the nev lowest are
+// returned once each residual is below tol max(1, |theta|).  The start
+// block is fixed, so results are reproducible.  Returns 1 when the Krylov
+// space reaches its limit without convergence, or a new block is
+// numerically rank-deficient, for the dense solver to take over.
+int solve_lanczos(bsrg_ctx* c, const Matrices& H, const Matrices& S, int nev, double* e, double tol) {
+  const int n = static_cast<int>(static_cast<size_t>(H.nch) * H.ns + H.ncp);
+  const int p = std::min(64, std::max(16, nev + 6));
+  const int mmax = std::min(n - p, std::max(64 * p, 1024)) / p * p;
+  if (mmax < 4 * p) return 1;
+  hipStream_t s = c->stream;
+  rocblas_handle h = c->blas;
+  DevVec<double> A, B, Q, Hh, Tm, Tev, Twk, hbb, hbbS;
+  DevVec<int> info;
+  HIPCHK(A.resize(static_cast<size_t>(n) * n));
+  HIPCHK(B.resize(static_cast<size_t>(n) * n));
+  HIPCHK(Q.resize(static_cast<size_t>(n) * (mmax + p)));
+  HIPCHK(Hh.resize(static_cast<size_t>(mmax + p) * p));
+  HIPCHK(info.resize(2));
+  if (int rc = densify(H, A.p, n, hbb, s)) return rc;
+  if (int rc = densify(S, B.p, n, hbbS, s)) return rc;
+  RBCHK(rocsolver_dpotrf(h, rocblas_fill_lower, n, B.p, n, info.p));
+  int hinfo = 0;
+  HIPCHK(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (hinfo != 0) return fail(-40, "S is not positive definite (DPOTRF: leading minor " + std::to_string(hinfo) + ")");
+  RBCHK(rocsolver_dsygst(h, rocblas_eform_ax, rocblas_fill_lower, n, A.p, n, B.p, n));
+  B.release();
+  {
+    const int nt = (n + 31) / 32;
+    hipLaunchKernelGGL(k_sym_upper, dim3(static_cast<unsigned>(nt) * (nt + 1) / 2), dim3(256), 0, s, n, A.p,
+                       static_cast<size_t>(n));
+  }
+  const size_t np = static_cast<size_t>(n) * p;
+  hipLaunchKernelGGL(k_start_block, dim3(static_cast<unsigned>((np + 255) / 256)), dim3(256), 0, s, n, p, Q.p);
+  HIPCHK(hipGetLastError());
+  const double one = 1.0, zero = 0.0, mone = -1.0;
+  std::vector<double> G(static_cast<size_t>(p) * p), Rb(static_cast<size_t>(p) * p), R1(static_cast<size_t>(p) * p);
+  DevVec<double> Gd;
+  HIPCHK(Gd.resize(static_cast<size_t>(p) * p));
+  // W <- W R^-1 with R^T R = W^T W, twice; R = R2 R1 on return
+  auto chol_qr = [&](double* W, std::vector<double>& R) -> int {
+    for (int pass = 0; pass < 2; ++pass) {
+      RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, p, p, n, &one, W, n, W, n, &zero,
+                          Gd.p, p));
+      HIPCHK(hipMemcpyAsync(G.data(), Gd.p, G.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (!chol_upper(p, G.data())) return 1;
+      HIPCHK(hipMemcpyAsync(Gd.p, G.data(), G.size() * sizeof(double), hipMemcpyHostToDevice, s));
+      RBCHK(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                          rocblas_diagonal_non_unit, n, p, &one, Gd.p, p, W, n));
+      if (pass == 0) {
+        R1 = G;
+      } else {  // R = G R1 (both upper)
+        for (int j = 0; j < p; ++j)
+          for (int i = 0; i < p; ++i) {
+            double v = 0.0;
+            for (int k = i; k <= j; ++k) v += G[i + p * k] * R1[k + p * j];
+            R[i + p * j] = v;
+          }
+      }
+    }
+    return 0;
+  };
+  if (int rc = chol_qr(Q.p, Rb)) return rc > 0 ? 1 : rc;
+  std::vector<double> T(static_cast<size_t>(mmax) * mmax, 0.0), Aj(static_cast<size_t>(p) * p);
+  HIPCHK(Tm.resize(static_cast<size_t>(mmax) * mmax));
+  HIPCHK(Tev.resize(mmax));
+  HIPCHK(Twk.resize(mmax));
+  std::vector<double> ev(mmax), Vlast(static_cast<size_t>(p) * nev);
+  int next_check = 4 * p;
+  for (int m = p; m <= mmax; m += p) {
+    // block j = m/p - 1 at columns [m-p, m); the next one goes to [m, m+p)
+    const double* Vj = Q.p + static_cast<size_t>(n) * (m - p);
+    double* W = Q.p + static_cast<size_t>(n) * m;
+    RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, p, n, &one, A.p, n, Vj, n, &zero, W,
+                        n));
+    // the diagonal block, then W orthogonalised against every block so far
+    RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, p, p, n, &one, Vj, n, W, n, &zero,
+                        Gd.p, p));
+    HIPCHK(hipMemcpyAsync(Aj.data(), Gd.p, Aj.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    for (int pass = 0; pass < 2; ++pass) {
+      RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, p, n, &one, Q.p, n, W, n, &zero,
+                          Hh.p, m));
+      RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, p, m, &mone, Q.p, n, Hh.p, m, &one, W,
+                          n));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    for (int j = 0; j < p; ++j)
+      for (int i = 0; i < p; ++i) T[(m - p + i) + static_cast<size_t>(mmax) * (m - p + j)] = 0.5 * (Aj[i + p * j] + Aj[j + p * i]);
+    // a new block that is numerically rank-deficient ends the iteration
+    // (the dense solver takes over)
+    if (chol_qr(W, Rb) != 0) return 1;
+    if (m < mmax)
+      for (int j = 0; j < p; ++j)
+        for (int i = 0; i < p; ++i) {
+          T[(m + i) + static_cast<size_t>(mmax) * (m - p + j)] = Rb[i + p * j];
+          T[(m - p + j) + static_cast<size_t>(mmax) * (m + i)] = Rb[i + p * j];
+        }
+    if (m < next_check && m < mmax) continue;
+    next_check = m + std::max(4 * p, m / 2);
+    // eigenpairs of T(0:m, 0:m) on the device
+    HIPCHK(hipMemcpy2DAsync(Tm.p, static_cast<size_t>(m) * sizeof(double), T.data(),
+                            static_cast<size_t>(mmax) * sizeof(double), static_cast<size_t>(m) * sizeof(double), m,
+                            hipMemcpyHostToDevice, s));
+    RBCHK(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_lower, m, Tm.p, m, Tev.p, Twk.p, info.p));
+    HIPCHK(hipMemcpyAsync(ev.data(), Tev.p, static_cast<size_t>(m) * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    // the last block of the nev lowest eigenvectors: rows m-p.. of columns 0..nev
+    HIPCHK(hipMemcpy2DAsync(Vlast.data(), static_cast<size_t>(p) * sizeof(double), Tm.p + (m - p),
+                            static_cast<size_t>(m) * sizeof(double), static_cast<size_t>(p) * sizeof(double), nev,
+                            hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hinfo != 0) return 1;
+    double worst = 0.0;
+    bool conv = m >= 2 * nev;
+    for (int i = 0; i < nev; ++i) {
+      double r = 0.0;
+      for (int a = 0; a < p; ++a) {  // (B s_last)_a, B = Rb upper
+          double v = 0.0;
+          for (int b = a; b < p; ++b) v += Rb[a + p * b] * Vlast[b + static_cast<size_t>(p) * i];
+          r += v * v;
+        }
+      r = std::sqrt(r);
+      worst = std::max(worst, r);
+      conv = conv && r <= tol * std::max(1.0, std::fabs(ev[i]));
+    }
+    c->eigen_dim = m;
+    c->eigen_resid = worst;
+    if (conv) {
+      std::copy(ev.begin(), ev.begin() + nev, e);
+      return 0;
+    }
+    if (m + p > mmax) break;
+  }
+  return 1;
+}
+
 // The nev lowest eigenvalues of H x = E S x for matrices in the block form.
 int solve_lowest(bsrg_ctx* c, const Matrices& H, const Matrices& S, int nev, double* e) {
   const size_t n = static_cast<size_t>(H.nch) * H.ns + H.ncp;
@@ -68,6 +275,19 @@ int solve_lowest(bsrg_ctx* c, const Matrices& H, const Matrices& S, int nev, dou
   if (n > 46000) return fail(-1, "eigenvalues: nhm above 46000 (32-bit sizes of the solver)");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
+  c->eigen_dim = 0;
+  c->eigen_resid = 0.0;
+  const bool lanczos = c->eigen_method == 2 || (c->eigen_method == 0 && n >= 2048);
+  if (lanczos && 2 * nev + 64 <= static_cast<int>(n)) {
+    const int rc = solve_lanczos(c, H, S, nev, e, 1e-11);
+    if (rc <= 0) {
+      c->eigen_used = 2;
+      return rc;
+    }
+    // not converged within its Krylov space: the dense solver below
+  }
+  c->eigen_used = 1;
+  c->eigen_dim = static_cast<int>(n);
   // dense work memory lives for this call only: 16 nhm^2 bytes
   DevVec<double> A, B, W, Z, hbb, hbbS;
   DevVec<int> info;
@@ -150,6 +370,21 @@ int read_matrix(RecordReader& r, int ns, int nch, int ncp, Matrices& M, std::vec
 }
 
 }  // namespace
+
+int bsrg_set_eigen_method(bsrg_ctx* c, int method) {
+  if (!c) return fail(-2, "null context");
+  if (method < 0 || method > 2) return fail(-1, "bsrg_set_eigen_method: method must be 0, 1 or 2");
+  c->eigen_method = method;
+  return 0;
+}
+
+int bsrg_eigen_info(bsrg_ctx* c, int* method, int* dim, double* resid) {
+  if (!c) return fail(-2, "null context");
+  if (method) *method = c->eigen_used;
+  if (dim) *dim = c->eigen_dim;
+  if (resid) *resid = c->eigen_resid;
+  return 0;
+}
 
 int bsrg_wave_eigen(bsrg_ctx* c, int nev, double* e) {
   if (!c) return fail(-2, "null context");

@@ -80,6 +80,8 @@ SIGNATURES = {
     "bsrg_wave_diff": [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)],
     "bsrg_wave_eigen": [C.c_void_p, C.c_int, _dp],
     "bsrg_matfile_eigen": [C.c_void_p, C.c_char_p, C.c_int, _dp],
+    "bsrg_set_eigen_method": [C.c_void_p, C.c_int],
+    "bsrg_eigen_info": [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)],
     "bsrg_dbs_set_grid": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp],
     "bsrg_dbs_mrk": [C.c_void_p, C.c_int, C.c_int, _dp],
     "bsrg_dbs_moments": [C.c_void_p, C.c_int, C.c_int, _dp, _dp],

@@ -515,6 +515,20 @@ int bsrg_wave_eigen(bsrg_ctx* ctx, int nev, double* e);
  * ic <= 0, for S then H).  -50: malformed file; -51: cannot open. */
 int bsrg_matfile_eigen(bsrg_ctx* ctx, const char* path, int nev, double* e);
 
+/* How bsrg_wave_eigen / bsrg_matfile_eigen solve: 1 dense (DPOTRF, DSYGST
+ * and the tridiagonal reduction of the whole standard-form matrix,
+ * rocSOLVER sygvdx); 2 block Lanczos on the standard-form matrix (DPOTRF
+ * and DSYGST as above, then blocks of max(16, nev+6) vectors with full
+ * reorthogonalisation until each of the nev lowest Ritz values has a
+ * residual bound below 1e-11 max(1, |E|) -- an eigenvalue lies within that
+ * bound -- falling back to the dense solver when its Krylov space is
+ * exhausted first); 0 (default) block Lanczos from nhm = 2048 on, dense
+ * below.  eigen_info: the method the last call used, its Krylov dimension
+ * (nhm for the dense solver) and largest residual bound.  No counterpart in
+ * the reference (diag_hd.f90:312 always calls DSYEV). */
+int bsrg_set_eigen_method(bsrg_ctx* ctx, int method);
+int bsrg_eigen_info(bsrg_ctx* ctx, int* method, int* dim, double* resid);
+
 /* Forget the resident R^k bands and orbital bases (of the BS and of the
  * Dirac partial waves) so the next call rebuilds them (bsrg_set_grid,
  * bsrg_set_orbitals, bsrg_dbs_set_grid and bsrg_dirac_set_orbitals keep them

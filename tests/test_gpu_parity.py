@@ -337,6 +337,36 @@ def test_split_precision_wave(ctx, oracle_lib, mode, nslices):
     assert de <= 1e-9
 
 
+@pytest.mark.parametrize("method", [1, 2])
+def test_eigen_methods_vs_lapack(ctx, oracle_lib, method):
+    """The lowest 10 eigenvalues of H x = E S x for a wave of nhm = 2420
+    (12 channels of ns = 200 and 20 perturbers) by the dense solver (1) and
+    by block Lanczos (2), each within 1e-9 of LAPACK on the same matrices;
+    Lanczos reports a residual bound below its tolerance, and the automatic
+    choice at this size is Lanczos."""
+    O = oracle_lib
+    from bsr_amd import synth
+    import scipy.linalg as sl
+    w = synth.make_wave(91, 8, 200, 18.0, 40.0, nch=12, ncp=20, mk=10, nb=5, ntarg=5, fill=0.05, name="eig")
+    ctx.wave_assemble(w)
+    S, H = ctx.wave_get(0), ctx.wave_get(1)
+    e_ref = sl.eigh(O.dense_from_wave(H), O.dense_from_wave(S), eigvals_only=True, subset_by_index=[0, 9])
+    try:
+        ctx.set_eigen_method(method)
+        e = ctx.wave_eigen(10)
+        info = ctx.eigen_info()
+    finally:
+        ctx.set_eigen_method(0)
+    d = np.abs(e - e_ref).max()
+    print(f"[eigen] method {method}: {info}, max |dE| vs LAPACK {d:.2e}", flush=True)
+    assert info["method"] == method
+    assert d <= 1e-9
+    if method == 2:
+        assert 0 < info["dim"] < w.nhm and info["resid"] <= 1e-11 * max(1.0, np.abs(e).max())
+        ctx.wave_eigen(10)
+        assert ctx.eigen_info()["method"] == 2   # the automatic choice from nhm 2048 on
+
+
 @pytest.mark.parametrize("engine", [0, 1, 2, 3, 4])
 def test_fp64_gemm_engines(ctx, oracle_lib, engine):
     """Each FP64 engine of the exchange contraction (library GEMM, the three
