@@ -1026,6 +1026,8 @@ int bsrg_create(bsrg_ctx** out, int device) {
   HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
   if (rocblas_create_handle(&c->blas) != rocblas_status_success) return fail(-101, "rocblas_create_handle failed");
   rocblas_set_stream(c->blas, c->stream);
+  // no atomics in rocBLAS's kernels: the same products on every run
+  rocblas_set_atomics_mode(c->blas, rocblas_atomics_not_allowed);
   for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
   for (Staged* st : {&c->stS, &c->stH, &c->stC, &c->dw.st}) HIPCHK(hipEventCreateWithFlags(&st->done, hipEventDisableTiming));
   *out = c.release();

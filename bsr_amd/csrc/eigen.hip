@@ -9,7 +9,8 @@
 // The channel blocks, channel-perturber vectors and perturber scalars are
 // expanded on the device into the lower triangle of dense nhm x nhm
 // matrices (nhm = nch ns + ncp, channel i at rows (i-1) ns.., perturbers
-// last), then rocSOLVER's sygvdx returns eigenvalues il = 1 .. iu = nev.
+// last), then block Lanczos on the standard form (solve_lanczos) or
+// rocSOLVER's sygvdx returns eigenvalues il = 1 .. iu = nev.
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
@@ -84,6 +85,21 @@ __global__ void k_sym_upper(int n, double* A, size_t lda) {
   }
 }
 
+// Chunk sums C[q] = sum_b P[q + b mp], q < mp, added in chunk order.
+__global__ void k_sum_chunks(size_t mp, int nb, const double* __restrict__ P, double* __restrict__ C) {
+  const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= mp) return;
+  double v = 0.0;
+  for (int b = 0; b < nb; ++b) v += P[q + static_cast<size_t>(b) * mp];
+  C[q] = v;
+}
+
+void sym_upper(int n, double* A, hipStream_t s) {
+  const int nt = (n + 31) / 32;
+  hipLaunchKernelGGL(k_sym_upper, dim3(static_cast<unsigned>(nt) * (nt + 1) / 2), dim3(256), 0, s, n, A,
+                     static_cast<size_t>(n));
+}
+
 // A deterministic start block: V(i, j) uniform in [-1, 1) from a hash of (i, j).
 __global__ void k_start_block(int n, int p, double* V) {
   const size_t q = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -121,8 +137,36 @@ bool chol_upper(int p, double* G) {
       return fail(-100, std::string(#x " failed: ") + rocblas_status_to_string(rs_)); \
   } while (0)
 
+// C (m x p, leading dimension m) = A^T B, A n x m and B n x p with leading
+// dimensions lda and ldb.  rocBLAS gives a transposed product with a long
+// inner dimension and a small result a handful of workgroups (2 ms for
+// 16 x 16 x 12400); here the inner dimension is cut into nb chunks, one
+// strided-batched product makes the nb partial results and a fixed-order
+// sum adds them: enough workgroups, and the same result on every run.
+int gemm_tn(rocblas_handle h, hipStream_t s, int m, int p, int n, const double* A, int lda, const double* B, int ldb,
+            double* C, DevVec<double>& part) {
+  const int tiles = ((m + 63) / 64) * ((p + 63) / 64);
+  const int nb = std::max(1, std::min(512 / tiles, n / 64));
+  const int kc = n / nb, r = n - kc * nb, nparts = nb + (r > 0 ? 1 : 0);
+  const size_t mp = static_cast<size_t>(m) * p;
+  const double one = 1.0, zero = 0.0;
+  HIPCHK(part.resize(mp * nparts));
+  RBCHK(rocblas_dgemm_strided_batched(h, rocblas_operation_transpose, rocblas_operation_none, m, p, kc, &one, A, lda,
+                                      kc, B, ldb, kc, &zero, part.p, m, static_cast<rocblas_stride>(mp), nb));
+  if (r > 0)
+    RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, p, r, &one,
+                        A + static_cast<size_t>(kc) * nb, lda, B + static_cast<size_t>(kc) * nb, ldb, &zero,
+                        part.p + mp * nb, m));
+  hipLaunchKernelGGL(k_sum_chunks, dim3(static_cast<unsigned>((mp + 255) / 256)), dim3(256), 0, s, mp, nparts,
+                     part.p, C);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // The nev lowest eigenvalues of H x = E S x by block Lanczos.  S = L L^T
-// (potrf), C = L^-1 H L^-T (sygst) in full storage, then block Lanczos on C
+// (potrf), C = L^-1 H L^-T by two triangular solves on H in full storage
+// (level-3 work, where sygst's reduction runs thousands of level-2 steps),
+// made exactly symmetric from its lower triangle, then block Lanczos on C
 // with blocks of p vectors and full reorthogonalisation (two classical
 // Gram-Schmidt passes against every earlier block, Cholesky QR twice for
 // the new block): T = Q^T C Q is block tridiagonal with the symmetrised
@@ -142,7 +186,7 @@ int solve_lanczos(bsrg_ctx* c, const Matrices& H, const Matrices& S, int nev, do
   if (mmax < 4 * p) return 1;
   hipStream_t s = c->stream;
   rocblas_handle h = c->blas;
-  DevVec<double> A, B, Q, Hh, Tm, Tev, Twk, hbb, hbbS;
+  DevVec<double> A, B, Q, Hh, Tm, Tev, Twk, hbb, hbbS, part;
   DevVec<int> info;
   HIPCHK(A.resize(static_cast<size_t>(n) * n));
   HIPCHK(B.resize(static_cast<size_t>(n) * n));
@@ -156,25 +200,25 @@ int solve_lanczos(bsrg_ctx* c, const Matrices& H, const Matrices& S, int nev, do
   HIPCHK(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (hinfo != 0) return fail(-40, "S is not positive definite (DPOTRF: leading minor " + std::to_string(hinfo) + ")");
-  RBCHK(rocsolver_dsygst(h, rocblas_eform_ax, rocblas_fill_lower, n, A.p, n, B.p, n));
+  const double one = 1.0, mone = -1.0;
+  sym_upper(n, A.p, s);
+  RBCHK(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit, n,
+                      n, &one, B.p, n, A.p, n));
+  RBCHK(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                      rocblas_diagonal_non_unit, n, n, &one, B.p, n, A.p, n));
+  HIPCHK(hipStreamSynchronize(s));
   B.release();
-  {
-    const int nt = (n + 31) / 32;
-    hipLaunchKernelGGL(k_sym_upper, dim3(static_cast<unsigned>(nt) * (nt + 1) / 2), dim3(256), 0, s, n, A.p,
-                       static_cast<size_t>(n));
-  }
+  sym_upper(n, A.p, s);
   const size_t np = static_cast<size_t>(n) * p;
   hipLaunchKernelGGL(k_start_block, dim3(static_cast<unsigned>((np + 255) / 256)), dim3(256), 0, s, n, p, Q.p);
   HIPCHK(hipGetLastError());
-  const double one = 1.0, zero = 0.0, mone = -1.0;
   std::vector<double> G(static_cast<size_t>(p) * p), Rb(static_cast<size_t>(p) * p), R1(static_cast<size_t>(p) * p);
   DevVec<double> Gd;
   HIPCHK(Gd.resize(static_cast<size_t>(p) * p));
   // W <- W R^-1 with R^T R = W^T W, twice; R = R2 R1 on return
   auto chol_qr = [&](double* W, std::vector<double>& R) -> int {
     for (int pass = 0; pass < 2; ++pass) {
-      RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, p, p, n, &one, W, n, W, n, &zero,
-                          Gd.p, p));
+      if (int rc = gemm_tn(h, s, p, p, n, W, n, W, n, Gd.p, part)) return rc;
       HIPCHK(hipMemcpyAsync(G.data(), Gd.p, G.size() * sizeof(double), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
       if (!chol_upper(p, G.data())) return 1;
@@ -205,15 +249,13 @@ int solve_lanczos(bsrg_ctx* c, const Matrices& H, const Matrices& S, int nev, do
     // block j = m/p - 1 at columns [m-p, m); the next one goes to [m, m+p)
     const double* Vj = Q.p + static_cast<size_t>(n) * (m - p);
     double* W = Q.p + static_cast<size_t>(n) * m;
-    RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, p, n, &one, A.p, n, Vj, n, &zero, W,
-                        n));
+    // W = C Vj as C^T Vj (C is symmetric in full storage)
+    if (int rc = gemm_tn(h, s, n, p, n, A.p, n, Vj, n, W, part)) return rc;
     // the diagonal block, then W orthogonalised against every block so far
-    RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, p, p, n, &one, Vj, n, W, n, &zero,
-                        Gd.p, p));
+    if (int rc = gemm_tn(h, s, p, p, n, Vj, n, W, n, Gd.p, part)) return rc;
     HIPCHK(hipMemcpyAsync(Aj.data(), Gd.p, Aj.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     for (int pass = 0; pass < 2; ++pass) {
-      RBCHK(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, m, p, n, &one, Q.p, n, W, n, &zero,
-                          Hh.p, m));
+      if (int rc = gemm_tn(h, s, m, p, n, Q.p, n, W, n, Hh.p, part)) return rc;
       RBCHK(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, n, p, m, &mone, Q.p, n, Hh.p, m, &one, W,
                           n));
     }
