@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsr_gpu.so")
-SOURCES = ["host_tables.cpp", "lists.cpp", "kernels.hip", "capi.hip", "ozaki.hip", "dbs.hip", "dirac.hip", "eigen.hip", "xasm.hip", "dirac_diag.hip", "state_res.cpp"]
+SOURCES = ["host_tables.cpp", "lists.cpp", "kernels.hip", "capi.hip", "lists_dev.hip", "ozaki.hip", "dbs.hip", "dirac.hip", "eigen.hip", "xasm.hip", "dirac_diag.hip", "state_res.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",

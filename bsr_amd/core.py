@@ -347,6 +347,24 @@ class Context:
             len(o.c), i32(o.k3), i32(o.k4), f64(o.c), w.eps_c, w.ec, w.ntarg, f64(w.etarg), i32(w.iptar)))
         self._wave = (w.nch, w.ncp, w.mk)
 
+    def lists_upload(self, w):
+        """The R^k lists of a PartialWave into device memory (bsrg_lists_upload)."""
+        jt, kp, k1, k2, k3, k4, c = w.concat()
+        L.check(L.lib().bsrg_lists_upload(self._p, len(c), i32(jt), i32(kp), i32(k1), i32(k2), i32(k3), i32(k4),
+                                          f64(c)))
+
+    def wave_assemble_resident(self, w, set_inputs=True):
+        """wave_assemble with the lists of the last lists_upload, decoded on
+        the device (the same S and H, bit for bit)."""
+        if set_inputs:
+            self.set_grid(w.ks, w.ns, w.ml, w.me, w.h, w.t)
+            self.set_orbitals(w.pbs, w.iech, w.lbs, w.ibort, w.eps_ovl)
+        o = w.olist
+        L.check(L.lib().bsrg_wave_assemble_resident(
+            self._p, w.nch, w.ncp, w.mk, len(o.c), i32(o.k3), i32(o.k4), f64(o.c), w.eps_c, w.ec, w.ntarg,
+            f64(w.etarg), i32(w.iptar)))
+        self._wave = (w.nch, w.ncp, w.mk)
+
     def wave_prepare(self, w, set_inputs=True):
         """Host lists, uploads, bands and bases; inputs stay resident."""
         if set_inputs:

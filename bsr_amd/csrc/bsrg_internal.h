@@ -247,4 +247,65 @@ int ozaki_dgemm(int m, int n, int K, const double* A, size_t lda, const double* 
 // false on a write error or a record of 2 GiB or more.
 bool put_record(std::FILE* f, const void* data, size_t bytes);
 
+// ---- R^k lists decoded on the device (lists_dev.hip) ----
+// Item categories: the term kinds of the term pass (band and rank-1 terms
+// by block, vector terms by channel-perturber position), exchange
+// coefficients (by slot of the coefficient matrix), host scalars (HB by
+// perturber pair, target matrix by block) and ACF group sums (by multipole
+// and channel pair).  An item's sort key is (category << kb) | index.
+enum { kDlCatBand = 0, kDlCatR1, kDlCatVec, kDlCatGemm, kDlCatHbb, kDlCatHtarg, kDlCatAcf, kDlNumCat };
+// Messages of a record the device decoder rejects (the host decoder's
+// codes and texts for the same record, capi.hip dl_message).
+enum {
+  kDlItype = 1, kDlMultipole, kDlPrepared, kDlOrbRange, kDlChanDensity, kDlHb, kDlGetvRange, kDlGetvZero,
+  kDlHvChannel, kDlHvBound, kDlHwChannel, kDlCfChannel, kDlStruct1, kDlBav, kDlStruct25, kDlHxChannel
+};
+struct DecodeArgs {
+  int nrec = 0, base = 0;  // records; record r's items are base + 2r and base + 2r + 1
+  const int *jt = nullptr, *kp = nullptr, *k1 = nullptr, *k2 = nullptr, *k3 = nullptr, *k4 = nullptr;
+  const double* cd = nullptr;
+  int nbf = 0, nb = 0, np = 0, nch = 0, ncp = 0, mk = 0, basis_nk = 0, ns = 0;
+  const int *bidx = nullptr, *iech = nullptr, *lbs = nullptr, *ibort = nullptr;  // nbf, nbf, nbf, nbf^2
+  const double *obs = nullptr, *qbs = nullptr;  // nbf^2, ns x nbf
+  const double* Qb = nullptr;                   // <p_a|r^k|p_b> over bound orbitals, (k*nb + a)*nb + b
+  const double* St = nullptr;                   // scalar R^k table
+  double eps_ovl = 0.0;
+  int w0 = 0;       // vector-table id of GET_V pair (ii, jj): w0 + (ii-1) + nbf*(jj-1)
+  long nt = 0;      // exchange coefficient rows per block
+  int kb = 0;       // index bits of a sort key
+  unsigned* key = nullptr;  // items
+  int *ia = nullptr, *ib = nullptr, *is = nullptr;
+  double* ic = nullptr;
+  unsigned char* need = nullptr;  // nbf^2: GET_V pairs the records name
+  unsigned long long* err = nullptr;  // min over rejected records of (record << 8 | message)
+};
+struct DecodeSort {
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  unsigned *skey = nullptr, *sval = nullptr;
+  const unsigned* iota = nullptr;
+  int nitems = 0;
+};
+struct DecodeCsr {  // the term arrays of the term pass (Staged layout)
+  int nblk = 0, nq = 0;
+  double *bc = nullptr, *rc = nullptr, *vc = nullptr;
+  int *bptr = nullptr, *bid = nullptr, *rptr = nullptr, *ru = nullptr, *rw = nullptr, *rs = nullptr,
+      *vptr = nullptr, *vid = nullptr, *vs = nullptr;
+};
+struct DecodeOut {
+  int* cb = nullptr;  // kDlNumCat + 1 category starts in the sorted items
+  double *gemm = nullptr, *hbb = nullptr, *htarg = nullptr, *acf = nullptr;  // dense, zeroed by the caller
+  double* VT = nullptr;                                                     // vector table (GET_V rows)
+};
+size_t dl_sort_bytes(int nitems, int end_bit);
+// The caller's items (nh0 before the records' items, nh1 from item tail on),
+// staged in device memory as ic (doubles), key, ia, ib, is.
+size_t dl_host_item_bytes(int nh);
+void launch_dl_host_items(int nh0, int nh1, size_t tail, const void* staged, const DecodeArgs& a, hipStream_t s);
+void launch_dl_iota(size_t n, unsigned* v, hipStream_t s);
+// Items of the records, GET_V vectors, sort, CSR arrays and run sums; the
+// items before base and after base + 2 nrec are the caller's (host terms).
+int launch_dl_decode(const DecodeArgs& a, const DecodeSort& so, const DecodeCsr& o, const DecodeOut& out,
+                     hipStream_t s);
+
 }  // namespace bsrg

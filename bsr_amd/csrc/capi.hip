@@ -363,6 +363,22 @@ void counting_sort(const std::vector<Terms*>& parts, Vec vec, int nkeys, Key key
     for (const auto& x : (P->*vec)) pos[q++] = cnt[key(x)]++;
 }
 
+// Byte offsets of the CSR arrays of st (st.nband, nr1, nvec, nblk and nq
+// set) in its arena, 256-byte aligned: doubles (coefficients) first, then
+// ints, then the live map; sets st.bytes and returns it.
+size_t layout_staged(Staged& st, bool live) {
+  size_t o = 0;
+  auto put = [&o](size_t count, size_t elem) { const size_t at = o; o += ((count * elem + 255) / 256) * 256; return at; };
+  st.o_bc = put(st.nband, 8); st.o_rc = put(st.nr1, 8); st.o_vc = put(st.nvec, 8);
+  st.o_bptr = put(st.nblk + 1, 4); st.o_bid = put(st.nband, 4);
+  st.o_rptr = put(st.nblk + 1, 4); st.o_ru = put(st.nr1, 4); st.o_rw = put(st.nr1, 4); st.o_rs = put(st.nr1, 4);
+  st.o_vptr = put(st.nq + 1, 4); st.o_vid = put(st.nvec, 4); st.o_vs = put(st.nvec, 4);
+  st.has_live = live;
+  st.o_live = put(live ? st.nblk : 0, 1);
+  st.bytes = o;
+  return o;
+}
+
 // Sort the terms of the parts by output and lay the CSR arrays out in the
 // pinned arena: doubles (coefficients) first, then ints.  The band, rank-1
 // and vector sets are independent and are staged concurrently when a pool
@@ -378,15 +394,8 @@ int stage_terms(const std::vector<Terms*>& parts, Staged& st, int nblk, int nq, 
   }
   st.nblk = nblk;
   st.nq = nq;
-  size_t o = 0;
+  size_t o = layout_staged(st, live && live->size() >= static_cast<size_t>(nblk));
   auto put = [&o](size_t count, size_t elem) { const size_t at = o; o += ((count * elem + 255) / 256) * 256; return at; };
-  st.o_bc = put(st.nband, 8); st.o_rc = put(st.nr1, 8); st.o_vc = put(st.nvec, 8);
-  st.o_bptr = put(st.nblk + 1, 4); st.o_bid = put(st.nband, 4);
-  st.o_rptr = put(st.nblk + 1, 4); st.o_ru = put(st.nr1, 4); st.o_rw = put(st.nr1, 4); st.o_rs = put(st.nr1, 4);
-  st.o_vptr = put(st.nq + 1, 4); st.o_vid = put(st.nvec, 4); st.o_vs = put(st.nvec, 4);
-  st.has_live = live && live->size() >= static_cast<size_t>(nblk);
-  st.o_live = put(st.has_live ? nblk : 0, 1);
-  st.bytes = o;  // the position scratch below is not uploaded
   const size_t o_pb = put(st.nband, 4), o_pr = put(st.nr1, 4), o_pv = put(st.nvec, 4);
   if (o > st.h.cap) HIPCHK(st.h.reserve(o));
   if (st.has_live) std::memcpy(st.hp<unsigned char>(st.o_live), live->data(), nblk);
@@ -462,22 +471,15 @@ int ensure_nobands(bsrg_ctx* c, size_t nblk) {
   return 0;
 }
 
-// Upload a staged term set (and the GET_V vectors) for a later enqueue:
-// one copy of the arena, one of the exchange coefficients.
-// The exchange coefficients of a term set (and the split-precision
-// workspaces its GEMM needs): all a GEMM launch reads besides the bases.
-int upload_gemm(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
+// The split-precision workspaces of a staged term set's GEMM (st.gemm_klo
+// and st.gemm_khi set).
+int gemm_workspaces(bsrg_ctx* c, const Matrices& M, Staged& st) {
   hipStream_t s = c->stream;
-  st.has_gemm = T.gemm_n > 0 && T.gemm_khi >= 0;
-  if (st.has_gemm) {
-    HIPCHK(st.gemm.resize(T.gemm_n));
-    HIPCHK(hipMemcpyAsync(st.gemm.p, T.gemm.p, T.gemm_n * sizeof(double), hipMemcpyHostToDevice, s));
-    st.gemm_klo = T.gemm_klo;
-    st.gemm_khi = T.gemm_khi;
+  {
     if (c->precision_mode == 1) {
       const int nb = c->nb;
       const size_t nss = static_cast<size_t>(c->G.ns) * c->G.ns;
-      const int nr = (T.gemm_khi - T.gemm_klo + 1) * nb * nb;
+      const int nr = (st.gemm_khi - st.gemm_klo + 1) * nb * nb;
       HIPCHK(c->ozk.resize(ozaki_workspace_bytes(static_cast<int>(nss), static_cast<int>(M.nblk()), nr,
                                                  c->nslices)));
     } else if (c->precision_mode == 2) {
@@ -492,6 +494,18 @@ int upload_gemm(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
     }
   }
   return 0;
+}
+
+// The exchange coefficients of a term set (and the split-precision
+// workspaces its GEMM needs): all a GEMM launch reads besides the bases.
+int upload_gemm(bsrg_ctx* c, const Matrices& M, const Terms& T, Staged& st) {
+  st.has_gemm = T.gemm_n > 0 && T.gemm_khi >= 0;
+  if (!st.has_gemm) return 0;
+  HIPCHK(st.gemm.resize(T.gemm_n));
+  HIPCHK(hipMemcpyAsync(st.gemm.p, T.gemm.p, T.gemm_n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  st.gemm_klo = T.gemm_klo;
+  st.gemm_khi = T.gemm_khi;
+  return gemm_workspaces(c, M, st);
 }
 
 // The staged arena of a term set (and the GET_V vectors) for a later
@@ -924,6 +938,270 @@ int add_odata(bsrg_ctx* c, const Matrices& M, Terms& T, int n, const int* ipt, c
   return 0;
 }
 
+// ---- R^k lists decoded on the device (lists_dev.hip) ----
+
+// The host decoder's code and text for a record the device decoder rejects.
+const char* dl_message(int id, int& code) {
+  switch (id) {
+    case kDlItype: code = -21; return " I_data: unknown itype ";
+    case kDlMultipole: code = -17; return "UPDATE_CF: multipole index out of range";
+    case kDlPrepared: code = -33; return "multipole outside the prepared range";
+    case kDlOrbRange: code = -30; return "orbital index out of range";
+    case kDlChanDensity: code = -31; return "R^k density of a channel orbital (no B-spline expansion)";
+    case kDlHb: code = -12; return "UPDATE_HB: indeces out of range";
+    case kDlGetvRange: code = -18; return "get_V: orbital index out of range";
+    case kDlGetvZero: code = -19; return "get_V: <kl|nl> = 0 ?";
+    case kDlHvChannel: code = -13; return "UPDATE_HV: channel index out of range";
+    case kDlHvBound: code = -14; return "UPDATE_HV: bound index out of range";
+    case kDlHwChannel: code = -10; return "UPDATE_HW: channel index out of range";
+    case kDlCfChannel: code = -10; return "UPDATE_CF: channel index out of range";
+    case kDlStruct1: code = -22; return " I_data: unknown structure for itype=1";
+    case kDlBav: code = -31; return "BAV with a channel orbital";
+    case kDlStruct25: code = -23; return " I_data: uknown structure for itype 2-5 ";
+    case kDlHxChannel: code = -10; return "UPDATE_HX: channel index out of range";
+    default: code = -100; return "device list decoding: unknown rejection";
+  }
+}
+
+// The smallest b with n < 2^b.
+int bits_for(size_t n) {
+  int b = 1;
+  while (b < 63 && (size_t{1} << b) <= n) ++b;
+  return b;
+}
+
+// Orbital tables of the device decoder (once per orbital set) and
+// <p_a|r^k|p_b> over the bound orbitals for k <= mk.
+int dl_tables(bsrg_ctx* c, int mk) {
+  auto& R = c->rl;
+  const int nbf = c->nbf, ns = c->G.ns, nb = c->nb;
+  hipStream_t s = c->stream;
+  if (R.tab_ver != c->orb_ver) {
+    std::vector<int> it(3 * static_cast<size_t>(nbf) + static_cast<size_t>(nbf) * nbf);
+    std::copy(c->bidx.begin(), c->bidx.end(), it.begin());
+    std::copy(c->iech.begin(), c->iech.end(), it.begin() + nbf);
+    std::copy(c->lbs.begin(), c->lbs.end(), it.begin() + 2 * nbf);
+    std::copy(c->ibort.begin(), c->ibort.end(), it.begin() + 3 * nbf);
+    std::vector<double> dt(c->obs);
+    dt.insert(dt.end(), c->qbs.begin(), c->qbs.end());
+    HIPCHK(R.itab.upload(it.data(), it.size(), s));
+    HIPCHK(R.dtab.upload(dt.data(), dt.size(), s));
+    HIPCHK(hipStreamSynchronize(s));  // the host vectors go out of scope
+    R.tab_ver = c->orb_ver;
+    R.qb_mk = -1;
+  }
+  if (R.qb_mk < mk) {
+    std::vector<double> q(static_cast<size_t>(mk + 1) * std::max(nb, 1) * std::max(nb, 1), 0.0);
+    for (int k = 0; k <= mk; ++k)
+      for (int a = 0; a < nb; ++a)
+        for (int b = 0; b < nb; ++b)
+          q[(static_cast<size_t>(k) * nb + a) * nb + b] = quadr(c, c->bound[a], c->bound[b], k);
+    HIPCHK(R.Qb.upload(q.data(), q.size(), s));
+    HIPCHK(hipStreamSynchronize(s));
+    R.qb_mk = mk;
+  }
+  (void)ns;
+  return 0;
+}
+
+// The H terms of the resident R^k lists, decoded on the device, with the
+// host terms of H before (TH) and after (THt) them: the term set stH (CSR
+// arrays in its device arena, exchange coefficients in its gemm), GET_V
+// vectors in the vector table, and the host scalars and ACF sums, read back
+// for bsrg_wave_finish.
+int dl_decode(bsrg_ctx* c) {
+  auto& R = c->rl;
+  Matrices& M = c->H;
+  Staged& st = c->stH;
+  hipStream_t s = c->stream;
+  const int nbf = c->nbf, ns = c->G.ns, ks = c->G.ks, nb = c->nb, np = c->npair;
+  if (int rc = dl_tables(c, M.mk)) return rc;
+  // vector table: the V part, the GET_V vectors of the host terms, then one
+  // slot per orbital pair for the records' GET_V vectors
+  const size_t nV = static_cast<size_t>(c->basis_nk) * np * nb;
+  const size_t nW = c->wvec.size() / std::max(ns, 1);
+  const size_t w0 = nV + nW, nvt = w0 + static_cast<size_t>(nbf) * nbf;
+  if (w0 + static_cast<size_t>(nbf) * nbf >= (size_t{1} << 31)) return fail(-1, "device list decoding: vector table too large");
+  if (c->VT.n < nvt * ns) {
+    DevVec<double> nvtab;
+    HIPCHK(nvtab.resize((nvt + 16) * ns));
+    if (nV) HIPCHK(hipMemcpyAsync(nvtab.p, c->VT.p, nV * ns * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::swap(nvtab.p, c->VT.p);
+    std::swap(nvtab.n, c->VT.n);
+    c->wvec_uploaded = -1;
+  }
+  if (int rc = upload_wvec(c)) return rc;
+  // the host terms as items, in the order of their term lists
+  const Terms* host[2] = {&c->TH, &c->THt};
+  size_t nh[2];
+  for (int h = 0; h < 2; ++h) {
+    const Terms& T = *host[h];
+    nh[h] = T.band.size() + T.r1.size() + T.vec.size() + T.scal.size();
+    for (const auto& x : T.scal)
+      if (x.kind == 2) return fail(-100, "device list decoding: overlap target term in H");
+  }
+  const size_t nrec = static_cast<size_t>(R.nrec), nitems = nh[0] + 2 * nrec + nh[1];
+  if (nitems >= (size_t{1} << 31)) return fail(-1, "device list decoding: too many records");
+  const size_t nblk = M.nblk(), nq = static_cast<size_t>(M.nch) * M.ncp;
+  const size_t nt = static_cast<size_t>(c->basis_nk) * nb * nb;
+  R.n_hbb = M.hbb.size();
+  R.n_htarg = nblk;
+  R.n_acf = M.acf.size();
+  const size_t kmax = std::max({nblk + 1, nq + 1, R.n_hbb, R.n_acf, R.any_exch ? nt * nblk : size_t{0}});
+  const int kb = bits_for(kmax);
+  if (kb + 3 > 32) return fail(-1, "device list decoding: too many outputs for the 32-bit sort keys");
+  const int nhi = static_cast<int>(nh[0] + nh[1]);
+  if (R.hitems_pending) {
+    HIPCHK(hipEventSynchronize(R.hitems_done));
+    R.hitems_pending = false;
+  }
+  const size_t hbytes = dl_host_item_bytes(nhi);
+  HIPCHK(R.hitems.reserve(std::max<size_t>(hbytes, 8)));
+  {
+    double* sc = reinterpret_cast<double*>(R.hitems.p);
+    unsigned* sk = reinterpret_cast<unsigned*>(sc + nhi);
+    int* sa = reinterpret_cast<int*>(sk + nhi);
+    int* sb = sa + nhi;
+    int* ss = sb + nhi;
+    size_t t = 0;
+    auto put = [&](int cat, size_t idx, int a, int b, int sidx, double v) {
+      sk[t] = (static_cast<unsigned>(cat) << kb) | static_cast<unsigned>(idx);
+      sa[t] = a; sb[t] = b; ss[t] = sidx; sc[t] = v;
+      ++t;
+    };
+    for (const Terms* T : host) {
+      for (const auto& x : T->band) put(kDlCatBand, x.blk, x.bid, 0, -1, x.c);
+      for (const auto& x : T->r1) put(kDlCatR1, x.blk, x.u, x.w, x.sidx, x.c);
+      for (const auto& x : T->vec) put(kDlCatVec, x.q, x.v, 0, x.sidx, x.c);
+      for (const auto& x : T->scal) put(x.kind == 0 ? kDlCatHbb : kDlCatHtarg, x.idx, 0, 0, x.sidx, x.c);
+    }
+  }
+  if (!R.hitems_done) HIPCHK(hipEventCreateWithFlags(&R.hitems_done, hipEventDisableTiming));
+  HIPCHK(R.hstage.resize(std::max<size_t>(hbytes, 8)));
+  if (hbytes) HIPCHK(hipMemcpyAsync(R.hstage.p, R.hitems.p, hbytes, hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(R.hitems_done, s));
+  R.hitems_pending = true;
+  // item and sort arrays
+  for (DevVec<unsigned>* v : {&R.key, &R.skey, &R.sval, &R.iota}) HIPCHK(v->resize(std::max<size_t>(nitems, 1)));
+  for (DevVec<int>* v : {&R.ia, &R.ib, &R.is}) HIPCHK(v->resize(std::max<size_t>(nitems, 1)));
+  HIPCHK(R.ic.resize(std::max<size_t>(nitems, 1)));
+  HIPCHK(R.need.resize(static_cast<size_t>(nbf) * nbf));
+  HIPCHK(R.cb.resize(kDlNumCat + 1));
+  HIPCHK(R.err.resize(1));
+  if (R.sort_items != static_cast<int>(nitems) || R.sort_bits != kb + 3) {
+    R.sort_tmp_bytes = dl_sort_bytes(static_cast<int>(nitems), kb + 3);
+    HIPCHK(R.sort_tmp.resize(std::max<size_t>(R.sort_tmp_bytes, 1)));
+    R.sort_items = static_cast<int>(nitems);
+    R.sort_bits = kb + 3;
+  }
+  const size_t nout = R.n_hbb + R.n_htarg + R.n_acf;
+  HIPCHK(R.outs.resize(std::max<size_t>(nout, 1)));
+  launch_fill(std::max<size_t>(nout, 1), 0.0, R.outs.p, s);
+  // the term set: CSR capacities of every item, exchange coefficients
+  if (int rc = wait_staged(st)) return rc;
+  st.nblk = static_cast<int>(nblk);
+  st.nq = static_cast<int>(nq);
+  st.nband = st.nr1 = st.nvec = nitems;
+  layout_staged(st, false);
+  HIPCHK(st.d.resize(st.bytes));
+  st.has_gemm = R.any_exch && nb > 0 && R.xkhi >= 0;
+  if (st.has_gemm) {
+    st.gemm_klo = std::max(0, R.xklo);
+    st.gemm_khi = std::min(R.xkhi, c->basis_nk - 1);
+    if (st.gemm_khi < st.gemm_klo) st.has_gemm = false;
+  }
+  if (st.has_gemm) {
+    HIPCHK(st.gemm.resize(nt * nblk));
+    launch_fill(nt * nblk, 0.0, st.gemm.p, s);
+    if (int rc = gemm_workspaces(c, M, st)) return rc;
+  }
+  HIPCHK(c->bsum.resize(nblk * ns * ks));
+  HIPCHK(c->rfac.resize(std::max<size_t>(nitems, 1)));
+  DecodeArgs a;
+  a.nrec = R.nrec;
+  a.base = static_cast<int>(nh[0]);
+  a.jt = R.rec.p;
+  a.kp = R.rec.p + nrec;
+  a.k1 = R.rec.p + 2 * nrec;
+  a.k2 = R.rec.p + 3 * nrec;
+  a.k3 = R.rec.p + 4 * nrec;
+  a.k4 = R.rec.p + 5 * nrec;
+  a.cd = R.cd.p;
+  a.nbf = nbf; a.nb = nb; a.np = np; a.nch = M.nch; a.ncp = M.ncp; a.mk = M.mk; a.basis_nk = c->basis_nk; a.ns = ns;
+  a.bidx = R.itab.p; a.iech = R.itab.p + nbf; a.lbs = R.itab.p + 2 * nbf; a.ibort = R.itab.p + 3 * nbf;
+  a.obs = R.dtab.p; a.qbs = R.dtab.p + static_cast<size_t>(nbf) * nbf;
+  a.Qb = R.Qb.p;
+  a.St = c->St.p;
+  a.eps_ovl = c->eps_ovl;
+  a.w0 = static_cast<int>(w0);
+  a.nt = static_cast<long>(nt);
+  a.kb = kb;
+  a.key = R.key.p; a.ia = R.ia.p; a.ib = R.ib.p; a.is = R.is.p; a.ic = R.ic.p;
+  a.need = R.need.p;
+  a.err = R.err.p;
+  launch_dl_host_items(static_cast<int>(nh[0]), static_cast<int>(nh[1]), nh[0] + 2 * nrec, R.hstage.p, a, s);
+  launch_dl_iota(nitems, R.iota.p, s);
+  DecodeSort so;
+  so.tmp = R.sort_tmp.p; so.tmp_bytes = R.sort_tmp_bytes;
+  so.skey = R.skey.p; so.sval = R.sval.p; so.iota = R.iota.p;
+  so.nitems = static_cast<int>(nitems);
+  DecodeCsr o;
+  o.nblk = static_cast<int>(nblk); o.nq = static_cast<int>(nq);
+  char* d = st.d.p;
+  o.bc = reinterpret_cast<double*>(d + st.o_bc); o.rc = reinterpret_cast<double*>(d + st.o_rc);
+  o.vc = reinterpret_cast<double*>(d + st.o_vc);
+  o.bptr = reinterpret_cast<int*>(d + st.o_bptr); o.bid = reinterpret_cast<int*>(d + st.o_bid);
+  o.rptr = reinterpret_cast<int*>(d + st.o_rptr); o.ru = reinterpret_cast<int*>(d + st.o_ru);
+  o.rw = reinterpret_cast<int*>(d + st.o_rw); o.rs = reinterpret_cast<int*>(d + st.o_rs);
+  o.vptr = reinterpret_cast<int*>(d + st.o_vptr); o.vid = reinterpret_cast<int*>(d + st.o_vid);
+  o.vs = reinterpret_cast<int*>(d + st.o_vs);
+  DecodeOut out;
+  out.cb = R.cb.p;
+  out.gemm = st.has_gemm ? st.gemm.p : nullptr;
+  out.hbb = R.outs.p;
+  out.htarg = R.outs.p + R.n_hbb;
+  out.acf = R.outs.p + R.n_hbb + R.n_htarg;
+  out.VT = c->VT.p;
+  if (launch_dl_decode(a, so, o, out, s)) return fail(-100, "device list decoding: launch failed");
+  // category starts, rejection word and sums, read back by bsrg_wave_finish
+  const size_t back = 64 + nout * sizeof(double);
+  if (back > R.back.cap) {
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(R.back.reserve(back));
+  }
+  HIPCHK(hipMemcpyAsync(R.back.p, R.cb.p, (kDlNumCat + 1) * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(R.back.p + 48, R.err.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  if (nout) HIPCHK(hipMemcpyAsync(R.back.p + 64, R.outs.p, nout * sizeof(double), hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+// After the stream of a resident wave completed: the rejection word, the
+// term counts, and the host scalars of H (HB, target matrix, ACF sums).
+int dl_finish(bsrg_ctx* c) {
+  auto& R = c->rl;
+  Matrices& M = c->H;
+  const int* cb = reinterpret_cast<const int*>(R.back.p);
+  unsigned long long err;
+  std::memcpy(&err, R.back.p + 48, sizeof(err));
+  if (err != ~0ull) {
+    int code;
+    const char* text = dl_message(static_cast<int>(err & 0xFF), code);
+    return fail(code, text);
+  }
+  c->stH.nband = static_cast<size_t>(cb[kDlCatR1] - cb[kDlCatBand]);
+  c->stH.nr1 = static_cast<size_t>(cb[kDlCatVec] - cb[kDlCatR1]);
+  c->stH.nvec = static_cast<size_t>(cb[kDlCatVec + 1] - cb[kDlCatVec]);
+  const double* v = reinterpret_cast<const double*>(R.back.p + 64);
+  std::fill(M.hbb.begin(), M.hbb.end(), 0.0);
+  for (size_t q = 0; q < R.n_hbb && q < M.hbb.size(); ++q) M.hbb[q] += v[q];
+  v += R.n_hbb;
+  for (size_t q = 0; q < R.n_htarg && q < M.htarg.size(); ++q) M.htarg[q] += v[q];
+  v += R.n_htarg;
+  for (size_t q = 0; q < R.n_acf && q < M.acf.size(); ++q) M.acf[q] += v[q];
+  return 0;
+}
+
 void acf_finish(Matrices& M, int kcore) {
   const int nch = M.nch;
   const double eps_acf = 1e-5;
@@ -1042,6 +1320,7 @@ int bsrg_destroy(bsrg_ctx* c) {
   if (c->blas) rocblas_destroy_handle(c->blas);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   for (Staged* st : {&c->stS, &c->stH, &c->stC, &c->dw.st}) if (st->done) (void)hipEventDestroy(st->done);
+  if (c->rl.hitems_done) (void)hipEventDestroy(c->rl.hitems_done);
   hipStream_t s = c->stream, cs = c->copy_stream;
   delete c;
   if (s) (void)hipStreamDestroy(s);
@@ -1212,6 +1491,7 @@ int bsrg_set_orbitals(bsrg_ctx* c, int nbf, const double* pbs, const int* iech, 
   c->wvec_uploaded = -1;
   c->lval_ok = false;
   c->orb_ok = true;
+  ++c->orb_ver;
   return 0;
 }
 
@@ -1636,21 +1916,29 @@ int bsrg_get_matrix(bsrg_ctx* c, double* hcc, double* hcb, double* hbb, double* 
   return get_matrices(c, c->cur, hcc, hcb, hbb, acf, htarg, otarg);
 }
 
-int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int* jtype, const int* kpol,
-                      const int* k1, const int* k2, const int* k3, const int* k4, const double* cd, int novl,
-                      const int* ok3, const int* ok4, const double* ocd, double eps_c, double ec, int ntarg,
-                      const double* etarg, const int* iptar) {
+namespace {
+
+// bsrg_wave_prepare; resident: the R^k lists are the context's resident
+// ones (bsrg_lists_upload), decoded on the device, and nrec and the list
+// arrays are unused.
+int wave_prepare(bsrg_ctx* c, bool resident, int nch, int ncp, int mk, int nrec, const int* jtype, const int* kpol,
+                 const int* k1, const int* k2, const int* k3, const int* k4, const double* cd, int novl,
+                 const int* ok3, const int* ok4, const double* ocd, double eps_c, double ec, int ntarg,
+                 const double* etarg, const int* iptar) {
   if (int rc = check_ctx(c)) return rc;
   if (!c->orb_ok) return fail(-4, "orbitals not set (bsrg_set_orbitals)");
+  if (resident && c->rl.nrec < 0) return fail(-8, "no R^k lists in device memory (bsrg_lists_upload)");
+  if (resident) nrec = 0;
   if (nch < 1 || ncp < 0 || mk < 0 || nrec < 0 || novl < 0) return fail(-1, "bsrg_wave_assemble: bad sizes");
   HIPCHK(hipSetDevice(c->device));
   c->wave_ok = false;
   c->wave_prepared = false;
   c->eager_done = false;
+  c->rl.active = false;
   using clk = std::chrono::steady_clock;
   const auto h0 = clk::now();
   // record checks on the host workers (the first bad record in list order
-  // is the one reported)
+  // is the one reported); the device decoder checks resident records itself
   if (!c->pool) c->pool.reset(new Pool(host_workers()));
   const int nchk = static_cast<int>(std::min<long>(c->pool->size(), nrec / 16384L + 1));
   std::vector<int> bad(nchk, -1), exch(nchk, 0);
@@ -1664,9 +1952,9 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
     }
     exch[t] = ex;
   };
-  c->pool->run(nchk, check);
-  bool any_exch = false;
-  for (int t = 0; t < nchk; ++t) {
+  if (nrec > 0) c->pool->run(nchk, check);
+  bool any_exch = resident && c->rl.any_exch;
+  for (int t = 0; t < nchk && nrec > 0; ++t) {
     if (bad[t] >= 0) {
       const int r = bad[t];
       if (jtype[r] < 1 || jtype[r] > 9) return fail(-21, " I_data: unknown itype ");
@@ -1702,7 +1990,7 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   if (int rc = add_odata(c, c->H, TH, novl, nullptr, ok3, ok4, ocd, eps_c, ec, false)) return rc;
   c->partsS.assign(1, &TS);
   c->s_enqueued = false;
-  if (c->early_gemm) {
+  if (c->early_gemm || resident) {
     // a one-shot wave: S is complete, and its term pass runs behind the
     // bases while the R^k lists are decoded
     if (int rc = stage_terms(c->partsS, c->stS, static_cast<int>(c->S.nblk()), c->S.nch * c->S.ncp, nullptr,
@@ -1714,6 +2002,23 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   }
   const auto hb = clk::now();
   c->gemm_enqueued = false;
+  if (resident) {
+    // the lists on the device: H's terms, GET_V vectors and scalars there
+    for (int ich = 1; ich <= nch; ++ich) band_hl(THt, ich, ich, etarg[iptar[ich - 1] - 1] - ec);
+    if (int rc = dl_decode(c)) return rc;
+    if (int rc = autotune_gemm(c, c->H, c->stH, c->capture_safe)) return rc;
+    c->partsH.clear();
+    c->rl.active = true;
+    const auto h3 = clk::now();
+    auto ms = [](clk::time_point x, clk::time_point y) { return std::chrono::duration<double, std::milli>(y - x).count(); };
+    for (double& x : c->host_ms) x = 0.0;
+    c->host_ms[0] = ms(h0, ha);
+    c->host_ms[1] = ms(ha, hb);
+    c->host_ms[3] = ms(hb, h3);
+    c->host_ms[7] = ms(h0, h3);
+    c->wave_prepared = true;
+    return 0;
+  }
   if (int rc = idata_parallel(c, nrec, jtype, kpol, k1, k2, k3, k4, cd, any_exch)) return rc;
   for (int ich = 1; ich <= nch; ++ich) band_hl(THt, ich, ich, etarg[iptar[ich - 1] - 1] - ec);
   c->partsH.assign(1, &TH);
@@ -1754,6 +2059,56 @@ int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int
   return 0;
 }
 
+}  // namespace
+
+int bsrg_wave_prepare(bsrg_ctx* c, int nch, int ncp, int mk, int nrec, const int* jtype, const int* kpol,
+                      const int* k1, const int* k2, const int* k3, const int* k4, const double* cd, int novl,
+                      const int* ok3, const int* ok4, const double* ocd, double eps_c, double ec, int ntarg,
+                      const double* etarg, const int* iptar) {
+  return wave_prepare(c, false, nch, ncp, mk, nrec, jtype, kpol, k1, k2, k3, k4, cd, novl, ok3, ok4, ocd, eps_c, ec,
+                      ntarg, etarg, iptar);
+}
+
+int bsrg_lists_upload(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, const int* k1, const int* k2,
+                      const int* k3, const int* k4, const double* cd) {
+  if (int rc = check_ctx(c)) return rc;
+  if (nrec < 0) return fail(-1, "bsrg_lists_upload: bad sizes");
+  if (nrec > 0 && (!jtype || !kpol || !k1 || !k2 || !k3 || !k4 || !cd)) return fail(-2, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));  // no kernel still reads the previous lists
+  auto& R = c->rl;
+  R.nrec = -1;
+  R.any_exch = false;
+  R.xklo = 1 << 30;
+  R.xkhi = -1;
+  for (int r = 0; r < nrec; ++r)
+    if (jtype[r] >= 8 && jtype[r] <= 9) {
+      R.any_exch = true;
+      R.xklo = std::min(R.xklo, kpol[r]);
+      R.xkhi = std::max(R.xkhi, kpol[r]);
+    }
+  const size_t n = static_cast<size_t>(nrec);
+  HIPCHK(R.rec.resize(std::max<size_t>(6 * n, 1)));
+  HIPCHK(R.cd.resize(std::max<size_t>(n, 1)));
+  const int* src[6] = {jtype, kpol, k1, k2, k3, k4};
+  for (int q = 0; q < 6 && n; ++q)
+    HIPCHK(hipMemcpy(R.rec.p + q * n, src[q], n * sizeof(int), hipMemcpyHostToDevice));
+  if (n) HIPCHK(hipMemcpy(R.cd.p, cd, n * sizeof(double), hipMemcpyHostToDevice));
+  R.nrec = nrec;
+  return 0;
+}
+
+int bsrg_wave_assemble_resident(bsrg_ctx* c, int nch, int ncp, int mk, int novl, const int* ok3, const int* ok4,
+                                const double* ocd, double eps_c, double ec, int ntarg, const double* etarg,
+                                const int* iptar) {
+  if (int rc = check_ctx(c)) return rc;
+  const int rc = wave_prepare(c, true, nch, ncp, mk, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                              novl, ok3, ok4, ocd, eps_c, ec, ntarg, etarg, iptar);
+  if (rc) return rc;
+  if (int rc2 = bsrg_wave_launch(c)) return rc2;
+  return bsrg_wave_finish(c);
+}
+
 int bsrg_wave_launch(bsrg_ctx* c) {
   if (int rc = check_ctx(c)) return rc;
   if (!c->wave_prepared) return fail(-8, "no prepared wave (bsrg_wave_prepare)");
@@ -1767,7 +2122,12 @@ int bsrg_wave_launch(bsrg_ctx* c) {
     return rc;
   c->gemm_was_early = c->gemm_enqueued;
   c->gemm_enqueued = false;
-  update_live(c, c->H, c->stH);
+  if (c->rl.active) {  // no host copy of the device-made term arrays: any block may hold nonzeros
+    if (c->H.live.size() < static_cast<size_t>(c->stH.nblk)) c->H.live.resize(c->stH.nblk, 1);
+    std::fill(c->H.live.begin(), c->H.live.end(), 1);
+  } else {
+    update_live(c, c->H, c->stH);
+  }
   launch_symmetrize_diag(c->G.ns, c->H.nch, c->H.hcc.p, c->stream);
   HIPCHK(hipGetLastError());
   if (timed) (void)hipEventRecord(c->ev[5], c->stream);
@@ -1783,7 +2143,11 @@ int bsrg_wave_finish(bsrg_ctx* c) {
   float ms = 0, ms2 = 0;
   if (!c->wave_ok) {
     finish_scalars(c, c->S, c->partsS, true);
-    finish_scalars(c, c->H, c->partsH, true);
+    if (c->rl.active) {
+      if (int rc = dl_finish(c)) return rc;
+    } else {
+      finish_scalars(c, c->H, c->partsH, true);
+    }
     acf_finish(c->H, 0);
     // the reference keeps one bsr_matrix module: H is recorded after the
     // overlap pass, so it carries the otarg accumulated there
@@ -1835,6 +2199,19 @@ int bsrg_wave_term_counts(bsrg_ctx* c, int which, long long* n8) {
   const int* bp = st.hp<int>(st.o_bptr);
   const int* rp = st.hp<int>(st.o_rptr);
   const int* vp = st.hp<int>(st.o_vptr);
+  std::vector<int> dptr;
+  if (which == 1 && c->rl.active) {  // the term arrays were made on the device
+    if (!c->wave_ok) return fail(-8, "bsrg_wave_term_counts: the resident wave has not finished");
+    dptr.resize(2 * (static_cast<size_t>(st.nblk) + 1) + st.nq + 1);
+    HIPCHK(hipMemcpy(dptr.data(), st.d.p + st.o_bptr, (st.nblk + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dptr.data() + st.nblk + 1, st.d.p + st.o_rptr, (st.nblk + 1) * sizeof(int),
+                     hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dptr.data() + 2 * (st.nblk + 1), st.d.p + st.o_vptr, (st.nq + 1) * sizeof(int),
+                     hipMemcpyDeviceToHost));
+    bp = dptr.data();
+    rp = bp + st.nblk + 1;
+    vp = rp + st.nblk + 1;
+  }
   long long bb = 0, rb = 0, vq = 0, live = 0;
   for (int b = 0; b < st.nblk; ++b) {
     bb += bp[b + 1] > bp[b];

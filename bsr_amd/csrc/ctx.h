@@ -421,6 +421,40 @@ struct bsrg_ctx {
   DevVec<double> xc;
   DevVec<double> bsumS, rfacS;  // the same for S, whose pass may be queued while H's scratch grows
 
+  // orbital set counter: changes whenever bsrg_set_orbitals installs new orbitals
+  long orb_ver = 0;
+  // R^k lists kept in device memory (bsrg_lists_upload) and decoded there
+  // by bsrg_wave_assemble_resident (lists_dev.hip)
+  struct ResidentLists {
+    int nrec = -1;  // -1: nothing uploaded
+    bool any_exch = false;
+    int xklo = 1 << 30, xkhi = -1;  // multipoles of the exchange records
+    DevVec<int> rec;                // jtype, kpol, k1, k2, k3, k4: nrec each
+    DevVec<double> cd;
+    long tab_ver = -1;              // orbital set the tables below hold
+    DevVec<int> itab;               // bidx, iech, lbs (nbf each), ibort (nbf^2)
+    DevVec<double> dtab;            // obs (nbf^2), qbs (ns x nbf)
+    DevVec<double> Qb;              // (mk+1) x nb x nb
+    int qb_mk = -1;
+    DevVec<unsigned char> need;
+    DevVec<unsigned> key, skey, sval, iota;
+    DevVec<int> ia, ib, is;
+    DevVec<double> ic;
+    DevVec<char> sort_tmp;
+    size_t sort_tmp_bytes = 0;
+    int sort_items = -1, sort_bits = -1;
+    DevVec<double> outs;            // sums of HB (perturber pairs), target-matrix and ACF items
+    DevVec<int> cb;
+    DevVec<unsigned long long> err;
+    PinnedVec<char> hitems;         // the host terms of H as items
+    DevVec<char> hstage;            // and their device copy
+    hipEvent_t hitems_done = nullptr;
+    bool hitems_pending = false;
+    PinnedVec<char> back;           // category starts, rejection word and the sums, read back
+    size_t n_hbb = 0, n_htarg = 0, n_acf = 0;
+    bool active = false;            // the prepared wave decodes its lists on the device
+  } rl;
+
   // Dirac B-spline grid and the resident bands of the four families
   struct Dbs {
     bool ok = false;

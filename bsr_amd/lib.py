@@ -123,6 +123,9 @@ SIGNATURES = {
     "bsrg_wave_prepare": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
                           C.c_int, _ip, _ip, _dp, C.c_double, C.c_double, C.c_int, _dp, _ip],
     "bsrg_wave_launch": [C.c_void_p],
+    "bsrg_lists_upload": [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp],
+    "bsrg_wave_assemble_resident": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _dp, C.c_double,
+                                    C.c_double, C.c_int, _dp, _ip],
     "bsrg_wave_finish": [C.c_void_p],
     "bsrg_wave_engine": [C.c_void_p],
     "bsrg_set_capture_safe": [C.c_void_p, C.c_int],

@@ -405,6 +405,26 @@ int bsrg_wave_assemble(bsrg_ctx* ctx, int nch, int ncp, int mk, int nrec, const 
                        const int* ok4, const double* ocdata, double eps_c, double ec,
                        int ntarg, const double* etarg, const int* iptar);
 
+/* The R^k lists of a partial wave (arguments as bsrg_wave_assemble's
+ * nrec .. cdata) copied into device memory owned by the context, replacing
+ * the ones uploaded before; they stay there until the next upload or
+ * bsrg_destroy, for any number of bsrg_wave_assemble_resident calls.  No
+ * counterpart in the reference: its bsr_mat reads the lists of int_bnk on
+ * the host (State_res, src/bsr/BSR_MAT3/state_res.f90:2-160). */
+int bsrg_lists_upload(bsrg_ctx* ctx, int nrec, const int* jtype, const int* kpol, const int* k1,
+                      const int* k2, const int* k3, const int* k4, const double* cdata);
+
+/* bsrg_wave_assemble with the R^k lists of the last bsrg_lists_upload,
+ * decoded on the device: I_data (src/bsr/BSR_MAT3/I_data.f90:2-224) and
+ * GET_V (get_V.f90:2-27) per record in one kernel, a stable sort of the
+ * terms by output, and sums in list order -- S, H and the host scalars
+ * equal bsrg_wave_assemble's bit for bit.  A record the host decoder would
+ * reject returns that decoder's status and message (for the first such
+ * record in list order), found when the wave finishes. */
+int bsrg_wave_assemble_resident(bsrg_ctx* ctx, int nch, int ncp, int mk, int novl, const int* ok3,
+                                const int* ok4, const double* ocdata, double eps_c, double ec,
+                                int ntarg, const double* etarg, const int* iptar);
+
 /* After bsrg_wave_assemble: copy S (which=0) or H (which=1) to host. */
 int bsrg_wave_get(bsrg_ctx* ctx, int which, double* hcc, double* hcb, double* hbb,
                   double* acf, double* htarg, double* otarg);
