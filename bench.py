@@ -4,16 +4,21 @@ Headline workload: one production-size partial wave -- ks=8, ns=200
 B-splines on a semi-exponential grid to Rmax=40 a0 (Z=18), multipoles
 k=0..10, 60 channels + 400 perturbers, 5 bound orbitals, angular lists at 5%
 fill with coefficients U[-1,1]: H and S of dimension 12400 (synthetic,
-seeded).  A step is the whole job from resident inputs: R^k bands for every
-k, orbital contractions, list processing, and the S and H blocks complete on
-the device, in plain FP64.  value = unique H elements nhm(nhm+1)/2 per second.
+seeded).  A step is the whole job from inputs resident in HBM (grid,
+orbitals, and the R^k lists uploaded once): R^k bands for every k, orbital
+contractions, the lists decoded into terms on the device, and the S and H
+blocks complete on the device, in plain FP64.  value = unique H elements
+nhm(nhm+1)/2 per second.
 
 Beside the headline line's required fields, the JSON line carries:
   roofline the exchange contraction (the step's largest kernel; by default
-           the product over each block's nonzero coefficients) against the
-           HBM bound: algorithmic bytes (bases read once, nonzero list, H
-           written once) over its event-timed duration, with the counter
-           traffic of the same kernel;
+           the product over each block's nonzero coefficients with the band
+           terms in its epilogue) against the HBM bound: algorithmic bytes
+           (bases read once, nonzero list, summed bands read once, H written
+           once) over its event-timed duration, with the counter traffic of
+           the same kernel;
+  host_lists  the same step from host lists decoded by the host workers
+           (the lists cross the C ABI as host arrays in every step);
   step_hbm the step's algorithmic bytes (bands read once, S and H written
            once, lists, orbitals) over the step time, against 8 TB/s;
   latency  the device-resident step, and the extra time to copy S and H to
@@ -255,14 +260,20 @@ def step_bytes(w):
     return bands + mats + 24 * w.n_records() + 8 * ns * (w.nb + ncp)
 
 
-def run_steps(ctx, w, warmup, steps):
+def run_steps(ctx, w, warmup, steps, resident=False):
+    """resident: the R^k lists are in device memory (ctx.lists_upload(w)
+    before) and decoded there; else they are host arrays decoded by the host
+    workers within each step."""
     def step():
         # a new partial wave: grid and orbitals arrive, bands and bases are
         # rebuilt, lists processed, S and H assembled on the device
         ctx.drop_caches()
         ctx.set_grid(w.ks, w.ns, w.ml, w.me, w.h, w.t)
         ctx.set_orbitals(w.pbs, w.iech, w.lbs, w.ibort, w.eps_ovl)
-        ctx.wave_assemble(w, set_inputs=False)
+        if resident:
+            ctx.wave_assemble_resident(w, set_inputs=False)
+        else:
+            ctx.wave_assemble(w, set_inputs=False)
 
     for _ in range(warmup):
         step()
@@ -533,8 +544,16 @@ def main():
     log("generating workload")
     w = synth.c2(nch=a.nch, ncp=a.ncp)
     ctx = Context(0)
-    ms, dev, host = run_steps(ctx, w, a.warmup, a.steps)
-    log(f"fp64: {a.steps} steps, {ms:.3f} ms/step")
+    # the inputs in HBM before the timed steps: grid, orbitals and the R^k
+    # lists (decoded on the device in each step)
+    ctx.set_grid(w.ks, w.ns, w.ml, w.me, w.h, w.t)
+    ctx.set_orbitals(w.pbs, w.iech, w.lbs, w.ibort, w.eps_ovl)
+    ctx.lists_upload(w)
+    ms, dev, host = run_steps(ctx, w, a.warmup, a.steps, resident=True)
+    log(f"fp64: {a.steps} steps, {ms:.3f} ms/step (lists decoded on the device)")
+    # the same step from host lists, decoded by the host workers
+    hms, hdev, hhost = run_steps(ctx, w, a.warmup, a.steps)
+    log(f"fp64 from host lists: {hms:.3f} ms/step; device ms {json.dumps(hdev)}; host ms {json.dumps(hhost)}")
     terms = {"S": ctx.wave_term_counts(0), "H": ctx.wave_term_counts(1)}
     log("  terms: " + json.dumps(terms))
     log("  device ms: " + json.dumps(dev))
@@ -545,10 +564,11 @@ def main():
     nblk = w.nch * (w.nch + 1) // 2
     nt = (w.mk + 1) * w.nb * w.nb
     nnz = exchange_nonzeros(w)
-    # the exchange contraction (the largest kernel of the step): algorithmic
-    # bytes = the used exchange bases read once + the nonzero list + every H
-    # block written once
-    xbytes = 8.0 * nss * nt + 12.0 * nnz + 8.0 * nss * nblk
+    # the exchange contraction with the band terms in its epilogue (the
+    # largest kernel of the step): algorithmic bytes = the used exchange bases
+    # read once + the nonzero list + the summed band of every block read once
+    # + every H block written once
+    xbytes = 8.0 * nss * nt + 12.0 * nnz + 8.0 * w.ns * w.ks * nblk + 8.0 * nss * nblk
     ach = xbytes / (gemm_ms * 1e-3) / 1e9 if gemm_ms > 0 else 0.0
     xflops = 2.0 * nss * nnz
     sbytes = step_bytes(w)
@@ -569,9 +589,10 @@ def main():
                    "device_ms_per_step": dev["total"], "device_ms": dev, "host_ms": host, "terms": terms},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM, "unit": "GB/s", "frac": ach / PEAK_HBM,
                      "traffic": XPROD_TRAFFIC_BYTES if engine == 4 and (w.nch, w.ncp) == (60, 400) else None,
-                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of k_xasm<false> at this shape, "
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE of k_xasm<true> at this shape, "
                                        "copy-calibrated: profiles/wave_c2_hbm_traffic.md",
-                     "kernel": "exchange contraction over the nonzero coefficients, k_xasm<false> (engine 4)",
+                     "kernel": "exchange contraction over the nonzero coefficients with the band terms in its "
+                               "epilogue, k_xasm<true> (engine 4)",
                      "engine": engine, "kernel_ms": gemm_ms, "algorithmic_bytes": xbytes,
                      "coefficient_nonzeros": nnz, "coefficient_density": nnz / (nt * nblk),
                      "fp64_flops": xflops,
@@ -592,6 +613,8 @@ def main():
         t0 = time.perf_counter()
         ctx.wave_write(os.path.join(td, "bsr_mat.001"), w)
         fms = (time.perf_counter() - t0) * 1e3
+    out["host_lists"] = {"ms_per_step": hms, "elements_per_s": w.n_elements / (hms * 1e-3), "device_ms": hdev,
+                         "host_ms": hhost}
     out["latency"] = {"device_resident_ms": ms, "d2h_ms": d2h, "file_ms": fms}
     log(f"latency: step {ms:.2f} ms, +D2H {d2h:.1f} ms, +file {fms:.1f} ms")
     # the same step with each FP64 product engine forced: library GEMM (0),
@@ -602,7 +625,7 @@ def main():
         ems, edev, _ = run_steps(ctx, w, 2, max(3, a.steps // 2))
         eng[str(e)] = {"ms_per_step": ems, "product_ms": edev["gemm"], "terms_ms": edev["terms"]}
     ctx.set_gemm_engine(-1)
-    eng[str(engine)] = {"ms_per_step": ms, "product_ms": gemm_ms, "terms_ms": dev["terms"]}
+    eng[str(engine)] = {"ms_per_step": hms, "product_ms": hdev["gemm"], "terms_ms": hdev["terms"]}
     out["fp64_engines"] = eng
     log("engines: " + json.dumps(eng))
     e_ref = None

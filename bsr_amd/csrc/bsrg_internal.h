@@ -132,7 +132,7 @@ int launch_xasm(int ns, int ks, int nblk, int nr, const double* E, const int* xo
                 const int* bptr, const int* bid, const double* bc, const double* bands, const int* rptr,
                 const int* ru, const int* rw, const int* rs, const double* rc, const double* stab, const double* V,
                 double* hcc, double* bsum, double* rf, const unsigned char* live, const int* nobands,
-                hipStream_t s);
+                hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // The band terms of an overwriting exchange product, added in the product
 // kernel's epilogue (term_epi.h): per element, the product, then its band
 // term -- the first step of k_assemble after a stored product, which then
@@ -303,9 +303,11 @@ size_t dl_sort_bytes(int nitems, int end_bit);
 size_t dl_host_item_bytes(int nh);
 void launch_dl_host_items(int nh0, int nh1, size_t tail, const void* staged, const DecodeArgs& a, hipStream_t s);
 void launch_dl_iota(size_t n, unsigned* v, hipStream_t s);
-// Items of the records, GET_V vectors, sort, CSR arrays and run sums; the
-// items before base and after base + 2 nrec are the caller's (host terms).
+// Items of the records, GET_V vectors, sort and CSR arrays (the items
+// before base and after base + 2 nrec are the caller's: host terms); then,
+// once the scalar R^k table is made, the run sums.
 int launch_dl_decode(const DecodeArgs& a, const DecodeSort& so, const DecodeCsr& o, const DecodeOut& out,
                      hipStream_t s);
+int launch_dl_runs(const DecodeArgs& a, const DecodeSort& so, const DecodeOut& out, hipStream_t s);
 
 }  // namespace bsrg

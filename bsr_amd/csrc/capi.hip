@@ -656,8 +656,7 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, Staged& st, bool overwrite, hipEvent
   hipStream_t s = c->stream;
   const bool fused = st.has_gemm && !gemm_done && st.engine == 4 && overwrite && c->precision_mode == 0;
   if (fused) {
-    if (e0) (void)hipEventRecord(e0, s);  // no separate product phase
-    if (e1) (void)hipEventRecord(e1, s);
+    // the product phase is the product kernel itself (events around it below)
   } else if (st.has_gemm && !gemm_done) {
     if (int rc = enqueue_gemm(c, M, st, overwrite, e0, e1)) return rc;
   } else if (!st.has_gemm) {
@@ -682,7 +681,7 @@ int enqueue_terms(bsrg_ctx* c, Matrices& M, Staged& st, bool overwrite, hipEvent
                                st.has_live && (!c->capturing || c->live_in_capture)
                                    ? st.dp<unsigned char>(st.o_live)
                                    : nullptr,
-                               c->xzero.p, s);
+                               c->xzero.p, s, e0, e1);
     if (xr) return fail(-100, "fused exchange pass: bad shape");
     launch_vec_assemble(ns, st.nq, 0, st.dp<int>(st.o_vptr), st.dp<int>(st.o_vid), st.dp<int>(st.o_vs),
                         st.dp<double>(st.o_vc), c->St.p, c->VT.p, M.hcb.p, s);
@@ -1077,10 +1076,14 @@ int dl_decode(bsrg_ctx* c) {
       for (const auto& x : T->scal) put(x.kind == 0 ? kDlCatHbb : kDlCatHtarg, x.idx, 0, 0, x.sidx, x.c);
     }
   }
+  // the decoding runs on the second stream, beside the band and basis
+  // kernels; only the run sums (which read the scalar R^k table) wait for them
+  hipStream_t ds = c->copy_stream;
   if (!R.hitems_done) HIPCHK(hipEventCreateWithFlags(&R.hitems_done, hipEventDisableTiming));
+  if (!R.dec_done) HIPCHK(hipEventCreateWithFlags(&R.dec_done, hipEventDisableTiming));
   HIPCHK(R.hstage.resize(std::max<size_t>(hbytes, 8)));
-  if (hbytes) HIPCHK(hipMemcpyAsync(R.hstage.p, R.hitems.p, hbytes, hipMemcpyHostToDevice, s));
-  HIPCHK(hipEventRecord(R.hitems_done, s));
+  if (hbytes) HIPCHK(hipMemcpyAsync(R.hstage.p, R.hitems.p, hbytes, hipMemcpyHostToDevice, ds));
+  HIPCHK(hipEventRecord(R.hitems_done, ds));
   R.hitems_pending = true;
   // item and sort arrays
   for (DevVec<unsigned>* v : {&R.key, &R.skey, &R.sval, &R.iota}) HIPCHK(v->resize(std::max<size_t>(nitems, 1)));
@@ -1097,7 +1100,7 @@ int dl_decode(bsrg_ctx* c) {
   }
   const size_t nout = R.n_hbb + R.n_htarg + R.n_acf;
   HIPCHK(R.outs.resize(std::max<size_t>(nout, 1)));
-  launch_fill(std::max<size_t>(nout, 1), 0.0, R.outs.p, s);
+  launch_fill(std::max<size_t>(nout, 1), 0.0, R.outs.p, ds);
   // the term set: CSR capacities of every item, exchange coefficients
   if (int rc = wait_staged(st)) return rc;
   st.nblk = static_cast<int>(nblk);
@@ -1113,7 +1116,7 @@ int dl_decode(bsrg_ctx* c) {
   }
   if (st.has_gemm) {
     HIPCHK(st.gemm.resize(nt * nblk));
-    launch_fill(nt * nblk, 0.0, st.gemm.p, s);
+    launch_fill(nt * nblk, 0.0, st.gemm.p, ds);
     if (int rc = gemm_workspaces(c, M, st)) return rc;
   }
   HIPCHK(c->bsum.resize(nblk * ns * ks));
@@ -1140,8 +1143,8 @@ int dl_decode(bsrg_ctx* c) {
   a.key = R.key.p; a.ia = R.ia.p; a.ib = R.ib.p; a.is = R.is.p; a.ic = R.ic.p;
   a.need = R.need.p;
   a.err = R.err.p;
-  launch_dl_host_items(static_cast<int>(nh[0]), static_cast<int>(nh[1]), nh[0] + 2 * nrec, R.hstage.p, a, s);
-  launch_dl_iota(nitems, R.iota.p, s);
+  launch_dl_host_items(static_cast<int>(nh[0]), static_cast<int>(nh[1]), nh[0] + 2 * nrec, R.hstage.p, a, ds);
+  launch_dl_iota(nitems, R.iota.p, ds);
   DecodeSort so;
   so.tmp = R.sort_tmp.p; so.tmp_bytes = R.sort_tmp_bytes;
   so.skey = R.skey.p; so.sval = R.sval.p; so.iota = R.iota.p;
@@ -1163,7 +1166,10 @@ int dl_decode(bsrg_ctx* c) {
   out.htarg = R.outs.p + R.n_hbb;
   out.acf = R.outs.p + R.n_hbb + R.n_htarg;
   out.VT = c->VT.p;
-  if (launch_dl_decode(a, so, o, out, s)) return fail(-100, "device list decoding: launch failed");
+  if (launch_dl_decode(a, so, o, out, ds)) return fail(-100, "device list decoding: launch failed");
+  HIPCHK(hipEventRecord(R.dec_done, ds));
+  HIPCHK(hipStreamWaitEvent(s, R.dec_done, 0));
+  if (launch_dl_runs(a, so, out, s)) return fail(-100, "device list decoding: launch failed");
   // category starts, rejection word and sums, read back by bsrg_wave_finish
   const size_t back = 64 + nout * sizeof(double);
   if (back > R.back.cap) {
@@ -1321,6 +1327,7 @@ int bsrg_destroy(bsrg_ctx* c) {
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   for (Staged* st : {&c->stS, &c->stH, &c->stC, &c->dw.st}) if (st->done) (void)hipEventDestroy(st->done);
   if (c->rl.hitems_done) (void)hipEventDestroy(c->rl.hitems_done);
+  if (c->rl.dec_done) (void)hipEventDestroy(c->rl.dec_done);
   hipStream_t s = c->stream, cs = c->copy_stream;
   delete c;
   if (s) (void)hipStreamDestroy(s);
@@ -2076,6 +2083,7 @@ int bsrg_lists_upload(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, 
   if (nrec > 0 && (!jtype || !kpol || !k1 || !k2 || !k3 || !k4 || !cd)) return fail(-2, "null argument");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));  // no kernel still reads the previous lists
+  HIPCHK(hipStreamSynchronize(c->copy_stream));
   auto& R = c->rl;
   R.nrec = -1;
   R.any_exch = false;

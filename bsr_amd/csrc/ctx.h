@@ -449,6 +449,7 @@ struct bsrg_ctx {
     PinnedVec<char> hitems;         // the host terms of H as items
     DevVec<char> hstage;            // and their device copy
     hipEvent_t hitems_done = nullptr;
+    hipEvent_t dec_done = nullptr;  // end of the decoding on the second stream
     bool hitems_pending = false;
     PinnedVec<char> back;           // category starts, rejection word and the sums, read back
     size_t n_hbb = 0, n_htarg = 0, n_acf = 0;

@@ -374,6 +374,10 @@ int launch_dl_decode(const DecodeArgs& a, const DecodeSort& so, const DecodeCsr&
                      o.bptr, o.rptr, o.vptr);
   hipLaunchKernelGGL(k_dl_gather, dim3(grid_of(so.nitems, 256)), dim3(256), 0, s, so.sval, out.cb, a.ia, a.ib, a.is,
                      a.ic, o);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_dl_runs(const DecodeArgs& a, const DecodeSort& so, const DecodeOut& out, hipStream_t s) {
   hipLaunchKernelGGL(k_dl_runs, dim3(grid_of(so.nitems, 256)), dim3(256), 0, s, so.skey, so.sval, out.cb, a.kb, a.is,
                      a.ic, a.St, out.gemm, out.hbb, out.htarg, out.acf);
   return hipGetLastError() == hipSuccess ? 0 : -1;

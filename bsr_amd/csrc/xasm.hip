@@ -390,16 +390,18 @@ int launch_xasm(int ns, int ks, int nblk, int nr, const double* E, const int* xo
                 const int* bptr, const int* bid, const double* bc, const double* bands, const int* rptr,
                 const int* ru, const int* rw, const int* rs, const double* rc, const double* stab, const double* V,
                 double* hcc, double* bsum, double* rf, const unsigned char* live, const int* nobands,
-                hipStream_t s) {
+                hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   if (nblk <= 0) return 0;
   if (nr < 1) return -1;
   launch_term_prep(ns, ks, nblk, bptr, bid, bc, bands, bsum, rptr, rs, rc, stab, rf, s);
   int ntile, ngroup;
   xasm_grid(ns, nblk, ntile, ngroup);
   const TermEpi te{ns, ks, bptr, bsum};
+  if (e0) (void)hipEventRecord(e0, s);
   hipLaunchKernelGGL(k_xasm<true>, dim3(static_cast<unsigned>(ntile) * ngroup), dim3(64 * kXWaves),
                      static_cast<size_t>(nr) * 64 * sizeof(double), s, ns, nblk, nr, xasm_slot(nr), ntile, ngroup, E,
                      xo, xc, xn, te, rptr, live, hcc);
+  if (e1) (void)hipEventRecord(e1, s);
   // the rank-1 terms: added by the term pass to the blocks that have them
   // (no band terms: nobands is an all-zero pointer array)
   launch_assemble(ns, ks, nblk, 1, nobands, bid, bc, bands, rptr, ru, rw, rs, rc, stab, V, hcc, bsum, rf, s, nullptr,
