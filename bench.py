@@ -75,11 +75,11 @@ PEAK_FP64_VECTOR = 78.6  # TFLOP/s, MI355X FP64 vector
 PEAK_HBM = 8000.0       # GB/s
 PEAK_INT8_MFMA = 5000.0  # TOP/s, MI355X dense int8 matrix
 PEAK_BF16_MFMA = 2500.0  # TFLOP/s, MI355X dense bf16 matrix
-# HBM bytes of one launch of the sparse exchange product (k_xasm<false>) at
-# the headline shape: two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE)
-# scaled by a 1 GiB device copy in the same process (read 308.0 MB, write
-# 585.6 MB)
-XPROD_TRAFFIC_BYTES = 893.6e6
+# HBM bytes of one launch of the sparse exchange product with the band terms
+# in its epilogue (k_xasm<true>) at the headline shape: two rocprofv3 counter
+# runs (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB device copy in the same
+# process (read 515.7 MB, write 585.6 MB)
+XPROD_TRAFFIC_BYTES = 1101.3e6
 
 
 def log(msg):
