@@ -164,6 +164,23 @@ module bsr_gpu
       real(c_double) :: c(*), oc(*), etarg(*)
       real(c_double), value :: eps_c, ec
     end function
+    integer(c_int) function bsrg_lists_upload(ctx, nrec, jtype, kpol, k1, k2, k3, k4, c) &
+        bind(C, name="bsrg_lists_upload")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nrec
+      integer(c_int) :: jtype(*), kpol(*), k1(*), k2(*), k3(*), k4(*)
+      real(c_double) :: c(*)
+    end function
+    integer(c_int) function bsrg_wave_assemble_resident(ctx, nch, ncp, mk, novl, ok3, ok4, oc, eps_c, ec, &
+        ntarg, etarg, iptar) bind(C, name="bsrg_wave_assemble_resident")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nch, ncp, mk, novl, ntarg
+      integer(c_int) :: ok3(*), ok4(*), iptar(*)
+      real(c_double) :: oc(*), etarg(*)
+      real(c_double), value :: eps_c, ec
+    end function
     integer(c_int) function bsrg_wave_write(ctx, path, tmax, ntarg, etarg, ec) bind(C, name="bsrg_wave_write")
       import :: c_ptr, c_int, c_double, c_char
       type(c_ptr), value :: ctx
