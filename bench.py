@@ -297,11 +297,17 @@ def serving(stream, precision, window_s, nslots=6):
         warm = [srv.submit(w) for _, w in stream[:nslots]]
         for j in warm:
             j.event.wait()
+        d0 = srv.device_list_jobs
         paced = summarize(srv.run_stream(stream))
+        d1 = srv.device_list_jobs
         burst = summarize(srv.run_stream([(0.0, w) for _, w in stream]))
+        d2 = srv.device_list_jobs
     finally:
         srv.close()
-    return dict(rate_jobs_per_s=round(len(stream) / window_s, 2), p50_ms=paced["p50_ms"],
+    # the slots decode a job's lists on the device when no job waits, else on the host
+    return dict(rate_jobs_per_s=round(len(stream) / window_s, 2),
+                device_list_jobs={"paced": d1 - d0, "capacity": d2 - d1, "of": len(stream)},
+                p50_ms=paced["p50_ms"],
                 p99_ms=paced["p99_ms"], elements_per_s_at_rate=paced["elements_per_s"],
                 capacity_elements_per_s=burst["elements_per_s"], capacity_p50_ms=burst["p50_ms"],
                 capacity_p99_ms=burst["p99_ms"], service_ms_mean=paced["service_ms_mean"],

@@ -2100,8 +2100,9 @@ int bsrg_lists_upload(bsrg_ctx* c, int nrec, const int* jtype, const int* kpol, 
   HIPCHK(R.cd.resize(std::max<size_t>(n, 1)));
   const int* src[6] = {jtype, kpol, k1, k2, k3, k4};
   for (int q = 0; q < 6 && n; ++q)
-    HIPCHK(hipMemcpy(R.rec.p + q * n, src[q], n * sizeof(int), hipMemcpyHostToDevice));
-  if (n) HIPCHK(hipMemcpy(R.cd.p, cd, n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(R.rec.p + q * n, src[q], n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  if (n) HIPCHK(hipMemcpyAsync(R.cd.p, cd, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));  // the caller's arrays may go away on return
   R.nrec = nrec;
   return 0;
 }

@@ -16,7 +16,7 @@
 // are added run by run in that order -- the order of the host decoder.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "bsrg_internal.h"
 
@@ -24,6 +24,12 @@ namespace bsrg {
 namespace {
 
 constexpr unsigned kNone = 0xFFFFFFFFu;
+
+// The stable LSD radix sort in rocPRIM's onesweep form at every size (its
+// default sorts fewer than 2^20 items by block sorts and some twenty merge
+// passes, one launch each).
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, 0>;
 
 __device__ __forceinline__ unsigned cat_key(int cat, int kb, long idx) {
   return (static_cast<unsigned>(cat) << kb) | static_cast<unsigned>(idx);
@@ -338,9 +344,10 @@ unsigned grid_of(size_t n, unsigned b) { return static_cast<unsigned>((n + b - 1
 
 size_t dl_sort_bytes(int nitems, int end_bit) {
   size_t bytes = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, static_cast<const unsigned*>(nullptr),
-                                           static_cast<unsigned*>(nullptr), static_cast<const unsigned*>(nullptr),
-                                           static_cast<unsigned*>(nullptr), nitems, 0, end_bit);
+  (void)rocprim::radix_sort_pairs<SortConfig>(nullptr, bytes, static_cast<const unsigned*>(nullptr),
+                                              static_cast<unsigned*>(nullptr), static_cast<const unsigned*>(nullptr),
+                                              static_cast<unsigned*>(nullptr), nitems, 0u,
+                                              static_cast<unsigned>(end_bit));
   return bytes;
 }
 
@@ -365,8 +372,8 @@ int launch_dl_decode(const DecodeArgs& a, const DecodeSort& so, const DecodeCsr&
   if (a.nrec > 0) hipLaunchKernelGGL(k_dl_items, dim3(grid_of(a.nrec, 256)), dim3(256), 0, s, a);
   hipLaunchKernelGGL(k_dl_getv, dim3(static_cast<unsigned>(npair)), dim3(256), 0, s, a, out.VT);
   size_t bytes = so.tmp_bytes;
-  if (hipcub::DeviceRadixSort::SortPairs(so.tmp, bytes, a.key, so.skey, so.iota, so.sval, so.nitems, 0, a.kb + 3, s) !=
-      hipSuccess)
+  if (rocprim::radix_sort_pairs<SortConfig>(so.tmp, bytes, a.key, so.skey, so.iota, so.sval, so.nitems, 0u,
+                                             static_cast<unsigned>(a.kb + 3), s) != hipSuccess)
     return -1;
   hipLaunchKernelGGL(k_dl_bounds, dim3(1), dim3(64), 0, s, so.skey, so.nitems, a.kb, out.cb);
   const int nmax = std::max(o.nblk, o.nq) + 1;

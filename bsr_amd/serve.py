@@ -49,7 +49,17 @@ class Job:
 
 class Server:
     def __init__(self, nslots: int = 4, hbm_budget: float = 200e9, precision: int = 0, nslices: int = 0,
-                 device: int = 0):
+                 device: int = 0, device_lists="auto"):
+        # device_lists: True, a BS job's R^k lists go to device memory and are
+        # decoded there (bsrg_wave_assemble_resident: the host out of the
+        # job's way, lowest latency); False, the host workers of its slot
+        # decode them (the device does less per job: highest capacity when
+        # jobs queue); "auto", the first when no job waits in the queue as
+        # the job starts, else the second.  Both give the same bits.
+        if device_lists not in (True, False, "auto"):
+            raise ValueError("device_lists must be True, False or 'auto'")
+        self.device_lists = device_lists
+        self.device_list_jobs = 0
         self.slots = [Context(device) for _ in range(nslots)]
         for c in self.slots:
             c.set_precision(precision, nslices if precision else 0)
@@ -113,6 +123,8 @@ class Server:
                 if not self.inbox[s]:
                     return
                 job = self.inbox[s].popleft()
+                dev_lists = self.device_lists is True or (self.device_lists == "auto" and not self.queue)
+                self.device_list_jobs += int(dev_lists)
             job.start = self.now()
             try:
                 if is_dirac(job.wave):
@@ -122,7 +134,11 @@ class Server:
                     if job.keep:
                         job.result = ctx.dirac_get()
                 else:
-                    ctx.wave_assemble(job.wave)
+                    if dev_lists:
+                        ctx.lists_upload(job.wave)
+                        ctx.wave_assemble_resident(job.wave)
+                    else:
+                        ctx.wave_assemble(job.wave)
                     job.device_ms = ctx.wave_timings()["total"]
                     if job.keep:
                         job.result = (ctx.wave_get(0), ctx.wave_get(1))

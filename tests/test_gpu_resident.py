@@ -60,6 +60,22 @@ def test_resident_equals_host(ctx, name):
     _equal(host, _get(ctx), name + " (repeat)")
 
 
+@pytest.mark.parametrize("precision", [(2, 7), (1, 6)], ids=["int8x7", "bf16x6"])
+def test_resident_split_precision(ctx, precision):
+    """The exchange product in split precision (int8 or bf16 slices): the
+    same coefficients from either decoder give the same bits."""
+    w = _wave("perturbers")
+    try:
+        ctx.set_precision(*precision)
+        ctx.wave_assemble(w)
+        host = _get(ctx)
+        ctx.lists_upload(w)
+        ctx.wave_assemble_resident(w)
+        _equal(host, _get(ctx), f"split {precision}")
+    finally:
+        ctx.set_precision(0, 0)
+
+
 def test_resident_after_other_waves(ctx):
     """Lists stay resident across waves of other shapes assembled on the host
     in between; the GET_V vectors of each path are their own."""

@@ -10,8 +10,9 @@ from bsr_amd import serve
 
 
 class _FakeCtx:
-    """Stands in for a device context: wave_assemble sleeps for the wave's
-    duration (or raises), wave_timings reports it."""
+    """Stands in for a device context: wave_assemble (and its form with the
+    lists in device memory) sleeps for the wave's duration (or raises),
+    wave_timings reports it."""
     active = 0
     peak = 0
     lock = threading.Lock()
@@ -34,6 +35,12 @@ class _FakeCtx:
         finally:
             with _FakeCtx.lock:
                 _FakeCtx.active -= 1
+
+    def lists_upload(self, w):
+        pass
+
+    def wave_assemble_resident(self, w):
+        self.wave_assemble(w)
 
     def wave_timings(self):
         return {"total": self.last * 1e3}
