@@ -47,13 +47,15 @@ DRIVER = os.path.join(FORTRAN, "bsr_mat_gpu")
 
 
 def build_fortran(verbose: bool = False) -> str:
-    """The Fortran host: module bsr_gpu (ISO_C_BINDING) and the driver
-    program bsr_mat_gpu, linked against libbsr_gpu.so."""
+    """The Fortran host: module bsr_gpu (ISO_C_BINDING), the driver
+    program bsr_mat_gpu and the drop-in I_data/L_data/O_data it calls,
+    linked against libbsr_gpu.so."""
     fc = os.environ.get("FC", "/opt/rocm/bin/amdflang")
     mod = os.path.join(FORTRAN, "_mod")
     os.makedirs(mod, exist_ok=True)
     cmd = [fc, "-O2", "-module-dir", mod, os.path.join(FORTRAN, "bsr_gpu.f90"),
-           os.path.join(FORTRAN, "bsr_mat_gpu.f90"), "-o", DRIVER + ".tmp", "-L" + HERE, "-lbsr_gpu",
+           os.path.join(FORTRAN, "bsr_mat_gpu.f90"), os.path.join(FORTRAN, "bsr_gpu_dropin.f90"),
+           "-o", DRIVER + ".tmp", "-L" + HERE, "-lbsr_gpu",
            "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/.."]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -10,14 +10,32 @@
 !
 !  usage: bsr_mat_gpu <wave file> <bsr_mat output>
 !======================================================================
+!  The lists reach the device through the drop-in I_data, L_data and
+!  O_data of bsr_gpu_dropin.f90, which read them from module cmdata as the
+!  reference's do; the two modules below hold the names those read
+!  (mod_cmdata.f90:12-17, mod_param.f90:33,65).
+module cmdata
+  implicit none
+  integer :: ncdata = 0
+  real(8), allocatable :: CDATA(:)
+  integer, allocatable :: K1(:), K2(:), K3(:), K4(:), IPT(:)
+end module cmdata
+
+module bsr_mat
+  implicit none
+  integer :: iitar = 0
+  real(8) :: Eps_c = 1.0d-10
+end module bsr_mat
+
 program bsr_mat_gpu
   use bsr_gpu
+  use cmdata, only: ncdata, CDATA, K1, K2, K3, K4, IPT
+  use bsr_mat, only: Eps_c_m => Eps_c, iitar_m => iitar
   implicit none
   integer(c_int) :: ks, ns, ml, me, nbf, nch, ncp, mk, ntarg, nlists, novl, nllists, kclosd, iitar, orth
   real(c_double) :: h, ec, eps_c, eps_ovl, z
-  real(c_double), allocatable :: t(:), pbs(:,:), etarg(:), oc(:), c(:)
-  integer(c_int), allocatable :: iech(:), lbs(:), ibort(:,:), iptar(:), ok3(:), ok4(:), ipt(:)
-  integer(c_int), allocatable :: k1(:), k2(:), k3(:), k4(:)
+  real(c_double), allocatable :: t(:), pbs(:,:), etarg(:), oc(:)
+  integer(c_int), allocatable :: iech(:), lbs(:), ibort(:,:), iptar(:), ok3(:), ok4(:)
   real(c_double), allocatable :: hcc(:,:,:), hcb(:,:,:), hbb(:), acf(:,:,:), htarg(:), otarg(:)
   character(512) :: fin, fout
   integer :: nu, nw, il, jtype, kpol, n, ich, i, nij
@@ -47,10 +65,16 @@ program bsr_mat_gpu
   do ich = 1, nch
     call gpu_update_hl(ich, ich, 1.d0)
   end do
-  allocate(ipt(max(novl,1)))
-  ipt = [(i, i = 1, max(novl,1))]
-  if (novl > 0) call gpu_o_data(novl, ipt, ok3, ok4, oc, eps_c)
-  deallocate(ipt)
+  Eps_c_m = eps_c
+  iitar_m = iitar
+  if (novl > 0) then
+    ncdata = novl
+    allocate(K3(novl), K4(novl), CDATA(novl), IPT(novl))
+    K3 = ok3; K4 = ok4; CDATA = oc
+    IPT = [(i, i = 1, novl)]
+    call O_data
+    deallocate(K3, K4, CDATA, IPT)
+  end if
   call gpu_check(bsrg_symmetrize_diag(gpu_ctx))
   call gpu_check(bsrg_get_matrix(gpu_ctx, hcc, hcb, hbb, acf, htarg, otarg))
   open(newunit=nw, file=trim(fout), form='unformatted', status='replace')
@@ -62,19 +86,21 @@ program bsr_mat_gpu
   if (nllists > 0) call gpu_gen_lval(z, kclosd)
   do il = 1, nllists
     read(nu) jtype, kpol, n
-    allocate(k1(n), k2(n), k3(n), k4(n), c(n), ipt(n))
-    read(nu) k1, k2, k3, k4, c
-    ipt = [(i, i = 1, n)]
-    call gpu_l_data(jtype, kpol, n, ipt, k1, k2, k3, k4, c, iitar, eps_c)
-    deallocate(k1, k2, k3, k4, c, ipt)
+    ncdata = n
+    allocate(K1(n), K2(n), K3(n), K4(n), CDATA(n), IPT(n))
+    read(nu) K1, K2, K3, K4, CDATA
+    IPT = [(i, i = 1, n)]
+    call L_data(jtype, kpol)
+    deallocate(K1, K2, K3, K4, CDATA, IPT)
   end do
   do il = 1, nlists
     read(nu) jtype, kpol, n
-    allocate(k1(n), k2(n), k3(n), k4(n), c(n), ipt(n))
-    read(nu) k1, k2, k3, k4, c
-    ipt = [(i, i = 1, n)]
-    call gpu_i_data(jtype, kpol, n, ipt, k1, k2, k3, k4, c)
-    deallocate(k1, k2, k3, k4, c, ipt)
+    ncdata = n
+    allocate(K1(n), K2(n), K3(n), K4(n), CDATA(n), IPT(n))
+    read(nu) K1, K2, K3, K4, CDATA
+    IPT = [(i, i = 1, n)]
+    call I_data(jtype, kpol)
+    deallocate(K1, K2, K3, K4, CDATA, IPT)
   end do
   close(nu)
   do ich = 1, nch
