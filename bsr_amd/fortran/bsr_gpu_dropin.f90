@@ -39,9 +39,12 @@ End Subroutine L_data
 ! O_data.f90:2 -- overlap lists (-K3, -K4: the two sides of each overlap, O_data.f90:32)
 Subroutine O_data
   Use cmdata, only: ncdata, IPT, K3, K4, CDATA
-  Use bsr_mat, only: Eps_c
+  Use bsr_mat, only: Eps_c, iitar
   Use bsr_gpu, only: gpu_o_data
   Implicit none
+  ! the device overlap pass adds the target structure <i|j> to Target_h
+  ! only; O_data.f90:66 also adds c*sb to off-diagonal blocks when iitar > 1
+  if (iitar > 1) Stop ' O_data: iitar > 1 is not supported on the device '
   if (ncdata <= 0) Return
   Call gpu_o_data(ncdata, IPT, K3, K4, CDATA, Eps_c)
 End Subroutine O_data
